@@ -1,0 +1,520 @@
+"""b2g -- MI355X-native ``B2world::step`` for box2d-rs worlds (Python binding).
+
+Two layers over the C ABI of ``libb2g.so`` (declared in ``include/b2g.h``):
+
+* :class:`World` -- bulk, array-oriented access: add bodies/fixtures from numpy structured
+  arrays, step, step one phase, read body states, contacts, fat AABBs, solver order;
+* the reference-style API -- :class:`B2world`, :class:`B2bodyDef`, :class:`B2fixtureDef`,
+  :class:`B2polygonShape`, :class:`B2circleShape`, :class:`B2edgeShape`, :class:`B2body` --
+  with the names and argument meaning of box2d-rs (src/b2_world.rs, src/b2_body.rs,
+  src/b2_fixture.rs, src/shapes/*), so that code written against the reference ports by
+  renaming the import.
+
+The library is loaded eagerly; importing this package raises if ``libb2g.so`` has not been
+built (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libb2g.so")
+
+OK, ERR_INVALID_ARGUMENT, ERR_CAPACITY, ERR_DEVICE, ERR_DEGENERATE, ERR_UNSUPPORTED = range(6)
+B2_STATIC_BODY, B2_KINEMATIC_BODY, B2_DYNAMIC_BODY = 0, 1, 2
+SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX = 0, 1, 2, 4
+PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
+
+BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
+                     ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
+                     ("angular_velocity", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),
+                     ("gravity_scale", "<f4"), ("pad", "<f4")])
+SHAPE_DEF = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", "<f4"), ("params", "<f4", 20)])
+FIXTURE_DEF = np.dtype([("friction", "<f4"), ("restitution", "<f4"), ("restitution_threshold", "<f4"),
+                        ("density", "<f4"), ("is_sensor", "<i4"), ("category", "<u2"), ("mask", "<u2"),
+                        ("group", "<i2"), ("pad0", "<i2"), ("pad1", "<i4")])
+BODY_STATE = np.dtype([("p", "<f4", 2), ("q", "<f4", 2), ("c", "<f4", 2), ("a", "<f4"), ("v", "<f4", 2),
+                       ("w", "<f4"), ("c0", "<f4", 2), ("a0", "<f4"), ("sleep_time", "<f4"), ("awake", "<f4"),
+                       ("pad", "<f4")])
+POINT = np.dtype([("local_point", "<f4", 2), ("normal_impulse", "<f4"), ("tangent_impulse", "<f4"), ("id", "<u4"),
+                  ("pad", "<i4")])
+MANIFOLD = np.dtype([("type", "<i4"), ("point_count", "<i4"), ("local_normal", "<f4", 2),
+                     ("local_point", "<f4", 2), ("points", POINT, 2)])
+assert BODY_DEF.itemsize == 56 and SHAPE_DEF.itemsize == 96 and FIXTURE_DEF.itemsize == 32
+assert BODY_STATE.itemsize == 64 and MANIFOLD.itemsize == 72
+
+
+class B2gError(RuntimeError):
+    """A b2g C-ABI call returned a non-zero status."""
+
+    def __init__(self, status, message):
+        super().__init__("b2g error %d: %s" % (status, message))
+        self.status = status
+
+
+class _Profile(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_float), ("collide", ctypes.c_float), ("solve", ctypes.c_float),
+                ("broadphase", ctypes.c_float), ("pairs", ctypes.c_int32), ("touching", ctypes.c_int32),
+                ("colours", ctypes.c_int32), ("islands", ctypes.c_int32), ("position_iterations", ctypes.c_int32),
+                ("pad", ctypes.c_int32 * 3)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libb2g.so is not built (%s); run `make -C box2d-rs_amd`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, F = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
+    I64 = ctypes.c_int64
+    sig = {
+        "b2g_last_error": ([], ctypes.c_char_p),
+        "b2g_version": ([], ctypes.c_char_p),
+        "b2g_world_create": ([F, F, ctypes.POINTER(P)], I),
+        "b2g_world_destroy": ([P], I),
+        "b2g_world_set_flags": ([P, I, I, I, I], I),
+        "b2g_world_add": ([P, I, P, I, P, P, P, P], I),
+        "b2g_world_step": ([P, F, I, I], I),
+        "b2g_world_step_phase": ([P, I, F, I, I], I),
+        "b2g_world_body_count": ([P, P], I),
+        "b2g_world_fixture_count": ([P, P], I),
+        "b2g_world_contact_count": ([P, P], I),
+        "b2g_world_get_body_states": ([P, P, I], I),
+        "b2g_world_set_body_states": ([P, P, I], I),
+        "b2g_world_get_mass": ([P, P, I], I),
+        "b2g_world_get_fat_aabbs": ([P, P, I], I),
+        "b2g_world_set_fat_aabbs": ([P, P, I], I),
+        "b2g_world_get_contacts": ([P, I, P, P, P, P], I),
+        "b2g_world_get_solver_order": ([P, I, P, P, I, P, P], I),
+        "b2g_world_get_profile": ([P, P], I),
+        "b2g_collide_pairs": ([P, I, P, P, P, P], I),
+        "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+_lib = _load()
+EXPORTED = tuple(sorted(n for n in dir(_lib) if n.startswith("b2g_")))
+
+
+def version():
+    return _lib.b2g_version().decode()
+
+
+def _check(status):
+    if status != OK:
+        raise B2gError(status, _lib.b2g_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _as(arr, dtype):
+    """Structured array in this library's layout (fields matched by name)."""
+    if arr.dtype == dtype:
+        return np.ascontiguousarray(arr)
+    out = np.zeros(arr.shape, dtype)
+    for name in dtype.names:
+        if name in arr.dtype.names:
+            out[name] = arr[name]
+    return out
+
+
+class World:
+    """Array-oriented handle on one GPU world (one HIP stream)."""
+
+    def __init__(self, gravity=(0.0, -10.0)):
+        h = ctypes.c_void_p()
+        _check(_lib.b2g_world_create(float(gravity[0]), float(gravity[1]), ctypes.byref(h)))
+        self._h = h
+        self.gravity = (float(gravity[0]), float(gravity[1]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.b2g_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_flags(self, warm_starting=True, allow_sleep=False, continuous=False, block_solve=True):
+        _check(_lib.b2g_world_set_flags(self._h, int(warm_starting), int(allow_sleep), int(continuous),
+                                        int(block_solve)))
+
+    def add(self, body_defs, fixture_body, shape_defs, fixture_defs):
+        bd = _as(body_defs, BODY_DEF)
+        sd = _as(shape_defs, SHAPE_DEF)
+        fd = _as(fixture_defs, FIXTURE_DEF)
+        fb = np.ascontiguousarray(fixture_body, np.int32)
+        proxies = np.zeros(len(fb), np.int32)
+        _check(_lib.b2g_world_add(self._h, len(bd), _ptr(bd), len(fb), _ptr(fb), _ptr(sd), _ptr(fd), _ptr(proxies)))
+        return proxies
+
+    def step(self, dt=1.0 / 60.0, velocity_iterations=8, position_iterations=3):
+        _check(_lib.b2g_world_step(self._h, float(dt), int(velocity_iterations), int(position_iterations)))
+
+    def step_phase(self, phase, dt=1.0 / 60.0, velocity_iterations=8, position_iterations=3):
+        _check(_lib.b2g_world_step_phase(self._h, int(phase), float(dt), int(velocity_iterations),
+                                         int(position_iterations)))
+
+    def _count(self, fn):
+        out = ctypes.c_int32()
+        _check(fn(self._h, ctypes.byref(out)))
+        return out.value
+
+    @property
+    def body_count(self):
+        return self._count(_lib.b2g_world_body_count)
+
+    @property
+    def fixture_count(self):
+        return self._count(_lib.b2g_world_fixture_count)
+
+    @property
+    def contact_count(self):
+        return self._count(_lib.b2g_world_contact_count)
+
+    def body_states(self):
+        out = np.zeros(self.body_count, BODY_STATE)
+        _check(_lib.b2g_world_get_body_states(self._h, _ptr(out), len(out)))
+        return out
+
+    def set_body_states(self, states):
+        st = _as(states, BODY_STATE)
+        _check(_lib.b2g_world_set_body_states(self._h, _ptr(st), len(st)))
+
+    def mass(self):
+        out = np.zeros((self.body_count, 6), np.float32)
+        _check(_lib.b2g_world_get_mass(self._h, _ptr(out), len(out)))
+        return out
+
+    def fat_aabbs(self):
+        out = np.zeros((self.fixture_count, 4), np.float32)
+        _check(_lib.b2g_world_get_fat_aabbs(self._h, _ptr(out), len(out)))
+        return out
+
+    def set_fat_aabbs(self, fat):
+        fat = np.ascontiguousarray(fat, np.float32).reshape(-1, 4)
+        _check(_lib.b2g_world_set_fat_aabbs(self._h, _ptr(fat), len(fat)))
+
+    def contacts(self):
+        n = self.contact_count
+        fab = np.zeros((max(n, 1), 2), np.int32)
+        flags = np.zeros(max(n, 1), np.int32)
+        man = np.zeros(max(n, 1), MANIFOLD)
+        cnt = ctypes.c_int32()
+        _check(_lib.b2g_world_get_contacts(self._h, n, _ptr(fab), _ptr(flags), _ptr(man), ctypes.byref(cnt)))
+        k = min(cnt.value, n)
+        return fab[:k], flags[:k], man[:k]
+
+    def solver_order(self):
+        n = ctypes.c_int32()
+        nc = ctypes.c_int32()
+        _check(_lib.b2g_world_get_solver_order(self._h, 0, None, None, 0, ctypes.byref(n), ctypes.byref(nc)))
+        order = np.zeros(max(n.value, 1), np.int32)
+        offs = np.zeros(nc.value + 1, np.int32)
+        _check(_lib.b2g_world_get_solver_order(self._h, n.value, _ptr(order), _ptr(offs), len(offs), ctypes.byref(n),
+                                               ctypes.byref(nc)))
+        return order[:n.value], offs
+
+    def profile(self):
+        p = _Profile()
+        _check(_lib.b2g_world_get_profile(self._h, ctypes.byref(p)))
+        return {name: getattr(p, name) for name, _ in _Profile._fields_ if name != "pad"}
+
+    def collide_pairs(self, pairs, xf_a, xf_b):
+        pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        n = len(pairs)
+        xa = np.ascontiguousarray(xf_a, np.float32).reshape(n, 4)
+        xb = np.ascontiguousarray(xf_b, np.float32).reshape(n, 4)
+        out = np.zeros(max(n, 1), MANIFOLD)
+        _check(_lib.b2g_collide_pairs(self._h, n, _ptr(pairs), _ptr(xa), _ptr(xb), _ptr(out)))
+        return out[:n]
+
+    def broadphase_pairs(self, fat):
+        fat = np.ascontiguousarray(fat, np.float32).reshape(-1, 4)
+        cnt = ctypes.c_int64()
+        _check(_lib.b2g_broadphase_pairs(self._h, _ptr(fat), 0, None, ctypes.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), np.int32)
+        _check(_lib.b2g_broadphase_pairs(self._h, _ptr(fat), cnt.value, _ptr(out), ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+
+# ---- reference-style API ---------------------------------------------------------------------
+
+class B2vec2(tuple):
+    def __new__(cls, x=0.0, y=0.0):
+        return tuple.__new__(cls, (float(x), float(y)))
+
+    @property
+    def x(self):
+        return self[0]
+
+    @property
+    def y(self):
+        return self[1]
+
+
+class B2circleShape:
+    """src/shapes/b2_circle_shape.rs: m_p (local centre) and m_radius."""
+
+    def __init__(self, radius=0.0, p=(0.0, 0.0)):
+        self.m_radius = float(radius)
+        self.m_p = B2vec2(*p)
+
+    def _def(self):
+        d = np.zeros((), SHAPE_DEF)
+        d["type"] = SHAPE_CIRCLE
+        d["radius"] = self.m_radius
+        d["params"][:2] = self.m_p
+        return d
+
+
+class B2polygonShape:
+    """src/shapes/b2_polygon_shape.rs: set_as_box, set_as_box_angle, set (convex hull)."""
+
+    def __init__(self):
+        self._d = None
+
+    def set_as_box(self, hx, hy):
+        d = np.zeros((), SHAPE_DEF)
+        d["type"] = SHAPE_BOX
+        d["params"][:2] = (hx, hy)
+        self._d = d
+
+    def set_as_box_angle(self, hx, hy, center, angle):
+        d = np.zeros((), SHAPE_DEF)
+        d["type"] = SHAPE_BOX
+        d["flags"] = 1
+        d["params"][:5] = (hx, hy, center[0], center[1], angle)
+        self._d = d
+
+    def set(self, vertices):
+        v = np.asarray(vertices, np.float32).reshape(-1, 2)
+        if not 3 <= len(v) <= 8:
+            raise ValueError("a polygon needs 3..8 vertices")
+        d = np.zeros((), SHAPE_DEF)
+        d["type"] = SHAPE_POLYGON
+        d["count"] = len(v)
+        d["params"][:2 * len(v)] = v.ravel()
+        self._d = d
+
+    def _def(self):
+        if self._d is None:
+            raise ValueError("polygon shape not set")
+        return self._d
+
+
+class B2edgeShape:
+    """src/shapes/b2_edge_shape.rs: set_two_sided, set_one_sided."""
+
+    def __init__(self):
+        self._d = None
+
+    def set_two_sided(self, v1, v2):
+        d = np.zeros((), SHAPE_DEF)
+        d["type"] = SHAPE_EDGE
+        d["params"][:4] = (v1[0], v1[1], v2[0], v2[1])
+        self._d = d
+
+    def set_one_sided(self, v0, v1, v2, v3):
+        d = np.zeros((), SHAPE_DEF)
+        d["type"] = SHAPE_EDGE
+        d["flags"] = 1
+        d["params"][:8] = (v1[0], v1[1], v2[0], v2[1], v0[0], v0[1], v3[0], v3[1])
+        self._d = d
+
+    def _def(self):
+        if self._d is None:
+            raise ValueError("edge shape not set")
+        return self._d
+
+
+class B2filter:
+    def __init__(self, category_bits=0x0001, mask_bits=0xFFFF, group_index=0):
+        self.category_bits = category_bits
+        self.mask_bits = mask_bits
+        self.group_index = group_index
+
+
+class B2fixtureDef:
+    """src/b2_fixture.rs:44-58 defaults: friction 0.2, restitution 0, threshold 1, density 0."""
+
+    def __init__(self, shape=None, friction=0.2, restitution=0.0, restitution_threshold=1.0, density=0.0,
+                 is_sensor=False, filter=None):
+        self.shape = shape
+        self.friction = friction
+        self.restitution = restitution
+        self.restitution_threshold = restitution_threshold
+        self.density = density
+        self.is_sensor = is_sensor
+        self.filter = filter or B2filter()
+
+    def _def(self):
+        d = np.zeros((), FIXTURE_DEF)
+        d["friction"] = self.friction
+        d["restitution"] = self.restitution
+        d["restitution_threshold"] = self.restitution_threshold
+        d["density"] = self.density
+        d["is_sensor"] = int(self.is_sensor)
+        d["category"] = self.filter.category_bits
+        d["mask"] = self.filter.mask_bits
+        d["group"] = self.filter.group_index
+        return d
+
+
+class B2bodyDef:
+    """src/b2_body.rs:39-58 defaults."""
+
+    def __init__(self, body_type=B2_STATIC_BODY, position=(0.0, 0.0), angle=0.0, linear_velocity=(0.0, 0.0),
+                 angular_velocity=0.0, linear_damping=0.0, angular_damping=0.0, allow_sleep=True, awake=True,
+                 fixed_rotation=False, gravity_scale=1.0):
+        self.body_type = body_type
+        self.position = position
+        self.angle = angle
+        self.linear_velocity = linear_velocity
+        self.angular_velocity = angular_velocity
+        self.linear_damping = linear_damping
+        self.angular_damping = angular_damping
+        self.allow_sleep = allow_sleep
+        self.awake = awake
+        self.fixed_rotation = fixed_rotation
+        self.gravity_scale = gravity_scale
+
+    def _def(self):
+        d = np.zeros((), BODY_DEF)
+        d["type"] = self.body_type
+        d["allow_sleep"] = int(self.allow_sleep)
+        d["awake"] = int(self.awake)
+        d["fixed_rotation"] = int(self.fixed_rotation)
+        d["position"] = self.position
+        d["angle"] = self.angle
+        d["linear_velocity"] = self.linear_velocity
+        d["angular_velocity"] = self.angular_velocity
+        d["linear_damping"] = self.linear_damping
+        d["angular_damping"] = self.angular_damping
+        d["gravity_scale"] = self.gravity_scale
+        return d
+
+
+class B2body:
+    """Handle on one body of a :class:`B2world` (src/b2_body.rs getters)."""
+
+    def __init__(self, world, index):
+        self._world = world
+        self.index = index
+
+    def create_fixture(self, fixture_def):
+        return self._world._queue_fixture(self.index, fixture_def)
+
+    def create_fixture_by_shape(self, shape, density):
+        return self.create_fixture(B2fixtureDef(shape=shape, density=density))
+
+    def _state(self):
+        return self._world._states()[self.index]
+
+    def get_position(self):
+        return B2vec2(*self._state()["p"])
+
+    def get_angle(self):
+        return float(self._state()["a"])
+
+    def get_world_center(self):
+        return B2vec2(*self._state()["c"])
+
+    def get_linear_velocity(self):
+        return B2vec2(*self._state()["v"])
+
+    def get_angular_velocity(self):
+        return float(self._state()["w"])
+
+    def get_mass(self):
+        return float(self._world._world.mass()[self.index, 0])
+
+    def get_inertia(self):
+        m = self._world._world.mass()[self.index]
+        return float(m[2] + m[0] * (m[4] * m[4] + m[5] * m[5]))
+
+
+class B2world:
+    """``B2world`` (src/b2_world.rs) stepping on the GPU.
+
+    Bodies and fixtures are queued on the host and handed to the device in one bulk call on
+    the next step or query, in creation order, so proxy ids and mass data match the reference.
+    """
+
+    def __init__(self, gravity=(0.0, -10.0)):
+        self._world = World(gravity)
+        self._bodies = []
+        self._fixtures = []
+        self._n_flushed_bodies = 0
+        self._cache = None
+        self._world.set_flags(True, False, False, True)
+
+    @classmethod
+    def new(cls, gravity):
+        return cls(gravity)
+
+    def set_warm_starting(self, flag):
+        self._flags(warm_starting=flag)
+
+    def set_continuous_physics(self, flag):
+        if flag:
+            raise B2gError(ERR_UNSUPPORTED, "continuous physics (TOI) is not on the GPU path")
+
+    def set_allow_sleeping(self, flag):
+        if flag:
+            raise B2gError(ERR_UNSUPPORTED, "sleeping islands are not on the GPU path")
+
+    def _flags(self, warm_starting=True):
+        self._world.set_flags(warm_starting, False, False, True)
+
+    def create_body(self, body_def):
+        self._bodies.append(body_def._def())
+        self._cache = None
+        return B2body(self, self._n_flushed_bodies + len(self._bodies) - 1)
+
+    def _queue_fixture(self, body_index, fixture_def):
+        if fixture_def.shape is None:
+            raise ValueError("fixture definition needs a shape")
+        self._fixtures.append((body_index, fixture_def.shape._def(), fixture_def._def()))
+        self._cache = None
+
+    def _flush(self):
+        if not self._bodies and not self._fixtures:
+            return
+        base = self._n_flushed_bodies
+        bd = np.array(self._bodies, BODY_DEF) if self._bodies else np.zeros(0, BODY_DEF)
+        fb = np.array([b - base for b, _, _ in self._fixtures], np.int32)
+        sd = np.array([s for _, s, _ in self._fixtures], SHAPE_DEF) if self._fixtures else np.zeros(0, SHAPE_DEF)
+        fd = np.array([f for _, _, f in self._fixtures], FIXTURE_DEF) if self._fixtures else np.zeros(0, FIXTURE_DEF)
+        self._world.add(bd, fb, sd, fd)
+        self._n_flushed_bodies += len(self._bodies)
+        self._bodies = []
+        self._fixtures = []
+
+    def step(self, time_step, velocity_iterations, position_iterations):
+        self._flush()
+        self._world.step(time_step, velocity_iterations, position_iterations)
+        self._cache = None
+
+    def _states(self):
+        self._flush()
+        if self._cache is None:
+            self._cache = self._world.body_states()
+        return self._cache
+
+    def get_body_count(self):
+        self._flush()
+        return self._world.body_count
+
+    def get_contact_count(self):
+        self._flush()
+        return self._world.contact_count
+
+    def get_profile(self):
+        return self._world.profile()

@@ -1,0 +1,656 @@
+// b2g C ABI: world lifecycle, host-side creation of bodies and fixtures, upload, the step
+// driver and readbacks. See include/b2g.h for the contract of each entry point.
+#include <algorithm>
+#include <cstring>
+
+#include "b2g_internal.h"
+
+struct b2g_world {
+  b2g::World w;
+};
+
+namespace b2g {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+static constexpr int kBlock = 256;
+
+// carry manifolds of the previous set into the current one through old_index (new pairs get
+// empty manifolds): used when the current set has not been through a narrowphase yet
+__global__ void __launch_bounds__(kBlock) k_carry(int n, const int* __restrict__ old_index, const int2* __restrict__ pairs,
+                                                  const int4* __restrict__ fshape, const int* __restrict__ ohdr,
+                                                  const float4* __restrict__ ogeo, const float4* __restrict__ opt0,
+                                                  const float4* __restrict__ opt1, const uint2* __restrict__ oids,
+                                                  const int2* __restrict__ oab, int2* __restrict__ ab, int* __restrict__ hdr,
+                                                  float4* __restrict__ geo, float4* __restrict__ pt0,
+                                                  float4* __restrict__ pt1, uint2* __restrict__ ids,
+                                                  int* __restrict__ touching) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int o = old_index[k];
+  if (o >= 0) {
+    ab[k] = oab[o];
+    hdr[k] = ohdr[o];
+    geo[k] = ogeo[o];
+    pt0[k] = opt0[o];
+    pt1[k] = opt1[o];
+    ids[k] = oids[o];
+    touching[k] = (ohdr[o] >> 8) > 0;
+  } else {
+    int2 pr = pairs[k];
+    int lo = pr.x < pr.y ? pr.x : pr.y, hi = pr.x < pr.y ? pr.y : pr.x;
+    bool swap = registry_swap(fshape[lo].x & 0xff, fshape[hi].x & 0xff) == 1;
+    ab[k] = swap ? make_int2(hi, lo) : make_int2(lo, hi);
+    hdr[k] = 0;
+    geo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pt0[k] = geo[k];
+    pt1[k] = geo[k];
+    ids[k] = make_uint2(0u, 0u);
+    touching[k] = 0;
+  }
+}
+
+static int carry_manifolds(World* w) {
+  if (w->manifolds_valid[w->cur]) return B2G_OK;
+  int n = w->n_pairs;
+  int st;
+  if ((st = w->s.touch_flag.reserve(std::max(n, 1)))) return st;
+  ContactArrays& c = w->c[w->cur];
+  ContactArrays& o = w->c[1 - w->cur];
+  if (n > 0) {
+    hipLaunchKernelGGL(k_carry, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, c.old_index.p, c.pair.p,
+                       w->f.shape.p, o.hdr.p, o.geo.p, o.pt0.p, o.pt1.p, o.ids.p, o.ab.p, c.ab.p, c.hdr.p, c.geo.p,
+                       c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
+    B2G_HIP(hipGetLastError());
+  }
+  w->manifolds_valid[w->cur] = true;
+  return B2G_OK;
+}
+
+// ---- host-side creation (create_body / create_fixture / reset_mass_data) --------------------
+// src/private/dynamics/b2_body.rs:15-89 (body), :292-349 (reset_mass_data)
+static void reset_mass(World* w, int b) {
+  float4& ms = w->h_mass[b];
+  float4& pr = w->h_props[b];
+  float4& pos = w->h_pos[b];
+  float4& vel = w->h_vel[b];
+  const float4 xf = w->h_xf[b];
+  float mass = 0.0f, I = 0.0f, inv_mass = 0.0f, inv_I = 0.0f;
+  V2 lc = v2(0.0f, 0.0f);
+  int type = w->host_type[b];
+  if (type == B2G_STATIC || type == B2G_KINEMATIC) {
+    ms = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pr.x = 0.0f;
+    pos.x = xf.x;
+    pos.y = xf.y;
+    w->h_pos0[b] = pos;
+    return;
+  }
+  // fixtures of the body, newest first (the reference's fixture list order)
+  const std::vector<int>& fl = w->body_fixtures[b];
+  for (auto it = fl.rbegin(); it != fl.rend(); ++it) {
+    int f = *it;
+    float density = w->h_fdensity[f];
+    if (density == 0.0f) continue;
+    float m;
+    V2 c;
+    float i;
+    host_shape_mass(w->h_shapes[f], density, &m, &c, &i);
+    mass += m;
+    lc = lc + m * c;
+    I += i;
+  }
+  if (mass > 0.0f) {
+    inv_mass = 1.0f / mass;
+    lc = inv_mass * lc;
+  }
+  if (I > 0.0f && !w->body_defs[b].fixed_rotation) {
+    I -= mass * dot(lc, lc);
+    inv_I = 1.0f / I;
+  } else {
+    I = 0.0f;
+    inv_I = 0.0f;
+  }
+  V2 old_c = v2(pos.x, pos.y);
+  Xf t;
+  t.p = v2(xf.x, xf.y);
+  t.q.s = xf.z;
+  t.q.c = xf.w;
+  V2 c = mul(t, lc);
+  pos.x = c.x;
+  pos.y = c.y;
+  w->h_pos0[b] = pos;
+  V2 dv = cross(vel.z, c - old_c);
+  vel.x += dv.x;
+  vel.y += dv.y;
+  ms = make_float4(inv_mass, inv_I, lc.x, lc.y);
+  pr.x = mass;
+  w->h_I[b] = I;
+}
+
+static int add_items(World* w, int nbn, const b2g_body_def* bodies, int nf, const int32_t* fixture_body,
+                     const b2g_shape_def* shapes, const b2g_fixture_def* fixtures, int32_t* proxy_ids) {
+  for (int i = 0; i < nbn; ++i) {
+    const b2g_body_def& d = bodies[i];
+    if (d.type < 0 || d.type > 2) { set_error("invalid body type"); return B2G_ERR_INVALID_ARGUMENT; }
+    if (d.linear_damping < 0.0f || d.angular_damping < 0.0f) {
+      set_error("damping must be non-negative");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+  }
+  const int base = w->nb;
+  for (int k = 0; k < nf; ++k) {
+    // relative to the first body of this call; negative values address earlier bodies
+    int b = fixture_body[k];
+    if (base + b < 0 || b >= nbn) {
+      set_error("fixture_body index out of range");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+    if (fixtures[k].is_sensor) {
+      set_error("sensor fixtures are not supported on the GPU path");
+      return B2G_ERR_UNSUPPORTED;
+    }
+  }
+  // build all shapes first so that a degenerate polygon leaves the world unchanged
+  std::vector<HostShape> hs(nf);
+  for (int k = 0; k < nf; ++k) {
+    int st = build_host_shape(shapes[k], &hs[k]);
+    if (st != B2G_OK) {
+      set_error(st == B2G_ERR_DEGENERATE ? "degenerate polygon" : "unsupported shape type");
+      return st;
+    }
+  }
+  for (int i = 0; i < nbn; ++i) {
+    const b2g_body_def& d = bodies[i];
+    w->body_defs.push_back(d);
+    w->host_type.push_back(d.type);
+    Rot q = rot(d.angle);
+    w->h_xf.push_back(make_float4(d.position[0], d.position[1], q.s, q.c));
+    float4 p = make_float4(d.position[0], d.position[1], d.angle, 0.0f);
+    w->h_pos.push_back(p);
+    w->h_pos0.push_back(p);
+    w->h_vel.push_back(make_float4(d.linear_velocity[0], d.linear_velocity[1], d.angular_velocity, 0.0f));
+    w->h_mass.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    w->h_props.push_back(make_float4(0.0f, d.gravity_scale, d.linear_damping, d.angular_damping));
+    w->h_I.push_back(0.0f);
+    w->body_fixtures.emplace_back();
+  }
+  w->nb += nbn;
+  for (int k = 0; k < nf; ++k) {
+    int b = base + fixture_body[k];
+    const b2g_fixture_def& fd = fixtures[k];
+    const HostShape& s = hs[k];
+    int f = w->np;
+    w->h_shapes.push_back(s);
+    w->h_fbody.push_back(b);
+    w->h_fdensity.push_back(fd.density);
+    int off = int(w->h_pool.size());
+    if (s.type == kCircle) {
+      w->h_pool.push_back(make_float2(s.p.x, s.p.y));
+    } else if (s.type == kPolygon) {
+      for (int i = 0; i < s.count; ++i) w->h_pool.push_back(make_float2(s.v[i].x, s.v[i].y));
+      for (int i = 0; i < s.count; ++i) w->h_pool.push_back(make_float2(s.n[i].x, s.n[i].y));
+      w->h_pool.push_back(make_float2(s.p.x, s.p.y));
+    } else {
+      for (int i = 0; i < 4; ++i) w->h_pool.push_back(make_float2(s.v[i].x, s.v[i].y));
+    }
+    int radius_bits;
+    std::memcpy(&radius_bits, &s.radius, 4);
+    w->h_fshape.push_back(make_int4(s.type | (s.count << 8) | (s.one_sided << 16), off, radius_bits, 0));
+    w->h_fmat.push_back(make_float4(fd.friction, fd.restitution, fd.restitution_threshold, 0.0f));
+    w->h_ffilter.push_back(make_int2(int(fd.category_bits) | (int(fd.mask_bits) << 16), fd.group_index));
+    // create_proxies: AABB at the body transform, fattened by the AABB extension
+    // (src/private/dynamics/b2_fixture.rs:114-133, src/private/collision/b2_dynamic_tree.rs:81-99)
+    float4 x = w->h_xf[b];
+    Xf t;
+    t.p = v2(x.x, x.y);
+    t.q.s = x.z;
+    t.q.c = x.w;
+    Aabb a = shape_aabb(host_shape_view(s), t);
+    w->h_faabb.push_back(make_float4(a.lo.x, a.lo.y, a.hi.x, a.hi.y));
+    w->h_ffat.push_back(make_float4(a.lo.x - kAabbExtension, a.lo.y - kAabbExtension, a.hi.x + kAabbExtension,
+                                    a.hi.y + kAabbExtension));
+    w->body_fixtures[b].push_back(f);
+    w->np += 1;
+    if (fd.density > 0.0f) reset_mass(w, b);
+    // proxy id the reference's dynamic tree assigns in a create-only world: node 0 for the
+    // first leaf, then each insert allocates a parent and the leaf (2k - 1)
+    if (proxy_ids) proxy_ids[k] = f == 0 ? 0 : 2 * f - 1;
+  }
+  w->dirty = true;
+  w->need_broadphase = true;
+  return B2G_OK;
+}
+
+// ---- upload / download ---------------------------------------------------------------------
+template <class T>
+static int up(DevArray<T>& d, const std::vector<T>& h, hipStream_t s) {
+  int st = d.reserve(std::max<size_t>(h.size(), 1));
+  if (st) return st;
+  if (!h.empty()) B2G_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+  return B2G_OK;
+}
+template <class T>
+static int down(std::vector<T>& h, const DevArray<T>& d, size_t n, hipStream_t s) {
+  h.resize(n);
+  if (n) B2G_HIP(hipMemcpyAsync(h.data(), d.p, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+  return B2G_OK;
+}
+
+// pulls the device-owned dynamic state into the host mirrors
+static int download_state(World* w) {
+  if (w->dev_nb == 0 && w->dev_np == 0) return B2G_OK;
+  std::vector<float4> pos, vel, pos0, xf, fat, faabb;
+  int st;
+  if ((st = down(pos, w->b.pos, w->dev_nb, w->stream)) || (st = down(vel, w->b.vel, w->dev_nb, w->stream)) ||
+      (st = down(pos0, w->b.pos0, w->dev_nb, w->stream)) || (st = down(xf, w->b.xf, w->dev_nb, w->stream)) ||
+      (st = down(fat, w->f.fat, w->dev_np, w->stream)) || (st = down(faabb, w->f.aabb, w->dev_np, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  std::copy(pos.begin(), pos.end(), w->h_pos.begin());
+  std::copy(vel.begin(), vel.end(), w->h_vel.begin());
+  std::copy(pos0.begin(), pos0.end(), w->h_pos0.begin());
+  std::copy(xf.begin(), xf.end(), w->h_xf.begin());
+  std::copy(fat.begin(), fat.end(), w->h_ffat.begin());
+  std::copy(faabb.begin(), faabb.end(), w->h_faabb.begin());
+  return B2G_OK;
+}
+
+int upload_world(World* w) {
+  if (!w->dirty) return B2G_OK;
+  int st;
+  // contact sets index proxies, which only grow, so they stay valid; their offset arrays
+  // are extended with empty segments for the new proxies
+  for (int i = 0; i < 2; ++i) {
+    ContactArrays& c = w->c[i];
+    if (c.off.p && w->set_np[i] > 0 && w->np > w->set_np[i]) {
+      if ((st = c.off.reserve(w->np + 1, true, w->set_np[i] + 1))) return st;
+    }
+  }
+  std::vector<float4> force(w->nb, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  if (w->dev_nb > 0) {
+    std::vector<float4> f;
+    if ((st = down(f, w->b.force, w->dev_nb, w->stream))) return st;
+    B2G_HIP(hipStreamSynchronize(w->stream));
+    std::copy(f.begin(), f.end(), force.begin());
+  }
+  if ((st = up(w->b.pos, w->h_pos, w->stream)) || (st = up(w->b.vel, w->h_vel, w->stream)) ||
+      (st = up(w->b.pos0, w->h_pos0, w->stream)) || (st = up(w->b.xf, w->h_xf, w->stream)) ||
+      (st = up(w->b.mass, w->h_mass, w->stream)) || (st = up(w->b.props, w->h_props, w->stream)) ||
+      (st = up(w->b.force, force, w->stream)) || (st = up(w->b.type, w->host_type, w->stream)) ||
+      (st = up(w->f.body, w->h_fbody, w->stream)) || (st = up(w->f.shape, w->h_fshape, w->stream)) ||
+      (st = up(w->f.mat, w->h_fmat, w->stream)) || (st = up(w->f.filter, w->h_ffilter, w->stream)) ||
+      (st = up(w->f.aabb, w->h_faabb, w->stream)) || (st = up(w->f.fat, w->h_ffat, w->stream)) ||
+      (st = up(w->f.pool, w->h_pool, w->stream)) || (st = w->f.moved.reserve(std::max(w->np, 1))))
+    return st;
+  // extend the open contact-set offset arrays: new proxies have empty segments
+  for (int i = 0; i < 2; ++i) {
+    ContactArrays& c = w->c[i];
+    if (c.off.p && w->set_np[i] > 0 && w->np > w->set_np[i]) {
+      std::vector<int> last(1);
+      B2G_HIP(hipMemcpyAsync(last.data(), c.off.p + w->set_np[i], sizeof(int), hipMemcpyDeviceToHost, w->stream));
+      B2G_HIP(hipStreamSynchronize(w->stream));
+      std::vector<int> tail(w->np - w->set_np[i], last[0]);
+      B2G_HIP(hipMemcpyAsync(c.off.p + w->set_np[i] + 1, tail.data(), sizeof(int) * tail.size(),
+                             hipMemcpyHostToDevice, w->stream));
+      B2G_HIP(hipStreamSynchronize(w->stream));
+      w->set_np[i] = w->np;
+    }
+  }
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  w->dev_nb = w->nb;
+  w->dev_np = w->np;
+  w->dirty = false;
+  return B2G_OK;
+}
+
+static int ensure_uploaded(World* w) { return upload_world(w); }
+
+static int run_phase(World* w, int phase, float dt, int vi, int pi) {
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  if (phase == B2G_PHASE_COLLIDE) {
+    if (w->need_broadphase) {
+      if ((st = carry_manifolds(w))) return st;
+      if ((st = phase_broadphase(w))) return st;
+      w->manifolds_valid[w->cur] = false;
+      w->need_broadphase = false;
+    }
+    if ((st = phase_collide(w))) return st;
+    w->manifolds_valid[w->cur] = true;
+    return B2G_OK;
+  }
+  if (phase == B2G_PHASE_SOLVE) {
+    if (dt <= 0.0f) return B2G_OK;
+    float dt_ratio = w->inv_dt0 * dt;
+    if ((st = phase_solve(w, dt, dt_ratio, vi, pi))) return st;
+    return B2G_OK;
+  }
+  if (phase == B2G_PHASE_FIND) {
+    if (dt > 0.0f) {
+      if ((st = phase_synchronize(w))) return st;
+      if ((st = carry_manifolds(w))) return st;
+      if ((st = phase_broadphase(w))) return st;
+      w->manifolds_valid[w->cur] = false;
+      w->inv_dt0 = 1.0f / dt;
+    }
+    return B2G_OK;
+  }
+  set_error("unknown phase");
+  return B2G_ERR_INVALID_ARGUMENT;
+}
+
+}  // namespace b2g
+
+using namespace b2g;
+
+extern "C" {
+
+const char* b2g_last_error(void) { return g_error.c_str(); }
+const char* b2g_version(void) { return "b2g 0.1 (gfx950)"; }
+
+int b2g_world_create(float gx, float gy, b2g_world** out) {
+  if (!out) { set_error("null output pointer"); return B2G_ERR_INVALID_ARGUMENT; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device available");
+    return B2G_ERR_DEVICE;
+  }
+  b2g_world* h = new b2g_world();
+  World* w = &h->w;
+  w->gravity = v2(gx, gy);
+  if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+      w->counters.reserve(16) != B2G_OK ||
+      hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    set_error("world allocation failed");
+    delete h;
+    return B2G_ERR_DEVICE;
+  }
+  for (int i = 0; i < 8; ++i) (void)hipEventCreate(&w->ev[i]);
+  *out = h;
+  return B2G_OK;
+}
+
+int b2g_world_destroy(b2g_world* h) {
+  if (!h) return B2G_OK;
+  World* w = &h->w;
+  (void)hipStreamSynchronize(w->stream);
+  auto rel = [](auto& a) { a.release(); };
+  rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force);
+  rel(w->b.type);
+  rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
+  rel(w->f.pool);
+  for (auto& c : w->c) {
+    rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids);
+  }
+  SolverArrays& s = w->s;
+  rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
+  rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);
+  rel(s.plp); rel(s.pgeo); rel(s.plc); rel(s.prad); rel(s.deg); rel(s.adj_off); rel(s.adj_cursor); rel(s.adj);
+  rel(s.color_state); rel(s.color_count); rel(s.color_off); rel(s.color_cursor); rel(s.order); rel(s.parent);
+  rel(s.isl_minsep); rel(s.isl_done);
+  BroadphaseArrays& bp = w->bp;
+  rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.node_parent);
+  rel(bp.node_child); rel(bp.node_aabb); rel(bp.node_flag); rel(bp.count); rel(bp.bounds); rel(bp.scan_tmp);
+  rel(w->counters);
+  if (w->h_counters) (void)hipHostFree(w->h_counters);
+  for (int i = 0; i < 8; ++i)
+    if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+  if (w->stream) (void)hipStreamDestroy(w->stream);
+  delete h;
+  return B2G_OK;
+}
+
+int b2g_world_set_flags(b2g_world* h, int32_t warm_starting, int32_t allow_sleep, int32_t continuous,
+                        int32_t block_solve) {
+  if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  if (allow_sleep || continuous) {
+    set_error("allow_sleep and continuous physics are not supported on the GPU path");
+    return B2G_ERR_UNSUPPORTED;
+  }
+  h->w.warm_starting = warm_starting != 0;
+  h->w.block_solve = block_solve != 0;
+  return B2G_OK;
+}
+
+int b2g_world_add(b2g_world* h, int32_t n_bodies, const b2g_body_def* bodies, int32_t n_fixtures,
+                  const int32_t* fixture_body, const b2g_shape_def* shapes, const b2g_fixture_def* fixtures,
+                  int32_t* proxy_ids) {
+  if (!h || n_bodies < 0 || n_fixtures < 0 || (n_bodies > 0 && !bodies) ||
+      (n_fixtures > 0 && (!fixture_body || !shapes || !fixtures))) {
+    set_error("b2g_world_add: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if (w->dev_nb > 0 && (st = download_state(w))) return st;
+  return add_items(w, n_bodies, bodies, n_fixtures, fixture_body, shapes, fixtures, proxy_ids);
+}
+
+int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {
+  if (!h || vi < 0 || pi < 0) { set_error("b2g_world_step: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  (void)hipEventRecord(w->ev[0], w->stream);
+  if ((st = run_phase(w, B2G_PHASE_COLLIDE, dt, vi, pi))) return st;
+  (void)hipEventRecord(w->ev[1], w->stream);
+  if ((st = run_phase(w, B2G_PHASE_SOLVE, dt, vi, pi))) return st;
+  (void)hipEventRecord(w->ev[2], w->stream);
+  if ((st = run_phase(w, B2G_PHASE_FIND, dt, vi, pi))) return st;
+  (void)hipEventRecord(w->ev[3], w->stream);
+  w->profile_pending = true;
+  return B2G_OK;
+}
+
+int b2g_world_step_phase(b2g_world* h, int32_t phase, float dt, int32_t vi, int32_t pi) {
+  if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  int st = run_phase(&h->w, phase, dt, vi, pi);
+  if (st == B2G_OK && hipStreamSynchronize(h->w.stream) != hipSuccess) {
+    set_error("device error");
+    return B2G_ERR_DEVICE;
+  }
+  return st;
+}
+
+int b2g_world_body_count(b2g_world* h, int32_t* out) {
+  if (!h || !out) { set_error("invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  *out = h->w.nb;
+  return B2G_OK;
+}
+int b2g_world_fixture_count(b2g_world* h, int32_t* out) {
+  if (!h || !out) { set_error("invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  *out = h->w.np;
+  return B2G_OK;
+}
+int b2g_world_contact_count(b2g_world* h, int32_t* out) {
+  if (!h || !out) { set_error("invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  *out = h->w.n_pairs;
+  return B2G_OK;
+}
+
+int b2g_world_get_body_states(b2g_world* h, b2g_body_state* out, int32_t n) {
+  if (!h || !out || n != h->w.nb) { set_error("b2g_world_get_body_states: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  std::vector<float4> pos, vel, pos0, xf;
+  if ((st = down(pos, w->b.pos, n, w->stream)) || (st = down(vel, w->b.vel, n, w->stream)) ||
+      (st = down(pos0, w->b.pos0, n, w->stream)) || (st = down(xf, w->b.xf, n, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  for (int i = 0; i < n; ++i) {
+    b2g_body_state& s = out[i];
+    s.p[0] = xf[i].x; s.p[1] = xf[i].y; s.q[0] = xf[i].z; s.q[1] = xf[i].w;
+    s.c[0] = pos[i].x; s.c[1] = pos[i].y; s.a = pos[i].z;
+    s.v[0] = vel[i].x; s.v[1] = vel[i].y; s.w = vel[i].z;
+    s.c0[0] = pos0[i].x; s.c0[1] = pos0[i].y; s.a0 = pos0[i].z;
+    s.sleep_time = 0.0f;
+    s.awake = w->host_type[i] != B2G_STATIC ? 1.0f : 0.0f;
+    s.pad = 0.0f;
+  }
+  return B2G_OK;
+}
+
+int b2g_world_set_body_states(b2g_world* h, const b2g_body_state* in, int32_t n) {
+  if (!h || !in || n != h->w.nb) { set_error("b2g_world_set_body_states: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  std::vector<float4> pos(n), vel(n), pos0(n), xf(n);
+  for (int i = 0; i < n; ++i) {
+    const b2g_body_state& s = in[i];
+    pos[i] = make_float4(s.c[0], s.c[1], s.a, 0.0f);
+    vel[i] = make_float4(s.v[0], s.v[1], s.w, 0.0f);
+    pos0[i] = make_float4(s.c0[0], s.c0[1], s.a0, 0.0f);
+    xf[i] = make_float4(s.p[0], s.p[1], s.q[0], s.q[1]);
+  }
+  if ((st = up(w->b.pos, pos, w->stream)) || (st = up(w->b.vel, vel, w->stream)) ||
+      (st = up(w->b.pos0, pos0, w->stream)) || (st = up(w->b.xf, xf, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  w->h_pos = pos;
+  w->h_vel = vel;
+  w->h_pos0 = pos0;
+  w->h_xf = xf;
+  return B2G_OK;
+}
+
+int b2g_world_get_mass(b2g_world* h, float* out, int32_t n) {
+  if (!h || !out || n != h->w.nb) { set_error("b2g_world_get_mass: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  for (int i = 0; i < n; ++i) {
+    float* o = out + 6 * i;
+    o[0] = w->h_props[i].x;
+    o[1] = w->h_mass[i].x;
+    o[2] = w->h_I[i];
+    o[3] = w->h_mass[i].y;
+    o[4] = w->h_mass[i].z;
+    o[5] = w->h_mass[i].w;
+  }
+  return B2G_OK;
+}
+
+int b2g_world_get_fat_aabbs(b2g_world* h, float* out, int32_t n) {
+  if (!h || !out || n != h->w.np) { set_error("b2g_world_get_fat_aabbs: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  if (n) B2G_HIP(hipMemcpyAsync(out, w->f.fat.p, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream));
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  return B2G_OK;
+}
+
+int b2g_world_set_fat_aabbs(b2g_world* h, const float* in, int32_t n) {
+  if (!h || !in || n != h->w.np) { set_error("b2g_world_set_fat_aabbs: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  if (n) B2G_HIP(hipMemcpyAsync(w->f.fat.p, in, sizeof(float4) * n, hipMemcpyHostToDevice, w->stream));
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  std::memcpy(w->h_ffat.data(), in, sizeof(float4) * n);
+  return B2G_OK;
+}
+
+int b2g_world_get_contacts(b2g_world* h, int32_t cap, int32_t* fixtures_ab, int32_t* flags, b2g_manifold* manifolds,
+                           int32_t* count) {
+  if (!h || !count || cap < 0) { set_error("b2g_world_get_contacts: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  if ((st = carry_manifolds(w))) return st;
+  int n = w->n_pairs;
+  *count = n;
+  int m = std::min(cap, n);
+  if (m == 0) return B2G_OK;
+  ContactArrays& c = w->c[w->cur];
+  std::vector<int2> ab;
+  std::vector<int> hdr, tf;
+  std::vector<float4> geo, p0, p1;
+  std::vector<uint2> ids;
+  if ((st = down(ab, c.ab, m, w->stream)) || (st = down(hdr, c.hdr, m, w->stream)) ||
+      (st = down(geo, c.geo, m, w->stream)) || (st = down(p0, c.pt0, m, w->stream)) ||
+      (st = down(p1, c.pt1, m, w->stream)) || (st = down(ids, c.ids, m, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  for (int i = 0; i < m; ++i) {
+    if (fixtures_ab) { fixtures_ab[2 * i] = ab[i].x; fixtures_ab[2 * i + 1] = ab[i].y; }
+    if (flags) flags[i] = (hdr[i] >> 8) > 0 ? 1 : 0;
+    if (manifolds) {
+      b2g_manifold& mf = manifolds[i];
+      mf.type = hdr[i] & 0xff;
+      mf.point_count = hdr[i] >> 8;
+      mf.local_normal[0] = geo[i].x; mf.local_normal[1] = geo[i].y;
+      mf.local_point[0] = geo[i].z; mf.local_point[1] = geo[i].w;
+      mf.points[0].local_point[0] = p0[i].x; mf.points[0].local_point[1] = p0[i].y;
+      mf.points[0].normal_impulse = p0[i].z; mf.points[0].tangent_impulse = p0[i].w;
+      mf.points[0].id = ids[i].x; mf.points[0].pad = 0;
+      mf.points[1].local_point[0] = p1[i].x; mf.points[1].local_point[1] = p1[i].y;
+      mf.points[1].normal_impulse = p1[i].z; mf.points[1].tangent_impulse = p1[i].w;
+      mf.points[1].id = ids[i].y; mf.points[1].pad = 0;
+    }
+  }
+  return B2G_OK;
+}
+
+int b2g_world_get_solver_order(b2g_world* h, int32_t cap, int32_t* contact_order, int32_t* colour_offsets,
+                               int32_t colour_cap, int32_t* n_constraints, int32_t* n_colours) {
+  if (!h || !n_constraints || !n_colours) { set_error("invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  *n_constraints = w->n_touch;
+  *n_colours = w->n_colors;
+  int m = std::min(cap, w->n_touch);
+  if (contact_order && m > 0) {
+    B2G_HIP(hipMemcpyAsync(contact_order, w->s.k_contact.p, sizeof(int) * m, hipMemcpyDeviceToHost, w->stream));
+    B2G_HIP(hipStreamSynchronize(w->stream));
+  }
+  if (colour_offsets)
+    for (int i = 0; i < std::min<int>(colour_cap, int(w->h_color_off.size())); ++i) colour_offsets[i] = w->h_color_off[i];
+  return B2G_OK;
+}
+
+int b2g_world_get_profile(b2g_world* h, b2g_profile* out) {
+  if (!h || !out) { set_error("invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  std::memset(out, 0, sizeof(*out));
+  if (w->profile_pending) {
+    (void)hipEventSynchronize(w->ev[3]);
+    (void)hipEventElapsedTime(&w->prof.collide_ms, w->ev[0], w->ev[1]);
+    (void)hipEventElapsedTime(&w->prof.solve_ms, w->ev[1], w->ev[2]);
+    (void)hipEventElapsedTime(&w->prof.broadphase_ms, w->ev[2], w->ev[3]);
+    (void)hipEventElapsedTime(&w->prof.step_ms, w->ev[0], w->ev[3]);
+    w->profile_pending = false;
+  }
+  out->step = w->prof.step_ms;
+  out->collide = w->prof.collide_ms;
+  out->solve = w->prof.solve_ms;
+  out->broadphase = w->prof.broadphase_ms;
+  out->pairs = w->n_pairs;
+  out->touching = w->n_touch;
+  out->colours = w->n_colors;
+  out->position_iterations = w->pos_iters_run;
+  return B2G_OK;
+}
+
+int b2g_collide_pairs(b2g_world* h, int32_t n, const int32_t* pairs, const float* xf_a, const float* xf_b,
+                      b2g_manifold* out) {
+  if (!h || n < 0 || (n > 0 && (!pairs || !xf_a || !xf_b || !out))) {
+    set_error("b2g_collide_pairs: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  int st;
+  if ((st = ensure_uploaded(&h->w))) return st;
+  return collide_fixture_pairs(&h->w, n, pairs, xf_a, xf_b, out);
+}
+
+int b2g_broadphase_pairs(b2g_world* h, const float* fat_aabbs, int64_t cap, int32_t* pairs, int64_t* count) {
+  if (!h || !fat_aabbs || !count || cap < 0 || (cap > 0 && !pairs)) {
+    set_error("b2g_broadphase_pairs: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  int st;
+  if ((st = ensure_uploaded(&h->w))) return st;
+  return broadphase_pairs_for(&h->w, fat_aabbs, cap, pairs, count);
+}
+
+}  // extern "C"

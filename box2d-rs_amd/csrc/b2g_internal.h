@@ -1,0 +1,247 @@
+// b2g internals: world state, device buffers and the phase drivers shared by the .hip files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/b2g.h"
+#include "b2g_collide.h"
+
+namespace b2g {
+
+void set_error(const std::string& msg);
+#define B2G_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                          \
+      return B2G_ERR_DEVICE;                                                                 \
+    }                                                                                        \
+  } while (0)
+
+// A typed device array with capacity growth.
+template <class T>
+struct DevArray {
+  T* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t n, bool keep = false, size_t keep_n = 0) {
+    if (n <= cap) return B2G_OK;
+    size_t ncap = cap ? cap : 64;
+    while (ncap < n) ncap *= 2;
+    T* q = nullptr;
+    if (hipMalloc(&q, ncap * sizeof(T)) != hipSuccess) {
+      set_error("hipMalloc failed");
+      return B2G_ERR_DEVICE;
+    }
+    if (keep && p && keep_n)
+      if (hipMemcpy(q, p, keep_n * sizeof(T), hipMemcpyDeviceToDevice) != hipSuccess) {
+        set_error("hipMemcpy failed");
+        return B2G_ERR_DEVICE;
+      }
+    if (p) (void)hipFree(p);
+    p = q;
+    cap = ncap;
+    return B2G_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Host-side shape after construction (hull, box, ...): the pool layout written to the device.
+struct HostShape {
+  int type = kCircle;
+  int count = 0;
+  int one_sided = 0;
+  float radius = 0.0f;
+  V2 p = {0.0f, 0.0f};       // circle centre / polygon centroid
+  V2 v[8] = {};              // polygon vertices / edge v1, v2, v0, v3
+  V2 n[8] = {};              // polygon normals
+};
+int build_host_shape(const b2g_shape_def& def, HostShape* out);  // B2G_OK or B2G_ERR_DEGENERATE
+void host_shape_mass(const HostShape& s, float density, float* mass, V2* center, float* I);
+ShapeV host_shape_view(const HostShape& s);
+
+// Device-side body arrays (SoA, index = body creation order).
+struct BodyArrays {
+  DevArray<float4> pos;    // sweep c.x, c.y, a, 0
+  DevArray<float4> vel;    // v.x, v.y, w, 0
+  DevArray<float4> pos0;   // sweep c0.x, c0.y, a0, 0
+  DevArray<float4> xf;     // p.x, p.y, sin, cos
+  DevArray<float4> mass;   // inv_mass, inv_I, local_center.x, local_center.y
+  DevArray<float4> props;  // mass, gravity_scale, linear_damping, angular_damping
+  DevArray<float4> force;  // force.x, force.y, torque, 0
+  DevArray<int> type;      // 0 static, 1 kinematic, 2 dynamic
+};
+
+// Device-side proxy arrays (one proxy per fixture; index = fixture creation order, which is
+// also the order of the reference proxy ids).
+struct ProxyArrays {
+  DevArray<int> body;
+  DevArray<int4> shape;     // type | count << 8 | one_sided << 16, pool offset, radius bits, 0
+  DevArray<float4> mat;     // friction, restitution, restitution threshold, 0
+  DevArray<int2> filter;    // category | mask << 16, group
+  DevArray<float4> aabb;    // swept AABB (proxy.aabb)
+  DevArray<float4> fat;     // tree fat AABB
+  DevArray<int> moved;      // the tree's moved flag (set when the fat AABB was re-fattened)
+  DevArray<float2> pool;    // shape vertex pool
+};
+
+// Contact set: pairs grouped by their emitting proxy, plus manifolds.
+struct ContactArrays {
+  DevArray<int2> pair;       // (emitter proxy, other proxy)
+  DevArray<int> off;         // CSR offsets by emitter proxy (np + 1)
+  DevArray<int> old_index;   // index of the same pair in the previous set, or -1
+  DevArray<int2> ab;         // fixtures A, B after registry ordering
+  DevArray<int> hdr;         // manifold type | point_count << 8
+  DevArray<float4> geo;      // local normal, local point
+  DevArray<float4> pt0, pt1; // local point, normal impulse, tangent impulse
+  DevArray<uint2> ids;
+};
+
+// Per-step solver arrays (indexed by constraint, colour-sorted).
+struct SolverArrays {
+  DevArray<int> touch_flag, touch_scan, touch_list;   // compaction of touching contacts
+  DevArray<int> k_contact;   // contact index of constraint
+  DevArray<int2> k_bodies;
+  DevArray<int4> k_info;     // manifold type, point count, velocity point count, island
+  DevArray<float4> vnf;      // normal.x, normal.y, friction, tangent_speed
+  DevArray<float4> vm;       // mA, iA, mB, iB
+  DevArray<float4> vr0, vr1; // rA.x, rA.y, rB.x, rB.y per point
+  DevArray<float4> vms;      // normal_mass0, tangent_mass0, normal_mass1, tangent_mass1
+  DevArray<float2> vbias;    // velocity bias per point
+  DevArray<float4> K, NM;    // block matrix and its inverse (ex.x, ex.y, ey.x, ey.y)
+  DevArray<float4> lam;      // normal0, tangent0, normal1, tangent1 impulses
+  DevArray<float4> plp;      // position constraint local points 0, 1
+  DevArray<float4> pgeo;     // local normal, local point
+  DevArray<float4> plc;      // local centre A, local centre B
+  DevArray<float2> prad;     // radius A, radius B
+  // colouring
+  DevArray<int> deg, adj_off, adj_cursor, adj;  // body -> constraints (dynamic bodies only)
+  DevArray<int> color_state;                    // (round << 8) | colour, or -1
+  DevArray<int> color_count, color_off, color_cursor;
+  DevArray<int> order;                          // colour-sorted constraint -> compacted index
+  // islands
+  DevArray<int> parent, isl_minsep, isl_done;
+};
+
+struct BroadphaseArrays {
+  DevArray<unsigned> keys, keys_alt;
+  DevArray<unsigned> vals, vals_alt;
+  DevArray<unsigned> hist;       // radix histograms
+  DevArray<int> node_parent;     // 2n-1 nodes
+  DevArray<int2> node_child;     // internal nodes
+  DevArray<float4> node_aabb;    // all nodes
+  DevArray<int> node_flag;
+  DevArray<int> count;           // pairs per proxy
+  DevArray<float4> bounds;       // scene bounds scratch
+  DevArray<int> scan_tmp;        // block sums for scans
+};
+
+struct StepProfile {
+  float collide_ms = 0, solve_ms = 0, broadphase_ms = 0, step_ms = 0;
+};
+
+struct World {
+  // settings
+  V2 gravity;
+  bool warm_starting = true, allow_sleep = false, continuous = false, block_solve = true;
+  float inv_dt0 = 0.0f;
+  // host mirrors of static data
+  std::vector<b2g_body_def> body_defs;
+  std::vector<int> host_type;
+  std::vector<float4> h_pos, h_pos0, h_vel, h_xf, h_mass, h_props;
+  std::vector<float> h_I;
+  std::vector<std::vector<int>> body_fixtures;  // creation order
+  std::vector<int> h_fbody;
+  std::vector<float> h_fdensity;
+  std::vector<int4> h_fshape;
+  std::vector<float4> h_fmat, h_ffat, h_faabb;
+  std::vector<int2> h_ffilter;
+  std::vector<float2> h_pool;
+  std::vector<HostShape> h_shapes;
+  int nb = 0, np = 0;          // bodies, proxies on the host
+  int dev_nb = 0, dev_np = 0;  // uploaded
+  bool need_broadphase = false;  // new fixtures: find pairs before the next collide
+  bool dirty = false;            // host items not yet on the device
+  bool manifolds_valid[2] = {true, true};  // set went through a narrowphase (or a carry)
+  bool profile_pending = false;
+  // device state
+  hipStream_t stream = nullptr;
+  BodyArrays b;
+  ProxyArrays f;
+  ContactArrays c[2];  // current and previous contact sets
+  int cur = 0;         // index of the current contact set in c[]
+  int n_pairs = 0;     // pairs in the current set
+  int set_np[2] = {0, 0};  // proxy count when each contact set was built (0: never built)
+  SolverArrays s;
+  BroadphaseArrays bp;
+  DevArray<int> counters;   // small device counters
+  int* h_counters = nullptr;  // pinned host mirror
+  int n_touch = 0, n_colors = 0, n_islands = 0;
+  std::vector<int> h_color_off;
+  int pos_iters_run = 0;
+  StepProfile prof;
+  hipEvent_t ev[8] = {};
+};
+
+// phase drivers (each returns a B2G_* status)
+int upload_world(World* w);
+int phase_broadphase(World* w);  // fat AABBs -> next contact set (pairs + old indices)
+int phase_collide(World* w);     // narrowphase on the current contact set
+int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters);
+int phase_synchronize(World* w); // swept + fat AABBs of non-static proxies
+// primitives
+int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev);
+int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits);
+// stateless collide of fixture pairs under given transforms (parity entry point)
+int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
+                          b2g_manifold* out);
+int broadphase_pairs_for(World* w, const float* fat, int64_t cap, int32_t* out, int64_t* count);
+
+// launch geometry helpers
+inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
+
+// float <-> order-preserving int for atomicMin on floats
+__host__ __device__ inline int float_flip(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__host__ __device__ inline float float_unflip(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+// device shape loader
+__device__ inline ShapeV load_shape(int4 h, const float2* __restrict__ pool) {
+  ShapeV s;
+  s.type = h.x & 0xff;
+  s.count = (h.x >> 8) & 0xff;
+  s.one_sided = (h.x >> 16) & 0xff;
+  s.radius = __int_as_float(h.z);
+  const float2* q = pool + h.y;
+  if (s.type == kCircle) {
+    s.p = v2(q[0].x, q[0].y);
+  } else if (s.type == kPolygon) {
+    for (int i = 0; i < s.count; ++i) {
+      s.v[i] = v2(q[i].x, q[i].y);
+      s.n[i] = v2(q[s.count + i].x, q[s.count + i].y);
+    }
+    s.p = v2(q[2 * s.count].x, q[2 * s.count].y);
+  } else {
+    for (int i = 0; i < 4; ++i) s.v[i] = v2(q[i].x, q[i].y);
+    s.p = v2(0.0f, 0.0f);
+  }
+  return s;
+}
+
+__device__ inline Xf xf_of(float4 t) {
+  Xf x;
+  x.p = v2(t.x, t.y);
+  x.q.s = t.z;
+  x.q.c = t.w;
+  return x;
+}
+
+}  // namespace b2g

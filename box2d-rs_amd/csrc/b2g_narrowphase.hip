@@ -1,0 +1,166 @@
+// b2g narrowphase: B2contactManager::collide + B2contact::update for every contact of the
+// current set, one thread per contact (src/private/dynamics/b2_contact_manager.rs:83-171,
+// src/private/dynamics/b2_contact.rs:103-217).
+//
+// The contact set is exactly the set of fat-AABB-overlapping pairs (see b2g_broadphase.hip),
+// so the reference's "destroy if the fat AABBs no longer overlap" branch never fires here.
+// Each thread evaluates the manifold with the bodies' current transforms, then copies the
+// normal/tangent impulses of the previous manifold of the same pair by contact-feature id.
+// Manifolds are stored SoA (header, geometry, two points, ids) so the writes of a wave are
+// coalesced.
+#include "b2g_internal.h"
+
+namespace b2g {
+
+static constexpr int kBlock = 128;
+
+__device__ inline void store_manifold(int k, const Manifold& m, const float imp[4], int* __restrict__ hdr,
+                                      float4* __restrict__ geo, float4* __restrict__ pt0, float4* __restrict__ pt1,
+                                      uint2* __restrict__ ids) {
+  hdr[k] = m.type | (m.count << 8);
+  geo[k] = make_float4(m.local_normal.x, m.local_normal.y, m.local_point.x, m.local_point.y);
+  pt0[k] = make_float4(m.lp[0].x, m.lp[0].y, imp[0], imp[1]);
+  pt1[k] = make_float4(m.lp[1].x, m.lp[1].y, imp[2], imp[3]);
+  ids[k] = make_uint2(m.id[0], m.id[1]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restrict__ pairs,
+                                                    const int* __restrict__ old_index, const int* __restrict__ fbody,
+                                                    const int4* __restrict__ fshape, const float2* __restrict__ pool,
+                                                    const float4* __restrict__ bxf,
+                                                    // previous manifolds
+                                                    const int* __restrict__ ohdr, const float4* __restrict__ opt0,
+                                                    const float4* __restrict__ opt1, const uint2* __restrict__ oids,
+                                                    // outputs
+                                                    int2* __restrict__ ab, int* __restrict__ hdr,
+                                                    float4* __restrict__ geo, float4* __restrict__ pt0,
+                                                    float4* __restrict__ pt1, uint2* __restrict__ ids,
+                                                    int* __restrict__ touching) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int2 pr = pairs[k];
+  // the reference pair is (lower proxy id, higher proxy id); b2_contact_create then lets the
+  // registry decide which fixture is A
+  int lo = pr.x < pr.y ? pr.x : pr.y, hi = pr.x < pr.y ? pr.y : pr.x;
+  int4 hq = fshape[lo], hp = fshape[hi];
+  bool swap = registry_swap(hq.x & 0xff, hp.x & 0xff) == 1;
+  int fa = swap ? hi : lo, fb = swap ? lo : hi;
+  int4 ha = swap ? hp : hq, hb = swap ? hq : hp;
+  ShapeV sa = load_shape(ha, pool);
+  ShapeV sb = load_shape(hb, pool);
+  Xf xa = xf_of(bxf[fbody[fa]]);
+  Xf xb = xf_of(bxf[fbody[fb]]);
+  Manifold m;
+  evaluate(&m, sa, xa, sb, xb);
+  float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int oi = old_index[k];
+  if (oi >= 0 && m.count > 0) {
+    int ocount = ohdr[oi] >> 8;
+    if (ocount > 0) {
+      uint2 oid = oids[oi];
+      float4 o0 = opt0[oi], o1 = opt1[oi];
+      for (int i = 0; i < m.count; ++i) {
+        if (m.id[i] == oid.x) { imp[2 * i] = o0.z; imp[2 * i + 1] = o0.w; }
+        else if (ocount > 1 && m.id[i] == oid.y) { imp[2 * i] = o1.z; imp[2 * i + 1] = o1.w; }
+      }
+    }
+  }
+  ab[k] = make_int2(fa, fb);
+  store_manifold(k, m, imp, hdr, geo, pt0, pt1, ids);
+  touching[k] = m.count > 0 ? 1 : 0;
+}
+
+int phase_collide(World* w) {
+  int n = w->n_pairs;
+  int st;
+  if ((st = w->s.touch_flag.reserve(std::max(n, 1)))) return st;
+  if (n == 0) return B2G_OK;
+  ContactArrays& c = w->c[w->cur];
+  ContactArrays& o = w->c[1 - w->cur];
+  hipLaunchKernelGGL(k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, c.pair.p, c.old_index.p,
+                     w->f.body.p, w->f.shape.p, w->f.pool.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p, o.ids.p, c.ab.p,
+                     c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
+  B2G_HIP(hipGetLastError());
+  return B2G_OK;
+}
+
+// ---- stateless entry point ----------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_collide_given(int n, const int2* __restrict__ pairs,
+                                                          const float4* __restrict__ xa4, const float4* __restrict__ xb4,
+                                                          const int4* __restrict__ fshape,
+                                                          const float2* __restrict__ pool, int* __restrict__ hdr,
+                                                          float4* __restrict__ geo, float4* __restrict__ pt0,
+                                                          float4* __restrict__ pt1, uint2* __restrict__ ids) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int2 pr = pairs[k];
+  ShapeV sa = load_shape(fshape[pr.x], pool);
+  ShapeV sb = load_shape(fshape[pr.y], pool);
+  Manifold m;
+  evaluate(&m, sa, xf_of(xa4[k]), sb, xf_of(xb4[k]));
+  float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  store_manifold(k, m, imp, hdr, geo, pt0, pt1, ids);
+}
+
+int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
+                          b2g_manifold* out) {
+  if (n <= 0) return B2G_OK;
+  for (int i = 0; i < 2 * n; ++i)
+    if (pairs[i] < 0 || pairs[i] >= w->np) {
+      set_error("b2g_collide_pairs: fixture index out of range");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+  for (int i = 0; i < n; ++i) {
+    int ta = w->h_fshape[pairs[2 * i]].x & 0xff, tb = w->h_fshape[pairs[2 * i + 1]].x & 0xff;
+    if (registry_swap(ta, tb) != 0) {
+      set_error("b2g_collide_pairs: pair is not in registry order (A must be the primary shape type)");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+  }
+  DevArray<int2> dp;
+  DevArray<float4> da, db, geo, p0, p1;
+  DevArray<int> hdr;
+  DevArray<uint2> ids;
+  int st;
+  if ((st = dp.reserve(n)) || (st = da.reserve(n)) || (st = db.reserve(n)) || (st = geo.reserve(n)) ||
+      (st = p0.reserve(n)) || (st = p1.reserve(n)) || (st = hdr.reserve(n)) || (st = ids.reserve(n)))
+    return st;
+  std::vector<int> hh(n);
+  std::vector<float4> hg(n), h0(n), h1(n);
+  std::vector<uint2> hi(n);
+  bool ok = hipMemcpyAsync(dp.p, pairs, sizeof(int2) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess &&
+            hipMemcpyAsync(da.p, xfa, sizeof(float4) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess &&
+            hipMemcpyAsync(db.p, xfb, sizeof(float4) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_collide_given, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, dp.p, da.p, db.p,
+                       w->f.shape.p, w->f.pool.p, hdr.p, geo.p, p0.p, p1.p, ids.p);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(hh.data(), hdr.p, sizeof(int) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
+         hipMemcpyAsync(hg.data(), geo.p, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
+         hipMemcpyAsync(h0.data(), p0.p, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
+         hipMemcpyAsync(h1.data(), p1.p, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
+         hipMemcpyAsync(hi.data(), ids.p, sizeof(uint2) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
+         hipStreamSynchronize(w->stream) == hipSuccess;
+  }
+  dp.release(); da.release(); db.release(); geo.release(); p0.release(); p1.release(); hdr.release(); ids.release();
+  if (!ok) {
+    set_error("b2g_collide_pairs: device error");
+    return B2G_ERR_DEVICE;
+  }
+  for (int i = 0; i < n; ++i) {
+    b2g_manifold& m = out[i];
+    m.type = hh[i] & 0xff;
+    m.point_count = hh[i] >> 8;
+    m.local_normal[0] = hg[i].x; m.local_normal[1] = hg[i].y;
+    m.local_point[0] = hg[i].z; m.local_point[1] = hg[i].w;
+    m.points[0].local_point[0] = h0[i].x; m.points[0].local_point[1] = h0[i].y;
+    m.points[0].normal_impulse = h0[i].z; m.points[0].tangent_impulse = h0[i].w;
+    m.points[0].id = hi[i].x; m.points[0].pad = 0;
+    m.points[1].local_point[0] = h1[i].x; m.points[1].local_point[1] = h1[i].y;
+    m.points[1].normal_impulse = h1[i].z; m.points[1].tangent_impulse = h1[i].w;
+    m.points[1].id = hi[i].y; m.points[1].pad = 0;
+  }
+  return B2G_OK;
+}
+
+}  // namespace b2g

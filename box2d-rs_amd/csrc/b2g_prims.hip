@@ -1,0 +1,209 @@
+// b2g primitives: device-wide exclusive scan and LSD radix sort of (key, value) pairs.
+//
+// Both are written for 64-wide wavefronts: block-level scans use wave prefix sums built on
+// DPP-free shuffles plus one LDS exchange per wave, and the radix sort ranks keys within a
+// wave by matching digits with __ballot (wave multi-split), so ranks are stable without
+// per-key LDS atomics.
+#include "b2g_internal.h"
+
+namespace b2g {
+
+static constexpr int kScanBlock = 256;
+static constexpr int kScanItems = 8;   // items per thread
+static constexpr int kScanTile = kScanBlock * kScanItems;
+
+// inclusive prefix sum within a 64-lane wave
+__device__ inline int wave_inclusive(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// exclusive block scan of one value per thread; returns the block total in *total
+__device__ inline int block_exclusive(int x, int* total) {
+  __shared__ int wsum[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = wave_inclusive(x);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    int s = wsum[w];
+    if (w < wid) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - x;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ sums) {
+  const int base = blockIdx.x * kScanTile;
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int idx = base + i * kScanBlock + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  int tot;
+  block_exclusive(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// exclusive scan of the block sums by one block, chunk by chunk
+__global__ void __launch_bounds__(kScanBlock) k_scan_sums(int* __restrict__ sums, int nblocks, int* __restrict__ total) {
+  int carry = 0;
+  for (int base = 0; base < nblocks; base += kScanBlock) {
+    int idx = base + threadIdx.x;
+    int x = idx < nblocks ? sums[idx] : 0;
+    int tot;
+    int ex = block_exclusive(x, &tot);
+    if (idx < nblocks) sums[idx] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                          const int* __restrict__ sums) {
+  // each thread owns kScanItems consecutive items so the block covers a contiguous tile
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int idx = base + i;
+    v[i] = idx < n ? in[idx] : 0;
+    s += v[i];
+  }
+  int tot;
+  int ex = block_exclusive(s, &tot) + sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int idx = base + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+}
+
+// out[i] = sum of in[0..i); *total_dev (if given) = sum of all. in and out may alias.
+int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev) {
+  if (n <= 0) {
+    if (total_dev) B2G_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), w->stream));
+    return B2G_OK;
+  }
+  int nblocks = blocks_for(n, kScanTile);
+  int st = w->bp.scan_tmp.reserve(nblocks);
+  if (st) return st;
+  int* sums = w->bp.scan_tmp.p;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, w->stream, sums, nblocks, total_dev);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, out, n, sums);
+  B2G_HIP(hipGetLastError());
+  return B2G_OK;
+}
+
+// ---- radix sort ---------------------------------------------------------------------------
+static constexpr int kSortBlock = 256;
+static constexpr int kSortItems = 4;
+static constexpr int kSortTile = kSortBlock * kSortItems;
+static constexpr int kRadix = 256;
+
+// histogram of one 8-bit digit per tile; hist is digit-major: hist[d * nblocks + block]
+__global__ void __launch_bounds__(kSortBlock) k_radix_hist(const unsigned* __restrict__ keys, int n, int shift,
+                                                           unsigned* __restrict__ hist, int nblocks) {
+  __shared__ unsigned h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kSortTile;
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    int idx = base + i * kSortBlock + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// stable scatter: the tile is processed as kSortItems sub-tiles of kSortBlock keys in index
+// order; within a sub-tile each wave ranks its keys by digit with ballots.
+__global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const unsigned* __restrict__ keys,
+                                                              const unsigned* __restrict__ vals, int n, int shift,
+                                                              const unsigned* __restrict__ offs, int nblocks,
+                                                              unsigned* __restrict__ okeys, unsigned* __restrict__ ovals) {
+  __shared__ unsigned running[kRadix];
+  __shared__ unsigned wcount[kSortBlock / 64][kRadix];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  running[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  const int base = blockIdx.x * kSortTile;
+  for (int sub = 0; sub < kSortItems; ++sub) {
+    for (int w = 0; w < kSortBlock / 64; ++w) wcount[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int idx = base + sub * kSortBlock + threadIdx.x;
+    const bool valid = idx < n;
+    unsigned key = valid ? keys[idx] : 0u;
+    unsigned val = valid ? vals[idx] : 0u;
+    unsigned d = (key >> shift) & 0xffu;
+    // lanes of this wave holding the same digit
+    unsigned long long same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      unsigned long long bb = __ballot((d >> b) & 1u);
+      same &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned rank = __popcll(same & below);
+    bool leader = valid && (same & below) == 0ull;
+    if (leader) wcount[wid][d] = __popcll(same);
+    __syncthreads();
+    if (valid) {
+      unsigned pre = running[d];
+      for (int w = 0; w < wid; ++w) pre += wcount[w][d];
+      okeys[pre + rank] = key;
+      ovals[pre + rank] = val;
+    }
+    __syncthreads();
+    unsigned add = 0;
+    for (int w = 0; w < kSortBlock / 64; ++w) add += wcount[w][threadIdx.x];
+    running[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// Sorts n (key, value) pairs by the low `bits` bits of the key, stable. The result ends in
+// keys/vals (the alternate buffers are scratch).
+int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits) {
+  if (n <= 1) return B2G_OK;
+  BroadphaseArrays& bp = w->bp;
+  int st;
+  if ((st = bp.keys_alt.reserve(n))) return st;
+  if ((st = bp.vals_alt.reserve(n))) return st;
+  int nblocks = blocks_for(n, kSortTile);
+  if ((st = bp.hist.reserve(size_t(kRadix) * nblocks))) return st;
+  unsigned *k0 = keys, *v0 = vals, *k1 = bp.keys_alt.p, *v1 = bp.vals_alt.p;
+  int passes = (bits + 7) / 8;
+  for (int p = 0; p < passes; ++p) {
+    int shift = 8 * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortBlock), 0, w->stream, k0, n, shift, bp.hist.p, nblocks);
+    if ((st = exclusive_scan(w, reinterpret_cast<int*>(bp.hist.p), reinterpret_cast<int*>(bp.hist.p),
+                             kRadix * nblocks, nullptr)))
+      return st;
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortBlock), 0, w->stream, k0, v0, n, shift, bp.hist.p,
+                       nblocks, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  if (k0 != keys) {
+    B2G_HIP(hipMemcpyAsync(keys, k0, n * sizeof(unsigned), hipMemcpyDeviceToDevice, w->stream));
+    B2G_HIP(hipMemcpyAsync(vals, v0, n * sizeof(unsigned), hipMemcpyDeviceToDevice, w->stream));
+  }
+  B2G_HIP(hipGetLastError());
+  return B2G_OK;
+}
+
+}  // namespace b2g

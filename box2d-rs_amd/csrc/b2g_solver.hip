@@ -1,0 +1,811 @@
+// b2g solver: the island solve of B2world::solve rebuilt as graph-coloured Gauss-Seidel.
+//
+// Reference (box2d-rs 0.0.4): src/private/dynamics/b2_world.rs:356-507 (islands),
+// src/private/dynamics/b2_island_private.rs:129-328 (integrate, solve, copy back) and
+// src/private/dynamics/b2_contact_solver_private.rs:20-730 (contact solver).
+//
+// The touching contacts are coloured so that no two contacts of one colour share a dynamic
+// body; a colour's contacts are then independent and are solved in parallel, colour after
+// colour, which is sequential Gauss-Seidel in colour order (a CPU replay in that order gives
+// the same numbers). Islands are the connected components of the contact graph through
+// non-static bodies (union-find); they carry the reference's per-island early exit of the
+// position iterations. Colouring is Jones-Plassmann-Luby with hash priorities and first-fit
+// colours, so it is deterministic. Body state is SoA float4 in HBM.
+#include "b2g_internal.h"
+
+namespace b2g {
+
+static constexpr int kBlock = 256;
+static constexpr int kMaxColors = 128;
+
+// ---- integration --------------------------------------------------------------------------
+// src/private/dynamics/b2_island_private.rs:135-170
+__global__ void __launch_bounds__(kBlock) k_integrate_velocities(int nb, const int* __restrict__ btype,
+                                                                 const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                                 float4* __restrict__ pos0, const float4* __restrict__ props,
+                                                                 const float4* __restrict__ bmass,
+                                                                 const float4* __restrict__ force, float gx, float gy,
+                                                                 float h) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int t = btype[b];
+  if (t == 0) return;
+  pos0[b] = pos[b];
+  if (t != 2) return;
+  float4 v = vel[b];
+  float4 pr = props[b];  // mass, gravity scale, linear damping, angular damping
+  float4 ms = bmass[b];
+  float4 f = force[b];
+  float s = h * ms.x;
+  float gm = pr.y * pr.x;
+  v.x += s * (gm * gx + f.x);
+  v.y += s * (gm * gy + f.y);
+  v.z += h * ms.y * f.z;
+  float dl = 1.0f / (1.0f + h * pr.z);
+  float da = 1.0f / (1.0f + h * pr.w);
+  v.x *= dl;
+  v.y *= dl;
+  v.z *= da;
+  vel[b] = v;
+}
+
+// src/private/dynamics/b2_island_private.rs:221-252
+__global__ void __launch_bounds__(kBlock) k_integrate_positions(int nb, const int* __restrict__ btype,
+                                                                float4* __restrict__ pos, float4* __restrict__ vel, float h) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (btype[b] == 0) return;
+  float4 p = pos[b], v = vel[b];
+  V2 tr = h * v2(v.x, v.y);
+  if (dot(tr, tr) > kMaxTranslation * kMaxTranslation) {
+    float ratio = kMaxTranslation / len(tr);
+    v.x *= ratio;
+    v.y *= ratio;
+  }
+  float rotation = h * v.z;
+  if (rotation * rotation > kMaxRotation * kMaxRotation) {
+    float ratio = kMaxRotation / fabsf(rotation);
+    v.z *= ratio;
+  }
+  p.x += h * v.x;
+  p.y += h * v.y;
+  p.z += h * v.z;
+  pos[b] = p;
+  vel[b] = v;
+}
+
+// copy back + synchronize_transform (src/private/dynamics/b2_island_private.rs:276-285,
+// src/b2_body.rs:974-977). Static bodies keep the transform computed at creation (their
+// angle never changes, so the reference recomputes the same value).
+__global__ void __launch_bounds__(kBlock) k_finalize(int nb, const int* __restrict__ btype, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ bmass, float4* __restrict__ xf) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (btype[b] == 0) return;
+  float4 p = pos[b], ms = bmass[b];
+  Rot q = rot(p.z);
+  V2 o = v2(p.x, p.y) - mul(q, v2(ms.z, ms.w));
+  xf[b] = make_float4(o.x, o.y, q.s, q.c);
+}
+
+// ---- touching contacts, islands, colouring -----------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_compact(int n, const int* __restrict__ flag, const int* __restrict__ scan,
+                                                    int* __restrict__ list) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (flag[k]) list[scan[k]] = k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_constraint_bodies(int n, const int* __restrict__ list,
+                                                              const int2* __restrict__ ab, const int* __restrict__ fbody,
+                                                              const int* __restrict__ btype, int2* __restrict__ bodies,
+                                                              int* __restrict__ deg) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int2 f = ab[list[j]];
+  int ba = fbody[f.x], bb = fbody[f.y];
+  bodies[j] = make_int2(ba, bb);
+  if (btype[ba] == 2) atomicAdd(&deg[ba], 1);
+  if (btype[bb] == 2) atomicAdd(&deg[bb], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_adjacency(int n, const int2* __restrict__ bodies,
+                                                           const int* __restrict__ btype, const int* __restrict__ off,
+                                                           int* __restrict__ cursor, int* __restrict__ adj) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int2 b = bodies[j];
+  if (btype[b.x] == 2) adj[off[b.x] + atomicAdd(&cursor[b.x], 1)] = j;
+  if (btype[b.y] == 2) adj[off[b.y] + atomicAdd(&cursor[b.y], 1)] = j;
+}
+
+__global__ void __launch_bounds__(kBlock) k_iota(int n, int* __restrict__ a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+__device__ inline int uf_find(int* parent, int x) {
+  int p = parent[x];
+  while (p != x) {
+    int g = parent[p];
+    x = p;
+    p = g;
+  }
+  return x;
+}
+
+// union of the two non-static bodies of each constraint: hook the larger root under the smaller
+__global__ void __launch_bounds__(kBlock) k_union(int n, const int2* __restrict__ bodies, const int* __restrict__ btype,
+                                                  int* parent) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int2 b = bodies[j];
+  if (btype[b.x] == 0 || btype[b.y] == 0) return;
+  int a = b.x, c = b.y;
+  for (;;) {
+    a = uf_find(parent, a);
+    c = uf_find(parent, c);
+    if (a == c) return;
+    int hi = a > c ? a : c, lo = a > c ? c : a;
+    int prev = atomicCAS(&parent[hi], hi, lo);
+    if (prev == hi) return;
+    a = hi;
+    c = lo;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* __restrict__ minsep,
+                                                     int* __restrict__ done) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  parent[b] = uf_find(parent, b);
+  minsep[b] = float_flip(0.0f);
+  done[b] = 0;
+}
+
+__device__ inline unsigned long long priority(int j) {
+  unsigned x = unsigned(j) * 0x9E3779B1u;
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  x ^= x >> 16;
+  return (static_cast<unsigned long long>(x) << 32) | unsigned(j);
+}
+
+// One Jones-Plassmann-Luby round: an uncoloured constraint whose priority beats every
+// uncoloured neighbour (a constraint sharing a dynamic body) takes the smallest colour no
+// coloured neighbour has. Neighbours coloured in this same round count as uncoloured.
+__global__ void __launch_bounds__(kBlock) k_color_round(int n, int round, const int2* __restrict__ bodies,
+                                                        const int* __restrict__ btype, const int* __restrict__ off,
+                                                        const int* __restrict__ adj, int* state, int* __restrict__ remaining,
+                                                        int* __restrict__ overflow) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  if (state[j] != -1) return;
+  const unsigned long long pj = priority(j);
+  unsigned long long used0 = 0, used1 = 0;
+  int2 b = bodies[j];
+  int side[2] = {b.x, b.y};
+  for (int s = 0; s < 2; ++s) {
+    int body = side[s];
+    if (btype[body] != 2) continue;
+    for (int e = off[body], end = off[body + 1]; e < end; ++e) {
+      int o = adj[e];
+      if (o == j) continue;
+      int so = __hip_atomic_load(&state[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (so == -1 || (so >> 8) == round) {
+        if (priority(o) > pj) return;
+      } else {
+        int c = so & 0xff;
+        if (c < 64) used0 |= 1ull << c;
+        else used1 |= 1ull << (c - 64);
+      }
+    }
+  }
+  int c;
+  if (~used0) c = __ffsll(static_cast<long long>(~used0)) - 1;
+  else if (~used1) c = 64 + __ffsll(static_cast<long long>(~used1)) - 1;
+  else { atomicExch(overflow, 1); c = kMaxColors - 1; }
+  __hip_atomic_store(&state[j], (round << 8) | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  atomicSub(remaining, 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_hist(int n, const int* __restrict__ state, int* __restrict__ count) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  atomicAdd(&count[state[j] & 0xff], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __restrict__ state, const int* __restrict__ off,
+                                                          int* __restrict__ cursor, int* __restrict__ order) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int c = state[j] & 0xff;
+  order[off[c] + atomicAdd(&cursor[c], 1)] = j;
+}
+
+// ---- constraint setup ---------------------------------------------------------------------------
+// B2contactSolver::new + initialize_velocity_constraints for constraint i (colour order):
+// src/private/dynamics/b2_contact_solver_private.rs:20-226
+struct InitArgs {
+  int n;
+  const int* order;
+  const int* list;
+  const int2* ab;
+  const int* fbody;
+  const int4* fshape;
+  const float4* fmat;
+  const int* btype;
+  const int* parent;
+  const float4* bpos;
+  const float4* bvel;
+  const float4* bmass;
+  const int* hdr;
+  const float4* geo;
+  const float4* pt0;
+  const float4* pt1;
+  float dt_ratio;
+  int warm;
+  int block_solve;
+  // outputs
+  int* k_contact;
+  int2* k_bodies;
+  int4* k_info;
+  float4 *vnf, *vm, *vr0, *vr1, *vms;
+  float2* vbias;
+  float4 *K, *NM, *lam, *plp, *pgeo, *plc;
+  float2* prad;
+};
+
+__global__ void __launch_bounds__(kBlock) k_init_constraints(InitArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int j = a.order[i];
+  int k = a.list[j];
+  int2 f = a.ab[k];
+  int bA = a.fbody[f.x], bB = a.fbody[f.y];
+  float4 matA = a.fmat[f.x], matB = a.fmat[f.y];
+  // mixing laws, src/b2_contact.rs:29-46
+  float friction = sqrtf(matA.x * matB.x);
+  float restitution = matA.y > matB.y ? matA.y : matB.y;
+  float threshold = matA.z < matB.z ? matA.z : matB.z;
+  float tangent_speed = 0.0f;
+  float rA = __int_as_float(a.fshape[f.x].z), rB = __int_as_float(a.fshape[f.y].z);
+  int h = a.hdr[k];
+  int mtype = h & 0xff, mcount = h >> 8;
+  float4 g = a.geo[k], p0 = a.pt0[k], p1 = a.pt1[k];
+  float4 msA = a.bmass[bA], msB = a.bmass[bB];
+  float mA = msA.x, iA = msA.y, mB = msB.x, iB = msB.y;
+  V2 lcA = v2(msA.z, msA.w), lcB = v2(msB.z, msB.w);
+  float4 pA = a.bpos[bA], pB = a.bpos[bB];
+  float4 vA = a.bvel[bA], vB = a.bvel[bB];
+  int tA = a.btype[bA], tB = a.btype[bB];
+  int island = tA != 0 ? a.parent[bA] : a.parent[bB];
+  float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (a.warm) {
+    lam.x = a.dt_ratio * p0.z;
+    lam.y = a.dt_ratio * p0.w;
+    if (mcount > 1) {
+      lam.z = a.dt_ratio * p1.z;
+      lam.w = a.dt_ratio * p1.w;
+    }
+  }
+  Xf xfA, xfB;
+  xfA.q = rot(pA.z);
+  xfB.q = rot(pB.z);
+  xfA.p = v2(pA.x, pA.y) - mul(xfA.q, lcA);
+  xfB.p = v2(pB.x, pB.y) - mul(xfB.q, lcB);
+  V2 lp[2] = {v2(p0.x, p0.y), v2(p1.x, p1.y)};
+  WorldManifold wm;
+  world_manifold(&wm, mtype, mcount, v2(g.x, g.y), v2(g.z, g.w), lp, xfA, rA, xfB, rB);
+  V2 normal = wm.normal;
+  V2 cA = v2(pA.x, pA.y), cB = v2(pB.x, pB.y);
+  V2 va = v2(vA.x, vA.y), vb = v2(vB.x, vB.y);
+  float4 r[2];
+  float nm[2] = {0.0f, 0.0f}, tm[2] = {0.0f, 0.0f}, bias[2] = {0.0f, 0.0f};
+  r[0] = r[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int p = 0; p < mcount; ++p) {
+    V2 ra = wm.points[p] - cA;
+    V2 rb = wm.points[p] - cB;
+    r[p] = make_float4(ra.x, ra.y, rb.x, rb.y);
+    float rnA = cross(ra, normal), rnB = cross(rb, normal);
+    float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+    nm[p] = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+    V2 tangent = cross(normal, 1.0f);
+    float rtA = cross(ra, tangent), rtB = cross(rb, tangent);
+    float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+    tm[p] = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+    float vRel = dot(normal, ((vb + cross(vB.z, rb)) - va) - cross(vA.z, ra));
+    if (vRel < -threshold) bias[p] = -restitution * vRel;
+  }
+  int vcount = mcount;
+  float4 K = make_float4(0.0f, 0.0f, 0.0f, 0.0f), NM = K;
+  if (mcount == 2 && a.block_solve) {
+    V2 r1a = v2(r[0].x, r[0].y), r1b = v2(r[0].z, r[0].w), r2a = v2(r[1].x, r[1].y), r2b = v2(r[1].z, r[1].w);
+    float rn1A = cross(r1a, normal), rn1B = cross(r1b, normal);
+    float rn2A = cross(r2a, normal), rn2B = cross(r2b, normal);
+    float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
+    float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
+    float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
+    if (k11 * k11 < 1000.0f * (k11 * k22 - k12 * k12)) {
+      K = make_float4(k11, k12, k12, k22);
+      // B2Mat22::get_inverse (src/b2_math.rs:261-274)
+      float det = k11 * k22 - k12 * k12;
+      if (det != 0.0f) det = 1.0f / det;
+      NM = make_float4(det * k22, -det * k12, -det * k12, det * k11);
+    } else {
+      vcount = 1;
+    }
+  }
+  a.k_contact[i] = k;
+  a.k_bodies[i] = make_int2(bA, bB);
+  a.k_info[i] = make_int4(mtype | ((tA == 2) << 8) | ((tB == 2) << 9), mcount, vcount, island);
+  a.vnf[i] = make_float4(normal.x, normal.y, friction, tangent_speed);
+  a.vm[i] = make_float4(mA, iA, mB, iB);
+  a.vr0[i] = r[0];
+  a.vr1[i] = r[1];
+  a.vms[i] = make_float4(nm[0], tm[0], nm[1], tm[1]);
+  a.vbias[i] = make_float2(bias[0], bias[1]);
+  a.K[i] = K;
+  a.NM[i] = NM;
+  a.lam[i] = lam;
+  a.plp[i] = make_float4(p0.x, p0.y, p1.x, p1.y);
+  a.pgeo[i] = g;
+  a.plc[i] = make_float4(lcA.x, lcA.y, lcB.x, lcB.y);
+  a.prad[i] = make_float2(rA, rB);
+}
+
+// ---- velocity passes --------------------------------------------------------------------------
+struct VelArgs {
+  const int2* bodies;
+  const int4* info;
+  const float4 *vnf, *vm, *vr0, *vr1, *vms;
+  const float2* vbias;
+  const float4 *K, *NM;
+  float4* lam;
+  float4* vel;
+};
+
+// B2contactSolver::warm_start for one constraint (src/private/dynamics/b2_contact_solver_private.rs:228-266)
+__global__ void __launch_bounds__(kBlock) k_warm_start(int begin, int end, VelArgs a) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int2 bd = a.bodies[i];
+  int4 info = a.info[i];
+  float4 m = a.vm[i];
+  float4 nf = a.vnf[i];
+  float4 lam = a.lam[i];
+  float4 vA = a.vel[bd.x], vB = a.vel[bd.y];
+  V2 va = v2(vA.x, vA.y), vb = v2(vB.x, vB.y);
+  float wa = vA.z, wb = vB.z;
+  V2 normal = v2(nf.x, nf.y);
+  V2 tangent = cross(normal, 1.0f);
+  int vcount = info.z;
+  for (int p = 0; p < vcount; ++p) {
+    float4 r = p == 0 ? a.vr0[i] : a.vr1[i];
+    float ni = p == 0 ? lam.x : lam.z, ti = p == 0 ? lam.y : lam.w;
+    V2 P = ni * normal + ti * tangent;
+    wa -= m.y * cross(v2(r.x, r.y), P);
+    va = va - m.x * P;
+    wb += m.w * cross(v2(r.z, r.w), P);
+    vb = vb + m.z * P;
+  }
+  if (info.x & 0x100) a.vel[bd.x] = make_float4(va.x, va.y, wa, 0.0f);
+  if (info.x & 0x200) a.vel[bd.y] = make_float4(vb.x, vb.y, wb, 0.0f);
+}
+
+// B2contactSolver::solve_velocity_constraints for one constraint
+// (src/private/dynamics/b2_contact_solver_private.rs:268-583)
+__device__ inline void apply_normal_pair(V2& va, float& wa, V2& vb, float& wb, float mA, float iA, float mB, float iB,
+                                         V2 normal, V2 r1a, V2 r2a, V2 r1b, V2 r2b, V2 d) {
+  V2 P1 = d.x * normal, P2 = d.y * normal;
+  va = va - mA * (P1 + P2);
+  wa -= iA * (cross(r1a, P1) + cross(r2a, P2));
+  vb = vb + mB * (P1 + P2);
+  wb += iB * (cross(r1b, P1) + cross(r2b, P2));
+}
+
+__global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, int block_solve, VelArgs a) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int2 bd = a.bodies[i];
+  int4 info = a.info[i];
+  float4 m = a.vm[i];
+  float4 nf = a.vnf[i];
+  float4 lam = a.lam[i];
+  float4 ms = a.vms[i];
+  float4 r0 = a.vr0[i];
+  float4 vA = a.vel[bd.x], vB = a.vel[bd.y];
+  const float mA = m.x, iA = m.y, mB = m.z, iB = m.w;
+  V2 va = v2(vA.x, vA.y), vb = v2(vB.x, vB.y);
+  float wa = vA.z, wb = vB.z;
+  V2 normal = v2(nf.x, nf.y);
+  V2 tangent = cross(normal, 1.0f);
+  const float friction = nf.z;
+  const int vcount = info.z;
+  V2 r1a = v2(r0.x, r0.y), r1b = v2(r0.z, r0.w);
+  V2 r2a = v2(0.0f, 0.0f), r2b = v2(0.0f, 0.0f);
+  if (vcount == 2) {
+    float4 r1 = a.vr1[i];
+    r2a = v2(r1.x, r1.y);
+    r2b = v2(r1.z, r1.w);
+  }
+  // tangent constraints
+  for (int p = 0; p < vcount; ++p) {
+    V2 ra = p == 0 ? r1a : r2a, rb = p == 0 ? r1b : r2b;
+    float tmass = p == 0 ? ms.y : ms.w;
+    float nimp = p == 0 ? lam.x : lam.z;
+    float timp = p == 0 ? lam.y : lam.w;
+    V2 dv = ((vb + cross(wb, rb)) - va) - cross(wa, ra);
+    float vt = dot(dv, tangent) - nf.w;
+    float lambda = tmass * (-vt);
+    float maxFriction = friction * nimp;
+    float newImpulse = bclamp(timp + lambda, -maxFriction, maxFriction);
+    lambda = newImpulse - timp;
+    if (p == 0) lam.y = newImpulse; else lam.w = newImpulse;
+    V2 P = lambda * tangent;
+    va = va - mA * P;
+    wa -= iA * cross(ra, P);
+    vb = vb + mB * P;
+    wb += iB * cross(rb, P);
+  }
+  float2 bias = a.vbias[i];
+  if (vcount == 1 || !block_solve) {
+    for (int p = 0; p < vcount; ++p) {
+      V2 ra = p == 0 ? r1a : r2a, rb = p == 0 ? r1b : r2b;
+      float nmass = p == 0 ? ms.x : ms.z;
+      float vbias = p == 0 ? bias.x : bias.y;
+      float nimp = p == 0 ? lam.x : lam.z;
+      V2 dv = ((vb + cross(wb, rb)) - va) - cross(wa, ra);
+      float vn = dot(dv, normal);
+      float lambda = -nmass * (vn - vbias);
+      float newImpulse = bmax(nimp + lambda, 0.0f);
+      lambda = newImpulse - nimp;
+      if (p == 0) lam.x = newImpulse; else lam.z = newImpulse;
+      V2 P = lambda * normal;
+      va = va - mA * P;
+      wa -= iA * cross(ra, P);
+      vb = vb + mB * P;
+      wb += iB * cross(rb, P);
+    }
+  } else {
+    float4 K = a.K[i], NM = a.NM[i];  // (ex.x, ex.y, ey.x, ey.y)
+    V2 aa = v2(lam.x, lam.z);
+    V2 dv1 = ((vb + cross(wb, r1b)) - va) - cross(wa, r1a);
+    V2 dv2 = ((vb + cross(wb, r2b)) - va) - cross(wa, r2a);
+    float vn1 = dot(dv1, normal);
+    float vn2 = dot(dv2, normal);
+    V2 b = v2(vn1 - bias.x, vn2 - bias.y);
+    V2 Ka = v2(K.x * aa.x + K.z * aa.y, K.y * aa.x + K.w * aa.y);
+    b = b - Ka;
+    for (;;) {
+      // case 1: vn = 0
+      V2 x = -v2(NM.x * b.x + NM.z * b.y, NM.y * b.x + NM.w * b.y);
+      if (x.x >= 0.0f && x.y >= 0.0f) {
+        apply_normal_pair(va, wa, vb, wb, mA, iA, mB, iB, normal, r1a, r2a, r1b, r2b, x - aa);
+        lam.x = x.x;
+        lam.z = x.y;
+        break;
+      }
+      // case 2: vn1 = 0 and x2 = 0
+      x.x = -ms.x * b.x;
+      x.y = 0.0f;
+      vn2 = K.y * x.x + b.y;
+      if (x.x >= 0.0f && vn2 >= 0.0f) {
+        apply_normal_pair(va, wa, vb, wb, mA, iA, mB, iB, normal, r1a, r2a, r1b, r2b, x - aa);
+        lam.x = x.x;
+        lam.z = x.y;
+        break;
+      }
+      // case 3: vn2 = 0 and x1 = 0
+      x.x = 0.0f;
+      x.y = -ms.z * b.y;
+      vn1 = K.z * x.y + b.x;
+      if (x.y >= 0.0f && vn1 >= 0.0f) {
+        apply_normal_pair(va, wa, vb, wb, mA, iA, mB, iB, normal, r1a, r2a, r1b, r2b, x - aa);
+        lam.x = x.x;
+        lam.z = x.y;
+        break;
+      }
+      // case 4: x1 = 0 and x2 = 0
+      x.x = 0.0f;
+      x.y = 0.0f;
+      vn1 = b.x;
+      vn2 = b.y;
+      if (vn1 >= 0.0f && vn2 >= 0.0f) {
+        apply_normal_pair(va, wa, vb, wb, mA, iA, mB, iB, normal, r1a, r2a, r1b, r2b, x - aa);
+        lam.x = x.x;
+        lam.z = x.y;
+        break;
+      }
+      break;  // no solution: give up
+    }
+  }
+  a.lam[i] = lam;
+  if (info.x & 0x100) a.vel[bd.x] = make_float4(va.x, va.y, wa, 0.0f);
+  if (info.x & 0x200) a.vel[bd.y] = make_float4(vb.x, vb.y, wb, 0.0f);
+}
+
+// store_impulses (src/private/dynamics/b2_contact_solver_private.rs:585-598)
+__global__ void __launch_bounds__(kBlock) k_store_impulses(int n, const int* __restrict__ kc, const int4* __restrict__ info,
+                                                           const float4* __restrict__ lam, float4* __restrict__ pt0,
+                                                           float4* __restrict__ pt1) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = kc[i];
+  int vcount = info[i].z;
+  float4 l = lam[i];
+  float4 p = pt0[k];
+  p.z = l.x;
+  p.w = l.y;
+  pt0[k] = p;
+  if (vcount > 1) {
+    float4 q = pt1[k];
+    q.z = l.z;
+    q.w = l.w;
+    pt1[k] = q;
+  }
+}
+
+// ---- position passes ----------------------------------------------------------------------------
+struct PosArgs {
+  const int2* bodies;
+  const int4* info;
+  const float4 *vm, *plp, *pgeo, *plc;
+  const float2* prad;
+  float4* pos;
+  int* minsep;
+  const int* done;
+};
+
+// B2contactSolver::solve_position_constraints for one constraint
+// (src/private/dynamics/b2_contact_solver_private.rs:600-730)
+__global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, PosArgs a) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int4 info = a.info[i];
+  if (a.done[info.w]) return;
+  int2 bd = a.bodies[i];
+  float4 m = a.vm[i];
+  float4 lp = a.plp[i], g = a.pgeo[i], lc = a.plc[i];
+  float2 rad = a.prad[i];
+  const float mA = m.x, iA = m.y, mB = m.z, iB = m.w;
+  V2 lcA = v2(lc.x, lc.y), lcB = v2(lc.z, lc.w);
+  float4 PA = a.pos[bd.x], PB = a.pos[bd.y];
+  V2 cA = v2(PA.x, PA.y), cB = v2(PB.x, PB.y);
+  float aA = PA.z, aB = PB.z;
+  const int type = info.x & 0xff;
+  const int count = info.y;
+  V2 ln = v2(g.x, g.y), lpt = v2(g.z, g.w);
+  float minSep = 0.0f;
+  for (int p = 0; p < count; ++p) {
+    Xf xfA, xfB;
+    xfA.q = rot(aA);
+    xfB.q = rot(aB);
+    xfA.p = cA - mul(xfA.q, lcA);
+    xfB.p = cB - mul(xfB.q, lcB);
+    V2 normal, point;
+    float separation;
+    V2 lpp = p == 0 ? v2(lp.x, lp.y) : v2(lp.z, lp.w);
+    if (type == kCircles) {
+      V2 pA = mul(xfA, lpt);
+      V2 pB = mul(xfB, v2(lp.x, lp.y));
+      normal = pB - pA;
+      normalize(&normal);
+      point = 0.5f * (pA + pB);
+      separation = dot(pB - pA, normal) - rad.x - rad.y;
+    } else if (type == kFaceA) {
+      normal = mul(xfA.q, ln);
+      V2 plane = mul(xfA, lpt);
+      V2 clip = mul(xfB, lpp);
+      separation = dot(clip - plane, normal) - rad.x - rad.y;
+      point = clip;
+    } else {
+      normal = mul(xfB.q, ln);
+      V2 plane = mul(xfB, lpt);
+      V2 clip = mul(xfA, lpp);
+      separation = dot(clip - plane, normal) - rad.x - rad.y;
+      point = clip;
+      normal = -normal;
+    }
+    V2 rA = point - cA, rB = point - cB;
+    minSep = bmin(minSep, separation);
+    float C = bclamp(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+    float rnA = cross(rA, normal), rnB = cross(rB, normal);
+    float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+    float impulse = K > 0.0f ? -C / K : 0.0f;
+    V2 P = impulse * normal;
+    cA = cA - mA * P;
+    aA -= iA * cross(rA, P);
+    cB = cB + mB * P;
+    aB += iB * cross(rB, P);
+  }
+  if (info.x & 0x100) a.pos[bd.x] = make_float4(cA.x, cA.y, aA, 0.0f);
+  if (info.x & 0x200) a.pos[bd.y] = make_float4(cB.x, cB.y, aB, 0.0f);
+  if (minSep < 0.0f) atomicMin(&a.minsep[info.w], float_flip(minSep));
+}
+
+// end of a position iteration: islands whose error is small stop iterating
+// (src/private/dynamics/b2_island_private.rs:257-273)
+__global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, const int* __restrict__ btype,
+                                                                   const int* __restrict__ parent, int* __restrict__ minsep,
+                                                                   int* __restrict__ done, int* __restrict__ active) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (btype[b] == 0 || parent[b] != b || done[b]) return;
+  float ms = float_unflip(minsep[b]);
+  if (ms >= -3.0f * kLinearSlop) done[b] = 1;
+  else atomicAdd(active, 1);
+  minsep[b] = float_flip(0.0f);
+}
+
+// ---- driver ---------------------------------------------------------------------------------------
+static int reserve_solver(World* w, int nc) {
+  SolverArrays& s = w->s;
+  int n = std::max(nc, 1);
+  int st;
+  if ((st = s.touch_scan.reserve(std::max(w->n_pairs, 1) + 1)) || (st = s.touch_list.reserve(n)) ||
+      (st = s.k_contact.reserve(n)) || (st = s.k_bodies.reserve(n)) || (st = s.k_info.reserve(n)) ||
+      (st = s.vnf.reserve(n)) || (st = s.vm.reserve(n)) || (st = s.vr0.reserve(n)) || (st = s.vr1.reserve(n)) ||
+      (st = s.vms.reserve(n)) || (st = s.vbias.reserve(n)) || (st = s.K.reserve(n)) || (st = s.NM.reserve(n)) ||
+      (st = s.lam.reserve(n)) || (st = s.plp.reserve(n)) || (st = s.pgeo.reserve(n)) || (st = s.plc.reserve(n)) ||
+      (st = s.prad.reserve(n)) || (st = s.adj.reserve(2 * n)) || (st = s.color_state.reserve(n)) ||
+      (st = s.order.reserve(n)))
+    return st;
+  return B2G_OK;
+}
+
+int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters) {
+  const int nb = w->nb;
+  if (nb == 0) return B2G_OK;
+  SolverArrays& s = w->s;
+  ContactArrays& c = w->c[w->cur];
+  int st;
+  int* cnt = w->counters.p;  // [0] touching, [1] remaining, [2] overflow, [3] active islands
+  hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
+                     w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p, w->gravity.x,
+                     w->gravity.y, dt);
+  // touching contacts
+  int n_pairs = w->n_pairs;
+  if ((st = s.touch_scan.reserve(std::max(n_pairs, 1) + 1))) return st;
+  if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
+  if ((st = s.deg.reserve(nb + 1)) || (st = s.adj_off.reserve(nb + 1)) || (st = s.adj_cursor.reserve(nb)) ||
+      (st = s.parent.reserve(nb)) || (st = s.isl_minsep.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
+      (st = s.color_count.reserve(kMaxColors + 1)) || (st = s.color_off.reserve(kMaxColors + 1)) ||
+      (st = s.color_cursor.reserve(kMaxColors)))
+    return st;
+  B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, sizeof(int), hipMemcpyDeviceToHost, w->stream));
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  const int nc = w->h_counters[0];
+  w->n_touch = nc;
+  if ((st = reserve_solver(w, nc))) return st;
+  if (n_pairs > 0)
+    hipLaunchKernelGGL(k_compact, dim3(blocks_for(n_pairs, kBlock)), dim3(kBlock), 0, w->stream, n_pairs,
+                       s.touch_flag.p, s.touch_scan.p, s.touch_list.p);
+  // constraint bodies, degrees, islands
+  B2G_HIP(hipMemsetAsync(s.deg.p, 0, sizeof(int) * (nb + 1), w->stream));
+  B2G_HIP(hipMemsetAsync(s.adj_cursor.p, 0, sizeof(int) * nb, w->stream));
+  hipLaunchKernelGGL(k_iota, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p);
+  if (nc > 0) {
+    hipLaunchKernelGGL(k_constraint_bodies, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
+                       s.touch_list.p, c.ab.p, w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p);
+    hipLaunchKernelGGL(k_union, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_bodies.p,
+                       w->b.type.p, s.parent.p);
+  }
+  hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
+                     s.isl_minsep.p, s.isl_done.p);
+  int ncolors = 0;
+  if (nc > 0) {
+    if ((st = exclusive_scan(w, s.deg.p, s.adj_off.p, nb + 1, nullptr))) return st;
+    hipLaunchKernelGGL(k_fill_adjacency, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_bodies.p,
+                       w->b.type.p, s.adj_off.p, s.adj_cursor.p, s.adj.p);
+    B2G_HIP(hipMemsetAsync(s.color_state.p, 0xff, sizeof(int) * nc, w->stream));
+    int init[4] = {nc, nc, 0, 0};
+    B2G_HIP(hipMemcpyAsync(cnt, init, sizeof(init), hipMemcpyHostToDevice, w->stream));
+    int round = 0;
+    for (;;) {
+      int batch = round == 0 ? 8 : 4;
+      for (int r = 0; r < batch; ++r, ++round)
+        hipLaunchKernelGGL(k_color_round, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, round,
+                           s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
+      B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, 4 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
+      B2G_HIP(hipStreamSynchronize(w->stream));
+      if (w->h_counters[1] == 0) break;
+      if (round > 4 * nc + 64) {
+        set_error("colouring did not converge");
+        return B2G_ERR_DEVICE;
+      }
+    }
+    if (w->h_counters[2]) {
+      set_error("colouring needs more than 127 colours (a dynamic body with too many contacts)");
+      return B2G_ERR_CAPACITY;
+    }
+    B2G_HIP(hipMemsetAsync(s.color_count.p, 0, sizeof(int) * (kMaxColors + 1), w->stream));
+    B2G_HIP(hipMemsetAsync(s.color_cursor.p, 0, sizeof(int) * kMaxColors, w->stream));
+    hipLaunchKernelGGL(k_color_hist, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.color_state.p,
+                       s.color_count.p);
+    std::vector<int> hc(kMaxColors);
+    B2G_HIP(hipMemcpyAsync(hc.data(), s.color_count.p, sizeof(int) * kMaxColors, hipMemcpyDeviceToHost, w->stream));
+    B2G_HIP(hipStreamSynchronize(w->stream));
+    w->h_color_off.assign(1, 0);
+    for (int i = 0; i < kMaxColors; ++i) {
+      if (hc[i] == 0) break;  // first-fit colours are contiguous from 0
+      ncolors = i + 1;
+      w->h_color_off.push_back(w->h_color_off.back() + hc[i]);
+    }
+    std::vector<int> offs(kMaxColors + 1, nc);
+    for (int i = 0; i <= ncolors; ++i) offs[i] = w->h_color_off[i];
+    B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * (kMaxColors + 1), hipMemcpyHostToDevice,
+                           w->stream));
+    hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
+                       s.color_state.p, s.color_off.p, s.color_cursor.p, s.order.p);
+    InitArgs ia;
+    ia.n = nc;
+    ia.order = s.order.p;
+    ia.list = s.touch_list.p;
+    ia.ab = c.ab.p;
+    ia.fbody = w->f.body.p;
+    ia.fshape = w->f.shape.p;
+    ia.fmat = w->f.mat.p;
+    ia.btype = w->b.type.p;
+    ia.parent = s.parent.p;
+    ia.bpos = w->b.pos.p;
+    ia.bvel = w->b.vel.p;
+    ia.bmass = w->b.mass.p;
+    ia.hdr = c.hdr.p;
+    ia.geo = c.geo.p;
+    ia.pt0 = c.pt0.p;
+    ia.pt1 = c.pt1.p;
+    ia.dt_ratio = dt_ratio;
+    ia.warm = w->warm_starting;
+    ia.block_solve = w->block_solve;
+    ia.k_contact = s.k_contact.p;
+    ia.k_bodies = s.k_bodies.p;  // overwritten in colour order (the unsorted copy is no longer needed)
+    ia.k_info = s.k_info.p;
+    ia.vnf = s.vnf.p; ia.vm = s.vm.p; ia.vr0 = s.vr0.p; ia.vr1 = s.vr1.p; ia.vms = s.vms.p; ia.vbias = s.vbias.p;
+    ia.K = s.K.p; ia.NM = s.NM.p; ia.lam = s.lam.p; ia.plp = s.plp.p; ia.pgeo = s.pgeo.p; ia.plc = s.plc.p;
+    ia.prad = s.prad.p;
+    // k_bodies is both read (colouring, done) and rewritten here in sorted order: the init
+    // kernel recomputes bodies from the contact, so the aliasing is safe.
+    hipLaunchKernelGGL(k_init_constraints, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, ia);
+  }
+  w->n_colors = ncolors;
+  VelArgs va{s.k_bodies.p, s.k_info.p, s.vnf.p, s.vm.p, s.vr0.p, s.vr1.p, s.vms.p, s.vbias.p, s.K.p, s.NM.p, s.lam.p,
+             w->b.vel.p};
+  const std::vector<int>& co = w->h_color_off;
+  if (nc > 0 && w->warm_starting)
+    for (int col = 0; col < ncolors; ++col) {
+      int b0 = co[col], b1 = co[col + 1];
+      hipLaunchKernelGGL(k_warm_start, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1, va);
+    }
+  for (int it = 0; it < vel_iters && nc > 0; ++it)
+    for (int col = 0; col < ncolors; ++col) {
+      int b0 = co[col], b1 = co[col + 1];
+      hipLaunchKernelGGL(k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1,
+                         int(w->block_solve), va);
+    }
+  if (nc > 0)
+    hipLaunchKernelGGL(k_store_impulses, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_contact.p,
+                       s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p);
+  hipLaunchKernelGGL(k_integrate_positions, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
+                     w->b.pos.p, w->b.vel.p, dt);
+  PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
+             s.isl_done.p};
+  w->pos_iters_run = 0;
+  for (int it = 0; it < pos_iters; ++it) {
+    for (int col = 0; col < ncolors; ++col) {
+      int b0 = co[col], b1 = co[col + 1];
+      hipLaunchKernelGGL(k_solve_position, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1, pa);
+    }
+    hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb,
+                       w->b.type.p, s.parent.p, s.isl_minsep.p, s.isl_done.p, cnt + 3);
+    ++w->pos_iters_run;
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p, w->b.pos.p,
+                     w->b.mass.p, w->b.xf.p);
+  B2G_HIP(hipGetLastError());
+  return B2G_OK;
+}
+
+}  // namespace b2g

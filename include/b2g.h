@@ -1,0 +1,194 @@
+/* b2g -- MI355X-native B2world::step for box2d-rs worlds: the C ABI.
+ *
+ * This is the drop-in boundary. A Rust FFI crate (or any C caller) builds a world with the
+ * same definitions box2d-rs uses (B2bodyDef, B2fixtureDef, shapes) and calls b2g_world_step
+ * in place of B2world::step. Everything crosses as plain C structs, pointers and sizes.
+ *
+ * Conventions
+ *   - Every function returns an int status: B2G_OK (0) or one of the B2G_ERR_* codes; the
+ *     message of the last error of the calling thread is returned by b2g_last_error().
+ *   - Caller-owned host buffers with explicit lengths; the library owns all device memory.
+ *   - One HIP stream per world; a world must be used by one host thread at a time.
+ *   - Citations are to the box2d-rs 0.0.4 sources.
+ */
+#ifndef B2G_H
+#define B2G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define B2G_OK 0
+#define B2G_ERR_INVALID_ARGUMENT 1
+#define B2G_ERR_CAPACITY 2
+#define B2G_ERR_DEVICE 3
+#define B2G_ERR_DEGENERATE 4
+#define B2G_ERR_UNSUPPORTED 5
+
+/* Body types: B2bodyType (src/b2_body.rs:20-30). */
+#define B2G_STATIC 0
+#define B2G_KINEMATIC 1
+#define B2G_DYNAMIC 2
+
+/* Shape kinds of b2g_shape_def::type. */
+#define B2G_SHAPE_CIRCLE 0  /* params[0..2) = local centre m_p, radius = m_radius        */
+#define B2G_SHAPE_EDGE 1    /* params = v1, v2 (two-sided) [+ v0, v3 when flags bit0]     */
+#define B2G_SHAPE_POLYGON 2 /* count raw points in params, passed to the hull (set())     */
+#define B2G_SHAPE_BOX 4     /* params = hx, hy [, cx, cy, angle when flags bit0]          */
+
+/* Body definition: B2bodyDef (src/b2_body.rs:32-100). 56 bytes. */
+typedef struct b2g_body_def {
+  int32_t type;
+  int32_t allow_sleep, awake, fixed_rotation;
+  float position[2];
+  float angle;
+  float linear_velocity[2];
+  float angular_velocity;
+  float linear_damping, angular_damping, gravity_scale;
+  float pad;
+} b2g_body_def;
+
+/* Shape definition: B2circleShape / B2edgeShape::set_two_sided, set_one_sided /
+ * B2polygonShape::set, set_as_box, set_as_box_angle (src/private/collision/b2_*_shape.rs). */
+typedef struct b2g_shape_def {
+  int32_t type;
+  int32_t count;
+  int32_t flags;
+  float radius;
+  float params[20];
+} b2g_shape_def;
+
+/* Fixture definition: B2fixtureDef + B2filter (src/b2_fixture.rs:14-58). 32 bytes. */
+typedef struct b2g_fixture_def {
+  float friction, restitution, restitution_threshold, density;
+  int32_t is_sensor;
+  uint16_t category_bits, mask_bits;
+  int16_t group_index;
+  int16_t pad0;
+  int32_t pad1;
+} b2g_fixture_def;
+
+/* Body state as B2body holds it: transform m_xf, sweep, velocities. 64 bytes. */
+typedef struct b2g_body_state {
+  float p[2], q[2]; /* m_xf.p, (m_xf.q.s, m_xf.q.c) */
+  float c[2], a;    /* m_sweep.c, m_sweep.a */
+  float v[2], w;    /* m_linear_velocity, m_angular_velocity */
+  float c0[2], a0;  /* m_sweep.c0, m_sweep.a0 */
+  float sleep_time, awake, pad;
+} b2g_body_state;
+
+/* B2manifold (src/b2_collision.rs:51-114) with packed contact-feature ids:
+ * id = index_a | index_b << 8 | type_a << 16 | type_b << 24. 72 bytes. */
+typedef struct b2g_manifold_point {
+  float local_point[2];
+  float normal_impulse, tangent_impulse;
+  uint32_t id;
+  int32_t pad;
+} b2g_manifold_point;
+typedef struct b2g_manifold {
+  int32_t type; /* 0 circles, 1 face A, 2 face B */
+  int32_t point_count;
+  float local_normal[2], local_point[2];
+  b2g_manifold_point points[2];
+} b2g_manifold;
+
+/* Per-step profile: B2Profile (src/b2_time_step.rs:5-14) plus device counters. */
+typedef struct b2g_profile {
+  float step, collide, solve, broadphase; /* milliseconds, device time */
+  int32_t pairs, touching, colours, islands, position_iterations;
+  int32_t pad[3];
+} b2g_profile;
+
+typedef struct b2g_world b2g_world;
+
+/* Message of the last failed call on this thread. */
+const char* b2g_last_error(void);
+/* Library version string. */
+const char* b2g_version(void);
+
+/* B2world::new(gravity) -- src/b2_world.rs:22-30. */
+int b2g_world_create(float gravity_x, float gravity_y, b2g_world** out);
+int b2g_world_destroy(b2g_world* world);
+
+/* B2world::set_warm_starting / set_allow_sleeping / set_continuous_physics
+ * (src/b2_world.rs:164-185) and G_BLOCK_SOLVE (src/b2_contact.rs:25). allow_sleep and
+ * continuous must be 0: sleeping islands and TOI sub-stepping are not on the GPU path. */
+int b2g_world_set_flags(b2g_world* world, int32_t warm_starting, int32_t allow_sleep, int32_t continuous,
+                        int32_t block_solve);
+
+/* B2world::create_body + B2body::create_fixture in bulk (src/private/dynamics/b2_world.rs:83-98,
+ * src/private/dynamics/b2_body.rs:155-200). Creates n_bodies bodies, then n_fixtures fixtures in
+ * array order, fixture k on body fixture_body[k] (indices relative to the first body created by
+ * this call). Writes the broad-phase proxy id of each fixture (the dynamic-tree node index the
+ * reference would assign in a create-only world: 0, 1, 3, 5, ...) when proxy_ids is non-null. */
+int b2g_world_add(b2g_world* world, int32_t n_bodies, const b2g_body_def* bodies, int32_t n_fixtures,
+                  const int32_t* fixture_body, const b2g_shape_def* shapes, const b2g_fixture_def* fixtures,
+                  int32_t* proxy_ids);
+
+/* B2world::step(dt, velocity_iterations, position_iterations) -- src/b2_world.rs:98-100,
+ * src/private/dynamics/b2_world.rs:903-959. */
+int b2g_world_step(b2g_world* world, float dt, int32_t velocity_iterations, int32_t position_iterations);
+
+/* The three phases of one step, for per-phase parity checks; calling them in order is one
+ * b2g_world_step:
+ *   B2G_PHASE_COLLIDE : B2contactManager::collide (src/private/dynamics/b2_contact_manager.rs:83-171)
+ *   B2G_PHASE_SOLVE   : island solve of all bodies (src/private/dynamics/b2_world.rs:356-507,
+ *                       src/private/dynamics/b2_island_private.rs:129-328)
+ *   B2G_PHASE_FIND    : synchronize_fixtures + find_new_contacts
+ *                       (src/private/dynamics/b2_world.rs:509-528, src/b2_broad_phase.rs:200-249) */
+#define B2G_PHASE_COLLIDE 0
+#define B2G_PHASE_SOLVE 1
+#define B2G_PHASE_FIND 2
+int b2g_world_step_phase(b2g_world* world, int32_t phase, float dt, int32_t velocity_iterations,
+                         int32_t position_iterations);
+
+int b2g_world_body_count(b2g_world* world, int32_t* out);
+int b2g_world_fixture_count(b2g_world* world, int32_t* out);
+int b2g_world_contact_count(b2g_world* world, int32_t* out);
+
+/* B2body getters/setters in bulk (get_transform, get_linear_velocity, ...; set_transform
+ * and set_linear_velocity semantics without re-synchronising proxies). */
+int b2g_world_get_body_states(b2g_world* world, b2g_body_state* out, int32_t n);
+int b2g_world_set_body_states(b2g_world* world, const b2g_body_state* in, int32_t n);
+/* Mass data per body: mass, inv_mass, I, inv_I, local_center.x, local_center.y. */
+int b2g_world_get_mass(b2g_world* world, float* out, int32_t n);
+
+/* Fat AABB of every proxy (B2broadPhase::get_fat_aabb, src/b2_broad_phase.rs) as
+ * lower.x, lower.y, upper.x, upper.y. */
+int b2g_world_get_fat_aabbs(b2g_world* world, float* out, int32_t n);
+/* Overwrites the fat AABB of every proxy (for per-phase parity runs from a given state). */
+int b2g_world_set_fat_aabbs(b2g_world* world, const float* in, int32_t n);
+
+/* The contact set (the world contact list, src/b2_contact_manager.rs:67-74), in the device's
+ * order: fixture indices A and B (after the registry ordering of
+ * src/private/dynamics/b2_contact_registers.rs:67-103), flags (bit0 touching) and manifolds.
+ * *count receives the number of contacts; at most cap are written. */
+int b2g_world_get_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, int32_t* flags,
+                           b2g_manifold* manifolds, int32_t* count);
+
+/* Solver constraint order of the last SOLVE phase: contact indices in solve order (colour by
+ * colour) and colour offsets (n_colours + 1 entries). Used to replay the solve on the CPU. */
+int b2g_world_get_solver_order(b2g_world* world, int32_t cap, int32_t* contact_order, int32_t* colour_offsets,
+                               int32_t colour_cap, int32_t* n_constraints, int32_t* n_colours);
+
+int b2g_world_get_profile(b2g_world* world, b2g_profile* out);
+
+/* Stateless narrowphase: B2contactDynTrait::evaluate (src/b2_contact.rs:97-102) for n pairs of
+ * fixtures of this world under the given transforms (4 floats each: p.x, p.y, sin, cos). The
+ * pair order is taken as given (A = pairs[2i], B = pairs[2i+1]; A must be the registry's
+ * primary shape type). */
+int b2g_collide_pairs(b2g_world* world, int32_t n, const int32_t* pairs, const float* xf_a, const float* xf_b,
+                      b2g_manifold* out);
+
+/* Stateless broadphase: B2broadPhase::update_pairs pair search (src/b2_broad_phase.rs:200-249)
+ * over the given fat AABBs of this world's proxies (n = fixture count, 4 floats each), with the
+ * world's filters. Writes at most cap pairs (lower, higher fixture index) and the total count. */
+int b2g_broadphase_pairs(b2g_world* world, const float* fat_aabbs, int64_t cap, int32_t* pairs, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B2G_H */
