@@ -64,7 +64,23 @@ B2G_HD float normalize(V2* v) {
 struct Rot {
   float s, c;
 };
-B2G_HD Rot rot(float angle) { Rot q; q.s = sinf(angle); q.c = cosf(angle); return q; }
+// B2Rot::set (src/b2_math.rs:372-376) is f32::sin / f32::cos, i.e. glibc sinf/cosf, which
+// evaluate in double precision and round once to float. The device does the same: double
+// sin/cos, one rounding; this matches glibc except when the double result lies within a
+// hair of a float rounding boundary.
+B2G_HD Rot rot(float angle) {
+  Rot q;
+#if defined(__HIP_DEVICE_COMPILE__)
+  double s, c;
+  sincos(static_cast<double>(angle), &s, &c);
+  q.s = static_cast<float>(s);
+  q.c = static_cast<float>(c);
+#else
+  q.s = sinf(angle);
+  q.c = cosf(angle);
+#endif
+  return q;
+}
 struct Xf {
   V2 p;
   Rot q;

@@ -21,22 +21,22 @@ def _dtypes(real):
     body_def = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                          ("position", real, 2), ("angle", real), ("linear_velocity", real, 2),
                          ("angular_velocity", real), ("linear_damping", real), ("angular_damping", real),
-                         ("gravity_scale", real), ("pad", real)])
+                         ("gravity_scale", real), ("pad", real)], align=True)
     shape_def = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", real),
-                          ("params", real, 20)])
+                          ("params", real, 20)], align=True)
     fixture_def = np.dtype([("friction", real), ("restitution", real), ("restitution_threshold", real),
                             ("density", real), ("is_sensor", "<i4"), ("category", "<u2"), ("mask", "<u2"),
-                            ("group", "<i2"), ("pad0", "<i2"), ("pad1", "<i4")])
+                            ("group", "<i2"), ("pad0", "<i2"), ("pad1", "<i4")], align=True)
     body_state = np.dtype([("p", real, 2), ("q", real, 2), ("c", real, 2), ("a", real), ("v", real, 2),
                            ("w", real), ("c0", real, 2), ("a0", real), ("sleep_time", real), ("awake", real),
-                           ("pad", real)])
+                           ("pad", real)], align=True)
     point = np.dtype([("local_point", real, 2), ("normal_impulse", real), ("tangent_impulse", real),
-                      ("id", "<u4"), ("pad", "<i4")])
+                      ("id", "<u4"), ("pad", "<i4")], align=True)
     manifold = np.dtype([("type", "<i4"), ("point_count", "<i4"), ("local_normal", real, 2),
-                         ("local_point", real, 2), ("points", point, 2)])
+                         ("local_point", real, 2), ("points", point, 2)], align=True)
     shape = np.dtype([("type", "<i4"), ("count", "<i4"), ("one_sided", "<i4"), ("pad", "<i4"), ("radius", real),
                       ("p", real, 2), ("centroid", real, 2), ("v0", real, 2), ("v1", real, 2), ("v2", real, 2),
-                      ("v3", real, 2), ("vertices", real, 16), ("normals", real, 16)])
+                      ("v3", real, 2), ("vertices", real, 16), ("normals", real, 16)], align=True)
     return dict(body_def=body_def, shape_def=shape_def, fixture_def=fixture_def, body_state=body_state,
                 manifold=manifold, shape=shape)
 

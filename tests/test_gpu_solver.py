@@ -1,0 +1,206 @@
+"""Solver parity on the GPU.
+
+* Colour-order replay: one SOLVE phase on the GPU from a given state and contact set, replayed
+  by the oracle's island solver (src/private/dynamics/b2_island_private.rs:129-328,
+  b2_contact_solver_private.rs:20-730) visiting the constraints in the GPU's colour order, must
+  agree within 1e-5 (positions, angles, velocities, stored impulses).
+* The colouring is conflict-free, the step is deterministic, dynamic-dynamic contacts conserve
+  linear momentum, and over 600 steps the pyramid's penetration, settling and rest state stay
+  within a stated band of the oracle run in the reference's sequential order.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DT = np.float32(1.0 / 60.0)
+POLY_RADIUS = np.float32(2.0) * np.float32(0.005)
+
+
+def _world(scene):
+    import b2g
+    w = b2g.World()
+    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    return w
+
+
+def _radius(scene, f):
+    sd = scene.shape_defs[f]
+    return np.where(sd["type"] == 0, sd["radius"], POLY_RADIUS).astype(np.float32)
+
+
+def replay_inputs(scene, w):
+    """Body and touching-contact inputs of the oracle replay, read from the GPU world after its
+    COLLIDE phase."""
+    st = w.body_states()
+    mass = w.mass()
+    bd = scene.body_defs
+    props = np.zeros((len(st), 10), np.float32)
+    props[:, 0] = mass[:, 1]
+    props[:, 1] = mass[:, 3]
+    props[:, 2] = mass[:, 0]
+    props[:, 3:5] = mass[:, 4:6]
+    props[:, 5] = bd["gravity_scale"]
+    props[:, 6] = bd["linear_damping"]
+    props[:, 7] = bd["angular_damping"]
+    fab, flags, man = w.contacts()
+    touch = np.nonzero(flags & 1)[0]
+    fa, fb = fab[touch, 0], fab[touch, 1]
+    body = np.asarray(scene.fixture_body)
+    fd = scene.fixture_defs
+    params = np.zeros((len(touch), 6), np.float32)
+    params[:, 0] = np.sqrt(fd["friction"][fa].astype(np.float32) * fd["friction"][fb].astype(np.float32))
+    params[:, 1] = np.maximum(fd["restitution"][fa], fd["restitution"][fb])
+    params[:, 2] = np.minimum(fd["restitution_threshold"][fa], fd["restitution_threshold"][fb])
+    params[:, 4] = _radius(scene, fa)
+    params[:, 5] = _radius(scene, fb)
+    cb = np.stack([body[fa], body[fb]], 1).astype(np.int32)
+    return st, props, touch, cb, params, man[touch]
+
+
+def solver_replay_check(oracle, scene, warm_steps, tol=1e-5):
+    w = _world(scene)
+    for _ in range(warm_steps):
+        w.step(DT, 8, 3)
+    import b2g
+    w.step_phase(b2g.PHASE_COLLIDE, DT, 8, 3)
+    st0, props, touch, cb, params, man = replay_inputs(scene, w)
+    w.step_phase(b2g.PHASE_SOLVE, DT, 8, 3)
+    st1 = w.body_states()
+    order_c, offs = w.solver_order()
+    _, _, man1 = w.contacts()
+    assert len(order_c) == len(touch) > 0
+    pos = {int(k): i for i, k in enumerate(touch)}
+    order = np.array([pos[int(k)] for k in order_c], np.int32)
+    inv_dt0 = np.float32(1.0) / DT if warm_steps > 0 else np.float32(0.0)
+    dt_ratio = np.float32(inv_dt0 * DT)
+    btype = scene.body_defs["type"]
+    ref_st, ref_man, _, n_islands = oracle.solver_replay(btype, props, st0, cb, params, man, order, DT, dt_ratio,
+                                                         w.gravity, 8, 3, True, True)
+    dyn = btype != 0
+    scale = max(1.0, float(np.abs(ref_st["c"][dyn]).max()))
+    for f in ("c", "a", "v", "w"):
+        g = st1[f][dyn].astype(np.float64)
+        r = ref_st[f][dyn].astype(np.float64)
+        err = np.abs(g - r).max()
+        s = scale if f in ("c",) else max(1.0, float(np.abs(r).max()))
+        assert err <= tol * s, "%s differs by %g (scale %g)" % (f, err, s)
+    gi = man1[touch]["points"][:, :]["normal_impulse"].astype(np.float64)
+    ri = ref_man["points"]["normal_impulse"].astype(np.float64)
+    gt = man1[touch]["points"]["tangent_impulse"].astype(np.float64)
+    rt = ref_man["points"]["tangent_impulse"].astype(np.float64)
+    iscale = max(1.0, float(np.abs(ri).max()))
+    assert np.abs(gi - ri).max() <= tol * iscale
+    assert np.abs(gt - rt).max() <= tol * iscale
+    exact = all(np.array_equal(st1[f][dyn], ref_st[f][dyn]) for f in ("c", "a", "v", "w"))
+    return {"constraints": len(order), "colours": len(offs) - 1, "islands": n_islands, "bit_exact": exact,
+            "order": order_c, "offsets": offs, "contact_bodies": cb, "touch": touch}
+
+
+@pytest.mark.parametrize("warm_steps", [0, 1, 30])
+def test_solver_replay_pyramid(oracle, warm_steps):
+    from b2g import scenes
+    info = solver_replay_check(oracle, scenes.pyramid(), warm_steps)
+    assert info["colours"] >= 2
+
+
+@pytest.mark.parametrize("warm_steps", [1, 20])
+def test_solver_replay_heap(oracle, warm_steps):
+    """Mixed circles / boxes / hulls in a box container: edge-circle, edge-polygon, polygon-circle
+    and polygon-polygon constraints, 1- and 2-point manifolds, several islands."""
+    from b2g import scenes
+    info = solver_replay_check(oracle, scenes.heap_world(600, seed=9, half_extent=12.0), warm_steps)
+    assert info["islands"] >= 1
+
+
+def test_colouring_is_conflict_free(oracle):
+    from b2g import scenes
+    sc = scenes.heap_world(1500, seed=4, half_extent=15.0)
+    info = solver_replay_check(oracle, sc, 25)
+    order, offs, cb, touch = info["order"], info["offsets"], info["contact_bodies"], info["touch"]
+    pos = {int(k): i for i, k in enumerate(touch)}
+    btype = sc.body_defs["type"]
+    for c in range(len(offs) - 1):
+        seen = set()
+        for k in order[offs[c]:offs[c + 1]]:
+            for b in cb[pos[int(k)]]:
+                if btype[b] == 2:
+                    assert b not in seen, "colour %d uses dynamic body %d twice" % (c, b)
+                    seen.add(b)
+
+
+def test_step_is_deterministic():
+    from b2g import scenes
+    sc = scenes.heap_world(800, seed=2, half_extent=12.0)
+    a, b = _world(sc), _world(sc)
+    for _ in range(60):
+        a.step(DT, 8, 3)
+        b.step(DT, 8, 3)
+    sa, sb = a.body_states(), b.body_states()
+    for f in ("c", "a", "v", "w"):
+        assert np.array_equal(sa[f], sb[f])
+
+
+def test_linear_momentum_conserved_without_gravity():
+    """Dynamic-dynamic impulses are equal and opposite: with no gravity and no wall contact,
+    total linear momentum is conserved to f32 round-off."""
+    import b2g
+    from b2g import scenes
+    sc = scenes.heap_world(900, seed=8, half_extent=200.0)  # walls far away
+    w = b2g.World(gravity=(0.0, 0.0))
+    w.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
+    mass = w.mass()[:, 0].astype(np.float64)
+
+    def momentum():
+        s = w.body_states()
+        return (mass[:, None] * s["v"].astype(np.float64)).sum(0), (mass[:, None] * np.abs(s["v"])).sum()
+
+    p0, ref = momentum()
+    steps_with_contacts = 0
+    for _ in range(60):
+        w.step(DT, 8, 3)
+        steps_with_contacts += w.profile()["touching"] > 0
+    p1, _ = momentum()
+    assert steps_with_contacts > 30
+    assert np.abs(p1 - p0).max() <= 1e-4 * ref
+
+
+def _max_penetration(oracle, scene, states, fab, flags, man):
+    touch = np.nonzero(flags & 1)[0]
+    if len(touch) == 0:
+        return 0.0
+    body = np.asarray(scene.fixture_body)
+    fa, fb = fab[touch, 0], fab[touch, 1]
+    xa = np.concatenate([states["p"][body[fa]], states["q"][body[fa]]], 1)
+    xb = np.concatenate([states["p"][body[fb]], states["q"][body[fb]]], 1)
+    wm = oracle.world_manifold(man[touch], xa, _radius(scene, fa), xb, _radius(scene, fb))
+    cnt = man[touch]["point_count"]
+    sep = np.where(np.arange(2)[None, :] < cnt[:, None], wm[:, 6:8], np.inf)
+    return float(max(0.0, -sep.min()))
+
+
+def test_pyramid_600_step_band(oracle):
+    """Pyramid, 600 steps: the GPU (coloured order) against the oracle (reference DFS order).
+    Band: max penetration <= oracle max + 0.01 m and <= 3x linear slop + 0.01; the top box ends
+    within 0.05 m of the oracle's; both end at rest (max speed < 0.05 m/s)."""
+    import b2g
+    from b2g import scenes
+    sc = scenes.pyramid()
+    w = _world(sc)
+    ow = oracle.world()
+    ow.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
+    pen_g, pen_o = 0.0, 0.0
+    for i in range(600):
+        w.step(DT, 8, 3)
+        ow.step(DT, 8, 3)
+        if i % 20 == 19:
+            fab, flags, man = w.contacts()
+            pen_g = max(pen_g, _max_penetration(oracle, sc, w.body_states(), fab, flags, man))
+            ofab, oflags, oman, _ = ow.contacts()
+            pen_o = max(pen_o, _max_penetration(oracle, sc, ow.bodies(), ofab, oflags, oman))
+    sg, so = w.body_states(), ow.bodies()
+    assert pen_g <= pen_o + 0.01
+    assert pen_g <= 3 * 0.005 + 0.01
+    assert abs(float(sg[-1]["c"][1]) - float(so[-1]["c"][1])) < 0.05
+    assert float(np.abs(sg["v"][1:]).max()) < 0.05
+    assert float(np.abs(so["v"][1:]).max()) < 0.05
