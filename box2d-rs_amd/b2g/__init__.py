@@ -73,6 +73,8 @@ def _load():
         "b2g_world_destroy": ([P], I),
         "b2g_world_set_flags": ([P, I, I, I, I], I),
         "b2g_world_add": ([P, I, P, I, P, P, P, P], I),
+        "b2g_world_add_batched": ([P, I, P, P, I, P, P, P, P], I),
+        "b2g_world_world_count": ([P, P], I),
         "b2g_world_step": ([P, F, I, I], I),
         "b2g_world_step_phase": ([P, I, F, I, I], I),
         "b2g_world_body_count": ([P, P], I),
@@ -153,6 +155,22 @@ class World:
         proxies = np.zeros(len(fb), np.int32)
         _check(_lib.b2g_world_add(self._h, len(bd), _ptr(bd), len(fb), _ptr(fb), _ptr(sd), _ptr(fd), _ptr(proxies)))
         return proxies
+
+    def add_batched(self, body_defs, body_world, fixture_body, shape_defs, fixture_defs):
+        """Bodies of many independent worlds (world id per body) stepped together."""
+        bd = _as(body_defs, BODY_DEF)
+        sd = _as(shape_defs, SHAPE_DEF)
+        fd = _as(fixture_defs, FIXTURE_DEF)
+        bw = np.ascontiguousarray(body_world, np.int32)
+        fb = np.ascontiguousarray(fixture_body, np.int32)
+        proxies = np.zeros(len(fb), np.int32)
+        _check(_lib.b2g_world_add_batched(self._h, len(bd), _ptr(bd), _ptr(bw), len(fb), _ptr(fb), _ptr(sd), _ptr(fd),
+                                          _ptr(proxies)))
+        return proxies
+
+    @property
+    def world_count(self):
+        return self._count(_lib.b2g_world_world_count)
 
     def step(self, dt=1.0 / 60.0, velocity_iterations=8, position_iterations=3):
         _check(_lib.b2g_world_step(self._h, float(dt), int(velocity_iterations), int(position_iterations)))

@@ -24,6 +24,7 @@ from . import BODY_DEF, FIXTURE_DEF, SHAPE_DEF, B2_DYNAMIC_BODY, B2_STATIC_BODY,
     SHAPE_EDGE, SHAPE_POLYGON
 
 Scene = namedtuple("Scene", "body_defs fixture_body shape_defs fixture_defs meta")
+BatchScene = namedtuple("BatchScene", "body_defs body_world fixture_body shape_defs fixture_defs meta")
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -248,3 +249,37 @@ def stack_world(n, seed, half_extent=20.0):
     fd = _fixture_defs(n)
     parts = concat(_container(width, half_extent), (bd, np.arange(n, dtype=np.int32), sd, fd))
     return Scene(*parts, {"name": "stack_world", "dynamic": n, "seed": seed})
+
+
+def world_seed(config, world):
+    return (int(config) << 32) | int(world)
+
+
+def batched_worlds(n_worlds=4096, config=3, lo=16, hi=1024):
+    """n_worlds independent worlds, body count per world round(exp(U[ln lo, ln hi])); even
+    worlds are heaps of the large-world mix in a 40 x 40 m box, odd worlds are box stacks of
+    height U{5..30}. World w uses seed (config << 32) | w."""
+    parts, worlds, sizes, kinds = [], [], [], []
+    for wi in range(n_worlds):
+        seed = world_seed(config, wi)
+        u = uniform(seed ^ 0xC0FFEE, np.arange(1, dtype=np.uint64))[0]
+        nw = int(round(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))))
+        sc = heap_world(nw, seed, half_extent=20.0) if wi % 2 == 0 else stack_world(nw, seed, half_extent=20.0)
+        parts.append(sc[:4])
+        worlds.append(np.full(len(sc.body_defs), wi, np.int32))
+        sizes.append(nw)
+        kinds.append("heap" if wi % 2 == 0 else "stack")
+    bd, fb, sd, fd = concat(*parts)
+    meta = {"name": "batched_worlds", "worlds": n_worlds, "dynamic": int(sum(sizes)), "sizes": sizes,
+            "kinds": kinds, "config": config}
+    return BatchScene(bd, np.concatenate(worlds), fb, sd, fd, meta)
+
+
+def world_of(batch, wi):
+    """The single-world Scene of world wi of a batch (for per-world references)."""
+    bsel = np.nonzero(batch.body_world == wi)[0]
+    fsel = np.nonzero(np.isin(batch.fixture_body, bsel))[0]
+    remap = np.full(len(batch.body_world), -1, np.int64)
+    remap[bsel] = np.arange(len(bsel))
+    return Scene(batch.body_defs[bsel], remap[batch.fixture_body[fsel]].astype(np.int32), batch.shape_defs[fsel],
+                 batch.fixture_defs[fsel], {"name": "world %d of batch" % wi, "dynamic": int(batch.meta["sizes"][wi])})

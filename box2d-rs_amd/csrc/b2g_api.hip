@@ -129,10 +129,15 @@ static void reset_mass(World* w, int b) {
   w->h_I[b] = I;
 }
 
-static int add_items(World* w, int nbn, const b2g_body_def* bodies, int nf, const int32_t* fixture_body,
-                     const b2g_shape_def* shapes, const b2g_fixture_def* fixtures, int32_t* proxy_ids) {
+static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_t* body_world, int nf,
+                     const int32_t* fixture_body, const b2g_shape_def* shapes, const b2g_fixture_def* fixtures,
+                     int32_t* proxy_ids) {
   for (int i = 0; i < nbn; ++i) {
     const b2g_body_def& d = bodies[i];
+    if (body_world && (body_world[i] < 0 || body_world[i] >= (1 << 20))) {
+      set_error("world id out of range [0, 2^20)");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
     if (d.type < 0 || d.type > 2) { set_error("invalid body type"); return B2G_ERR_INVALID_ARGUMENT; }
     if (d.linear_damping < 0.0f || d.angular_damping < 0.0f) {
       set_error("damping must be non-negative");
@@ -175,6 +180,9 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, int nf, cons
     w->h_props.push_back(make_float4(0.0f, d.gravity_scale, d.linear_damping, d.angular_damping));
     w->h_I.push_back(0.0f);
     w->body_fixtures.emplace_back();
+    int wid = body_world ? body_world[i] : 0;
+    w->h_bworld.push_back(wid);
+    w->n_worlds = std::max(w->n_worlds, wid + 1);
   }
   w->nb += nbn;
   for (int k = 0; k < nf; ++k) {
@@ -200,6 +208,7 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, int nf, cons
     w->h_fshape.push_back(make_int4(s.type | (s.count << 8) | (s.one_sided << 16), off, radius_bits, 0));
     w->h_fmat.push_back(make_float4(fd.friction, fd.restitution, fd.restitution_threshold, 0.0f));
     w->h_ffilter.push_back(make_int2(int(fd.category_bits) | (int(fd.mask_bits) << 16), fd.group_index));
+    w->h_fworld.push_back(w->h_bworld[b]);
     // create_proxies: AABB at the body transform, fattened by the AABB extension
     // (src/private/dynamics/b2_fixture.rs:114-133, src/private/collision/b2_dynamic_tree.rs:81-99)
     float4 x = w->h_xf[b];
@@ -282,7 +291,8 @@ int upload_world(World* w) {
       (st = up(w->f.body, w->h_fbody, w->stream)) || (st = up(w->f.shape, w->h_fshape, w->stream)) ||
       (st = up(w->f.mat, w->h_fmat, w->stream)) || (st = up(w->f.filter, w->h_ffilter, w->stream)) ||
       (st = up(w->f.aabb, w->h_faabb, w->stream)) || (st = up(w->f.fat, w->h_ffat, w->stream)) ||
-      (st = up(w->f.pool, w->h_pool, w->stream)) || (st = w->f.moved.reserve(std::max(w->np, 1))))
+      (st = up(w->f.pool, w->h_pool, w->stream)) || (st = up(w->f.world, w->h_fworld, w->stream)) ||
+      (st = w->f.moved.reserve(std::max(w->np, 1))))
     return st;
   // extend the open contact-set offset arrays: new proxies have empty segments
   for (int i = 0; i < 2; ++i) {
@@ -362,7 +372,7 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
   w->gravity = v2(gx, gy);
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
       w->counters.reserve(16) != B2G_OK ||
-      hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     set_error("world allocation failed");
     delete h;
     return B2G_ERR_DEVICE;
@@ -380,7 +390,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force);
   rel(w->b.type);
   rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
-  rel(w->f.pool);
+  rel(w->f.pool); rel(w->f.world);
   for (auto& c : w->c) {
     rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids);
   }
@@ -392,7 +402,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.isl_minsep); rel(s.isl_done);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.node_parent);
-  rel(bp.node_child); rel(bp.node_aabb); rel(bp.node_flag); rel(bp.count); rel(bp.bounds); rel(bp.scan_tmp);
+  rel(bp.node_child); rel(bp.node_worlds); rel(bp.node_aabb); rel(bp.node_flag); rel(bp.count); rel(bp.bounds); rel(bp.scan_tmp);
   rel(w->counters);
   if (w->h_counters) (void)hipHostFree(w->h_counters);
   for (int i = 0; i < 8; ++i)
@@ -425,7 +435,27 @@ int b2g_world_add(b2g_world* h, int32_t n_bodies, const b2g_body_def* bodies, in
   World* w = &h->w;
   int st;
   if (w->dev_nb > 0 && (st = download_state(w))) return st;
-  return add_items(w, n_bodies, bodies, n_fixtures, fixture_body, shapes, fixtures, proxy_ids);
+  return add_items(w, n_bodies, bodies, nullptr, n_fixtures, fixture_body, shapes, fixtures, proxy_ids);
+}
+
+int b2g_world_add_batched(b2g_world* h, int32_t n_bodies, const b2g_body_def* bodies, const int32_t* body_world,
+                          int32_t n_fixtures, const int32_t* fixture_body, const b2g_shape_def* shapes,
+                          const b2g_fixture_def* fixtures, int32_t* proxy_ids) {
+  if (!h || n_bodies < 0 || n_fixtures < 0 || (n_bodies > 0 && (!bodies || !body_world)) ||
+      (n_fixtures > 0 && (!fixture_body || !shapes || !fixtures))) {
+    set_error("b2g_world_add_batched: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if (w->dev_nb > 0 && (st = download_state(w))) return st;
+  return add_items(w, n_bodies, bodies, body_world, n_fixtures, fixture_body, shapes, fixtures, proxy_ids);
+}
+
+int b2g_world_world_count(b2g_world* h, int32_t* out) {
+  if (!h || !out) { set_error("invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  *out = h->w.n_worlds;
+  return B2G_OK;
 }
 
 int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {

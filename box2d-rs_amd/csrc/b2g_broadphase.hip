@@ -105,19 +105,24 @@ __device__ inline unsigned spread15(unsigned x) {
   return x;
 }
 
+// key = world id in the high bits, then a Morton code of `mbits` bits per axis of the fat-AABB
+// centre; sorting by it groups each world's proxies and orders them along a Z curve
 __global__ void __launch_bounds__(kBlock) k_morton(int n, const float4* __restrict__ fat, const int* __restrict__ bounds,
+                                                   const int* __restrict__ world, int mbits,
                                                    unsigned* __restrict__ keys, unsigned* __restrict__ vals) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float cells = float((1u << mbits) - 1u);
   float lx = float_unflip(bounds[0]), ly = float_unflip(bounds[1]);
   float hx = float_unflip(bounds[2]), hy = float_unflip(bounds[3]);
-  float sx = hx > lx ? 32767.0f / (hx - lx) : 0.0f;
-  float sy = hy > ly ? 32767.0f / (hy - ly) : 0.0f;
+  float sx = hx > lx ? cells / (hx - lx) : 0.0f;
+  float sy = hy > ly ? cells / (hy - ly) : 0.0f;
   float4 a = fat[i];
   float cx = 0.5f * (a.x + a.z), cy = 0.5f * (a.y + a.w);
-  unsigned ux = unsigned(fminf(fmaxf((cx - lx) * sx, 0.0f), 32767.0f));
-  unsigned uy = unsigned(fminf(fmaxf((cy - ly) * sy, 0.0f), 32767.0f));
-  keys[i] = spread15(ux) | (spread15(uy) << 1);
+  unsigned ux = unsigned(fminf(fmaxf((cx - lx) * sx, 0.0f), cells));
+  unsigned uy = unsigned(fminf(fmaxf((cy - ly) * sy, 0.0f), cells));
+  unsigned code = spread15(ux) | (spread15(uy) << 1);
+  keys[i] = (unsigned(world[i]) << (2 * mbits)) | code;
   vals[i] = unsigned(i);
 }
 
@@ -130,8 +135,9 @@ __device__ inline int lbvh_delta(const unsigned* __restrict__ keys, int n, int i
   return __clz(a ^ b);
 }
 
-__global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __restrict__ keys,
-                                                       int2* __restrict__ child, int* __restrict__ parent) {
+__global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __restrict__ keys, int wshift,
+                                                       int2* __restrict__ child, int2* __restrict__ worlds,
+                                                       int* __restrict__ parent) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -154,6 +160,7 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __
   int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
   int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
   child[i] = make_int2(left, right);
+  worlds[i] = make_int2(int(keys[lo] >> wshift), int(keys[hi] >> wshift));
   parent[left] = i;
   parent[right] = i;
 }
@@ -188,9 +195,11 @@ struct PairFilter {
   const int* __restrict__ btype;
   const int2* __restrict__ filter;
   const int4* __restrict__ fshape;
+  const int* __restrict__ world;
   __device__ bool ok(int q, int p) const {
     int bq = fbody[q], bp = fbody[p];
     if (bq == bp) return false;
+    if (world[q] != world[p]) return false;  // batched worlds never interact
     // B2body::should_collide: at least one body dynamic (src/private/dynamics/b2_body.rs:391-416)
     if (btype[bq] != 2 && btype[bp] != 2) return false;
     // default contact filter (src/private/dynamics/b2_world_callbacks.rs:6-18)
@@ -213,6 +222,7 @@ static constexpr int kStack = 64;
 template <bool WRITE>
 __global__ void __launch_bounds__(kTravBlock) k_traverse(int n, const unsigned* __restrict__ vals,
                                                          const float4* __restrict__ fat, const int2* __restrict__ child,
+                                                         const int2* __restrict__ worlds, int batched,
                                                          const float4* __restrict__ box, PairFilter pf,
                                                          int* __restrict__ count, const int* __restrict__ off,
                                                          int2* __restrict__ pairs, int* __restrict__ overflow) {
@@ -234,6 +244,7 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int n, const unsigned* 
   qb.hi = v2(qa.z, qa.w);
   int c = 0;
   int out = WRITE ? off[q] : 0;
+  const int qw = pf.world[q];
   int sp = 0;
   stack[sp++][threadIdx.x] = 0;
   while (sp > 0) {
@@ -251,6 +262,10 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int n, const unsigned* 
         ++c;
       }
     } else {
+      if (batched) {
+        int2 wr = worlds[node];
+        if (qw < wr.x || qw > wr.y) continue;
+      }
       if (sp + 2 > kStack) { atomicExch(overflow, 1); break; }
       int2 ch = child[node];
       stack[sp++][threadIdx.x] = ch.y;
@@ -282,7 +297,8 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   BroadphaseArrays& bp = w->bp;
   int st;
   if ((st = bp.keys.reserve(np)) || (st = bp.vals.reserve(np)) || (st = bp.node_parent.reserve(2 * np)) ||
-      (st = bp.node_child.reserve(np)) || (st = bp.node_aabb.reserve(2 * np)) || (st = bp.node_flag.reserve(np)) ||
+      (st = bp.node_child.reserve(np)) || (st = bp.node_worlds.reserve(np)) || (st = bp.node_aabb.reserve(2 * np)) ||
+      (st = bp.node_flag.reserve(np)) ||
       (st = bp.count.reserve(np + 1)) || (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(2)))
     return st;
   int* bnd = reinterpret_cast<int*>(bp.bounds.p);
@@ -291,20 +307,27 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   B2G_HIP(hipMemsetAsync(ovf, 0, sizeof(int), w->stream));
   int gb = std::min(blocks_for(np, kBlock), 1024);
   hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, np, fat, bnd);
-  hipLaunchKernelGGL(k_morton, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, fat, bnd, bp.keys.p,
-                     bp.vals.p);
-  if ((st = radix_sort_pairs(w, bp.keys.p, bp.vals.p, np, 30))) return st;
+  // world bits above Morton bits: 30 Morton bits for one world, fewer for many
+  int wbits = 0;
+  while ((1 << wbits) < w->n_worlds) ++wbits;
+  int mbits = std::min(15, (32 - wbits) / 2);
+  int key_bits = wbits + 2 * mbits;
+  hipLaunchKernelGGL(k_morton, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, fat, bnd, w->f.world.p,
+                     mbits, bp.keys.p, bp.vals.p);
+  if ((st = radix_sort_pairs(w, bp.keys.p, bp.vals.p, np, key_bits))) return st;
   if (np > 1) {
     B2G_HIP(hipMemsetAsync(bp.node_parent.p, 0xff, sizeof(int) * (2 * np - 1), w->stream));
     hipLaunchKernelGGL(k_lbvh_build, dim3(blocks_for(np - 1, kBlock)), dim3(kBlock), 0, w->stream, np, bp.keys.p,
-                       bp.node_child.p, bp.node_parent.p);
+                       2 * mbits, bp.node_child.p, bp.node_worlds.p, bp.node_parent.p);
     B2G_HIP(hipMemsetAsync(bp.node_flag.p, 0, sizeof(int) * np, w->stream));
   }
   hipLaunchKernelGGL(k_lbvh_refit, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, bp.vals.p, fat,
                      bp.node_child.p, bp.node_parent.p, bp.node_aabb.p, bp.node_flag.p);
-  PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p};
+  PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p};
+  const int batched = wbits > 0;
   hipLaunchKernelGGL(k_traverse<false>, dim3(blocks_for(np, kTravBlock)), dim3(kTravBlock), 0, w->stream, np,
-                     bp.vals.p, fat, bp.node_child.p, bp.node_aabb.p, pf, bp.count.p, nullptr, nullptr, ovf);
+                     bp.vals.p, fat, bp.node_child.p, bp.node_worlds.p, batched, bp.node_aabb.p, pf, bp.count.p,
+                     nullptr, nullptr, ovf);
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
   int h[2];
   B2G_HIP(hipMemcpyAsync(&h[0], dst.off.p + np, sizeof(int), hipMemcpyDeviceToHost, w->stream));
@@ -317,7 +340,8 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   int total = h[0];
   if ((st = dst.pair.reserve(std::max(total, 1)))) return st;
   hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(np, kTravBlock)), dim3(kTravBlock), 0, w->stream, np,
-                     bp.vals.p, fat, bp.node_child.p, bp.node_aabb.p, pf, nullptr, dst.off.p, dst.pair.p, ovf);
+                     bp.vals.p, fat, bp.node_child.p, bp.node_worlds.p, batched, bp.node_aabb.p, pf, nullptr,
+                     dst.off.p, dst.pair.p, ovf);
   B2G_HIP(hipGetLastError());
   *npairs = total;
   return B2G_OK;

@@ -88,6 +88,7 @@ struct ProxyArrays {
   DevArray<float4> aabb;    // swept AABB (proxy.aabb)
   DevArray<float4> fat;     // tree fat AABB
   DevArray<int> moved;      // the tree's moved flag (set when the fat AABB was re-fattened)
+  DevArray<int> world;      // world id of the proxy's body (batched worlds never interact)
   DevArray<float2> pool;    // shape vertex pool
 };
 
@@ -135,6 +136,7 @@ struct BroadphaseArrays {
   DevArray<unsigned> hist;       // radix histograms
   DevArray<int> node_parent;     // 2n-1 nodes
   DevArray<int2> node_child;     // internal nodes
+  DevArray<int2> node_worlds;    // internal nodes: world ids of the first and last leaf below
   DevArray<float4> node_aabb;    // all nodes
   DevArray<int> node_flag;
   DevArray<int> count;           // pairs per proxy
@@ -162,6 +164,8 @@ struct World {
   std::vector<int4> h_fshape;
   std::vector<float4> h_fmat, h_ffat, h_faabb;
   std::vector<int2> h_ffilter;
+  std::vector<int> h_bworld, h_fworld;  // world id per body / per proxy
+  int n_worlds = 1;
   std::vector<float2> h_pool;
   std::vector<HostShape> h_shapes;
   int nb = 0, np = 0;          // bodies, proxies on the host
@@ -181,8 +185,10 @@ struct World {
   SolverArrays s;
   BroadphaseArrays bp;
   DevArray<int> counters;   // small device counters
-  int* h_counters = nullptr;  // pinned host mirror
+  int* h_counters = nullptr;  // pinned host mirror (4 counters + 128 colour counts)
   int n_touch = 0, n_colors = 0, n_islands = 0;
+  int color_rounds = 16;       // colouring rounds launched before the first readback
+  int color_rounds_used = 0;
   std::vector<int> h_color_off;
   int pos_iters_run = 0;
   StepProfile prof;

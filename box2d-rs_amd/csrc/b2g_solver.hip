@@ -96,12 +96,14 @@ __global__ void __launch_bounds__(kBlock) k_compact(int n, const int* __restrict
   if (flag[k]) list[scan[k]] = k;
 }
 
-__global__ void __launch_bounds__(kBlock) k_constraint_bodies(int n, const int* __restrict__ list,
+// Kernels over the touching contacts read their count from device memory (*ncp) so that the
+// host does not wait for the compaction; they are launched over the pair count, an upper bound.
+__global__ void __launch_bounds__(kBlock) k_constraint_bodies(const int* __restrict__ ncp, const int* __restrict__ list,
                                                               const int2* __restrict__ ab, const int* __restrict__ fbody,
                                                               const int* __restrict__ btype, int2* __restrict__ bodies,
                                                               int* __restrict__ deg) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= *ncp) return;
   int2 f = ab[list[j]];
   int ba = fbody[f.x], bb = fbody[f.y];
   bodies[j] = make_int2(ba, bb);
@@ -109,11 +111,11 @@ __global__ void __launch_bounds__(kBlock) k_constraint_bodies(int n, const int* 
   if (btype[bb] == 2) atomicAdd(&deg[bb], 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_adjacency(int n, const int2* __restrict__ bodies,
+__global__ void __launch_bounds__(kBlock) k_fill_adjacency(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                            const int* __restrict__ btype, const int* __restrict__ off,
                                                            int* __restrict__ cursor, int* __restrict__ adj) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= *ncp) return;
   int2 b = bodies[j];
   if (btype[b.x] == 2) adj[off[b.x] + atomicAdd(&cursor[b.x], 1)] = j;
   if (btype[b.y] == 2) adj[off[b.y] + atomicAdd(&cursor[b.y], 1)] = j;
@@ -122,6 +124,13 @@ __global__ void __launch_bounds__(kBlock) k_fill_adjacency(int n, const int2* __
 __global__ void __launch_bounds__(kBlock) k_iota(int n, int* __restrict__ a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = i;
+}
+
+// per-step solver counters: [0] touching, [1] uncoloured, [2] colour overflow, [3] active islands
+__global__ void k_solver_counters_init(int* c) {
+  c[1] = c[0];
+  c[2] = 0;
+  c[3] = 0;
 }
 
 __device__ inline int uf_find(int* parent, int x) {
@@ -135,10 +144,10 @@ __device__ inline int uf_find(int* parent, int x) {
 }
 
 // union of the two non-static bodies of each constraint: hook the larger root under the smaller
-__global__ void __launch_bounds__(kBlock) k_union(int n, const int2* __restrict__ bodies, const int* __restrict__ btype,
-                                                  int* parent) {
+__global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, const int2* __restrict__ bodies,
+                                                  const int* __restrict__ btype, int* parent) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= *ncp) return;
   int2 b = bodies[j];
   if (btype[b.x] == 0 || btype[b.y] == 0) return;
   int a = b.x, c = b.y;
@@ -176,12 +185,13 @@ __device__ inline unsigned long long priority(int j) {
 // One Jones-Plassmann-Luby round: an uncoloured constraint whose priority beats every
 // uncoloured neighbour (a constraint sharing a dynamic body) takes the smallest colour no
 // coloured neighbour has. Neighbours coloured in this same round count as uncoloured.
-__global__ void __launch_bounds__(kBlock) k_color_round(int n, int round, const int2* __restrict__ bodies,
-                                                        const int* __restrict__ btype, const int* __restrict__ off,
-                                                        const int* __restrict__ adj, int* state, int* __restrict__ remaining,
+__global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ ncp, int round,
+                                                        const int2* __restrict__ bodies, const int* __restrict__ btype,
+                                                        const int* __restrict__ off, const int* __restrict__ adj,
+                                                        int* state, int* __restrict__ remaining,
                                                         int* __restrict__ overflow) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= *ncp) return;
   if (state[j] != -1) return;
   const unsigned long long pj = priority(j);
   unsigned long long used0 = 0, used1 = 0;
@@ -211,9 +221,10 @@ __global__ void __launch_bounds__(kBlock) k_color_round(int n, int round, const 
   atomicSub(remaining, 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_color_hist(int n, const int* __restrict__ state, int* __restrict__ count) {
+__global__ void __launch_bounds__(kBlock) k_color_hist(const int* __restrict__ ncp, const int* __restrict__ state,
+                                                       int* __restrict__ count) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= *ncp) return;
   atomicAdd(&count[state[j] & 0xff], 1);
 }
 
@@ -662,72 +673,71 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   SolverArrays& s = w->s;
   ContactArrays& c = w->c[w->cur];
   int st;
-  int* cnt = w->counters.p;  // [0] touching, [1] remaining, [2] overflow, [3] active islands
+  int* cnt = w->counters.p;  // [0] touching, [1] uncoloured, [2] colour overflow, [3] active islands
   hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
                      w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p, w->gravity.x,
                      w->gravity.y, dt);
-  // touching contacts
-  int n_pairs = w->n_pairs;
-  if ((st = s.touch_scan.reserve(std::max(n_pairs, 1) + 1))) return st;
-  if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
+  // touching contacts: compacted on the device; the constraint arrays are sized for all pairs
+  const int n_pairs = w->n_pairs;
+  if ((st = reserve_solver(w, n_pairs))) return st;
   if ((st = s.deg.reserve(nb + 1)) || (st = s.adj_off.reserve(nb + 1)) || (st = s.adj_cursor.reserve(nb)) ||
       (st = s.parent.reserve(nb)) || (st = s.isl_minsep.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
       (st = s.color_count.reserve(kMaxColors + 1)) || (st = s.color_off.reserve(kMaxColors + 1)) ||
       (st = s.color_cursor.reserve(kMaxColors)))
     return st;
-  B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, sizeof(int), hipMemcpyDeviceToHost, w->stream));
-  B2G_HIP(hipStreamSynchronize(w->stream));
-  const int nc = w->h_counters[0];
-  w->n_touch = nc;
-  if ((st = reserve_solver(w, nc))) return st;
+  if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
+  hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(1), 0, w->stream, cnt);
+  const int gp = blocks_for(std::max(n_pairs, 1), kBlock);
   if (n_pairs > 0)
-    hipLaunchKernelGGL(k_compact, dim3(blocks_for(n_pairs, kBlock)), dim3(kBlock), 0, w->stream, n_pairs,
-                       s.touch_flag.p, s.touch_scan.p, s.touch_list.p);
+    hipLaunchKernelGGL(k_compact, dim3(gp), dim3(kBlock), 0, w->stream, n_pairs, s.touch_flag.p, s.touch_scan.p,
+                       s.touch_list.p);
   // constraint bodies, degrees, islands
   B2G_HIP(hipMemsetAsync(s.deg.p, 0, sizeof(int) * (nb + 1), w->stream));
   B2G_HIP(hipMemsetAsync(s.adj_cursor.p, 0, sizeof(int) * nb, w->stream));
   hipLaunchKernelGGL(k_iota, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p);
-  if (nc > 0) {
-    hipLaunchKernelGGL(k_constraint_bodies, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
-                       s.touch_list.p, c.ab.p, w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p);
-    hipLaunchKernelGGL(k_union, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_bodies.p,
-                       w->b.type.p, s.parent.p);
+  if (n_pairs > 0) {
+    hipLaunchKernelGGL(k_constraint_bodies, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.touch_list.p, c.ab.p,
+                       w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p);
+    hipLaunchKernelGGL(k_union, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p, s.parent.p);
   }
   hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
                      s.isl_minsep.p, s.isl_done.p);
-  int ncolors = 0;
-  if (nc > 0) {
+  int ncolors = 0, nc = 0;
+  if (n_pairs > 0) {
     if ((st = exclusive_scan(w, s.deg.p, s.adj_off.p, nb + 1, nullptr))) return st;
-    hipLaunchKernelGGL(k_fill_adjacency, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_bodies.p,
-                       w->b.type.p, s.adj_off.p, s.adj_cursor.p, s.adj.p);
-    B2G_HIP(hipMemsetAsync(s.color_state.p, 0xff, sizeof(int) * nc, w->stream));
-    int init[4] = {nc, nc, 0, 0};
-    B2G_HIP(hipMemcpyAsync(cnt, init, sizeof(init), hipMemcpyHostToDevice, w->stream));
+    hipLaunchKernelGGL(k_fill_adjacency, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
+                       s.adj_off.p, s.adj_cursor.p, s.adj.p);
+    B2G_HIP(hipMemsetAsync(s.color_state.p, 0xff, sizeof(int) * n_pairs, w->stream));
+    // Jones-Plassmann-Luby rounds: a first batch without waiting (rounds after the last
+    // constraint is coloured find nothing to do), then more only if some remain
     int round = 0;
     for (;;) {
-      int batch = round == 0 ? 8 : 4;
+      int batch = round == 0 ? w->color_rounds : 4;
       for (int r = 0; r < batch; ++r, ++round)
-        hipLaunchKernelGGL(k_color_round, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, round,
-                           s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
+        hipLaunchKernelGGL(k_color_round, dim3(gp), dim3(kBlock), 0, w->stream, cnt, round, s.k_bodies.p,
+                           w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
+      B2G_HIP(hipMemsetAsync(s.color_count.p, 0, sizeof(int) * (kMaxColors + 1), w->stream));
+      hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, s.color_count.p);
       B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, 4 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
+      B2G_HIP(hipMemcpyAsync(w->h_counters + 4, s.color_count.p, sizeof(int) * kMaxColors, hipMemcpyDeviceToHost,
+                             w->stream));
       B2G_HIP(hipStreamSynchronize(w->stream));
       if (w->h_counters[1] == 0) break;
-      if (round > 4 * nc + 64) {
+      if (round > 4 * n_pairs + 64) {
         set_error("colouring did not converge");
         return B2G_ERR_DEVICE;
       }
     }
+    w->color_rounds_used = round;
+    nc = w->h_counters[0];
     if (w->h_counters[2]) {
       set_error("colouring needs more than 127 colours (a dynamic body with too many contacts)");
       return B2G_ERR_CAPACITY;
     }
-    B2G_HIP(hipMemsetAsync(s.color_count.p, 0, sizeof(int) * (kMaxColors + 1), w->stream));
-    B2G_HIP(hipMemsetAsync(s.color_cursor.p, 0, sizeof(int) * kMaxColors, w->stream));
-    hipLaunchKernelGGL(k_color_hist, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.color_state.p,
-                       s.color_count.p);
-    std::vector<int> hc(kMaxColors);
-    B2G_HIP(hipMemcpyAsync(hc.data(), s.color_count.p, sizeof(int) * kMaxColors, hipMemcpyDeviceToHost, w->stream));
-    B2G_HIP(hipStreamSynchronize(w->stream));
+  }
+  w->n_touch = nc;
+  if (nc > 0) {
+    const int* hc = w->h_counters + 4;
     w->h_color_off.assign(1, 0);
     for (int i = 0; i < kMaxColors; ++i) {
       if (hc[i] == 0) break;  // first-fit colours are contiguous from 0
@@ -738,6 +748,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     for (int i = 0; i <= ncolors; ++i) offs[i] = w->h_color_off[i];
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * (kMaxColors + 1), hipMemcpyHostToDevice,
                            w->stream));
+    B2G_HIP(hipMemsetAsync(s.color_cursor.p, 0, sizeof(int) * kMaxColors, w->stream));
     hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
                        s.color_state.p, s.color_off.p, s.color_cursor.p, s.order.p);
     InitArgs ia;

@@ -127,6 +127,15 @@ int b2g_world_add(b2g_world* world, int32_t n_bodies, const b2g_body_def* bodies
                   const int32_t* fixture_body, const b2g_shape_def* shapes, const b2g_fixture_def* fixtures,
                   int32_t* proxy_ids);
 
+/* Batched worlds: as b2g_world_add, with a world id per body (0 <= id < 2^20). Bodies of
+ * different worlds never interact; every phase of b2g_world_step advances all worlds in the
+ * same kernels. No counterpart in the reference (one B2world per world there). */
+int b2g_world_add_batched(b2g_world* world, int32_t n_bodies, const b2g_body_def* bodies, const int32_t* body_world,
+                          int32_t n_fixtures, const int32_t* fixture_body, const b2g_shape_def* shapes,
+                          const b2g_fixture_def* fixtures, int32_t* proxy_ids);
+/* Number of worlds (1 + largest world id added). */
+int b2g_world_world_count(b2g_world* world, int32_t* out);
+
 /* B2world::step(dt, velocity_iterations, position_iterations) -- src/b2_world.rs:98-100,
  * src/private/dynamics/b2_world.rs:903-959. */
 int b2g_world_step(b2g_world* world, float dt, int32_t velocity_iterations, int32_t position_iterations);

@@ -20,7 +20,10 @@ POLY_RADIUS = np.float32(2.0) * np.float32(0.005)
 def _world(scene):
     import b2g
     w = b2g.World()
-    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    if hasattr(scene, "body_world"):
+        w.add_batched(scene.body_defs, scene.body_world, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    else:
+        w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     return w
 
 
