@@ -266,9 +266,14 @@ static int download_state(World* w) {
   return B2G_OK;
 }
 
+static bool host_is_static(const World* w, int f) { return w->host_type[w->h_fbody[f]] == B2G_STATIC; }
+
 int upload_world(World* w) {
   if (!w->dirty) return B2G_OK;
   int st;
+  // arrays may be reallocated below (with copies on the default stream): let this world's
+  // stream finish first
+  B2G_HIP(hipStreamSynchronize(w->stream));
   // contact sets index proxies, which only grow, so they stay valid; their offset arrays
   // are extended with empty segments for the new proxies
   for (int i = 0; i < 2; ++i) {
@@ -294,6 +299,34 @@ int upload_world(World* w) {
       (st = up(w->f.pool, w->h_pool, w->stream)) || (st = up(w->f.world, w->h_fworld, w->stream)) ||
       (st = w->f.moved.reserve(std::max(w->np, 1))))
     return st;
+  // Big static proxies (ground, container walls: an AABB side over kLooseExtent) stay out of
+  // the BVH, where they would inflate every ancestor box; each world's list is tested
+  // directly by every emitting proxy of that world. Static fat AABBs never change.
+  {
+    // (host buffers are world members: the asynchronous copies below read them after return)
+    const float kLooseExtent = 8.0f;
+    std::vector<int>& tree = w->h_tree;
+    std::vector<int>& loose_off = w->h_loose_off;
+    std::vector<int>& loose = w->h_loose;
+    tree.clear();
+    loose.clear();
+    loose_off.assign(w->n_worlds + 1, 0);
+    std::vector<std::vector<int>> per_world(w->n_worlds);
+    for (int f = 0; f < w->np; ++f) {
+      float4 a = w->h_ffat[f];
+      bool big = host_is_static(w, f) && (a.z - a.x > kLooseExtent || a.w - a.y > kLooseExtent);
+      if (big) per_world[w->h_fworld[f]].push_back(f);
+      else tree.push_back(f);
+    }
+    for (int i = 0; i < w->n_worlds; ++i) {
+      loose_off[i + 1] = loose_off[i] + int(per_world[i].size());
+      loose.insert(loose.end(), per_world[i].begin(), per_world[i].end());
+    }
+    w->n_tree = int(tree.size());
+    if ((st = up(w->f.tree_list, tree, w->stream)) || (st = up(w->f.loose_off, loose_off, w->stream)) ||
+        (st = up(w->f.loose, loose, w->stream)))
+      return st;
+  }
   // extend the open contact-set offset arrays: new proxies have empty segments
   for (int i = 0; i < 2; ++i) {
     ContactArrays& c = w->c[i];
@@ -390,7 +423,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force);
   rel(w->b.type);
   rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
-  rel(w->f.pool); rel(w->f.world);
+  rel(w->f.pool); rel(w->f.world); rel(w->f.tree_list); rel(w->f.loose_off); rel(w->f.loose);
   for (auto& c : w->c) {
     rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids);
   }
@@ -401,8 +434,8 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.color_state); rel(s.color_count); rel(s.color_off); rel(s.color_cursor); rel(s.order); rel(s.parent);
   rel(s.isl_minsep); rel(s.isl_done);
   BroadphaseArrays& bp = w->bp;
-  rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.node_parent);
-  rel(bp.node_child); rel(bp.node_worlds); rel(bp.node_aabb); rel(bp.node_flag); rel(bp.count); rel(bp.bounds); rel(bp.scan_tmp);
+  rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.node_child);
+  rel(bp.node_worlds); rel(bp.node_aabb); rel(bp.count); rel(bp.slots); rel(bp.bounds); rel(bp.scan_tmp);
   rel(w->counters);
   if (w->h_counters) (void)hipHostFree(w->h_counters);
   for (int i = 0; i < 8; ++i)

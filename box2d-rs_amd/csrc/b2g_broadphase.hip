@@ -68,10 +68,12 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
 }
 
 // ---- scene bounds and Morton codes ---------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_bounds(int n, const float4* __restrict__ fat, int* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k_bounds(int n, const int* __restrict__ list, const float4* __restrict__ fat,
+                                                   int* __restrict__ out) {
+  __shared__ float red[4][kBlock / 64];
   float lx = kMaxFloat, ly = kMaxFloat, hx = -kMaxFloat, hy = -kMaxFloat;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 a = fat[i];
+    float4 a = fat[list[i]];
     float cx = 0.5f * (a.x + a.z), cy = 0.5f * (a.y + a.w);
     lx = fminf(lx, cx); ly = fminf(ly, cy); hx = fmaxf(hx, cx); hy = fmaxf(hy, cy);
   }
@@ -81,7 +83,13 @@ __global__ void __launch_bounds__(kBlock) k_bounds(int n, const float4* __restri
     hx = fmaxf(hx, __shfl_xor(hx, d, 64));
     hy = fmaxf(hy, __shfl_xor(hy, d, 64));
   }
-  if ((threadIdx.x & 63) == 0) {
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][wid] = lx; red[1][wid] = ly; red[2][wid] = hx; red[3][wid] = hy; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) {
+      lx = fminf(lx, red[0][w]); ly = fminf(ly, red[1][w]); hx = fmaxf(hx, red[2][w]); hy = fmaxf(hy, red[3][w]);
+    }
     atomicMin(&out[0], float_flip(lx));
     atomicMin(&out[1], float_flip(ly));
     atomicMax(&out[2], float_flip(hx));
@@ -94,6 +102,8 @@ __global__ void k_bounds_init(int* out) {
   out[1] = float_flip(kMaxFloat);
   out[2] = float_flip(-kMaxFloat);
   out[3] = float_flip(-kMaxFloat);
+  out[4] = 0;  // traversal stack overflow
+  out[5] = 0;  // proxies whose pairs overflow their slots
 }
 
 __device__ inline unsigned spread15(unsigned x) {
@@ -107,11 +117,12 @@ __device__ inline unsigned spread15(unsigned x) {
 
 // key = world id in the high bits, then a Morton code of `mbits` bits per axis of the fat-AABB
 // centre; sorting by it groups each world's proxies and orders them along a Z curve
-__global__ void __launch_bounds__(kBlock) k_morton(int n, const float4* __restrict__ fat, const int* __restrict__ bounds,
-                                                   const int* __restrict__ world, int mbits,
-                                                   unsigned* __restrict__ keys, unsigned* __restrict__ vals) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void __launch_bounds__(kBlock) k_morton(int n, const int* __restrict__ list, const float4* __restrict__ fat,
+                                                   const int* __restrict__ bounds, const int* __restrict__ world,
+                                                   int mbits, unsigned* __restrict__ keys, unsigned* __restrict__ vals) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = list[k];
   const float cells = float((1u << mbits) - 1u);
   float lx = float_unflip(bounds[0]), ly = float_unflip(bounds[1]);
   float hx = float_unflip(bounds[2]), hy = float_unflip(bounds[3]);
@@ -126,8 +137,57 @@ __global__ void __launch_bounds__(kBlock) k_morton(int n, const float4* __restri
   vals[i] = unsigned(i);
 }
 
-// ---- LBVH (Karras 2012) --------------------------------------------------------------------
-// Nodes: internal 0..n-2, leaves n-1..2n-2 (leaf of sorted position k is n-1+k).
+// ---- BVH -------------------------------------------------------------------------------------
+// Topology: Karras's binary radix tree over the sorted keys (internal node i splits at the
+// highest differing key bit, so a node covers a compact Z-order cell; internal 0..n-2, leaf of
+// sorted position k is n-1+k). Node boxes: each internal node covers a contiguous range of
+// sorted leaves, and its box is the union over that range, read from a segment tree built
+// level by level in LDS (level 0 = the sorted leaf boxes; node (L, i) = (L-1, 2i) u
+// (L-1, 2i+1)). Neither step needs atomics or communication between workgroups, so the build
+// is a handful of short launches and the tree is a pure function of the sorted order.
+struct Levels {
+  int count;
+  int size[32];
+  int off[32];
+};
+
+static constexpr int kReduceLevels = 8;
+
+__global__ void __launch_bounds__(kBlock) k_seg_reduce(Levels lv, int L0, const unsigned* __restrict__ vals,
+                                                       const float4* __restrict__ fat, float4* __restrict__ seg) {
+  __shared__ float4 sb[kBlock];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * kBlock + t;
+  float4 e = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
+  if (i < lv.size[L0]) {
+    if (L0 == 0) {
+      e = fat[vals[i]];
+      seg[i] = e;
+    } else {
+      e = seg[lv.off[L0] + i];
+    }
+  }
+  sb[t] = e;
+  __syncthreads();
+  for (int j = 1; j <= kReduceLevels; ++j) {
+    const int L = L0 + j;
+    if (L >= lv.count) break;
+    const int cnt = kBlock >> j;
+    float4 r = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
+    if (t < cnt) {
+      float4 a = sb[2 * t], b = sb[2 * t + 1];
+      r = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+    }
+    __syncthreads();
+    if (t < cnt) {
+      sb[t] = r;
+      const int gi = blockIdx.x * cnt + t;
+      if (gi < lv.size[L]) seg[lv.off[L] + gi] = r;
+    }
+    __syncthreads();
+  }
+}
+
 __device__ inline int lbvh_delta(const unsigned* __restrict__ keys, int n, int i, int j) {
   if (j < 0 || j >= n) return -1;
   unsigned a = keys[i], b = keys[j];
@@ -135,9 +195,14 @@ __device__ inline int lbvh_delta(const unsigned* __restrict__ keys, int n, int i
   return __clz(a ^ b);
 }
 
-__global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __restrict__ keys, int wshift,
-                                                       int2* __restrict__ child, int2* __restrict__ worlds,
-                                                       int* __restrict__ parent) {
+__device__ inline void box_union(float4& r, float4 b) {
+  r = make_float4(fminf(r.x, b.x), fminf(r.y, b.y), fmaxf(r.z, b.z), fmaxf(r.w, b.w));
+}
+
+// Karras 2012 internal node i: children, leaf range -> box (segment-tree query) and worlds
+__global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __restrict__ keys, int wshift, Levels lv,
+                                                       const float4* __restrict__ seg, int2* __restrict__ child,
+                                                       float4* __restrict__ box, int2* __restrict__ worlds) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -161,32 +226,16 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __
   int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
   child[i] = make_int2(left, right);
   worlds[i] = make_int2(int(keys[lo] >> wshift), int(keys[hi] >> wshift));
-  parent[left] = i;
-  parent[right] = i;
-}
-
-// bottom-up refit: the second child to arrive at a node computes its box and climbs on
-__global__ void __launch_bounds__(kBlock) k_lbvh_refit(int n, const unsigned* __restrict__ vals,
-                                                       const float4* __restrict__ fat, const int2* __restrict__ child,
-                                                       const int* __restrict__ parent, float4* __restrict__ box,
-                                                       int* __restrict__ flag) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int node = n - 1 + k;
-  box[node] = fat[vals[k]];
-  if (n == 1) return;
-  int p = parent[node];
-  while (p >= 0) {
-    __threadfence();
-    int prev = __hip_atomic_fetch_add(&flag[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == 0) return;  // sibling not done yet; it will continue
-    int2 c = child[p];
-    float4 a = box[c.x];
-    float4 b = box[c.y];
-    a = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
-    box[p] = a;
-    p = p == 0 ? -1 : parent[p];
+  // union of leaves [lo, hi]: bottom-up segment-tree query
+  float4 r = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
+  int a = lo, b = hi + 1;
+  for (int L = 0; a < b; ++L) {
+    if (a & 1) box_union(r, seg[lv.off[L] + a++]);
+    if (b & 1) box_union(r, seg[lv.off[L] + --b]);
+    a >>= 1;
+    b >>= 1;
   }
+  box[i] = r;
 }
 
 // ---- pair search --------------------------------------------------------------------------
@@ -218,61 +267,111 @@ struct PairFilter {
 
 static constexpr int kTravBlock = 64;
 static constexpr int kStack = 64;
+static constexpr int kSlots = 16;
 
-template <bool WRITE>
-__global__ void __launch_bounds__(kTravBlock) k_traverse(int n, const unsigned* __restrict__ vals,
-                                                         const float4* __restrict__ fat, const int2* __restrict__ child,
-                                                         const int2* __restrict__ worlds, int batched,
-                                                         const float4* __restrict__ box, PairFilter pf,
-                                                         int* __restrict__ count, const int* __restrict__ off,
-                                                         int2* __restrict__ pairs, int* __restrict__ overflow) {
+struct TreeView {
+  int n;                       // leaves in the tree
+  const unsigned* vals;        // sorted leaf -> proxy
+  const int2* child;           // internal nodes
+  const float4* box;           // internal node boxes
+  const int2* worlds;          // internal node world ranges
+  const float4* leaf;          // sorted leaf boxes (segment-tree level 0)
+  const int* loose_off;        // per world: range of big static proxies kept out of the tree
+  const int* loose;
+  int batched;
+};
+
+// One tree walk per emitting proxy (threads in Morton order, so a wave walks neighbouring
+// paths), then the world's big static proxies are tested directly. First pass: the first
+// kSlots pairs go to the proxy's slots and the count is recorded. The overflow pass re-walks
+// only proxies with more than kSlots pairs and writes their whole segment; the walk order is
+// deterministic, so both passes list the pairs in the same order.
+template <bool OVERFLOW_PASS>
+__global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const int* __restrict__ qlist,
+                                                         const float4* __restrict__ fat, TreeView tv, PairFilter pf,
+                                                         int* __restrict__ count, int* __restrict__ slots,
+                                                         const int* __restrict__ off, int2* __restrict__ pairs,
+                                                         int* __restrict__ flags) {
   __shared__ int stack[kStack][kTravBlock];
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int q = int(vals[k]);
+  if (k >= nq) return;
+  const int q = qlist[k];
+  if (OVERFLOW_PASS && count[q] <= kSlots) return;
   // Emitter rule: a pair is emitted once, by its non-static proxy; a pair of two non-static
-  // proxies by the lower index. Static proxies (e.g. long container edges) emit nothing, so
-  // no thread walks the hundreds of pairs a wall collects.
-  const bool q_static = pf.btype[pf.fbody[q]] == 0;
-  if (n == 1 || q_static) {
-    if (!WRITE) count[q] = 0;
+  // proxies by the lower index. Static proxies emit nothing.
+  if (pf.btype[pf.fbody[q]] == 0) {
+    if (!OVERFLOW_PASS) count[q] = 0;
     return;
   }
   const float4 qa = fat[q];
   Aabb qb;
   qb.lo = v2(qa.x, qa.y);
   qb.hi = v2(qa.z, qa.w);
-  int c = 0;
-  int out = WRITE ? off[q] : 0;
   const int qw = pf.world[q];
-  int sp = 0;
-  stack[sp++][threadIdx.x] = 0;
-  while (sp > 0) {
-    int node = stack[--sp][threadIdx.x];
-    float4 nb = box[node];
+  int c = 0;
+  const int out = OVERFLOW_PASS ? off[q] : 0;
+  auto visit_leaf = [&](int p) {
+    bool emit = p != q && (p > q || pf.btype[pf.fbody[p]] == 0);
+    if (emit && pf.ok(q, p)) {
+      if (OVERFLOW_PASS) pairs[out + c] = make_int2(q, p);
+      else if (c < kSlots) slots[q * kSlots + c] = p;
+      ++c;
+    }
+  };
+  const int n = tv.n;
+  if (n == 1) {
+    float4 nb = tv.leaf[0];
     Aabb b;
     b.lo = v2(nb.x, nb.y);
     b.hi = v2(nb.z, nb.w);
-    if (!overlap(b, qb)) continue;
-    if (node >= n - 1) {
-      int p = int(vals[node - (n - 1)]);
-      bool emit = p != q && (p > q || pf.btype[pf.fbody[p]] == 0);
-      if (emit && pf.ok(q, p)) {
-        if (WRITE) pairs[out + c] = make_int2(q, p);
-        ++c;
-      }
-    } else {
-      if (batched) {
-        int2 wr = worlds[node];
+    if (overlap(b, qb)) visit_leaf(int(tv.vals[0]));
+  } else if (n > 1) {
+    int sp = 0;
+    stack[sp++][threadIdx.x] = 0;
+    while (sp > 0) {
+      const int node = stack[--sp][threadIdx.x];
+      const bool leaf = node >= n - 1;
+      if (!leaf && tv.batched) {
+        int2 wr = tv.worlds[node];
         if (qw < wr.x || qw > wr.y) continue;
       }
-      if (sp + 2 > kStack) { atomicExch(overflow, 1); break; }
-      int2 ch = child[node];
-      stack[sp++][threadIdx.x] = ch.y;
-      stack[sp++][threadIdx.x] = ch.x;
+      float4 nb = leaf ? tv.leaf[node - (n - 1)] : tv.box[node];
+      Aabb b;
+      b.lo = v2(nb.x, nb.y);
+      b.hi = v2(nb.z, nb.w);
+      if (!overlap(b, qb)) continue;
+      if (leaf) {
+        visit_leaf(int(tv.vals[node - (n - 1)]));
+      } else {
+        if (sp + 2 > kStack) { atomicExch(&flags[0], 1); break; }
+        int2 ch = tv.child[node];
+        stack[sp++][threadIdx.x] = ch.y;
+        stack[sp++][threadIdx.x] = ch.x;
+      }
     }
   }
-  if (!WRITE) count[q] = c;
+  for (int e = tv.loose_off[qw], end = tv.loose_off[qw + 1]; e < end; ++e) {
+    const int p = tv.loose[e];
+    float4 nb = fat[p];
+    Aabb b;
+    b.lo = v2(nb.x, nb.y);
+    b.hi = v2(nb.z, nb.w);
+    if (overlap(b, qb)) visit_leaf(p);
+  }
+  if (!OVERFLOW_PASS) {
+    count[q] = c;
+    if (c > kSlots) atomicAdd(&flags[1], 1);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_from_slots(int np, const int* __restrict__ count,
+                                                             const int* __restrict__ slots, const int* __restrict__ off,
+                                                             int2* __restrict__ pairs) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  int c = min(count[q], kSlots);
+  int o = off[q];
+  for (int j = 0; j < c; ++j) pairs[o + j] = make_int2(q, slots[q * kSlots + j]);
 }
 
 // predecessor of each new pair in the previous contact set (same emitter segment)
@@ -295,53 +394,73 @@ __global__ void __launch_bounds__(kBlock) k_merge(int n, const int2* __restrict_
 // pair count in *npairs. Does not touch manifolds.
 static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, int* npairs) {
   BroadphaseArrays& bp = w->bp;
+  const int nt = w->n_tree;  // proxies in the tree (the others are big statics, tested directly)
   int st;
-  if ((st = bp.keys.reserve(np)) || (st = bp.vals.reserve(np)) || (st = bp.node_parent.reserve(2 * np)) ||
-      (st = bp.node_child.reserve(np)) || (st = bp.node_worlds.reserve(np)) || (st = bp.node_aabb.reserve(2 * np)) ||
-      (st = bp.node_flag.reserve(np)) ||
-      (st = bp.count.reserve(np + 1)) || (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(2)))
+  Levels lv;
+  lv.count = 0;
+  int total_nodes = 0;
+  for (int sz = std::max(nt, 1);; sz = (sz + 1) / 2) {
+    if (lv.count == 32) { set_error("broadphase: too many proxies"); return B2G_ERR_CAPACITY; }
+    lv.size[lv.count] = sz;
+    lv.off[lv.count] = total_nodes;
+    total_nodes += sz;
+    ++lv.count;
+    if (sz <= 1) break;
+  }
+  if ((st = bp.keys.reserve(std::max(nt, 1))) || (st = bp.vals.reserve(std::max(nt, 1))) ||
+      (st = bp.seg.reserve(total_nodes)) || (st = bp.node_child.reserve(std::max(nt, 1))) ||
+      (st = bp.node_aabb.reserve(std::max(nt, 1))) || (st = bp.node_worlds.reserve(std::max(nt, 1))) ||
+      (st = bp.count.reserve(np + 1)) || (st = bp.slots.reserve(size_t(np) * kSlots)) ||
+      (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(2)))
     return st;
   int* bnd = reinterpret_cast<int*>(bp.bounds.p);
-  int* ovf = bnd + 4;
+  int* flags = bnd + 4;  // [0] stack overflow, [1] proxies with more pairs than slots
   hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, w->stream, bnd);
-  B2G_HIP(hipMemsetAsync(ovf, 0, sizeof(int), w->stream));
-  int gb = std::min(blocks_for(np, kBlock), 1024);
-  hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, np, fat, bnd);
   // world bits above Morton bits: 30 Morton bits for one world, fewer for many
   int wbits = 0;
   while ((1 << wbits) < w->n_worlds) ++wbits;
-  int mbits = std::min(15, (32 - wbits) / 2);
-  int key_bits = wbits + 2 * mbits;
-  hipLaunchKernelGGL(k_morton, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, fat, bnd, w->f.world.p,
-                     mbits, bp.keys.p, bp.vals.p);
-  if ((st = radix_sort_pairs(w, bp.keys.p, bp.vals.p, np, key_bits))) return st;
-  if (np > 1) {
-    B2G_HIP(hipMemsetAsync(bp.node_parent.p, 0xff, sizeof(int) * (2 * np - 1), w->stream));
-    hipLaunchKernelGGL(k_lbvh_build, dim3(blocks_for(np - 1, kBlock)), dim3(kBlock), 0, w->stream, np, bp.keys.p,
-                       2 * mbits, bp.node_child.p, bp.node_worlds.p, bp.node_parent.p);
-    B2G_HIP(hipMemsetAsync(bp.node_flag.p, 0, sizeof(int) * np, w->stream));
+  const int mbits = std::min(15, (32 - wbits) / 2);
+  const int key_bits = wbits + 2 * mbits;
+  if (nt > 0) {
+    int gb = std::min(blocks_for(nt, kBlock), 1024);
+    hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat, bnd);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat,
+                       bnd, w->f.world.p, mbits, bp.keys.p, bp.vals.p);
+    if ((st = radix_sort_pairs(w, bp.keys.p, bp.vals.p, nt, key_bits))) return st;
+    for (int L0 = 0; L0 == 0 || L0 < lv.count - 1; L0 += kReduceLevels)
+      hipLaunchKernelGGL(k_seg_reduce, dim3(blocks_for(lv.size[L0], kBlock)), dim3(kBlock), 0, w->stream, lv, L0,
+                         bp.vals.p, fat, bp.seg.p);
+    if (nt > 1)
+      hipLaunchKernelGGL(k_lbvh_build, dim3(blocks_for(nt - 1, kBlock)), dim3(kBlock), 0, w->stream, nt, bp.keys.p,
+                         2 * mbits, lv, bp.seg.p, bp.node_child.p, bp.node_aabb.p, bp.node_worlds.p);
   }
-  hipLaunchKernelGGL(k_lbvh_refit, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, bp.vals.p, fat,
-                     bp.node_child.p, bp.node_parent.p, bp.node_aabb.p, bp.node_flag.p);
   PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p};
-  const int batched = wbits > 0;
-  hipLaunchKernelGGL(k_traverse<false>, dim3(blocks_for(np, kTravBlock)), dim3(kTravBlock), 0, w->stream, np,
-                     bp.vals.p, fat, bp.node_child.p, bp.node_worlds.p, batched, bp.node_aabb.p, pf, bp.count.p,
-                     nullptr, nullptr, ovf);
+  TreeView tv{nt, bp.vals.p, bp.node_child.p, bp.node_aabb.p, bp.node_worlds.p, bp.seg.p, w->f.loose_off.p,
+              w->f.loose.p, wbits > 0};
+  // emitters walk in Morton order; proxies outside the tree are static and emit nothing
+  const unsigned* qlist = nt > 0 ? bp.vals.p : nullptr;
+  if (nt < np) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
+  if (nt > 0)
+    hipLaunchKernelGGL(k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt,
+                       reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr,
+                       flags);
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
-  int h[2];
+  int h[3];
   B2G_HIP(hipMemcpyAsync(&h[0], dst.off.p + np, sizeof(int), hipMemcpyDeviceToHost, w->stream));
-  B2G_HIP(hipMemcpyAsync(&h[1], ovf, sizeof(int), hipMemcpyDeviceToHost, w->stream));
+  B2G_HIP(hipMemcpyAsync(&h[1], flags, 2 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
   B2G_HIP(hipStreamSynchronize(w->stream));
   if (h[1]) {
-    set_error("broadphase: LBVH traversal stack overflow");
+    set_error("broadphase: BVH traversal stack overflow");
     return B2G_ERR_CAPACITY;
   }
   int total = h[0];
   if ((st = dst.pair.reserve(std::max(total, 1)))) return st;
-  hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(np, kTravBlock)), dim3(kTravBlock), 0, w->stream, np,
-                     bp.vals.p, fat, bp.node_child.p, bp.node_worlds.p, batched, bp.node_aabb.p, pf, nullptr,
-                     dst.off.p, dst.pair.p, ovf);
+  hipLaunchKernelGGL(k_pairs_from_slots, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, bp.count.p,
+                     bp.slots.p, dst.off.p, dst.pair.p);
+  if (h[2] > 0)
+    hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt,
+                       reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, dst.off.p,
+                       dst.pair.p, flags);
   B2G_HIP(hipGetLastError());
   *npairs = total;
   return B2G_OK;

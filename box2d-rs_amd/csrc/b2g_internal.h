@@ -89,6 +89,9 @@ struct ProxyArrays {
   DevArray<float4> fat;     // tree fat AABB
   DevArray<int> moved;      // the tree's moved flag (set when the fat AABB was re-fattened)
   DevArray<int> world;      // world id of the proxy's body (batched worlds never interact)
+  DevArray<int> tree_list;  // proxies kept in the BVH
+  DevArray<int> loose_off;  // per world: range in `loose` of its big static proxies
+  DevArray<int> loose;      // big static proxies (ground, walls) tested directly, not in the BVH
   DevArray<float2> pool;    // shape vertex pool
 };
 
@@ -134,13 +137,13 @@ struct BroadphaseArrays {
   DevArray<unsigned> keys, keys_alt;
   DevArray<unsigned> vals, vals_alt;
   DevArray<unsigned> hist;       // radix histograms
-  DevArray<int> node_parent;     // 2n-1 nodes
-  DevArray<int2> node_child;     // internal nodes
-  DevArray<int2> node_worlds;    // internal nodes: world ids of the first and last leaf below
-  DevArray<float4> node_aabb;    // all nodes
-  DevArray<int> node_flag;
+  DevArray<float4> seg;          // segment tree over the sorted leaf boxes (level 0 = leaves)
+  DevArray<int2> node_child;     // LBVH internal nodes
+  DevArray<float4> node_aabb;
+  DevArray<int2> node_worlds;    // world ids of the first and last leaf below each node
   DevArray<int> count;           // pairs per proxy
-  DevArray<float4> bounds;       // scene bounds scratch
+  DevArray<int> slots;           // first pairs of each proxy, found by the single tree walk
+  DevArray<float4> bounds;       // scene bounds and flags
   DevArray<int> scan_tmp;        // block sums for scans
 };
 
@@ -165,6 +168,9 @@ struct World {
   std::vector<float4> h_fmat, h_ffat, h_faabb;
   std::vector<int2> h_ffilter;
   std::vector<int> h_bworld, h_fworld;  // world id per body / per proxy
+  int n_tree = 0;                       // proxies in the BVH
+  // host sources of asynchronous uploads must outlive the copy: kept here, not on the stack
+  std::vector<int> h_tree, h_loose_off, h_loose, h_color_off_dev;
   int n_worlds = 1;
   std::vector<float2> h_pool;
   std::vector<HostShape> h_shapes;

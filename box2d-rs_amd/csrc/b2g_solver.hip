@@ -133,10 +133,13 @@ __global__ void k_solver_counters_init(int* c) {
   c[3] = 0;
 }
 
+// find with path halving; the halving writes only ever replace a parent by one of its
+// ancestors, so concurrent finds and unions stay consistent
 __device__ inline int uf_find(int* parent, int x) {
   int p = parent[x];
   while (p != x) {
     int g = parent[p];
+    if (g != p) parent[x] = g;
     x = p;
     p = g;
   }
@@ -221,19 +224,39 @@ __global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ 
   atomicSub(remaining, 1);
 }
 
+// Colour histogram and scatter with block-aggregated atomics: ranks inside a block come from
+// LDS atomics and each block reserves one range per colour, so the ~10 global counters see one
+// atomic per block instead of one per constraint. The order inside a colour is free (its
+// constraints are independent), so no stable ranking is needed.
 __global__ void __launch_bounds__(kBlock) k_color_hist(const int* __restrict__ ncp, const int* __restrict__ state,
                                                        int* __restrict__ count) {
+  __shared__ int h[kMaxColors];
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) h[c] = 0;
+  __syncthreads();
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= *ncp) return;
-  atomicAdd(&count[state[j] & 0xff], 1);
+  if (j < *ncp) atomicAdd(&h[state[j] & 0xff], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
+    if (h[c]) atomicAdd(&count[c], h[c]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __restrict__ state, const int* __restrict__ off,
                                                           int* __restrict__ cursor, int* __restrict__ order) {
+  __shared__ int h[kMaxColors];
+  __shared__ int base[kMaxColors];
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) h[c] = 0;
+  __syncthreads();
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int c = state[j] & 0xff;
-  order[off[c] + atomicAdd(&cursor[c], 1)] = j;
+  int c = -1, r = 0;
+  if (j < n) {
+    c = state[j] & 0xff;
+    r = atomicAdd(&h[c], 1);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kMaxColors; k += blockDim.x)
+    if (h[k]) base[k] = off[k] + atomicAdd(&cursor[k], h[k]);
+  __syncthreads();
+  if (c >= 0) order[base[c] + r] = j;
 }
 
 // ---- constraint setup ---------------------------------------------------------------------------
@@ -574,13 +597,16 @@ struct PosArgs {
 // (src/private/dynamics/b2_contact_solver_private.rs:600-730)
 __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, PosArgs a) {
   int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= end) return;
-  int4 info = a.info[i];
-  if (a.done[info.w]) return;
-  int2 bd = a.bodies[i];
-  float4 m = a.vm[i];
-  float4 lp = a.plp[i], g = a.pgeo[i], lc = a.plc[i];
-  float2 rad = a.prad[i];
+  // inactive lanes stay in the wave (minSep = 0) for the island reduction at the end
+  const bool live = i < end;
+  int4 info = live ? a.info[i] : make_int4(0, 0, 0, 0);
+  const bool active = live && !a.done[info.w];
+  if (!active) info.y = 0;
+  int2 bd = active ? a.bodies[i] : make_int2(0, 0);
+  const int ii = active ? i : begin;
+  float4 m = a.vm[ii];
+  float4 lp = a.plp[ii], g = a.pgeo[ii], lc = a.plc[ii];
+  float2 rad = a.prad[ii];
   const float mA = m.x, iA = m.y, mB = m.z, iB = m.w;
   V2 lcA = v2(lc.x, lc.y), lcB = v2(lc.z, lc.w);
   float4 PA = a.pos[bd.x], PB = a.pos[bd.y];
@@ -632,9 +658,24 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
     cB = cB + mB * P;
     aB += iB * cross(rB, P);
   }
-  if (info.x & 0x100) a.pos[bd.x] = make_float4(cA.x, cA.y, aA, 0.0f);
-  if (info.x & 0x200) a.pos[bd.y] = make_float4(cB.x, cB.y, aB, 0.0f);
-  if (minSep < 0.0f) atomicMin(&a.minsep[info.w], float_flip(minSep));
+  if (active && (info.x & 0x100)) a.pos[bd.x] = make_float4(cA.x, cA.y, aA, 0.0f);
+  if (active && (info.x & 0x200)) a.pos[bd.y] = make_float4(cB.x, cB.y, aB, 0.0f);
+  // Island minimum separation. A pile is one island, so a plain per-thread atomic would
+  // serialise the whole colour on one address: lanes of a wave first reduce per island, and a
+  // lane skips the atomic when the stored minimum is already lower (it only decreases, so a
+  // stale read can only cause an unneeded atomic, never a missed one).
+  int key = float_flip(minSep);
+  const int isl = info.w;
+  unsigned long long pending = __ballot(minSep < 0.0f);
+  while (pending) {
+    const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+    const int lisl = __shfl(isl, leader, 64);
+    const bool mine = (pending >> (threadIdx.x & 63)) & 1ull && isl == lisl;
+    int v = mine ? key : 0x7fffffff;
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    if ((threadIdx.x & 63) == leader && v < a.minsep[lisl]) atomicMin(&a.minsep[lisl], v);
+    pending &= ~__ballot(mine);
+  }
 }
 
 // end of a position iteration: islands whose error is small stop iterating
@@ -744,7 +785,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       ncolors = i + 1;
       w->h_color_off.push_back(w->h_color_off.back() + hc[i]);
     }
-    std::vector<int> offs(kMaxColors + 1, nc);
+    std::vector<int>& offs = w->h_color_off_dev;  // outlives the asynchronous copy
+    offs.assign(kMaxColors + 1, nc);
     for (int i = 0; i <= ncolors; ++i) offs[i] = w->h_color_off[i];
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * (kMaxColors + 1), hipMemcpyHostToDevice,
                            w->stream));

@@ -24,3 +24,14 @@ def oracle():
 def oracle64():
     import b2o
     return b2o.Oracle("f64")
+
+
+@pytest.hookimpl(hookwrapper=True)
+def pytest_runtest_makereport(item, call):
+    """A device error (status 3: a fault, a lost device) leaves the GPU unusable for the rest of
+    the process: end the session at the first one, naming the test, instead of running the
+    remaining GPU tests against a dead device."""
+    outcome = yield
+    rep = outcome.get_result()
+    if rep.failed and call.excinfo is not None and getattr(call.excinfo.value, "status", None) == 3:
+        pytest.exit("device error in %s: %s" % (item.nodeid, call.excinfo.value), returncode=3)

@@ -100,8 +100,9 @@ def solver_replay_check(oracle, scene, warm_steps, tol=1e-5):
             "order": order_c, "offsets": offs, "contact_bodies": cb, "touch": touch}
 
 
-@pytest.mark.parametrize("warm_steps", [0, 1, 30])
+@pytest.mark.parametrize("warm_steps", [20, 45, 90])
 def test_solver_replay_pyramid(oracle, warm_steps):
+    """The pyramid's rows start 0.25 m apart: the first contacts touch at about step 10."""
     from b2g import scenes
     info = solver_replay_check(oracle, scenes.pyramid(), warm_steps)
     assert info["colours"] >= 2
@@ -149,7 +150,7 @@ def test_linear_momentum_conserved_without_gravity():
     total linear momentum is conserved to f32 round-off."""
     import b2g
     from b2g import scenes
-    sc = scenes.heap_world(900, seed=8, half_extent=200.0)  # walls far away
+    sc = scenes.large_world(n=900, seed=8)  # a 36 m grid in the middle of a 400 m box
     w = b2g.World(gravity=(0.0, 0.0))
     w.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
     mass = w.mass()[:, 0].astype(np.float64)
@@ -184,26 +185,33 @@ def _max_penetration(oracle, scene, states, fab, flags, man):
 
 def test_pyramid_600_step_band(oracle):
     """Pyramid, 600 steps: the GPU (coloured order) against the oracle (reference DFS order).
-    Band: max penetration <= oracle max + 0.01 m and <= 3x linear slop + 0.01; the top box ends
-    within 0.05 m of the oracle's; both end at rest (max speed < 0.05 m/s)."""
+    Band: over the whole run (including the drop of the rows) the GPU's max penetration stays
+    within 1.5x the oracle's + 0.01 m; once settled (steps 300-600) within the oracle's + 0.005 m
+    and below 3x linear slop + 0.005 m; the top box ends within 0.05 m of the oracle's; both end
+    at rest (max speed < 0.05 m/s)."""
     import b2g
     from b2g import scenes
     sc = scenes.pyramid()
     w = _world(sc)
     ow = oracle.world()
     ow.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
-    pen_g, pen_o = 0.0, 0.0
+    pen = {"g": [0.0, 0.0], "o": [0.0, 0.0]}  # [whole run, settled]
     for i in range(600):
         w.step(DT, 8, 3)
         ow.step(DT, 8, 3)
-        if i % 20 == 19:
+        if i % 10 == 9:
             fab, flags, man = w.contacts()
-            pen_g = max(pen_g, _max_penetration(oracle, sc, w.body_states(), fab, flags, man))
+            pg = _max_penetration(oracle, sc, w.body_states(), fab, flags, man)
             ofab, oflags, oman, _ = ow.contacts()
-            pen_o = max(pen_o, _max_penetration(oracle, sc, ow.bodies(), ofab, oflags, oman))
+            po = _max_penetration(oracle, sc, ow.bodies(), ofab, oflags, oman)
+            for key, val in (("g", pg), ("o", po)):
+                pen[key][0] = max(pen[key][0], val)
+                if i >= 300:
+                    pen[key][1] = max(pen[key][1], val)
     sg, so = w.body_states(), ow.bodies()
-    assert pen_g <= pen_o + 0.01
-    assert pen_g <= 3 * 0.005 + 0.01
+    assert pen["g"][0] <= 1.5 * pen["o"][0] + 0.01
+    assert pen["g"][1] <= pen["o"][1] + 0.005
+    assert pen["g"][1] <= 3 * 0.005 + 0.005
     assert abs(float(sg[-1]["c"][1]) - float(so[-1]["c"][1])) < 0.05
     assert float(np.abs(sg["v"][1:]).max()) < 0.05
     assert float(np.abs(so["v"][1:]).max()) < 0.05
