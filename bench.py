@@ -19,6 +19,16 @@ sys.path.insert(0, os.path.join(ROOT, "box2d-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# Algorithmic HBM bytes per item of the timed kernels (the step's byte model: f32/i32 sizes, each
+# structure read or written once per launch). A velocity iteration moves 224 B per constraint:
+# with this library's layout a two-point constraint reads a 160 B record (bodies, info, masses,
+# normal/friction, impulses, effective masses, two anchor pairs, bias, block K and its inverse),
+# gathers and scatters two body velocities (64 B) and writes its impulses (16 B), 240 B in all; a
+# one-point constraint moves 192 B. A position iteration moves 136 B per constraint.
+UNIT_BYTES = {"solve_velocity": 224, "solve_position": 136, "warm_start": 128, "init_constraints": 404}
+UNIT_NAME = {"solve_velocity": "constraint-iteration", "solve_position": "constraint-iteration",
+             "warm_start": "constraint", "init_constraints": "constraint"}
+
 
 def cpu_baseline(scene, seconds=15.0, max_steps=200):
     """The oracle (sequential C++ restatement of the reference step, 1 thread) on the same
@@ -38,6 +48,16 @@ def cpu_baseline(scene, seconds=15.0, max_steps=200):
                 steps, n, platform.processor() or platform.machine())}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary for this workload (calibrated against a copy kernel), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_%s_large_world_65536.json" % kernel)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +65,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-kernel", default="solve_velocity", choices=sorted(UNIT_BYTES))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single world or a batch of worlds is stepped on one "
@@ -61,12 +82,16 @@ def main():
     for _ in range(args.warmup):
         w.step(dt, 8, 3)
     w.body_states()  # synchronise
-    prof_acc = {"collide": 0.0, "solve": 0.0, "broadphase": 0.0, "pairs": 0, "touching": 0, "colours": 0}
+    # HIP events around every launch of the dominant kernel on the library's stream, over the
+    # timed region
+    w.time_kernel(args.roofline_kernel)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         w.step(dt, 8, 3)
     w.body_states()
     t1 = time.perf_counter()
+    k_ms, k_launches, k_items = w.kernel_time()
+    w.time_kernel("none")
     p = w.profile()
     elapsed = t1 - t0
     n = int(scene.meta["dynamic"])
@@ -88,6 +113,15 @@ def main():
                                "warm start on, sleep and continuous off" % n,
                    "bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}},
     }
+    bytes_per_launch = UNIT_BYTES[args.roofline_kernel] * k_items / max(k_launches, 1)
+    avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.roofline_kernel),
+                       "kernel": "k_" + args.roofline_kernel, "launches": k_launches,
+                       "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
+                       "unit_bytes": UNIT_BYTES[args.roofline_kernel],
+                       "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[args.roofline_kernel]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(out))

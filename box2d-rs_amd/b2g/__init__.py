@@ -25,6 +25,9 @@ OK, ERR_INVALID_ARGUMENT, ERR_CAPACITY, ERR_DEVICE, ERR_DEGENERATE, ERR_UNSUPPOR
 B2_STATIC_BODY, B2_KINEMATIC_BODY, B2_DYNAMIC_BODY = 0, 1, 2
 SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX = 0, 1, 2, 4
 PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
+# kernels b2g_world_time_kernel can time (B2G_KERNEL_*)
+KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2, "init_constraints": 3,
+           "traverse": 4, "collide": 5, "color_round": 6}
 
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
@@ -90,6 +93,8 @@ def _load():
         "b2g_world_get_profile": ([P, P], I),
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
+        "b2g_world_time_kernel": ([P, I], I),
+        "b2g_world_kernel_time": ([P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -243,6 +248,17 @@ class World:
         p = _Profile()
         _check(_lib.b2g_world_get_profile(self._h, ctypes.byref(p)))
         return {name: getattr(p, name) for name, _ in _Profile._fields_ if name != "pad"}
+
+    def time_kernel(self, name):
+        """Record HIP events around every launch of one kernel (a KERNELS name; "none" stops)."""
+        _check(_lib.b2g_world_time_kernel(self._h, KERNELS[name]))
+
+    def kernel_time(self):
+        """(device ms summed over launches, launches, items) of the timed kernel since the last
+        call; resets the totals."""
+        ms, launches, items = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.b2g_world_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(items)))
+        return ms.value, launches.value, items.value
 
     def collide_pairs(self, pairs, xf_a, xf_b):
         pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)

@@ -384,6 +384,22 @@ static int run_phase(World* w, int phase, float dt, int vi, int pi) {
   return B2G_ERR_INVALID_ARGUMENT;
 }
 
+int kt_harvest(World* w) {
+  KernelTimer& t = w->kt;
+  const size_t n = t.pending_items.size();
+  if (n == 0) return B2G_OK;
+  B2G_HIP(hipEventSynchronize(t.ev[2 * n - 1]));
+  for (size_t i = 0; i < n; ++i) {
+    float ms = 0.0f;
+    B2G_HIP(hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]));
+    t.ms += ms;
+    t.items += t.pending_items[i];
+  }
+  t.launches += static_cast<long long>(n);
+  t.pending_items.clear();
+  return B2G_OK;
+}
+
 }  // namespace b2g
 
 using namespace b2g;
@@ -440,6 +456,7 @@ int b2g_world_destroy(b2g_world* h) {
   if (w->h_counters) (void)hipHostFree(w->h_counters);
   for (int i = 0; i < 8; ++i)
     if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+  for (hipEvent_t e : w->kt.ev) (void)hipEventDestroy(e);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete h;
   return B2G_OK;
@@ -692,6 +709,36 @@ int b2g_world_get_profile(b2g_world* h, b2g_profile* out) {
   out->touching = w->n_touch;
   out->colours = w->n_colors;
   out->position_iterations = w->pos_iters_run;
+  return B2G_OK;
+}
+
+int b2g_world_time_kernel(b2g_world* h, int32_t kernel) {
+  if (!h || kernel < B2G_KERNEL_NONE || kernel >= B2G_KERNEL_COUNT) {
+    set_error("b2g_world_time_kernel: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = kt_harvest(w))) return st;
+  w->kt.kernel = kernel;
+  w->kt.ms = 0.0;
+  w->kt.launches = w->kt.items = 0;
+  return B2G_OK;
+}
+
+int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* items) {
+  if (!h || !ms || !launches || !items) {
+    set_error("b2g_world_kernel_time: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = kt_harvest(w))) return st;
+  *ms = w->kt.ms;
+  *launches = w->kt.launches;
+  *items = w->kt.items;
+  w->kt.ms = 0.0;
+  w->kt.launches = w->kt.items = 0;
   return B2G_OK;
 }
 

@@ -440,15 +440,19 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   // emitters walk in Morton order; proxies outside the tree are static and emit nothing
   const unsigned* qlist = nt > 0 ? bp.vals.p : nullptr;
   if (nt < np) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
-  if (nt > 0)
+  if (nt > 0) {
+    kt_begin(w, B2G_KERNEL_TRAVERSE);
     hipLaunchKernelGGL(k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt,
                        reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr,
                        flags);
+    kt_end(w, B2G_KERNEL_TRAVERSE, nt);
+  }
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
   int h[3];
   B2G_HIP(hipMemcpyAsync(&h[0], dst.off.p + np, sizeof(int), hipMemcpyDeviceToHost, w->stream));
   B2G_HIP(hipMemcpyAsync(&h[1], flags, 2 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
   B2G_HIP(hipStreamSynchronize(w->stream));
+  if ((st = kt_harvest(w))) return st;
   if (h[1]) {
     set_error("broadphase: BVH traversal stack overflow");
     return B2G_ERR_CAPACITY;

@@ -151,6 +151,16 @@ struct StepProfile {
   float collide_ms = 0, solve_ms = 0, broadphase_ms = 0, step_ms = 0;
 };
 
+// Event timing of every launch of one kernel (B2G_KERNEL_*): start/end event pairs are reused
+// and harvested wherever the stream has been synchronised.
+struct KernelTimer {
+  int kernel = B2G_KERNEL_NONE;
+  std::vector<hipEvent_t> ev;          // 2 per pending launch
+  std::vector<long long> pending_items;
+  double ms = 0.0;
+  long long launches = 0, items = 0;
+};
+
 struct World {
   // settings
   V2 gravity;
@@ -199,7 +209,31 @@ struct World {
   int pos_iters_run = 0;
   StepProfile prof;
   hipEvent_t ev[8] = {};
+  KernelTimer kt;
 };
+
+// kernel timer hooks: kt_begin before a launch, kt_end after it, kt_harvest once the stream has
+// been synchronised (it also waits for the last recorded event itself)
+inline void kt_begin(World* w, int kernel) {
+  KernelTimer& t = w->kt;
+  if (t.kernel != kernel) return;
+  size_t k = 2 * t.pending_items.size();
+  while (t.ev.size() < k + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    t.ev.push_back(e);
+  }
+  (void)hipEventRecord(t.ev[k], w->stream);
+}
+inline void kt_end(World* w, int kernel, long long items) {
+  KernelTimer& t = w->kt;
+  if (t.kernel != kernel) return;
+  size_t k = 2 * t.pending_items.size();
+  if (t.ev.size() < k + 2) return;  // event creation failed in kt_begin
+  (void)hipEventRecord(t.ev[k + 1], w->stream);
+  t.pending_items.push_back(items);
+}
+int kt_harvest(World* w);
 
 // phase drivers (each returns a B2G_* status)
 int upload_world(World* w);

@@ -4,6 +4,11 @@
 // library is built with -ffp-contract=off, so products and sums round exactly as the
 // reference's f32 code does. The header compiles as HIP device code and as plain C++ (the
 // B2G_HD macro), which lets the host test build run the same collision code on the CPU.
+//
+// The vector types are templates over the scalar: float for the simulation kernels, double for
+// host checks, and the forward-mode Dual of b2g_dual.h for the adjoint's local Jacobians. Generic
+// code writes its constants as T(double literal), which for T = float is the same value as the
+// float literal of the reference.
 #pragma once
 #include <math.h>
 #include <stdint.h>
@@ -29,41 +34,79 @@ constexpr float kMaxLinearCorrection = 0.2f;
 constexpr float kMaxTranslation = 2.0f;
 constexpr float kMaxRotation = 0.5f * kPi;
 constexpr float kBaumgarte = 0.2f;
+// the same settings as exact decimal values, converted to the generic scalar T at use
+constexpr double kLinearSlopD = 0.005;
+constexpr double kMaxLinearCorrectionD = 0.2;
+constexpr double kBaumgarteD = 0.2;
+constexpr double kMaxTranslationD = 2.0;
+constexpr double kPiD = 3.14159265358979323846;
 
-struct V2 {
-  float x, y;
+// Generic scalar helpers: float/double use the C library; other scalar types (Dual) overload
+// tsqrt/tsin/tcos/primal in their own header.
+B2G_HD float tsqrt(float x) { return sqrtf(x); }
+B2G_HD double tsqrt(double x) { return sqrt(x); }
+B2G_HD float primal(float x) { return x; }
+B2G_HD double primal(double x) { return x; }
+
+template <class T>
+struct TV2 {
+  T x, y;
 };
-B2G_HD V2 v2(float x, float y) { V2 r; r.x = x; r.y = y; return r; }
-B2G_HD V2 operator+(V2 a, V2 b) { return v2(a.x + b.x, a.y + b.y); }
-B2G_HD V2 operator-(V2 a, V2 b) { return v2(a.x - b.x, a.y - b.y); }
-B2G_HD V2 operator-(V2 a) { return v2(-a.x, -a.y); }
-B2G_HD V2 operator*(float s, V2 a) { return v2(s * a.x, s * a.y); }
-B2G_HD float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
-B2G_HD float cross(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
-B2G_HD V2 cross(V2 a, float s) { return v2(s * a.y, -s * a.x); }
-B2G_HD V2 cross(float s, V2 a) { return v2(-s * a.y, s * a.x); }
-B2G_HD float len(V2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
-B2G_HD float dist_sq(V2 a, V2 b) { V2 c = a - b; return dot(c, c); }
+using V2 = TV2<float>;
+
+template <class T>
+B2G_HD TV2<T> v2(T x, T y) { TV2<T> r; r.x = x; r.y = y; return r; }
+// converts a vector of another scalar type (a float shape vertex into the generic scalar)
+template <class T, class U>
+B2G_HD TV2<T> tv(TV2<U> a) { return v2(T(a.x), T(a.y)); }
+template <class T>
+B2G_HD TV2<T> operator+(TV2<T> a, TV2<T> b) { return v2(a.x + b.x, a.y + b.y); }
+template <class T>
+B2G_HD TV2<T> operator-(TV2<T> a, TV2<T> b) { return v2(a.x - b.x, a.y - b.y); }
+template <class T>
+B2G_HD TV2<T> operator-(TV2<T> a) { return v2(-a.x, -a.y); }
+template <class T>
+B2G_HD TV2<T> operator*(T s, TV2<T> a) { return v2(s * a.x, s * a.y); }
+template <class T>
+B2G_HD T dot(TV2<T> a, TV2<T> b) { return a.x * b.x + a.y * b.y; }
+template <class T>
+B2G_HD T cross(TV2<T> a, TV2<T> b) { return a.x * b.y - a.y * b.x; }
+template <class T>
+B2G_HD TV2<T> cross(TV2<T> a, T s) { return v2(s * a.y, -s * a.x); }
+template <class T>
+B2G_HD TV2<T> cross(T s, TV2<T> a) { return v2(-s * a.y, s * a.x); }
+template <class T>
+B2G_HD T len(TV2<T> a) { return tsqrt(a.x * a.x + a.y * a.y); }
+template <class T>
+B2G_HD T dist_sq(TV2<T> a, TV2<T> b) { TV2<T> c = a - b; return dot(c, c); }
 // b2_min/b2_max/b2_clamp exactly as src/b2_math.rs:704-731
-B2G_HD float bmin(float a, float b) { return a < b ? a : b; }
-B2G_HD float bmax(float a, float b) { return a > b ? a : b; }
-B2G_HD float bclamp(float a, float lo, float hi) { return bmax(lo, bmin(a, hi)); }
-B2G_HD V2 vmin(V2 a, V2 b) { return v2(bmin(a.x, b.x), bmin(a.y, b.y)); }
-B2G_HD V2 vmax(V2 a, V2 b) { return v2(bmax(a.x, b.x), bmax(a.y, b.y)); }
+template <class T>
+B2G_HD T bmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+B2G_HD T bmax(T a, T b) { return a > b ? a : b; }
+template <class T>
+B2G_HD T bclamp(T a, T lo, T hi) { return bmax(lo, bmin(a, hi)); }
+template <class T>
+B2G_HD TV2<T> vmin(TV2<T> a, TV2<T> b) { return v2(bmin(a.x, b.x), bmin(a.y, b.y)); }
+template <class T>
+B2G_HD TV2<T> vmax(TV2<T> a, TV2<T> b) { return v2(bmax(a.x, b.x), bmax(a.y, b.y)); }
 
 // B2vec2::normalize (src/b2_math.rs:82-92)
-B2G_HD float normalize(V2* v) {
-  float l = len(*v);
-  if (l < kEpsilon) return 0.0f;
-  float inv = 1.0f / l;
+template <class T>
+B2G_HD T normalize(TV2<T>* v) {
+  T l = len(*v);
+  if (l < T(kEpsilon)) return T(0.0);
+  T inv = T(1.0) / l;
   v->x *= inv;
   v->y *= inv;
   return l;
 }
 
-struct Rot {
-  float s, c;
+template <class T>
+struct TRot {
+  T s, c;
 };
+using Rot = TRot<float>;
 // B2Rot::set (src/b2_math.rs:372-376) is f32::sin / f32::cos, i.e. glibc sinf/cosf, which
 // evaluate in double precision and round once to float. The device does the same: double
 // sin/cos, one rounding; this matches glibc except when the double result lies within a
@@ -81,26 +124,46 @@ B2G_HD Rot rot(float angle) {
 #endif
   return q;
 }
-struct Xf {
-  V2 p;
-  Rot q;
+B2G_HD TRot<double> rot(double angle) {
+  TRot<double> q;
+  q.s = sin(angle);
+  q.c = cos(angle);
+  return q;
+}
+
+template <class T>
+struct TXf {
+  TV2<T> p;
+  TRot<T> q;
 };
+using Xf = TXf<float>;
 
 // src/b2_math.rs:611-680
-B2G_HD V2 mul(Rot q, V2 v) { return v2(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }
-B2G_HD V2 mulT(Rot q, V2 v) { return v2(q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y); }
-B2G_HD Rot mulT(Rot q, Rot r) { Rot o; o.s = q.c * r.s - q.s * r.c; o.c = q.c * r.c + q.s * r.s; return o; }
-B2G_HD V2 mul(Xf t, V2 v) {
-  float x = (t.q.c * v.x - t.q.s * v.y) + t.p.x;
-  float y = (t.q.s * v.x + t.q.c * v.y) + t.p.y;
+template <class T>
+B2G_HD TV2<T> mul(TRot<T> q, TV2<T> v) { return v2(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }
+template <class T>
+B2G_HD TV2<T> mulT(TRot<T> q, TV2<T> v) { return v2(q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y); }
+template <class T>
+B2G_HD TRot<T> mulT(TRot<T> q, TRot<T> r) {
+  TRot<T> o;
+  o.s = q.c * r.s - q.s * r.c;
+  o.c = q.c * r.c + q.s * r.s;
+  return o;
+}
+template <class T>
+B2G_HD TV2<T> mul(TXf<T> t, TV2<T> v) {
+  T x = (t.q.c * v.x - t.q.s * v.y) + t.p.x;
+  T y = (t.q.s * v.x + t.q.c * v.y) + t.p.y;
   return v2(x, y);
 }
-B2G_HD V2 mulT(Xf t, V2 v) {
-  float px = v.x - t.p.x, py = v.y - t.p.y;
+template <class T>
+B2G_HD TV2<T> mulT(TXf<T> t, TV2<T> v) {
+  T px = v.x - t.p.x, py = v.y - t.p.y;
   return v2(t.q.c * px + t.q.s * py, -t.q.s * px + t.q.c * py);
 }
-B2G_HD Xf mulT(Xf a, Xf b) {
-  Xf c;
+template <class T>
+B2G_HD TXf<T> mulT(TXf<T> a, TXf<T> b) {
+  TXf<T> c;
   c.q = mulT(a.q, b.q);
   c.p = mulT(a.q, b.p - a.p);
   return c;

@@ -196,6 +196,28 @@ int b2g_collide_pairs(b2g_world* world, int32_t n, const int32_t* pairs, const f
  * world's filters. Writes at most cap pairs (lower, higher fixture index) and the total count. */
 int b2g_broadphase_pairs(b2g_world* world, const float* fat_aabbs, int64_t cap, int32_t* pairs, int64_t* count);
 
+/* Kernels that b2g_world_time_kernel can time, and the item each launch counts. */
+enum {
+  B2G_KERNEL_NONE = -1,
+  B2G_KERNEL_SOLVE_VELOCITY = 0,  /* constraints of one colour, one velocity iteration */
+  B2G_KERNEL_SOLVE_POSITION = 1,  /* constraints of one colour, one position iteration */
+  B2G_KERNEL_WARM_START = 2,      /* constraints of one colour */
+  B2G_KERNEL_INIT_CONSTRAINTS = 3,/* touching contacts */
+  B2G_KERNEL_TRAVERSE = 4,        /* proxies in the tree */
+  B2G_KERNEL_COLLIDE = 5,         /* contacts (candidate pairs) */
+  B2G_KERNEL_COLOR_ROUND = 6,     /* candidate pairs (one colouring round) */
+  B2G_KERNEL_COUNT = 7
+};
+
+/* Measurement hook with no counterpart in the reference (its B2Profile times whole phases on
+ * the host): records a pair of HIP events on the world's stream around every launch of one
+ * kernel (B2G_KERNEL_NONE switches it off). The events add a little time to each timed launch's
+ * neighbours, not to its own duration. */
+int b2g_world_time_kernel(b2g_world* world, int32_t kernel);
+/* Totals of the timed kernel since the last call (which resets them): device milliseconds
+ * summed over launches, number of launches and items processed. Waits for the stream. */
+int b2g_world_kernel_time(b2g_world* world, double* ms, int64_t* launches, int64_t* items);
+
 #ifdef __cplusplus
 }
 #endif
