@@ -94,6 +94,13 @@ def _load():
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
         "b2g_world_time_kernel": ([P, I], I),
+        "b2g_tape_begin": ([P, I], I),
+        "b2g_tape_end": ([P], I),
+        "b2g_tape_free": ([P], I),
+        "b2g_tape_info": ([P, P, P], I),
+        "b2g_tape_replay": ([P, P], I),
+        "b2g_tape_backward": ([P, P, P, P], I),
+        "b2g_tape_export": ([P, I, I, P, P, P, I, P, P, P], I),
         "b2g_world_kernel_time": ([P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
@@ -259,6 +266,52 @@ class World:
         ms, launches, items = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
         _check(_lib.b2g_world_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(items)))
         return ms.value, launches.value, items.value
+
+    # ---- adjoint -------------------------------------------------------------------------
+    def tape_begin(self, checkpoint_every=16):
+        """Record the following steps for tape_backward (body-state checkpoints every
+        checkpoint_every steps)."""
+        _check(_lib.b2g_tape_begin(self._h, int(checkpoint_every)))
+
+    def tape_end(self):
+        _check(_lib.b2g_tape_end(self._h))
+
+    def tape_free(self):
+        _check(_lib.b2g_tape_free(self._h))
+
+    def tape_info(self):
+        steps, slots = ctypes.c_int32(), ctypes.c_int64()
+        _check(_lib.b2g_tape_info(self._h, ctypes.byref(steps), ctypes.byref(slots)))
+        return steps.value, slots.value
+
+    def tape_replay(self):
+        """Final body state (n, 6: c.x, c.y, a, v.x, v.y, w) of the frozen replay of the tape."""
+        out = np.zeros((self.body_count, 6), np.float32)
+        _check(_lib.b2g_tape_replay(self._h, _ptr(out)))
+        return out
+
+    def tape_backward(self, cotangent):
+        """Gradients of a loss with respect to the body state at the start of the recorded run
+        and to the fixture frictions, given dL/d(final state) as an (n, 6) array."""
+        cot = np.ascontiguousarray(cotangent, np.float32).reshape(self.body_count, 6)
+        g = np.zeros_like(cot)
+        gf = np.zeros(max(self.fixture_count, 1), np.float32)
+        _check(_lib.b2g_tape_backward(self._h, _ptr(cot), _ptr(g), _ptr(gf)))
+        return g, gf[:self.fixture_count]
+
+    def tape_export(self, step):
+        """One recorded step: slots (n, 10) int32, impulses (n, 8) float32, colour offsets and
+        step parameters (dt, dt_ratio, velocity/position iterations, warm, block)."""
+        n, nc = ctypes.c_int32(), ctypes.c_int32()
+        params = np.zeros(6, np.float32)
+        _check(_lib.b2g_tape_export(self._h, int(step), 0, None, None, None, 0, ctypes.byref(n), ctypes.byref(nc),
+                                    _ptr(params)))
+        slots = np.zeros((max(n.value, 1), 10), np.int32)
+        imp = np.zeros((max(n.value, 1), 8), np.float32)
+        offs = np.zeros(nc.value + 1, np.int32)
+        _check(_lib.b2g_tape_export(self._h, int(step), n.value, _ptr(slots), _ptr(imp), _ptr(offs), len(offs),
+                                    ctypes.byref(n), ctypes.byref(nc), _ptr(params)))
+        return slots[:n.value], imp[:n.value], offs, params
 
     def collide_pairs(self, pairs, xf_a, xf_b):
         pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)

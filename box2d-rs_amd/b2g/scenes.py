@@ -275,6 +275,28 @@ def batched_worlds(n_worlds=4096, config=3, lo=16, hi=1024):
     return BatchScene(bd, np.concatenate(worlds), fb, sd, fd, meta)
 
 
+def differentiable_worlds(n_worlds=1024, n=64, config=4):
+    """n_worlds worlds of n boxes and circles in a 20 x 20 m box, per-body friction U[0.2, 1.0],
+    and one goal point per world U([-8, 8]^2) for the loss |c_body0 - goal|^2 (body 0 = the
+    world's first dynamic body)."""
+    parts, worlds, goals, first = [], [], [], []
+    nb = 0
+    for wi in range(n_worlds):
+        seed = world_seed(config, wi)
+        u = uniform(seed ^ 0xF00D, np.arange(n + 2, dtype=np.uint64))
+        sc = heap_world(n, seed, half_extent=10.0, boxes_and_circles_only=True,
+                        friction=(0.2 + 0.8 * u[:n]).astype(np.float32))
+        parts.append(sc[:4])
+        worlds.append(np.full(len(sc.body_defs), wi, np.int32))
+        first.append(nb + int(np.nonzero(sc.body_defs["type"] == B2_DYNAMIC_BODY)[0][0]))
+        nb += len(sc.body_defs)
+        goals.append(-8.0 + 16.0 * u[n:n + 2])
+    bd, fb, sd, fd = concat(*parts)
+    meta = {"name": "differentiable_worlds", "worlds": n_worlds, "dynamic": n_worlds * n, "config": config,
+            "sizes": [n] * n_worlds, "goal": np.array(goals, np.float64), "body0": np.array(first, np.int64)}
+    return BatchScene(bd, np.concatenate(worlds), fb, sd, fd, meta)
+
+
 def world_of(batch, wi):
     """The single-world Scene of world wi of a batch (for per-world references)."""
     bsel = np.nonzero(batch.body_world == wi)[0]

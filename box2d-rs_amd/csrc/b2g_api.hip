@@ -457,6 +457,7 @@ int b2g_world_destroy(b2g_world* h) {
   for (int i = 0; i < 8; ++i)
     if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
   for (hipEvent_t e : w->kt.ev) (void)hipEventDestroy(e);
+  tape_release(w);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete h;
   return B2G_OK;
@@ -512,10 +513,22 @@ int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {
   if (!h || vi < 0 || pi < 0) { set_error("b2g_world_step: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;
   int st;
+  const bool taping = w->tape.active;
+  if (taping) {
+    if (dt <= 0.0f || w->dirty || w->need_broadphase) {
+      tape_release(w);
+      set_error("b2g_world_step: a recording tape needs dt > 0 and no bodies added while it is open");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+    if ((st = tape_step_begin(w))) return st;
+  }
   (void)hipEventRecord(w->ev[0], w->stream);
   if ((st = run_phase(w, B2G_PHASE_COLLIDE, dt, vi, pi))) return st;
+  if (taping && (st = tape_after_collide(w))) return st;
   (void)hipEventRecord(w->ev[1], w->stream);
+  const float dt_ratio = w->inv_dt0 * dt;
   if ((st = run_phase(w, B2G_PHASE_SOLVE, dt, vi, pi))) return st;
+  if (taping && (st = tape_after_solve(w, dt, dt_ratio, vi, pi))) return st;
   (void)hipEventRecord(w->ev[2], w->stream);
   if ((st = run_phase(w, B2G_PHASE_FIND, dt, vi, pi))) return st;
   (void)hipEventRecord(w->ev[3], w->stream);
@@ -525,6 +538,10 @@ int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {
 
 int b2g_world_step_phase(b2g_world* h, int32_t phase, float dt, int32_t vi, int32_t pi) {
   if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  if (h->w.tape.active) {
+    set_error("b2g_world_step_phase: close the recording tape first (b2g_tape_end)");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
   int st = run_phase(&h->w, phase, dt, vi, pi);
   if (st == B2G_OK && hipStreamSynchronize(h->w.stream) != hipSuccess) {
     set_error("device error");
@@ -739,6 +756,115 @@ int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* 
   *items = w->kt.items;
   w->kt.ms = 0.0;
   w->kt.launches = w->kt.items = 0;
+  return B2G_OK;
+}
+
+int b2g_tape_begin(b2g_world* h, int32_t checkpoint_every) {
+  if (!h || checkpoint_every < 1) { set_error("b2g_tape_begin: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  tape_release(w);
+  w->tape.active = true;
+  w->tape.every = checkpoint_every;
+  w->tape.nb = w->nb;
+  return B2G_OK;
+}
+
+int b2g_tape_end(b2g_world* h) {
+  if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  h->w.tape.active = false;
+  B2G_HIP(hipStreamSynchronize(h->w.stream));
+  return B2G_OK;
+}
+
+int b2g_tape_free(b2g_world* h) {
+  if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  B2G_HIP(hipStreamSynchronize(h->w.stream));
+  tape_release(&h->w);
+  return B2G_OK;
+}
+
+int b2g_tape_info(b2g_world* h, int32_t* steps, int64_t* slots) {
+  if (!h || !steps || !slots) { set_error("b2g_tape_info: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  *steps = int32_t(h->w.tape.steps.size());
+  *slots = h->w.tape.total;
+  return B2G_OK;
+}
+
+static int tape_ready(World* w, const char* fn) {
+  if (w->tape.active) {
+    set_error(std::string(fn) + ": close the recording tape first (b2g_tape_end)");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  if (w->tape.steps.empty()) {
+    set_error(std::string(fn) + ": no recorded steps");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  return B2G_OK;
+}
+
+int b2g_tape_replay(b2g_world* h, float* final_state) {
+  if (!h || !final_state) { set_error("b2g_tape_replay: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  int st;
+  if ((st = tape_ready(&h->w, "b2g_tape_replay"))) return st;
+  B2G_HIP(hipStreamSynchronize(h->w.stream));
+  return tape_replay(&h->w, final_state);
+}
+
+int b2g_tape_backward(b2g_world* h, const float* cotangent, float* grad_state, float* grad_friction) {
+  if (!h || !cotangent || !grad_state) { set_error("b2g_tape_backward: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  int st;
+  if ((st = tape_ready(&h->w, "b2g_tape_backward"))) return st;
+  B2G_HIP(hipStreamSynchronize(h->w.stream));
+  return tape_backward(&h->w, cotangent, grad_state, grad_friction);
+}
+
+int b2g_tape_export(b2g_world* h, int32_t step, int32_t cap, int32_t* slots, float* impulses, int32_t* colour_offsets,
+                    int32_t colour_cap, int32_t* n_slots, int32_t* n_colours, float* step_params) {
+  if (!h || !n_slots || !n_colours || step < 0 || cap < 0) {
+    set_error("b2g_tape_export: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  Tape& t = w->tape;
+  if (step >= int(t.steps.size())) { set_error("b2g_tape_export: no such step"); return B2G_ERR_INVALID_ARGUMENT; }
+  const TapeStep& ts = t.steps[step];
+  *n_slots = ts.n;
+  *n_colours = int32_t(ts.color_off.size()) - 1;
+  if (step_params) {
+    step_params[0] = ts.dt; step_params[1] = ts.dt_ratio; step_params[2] = float(ts.vel_iters);
+    step_params[3] = float(ts.pos_iters); step_params[4] = ts.warm ? 1.0f : 0.0f; step_params[5] = ts.block ? 1.0f : 0.0f;
+  }
+  if (colour_offsets)
+    for (int i = 0; i < int(ts.color_off.size()) && i < colour_cap; ++i) colour_offsets[i] = ts.color_off[i];
+  const int m = std::min(cap, ts.n);
+  if (m > 0 && (slots || impulses)) {
+    std::vector<int4> sl(m), mt(m);
+    std::vector<uint2> id(m);
+    std::vector<float4> l0(m), ls(m);
+    hipStream_t s = w->stream;
+    B2G_HIP(hipMemcpyAsync(sl.data(), t.slot.p + ts.off, sizeof(int4) * m, hipMemcpyDeviceToHost, s));
+    B2G_HIP(hipMemcpyAsync(mt.data(), t.meta.p + ts.off, sizeof(int4) * m, hipMemcpyDeviceToHost, s));
+    B2G_HIP(hipMemcpyAsync(id.data(), t.ids.p + ts.off, sizeof(uint2) * m, hipMemcpyDeviceToHost, s));
+    B2G_HIP(hipMemcpyAsync(l0.data(), t.lam0.p + ts.off, sizeof(float4) * m, hipMemcpyDeviceToHost, s));
+    B2G_HIP(hipMemcpyAsync(ls.data(), t.lam_store.p + ts.off, sizeof(float4) * m, hipMemcpyDeviceToHost, s));
+    B2G_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < m; ++i) {
+      if (slots) {
+        int32_t* o = slots + 10 * i;
+        o[0] = sl[i].x; o[1] = sl[i].y; o[2] = sl[i].z; o[3] = sl[i].w;
+        o[4] = mt[i].x; o[5] = mt[i].y; o[6] = mt[i].z; o[7] = mt[i].w;
+        o[8] = int32_t(id[i].x); o[9] = int32_t(id[i].y);
+      }
+      if (impulses) {
+        float* o = impulses + 8 * i;
+        o[0] = l0[i].x; o[1] = l0[i].y; o[2] = l0[i].z; o[3] = l0[i].w;
+        o[4] = ls[i].x; o[5] = ls[i].y; o[6] = ls[i].z; o[7] = ls[i].w;
+      }
+    }
+  }
   return B2G_OK;
 }
 

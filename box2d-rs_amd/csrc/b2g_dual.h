@@ -82,6 +82,10 @@ template <int N, class S>
 B2G_HD bool operator<=(const Dual<N, S>& a, const Dual<N, S>& b) { return a.v <= b.v; }
 template <int N, class S>
 B2G_HD bool operator>=(const Dual<N, S>& a, const Dual<N, S>& b) { return a.v >= b.v; }
+template <int N, class S>
+B2G_HD bool operator==(const Dual<N, S>& a, const Dual<N, S>& b) { return a.v == b.v; }
+template <int N, class S>
+B2G_HD bool operator!=(const Dual<N, S>& a, const Dual<N, S>& b) { return a.v != b.v; }
 
 template <int N, class S>
 B2G_HD S primal(const Dual<N, S>& a) { return a.v; }

@@ -161,6 +161,35 @@ struct KernelTimer {
   long long launches = 0, items = 0;
 };
 
+// Recorded forward run for the adjoint. A slot is one touching contact of one step, in that
+// step's solve order; slots of all steps are concatenated.
+struct TapeStep {
+  int n = 0;                    // slots of this step
+  long long off = 0;            // first slot
+  std::vector<int> color_off;   // colour offsets (relative to off)
+  float dt = 0.0f, dt_ratio = 0.0f;
+  int vel_iters = 0, pos_iters = 0;
+  bool warm = true, block = true;
+};
+struct Tape {
+  bool active = false;
+  bool has_prev = false;        // the previous step was recorded (carry sources exist)
+  int every = 16;               // body-state checkpoint interval in steps
+  std::vector<TapeStep> steps;
+  long long total = 0;
+  DevArray<int4> slot;          // bodyA, bodyB, fixtureA, fixtureB
+  DevArray<int4> meta;          // type | count << 8, carry source of point 0 and 1 (slot*2 + point or -1), runs
+  DevArray<uint2> ids;          // feature ids (checked when the manifold is recomputed)
+  DevArray<float4> lam0;        // impulses as collide carried them (the source when carry is -1)
+  DevArray<float4> lam_store;   // impulses as store_impulses left them in the manifold
+  DevArray<float4> c_lam0;      // contact-indexed scratch of the current step
+  DevArray<int2> c_carry;       // per point: previous-set contact * 2 + point, or -1
+  DevArray<long long> c_slot[2];// slot of each contact of the two contact sets, or -1
+  DevArray<float4> ck_pos, ck_vel;  // body state at the start of every checkpointed step
+  int ck_count = 0;
+  int nb = 0;
+};
+
 struct World {
   // settings
   V2 gravity;
@@ -210,7 +239,16 @@ struct World {
   StepProfile prof;
   hipEvent_t ev[8] = {};
   KernelTimer kt;
+  Tape tape;
 };
+
+// adjoint tape hooks of b2g_world_step (b2g_adjoint.hip)
+int tape_step_begin(World* w);
+int tape_after_collide(World* w);
+int tape_after_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters);
+void tape_release(World* w);
+int tape_replay(World* w, float* final_state);
+int tape_backward(World* w, const float* cotangent, float* grad_state, float* grad_friction);
 
 // kernel timer hooks: kt_begin before a launch, kt_end after it, kt_harvest once the stream has
 // been synchronised (it also waits for the last recorded event itself)

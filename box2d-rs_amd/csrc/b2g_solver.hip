@@ -679,15 +679,16 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
 }
 
 // end of a position iteration: islands whose error is small stop iterating
-// (src/private/dynamics/b2_island_private.rs:257-273)
-__global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, const int* __restrict__ btype,
+// (src/private/dynamics/b2_island_private.rs:257-273). A finished island's flag holds the number
+// of iterations it ran (the adjoint's tape reads it).
+__global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
                                                                    const int* __restrict__ parent, int* __restrict__ minsep,
                                                                    int* __restrict__ done, int* __restrict__ active) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   if (btype[b] == 0 || parent[b] != b || done[b]) return;
   float ms = float_unflip(minsep[b]);
-  if (ms >= -3.0f * kLinearSlop) done[b] = 1;
+  if (ms >= -3.0f * kLinearSlop) done[b] = iteration + 1;
   else atomicAdd(active, 1);
   minsep[b] = float_flip(0.0f);
 }
@@ -863,7 +864,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       hipLaunchKernelGGL(k_solve_position, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1, pa);
       kt_end(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0);
     }
-    hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb,
+    hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
                        w->b.type.p, s.parent.p, s.isl_minsep.p, s.isl_done.p, cnt + 3);
     ++w->pos_iters_run;
   }

@@ -196,6 +196,33 @@ int b2g_collide_pairs(b2g_world* world, int32_t n, const int32_t* pairs, const f
  * world's filters. Writes at most cap pairs (lower, higher fixture index) and the total count. */
 int b2g_broadphase_pairs(b2g_world* world, const float* fat_aabbs, int64_t cap, int32_t* pairs, int64_t* count);
 
+/* ---- Adjoint (no counterpart in the reference: box2d-rs has no derivatives) ----
+ * Gradients of a run of steps with respect to the body state at its start and the fixture
+ * frictions, for callers that optimise through the simulation. The run is recorded by stepping
+ * with a tape open; its contact set, manifold types and feature ids, solve order and per-island
+ * position-iteration counts are then held fixed (the derivative of the branch the run took), and
+ * the backward pass replays segments from body-state checkpoints taken every
+ * checkpoint_every steps. While a tape is open, only b2g_world_step may advance the world and
+ * no bodies may be added. */
+int b2g_tape_begin(b2g_world* world, int32_t checkpoint_every);
+int b2g_tape_end(b2g_world* world);
+int b2g_tape_free(b2g_world* world);
+int b2g_tape_info(b2g_world* world, int32_t* steps, int64_t* slots);
+/* Replays the recorded run with the frozen structure from its start state; writes the final
+ * body state, 6 floats per body (c.x, c.y, a, v.x, v.y, w). */
+int b2g_tape_replay(b2g_world* world, float* final_state);
+/* Reverse pass. cotangent: dL/d(final state), 6 floats per body as above. Writes dL/d(start
+ * state) (6 floats per body) and, if grad_friction is not null, dL/d(friction) per fixture. */
+int b2g_tape_backward(b2g_world* world, const float* cotangent, float* grad_state, float* grad_friction);
+/* One recorded step: per slot 10 ints (body A, body B, fixture A, fixture B, manifold type |
+ * point count << 8, carry source of point 0 and 1 as slot * 2 + point over all recorded slots or
+ * -1, position iterations run, feature id 0 and 1) and 8 floats (impulses carried in by collide,
+ * impulses stored after the solve); colour offsets of the slots; step parameters (dt, dt_ratio,
+ * velocity and position iterations, warm starting, block solve). */
+int b2g_tape_export(b2g_world* world, int32_t step, int32_t cap, int32_t* slots, float* impulses,
+                    int32_t* colour_offsets, int32_t colour_cap, int32_t* n_slots, int32_t* n_colours,
+                    float* step_params);
+
 /* Kernels that b2g_world_time_kernel can time, and the item each launch counts. */
 enum {
   B2G_KERNEL_NONE = -1,

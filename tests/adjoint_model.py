@@ -1,0 +1,124 @@
+"""Host f64 model of a recorded run of frozen steps and its reverse pass (tests/host/adjoint_check.cpp),
+plus a recorder that takes the frozen structure from the oracle's own run (CPU only).
+
+The structure of a step is what b2g_tape_export returns for the GPU: per touching contact
+(slot) its bodies, fixtures, manifold type | point count << 8, the slots its two points carry
+their warm-start impulses from (slot * 2 + point over all steps, or -1), the number of position
+iterations and the two feature ids; plus the carried impulses and the step parameters.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "tests", "host", "_build", "libadjoint_check.so")
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def state6(states):
+    """(n, 6) float64: c.x, c.y, a, v.x, v.y, w from body-state records."""
+    s = np.zeros((len(states), 6))
+    s[:, 0:2], s[:, 2], s[:, 3:5], s[:, 5] = states["c"], states["a"], states["v"], states["w"]
+    return s
+
+
+class HostAdjoint:
+    def __init__(self, oracle32, scene, mass, gravity=(0.0, -10.0)):
+        self.lib = ctypes.CDLL(LIB)
+        self.lib.ac_new.restype = ctypes.c_void_p
+        self.lib.ac_new.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        self.lib.ac_free.argtypes = [ctypes.c_void_p]
+        self.lib.ac_add_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+        self.lib.ac_forward.argtypes = [ctypes.c_void_p] * 4
+        self.lib.ac_backward.argtypes = [ctypes.c_void_p] * 6
+        bd = scene.body_defs
+        self.nb, self.nf = len(bd), len(scene.fixture_defs)
+        m = np.zeros((self.nb, 8))
+        m[:, 0], m[:, 1], m[:, 2:4], m[:, 4] = mass[:, 1], mass[:, 3], mass[:, 4:6], mass[:, 0]
+        m[:, 5], m[:, 6], m[:, 7] = bd["gravity_scale"], bd["linear_damping"], bd["angular_damping"]
+        self._shapes, status = oracle32.make_shapes(scene.shape_defs)
+        assert np.all(status == 0)
+        fd = scene.fixture_defs
+        mat = np.stack([fd["friction"], fd["restitution"], fd["restitution_threshold"]], 1).astype(np.float64)
+        self.friction = mat[:, 0].copy()
+        bt = np.ascontiguousarray(bd["type"], np.int32)
+        self.btype = bt
+        force = np.zeros((self.nb, 3))
+        self.h = self.lib.ac_new(self.nb, _p(bt), _p(m), _p(force), self.nf, _p(self._shapes), _p(mat),
+                                 float(gravity[0]), float(gravity[1]))
+        self.steps = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ac_free(self.h)
+            self.h = None
+
+    def add_step(self, slots, lam0, params):
+        sl = np.ascontiguousarray(slots, np.int32).reshape(-1, 10)
+        l0 = np.ascontiguousarray(lam0, np.float64).reshape(-1, 4)
+        pr = np.ascontiguousarray(params, np.float64)
+        self.lib.ac_add_step(self.h, len(sl), _p(sl), _p(l0), _p(pr))
+        self.steps += 1
+
+    def forward(self, state0, friction=None):
+        s0 = np.ascontiguousarray(state0, np.float64)
+        fr = np.ascontiguousarray(self.friction if friction is None else friction, np.float64)
+        out = np.zeros((self.nb, 6))
+        mism = self.lib.ac_forward(self.h, _p(s0), _p(fr), _p(out))
+        return out, mism
+
+    def backward(self, state0, cot, friction=None):
+        s0 = np.ascontiguousarray(state0, np.float64)
+        fr = np.ascontiguousarray(self.friction if friction is None else friction, np.float64)
+        c = np.ascontiguousarray(cot, np.float64)
+        g = np.zeros((self.nb, 6))
+        gf = np.zeros(self.nf)
+        mism = self.lib.ac_backward(self.h, _p(s0), _p(fr), _p(c), _p(g), _p(gf))
+        return g, gf, mism
+
+
+def oracle_structure(oracle, scene, steps, dt=1.0 / 60.0, pos_iters=3):
+    """Frozen structure of `steps` steps of the oracle's own run: touching contacts in contact-list
+    order, warm-start carries matched by fixture pair and feature id, every constraint taking all
+    position iterations."""
+    ow = oracle.world()
+    ow.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    body = np.asarray(scene.fixture_body)
+    prev, total, out = {}, 0, []
+    for n in range(steps):
+        ow.collide()
+        fab, flags, man, _ = ow.contacts()
+        touch = np.nonzero(flags & 1)[0]
+        slots = np.zeros((len(touch), 10), np.int32)
+        lam0 = np.zeros((len(touch), 4))
+        cur = {}
+        for i, k in enumerate(touch):
+            fa, fb = int(fab[k, 0]), int(fab[k, 1])
+            m = man[k]
+            cnt = int(m["point_count"])
+            ids = [int(m["points"]["id"][p]) for p in range(2)]
+            src = [-1, -1]
+            old = prev.get((fa, fb))
+            if old is not None:
+                for p in range(cnt):
+                    for q in range(old[2]):
+                        if ids[p] == old[1][q]:
+                            src[p] = 2 * old[0] + q
+                            break
+            slots[i] = [body[fa], body[fb], fa, fb, int(m["type"]) | cnt << 8, src[0], src[1], pos_iters,
+                        ids[0], ids[1]]
+            lam0[i] = [m["points"]["normal_impulse"][0], m["points"]["tangent_impulse"][0],
+                       m["points"]["normal_impulse"][1], m["points"]["tangent_impulse"][1]]
+            cur[(fa, fb)] = (total + i, ids, cnt)
+        params = np.array([dt, 0.0 if n == 0 else 1.0, 8, pos_iters, 1, 1])
+        out.append((slots, lam0, params))
+        prev = cur
+        total += len(touch)
+        ow.step(dt, 8, pos_iters)
+    return out
