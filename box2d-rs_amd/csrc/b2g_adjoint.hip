@@ -114,6 +114,8 @@ int tape_step_begin(World* w) {
   if (t.steps.size() % size_t(t.every) != 0) return B2G_OK;
   const int nb = w->nb;
   int st;
+  // a growing array copies its contents outside the world's stream: let the stream finish first
+  if (size_t(t.ck_count + 1) * nb > t.ck_pos.cap) B2G_HIP(hipStreamSynchronize(w->stream));
   if ((st = t.ck_pos.reserve(size_t(t.ck_count + 1) * nb, true, size_t(t.ck_count) * nb)) ||
       (st = t.ck_vel.reserve(size_t(t.ck_count + 1) * nb, true, size_t(t.ck_count) * nb)))
     return st;
@@ -130,6 +132,8 @@ int tape_after_collide(World* w) {
   if (!t.active) return B2G_OK;
   const int n = w->n_pairs;
   int st;
+  if (size_t(std::max(n, 1)) > t.c_lam0.cap || size_t(std::max(n, 1)) > t.c_carry.cap)
+    B2G_HIP(hipStreamSynchronize(w->stream));
   if ((st = t.c_lam0.reserve(std::max(n, 1))) || (st = t.c_carry.reserve(std::max(n, 1)))) return st;
   if (n == 0) return B2G_OK;
   ContactArrays& c = w->c[w->cur];
@@ -154,6 +158,10 @@ int tape_after_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_
   }
   int st;
   const size_t need = size_t(t.total + std::max(nc, 1));
+  // a growing array copies its contents outside the world's stream: let the stream finish first
+  if (need > t.slot.cap || need > t.meta.cap || need > t.ids.cap || need > t.lam0.cap || need > t.lam_store.cap ||
+      size_t(std::max(np, 1)) > t.c_slot[w->cur].cap)
+    B2G_HIP(hipStreamSynchronize(w->stream));
   if ((st = t.slot.reserve(need, true, t.total)) || (st = t.meta.reserve(need, true, t.total)) ||
       (st = t.ids.reserve(need, true, t.total)) || (st = t.lam0.reserve(need, true, t.total)) ||
       (st = t.lam_store.reserve(need, true, t.total)) || (st = t.c_slot[w->cur].reserve(std::max(np, 1))) ||

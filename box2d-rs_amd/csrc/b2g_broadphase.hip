@@ -133,8 +133,9 @@ __global__ void __launch_bounds__(kBlock) k_morton(int n, const int* __restrict_
   unsigned ux = unsigned(fminf(fmaxf((cx - lx) * sx, 0.0f), cells));
   unsigned uy = unsigned(fminf(fmaxf((cy - ly) * sy, 0.0f), cells));
   unsigned code = spread15(ux) | (spread15(uy) << 1);
-  keys[i] = (unsigned(world[i]) << (2 * mbits)) | code;
-  vals[i] = unsigned(i);
+  // indexed by list position: the sort runs over the n listed proxies
+  keys[k] = (unsigned(world[i]) << (2 * mbits)) | code;
+  vals[k] = unsigned(i);
 }
 
 // ---- BVH -------------------------------------------------------------------------------------
