@@ -19,13 +19,16 @@ sys.path.insert(0, os.path.join(ROOT, "box2d-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per item of the timed kernels (the step's byte model: f32/i32 sizes, each
-# structure read or written once per launch). A velocity iteration moves 224 B per constraint:
-# with this library's layout a two-point constraint reads a 160 B record (bodies, info, masses,
+# Algorithmic HBM bytes of the timed kernels (the step's byte model: f32/i32 sizes, each structure
+# read or written once per pass), per item of one launch. A position iteration of one contact
+# (k_solve_position, one launch per colour) moves 136 B: the constraint (bodies, info, inverse
+# masses, local points, local normal and point, local centres, radii: 104 B) and the two body
+# positions gathered and scattered (32 B). A velocity iteration moves 224 B per contact: with
+# this library's layout a two-point constraint reads a 160 B record (bodies, info, masses,
 # normal/friction, impulses, effective masses, two anchor pairs, bias, block K and its inverse),
 # gathers and scatters two body velocities (64 B) and writes its impulses (16 B), 240 B in all; a
-# one-point constraint moves 192 B. A position iteration moves 136 B per constraint.
-UNIT_BYTES = {"solve_velocity": 224, "solve_position": 136, "warm_start": 128, "init_constraints": 404}
+# one-point constraint moves 192 B. Warm start 128 B, constraint set-up 404 B per contact.
+UNIT_BYTES = {"solve_position": 136, "solve_velocity": 224, "warm_start": 128, "init_constraints": 404}
 UNIT_NAME = {"solve_velocity": "constraint-iteration", "solve_position": "constraint-iteration",
              "warm_start": "constraint", "init_constraints": "constraint"}
 
@@ -65,7 +68,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--roofline-kernel", default="solve_velocity", choices=sorted(UNIT_BYTES))
+    ap.add_argument("--roofline-kernel", default="solve_position", choices=sorted(UNIT_BYTES))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single world or a batch of worlds is stepped on one "

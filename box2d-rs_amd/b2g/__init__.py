@@ -28,6 +28,8 @@ PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
 # kernels b2g_world_time_kernel can time (B2G_KERNEL_*)
 KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2, "init_constraints": 3,
            "traverse": 4, "collide": 5, "color_round": 6}
+# tuning knobs (B2G_TUNE_*)
+TUNE_COLOR_ROUNDS = 2
 
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
@@ -94,6 +96,7 @@ def _load():
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
         "b2g_world_time_kernel": ([P, I], I),
+        "b2g_world_tune": ([P, I, I], I),
         "b2g_tape_begin": ([P, I], I),
         "b2g_tape_end": ([P], I),
         "b2g_tape_free": ([P], I),
@@ -255,6 +258,10 @@ class World:
         p = _Profile()
         _check(_lib.b2g_world_get_profile(self._h, ctypes.byref(p)))
         return {name: getattr(p, name) for name, _ in _Profile._fields_ if name != "pad"}
+
+    def tune(self, param, value):
+        """Set a tuning knob (TUNE_*) of this world."""
+        _check(_lib.b2g_world_tune(self._h, int(param), int(value)))
 
     def time_kernel(self, name):
         """Record HIP events around every launch of one kernel (a KERNELS name; "none" stops)."""

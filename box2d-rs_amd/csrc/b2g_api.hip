@@ -427,6 +427,7 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
     return B2G_ERR_DEVICE;
   }
   for (int i = 0; i < 8; ++i) (void)hipEventCreate(&w->ev[i]);
+  std::memset(w->h_counters, 0, 256 * sizeof(int));
   *out = h;
   return B2G_OK;
 }
@@ -515,7 +516,7 @@ int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {
   int st;
   const bool taping = w->tape.active;
   if (taping) {
-    if (dt <= 0.0f || w->dirty || w->need_broadphase) {
+    if (dt <= 0.0f || w->dirty || w->nb != w->tape.nb) {
       tape_release(w);
       set_error("b2g_world_step: a recording tape needs dt > 0 and no bodies added while it is open");
       return B2G_ERR_INVALID_ARGUMENT;
@@ -727,6 +728,20 @@ int b2g_world_get_profile(b2g_world* h, b2g_profile* out) {
   out->colours = w->n_colors;
   out->position_iterations = w->pos_iters_run;
   return B2G_OK;
+}
+
+int b2g_world_tune(b2g_world* h, int32_t param, int32_t value) {
+  if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  switch (param) {
+    case B2G_TUNE_COLOR_ROUNDS:
+      if (value < 1 || value > 128) break;
+      w->color_rounds = value;
+      return B2G_OK;
+    default: break;
+  }
+  set_error("b2g_world_tune: unknown parameter or value out of range");
+  return B2G_ERR_INVALID_ARGUMENT;
 }
 
 int b2g_world_time_kernel(b2g_world* h, int32_t kernel) {

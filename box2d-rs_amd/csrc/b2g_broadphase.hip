@@ -268,7 +268,7 @@ struct PairFilter {
 
 static constexpr int kTravBlock = 64;
 static constexpr int kStack = 64;
-static constexpr int kSlots = 16;
+static constexpr int kSlots = 32;
 
 struct TreeView {
   int n;                       // leaves in the tree

@@ -50,10 +50,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate_velocities(int nb, const i
 }
 
 // src/private/dynamics/b2_island_private.rs:221-252
-__global__ void __launch_bounds__(kBlock) k_integrate_positions(int nb, const int* __restrict__ btype,
-                                                                float4* __restrict__ pos, float4* __restrict__ vel, float h) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+__device__ inline void integrate_position_one(int b, const int* __restrict__ btype, float4* __restrict__ pos,
+                                              float4* __restrict__ vel, float h) {
   if (btype[b] == 0) return;
   float4 p = pos[b], v = vel[b];
   V2 tr = h * v2(v.x, v.y);
@@ -73,19 +71,27 @@ __global__ void __launch_bounds__(kBlock) k_integrate_positions(int nb, const in
   pos[b] = p;
   vel[b] = v;
 }
+__global__ void __launch_bounds__(kBlock) k_integrate_positions(int nb, const int* __restrict__ btype,
+                                                                float4* __restrict__ pos, float4* __restrict__ vel, float h) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) integrate_position_one(b, btype, pos, vel, h);
+}
 
 // copy back + synchronize_transform (src/private/dynamics/b2_island_private.rs:276-285,
 // src/b2_body.rs:974-977). Static bodies keep the transform computed at creation (their
 // angle never changes, so the reference recomputes the same value).
-__global__ void __launch_bounds__(kBlock) k_finalize(int nb, const int* __restrict__ btype, const float4* __restrict__ pos,
-                                                     const float4* __restrict__ bmass, float4* __restrict__ xf) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+__device__ inline void finalize_one(int b, const int* __restrict__ btype, const float4* __restrict__ pos,
+                                    const float4* __restrict__ bmass, float4* __restrict__ xf) {
   if (btype[b] == 0) return;
   float4 p = pos[b], ms = bmass[b];
   Rot q = rot(p.z);
   V2 o = v2(p.x, p.y) - mul(q, v2(ms.z, ms.w));
   xf[b] = make_float4(o.x, o.y, q.s, q.c);
+}
+__global__ void __launch_bounds__(kBlock) k_finalize(int nb, const int* __restrict__ btype, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ bmass, float4* __restrict__ xf) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) finalize_one(b, btype, pos, bmass, xf);
 }
 
 // ---- touching contacts, islands, colouring -----------------------------------------------------
@@ -402,9 +408,7 @@ struct VelArgs {
 };
 
 // B2contactSolver::warm_start for one constraint (src/private/dynamics/b2_contact_solver_private.rs:228-266)
-__global__ void __launch_bounds__(kBlock) k_warm_start(int begin, int end, VelArgs a) {
-  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= end) return;
+__device__ inline void warm_start_one(int i, const VelArgs& a) {
   int2 bd = a.bodies[i];
   int4 info = a.info[i];
   float4 m = a.vm[i];
@@ -428,6 +432,10 @@ __global__ void __launch_bounds__(kBlock) k_warm_start(int begin, int end, VelAr
   if (info.x & 0x100) a.vel[bd.x] = make_float4(va.x, va.y, wa, 0.0f);
   if (info.x & 0x200) a.vel[bd.y] = make_float4(vb.x, vb.y, wb, 0.0f);
 }
+__global__ void __launch_bounds__(kBlock) k_warm_start(int begin, int end, VelArgs a) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end) warm_start_one(i, a);
+}
 
 // B2contactSolver::solve_velocity_constraints for one constraint
 // (src/private/dynamics/b2_contact_solver_private.rs:268-583)
@@ -440,9 +448,7 @@ __device__ inline void apply_normal_pair(V2& va, float& wa, V2& vb, float& wb, f
   wb += iB * (cross(r1b, P1) + cross(r2b, P2));
 }
 
-__global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, int block_solve, VelArgs a) {
-  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= end) return;
+__device__ inline void solve_velocity_one(int i, int block_solve, const VelArgs& a) {
   int2 bd = a.bodies[i];
   int4 info = a.info[i];
   float4 m = a.vm[i];
@@ -560,13 +566,14 @@ __global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, i
   if (info.x & 0x100) a.vel[bd.x] = make_float4(va.x, va.y, wa, 0.0f);
   if (info.x & 0x200) a.vel[bd.y] = make_float4(vb.x, vb.y, wb, 0.0f);
 }
+__global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, int block_solve, VelArgs a) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end) solve_velocity_one(i, block_solve, a);
+}
 
 // store_impulses (src/private/dynamics/b2_contact_solver_private.rs:585-598)
-__global__ void __launch_bounds__(kBlock) k_store_impulses(int n, const int* __restrict__ kc, const int4* __restrict__ info,
-                                                           const float4* __restrict__ lam, float4* __restrict__ pt0,
-                                                           float4* __restrict__ pt1) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ inline void store_one(int i, const int* __restrict__ kc, const int4* __restrict__ info,
+                                 const float4* __restrict__ lam, float4* __restrict__ pt0, float4* __restrict__ pt1) {
   int k = kc[i];
   int vcount = info[i].z;
   float4 l = lam[i];
@@ -580,6 +587,12 @@ __global__ void __launch_bounds__(kBlock) k_store_impulses(int n, const int* __r
     q.w = l.w;
     pt1[k] = q;
   }
+}
+__global__ void __launch_bounds__(kBlock) k_store_impulses(int n, const int* __restrict__ kc, const int4* __restrict__ info,
+                                                           const float4* __restrict__ lam, float4* __restrict__ pt0,
+                                                           float4* __restrict__ pt1) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_one(i, kc, info, lam, pt0, pt1);
 }
 
 // ---- position passes ----------------------------------------------------------------------------
@@ -595,10 +608,10 @@ struct PosArgs {
 
 // B2contactSolver::solve_position_constraints for one constraint
 // (src/private/dynamics/b2_contact_solver_private.rs:600-730)
-__global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, PosArgs a) {
-  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+// Called by every lane of a wave (the island reduction at the end uses the whole wave); lanes
+// past the colour's end pass live = false and stay inactive.
+__device__ inline void solve_position_one(int i, bool live, int begin, const PosArgs& a) {
   // inactive lanes stay in the wave (minSep = 0) for the island reduction at the end
-  const bool live = i < end;
   int4 info = live ? a.info[i] : make_int4(0, 0, 0, 0);
   const bool active = live && !a.done[info.w];
   if (!active) info.y = 0;
@@ -677,20 +690,28 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
     pending &= ~__ballot(mine);
   }
 }
+__global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, PosArgs a) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  solve_position_one(i, i < end, begin, a);
+}
 
 // end of a position iteration: islands whose error is small stop iterating
 // (src/private/dynamics/b2_island_private.rs:257-273). A finished island's flag holds the number
 // of iterations it ran (the adjoint's tape reads it).
-__global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
-                                                                   const int* __restrict__ parent, int* __restrict__ minsep,
-                                                                   int* __restrict__ done, int* __restrict__ active) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+__device__ inline void iteration_end_one(int b, int iteration, const int* __restrict__ btype,
+                                         const int* __restrict__ parent, int* __restrict__ minsep, int* __restrict__ done,
+                                         int* __restrict__ active) {
   if (btype[b] == 0 || parent[b] != b || done[b]) return;
   float ms = float_unflip(minsep[b]);
   if (ms >= -3.0f * kLinearSlop) done[b] = iteration + 1;
   else atomicAdd(active, 1);
   minsep[b] = float_flip(0.0f);
+}
+__global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
+                                                                   const int* __restrict__ parent, int* __restrict__ minsep,
+                                                                   int* __restrict__ done, int* __restrict__ active) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) iteration_end_one(b, iteration, btype, parent, minsep, done, active);
 }
 
 // ---- driver ---------------------------------------------------------------------------------------
@@ -833,6 +854,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   w->n_colors = ncolors;
   VelArgs va{s.k_bodies.p, s.k_info.p, s.vnf.p, s.vm.p, s.vr0.p, s.vr1.p, s.vms.p, s.vbias.p, s.K.p, s.NM.p, s.lam.p,
              w->b.vel.p};
+  PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
+             s.isl_done.p};
   const std::vector<int>& co = w->h_color_off;
   if (nc > 0 && w->warm_starting)
     for (int col = 0; col < ncolors; ++col) {
@@ -854,8 +877,6 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                        s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p);
   hipLaunchKernelGGL(k_integrate_positions, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
                      w->b.pos.p, w->b.vel.p, dt);
-  PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
-             s.isl_done.p};
   w->pos_iters_run = 0;
   for (int it = 0; it < pos_iters; ++it) {
     for (int col = 0; col < ncolors; ++col) {

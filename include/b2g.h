@@ -236,6 +236,11 @@ enum {
   B2G_KERNEL_COUNT = 7
 };
 
+/* Tuning knobs (no counterpart in the reference), set per world by b2g_world_tune:
+ *   B2G_TUNE_COLOR_ROUNDS         colouring rounds launched before the first read-back (default 16) */
+#define B2G_TUNE_COLOR_ROUNDS 2
+int b2g_world_tune(b2g_world* world, int32_t param, int32_t value);
+
 /* Measurement hook with no counterpart in the reference (its B2Profile times whole phases on
  * the host): records a pair of HIP events on the world's stream around every launch of one
  * kernel (B2G_KERNEL_NONE switches it off). The events add a little time to each timed launch's

@@ -215,3 +215,4 @@ def test_pyramid_600_step_band(oracle):
     assert abs(float(sg[-1]["c"][1]) - float(so[-1]["c"][1])) < 0.05
     assert float(np.abs(sg["v"][1:]).max()) < 0.05
     assert float(np.abs(so["v"][1:]).max()) < 0.05
+
