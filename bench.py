@@ -19,18 +19,21 @@ sys.path.insert(0, os.path.join(ROOT, "box2d-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes of the timed kernels (the step's byte model: f32/i32 sizes, each structure
-# read or written once per pass), per item of one launch. A position iteration of one contact
-# (k_solve_position, one launch per colour) moves 136 B: the constraint (bodies, info, inverse
-# masses, local points, local normal and point, local centres, radii: 104 B) and the two body
-# positions gathered and scattered (32 B). A velocity iteration moves 224 B per contact: with
-# this library's layout a two-point constraint reads a 160 B record (bodies, info, masses,
-# normal/friction, impulses, effective masses, two anchor pairs, bias, block K and its inverse),
-# gathers and scatters two body velocities (64 B) and writes its impulses (16 B), 240 B in all; a
-# one-point constraint moves 192 B. Warm start 128 B, constraint set-up 404 B per contact.
-UNIT_BYTES = {"solve_position": 136, "solve_velocity": 224, "warm_start": 128, "init_constraints": 404}
-UNIT_NAME = {"solve_velocity": "constraint-iteration", "solve_position": "constraint-iteration",
-             "warm_start": "constraint", "init_constraints": "constraint"}
+# Roofline kernel: k_traverse, the LBVH pair search, the longest launch of a step (one launch,
+# about a quarter millisecond on this workload). The solver's per-colour kernels take more time
+# in total but each launch lasts 4-11 us, so an event pair around one of them mostly measures
+# its dispatch; they run from a captured graph in the timed region.
+# Algorithmic HBM bytes of one traversal launch (f32/i32 sizes, each read or written once): per
+# query proxy its record -- fat AABB 16 B, body 4, body type 4, world 4, filter 8, shape header
+# 16 -- and its pair count 4 B: 56 B; per pair found the partner's record for the filters (body,
+# type, world, filter, shape header: 36 B), its sorted leaf box and proxy id (20 B) and the slot
+# write (4 B): 60 B. Internal-node reads of the walk are not counted: they are the search's
+# overhead above the algorithmic minimum.
+QUERY_BYTES, PAIR_BYTES = 56, 60
+UNIT_BYTES = {"traverse": QUERY_BYTES, "solve_position": 136, "solve_velocity": 224, "warm_start": 128,
+              "init_constraints": 404}
+UNIT_NAME = {"traverse": "proxy query", "solve_velocity": "constraint-iteration",
+             "solve_position": "constraint-iteration", "warm_start": "constraint", "init_constraints": "constraint"}
 
 
 def cpu_baseline(scene, seconds=15.0, max_steps=200):
@@ -68,7 +71,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--roofline-kernel", default="solve_position", choices=sorted(UNIT_BYTES))
+    ap.add_argument("--roofline-kernel", default="traverse", choices=sorted(UNIT_BYTES))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single world or a batch of worlds is stepped on one "
@@ -93,7 +96,7 @@ def main():
         w.step(dt, 8, 3)
     w.body_states()
     t1 = time.perf_counter()
-    k_ms, k_launches, k_items = w.kernel_time()
+    k_ms, k_launches, k_items, k_aux = w.kernel_time()
     w.time_kernel("none")
     p = w.profile()
     elapsed = t1 - t0
@@ -117,6 +120,8 @@ def main():
                    "bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}},
     }
     bytes_per_launch = UNIT_BYTES[args.roofline_kernel] * k_items / max(k_launches, 1)
+    if args.roofline_kernel == "traverse":
+        bytes_per_launch += PAIR_BYTES * k_aux / max(k_launches, 1)
     avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -124,7 +129,8 @@ def main():
                        "kernel": "k_" + args.roofline_kernel, "launches": k_launches,
                        "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
                        "unit_bytes": UNIT_BYTES[args.roofline_kernel],
-                       "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[args.roofline_kernel]}
+                       "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[args.roofline_kernel],
+                       "pairs_per_launch": k_aux / max(k_launches, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(out))

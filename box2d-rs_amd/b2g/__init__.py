@@ -29,7 +29,7 @@ PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
 KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2, "init_constraints": 3,
            "traverse": 4, "collide": 5, "color_round": 6}
 # tuning knobs (B2G_TUNE_*)
-TUNE_COLOR_ROUNDS = 2
+TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH = 2, 3
 
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
@@ -104,7 +104,7 @@ def _load():
         "b2g_tape_replay": ([P, P], I),
         "b2g_tape_backward": ([P, P, P, P], I),
         "b2g_tape_export": ([P, I, I, P, P, P, I, P, P, P], I),
-        "b2g_world_kernel_time": ([P, P, P, P], I),
+        "b2g_world_kernel_time": ([P, P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -268,11 +268,12 @@ class World:
         _check(_lib.b2g_world_time_kernel(self._h, KERNELS[name]))
 
     def kernel_time(self):
-        """(device ms summed over launches, launches, items) of the timed kernel since the last
-        call; resets the totals."""
-        ms, launches, items = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
-        _check(_lib.b2g_world_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(items)))
-        return ms.value, launches.value, items.value
+        """(device ms summed over launches, launches, items, aux) of the timed kernel since the
+        last call; resets the totals. aux is the traversal's pair count."""
+        ms, launches, items, aux = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.b2g_world_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(items),
+                                          ctypes.byref(aux)))
+        return ms.value, launches.value, items.value, aux.value
 
     # ---- adjoint -------------------------------------------------------------------------
     def tape_begin(self, checkpoint_every=16):

@@ -459,6 +459,7 @@ int b2g_world_destroy(b2g_world* h) {
     if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
   for (hipEvent_t e : w->kt.ev) (void)hipEventDestroy(e);
   tape_release(w);
+  release_solve_graphs(w);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete h;
   return B2G_OK;
@@ -734,6 +735,10 @@ int b2g_world_tune(b2g_world* h, int32_t param, int32_t value) {
   if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;
   switch (param) {
+    case B2G_TUNE_SOLVER_GRAPH:
+      w->solver_graphs = value != 0;
+      if (!w->solver_graphs) release_solve_graphs(w);
+      return B2G_OK;
     case B2G_TUNE_COLOR_ROUNDS:
       if (value < 1 || value > 128) break;
       w->color_rounds = value;
@@ -754,11 +759,11 @@ int b2g_world_time_kernel(b2g_world* h, int32_t kernel) {
   if ((st = kt_harvest(w))) return st;
   w->kt.kernel = kernel;
   w->kt.ms = 0.0;
-  w->kt.launches = w->kt.items = 0;
+  w->kt.launches = w->kt.items = w->kt.aux = 0;
   return B2G_OK;
 }
 
-int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* items) {
+int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* items, int64_t* aux) {
   if (!h || !ms || !launches || !items) {
     set_error("b2g_world_kernel_time: invalid argument");
     return B2G_ERR_INVALID_ARGUMENT;
@@ -769,8 +774,9 @@ int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* 
   *ms = w->kt.ms;
   *launches = w->kt.launches;
   *items = w->kt.items;
+  if (aux) *aux = w->kt.aux;
   w->kt.ms = 0.0;
-  w->kt.launches = w->kt.items = 0;
+  w->kt.launches = w->kt.items = w->kt.aux = 0;
   return B2G_OK;
 }
 

@@ -459,6 +459,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     return B2G_ERR_CAPACITY;
   }
   int total = h[0];
+  kt_aux(w, B2G_KERNEL_TRAVERSE, total);
   if ((st = dst.pair.reserve(std::max(total, 1)))) return st;
   hipLaunchKernelGGL(k_pairs_from_slots, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, bp.count.p,
                      bp.slots.p, dst.off.p, dst.pair.p);

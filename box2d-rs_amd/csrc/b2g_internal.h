@@ -159,6 +159,7 @@ struct KernelTimer {
   std::vector<long long> pending_items;
   double ms = 0.0;
   long long launches = 0, items = 0;
+  long long aux = 0;  // a second count known only after the launch (pairs found by the traversal)
 };
 
 // Recorded forward run for the adjoint. A slot is one touching contact of one step, in that
@@ -233,6 +234,14 @@ struct World {
   int* h_counters = nullptr;  // pinned host mirror (4 counters + 128 colour counts)
   int n_touch = 0, n_colors = 0, n_islands = 0;
   int color_rounds = 16;       // colouring rounds launched before the first readback
+  bool solver_graphs = true;   // replay the solve's per-colour launches as a captured HIP graph
+  struct CachedGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long last_use = 0;
+  };
+  std::vector<CachedGraph> graphs;
+  unsigned long long graph_clock = 0;
   int color_rounds_used = 0;
   std::vector<int> h_color_off;
   int pos_iters_run = 0;
@@ -271,6 +280,9 @@ inline void kt_end(World* w, int kernel, long long items) {
   (void)hipEventRecord(t.ev[k + 1], w->stream);
   t.pending_items.push_back(items);
 }
+inline void kt_aux(World* w, int kernel, long long value) {
+  if (w->kt.kernel == kernel) w->kt.aux += value;
+}
 int kt_harvest(World* w);
 
 // phase drivers (each returns a B2G_* status)
@@ -279,6 +291,7 @@ int phase_broadphase(World* w);  // fat AABBs -> next contact set (pairs + old i
 int phase_collide(World* w);     // narrowphase on the current contact set
 int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters);
 int phase_synchronize(World* w); // swept + fat AABBs of non-static proxies
+void release_solve_graphs(World* w);
 // primitives
 int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev);
 int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits);

@@ -11,6 +11,8 @@
 // non-static bodies (union-find); they carry the reference's per-island early exit of the
 // position iterations. Colouring is Jones-Plassmann-Luby with hash priorities and first-fit
 // colours, so it is deterministic. Body state is SoA float4 in HBM.
+#include <cstring>
+
 #include "b2g_internal.h"
 
 namespace b2g {
@@ -139,13 +141,18 @@ __global__ void k_solver_counters_init(int* c) {
   c[3] = 0;
 }
 
-// find with path halving; the halving writes only ever replace a parent by one of its
-// ancestors, so concurrent finds and unions stay consistent
-__device__ inline int uf_find(int* parent, int x) {
-  int p = parent[x];
+// Find with path halving for the union kernel, where other threads link roots with atomicCAS
+// at the same time: every access to parent[] is atomic (agent scope), the halving is a CAS that
+// only ever replaces a parent by its own parent, so it never undoes a concurrent link.
+__device__ inline int uf_find_atomic(int* parent, int x) {
+  int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   while (p != x) {
-    int g = parent[p];
-    if (g != p) parent[x] = g;
+    int g = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (g != p) {
+      int expected = p;
+      __hip_atomic_compare_exchange_strong(&parent[x], &expected, g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    }
     x = p;
     p = g;
   }
@@ -161,12 +168,14 @@ __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, c
   if (btype[b.x] == 0 || btype[b.y] == 0) return;
   int a = b.x, c = b.y;
   for (;;) {
-    a = uf_find(parent, a);
-    c = uf_find(parent, c);
+    a = uf_find_atomic(parent, a);
+    c = uf_find_atomic(parent, c);
     if (a == c) return;
     int hi = a > c ? a : c, lo = a > c ? c : a;
-    int prev = atomicCAS(&parent[hi], hi, lo);
-    if (prev == hi) return;
+    int prev = hi;
+    if (__hip_atomic_compare_exchange_strong(&parent[hi], &prev, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
     a = hi;
     c = lo;
   }
@@ -176,7 +185,11 @@ __global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* _
                                                      int* __restrict__ done) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
-  parent[b] = uf_find(parent, b);
+  // Read-only walk to the root; each thread writes only its own entry. (Halving writes here
+  // could land on another body's entry after that body stored its root and leave a non-root.)
+  int r = b;
+  for (int p = parent[r]; p != r; p = parent[r]) r = p;
+  parent[b] = r;
   minsep[b] = float_flip(0.0f);
   done[b] = 0;
 }
@@ -613,7 +626,9 @@ struct PosArgs {
 __device__ inline void solve_position_one(int i, bool live, int begin, const PosArgs& a) {
   // inactive lanes stay in the wave (minSep = 0) for the island reduction at the end
   int4 info = live ? a.info[i] : make_int4(0, 0, 0, 0);
-  const bool active = live && !a.done[info.w];
+  // island flags are read coherently (agent scope): another XCD may have written them
+  const bool active =
+      live && __hip_atomic_load(&a.done[info.w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
   if (!active) info.y = 0;
   int2 bd = active ? a.bodies[i] : make_int2(0, 0);
   const int ii = active ? i : begin;
@@ -674,9 +689,8 @@ __device__ inline void solve_position_one(int i, bool live, int begin, const Pos
   if (active && (info.x & 0x100)) a.pos[bd.x] = make_float4(cA.x, cA.y, aA, 0.0f);
   if (active && (info.x & 0x200)) a.pos[bd.y] = make_float4(cB.x, cB.y, aB, 0.0f);
   // Island minimum separation. A pile is one island, so a plain per-thread atomic would
-  // serialise the whole colour on one address: lanes of a wave first reduce per island, and a
-  // lane skips the atomic when the stored minimum is already lower (it only decreases, so a
-  // stale read can only cause an unneeded atomic, never a missed one).
+  // serialise the whole colour on one address: lanes of a wave first reduce per island and one
+  // lane per island and wave does the atomic.
   int key = float_flip(minSep);
   const int isl = info.w;
   unsigned long long pending = __ballot(minSep < 0.0f);
@@ -686,7 +700,8 @@ __device__ inline void solve_position_one(int i, bool live, int begin, const Pos
     const bool mine = (pending >> (threadIdx.x & 63)) & 1ull && isl == lisl;
     int v = mine ? key : 0x7fffffff;
     for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
-    if ((threadIdx.x & 63) == leader && v < a.minsep[lisl]) atomicMin(&a.minsep[lisl], v);
+    if ((threadIdx.x & 63) == leader)
+      __hip_atomic_fetch_min(&a.minsep[lisl], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     pending &= ~__ballot(mine);
   }
 }
@@ -702,7 +717,7 @@ __device__ inline void iteration_end_one(int b, int iteration, const int* __rest
                                          const int* __restrict__ parent, int* __restrict__ minsep, int* __restrict__ done,
                                          int* __restrict__ active) {
   if (btype[b] == 0 || parent[b] != b || done[b]) return;
-  float ms = float_unflip(minsep[b]);
+  float ms = float_unflip(__hip_atomic_load(&minsep[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   if (ms >= -3.0f * kLinearSlop) done[b] = iteration + 1;
   else atomicAdd(active, 1);
   minsep[b] = float_flip(0.0f);
@@ -712,6 +727,112 @@ __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int i
                                                                    int* __restrict__ done, int* __restrict__ active) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) iteration_end_one(b, iteration, btype, parent, minsep, done, active);
+}
+
+// ---- graph-mode variants: colour ranges and counts read from device memory -----------------
+// A captured graph replays with new colour offsets (copied before each launch) and a fixed
+// grid; the loops stride over whatever the colour holds this step.
+__global__ void __launch_bounds__(kBlock) k_warm_start_g(int c, const int* __restrict__ off, VelArgs a) {
+  const int b0 = off[c], b1 = off[c + 1];
+  for (int i = b0 + blockIdx.x * blockDim.x + threadIdx.x; i < b1; i += gridDim.x * blockDim.x) warm_start_one(i, a);
+}
+__global__ void __launch_bounds__(kBlock) k_solve_velocity_g(int c, const int* __restrict__ off, int block_solve,
+                                                             VelArgs a) {
+  const int b0 = off[c], b1 = off[c + 1];
+  for (int i = b0 + blockIdx.x * blockDim.x + threadIdx.x; i < b1; i += gridDim.x * blockDim.x)
+    solve_velocity_one(i, block_solve, a);
+}
+__global__ void __launch_bounds__(kBlock) k_solve_position_g(int c, const int* __restrict__ off, PosArgs a) {
+  const int b0 = off[c], b1 = off[c + 1];
+  // block-uniform trip count: whole waves take part in the island reduction
+  for (int base = blockIdx.x * blockDim.x; base < b1 - b0; base += gridDim.x * blockDim.x) {
+    const int i = b0 + base + threadIdx.x;
+    solve_position_one(i, i < b1, b0, a);
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_store_impulses_g(const int* __restrict__ ncp, const int* __restrict__ kc,
+                                                             const int4* __restrict__ info, const float4* __restrict__ lam,
+                                                             float4* __restrict__ pt0, float4* __restrict__ pt1) {
+  const int n = *ncp;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    store_one(i, kc, info, lam, pt0, pt1);
+}
+
+// everything a captured solve depends on: a cached graph is replayed only for the same key
+struct SolveGraphKey {
+  int ncolors, grid_col, grid_nc, nb, vel_iters, pos_iters, warm, block;
+  float dt;
+  const void* ptr[32];
+  bool operator==(const SolveGraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+// launches of the solve after the constraint set-up: warm start, velocity iterations, impulse
+// store, position integration, position iterations, transforms (graph-mode kernels)
+static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& va, const PosArgs& pa) {
+  SolverArrays& s = w->s;
+  ContactArrays& c = w->c[w->cur];
+  int* cnt = w->counters.p;
+  hipStream_t st = w->stream;
+  if (k.warm)
+    for (int col = 0; col < k.ncolors; ++col)
+      hipLaunchKernelGGL(k_warm_start_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, va);
+  for (int it = 0; it < k.vel_iters; ++it)
+    for (int col = 0; col < k.ncolors; ++col)
+      hipLaunchKernelGGL(k_solve_velocity_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, k.block, va);
+  hipLaunchKernelGGL(k_store_impulses_g, dim3(k.grid_nc), dim3(kBlock), 0, st, cnt + 0, s.k_contact.p, s.k_info.p,
+                     s.lam.p, c.pt0.p, c.pt1.p);
+  hipLaunchKernelGGL(k_integrate_positions, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p,
+                     w->b.pos.p, w->b.vel.p, k.dt);
+  for (int it = 0; it < k.pos_iters; ++it) {
+    for (int col = 0; col < k.ncolors; ++col)
+      hipLaunchKernelGGL(k_solve_position_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, pa);
+    hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, it,
+                       w->b.type.p, s.parent.p, s.isl_minsep.p, s.isl_done.p, cnt + 3);
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p, w->b.pos.p,
+                     w->b.mass.p, w->b.xf.p);
+}
+
+void release_solve_graphs(World* w) {
+  for (World::CachedGraph& g : w->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  w->graphs.clear();
+}
+
+// replays (capturing first if needed) the solve tail as a HIP graph
+static int solve_tail_graph(World* w, const SolveGraphKey& k, const VelArgs& va, const PosArgs& pa) {
+  const unsigned char* kb = reinterpret_cast<const unsigned char*>(&k);
+  World::CachedGraph* hit = nullptr;
+  for (World::CachedGraph& g : w->graphs)
+    if (g.key.size() == sizeof(k) && std::memcmp(g.key.data(), kb, sizeof(k)) == 0) hit = &g;
+  if (!hit) {
+    if (w->graphs.size() >= 16) {  // evict the least recently used
+      size_t lru = 0;
+      for (size_t i = 1; i < w->graphs.size(); ++i)
+        if (w->graphs[i].last_use < w->graphs[lru].last_use) lru = i;
+      (void)hipGraphExecDestroy(w->graphs[lru].exec);
+      w->graphs.erase(w->graphs.begin() + lru);
+    }
+    hipGraph_t graph = nullptr;
+    B2G_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    launch_solve_tail(w, k, va, pa);
+    B2G_HIP(hipStreamEndCapture(w->stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      return B2G_ERR_DEVICE;
+    }
+    World::CachedGraph g;
+    g.key.assign(kb, kb + sizeof(k));
+    g.exec = exec;
+    w->graphs.push_back(g);
+    hit = &w->graphs.back();
+  }
+  hit->last_use = ++w->graph_clock;
+  B2G_HIP(hipGraphLaunch(hit->exec, w->stream));
+  return B2G_OK;
 }
 
 // ---- driver ---------------------------------------------------------------------------------------
@@ -857,6 +978,35 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
              s.isl_done.p};
   const std::vector<int>& co = w->h_color_off;
+  if (nc > 0 && w->solver_graphs && w->kt.kernel == B2G_KERNEL_NONE) {
+    // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
+    int maxc = 0;
+    for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);
+    auto bucket = [](int n) {
+      int b = 256;
+      while (b < n) b <<= 1;
+      return b;
+    };
+    SolveGraphKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.ncolors = ncolors;
+    k.grid_col = blocks_for(bucket(maxc), kBlock);
+    k.grid_nc = blocks_for(bucket(nc), kBlock);
+    k.nb = nb;
+    k.vel_iters = vel_iters;
+    k.pos_iters = pos_iters;
+    k.warm = w->warm_starting;
+    k.block = w->block_solve;
+    k.dt = dt;
+    const void* ptrs[] = {va.bodies, va.info, va.vnf, va.vm, va.vr0, va.vr1, va.vms, va.vbias, va.K, va.NM, va.lam,
+                          va.vel, pa.plp, pa.pgeo, pa.plc, pa.prad, pa.pos, pa.minsep, pa.done, s.color_off.p, cnt,
+                          c.pt0.p, c.pt1.p, s.k_contact.p, w->b.type.p, s.parent.p, w->b.mass.p, w->b.xf.p};
+    static_assert(sizeof(ptrs) / sizeof(ptrs[0]) <= 32, "graph key pointers");
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) k.ptr[i] = ptrs[i];
+    if ((st = solve_tail_graph(w, k, va, pa))) return st;
+    w->pos_iters_run = pos_iters;
+    return B2G_OK;
+  }
   if (nc > 0 && w->warm_starting)
     for (int col = 0; col < ncolors; ++col) {
       int b0 = co[col], b1 = co[col + 1];

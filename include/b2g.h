@@ -237,8 +237,13 @@ enum {
 };
 
 /* Tuning knobs (no counterpart in the reference), set per world by b2g_world_tune:
- *   B2G_TUNE_COLOR_ROUNDS         colouring rounds launched before the first read-back (default 16) */
+ *   B2G_TUNE_COLOR_ROUNDS   colouring rounds launched before the first read-back (default 16)
+ *   B2G_TUNE_SOLVER_GRAPH   1: the solve's per-colour launches (warm start, velocity and position
+ *                           iterations) replay as a captured HIP graph, cached per colour count,
+ *                           grid size and buffers; 0: launched one by one (default 1; timing a
+ *                           kernel with b2g_world_time_kernel also launches them one by one) */
 #define B2G_TUNE_COLOR_ROUNDS 2
+#define B2G_TUNE_SOLVER_GRAPH 3
 int b2g_world_tune(b2g_world* world, int32_t param, int32_t value);
 
 /* Measurement hook with no counterpart in the reference (its B2Profile times whole phases on
@@ -247,8 +252,9 @@ int b2g_world_tune(b2g_world* world, int32_t param, int32_t value);
  * neighbours, not to its own duration. */
 int b2g_world_time_kernel(b2g_world* world, int32_t kernel);
 /* Totals of the timed kernel since the last call (which resets them): device milliseconds
- * summed over launches, number of launches and items processed. Waits for the stream. */
-int b2g_world_kernel_time(b2g_world* world, double* ms, int64_t* launches, int64_t* items);
+ * summed over launches, number of launches, items processed and, if aux is not null, a count
+ * known only after the launch (B2G_KERNEL_TRAVERSE: pairs found). Waits for the stream. */
+int b2g_world_kernel_time(b2g_world* world, double* ms, int64_t* launches, int64_t* items, int64_t* aux);
 
 #ifdef __cplusplus
 }
