@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-kernel", default="traverse", choices=sorted(UNIT_BYTES))
+    ap.add_argument("--solver-graph", type=int, default=1, choices=(0, 1),
+                    help="replay the solve's per-colour launches from a captured graph (default 1; profilers "
+                         "that intercept graph capture need 0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single world or a batch of worlds is stepped on one "
@@ -83,6 +86,7 @@ def main():
     scene = scenes.large_world(n=args.bodies, seed=1)
     w = b2g.World()
     w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
+    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     dt = 1.0 / 60.0
     for _ in range(args.warmup):

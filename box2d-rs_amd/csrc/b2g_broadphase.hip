@@ -442,11 +442,9 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   const unsigned* qlist = nt > 0 ? bp.vals.p : nullptr;
   if (nt < np) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
   if (nt > 0) {
-    kt_begin(w, B2G_KERNEL_TRAVERSE);
-    hipLaunchKernelGGL(k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt,
-                       reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr,
-                       flags);
-    kt_end(w, B2G_KERNEL_TRAVERSE, nt);
+    launch_timed(w, B2G_KERNEL_TRAVERSE, nt, k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock),
+                 nt, reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr,
+                 flags);
   }
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
   int h[3];

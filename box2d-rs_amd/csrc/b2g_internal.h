@@ -1,6 +1,7 @@
 // b2g internals: world state, device buffers and the phase drivers shared by the .hip files.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <string>
@@ -259,26 +260,28 @@ void tape_release(World* w);
 int tape_replay(World* w, float* final_state);
 int tape_backward(World* w, const float* cotangent, float* grad_state, float* grad_friction);
 
-// kernel timer hooks: kt_begin before a launch, kt_end after it, kt_harvest once the stream has
-// been synchronised (it also waits for the last recorded event itself)
-inline void kt_begin(World* w, int kernel) {
+// Launches a kernel; when it is the timed kernel of the world, the launch carries a start/stop
+// event pair (hipExtLaunchKernelGGL records them around the kernel's own dispatch, so the
+// elapsed time is the kernel's run, not the queue ahead of it). Pairs are harvested wherever the
+// stream has been synchronised.
+template <typename... Args, typename F>
+inline void launch_timed(World* w, int kernel, long long items, F func, dim3 grid, dim3 block, Args... args) {
   KernelTimer& t = w->kt;
-  if (t.kernel != kernel) return;
-  size_t k = 2 * t.pending_items.size();
-  while (t.ev.size() < k + 2) {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    t.ev.push_back(e);
+  if (t.kernel == kernel) {
+    const size_t k = 2 * t.pending_items.size();
+    bool ok = true;
+    while (ok && t.ev.size() < k + 2) {
+      hipEvent_t e;
+      ok = hipEventCreate(&e) == hipSuccess;
+      if (ok) t.ev.push_back(e);
+    }
+    if (ok) {
+      hipExtLaunchKernelGGL(func, grid, block, 0, w->stream, t.ev[k], t.ev[k + 1], 0, args...);
+      t.pending_items.push_back(items);
+      return;
+    }
   }
-  (void)hipEventRecord(t.ev[k], w->stream);
-}
-inline void kt_end(World* w, int kernel, long long items) {
-  KernelTimer& t = w->kt;
-  if (t.kernel != kernel) return;
-  size_t k = 2 * t.pending_items.size();
-  if (t.ev.size() < k + 2) return;  // event creation failed in kt_begin
-  (void)hipEventRecord(t.ev[k + 1], w->stream);
-  t.pending_items.push_back(items);
+  hipLaunchKernelGGL(func, grid, block, 0, w->stream, args...);
 }
 inline void kt_aux(World* w, int kernel, long long value) {
   if (w->kt.kernel == kernel) w->kt.aux += value;

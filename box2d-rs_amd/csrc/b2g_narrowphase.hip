@@ -77,11 +77,9 @@ int phase_collide(World* w) {
   if (n == 0) return B2G_OK;
   ContactArrays& c = w->c[w->cur];
   ContactArrays& o = w->c[1 - w->cur];
-  kt_begin(w, B2G_KERNEL_COLLIDE);
-  hipLaunchKernelGGL(k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, c.pair.p, c.old_index.p,
-                     w->f.body.p, w->f.shape.p, w->f.pool.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p, o.ids.p, c.ab.p,
-                     c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
-  kt_end(w, B2G_KERNEL_COLLIDE, n);
+  launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), n, c.pair.p,
+               c.old_index.p,
+               w->f.body.p, w->f.shape.p, w->f.pool.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p, o.ids.p, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }

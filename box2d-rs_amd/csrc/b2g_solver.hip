@@ -898,10 +898,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     for (;;) {
       int batch = round == 0 ? w->color_rounds : 4;
       for (int r = 0; r < batch; ++r, ++round) {
-        kt_begin(w, B2G_KERNEL_COLOR_ROUND);
-        hipLaunchKernelGGL(k_color_round, dim3(gp), dim3(kBlock), 0, w->stream, cnt, round, s.k_bodies.p,
-                           w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
-        kt_end(w, B2G_KERNEL_COLOR_ROUND, n_pairs);
+        launch_timed(w, B2G_KERNEL_COLOR_ROUND, n_pairs, k_color_round, dim3(gp), dim3(kBlock), cnt, round,
+                     s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
       }
       B2G_HIP(hipMemsetAsync(s.color_count.p, 0, sizeof(int) * (kMaxColors + 1), w->stream));
       hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, s.color_count.p);
@@ -968,9 +966,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     ia.prad = s.prad.p;
     // k_bodies is both read (colouring, done) and rewritten here in sorted order: the init
     // kernel recomputes bodies from the contact, so the aliasing is safe.
-    kt_begin(w, B2G_KERNEL_INIT_CONSTRAINTS);
-    hipLaunchKernelGGL(k_init_constraints, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, ia);
-    kt_end(w, B2G_KERNEL_INIT_CONSTRAINTS, nc);
+    launch_timed(w, B2G_KERNEL_INIT_CONSTRAINTS, nc, k_init_constraints, dim3(blocks_for(nc, kBlock)), dim3(kBlock),
+                 ia);
   }
   w->n_colors = ncolors;
   VelArgs va{s.k_bodies.p, s.k_info.p, s.vnf.p, s.vm.p, s.vr0.p, s.vr1.p, s.vms.p, s.vbias.p, s.K.p, s.NM.p, s.lam.p,
@@ -1010,17 +1007,14 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   if (nc > 0 && w->warm_starting)
     for (int col = 0; col < ncolors; ++col) {
       int b0 = co[col], b1 = co[col + 1];
-      kt_begin(w, B2G_KERNEL_WARM_START);
-      hipLaunchKernelGGL(k_warm_start, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1, va);
-      kt_end(w, B2G_KERNEL_WARM_START, b1 - b0);
+      launch_timed(w, B2G_KERNEL_WARM_START, b1 - b0, k_warm_start, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock),
+                   b0, b1, va);
     }
   for (int it = 0; it < vel_iters && nc > 0; ++it)
     for (int col = 0; col < ncolors; ++col) {
       int b0 = co[col], b1 = co[col + 1];
-      kt_begin(w, B2G_KERNEL_SOLVE_VELOCITY);
-      hipLaunchKernelGGL(k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1,
-                         int(w->block_solve), va);
-      kt_end(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0);
+      launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)),
+                   dim3(kBlock), b0, b1, int(w->block_solve), va);
     }
   if (nc > 0)
     hipLaunchKernelGGL(k_store_impulses, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_contact.p,
@@ -1031,9 +1025,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   for (int it = 0; it < pos_iters; ++it) {
     for (int col = 0; col < ncolors; ++col) {
       int b0 = co[col], b1 = co[col + 1];
-      kt_begin(w, B2G_KERNEL_SOLVE_POSITION);
-      hipLaunchKernelGGL(k_solve_position, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), 0, w->stream, b0, b1, pa);
-      kt_end(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0);
+      launch_timed(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0, k_solve_position, dim3(blocks_for(b1 - b0, kBlock)),
+                   dim3(kBlock), b0, b1, pa);
     }
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
                        w->b.type.p, s.parent.p, s.isl_minsep.p, s.isl_done.p, cnt + 3);
