@@ -6,6 +6,10 @@ iterations, synchronize + LBVH pair search) of the large-world scene: 65,536 dyn
 are resident in HBM before timing. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 600] [--warmup 10] [--bodies 65536] [--no-cpu]
+
+Other workloads (not the default line): --workload batched (4096 worlds, world-steps/s against
+the same worlds stepped one by one), stress (1.18M bodies: body-steps/s, pairs/s, manifolds/s,
+solver body-iterations/s), adjoint (1024 worlds x 64 bodies, 120 taped steps + reverse pass).
 """
 import argparse
 import json
@@ -64,13 +68,168 @@ def measured_traffic(kernel):
         return json.load(f).get("bytes_per_launch")
 
 
+def roofline(kernel, k_ms, k_launches, k_items, k_aux, traffic_key=None):
+    bytes_per_launch = UNIT_BYTES[kernel] * k_items / max(k_launches, 1)
+    if kernel == "traverse":
+        bytes_per_launch += PAIR_BYTES * k_aux / max(k_launches, 1)
+    avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel) if traffic_key else None,
+            "kernel": "k_" + kernel, "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
+            "bytes_per_launch": bytes_per_launch, "unit_bytes": UNIT_BYTES[kernel],
+            "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[kernel],
+            "pairs_per_launch": k_aux / max(k_launches, 1)}
+
+
+def timed_steps(w, steps, kernel, dt=1.0 / 60.0):
+    """K steps bracketed by a stream synchronisation on both sides, with the roofline kernel's
+    launches event-timed; returns (seconds, kernel totals)."""
+    w.body_states()
+    w.time_kernel(kernel)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.step(dt, 8, 3)
+    w.body_states()
+    t1 = time.perf_counter()
+    totals = w.kernel_time()
+    w.time_kernel("none")
+    return t1 - t0, totals
+
+
+def base_record(args, metric, value, unit, elapsed, steps, workload, extra):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": 1000.0 * elapsed / max(steps, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scenes)",
+            "config": dict({"workload": workload}, **extra)}
+
+
+def run_large_world(args, b2g, scenes):
+    scene = scenes.large_world(n=args.bodies, seed=1)
+    w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
+    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
+    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    for _ in range(args.warmup):
+        w.step(1.0 / 60.0, 8, 3)
+    steps = args.steps if args.steps is not None else 600
+    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    p = w.profile()
+    n = int(scene.meta["dynamic"])
+    out = base_record(args, "body-steps/s (64k-body world)", n * steps / elapsed, "body-steps/s", elapsed, steps,
+                      "large_world: %d dynamic bodies, 400x400 m box, dt 1/60, 8/3 iterations, warm start on, "
+                      "sleep and continuous off" % n,
+                      {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux,
+                               traffic_key=args.bodies == 65536)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene)
+    return out
+
+
+def run_batched(args, b2g, scenes):
+    """4096 independent worlds (16-1024 bodies each, heaps and stacks) stepped together, against
+    the same worlds stepped one at a time as separate worlds (a sample of them)."""
+    batch = scenes.batched_worlds(4096)
+    w = b2g.World()
+    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
+    w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
+    for _ in range(args.warmup):
+        w.step(1.0 / 60.0, 8, 3)
+    steps = args.steps if args.steps is not None else 300
+    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    nw = batch.meta["worlds"]
+    p = w.profile()
+    out = base_record(args, "world-steps/s (4096 batched worlds)", nw * steps / elapsed, "world-steps/s", elapsed,
+                      steps, "batched_worlds: 4096 worlds of 16-1024 bodies (heaps and box stacks), %d bodies"
+                      % batch.meta["dynamic"], {"worlds": nw, "bodies": int(batch.meta["dynamic"]),
+                                                "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    out["body_steps_per_s"] = batch.meta["dynamic"] * steps / elapsed
+    # plain per-world stepping: a sample of the same worlds, each its own world, stepped in turn
+    sample = list(range(0, nw, nw // 32))
+    per_world = []
+    for wi in sample:
+        sc = scenes.world_of(batch, wi)
+        one = b2g.World()
+        one.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
+        for _ in range(args.warmup):
+            one.step(1.0 / 60.0, 8, 3)
+        t, _ = timed_steps(one, 20, "none")
+        per_world.append(t / 20)
+    plain = 1.0 / (sum(per_world) / len(per_world))
+    out["plain_per_world"] = {"value": plain, "unit": "world-steps/s", "sample_worlds": len(sample),
+                              "ratio": out["value"] / plain}
+    out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux)
+    return out
+
+
+def run_stress(args, b2g, scenes):
+    """1,048,576 large-world bodies and 16,384 towers of 8 boxes in a 1600 x 1600 m box: body-steps/s,
+    plus the pair search's pairs/s (traversal time), the narrowphase's manifolds/s (collide time)
+    and the solver's body-iterations/s (solve phase time)."""
+    scene = scenes.stress_world()
+    w = b2g.World()
+    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
+    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    for _ in range(args.warmup):
+        w.step(1.0 / 60.0, 8, 3)
+    steps = args.steps if args.steps is not None else 40
+    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, "traverse")
+    n = int(scene.meta["dynamic"])
+    _, (c_ms, c_launches, c_items, _) = timed_steps(w, 5, "collide")
+    p = w.profile()
+    out = base_record(args, "body-steps/s (1.18M-body stress world)", n * steps / elapsed, "body-steps/s", elapsed,
+                      steps, "stress_world: 1,048,576 mixed bodies + 16,384 towers x 8 boxes, 1600x1600 m box",
+                      {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    out["pairs_per_s"] = k_aux / (k_ms / 1000.0) if k_ms else None
+    out["manifolds_per_s"] = c_items / (c_ms / 1000.0) if c_ms else None
+    out["solver_body_iterations_per_s"] = n * (8 + 3) / (p["solve"] / 1000.0) if p["solve"] else None
+    out["roofline"] = roofline("traverse", k_ms, k_launches, k_items, k_aux)
+    return out
+
+
+def run_adjoint(args, b2g, scenes):
+    """1024 worlds x 64 bodies: 120 steps recorded on a tape (body-state checkpoints every 16
+    steps), then the reverse pass for the loss sum_w |c_body0(w) - goal_w|^2."""
+    import numpy as np
+    batch = scenes.differentiable_worlds(1024, 64)
+    w = b2g.World()
+    w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
+    w.body_states()
+    steps = args.steps if args.steps is not None else 120
+    w.tape_begin(16)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.step(1.0 / 60.0, 8, 3)
+    st = w.body_states()
+    t1 = time.perf_counter()
+    w.tape_end()
+    cot = np.zeros((len(st), 6), np.float32)
+    b0 = batch.meta["body0"]
+    cot[b0, 0:2] = 2.0 * (st["c"][b0] - batch.meta["goal"])
+    t2 = time.perf_counter()
+    g, gf = w.tape_backward(cot)
+    t3 = time.perf_counter()
+    nw = batch.meta["worlds"]
+    out = base_record(args, "adjoint world-steps/s (1024 worlds x 64 bodies)", nw * steps / ((t1 - t0) + (t3 - t2)),
+                      "world-steps/s", (t1 - t0) + (t3 - t2), steps,
+                      "differentiable_worlds: 1024 worlds x 64 boxes/circles, 20x20 m boxes, taped forward + "
+                      "reverse pass, checkpoints every 16 steps", {"worlds": nw, "slots": w.tape_info()[1]})
+    out["forward_ms"] = 1000.0 * (t1 - t0)
+    out["backward_ms"] = 1000.0 * (t3 - t2)
+    out["grad_norm"] = float(np.linalg.norm(g[:, 3:6]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 600; batched 300, stress 40, "
+                                                            "adjoint 120)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint"))
     ap.add_argument("--roofline-kernel", default="traverse", choices=sorted(UNIT_BYTES))
     ap.add_argument("--solver-graph", type=int, default=1, choices=(0, 1),
                     help="replay the solve's per-colour launches from a captured graph (default 1; profilers "
@@ -82,62 +241,9 @@ def main():
         sys.exit(2)
     import b2g
     from b2g import scenes
-
-    scene = scenes.large_world(n=args.bodies, seed=1)
-    w = b2g.World()
-    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
-    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
-    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
-    dt = 1.0 / 60.0
-    for _ in range(args.warmup):
-        w.step(dt, 8, 3)
-    w.body_states()  # synchronise
-    # HIP events around every launch of the dominant kernel on the library's stream, over the
-    # timed region
-    w.time_kernel(args.roofline_kernel)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        w.step(dt, 8, 3)
-    w.body_states()
-    t1 = time.perf_counter()
-    k_ms, k_launches, k_items, k_aux = w.kernel_time()
-    w.time_kernel("none")
-    p = w.profile()
-    elapsed = t1 - t0
-    n = int(scene.meta["dynamic"])
-    value = n * args.steps / elapsed
-    out = {
-        "metric": "body-steps/s (64k-body world)",
-        "value": value,
-        "unit": "body-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1000.0 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded large-world scene)",
-        "config": {"workload": "large_world: %d dynamic bodies, 400x400 m box, dt 1/60, 8/3 iterations, "
-                               "warm start on, sleep and continuous off" % n,
-                   "bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}},
-    }
-    bytes_per_launch = UNIT_BYTES[args.roofline_kernel] * k_items / max(k_launches, 1)
-    if args.roofline_kernel == "traverse":
-        bytes_per_launch += PAIR_BYTES * k_aux / max(k_launches, 1)
-    avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
-    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.roofline_kernel),
-                       "kernel": "k_" + args.roofline_kernel, "launches": k_launches,
-                       "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
-                       "unit_bytes": UNIT_BYTES[args.roofline_kernel],
-                       "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[args.roofline_kernel],
-                       "pairs_per_launch": k_aux / max(k_launches, 1)}
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene)
-    print(json.dumps(out))
+    run = {"large_world": run_large_world, "batched": run_batched, "stress": run_stress,
+           "adjoint": run_adjoint}[args.workload]
+    print(json.dumps(run(args, b2g, scenes)))
 
 
 if __name__ == "__main__":

@@ -15,6 +15,7 @@ Workloads (sizes as named by the benchmark configuration list):
   large_world   65,536 mixed circles/boxes/polygons on a 1.2 m grid in a 400 x 400 m box
   heap_world    the large-world shape mix in a small container (batched worlds, adjoint)
   stack_world   columns of unit boxes 1.5 m apart (batched worlds)
+  stress_world  1,048,576 large-world bodies plus 16,384 towers of 8 boxes in a 1600 x 1600 m box
 """
 from collections import namedtuple
 
@@ -203,6 +204,35 @@ def large_world(n=65536, seed=1, spacing=1.2, jitter=0.05, half_extent=200.0):
     bd, fb, sd, fd, _ = mixed_bodies(n, seed, centers)
     parts = concat(_container(half_extent, half_extent), (bd, fb, sd, fd))
     return Scene(*parts, {"name": "large_world", "dynamic": n, "seed": seed})
+
+
+def stress_world(n=1 << 20, towers=16384, tower_height=8, seed=5, spacing=1.3, half_extent=800.0):
+    """The large-world mix on a sqrt(n) x sqrt(n) grid (spacing 1.3 m) in the lower part of a
+    1600 x 1600 m box, and `towers` columns of `tower_height` unit boxes in rows above it."""
+    side = int(np.ceil(np.sqrt(n)))
+    i = np.arange(n)
+    gx = (i % side - (side - 1) / 2.0) * spacing
+    gy = -half_extent + 1.0 + (i // side) * spacing
+    ju = uniform(seed ^ 0x5EED, np.arange(2 * n, dtype=np.uint64)).reshape(n, 2)
+    centers = np.stack([gx + (ju[:, 0] - 0.5) * 0.1, gy + (ju[:, 1] - 0.5) * 0.1], axis=1)
+    bd, fb, sd, fd, _ = mixed_bodies(n, seed, centers)
+    top = float(gy.max()) + 3.0
+    rows = max(1, int((half_extent - 2.0 - top) // (tower_height + 1.5)))
+    per_row = int(np.ceil(towers / rows))
+    pitch = (2 * half_extent - 4.0) / per_row
+    k = np.arange(towers)
+    tx = -half_extent + 2.0 + (k % per_row + 0.5) * pitch
+    ty = top + (k // per_row) * (tower_height + 1.5)
+    m = towers * tower_height
+    tb = _body_defs(m)
+    tb["position"][:, 0] = np.repeat(tx, tower_height)
+    tb["position"][:, 1] = (np.repeat(ty, tower_height) + 0.5 + np.tile(np.arange(tower_height), towers)).astype(np.float32)
+    tsd = np.zeros(m, SHAPE_DEF)
+    tsd["type"] = SHAPE_BOX
+    tsd["params"][:, :2] = 0.5
+    tfd = _fixture_defs(m)
+    parts = concat(_container(half_extent, half_extent), (bd, fb, sd, fd), (tb, np.arange(m, dtype=np.int32), tsd, tfd))
+    return Scene(*parts, {"name": "stress_world", "dynamic": n + m, "seed": seed})
 
 
 def heap_world(n, seed, half_extent=20.0, spacing=1.2, boxes_and_circles_only=False, friction=None):

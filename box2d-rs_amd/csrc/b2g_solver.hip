@@ -142,42 +142,39 @@ __global__ void k_solver_counters_init(int* c) {
 }
 
 // Find with path halving for the union kernel, where other threads link roots with atomicCAS
-// at the same time: every access to parent[] is atomic (agent scope), the halving is a CAS that
-// only ever replaces a parent by its own parent, so it never undoes a concurrent link.
-__device__ inline int uf_find_atomic(int* parent, int x) {
-  int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// at the same time. Every write to parent[] is a CAS (the halving replaces a parent by its own
+// parent only if it is still that parent), so no plain store can be written back over a
+// concurrent link. The first look-up reads plainly (a stale parent is still an ancestor, and a
+// root found from a stale read fails the linking CAS); after a failed CAS the reads are
+// coherent (agent scope), so the retry sees the other thread's link.
+template <bool COHERENT>
+__device__ inline int uf_read(int* parent, int x) {
+  return COHERENT ? __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : parent[x];
+}
+template <bool COHERENT>
+__device__ inline int uf_find_union(int* parent, int x) {
+  int p = uf_read<COHERENT>(parent, x);
   while (p != x) {
-    int g = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (g != p) {
-      int expected = p;
-      __hip_atomic_compare_exchange_strong(&parent[x], &expected, g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    }
+    const int g = uf_read<COHERENT>(parent, p);
+    if (g != p) atomicCAS(&parent[x], p, g);
     x = p;
     p = g;
   }
   return x;
 }
-
-// union of the two non-static bodies of each constraint: hook the larger root under the smaller
 __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                   const int* __restrict__ btype, int* parent) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= *ncp) return;
   int2 b = bodies[j];
   if (btype[b.x] == 0 || btype[b.y] == 0) return;
-  int a = b.x, c = b.y;
+  int a = uf_find_union<false>(parent, b.x), c = uf_find_union<false>(parent, b.y);
   for (;;) {
-    a = uf_find_atomic(parent, a);
-    c = uf_find_atomic(parent, c);
     if (a == c) return;
     int hi = a > c ? a : c, lo = a > c ? c : a;
-    int prev = hi;
-    if (__hip_atomic_compare_exchange_strong(&parent[hi], &prev, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      return;
-    a = hi;
-    c = lo;
+    if (atomicCAS(&parent[hi], hi, lo) == hi) return;
+    a = uf_find_union<true>(parent, hi);
+    c = uf_find_union<true>(parent, lo);
   }
 }
 
