@@ -24,16 +24,16 @@ sys.path.insert(0, os.path.join(ROOT, "box2d-rs_amd"))
 HBM_PEAK_GBS = 8000.0
 
 # Roofline kernel: k_traverse, the LBVH pair search, the longest launch of a step (one launch,
-# about a quarter millisecond on this workload). The solver's per-colour kernels take more time
-# in total but each launch lasts 4-11 us, so an event pair around one of them mostly measures
-# its dispatch; they run from a captured graph in the timed region.
+# under a tenth of a millisecond on this workload). The solver's per-colour kernels take more
+# time in total but each launch lasts 4-11 us, so an event pair around one of them mostly
+# measures its dispatch; they run from a captured graph in the timed region.
 # Algorithmic HBM bytes of one traversal launch (f32/i32 sizes, each read or written once): per
-# query proxy its record -- fat AABB 16 B, body 4, body type 4, world 4, filter 8, shape header
-# 16 -- and its pair count 4 B: 56 B; per pair found the partner's record for the filters (body,
-# type, world, filter, shape header: 36 B), its sorted leaf box and proxy id (20 B) and the slot
-# write (4 B): 60 B. Internal-node reads of the walk are not counted: they are the search's
-# overhead above the algorithmic minimum.
-QUERY_BYTES, PAIR_BYTES = 56, 60
+# query its packed leaf record (proxy, body, filter bits, group/shape type/body type: 16 B), its
+# fat AABB (16 B), its world (4 B) and its pair count (4 B): 40 B; per pair found the partner's
+# box (16 B, held in its parent's node record) and leaf record (16 B) and the slot write (4 B):
+# 36 B. Internal-node reads of the walk are not counted: they are the search's overhead above
+# the algorithmic minimum.
+QUERY_BYTES, PAIR_BYTES = 40, 36
 UNIT_BYTES = {"traverse": QUERY_BYTES, "solve_position": 136, "solve_velocity": 224, "warm_start": 128,
               "init_constraints": 404}
 UNIT_NAME = {"traverse": "proxy query", "solve_velocity": "constraint-iteration",

@@ -12,10 +12,11 @@
 //     overlapping. That set does not depend on the tree, so the GPU rebuilds it each step from
 //     an LBVH instead of maintaining the reference's incremental SAH tree.
 //
-// Each pair is emitted by one proxy (its emitter) into that proxy's segment, found by
-// count -> exclusive scan -> write, so the pair list is grouped by emitter and deterministic;
-// each new pair finds its predecessor in the previous step's segment of the same emitter
-// (warm starting). Pairs are stored as (emitter, other).
+// Each pair is emitted by one proxy (its emitter: the one with the lower sorted leaf position)
+// into that proxy's segment, found by count -> exclusive scan -> write, so the pair list is
+// grouped by emitter and deterministic; each new pair finds its predecessor in the previous
+// step's segment of either of its proxies (the emitter of a pair changes when the two proxies
+// swap places in the Morton order). Pairs are stored as (emitter, other).
 #include "b2g_internal.h"
 
 namespace b2g {
@@ -140,12 +141,13 @@ __global__ void __launch_bounds__(kBlock) k_morton(int n, const int* __restrict_
 
 // ---- BVH -------------------------------------------------------------------------------------
 // Topology: Karras's binary radix tree over the sorted keys (internal node i splits at the
-// highest differing key bit, so a node covers a compact Z-order cell; internal 0..n-2, leaf of
-// sorted position k is n-1+k). Node boxes: each internal node covers a contiguous range of
-// sorted leaves, and its box is the union over that range, read from a segment tree built
+// highest differing key bit, so a node covers a compact Z-order cell; internal 0..n-2, the
+// leaf of sorted position k is referenced as ~k). Boxes: each child covers a contiguous range
+// of sorted leaves, and its box is the union over that range, read from a segment tree built
 // level by level in LDS (level 0 = the sorted leaf boxes; node (L, i) = (L-1, 2i) u
-// (L-1, 2i+1)). Neither step needs atomics or communication between workgroups, so the build
-// is a handful of short launches and the tree is a pure function of the sorted order.
+// (L-1, 2i+1)) and stored in its parent's record. Neither step needs atomics or communication
+// between workgroups, so the build is a handful of short launches and the tree is a pure
+// function of the sorted order.
 struct Levels {
   int count;
   int size[32];
@@ -154,16 +156,33 @@ struct Levels {
 
 static constexpr int kReduceLevels = 8;
 
+// What the pair filters need of a proxy, packed for one 16-byte load per visited leaf:
+// {proxy, body, category | mask << 16, (group & 0xffff) | shape type << 16 | body type << 24}.
+struct LeafInputs {
+  const int* fbody;
+  const int* btype;
+  const int2* filter;
+  const int4* fshape;
+};
+__device__ inline int4 leaf_record(int p, const LeafInputs& li) {
+  const int b = li.fbody[p];
+  const int2 f = li.filter[p];
+  return make_int4(p, b, f.x, (f.y & 0xffff) | ((li.fshape[p].x & 0xff) << 16) | ((li.btype[b] & 0xff) << 24));
+}
+
 __global__ void __launch_bounds__(kBlock) k_seg_reduce(Levels lv, int L0, const unsigned* __restrict__ vals,
-                                                       const float4* __restrict__ fat, float4* __restrict__ seg) {
+                                                       const float4* __restrict__ fat, float4* __restrict__ seg,
+                                                       LeafInputs li, int4* __restrict__ leafrec) {
   __shared__ float4 sb[kBlock];
   const int t = threadIdx.x;
   const int i = blockIdx.x * kBlock + t;
   float4 e = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
   if (i < lv.size[L0]) {
     if (L0 == 0) {
-      e = fat[vals[i]];
+      const int p = int(vals[i]);
+      e = fat[p];
       seg[i] = e;
+      leafrec[i] = leaf_record(p, li);
     } else {
       e = seg[lv.off[L0] + i];
     }
@@ -200,10 +219,22 @@ __device__ inline void box_union(float4& r, float4 b) {
   r = make_float4(fminf(r.x, b.x), fminf(r.y, b.y), fmaxf(r.z, b.z), fmaxf(r.w, b.w));
 }
 
-// Karras 2012 internal node i: children, leaf range -> box (segment-tree query) and worlds
+// union of sorted leaves [a, b): bottom-up segment-tree query
+__device__ inline float4 seg_range(const float4* __restrict__ seg, const Levels& lv, int a, int b) {
+  float4 r = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
+  for (int L = 0; a < b; ++L) {
+    if (a & 1) box_union(r, seg[lv.off[L] + a++]);
+    if (b & 1) box_union(r, seg[lv.off[L] + --b]);
+    a >>= 1;
+    b >>= 1;
+  }
+  return r;
+}
+
+// Karras 2012 internal node i: split and children, then each child's box (a leaf's own box,
+// or a segment-tree query over the child's leaf range) and world range
 __global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __restrict__ keys, int wshift, Levels lv,
-                                                       const float4* __restrict__ seg, int2* __restrict__ child,
-                                                       float4* __restrict__ box, int2* __restrict__ worlds) {
+                                                       const float4* __restrict__ seg, BvhNode* __restrict__ nodes) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -223,20 +254,14 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __
   } while (t > 1);
   int gamma = i + s * d + (d < 0 ? -1 : 0);
   int lo = i < j ? i : j, hi = i < j ? j : i;
-  int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
-  int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
-  child[i] = make_int2(left, right);
-  worlds[i] = make_int2(int(keys[lo] >> wshift), int(keys[hi] >> wshift));
-  // union of leaves [lo, hi]: bottom-up segment-tree query
-  float4 r = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
-  int a = lo, b = hi + 1;
-  for (int L = 0; a < b; ++L) {
-    if (a & 1) box_union(r, seg[lv.off[L] + a++]);
-    if (b & 1) box_union(r, seg[lv.off[L] + --b]);
-    a >>= 1;
-    b >>= 1;
-  }
-  box[i] = r;
+  const bool left_leaf = lo == gamma, right_leaf = hi == gamma + 1;
+  BvhNode nd;
+  nd.box[0] = left_leaf ? seg[gamma] : seg_range(seg, lv, lo, gamma + 1);
+  nd.box[1] = right_leaf ? seg[gamma + 1] : seg_range(seg, lv, gamma + 1, hi + 1);
+  nd.link = make_int4(left_leaf ? ~gamma : gamma, right_leaf ? ~(gamma + 1) : gamma + 1, gamma, hi);
+  nd.worlds = make_int4(int(keys[lo] >> wshift), int(keys[gamma] >> wshift), int(keys[gamma + 1] >> wshift),
+                        int(keys[hi] >> wshift));
+  nodes[i] = nd;
 }
 
 // ---- pair search --------------------------------------------------------------------------
@@ -270,40 +295,53 @@ static constexpr int kTravBlock = 64;
 static constexpr int kStack = 64;
 static constexpr int kSlots = 32;
 
+// PairFilter::ok on two leaf records of one world (the node world ranges have already matched
+// the worlds): same body, at least one dynamic body, the default contact filter, and a contact
+// class for the shape pair.
+__device__ inline bool leaf_pair_ok(int4 a, int4 b) {
+  if (a.y == b.y) return false;
+  if (((a.w >> 24) & 0xff) != 2 && ((b.w >> 24) & 0xff) != 2) return false;
+  const int ga = static_cast<short>(a.w & 0xffff), gb = static_cast<short>(b.w & 0xffff);
+  if (ga == gb && ga != 0) {
+    if (ga < 0) return false;
+  } else {
+    const unsigned cata = a.z & 0xffff, maska = (unsigned(a.z) >> 16) & 0xffff;
+    const unsigned catb = b.z & 0xffff, maskb = (unsigned(b.z) >> 16) & 0xffff;
+    if ((maska & catb) == 0 || (cata & maskb) == 0) return false;
+  }
+  return registry_swap((a.w >> 16) & 0xff, (b.w >> 16) & 0xff) >= 0;
+}
+
 struct TreeView {
   int n;                       // leaves in the tree
-  const unsigned* vals;        // sorted leaf -> proxy
-  const int2* child;           // internal nodes
-  const float4* box;           // internal node boxes
-  const int2* worlds;          // internal node world ranges
-  const float4* leaf;          // sorted leaf boxes (segment-tree level 0)
+  const BvhNode* nodes;        // internal nodes
+  const int4* leafrec;         // sorted leaf records
   const int* loose_off;        // per world: range of big static proxies kept out of the tree
   const int* loose;
   int batched;
 };
 
-// One tree walk per emitting proxy (threads in Morton order, so a wave walks neighbouring
-// paths), then the world's big static proxies are tested directly. First pass: the first
-// kSlots pairs go to the proxy's slots and the count is recorded. The overflow pass re-walks
-// only proxies with more than kSlots pairs and writes their whole segment; the walk order is
-// deterministic, so both passes list the pairs in the same order.
+// One tree walk per tree proxy, thread k = sorted leaf k (neighbouring threads walk
+// neighbouring paths). The emitter of a pair of tree proxies is the one with the lower sorted
+// position, so a walk skips every child whose leaves all sit at or before its own position:
+// the last leaf under each child is in the node record. The node record also holds both child
+// boxes, so a visited node is one 64-byte load and a leaf is tested from its parent's record
+// without a load of its own. Then the world's big static proxies (outside the tree) are tested
+// directly; every tree proxy emits its pairs with them. First pass: the first kSlots pairs go
+// to the query's slots and the count is recorded. The overflow pass re-walks only queries with
+// more than kSlots pairs and writes their whole segment; the walk order is deterministic, so
+// both passes list the pairs in the same order.
 template <bool OVERFLOW_PASS>
-__global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const int* __restrict__ qlist,
-                                                         const float4* __restrict__ fat, TreeView tv, PairFilter pf,
-                                                         int* __restrict__ count, int* __restrict__ slots,
-                                                         const int* __restrict__ off, int2* __restrict__ pairs,
-                                                         int* __restrict__ flags) {
+__global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const float4* __restrict__ fat, TreeView tv,
+                                                         PairFilter pf, int* __restrict__ count,
+                                                         int* __restrict__ slots, const int* __restrict__ off,
+                                                         int2* __restrict__ pairs, int* __restrict__ flags) {
   __shared__ int stack[kStack][kTravBlock];
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nq) return;
-  const int q = qlist[k];
+  const int4 qr = tv.leafrec[k];
+  const int q = qr.x;
   if (OVERFLOW_PASS && count[q] <= kSlots) return;
-  // Emitter rule: a pair is emitted once, by its non-static proxy; a pair of two non-static
-  // proxies by the lower index. Static proxies emit nothing.
-  if (pf.btype[pf.fbody[q]] == 0) {
-    if (!OVERFLOW_PASS) count[q] = 0;
-    return;
-  }
   const float4 qa = fat[q];
   Aabb qb;
   qb.lo = v2(qa.x, qa.y);
@@ -311,43 +349,34 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const int* __re
   const int qw = pf.world[q];
   int c = 0;
   const int out = OVERFLOW_PASS ? off[q] : 0;
-  auto visit_leaf = [&](int p) {
-    bool emit = p != q && (p > q || pf.btype[pf.fbody[p]] == 0);
-    if (emit && pf.ok(q, p)) {
-      if (OVERFLOW_PASS) pairs[out + c] = make_int2(q, p);
-      else if (c < kSlots) slots[q * kSlots + c] = p;
-      ++c;
-    }
+  auto emit = [&](int p) {
+    if (OVERFLOW_PASS) pairs[out + c] = make_int2(q, p);
+    else if (c < kSlots) slots[size_t(c) * nq + k] = p;
+    ++c;
   };
-  const int n = tv.n;
-  if (n == 1) {
-    float4 nb = tv.leaf[0];
-    Aabb b;
-    b.lo = v2(nb.x, nb.y);
-    b.hi = v2(nb.z, nb.w);
-    if (overlap(b, qb)) visit_leaf(int(tv.vals[0]));
-  } else if (n > 1) {
+  if (tv.n > 1) {
     int sp = 0;
     stack[sp++][threadIdx.x] = 0;
     while (sp > 0) {
       const int node = stack[--sp][threadIdx.x];
-      const bool leaf = node >= n - 1;
-      if (!leaf && tv.batched) {
-        int2 wr = tv.worlds[node];
-        if (qw < wr.x || qw > wr.y) continue;
-      }
-      float4 nb = leaf ? tv.leaf[node - (n - 1)] : tv.box[node];
-      Aabb b;
-      b.lo = v2(nb.x, nb.y);
-      b.hi = v2(nb.z, nb.w);
-      if (!overlap(b, qb)) continue;
-      if (leaf) {
-        visit_leaf(int(tv.vals[node - (n - 1)]));
-      } else {
-        if (sp + 2 > kStack) { atomicExch(&flags[0], 1); break; }
-        int2 ch = tv.child[node];
-        stack[sp++][threadIdx.x] = ch.y;
-        stack[sp++][threadIdx.x] = ch.x;
+      const BvhNode nd = tv.nodes[node];
+      // right child first: an internal left child is pushed last and walked first
+      for (int s = 1; s >= 0; --s) {
+        if ((s ? nd.link.w : nd.link.z) <= k) continue;
+        if (tv.batched && (qw < (s ? nd.worlds.z : nd.worlds.x) || qw > (s ? nd.worlds.w : nd.worlds.y))) continue;
+        const float4 cb = nd.box[s];
+        Aabb b;
+        b.lo = v2(cb.x, cb.y);
+        b.hi = v2(cb.z, cb.w);
+        if (!overlap(b, qb)) continue;
+        const int ch = s ? nd.link.y : nd.link.x;
+        if (ch < 0) {
+          const int4 pr = tv.leafrec[~ch];
+          if (leaf_pair_ok(qr, pr)) emit(pr.x);
+        } else {
+          if (sp == kStack) { atomicExch(&flags[0], 1); sp = 0; break; }
+          stack[sp++][threadIdx.x] = ch;
+        }
       }
     }
   }
@@ -357,7 +386,7 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const int* __re
     Aabb b;
     b.lo = v2(nb.x, nb.y);
     b.hi = v2(nb.z, nb.w);
-    if (overlap(b, qb)) visit_leaf(p);
+    if (overlap(b, qb) && pf.ok(q, p)) emit(p);
   }
   if (!OVERFLOW_PASS) {
     count[q] = c;
@@ -365,29 +394,35 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const int* __re
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_pairs_from_slots(int np, const int* __restrict__ count,
+__global__ void __launch_bounds__(kBlock) k_pairs_from_slots(int nq, const int4* __restrict__ leafrec,
+                                                             const int* __restrict__ count,
                                                              const int* __restrict__ slots, const int* __restrict__ off,
                                                              int2* __restrict__ pairs) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
-  int c = min(count[q], kSlots);
-  int o = off[q];
-  for (int j = 0; j < c; ++j) pairs[o + j] = make_int2(q, slots[q * kSlots + j]);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  const int q = leafrec[k].x;
+  const int c = min(count[q], kSlots);
+  const int o = off[q];
+  for (int j = 0; j < c; ++j) pairs[o + j] = make_int2(q, slots[size_t(j) * nq + k]);
 }
 
-// predecessor of each new pair in the previous contact set (same emitter segment)
+// predecessor of each new pair in the previous contact set: the pair sits in the old segment of
+// one of its two proxies (its emitter then)
+__device__ inline int find_in_segment(int a, int b, int old_np, const int* __restrict__ old_off,
+                                      const int2* __restrict__ old_pairs) {
+  if (a >= old_np) return -1;
+  for (int i = old_off[a], e = old_off[a + 1]; i < e; ++i)
+    if (old_pairs[i].y == b) return i;
+  return -1;
+}
 __global__ void __launch_bounds__(kBlock) k_merge(int n, const int2* __restrict__ pairs, int old_np,
                                                   const int* __restrict__ old_off, const int2* __restrict__ old_pairs,
                                                   int* __restrict__ old_index) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int2 pr = pairs[k];
-  int found = -1;
-  if (pr.x < old_np) {
-    int s = old_off[pr.x], e = old_off[pr.x + 1];
-    for (int i = s; i < e; ++i)
-      if (old_pairs[i].y == pr.y) { found = i; break; }
-  }
+  int found = find_in_segment(pr.x, pr.y, old_np, old_off, old_pairs);
+  if (found < 0) found = find_in_segment(pr.y, pr.x, old_np, old_off, old_pairs);
   old_index[k] = found;
 }
 
@@ -409,8 +444,8 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     if (sz <= 1) break;
   }
   if ((st = bp.keys.reserve(std::max(nt, 1))) || (st = bp.vals.reserve(std::max(nt, 1))) ||
-      (st = bp.seg.reserve(total_nodes)) || (st = bp.node_child.reserve(std::max(nt, 1))) ||
-      (st = bp.node_aabb.reserve(std::max(nt, 1))) || (st = bp.node_worlds.reserve(std::max(nt, 1))) ||
+      (st = bp.seg.reserve(total_nodes)) || (st = bp.nodes.reserve(std::max(nt, 1))) ||
+      (st = bp.leafrec.reserve(std::max(nt, 1))) ||
       (st = bp.count.reserve(np + 1)) || (st = bp.slots.reserve(size_t(np) * kSlots)) ||
       (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(2)))
     return st;
@@ -422,6 +457,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   while ((1 << wbits) < w->n_worlds) ++wbits;
   const int mbits = std::min(15, (32 - wbits) / 2);
   const int key_bits = wbits + 2 * mbits;
+  const LeafInputs li{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p};
   if (nt > 0) {
     int gb = std::min(blocks_for(nt, kBlock), 1024);
     hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat, bnd);
@@ -430,21 +466,18 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     if ((st = radix_sort_pairs(w, bp.keys.p, bp.vals.p, nt, key_bits))) return st;
     for (int L0 = 0; L0 == 0 || L0 < lv.count - 1; L0 += kReduceLevels)
       hipLaunchKernelGGL(k_seg_reduce, dim3(blocks_for(lv.size[L0], kBlock)), dim3(kBlock), 0, w->stream, lv, L0,
-                         bp.vals.p, fat, bp.seg.p);
+                         bp.vals.p, fat, bp.seg.p, li, bp.leafrec.p);
     if (nt > 1)
       hipLaunchKernelGGL(k_lbvh_build, dim3(blocks_for(nt - 1, kBlock)), dim3(kBlock), 0, w->stream, nt, bp.keys.p,
-                         2 * mbits, lv, bp.seg.p, bp.node_child.p, bp.node_aabb.p, bp.node_worlds.p);
+                         2 * mbits, lv, bp.seg.p, bp.nodes.p);
   }
   PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p};
-  TreeView tv{nt, bp.vals.p, bp.node_child.p, bp.node_aabb.p, bp.node_worlds.p, bp.seg.p, w->f.loose_off.p,
-              w->f.loose.p, wbits > 0};
-  // emitters walk in Morton order; proxies outside the tree are static and emit nothing
-  const unsigned* qlist = nt > 0 ? bp.vals.p : nullptr;
+  TreeView tv{nt, bp.nodes.p, bp.leafrec.p, w->f.loose_off.p, w->f.loose.p, wbits > 0};
+  // tree proxies walk in Morton order; proxies outside the tree are static and emit nothing
   if (nt < np) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
   if (nt > 0) {
     launch_timed(w, B2G_KERNEL_TRAVERSE, nt, k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock),
-                 nt, reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr,
-                 flags);
+                 nt, fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr, flags);
   }
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
   int h[3];
@@ -459,12 +492,12 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   int total = h[0];
   kt_aux(w, B2G_KERNEL_TRAVERSE, total);
   if ((st = dst.pair.reserve(std::max(total, 1)))) return st;
-  hipLaunchKernelGGL(k_pairs_from_slots, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, w->stream, np, bp.count.p,
-                     bp.slots.p, dst.off.p, dst.pair.p);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_pairs_from_slots, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, w->stream, nt,
+                       bp.leafrec.p, bp.count.p, bp.slots.p, dst.off.p, dst.pair.p);
   if (h[2] > 0)
-    hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt,
-                       reinterpret_cast<const int*>(qlist), fat, tv, pf, bp.count.p, bp.slots.p, dst.off.p,
-                       dst.pair.p, flags);
+    hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt, fat,
+                       tv, pf, bp.count.p, bp.slots.p, dst.off.p, dst.pair.p, flags);
   B2G_HIP(hipGetLastError());
   *npairs = total;
   return B2G_OK;

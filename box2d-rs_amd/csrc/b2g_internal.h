@@ -134,16 +134,24 @@ struct SolverArrays {
   DevArray<int> parent, isl_minsep, isl_done;
 };
 
+// One internal LBVH node as a walk sees it: both child boxes, the children (internal node
+// index, or ~sorted position for a leaf), the last sorted leaf below each child and the world
+// range below each child. 64 bytes, so a visited node costs one cache line.
+struct alignas(16) BvhNode {
+  float4 box[2];
+  int4 link;    // {left, right, last leaf under left (the split), last leaf under right}
+  int4 worlds;  // {left first, left last, right first, right last}
+};
+
 struct BroadphaseArrays {
   DevArray<unsigned> keys, keys_alt;
   DevArray<unsigned> vals, vals_alt;
   DevArray<unsigned> hist;       // radix histograms
   DevArray<float4> seg;          // segment tree over the sorted leaf boxes (level 0 = leaves)
-  DevArray<int2> node_child;     // LBVH internal nodes
-  DevArray<float4> node_aabb;
-  DevArray<int2> node_worlds;    // world ids of the first and last leaf below each node
+  DevArray<BvhNode> nodes;       // LBVH internal nodes
+  DevArray<int4> leafrec;        // per sorted leaf: proxy, body, filter bits, group | shape type | body type
   DevArray<int> count;           // pairs per proxy
-  DevArray<int> slots;           // first pairs of each proxy, found by the single tree walk
+  DevArray<int> slots;           // first pairs of each query, column-major (slot j of query k at j * n + k)
   DevArray<float4> bounds;       // scene bounds and flags
   DevArray<int> scan_tmp;        // block sums for scans
 };

@@ -162,6 +162,22 @@ __device__ inline int uf_find_union(int* parent, int x) {
   }
   return x;
 }
+// Linking order: the root with the lower (hash, index) goes under the other. A bijective hash
+// makes this a random linking order, which keeps the expected tree depth logarithmic (linking by
+// plain index builds chains as long as a row of touching bodies when a wave links neighbours in
+// parallel). The final root of a component is its member with the highest (hash, index), a
+// function of the component alone, so island ids do not depend on the order of the links.
+__device__ inline unsigned uf_rank(int x) {
+  unsigned h = unsigned(x) * 0x9E3779B1u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+__device__ inline bool uf_below(int a, int c) {
+  const unsigned ha = uf_rank(a), hc = uf_rank(c);
+  return ha < hc || (ha == hc && a < c);
+}
 __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                   const int* __restrict__ btype, int* parent) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,10 +187,10 @@ __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, c
   int a = uf_find_union<false>(parent, b.x), c = uf_find_union<false>(parent, b.y);
   for (;;) {
     if (a == c) return;
-    int hi = a > c ? a : c, lo = a > c ? c : a;
-    if (atomicCAS(&parent[hi], hi, lo) == hi) return;
-    a = uf_find_union<true>(parent, hi);
-    c = uf_find_union<true>(parent, lo);
+    int lo = uf_below(a, c) ? a : c, hi = lo == a ? c : a;
+    if (atomicCAS(&parent[lo], lo, hi) == lo) return;
+    a = uf_find_union<true>(parent, lo);
+    c = uf_find_union<true>(parent, hi);
   }
 }
 
