@@ -29,7 +29,7 @@ PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
 KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2, "init_constraints": 3,
            "traverse": 4, "collide": 5, "color_round": 6}
 # tuning knobs (B2G_TUNE_*)
-TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH = 2, 3
+TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH, TUNE_COLOR_REUSE = 2, 3, 4
 
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),

@@ -442,13 +442,13 @@ int b2g_world_destroy(b2g_world* h) {
   rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
   rel(w->f.pool); rel(w->f.world); rel(w->f.tree_list); rel(w->f.loose_off); rel(w->f.loose);
   for (auto& c : w->c) {
-    rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids);
+    rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids); rel(c.colour);
   }
   SolverArrays& s = w->s;
   rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
   rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);
   rel(s.plp); rel(s.pgeo); rel(s.plc); rel(s.prad); rel(s.deg); rel(s.adj_off); rel(s.adj_cursor); rel(s.adj);
-  rel(s.color_state); rel(s.color_count); rel(s.color_off); rel(s.color_cursor); rel(s.order); rel(s.parent);
+  rel(s.color_state); rel(s.color_count); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.parent);
   rel(s.isl_minsep); rel(s.isl_done);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes);
@@ -742,6 +742,9 @@ int b2g_world_tune(b2g_world* h, int32_t param, int32_t value) {
     case B2G_TUNE_COLOR_ROUNDS:
       if (value < 1 || value > 128) break;
       w->color_rounds = value;
+      return B2G_OK;
+    case B2G_TUNE_COLOR_REUSE:
+      w->color_reuse = value != 0;
       return B2G_OK;
     default: break;
   }

@@ -507,7 +507,8 @@ static int reserve_contacts(ContactArrays& c, int n) {
   int st;
   n = std::max(n, 1);
   if ((st = c.pair.reserve(n)) || (st = c.old_index.reserve(n)) || (st = c.ab.reserve(n)) || (st = c.hdr.reserve(n)) ||
-      (st = c.geo.reserve(n)) || (st = c.pt0.reserve(n)) || (st = c.pt1.reserve(n)) || (st = c.ids.reserve(n)))
+      (st = c.geo.reserve(n)) || (st = c.pt0.reserve(n)) || (st = c.pt1.reserve(n)) || (st = c.ids.reserve(n)) ||
+      (st = c.colour.reserve(n)))
     return st;
   return B2G_OK;
 }
@@ -536,6 +537,8 @@ int phase_broadphase(World* w) {
   }
   if ((st = find_pairs(w, w->f.fat.p, w->np, dst, &n))) return st;
   if ((st = reserve_contacts(dst, n))) return st;
+  // no solve colours yet (the solve of this set writes those of its touching contacts)
+  if (n > 0) B2G_HIP(hipMemsetAsync(dst.colour.p, 0xff, sizeof(int) * n, w->stream));
   if (n > 0) {
     int old_np = w->set_np[w->cur];  // proxies when the old set was built (0: no old set)
     hipLaunchKernelGGL(k_merge, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, dst.pair.p, old_np,

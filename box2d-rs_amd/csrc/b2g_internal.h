@@ -106,6 +106,7 @@ struct ContactArrays {
   DevArray<float4> geo;      // local normal, local point
   DevArray<float4> pt0, pt1; // local point, normal impulse, tangent impulse
   DevArray<uint2> ids;
+  DevArray<int> colour;      // solve colour (launch index) of a touching contact, else -1
 };
 
 // Per-step solver arrays (indexed by constraint, colour-sorted).
@@ -128,7 +129,9 @@ struct SolverArrays {
   // colouring
   DevArray<int> deg, adj_off, adj_cursor, adj;  // body -> constraints (dynamic bodies only)
   DevArray<int> color_state;                    // (round << 8) | colour, or -1
-  DevArray<int> color_count, color_off, color_cursor;
+  DevArray<int> color_count, color_off, color_cursor;  // color_off: launch offsets, then per colour id
+                                                       // its base offset and its launch index
+  DevArray<int> prev_colour;                    // colour of the contact's predecessor, or -1
   DevArray<int> order;                          // colour-sorted constraint -> compacted index
   // islands
   DevArray<int> parent, isl_minsep, isl_done;
@@ -240,9 +243,12 @@ struct World {
   SolverArrays s;
   BroadphaseArrays bp;
   DevArray<int> counters;   // small device counters
-  int* h_counters = nullptr;  // pinned host mirror (4 counters + 128 colour counts)
+  int* h_counters = nullptr;  // pinned host mirror (4 counters, 128 colour counts, last round)
   int n_touch = 0, n_colors = 0, n_islands = 0;
   int color_rounds = 16;       // colouring rounds launched before the first readback
+  bool color_reuse = true;     // persisting contacts keep their previous colour
+  int scratch_colors = 0;      // colours of the last from-scratch colouring (0: none yet)
+  int reuse_rounds = 0;        // colouring rounds the last reusing step needed
   bool solver_graphs = true;   // replay the solve's per-colour launches as a captured HIP graph
   struct CachedGraph {
     std::vector<unsigned char> key;

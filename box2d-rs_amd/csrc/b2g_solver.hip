@@ -106,17 +106,28 @@ __global__ void __launch_bounds__(kBlock) k_compact(int n, const int* __restrict
 
 // Kernels over the touching contacts read their count from device memory (*ncp) so that the
 // host does not wait for the compaction; they are launched over the pair count, an upper bound.
+// Also looks up the solve colour of the contact's predecessor in the previous contact set
+// (old_colour null: no reuse).
 __global__ void __launch_bounds__(kBlock) k_constraint_bodies(const int* __restrict__ ncp, const int* __restrict__ list,
                                                               const int2* __restrict__ ab, const int* __restrict__ fbody,
                                                               const int* __restrict__ btype, int2* __restrict__ bodies,
-                                                              int* __restrict__ deg) {
+                                                              int* __restrict__ deg, const int* __restrict__ old_index,
+                                                              const int* __restrict__ old_colour,
+                                                              int* __restrict__ prev_colour) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= *ncp) return;
-  int2 f = ab[list[j]];
+  const int k = list[j];
+  int2 f = ab[k];
   int ba = fbody[f.x], bb = fbody[f.y];
   bodies[j] = make_int2(ba, bb);
   if (btype[ba] == 2) atomicAdd(&deg[ba], 1);
   if (btype[bb] == 2) atomicAdd(&deg[bb], 1);
+  int pc = -1;
+  if (old_colour) {
+    const int o = old_index[k];
+    if (o >= 0) pc = old_colour[o];
+  }
+  prev_colour[j] = pc >= 0 && pc < kMaxColors ? pc : -1;
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_adjacency(const int* __restrict__ ncp, const int2* __restrict__ bodies,
@@ -219,61 +230,147 @@ __device__ inline unsigned long long priority(int j) {
 
 // One Jones-Plassmann-Luby round: an uncoloured constraint whose priority beats every
 // uncoloured neighbour (a constraint sharing a dynamic body) takes the smallest colour no
-// coloured neighbour has. Neighbours coloured in this same round count as uncoloured.
+// coloured neighbour has. Neighbours coloured in this same round count as uncoloured, so the
+// decision depends only on the states at the start of the round (written by earlier launches):
+// whether a load sees a neighbour's new state or its old one makes no difference, and the
+// state loads and stores need no coherence beyond the launch boundary (workgroup-scope relaxed
+// atomics: plain cached accesses that the compiler may not tear or invent).
+// A neighbour o in state so: an uncoloured (or same-round) neighbour of higher priority makes
+// j wait; a coloured one marks its colour used.
+__device__ inline void color_look(int o, int so, int round, unsigned long long pj, bool& lose,
+                                  unsigned long long& used0, unsigned long long& used1) {
+  if (so == -1 || (so >> 8) == round) {
+    if (priority(o) > pj) lose = true;
+  } else {
+    const int c = so & 0xff;
+    if (c < 64) used0 |= 1ull << c;
+    else used1 |= 1ull << (c - 64);
+  }
+}
+// The first kColorUnroll adjacency entries of both bodies are loaded together and then all
+// their states together (two rounds of independent loads instead of a chain of dependent ones
+// per neighbour); longer lists finish in a loop. The outcome does not depend on the order in
+// which neighbours are looked at.
+static constexpr int kColorUnroll = 8;
+__device__ inline int color_pick(int j, int round, const int2* __restrict__ bodies, const int* __restrict__ btype,
+                                 const int* __restrict__ off, const int* __restrict__ adj, const int* state,
+                                 int* __restrict__ overflow) {
+  const unsigned long long pj = priority(j);
+  unsigned long long used0 = 0, used1 = 0;
+  bool lose = false;
+  const int2 b = bodies[j];
+  const bool dyn[2] = {btype[b.x] == 2, btype[b.y] == 2};
+  const int side[2] = {b.x, b.y};
+  int s0[2], e0[2];
+  for (int s = 0; s < 2; ++s) {
+    s0[s] = dyn[s] ? off[side[s]] : 0;
+    e0[s] = dyn[s] ? off[side[s] + 1] : 0;
+  }
+  int o[2 * kColorUnroll], so[2 * kColorUnroll];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int t = 0; t < kColorUnroll; ++t) o[s * kColorUnroll + t] = s0[s] + t < e0[s] ? adj[s0[s] + t] : j;
+#pragma unroll
+  for (int t = 0; t < 2 * kColorUnroll; ++t)
+    so[t] = o[t] != j ? __hip_atomic_load(&state[o[t]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
+#pragma unroll
+  for (int t = 0; t < 2 * kColorUnroll; ++t)
+    if (o[t] != j) color_look(o[t], so[t], round, pj, lose, used0, used1);
+  for (int s = 0; s < 2; ++s)
+    for (int e = s0[s] + kColorUnroll; e < e0[s]; ++e) {
+      const int oe = adj[e];
+      if (oe == j) continue;
+      color_look(oe, __hip_atomic_load(&state[oe], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), round, pj, lose,
+                 used0, used1);
+    }
+  if (lose) return -1;
+  if (~used0) return __ffsll(static_cast<long long>(~used0)) - 1;
+  if (~used1) return 64 + __ffsll(static_cast<long long>(~used1)) - 1;
+  atomicExch(overflow, 1);
+  return kMaxColors - 1;
+}
+// Round 0 of the colouring: a contact keeps its predecessor's colour unless a neighbour of
+// higher priority had the same colour (two contacts that persist keep their bodies, so they were
+// neighbours before and had different colours; the check only matters after the set changed
+// in other ways). Reads only prev_colour, so the outcome does not depend on the launch order.
+__global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ ncp, const int2* __restrict__ bodies,
+                                                       const int* __restrict__ btype, const int* __restrict__ off,
+                                                       const int* __restrict__ adj, const int* __restrict__ prev_colour,
+                                                       int* __restrict__ state, int* __restrict__ remaining) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  bool kept = false;
+  if (j < *ncp) {
+    const int pc = prev_colour[j];
+    if (pc >= 0) {
+      kept = true;
+      const unsigned long long pj = priority(j);
+      const int2 b = bodies[j];
+      const int side[2] = {b.x, b.y};
+      for (int s = 0; s < 2 && kept; ++s) {
+        if (btype[side[s]] != 2) continue;
+        for (int e = off[side[s]], end = off[side[s] + 1]; e < end; ++e) {
+          const int o = adj[e];
+          if (o != j && prev_colour[o] == pc && priority(o) > pj) {
+            kept = false;
+            break;
+          }
+        }
+      }
+    }
+    state[j] = kept ? pc : -1;  // round 0
+  }
+  const unsigned long long m = __ballot(kept);
+  if (m && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1) atomicSub(remaining, __popcll(m));
+}
+
 __global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ ncp, int round,
                                                         const int2* __restrict__ bodies, const int* __restrict__ btype,
                                                         const int* __restrict__ off, const int* __restrict__ adj,
                                                         int* state, int* __restrict__ remaining,
                                                         int* __restrict__ overflow) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= *ncp) return;
-  if (state[j] != -1) return;
-  const unsigned long long pj = priority(j);
-  unsigned long long used0 = 0, used1 = 0;
-  int2 b = bodies[j];
-  int side[2] = {b.x, b.y};
-  for (int s = 0; s < 2; ++s) {
-    int body = side[s];
-    if (btype[body] != 2) continue;
-    for (int e = off[body], end = off[body + 1]; e < end; ++e) {
-      int o = adj[e];
-      if (o == j) continue;
-      int so = __hip_atomic_load(&state[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (so == -1 || (so >> 8) == round) {
-        if (priority(o) > pj) return;
-      } else {
-        int c = so & 0xff;
-        if (c < 64) used0 |= 1ull << c;
-        else used1 |= 1ull << (c - 64);
-      }
-    }
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = -1;
+  if (j < *ncp && __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == -1) {
+    c = color_pick(j, round, bodies, btype, off, adj, state, overflow);
+    if (c >= 0) __hip_atomic_store(&state[j], (round << 8) | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
-  int c;
-  if (~used0) c = __ffsll(static_cast<long long>(~used0)) - 1;
-  else if (~used1) c = 64 + __ffsll(static_cast<long long>(~used1)) - 1;
-  else { atomicExch(overflow, 1); c = kMaxColors - 1; }
-  __hip_atomic_store(&state[j], (round << 8) | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  atomicSub(remaining, 1);
+  // one decrement of the shared counter per wave
+  const unsigned long long m = __ballot(c >= 0);
+  if (m && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1) atomicSub(remaining, __popcll(m));
 }
 
 // Colour histogram and scatter with block-aggregated atomics: ranks inside a block come from
 // LDS atomics and each block reserves one range per colour, so the ~10 global counters see one
 // atomic per block instead of one per constraint. The order inside a colour is free (its
 // constraints are independent), so no stable ranking is needed.
+// count[kMaxColors]: the last round in which a constraint was coloured
 __global__ void __launch_bounds__(kBlock) k_color_hist(const int* __restrict__ ncp, const int* __restrict__ state,
                                                        int* __restrict__ count) {
   __shared__ int h[kMaxColors];
+  __shared__ int last_round;
   for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) h[c] = 0;
+  if (threadIdx.x == 0) last_round = 0;
   __syncthreads();
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < *ncp) atomicAdd(&h[state[j] & 0xff], 1);
+  if (j < *ncp) {
+    const int st = state[j];
+    if (st >= 0) {  // uncoloured constraints (-1) are not counted
+      atomicAdd(&h[st & 0xff], 1);
+      atomicMax(&last_round, st >> 8);
+    }
+  }
   __syncthreads();
   for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
     if (h[c]) atomicAdd(&count[c], h[c]);
+  if (threadIdx.x == 0 && last_round > 0) atomicMax(&count[kMaxColors], last_round);
 }
 
-__global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __restrict__ state, const int* __restrict__ off,
-                                                          int* __restrict__ cursor, int* __restrict__ order) {
+// map: per colour id its base offset in the colour order ([0, kMaxColors)) and its launch
+// index ([kMaxColors, 2 kMaxColors)); the launch index is stored per pair for the next step.
+__global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __restrict__ state, const int* __restrict__ map,
+                                                          int* __restrict__ cursor, int* __restrict__ order,
+                                                          const int* __restrict__ list, int* __restrict__ pair_colour) {
   __shared__ int h[kMaxColors];
   __shared__ int base[kMaxColors];
   for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) h[c] = 0;
@@ -286,9 +383,12 @@ __global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __re
   }
   __syncthreads();
   for (int k = threadIdx.x; k < kMaxColors; k += blockDim.x)
-    if (h[k]) base[k] = off[k] + atomicAdd(&cursor[k], h[k]);
+    if (h[k]) base[k] = map[k] + atomicAdd(&cursor[k], h[k]);
   __syncthreads();
-  if (c >= 0) order[base[c] + r] = j;
+  if (c >= 0) {
+    order[base[c] + r] = j;
+    pair_colour[list[j]] = map[kMaxColors + c];
+  }
 }
 
 // ---- constraint setup ---------------------------------------------------------------------------
@@ -639,9 +739,8 @@ struct PosArgs {
 __device__ inline void solve_position_one(int i, bool live, int begin, const PosArgs& a) {
   // inactive lanes stay in the wave (minSep = 0) for the island reduction at the end
   int4 info = live ? a.info[i] : make_int4(0, 0, 0, 0);
-  // island flags are read coherently (agent scope): another XCD may have written them
-  const bool active =
-      live && __hip_atomic_load(&a.done[info.w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  // island flags are written by earlier launches only
+  const bool active = live && a.done[info.w] == 0;
   if (!active) info.y = 0;
   int2 bd = active ? a.bodies[i] : make_int2(0, 0);
   const int ii = active ? i : begin;
@@ -879,12 +978,17 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   if ((st = reserve_solver(w, n_pairs))) return st;
   if ((st = s.deg.reserve(nb + 1)) || (st = s.adj_off.reserve(nb + 1)) || (st = s.adj_cursor.reserve(nb)) ||
       (st = s.parent.reserve(nb)) || (st = s.isl_minsep.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
-      (st = s.color_count.reserve(kMaxColors + 1)) || (st = s.color_off.reserve(kMaxColors + 1)) ||
-      (st = s.color_cursor.reserve(kMaxColors)))
+      (st = s.color_count.reserve(kMaxColors + 1)) || (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
+      (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(std::max(n_pairs, 1))))
     return st;
   if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
   hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(1), 0, w->stream, cnt);
   const int gp = blocks_for(std::max(n_pairs, 1), kBlock);
+  // Colour reuse (persisting contacts keep their colour) lets the colour count creep up as new
+  // contacts take first-fit colours around kept ones; once it exceeds the last from-scratch
+  // count by kColorSlack, the step colours from scratch again.
+  constexpr int kColorSlack = 2;
+  const bool reuse = w->color_reuse && w->scratch_colors > 0 && w->n_colors <= w->scratch_colors + kColorSlack;
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_compact, dim3(gp), dim3(kBlock), 0, w->stream, n_pairs, s.touch_flag.p, s.touch_scan.p,
                        s.touch_list.p);
@@ -893,8 +997,11 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   B2G_HIP(hipMemsetAsync(s.adj_cursor.p, 0, sizeof(int) * nb, w->stream));
   hipLaunchKernelGGL(k_iota, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p);
   if (n_pairs > 0) {
+    // colours of the previous step's contacts (the old set's solve wrote them)
+    const int* old_colour = reuse ? w->c[1 - w->cur].colour.p : nullptr;
     hipLaunchKernelGGL(k_constraint_bodies, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.touch_list.p, c.ab.p,
-                       w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p);
+                       w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p, c.old_index.p, old_colour,
+                       s.prev_colour.p);
     hipLaunchKernelGGL(k_union, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p, s.parent.p);
   }
   hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
@@ -904,12 +1011,16 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     if ((st = exclusive_scan(w, s.deg.p, s.adj_off.p, nb + 1, nullptr))) return st;
     hipLaunchKernelGGL(k_fill_adjacency, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
                        s.adj_off.p, s.adj_cursor.p, s.adj.p);
-    B2G_HIP(hipMemsetAsync(s.color_state.p, 0xff, sizeof(int) * n_pairs, w->stream));
-    // Jones-Plassmann-Luby rounds: a first batch without waiting (rounds after the last
-    // constraint is coloured find nothing to do), then more only if some remain
-    int round = 0;
+    // round 0: contacts that persist keep their colour; then Jones-Plassmann-Luby rounds for the
+    // others, a first batch without waiting (rounds after the last constraint is coloured find
+    // nothing to do), then more only if some remain
+    hipLaunchKernelGGL(k_color_keep, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
+                       s.adj_off.p, s.adj.p, s.prev_colour.p, s.color_state.p, cnt + 1);
+    // first batch: from scratch the configured count; with reuse one more than the rounds the
+    // last reusing step needed (the batch size does not change the colouring)
+    int round = 1;
     for (;;) {
-      int batch = round == 0 ? w->color_rounds : 4;
+      int batch = round > 1 ? 4 : reuse ? std::max(2, std::min(w->color_rounds, w->reuse_rounds + 1)) : w->color_rounds;
       for (int r = 0; r < batch; ++r, ++round) {
         launch_timed(w, B2G_KERNEL_COLOR_ROUND, n_pairs, k_color_round, dim3(gp), dim3(kBlock), cnt, round,
                      s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
@@ -917,7 +1028,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       B2G_HIP(hipMemsetAsync(s.color_count.p, 0, sizeof(int) * (kMaxColors + 1), w->stream));
       hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, s.color_count.p);
       B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, 4 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
-      B2G_HIP(hipMemcpyAsync(w->h_counters + 4, s.color_count.p, sizeof(int) * kMaxColors, hipMemcpyDeviceToHost,
+      B2G_HIP(hipMemcpyAsync(w->h_counters + 4, s.color_count.p, sizeof(int) * (kMaxColors + 1), hipMemcpyDeviceToHost,
                              w->stream));
       B2G_HIP(hipStreamSynchronize(w->stream));
       if ((st = kt_harvest(w))) return st;
@@ -927,7 +1038,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
         return B2G_ERR_DEVICE;
       }
     }
-    w->color_rounds_used = round;
+    w->color_rounds_used = round - 1;
+    if (reuse) w->reuse_rounds = w->h_counters[4 + kMaxColors];
     nc = w->h_counters[0];
     if (w->h_counters[2]) {
       set_error("colouring needs more than 127 colours (a dynamic body with too many contacts)");
@@ -936,21 +1048,28 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   }
   w->n_touch = nc;
   if (nc > 0) {
+    // One launch per non-empty colour, in colour-id order (kept colours can leave gaps).
+    // Device copy: launch offsets (kMaxColors + 1, padded with nc), then per colour id its base
+    // offset and its launch index.
     const int* hc = w->h_counters + 4;
+    std::vector<int>& offs = w->h_color_off_dev;  // outlives the asynchronous copy
+    offs.assign(3 * kMaxColors + 1, 0);
+    int* base = offs.data() + kMaxColors + 1;
+    int* launch = base + kMaxColors;
     w->h_color_off.assign(1, 0);
     for (int i = 0; i < kMaxColors; ++i) {
-      if (hc[i] == 0) break;  // first-fit colours are contiguous from 0
-      ncolors = i + 1;
+      base[i] = w->h_color_off.back();
+      launch[i] = ncolors;
+      if (hc[i] == 0) continue;
+      ++ncolors;
       w->h_color_off.push_back(w->h_color_off.back() + hc[i]);
     }
-    std::vector<int>& offs = w->h_color_off_dev;  // outlives the asynchronous copy
-    offs.assign(kMaxColors + 1, nc);
-    for (int i = 0; i <= ncolors; ++i) offs[i] = w->h_color_off[i];
-    B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * (kMaxColors + 1), hipMemcpyHostToDevice,
-                           w->stream));
+    for (int i = 0; i <= kMaxColors; ++i) offs[i] = i <= ncolors ? w->h_color_off[i] : nc;
+    B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
     B2G_HIP(hipMemsetAsync(s.color_cursor.p, 0, sizeof(int) * kMaxColors, w->stream));
     hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
-                       s.color_state.p, s.color_off.p, s.color_cursor.p, s.order.p);
+                       s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.touch_list.p,
+                       c.colour.p);
     InitArgs ia;
     ia.n = nc;
     ia.order = s.order.p;
@@ -983,6 +1102,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                  ia);
   }
   w->n_colors = ncolors;
+  if (!reuse && ncolors > 0) w->scratch_colors = ncolors;
   VelArgs va{s.k_bodies.p, s.k_info.p, s.vnf.p, s.vm.p, s.vr0.p, s.vr1.p, s.vms.p, s.vbias.p, s.K.p, s.NM.p, s.lam.p,
              w->b.vel.p};
   PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,

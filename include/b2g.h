@@ -241,9 +241,14 @@ enum {
  *   B2G_TUNE_SOLVER_GRAPH   1: the solve's per-colour launches (warm start, velocity and position
  *                           iterations) replay as a captured HIP graph, cached per colour count,
  *                           grid size and buffers; 0: launched one by one (default 1; timing a
- *                           kernel with b2g_world_time_kernel also launches them one by one) */
+ *                           kernel with b2g_world_time_kernel also launches them one by one)
+ *   B2G_TUNE_COLOR_REUSE    1: a contact that persists from the previous step keeps its solve
+ *                           colour and only new contacts are coloured; 0: every step colours all
+ *                           touching contacts from scratch (default 1). Either way the colouring is
+ *                           deterministic; the colour order it gives is the Gauss-Seidel order. */
 #define B2G_TUNE_COLOR_ROUNDS 2
 #define B2G_TUNE_SOLVER_GRAPH 3
+#define B2G_TUNE_COLOR_REUSE 4
 int b2g_world_tune(b2g_world* world, int32_t param, int32_t value);
 
 /* Measurement hook with no counterpart in the reference (its B2Profile times whole phases on

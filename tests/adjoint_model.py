@@ -37,6 +37,7 @@ class HostAdjoint:
                                          ctypes.c_void_p]
         self.lib.ac_forward.argtypes = [ctypes.c_void_p] * 4
         self.lib.ac_backward.argtypes = [ctypes.c_void_p] * 6
+        self.lib.ac_mismatch_bodies.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         bd = scene.body_defs
         self.nb, self.nf = len(bd), len(scene.fixture_defs)
         m = np.zeros((self.nb, 8))
@@ -72,6 +73,13 @@ class HostAdjoint:
         out = np.zeros((self.nb, 6))
         mism = self.lib.ac_forward(self.h, _p(s0), _p(fr), _p(out))
         return out, mism
+
+    def mismatch_bodies(self):
+        """Bool per body: in a slot whose recomputed manifold (type, count or feature ids)
+        differed from the recording in the last forward or backward pass."""
+        flags = np.zeros(self.nb, np.int32)
+        self.lib.ac_mismatch_bodies(self.h, _p(flags))
+        return flags != 0
 
     def backward(self, state0, cot, friction=None):
         s0 = np.ascontiguousarray(state0, np.float64)

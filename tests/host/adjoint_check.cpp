@@ -37,6 +37,7 @@ struct Model {
   std::vector<double> lam0;    // 4 per slot
   std::vector<double> store;   // 4 per slot, written by the forward model
   int mismatches = 0;
+  std::vector<int> mismatch_body;  // per body: 1 if a slot of it mismatched in the last pass
 };
 
 TContactConst<double> contact_const(const Model& m, const int* sl) {
@@ -98,7 +99,10 @@ void forward_step(Model& m, const Step& st, std::vector<double>& q, std::vector<
     GeomFn<double> f{&m.shape[sl[2]], &m.shape[sl[3]], la, lb, &hdr, id};
     double x[6] = {q[3 * sl[0]], q[3 * sl[0] + 1], q[3 * sl[0] + 2], q[3 * sl[1]], q[3 * sl[1] + 1], q[3 * sl[1] + 2]};
     f(x, &t.geom[8 * i]);
-    if (hdr != sl[4] || id[0] != uint32_t(sl[8]) || id[1] != uint32_t(sl[9])) ++m.mismatches;
+    if (hdr != sl[4] || id[0] != uint32_t(sl[8]) || id[1] != uint32_t(sl[9])) {
+      ++m.mismatches;
+      m.mismatch_body[sl[0]] = m.mismatch_body[sl[1]] = 1;
+    }
     t.mu[i] = std::sqrt(m.mat[3 * sl[2]] * m.mat[3 * sl[3]]);
   }
   for (int b = 0; b < m.nb; ++b)
@@ -419,6 +423,7 @@ int ac_forward(void* h, const double* state0, const double* fric, double* final_
   Model* m = static_cast<Model*>(h);
   for (int f = 0; f < m->nf; ++f) m->mat[3 * f] = fric[f];
   m->mismatches = 0;
+  m->mismatch_body.assign(m->nb, 0);
   std::vector<double> q, v;
   split(state0, m->nb, q, v);
   StepTape t;
@@ -433,6 +438,7 @@ int ac_backward(void* h, const double* state0, const double* fric, const double*
   Model* m = static_cast<Model*>(h);
   for (int f = 0; f < m->nf; ++f) m->mat[3 * f] = fric[f];
   m->mismatches = 0;
+  m->mismatch_body.assign(m->nb, 0);
   const int N = int(m->steps.size());
   std::vector<std::vector<double>> qs(N), vs(N);
   std::vector<double> q, v;
@@ -454,6 +460,13 @@ int ac_backward(void* h, const double* state0, const double* fric, const double*
   join(grad0, m->nb, qb, vb);
   for (int f = 0; f < m->nf; ++f) grad_fric[f] = fg[f];
   return m->mismatches;
+}
+
+// bodies of the slots whose recomputed manifold differed from the recording in the last
+// forward or backward pass (flags: one int per body)
+void ac_mismatch_bodies(void* h, int* flags) {
+  const Model* m = static_cast<const Model*>(h);
+  for (int b = 0; b < m->nb; ++b) flags[b] = b < int(m->mismatch_body.size()) ? m->mismatch_body[b] : 0;
 }
 
 }  // extern "C"
