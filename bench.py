@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 # Roofline kernel: k_traverse, the LBVH pair search, the longest launch of a step (one launch,
 # under a tenth of a millisecond on this workload). The solver's per-colour kernels take more
 # time in total but each launch lasts 4-11 us, so an event pair around one of them mostly
-# measures its dispatch; they run from a captured graph in the timed region.
+# measures its dispatch.
 # Algorithmic HBM bytes of one traversal launch (f32/i32 sizes, each read or written once): per
 # query its packed leaf record (proxy, body, filter bits, group/shape type/body type: 16 B), its
 # fat AABB (16 B), its world (4 B) and its pair count (4 B): 40 B; per pair found the partner's
@@ -231,9 +231,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint"))
     ap.add_argument("--roofline-kernel", default="traverse", choices=sorted(UNIT_BYTES))
-    ap.add_argument("--solver-graph", type=int, default=1, choices=(0, 1),
-                    help="replay the solve's per-colour launches from a captured graph (default 1; profilers "
-                         "that intercept graph capture need 0)")
+    ap.add_argument("--solver-graph", type=int, default=0, choices=(0, 1),
+                    help="replay the solve's per-colour launches from a captured graph (default 0: direct "
+                         "launches are faster on MI355X; timing a solve kernel launches the solve directly)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a single world or a batch of worlds is stepped on one "

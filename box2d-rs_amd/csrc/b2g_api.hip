@@ -451,7 +451,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.color_state); rel(s.color_count); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.parent);
   rel(s.isl_minsep); rel(s.isl_done);
   BroadphaseArrays& bp = w->bp;
-  rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes);
+  rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes); rel(bp.nodes4);
   rel(bp.leafrec); rel(bp.count); rel(bp.slots); rel(bp.bounds); rel(bp.scan_tmp);
   rel(w->counters);
   if (w->h_counters) (void)hipHostFree(w->h_counters);

@@ -264,6 +264,47 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(int n, const unsigned* __
   nodes[i] = nd;
 }
 
+// Internal node i as a 4-wide node: each binary child that is internal is replaced by its two
+// children (their boxes and ranges come from the child's record); a leaf child stays as one slot.
+__global__ void __launch_bounds__(kBlock) k_bvh4_collapse(int n_internal, const BvhNode* __restrict__ nodes,
+                                                          Bvh4Node* __restrict__ nodes4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_internal) return;
+  const BvhNode nd = nodes[i];
+  Bvh4Node out;
+  const int link2[2] = {nd.link.x, nd.link.y};
+  const int last2[2] = {nd.link.z, nd.link.w};
+  const int wlo2[2] = {nd.worlds.x, nd.worlds.z}, whi2[2] = {nd.worlds.y, nd.worlds.w};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (link2[s] < 0) {
+      out.box[2 * s] = nd.box[s];
+      out.link[2 * s] = link2[s];
+      out.last[2 * s] = last2[s];
+      out.wlo[2 * s] = wlo2[s];
+      out.whi[2 * s] = whi2[s];
+      out.box[2 * s + 1] = make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
+      out.link[2 * s + 1] = 0;
+      out.last[2 * s + 1] = -1;
+      out.wlo[2 * s + 1] = 1;
+      out.whi[2 * s + 1] = 0;
+    } else {
+      const BvhNode ch = nodes[link2[s]];
+      out.box[2 * s] = ch.box[0];
+      out.box[2 * s + 1] = ch.box[1];
+      out.link[2 * s] = ch.link.x;
+      out.link[2 * s + 1] = ch.link.y;
+      out.last[2 * s] = ch.link.z;
+      out.last[2 * s + 1] = ch.link.w;
+      out.wlo[2 * s] = ch.worlds.x;
+      out.whi[2 * s] = ch.worlds.y;
+      out.wlo[2 * s + 1] = ch.worlds.z;
+      out.whi[2 * s + 1] = ch.worlds.w;
+    }
+  }
+  nodes4[i] = out;
+}
+
 // ---- pair search --------------------------------------------------------------------------
 struct PairFilter {
   const int* __restrict__ fbody;
@@ -293,7 +334,7 @@ struct PairFilter {
 
 static constexpr int kTravBlock = 64;
 static constexpr int kStack = 64;
-static constexpr int kSlots = 32;
+static constexpr int kSlots = 64;
 
 // PairFilter::ok on two leaf records of one world (the node world ranges have already matched
 // the worlds): same body, at least one dynamic body, the default contact filter, and a contact
@@ -314,7 +355,7 @@ __device__ inline bool leaf_pair_ok(int4 a, int4 b) {
 
 struct TreeView {
   int n;                       // leaves in the tree
-  const BvhNode* nodes;        // internal nodes
+  const Bvh4Node* nodes;       // internal nodes as 4-wide nodes
   const int4* leafrec;         // sorted leaf records
   const int* loose_off;        // per world: range of big static proxies kept out of the tree
   const int* loose;
@@ -324,9 +365,9 @@ struct TreeView {
 // One tree walk per tree proxy, thread k = sorted leaf k (neighbouring threads walk
 // neighbouring paths). The emitter of a pair of tree proxies is the one with the lower sorted
 // position, so a walk skips every child whose leaves all sit at or before its own position:
-// the last leaf under each child is in the node record. The node record also holds both child
-// boxes, so a visited node is one 64-byte load and a leaf is tested from its parent's record
-// without a load of its own. Then the world's big static proxies (outside the tree) are tested
+// the last leaf under each slot is in the node record. A visited node is the 4-wide record of a
+// binary node (its grandchildren's boxes), so the walk descends two levels per 128-byte load, and
+// a leaf is tested from its parent's record without a load of its own. Then the world's big static proxies (outside the tree) are tested
 // directly; every tree proxy emits its pairs with them. First pass: the first kSlots pairs go
 // to the query's slots and the count is recorded. The overflow pass re-walks only queries with
 // more than kSlots pairs and writes their whole segment; the walk order is deterministic, so
@@ -359,17 +400,18 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const float4* _
     stack[sp++][threadIdx.x] = 0;
     while (sp > 0) {
       const int node = stack[--sp][threadIdx.x];
-      const BvhNode nd = tv.nodes[node];
-      // right child first: an internal left child is pushed last and walked first
-      for (int s = 1; s >= 0; --s) {
-        if ((s ? nd.link.w : nd.link.z) <= k) continue;
-        if (tv.batched && (qw < (s ? nd.worlds.z : nd.worlds.x) || qw > (s ? nd.worlds.w : nd.worlds.y))) continue;
+      const Bvh4Node nd = tv.nodes[node];
+      // last slot first: internal slots are pushed so that the leftmost is walked first
+#pragma unroll
+      for (int s = 3; s >= 0; --s) {
+        if (nd.last[s] <= k) continue;
+        if (tv.batched && (qw < nd.wlo[s] || qw > nd.whi[s])) continue;
         const float4 cb = nd.box[s];
         Aabb b;
         b.lo = v2(cb.x, cb.y);
         b.hi = v2(cb.z, cb.w);
         if (!overlap(b, qb)) continue;
-        const int ch = s ? nd.link.y : nd.link.x;
+        const int ch = nd.link[s];
         if (ch < 0) {
           const int4 pr = tv.leafrec[~ch];
           if (leaf_pair_ok(qr, pr)) emit(pr.x);
@@ -445,6 +487,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   }
   if ((st = bp.keys.reserve(std::max(nt, 1))) || (st = bp.vals.reserve(std::max(nt, 1))) ||
       (st = bp.seg.reserve(total_nodes)) || (st = bp.nodes.reserve(std::max(nt, 1))) ||
+      (st = bp.nodes4.reserve(std::max(nt, 1))) ||
       (st = bp.leafrec.reserve(std::max(nt, 1))) ||
       (st = bp.count.reserve(np + 1)) || (st = bp.slots.reserve(size_t(np) * kSlots)) ||
       (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(2)))
@@ -467,12 +510,15 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     for (int L0 = 0; L0 == 0 || L0 < lv.count - 1; L0 += kReduceLevels)
       hipLaunchKernelGGL(k_seg_reduce, dim3(blocks_for(lv.size[L0], kBlock)), dim3(kBlock), 0, w->stream, lv, L0,
                          bp.vals.p, fat, bp.seg.p, li, bp.leafrec.p);
-    if (nt > 1)
+    if (nt > 1) {
       hipLaunchKernelGGL(k_lbvh_build, dim3(blocks_for(nt - 1, kBlock)), dim3(kBlock), 0, w->stream, nt, bp.keys.p,
                          2 * mbits, lv, bp.seg.p, bp.nodes.p);
+      hipLaunchKernelGGL(k_bvh4_collapse, dim3(blocks_for(nt - 1, kBlock)), dim3(kBlock), 0, w->stream, nt - 1,
+                         bp.nodes.p, bp.nodes4.p);
+    }
   }
   PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p};
-  TreeView tv{nt, bp.nodes.p, bp.leafrec.p, w->f.loose_off.p, w->f.loose.p, wbits > 0};
+  TreeView tv{nt, bp.nodes4.p, bp.leafrec.p, w->f.loose_off.p, w->f.loose.p, wbits > 0};
   // tree proxies walk in Morton order; proxies outside the tree are static and emit nothing
   if (nt < np) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
   if (nt > 0) {

@@ -146,12 +146,23 @@ struct alignas(16) BvhNode {
   int4 worlds;  // {left first, left last, right first, right last}
 };
 
+// The node a walk visits: a binary node's (up to) four grandchildren, so a walk descends two
+// levels per visited node. 128 bytes: one cache line. Unused slots have an empty box and
+// last = -1; a leaf in the place of a child fills one slot.
+struct alignas(16) Bvh4Node {
+  float4 box[4];
+  int link[4];  // binary internal node index, or ~sorted position for a leaf
+  int last[4];  // last sorted leaf under the slot
+  int wlo[4], whi[4];  // world range under the slot
+};
+
 struct BroadphaseArrays {
   DevArray<unsigned> keys, keys_alt;
   DevArray<unsigned> vals, vals_alt;
   DevArray<unsigned> hist;       // radix histograms
   DevArray<float4> seg;          // segment tree over the sorted leaf boxes (level 0 = leaves)
   DevArray<BvhNode> nodes;       // LBVH internal nodes
+  DevArray<Bvh4Node> nodes4;     // per internal node: its grandchildren (the walk's nodes)
   DevArray<int4> leafrec;        // per sorted leaf: proxy, body, filter bits, group | shape type | body type
   DevArray<int> count;           // pairs per proxy
   DevArray<int> slots;           // first pairs of each query, column-major (slot j of query k at j * n + k)
@@ -249,7 +260,7 @@ struct World {
   bool color_reuse = true;     // persisting contacts keep their previous colour
   int scratch_colors = 0;      // colours of the last from-scratch colouring (0: none yet)
   int reuse_rounds = 0;        // colouring rounds the last reusing step needed
-  bool solver_graphs = true;   // replay the solve's per-colour launches as a captured HIP graph
+  bool solver_graphs = false;  // replay the solve's per-colour launches as a captured HIP graph
   struct CachedGraph {
     std::vector<unsigned char> key;
     hipGraphExec_t exec = nullptr;

@@ -307,16 +307,28 @@ __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ n
       const unsigned long long pj = priority(j);
       const int2 b = bodies[j];
       const int side[2] = {b.x, b.y};
-      for (int s = 0; s < 2 && kept; ++s) {
-        if (btype[side[s]] != 2) continue;
-        for (int e = off[side[s]], end = off[side[s] + 1]; e < end; ++e) {
-          const int o = adj[e];
-          if (o != j && prev_colour[o] == pc && priority(o) > pj) {
-            kept = false;
-            break;
-          }
-        }
+      int s0[2], e0[2];
+      for (int s = 0; s < 2; ++s) {
+        const bool dyn = btype[side[s]] == 2;
+        s0[s] = dyn ? off[side[s]] : 0;
+        e0[s] = dyn ? off[side[s] + 1] : 0;
       }
+      // as in color_pick: the first kColorUnroll entries per body with independent loads
+      int o[2 * kColorUnroll], po[2 * kColorUnroll];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < kColorUnroll; ++t) o[s * kColorUnroll + t] = s0[s] + t < e0[s] ? adj[s0[s] + t] : j;
+#pragma unroll
+      for (int t = 0; t < 2 * kColorUnroll; ++t) po[t] = o[t] != j ? prev_colour[o[t]] : -1;
+#pragma unroll
+      for (int t = 0; t < 2 * kColorUnroll; ++t)
+        if (po[t] == pc && priority(o[t]) > pj) kept = false;
+      for (int s = 0; s < 2; ++s)
+        for (int e = s0[s] + kColorUnroll; e < e0[s]; ++e) {
+          const int oe = adj[e];
+          if (oe != j && prev_colour[oe] == pc && priority(oe) > pj) kept = false;
+        }
     }
     state[j] = kept ? pc : -1;  // round 0
   }
@@ -737,19 +749,21 @@ struct PosArgs {
 // Called by every lane of a wave (the island reduction at the end uses the whole wave); lanes
 // past the colour's end pass live = false and stay inactive.
 __device__ inline void solve_position_one(int i, bool live, int begin, const PosArgs& a) {
-  // inactive lanes stay in the wave (minSep = 0) for the island reduction at the end
-  int4 info = live ? a.info[i] : make_int4(0, 0, 0, 0);
-  // island flags are written by earlier launches only
-  const bool active = live && a.done[info.w] == 0;
-  if (!active) info.y = 0;
-  int2 bd = active ? a.bodies[i] : make_int2(0, 0);
-  const int ii = active ? i : begin;
+  // Inactive lanes stay in the wave (minSep = 0) for the island reduction at the end; they
+  // read the colour's first constraint and write nothing. All loads are issued up front: the
+  // constraint's records, then the body positions and the island flag (written by earlier
+  // launches only) side by side.
+  const int ii = live ? i : begin;
+  int4 info = a.info[ii];
+  const int2 bd = a.bodies[ii];
   float4 m = a.vm[ii];
   float4 lp = a.plp[ii], g = a.pgeo[ii], lc = a.plc[ii];
   float2 rad = a.prad[ii];
+  float4 PA = a.pos[bd.x], PB = a.pos[bd.y];
+  const bool active = live && a.done[info.w] == 0;
+  if (!active) info.y = 0;
   const float mA = m.x, iA = m.y, mB = m.z, iB = m.w;
   V2 lcA = v2(lc.x, lc.y), lcB = v2(lc.z, lc.w);
-  float4 PA = a.pos[bd.x], PB = a.pos[bd.y];
   V2 cA = v2(PA.x, PA.y), cB = v2(PB.x, PB.y);
   float aA = PA.z, aB = PB.z;
   const int type = info.x & 0xff;
@@ -1108,7 +1122,10 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
              s.isl_done.p};
   const std::vector<int>& co = w->h_color_off;
-  if (nc > 0 && w->solver_graphs && w->kt.kernel == B2G_KERNEL_NONE) {
+  // (graph nodes cannot carry the timing events of a timed solve kernel)
+  const bool timed_tail = w->kt.kernel == B2G_KERNEL_WARM_START || w->kt.kernel == B2G_KERNEL_SOLVE_VELOCITY ||
+                          w->kt.kernel == B2G_KERNEL_SOLVE_POSITION;
+  if (nc > 0 && w->solver_graphs && !timed_tail) {
     // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
     int maxc = 0;
     for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);

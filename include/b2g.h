@@ -240,8 +240,10 @@ enum {
  *   B2G_TUNE_COLOR_ROUNDS   colouring rounds launched before the first read-back (default 16)
  *   B2G_TUNE_SOLVER_GRAPH   1: the solve's per-colour launches (warm start, velocity and position
  *                           iterations) replay as a captured HIP graph, cached per colour count,
- *                           grid size and buffers; 0: launched one by one (default 1; timing a
- *                           kernel with b2g_world_time_kernel also launches them one by one)
+ *                           grid size and buffers; 0: launched one by one (default 0: on MI355X
+ *                           the stream of direct launches keeps the GPU busier than the graph's
+ *                           replay; timing a kernel with b2g_world_time_kernel also launches them
+ *                           one by one)
  *   B2G_TUNE_COLOR_REUSE    1: a contact that persists from the previous step keeps its solve
  *                           colour and only new contacts are coloured; 0: every step colours all
  *                           touching contacts from scratch (default 1). Either way the colouring is
