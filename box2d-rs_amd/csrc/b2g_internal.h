@@ -260,6 +260,7 @@ struct World {
   bool color_reuse = true;     // persisting contacts keep their previous colour
   int scratch_colors = 0;      // colours of the last from-scratch colouring (0: none yet)
   int reuse_rounds = 0;        // colouring rounds the last reusing step needed
+  int steps_since_scratch = 0; // solves since the last from-scratch colouring
   bool solver_graphs = false;  // replay the solve's per-colour launches as a captured HIP graph
   struct CachedGraph {
     std::vector<unsigned char> key;

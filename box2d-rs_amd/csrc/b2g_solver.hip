@@ -294,15 +294,17 @@ __device__ inline int color_pick(int j, int round, const int2* __restrict__ bodi
 // higher priority had the same colour (two contacts that persist keep their bodies, so they were
 // neighbours before and had different colours; the check only matters after the set changed
 // in other ways). Reads only prev_colour, so the outcome does not depend on the launch order.
+// Colours at or above `cap` are not kept: those contacts are coloured again first-fit, which
+// pulls the colour count back down as the contact graph changes.
 __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                        const int* __restrict__ btype, const int* __restrict__ off,
                                                        const int* __restrict__ adj, const int* __restrict__ prev_colour,
-                                                       int* __restrict__ state, int* __restrict__ remaining) {
+                                                       int cap, int* __restrict__ state, int* __restrict__ remaining) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   bool kept = false;
   if (j < *ncp) {
     const int pc = prev_colour[j];
-    if (pc >= 0) {
+    if (pc >= 0 && pc < cap) {
       kept = true;
       const unsigned long long pj = priority(j);
       const int2 b = bodies[j];
@@ -999,10 +1001,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(1), 0, w->stream, cnt);
   const int gp = blocks_for(std::max(n_pairs, 1), kBlock);
   // Colour reuse (persisting contacts keep their colour) lets the colour count creep up as new
-  // contacts take first-fit colours around kept ones; once it exceeds the last from-scratch
-  // count by kColorSlack, the step colours from scratch again.
-  constexpr int kColorSlack = 2;
-  const bool reuse = w->color_reuse && w->scratch_colors > 0 && w->n_colors <= w->scratch_colors + kColorSlack;
+  // contacts take first-fit colours around kept ones; the step colours from scratch again once
+  // the count exceeds the last from-scratch count by kColorSlack, and at least every
+  // kColorRefresh steps (which keeps that count, the cap on kept colours, current).
+  constexpr int kColorSlack = 2, kColorRefresh = 32;
+  const bool reuse = w->color_reuse && w->scratch_colors > 0 && w->n_colors <= w->scratch_colors + kColorSlack &&
+                     w->steps_since_scratch < kColorRefresh;
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_compact, dim3(gp), dim3(kBlock), 0, w->stream, n_pairs, s.touch_flag.p, s.touch_scan.p,
                        s.touch_list.p);
@@ -1029,7 +1033,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // others, a first batch without waiting (rounds after the last constraint is coloured find
     // nothing to do), then more only if some remain
     hipLaunchKernelGGL(k_color_keep, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
-                       s.adj_off.p, s.adj.p, s.prev_colour.p, s.color_state.p, cnt + 1);
+                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, s.color_state.p, cnt + 1);
     // first batch: from scratch the configured count; with reuse one more than the rounds the
     // last reusing step needed (the batch size does not change the colouring)
     int round = 1;
@@ -1116,7 +1120,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                  ia);
   }
   w->n_colors = ncolors;
-  if (!reuse && ncolors > 0) w->scratch_colors = ncolors;
+  if (!reuse && ncolors > 0) {
+    w->scratch_colors = ncolors;
+    w->steps_since_scratch = 0;
+  } else {
+    ++w->steps_since_scratch;
+  }
   VelArgs va{s.k_bodies.p, s.k_info.p, s.vnf.p, s.vm.p, s.vr0.p, s.vr1.p, s.vms.p, s.vbias.p, s.K.p, s.NM.p, s.lam.p,
              w->b.vel.p};
   PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
