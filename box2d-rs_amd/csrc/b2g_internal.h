@@ -133,8 +133,9 @@ struct SolverArrays {
                                                        // its base offset and its launch index
   DevArray<int> prev_colour;                    // colour of the contact's predecessor, or -1
   DevArray<int> order;                          // colour-sorted constraint -> compacted index
+  DevArray<int> inv_order;                      // compacted index -> colour-sorted constraint
   // islands
-  DevArray<int> parent, isl_minsep, isl_done;
+  DevArray<int> parent, isl_violated, isl_done;
 };
 
 // One internal LBVH node as a walk sees it: both child boxes, the children (internal node

@@ -11,6 +11,7 @@
 // non-static bodies (union-find); they carry the reference's per-island early exit of the
 // position iterations. Colouring is Jones-Plassmann-Luby with hash priorities and first-fit
 // colours, so it is deterministic. Body state is SoA float4 in HBM.
+#include <climits>
 #include <cstring>
 
 #include "b2g_internal.h"
@@ -205,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, c
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* __restrict__ minsep,
+__global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* __restrict__ violated,
                                                      int* __restrict__ done) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* _
   int r = b;
   for (int p = parent[r]; p != r; p = parent[r]) r = p;
   parent[b] = r;
-  minsep[b] = float_flip(0.0f);
+  violated[b] = 0;
   done[b] = 0;
 }
 
@@ -384,7 +385,8 @@ __global__ void __launch_bounds__(kBlock) k_color_hist(const int* __restrict__ n
 // index ([kMaxColors, 2 kMaxColors)); the launch index is stored per pair for the next step.
 __global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __restrict__ state, const int* __restrict__ map,
                                                           int* __restrict__ cursor, int* __restrict__ order,
-                                                          const int* __restrict__ list, int* __restrict__ pair_colour) {
+                                                          int* __restrict__ inv_order, const int* __restrict__ list,
+                                                          int* __restrict__ pair_colour) {
   __shared__ int h[kMaxColors];
   __shared__ int base[kMaxColors];
   for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) h[c] = 0;
@@ -401,6 +403,7 @@ __global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __re
   __syncthreads();
   if (c >= 0) {
     order[base[c] + r] = j;
+    inv_order[j] = base[c] + r;
     pair_colour[list[j]] = map[kMaxColors + c];
   }
 }
@@ -547,34 +550,74 @@ struct VelArgs {
   float4* vel;
 };
 
-// B2contactSolver::warm_start for one constraint (src/private/dynamics/b2_contact_solver_private.rs:228-266)
-__device__ inline void warm_start_one(int i, const VelArgs& a) {
-  int2 bd = a.bodies[i];
-  int4 info = a.info[i];
-  float4 m = a.vm[i];
-  float4 nf = a.vnf[i];
-  float4 lam = a.lam[i];
-  float4 vA = a.vel[bd.x], vB = a.vel[bd.y];
-  V2 va = v2(vA.x, vA.y), vb = v2(vB.x, vB.y);
-  float wa = vA.z, wb = vB.z;
-  V2 normal = v2(nf.x, nf.y);
-  V2 tangent = cross(normal, 1.0f);
-  int vcount = info.z;
-  for (int p = 0; p < vcount; ++p) {
-    float4 r = p == 0 ? a.vr0[i] : a.vr1[i];
-    float ni = p == 0 ? lam.x : lam.z, ti = p == 0 ? lam.y : lam.w;
-    V2 P = ni * normal + ti * tangent;
-    wa -= m.y * cross(v2(r.x, r.y), P);
-    va = va - m.x * P;
-    wb += m.w * cross(v2(r.z, r.w), P);
-    vb = vb + m.z * P;
+// B2contactSolver::warm_start (src/private/dynamics/b2_contact_solver_private.rs:228-266) as
+// one pass over dynamic bodies: a body applies the warm-start impulses of its constraints in
+// colour order (ascending colour-sorted index), the order in which a sweep colour by colour
+// applies them to it, with the same operations, so the velocities are the same bit for bit (two
+// constraints of one colour never share a dynamic body). One launch instead of one per colour.
+__device__ inline void warm_apply(int b, int i, const VelArgs& a, V2& v, float& w) {
+  const int2 bd = a.bodies[i];
+  const int4 info = a.info[i];
+  const float4 m = a.vm[i];
+  const float4 nf = a.vnf[i];
+  const float4 lam = a.lam[i];
+  const V2 normal = v2(nf.x, nf.y);
+  const V2 tangent = cross(normal, 1.0f);
+  const bool side_a = bd.x == b;
+  for (int p = 0; p < info.z; ++p) {
+    const float4 r = p == 0 ? a.vr0[i] : a.vr1[i];
+    const float ni = p == 0 ? lam.x : lam.z, ti = p == 0 ? lam.y : lam.w;
+    const V2 P = ni * normal + ti * tangent;
+    if (side_a) {
+      w -= m.y * cross(v2(r.x, r.y), P);
+      v = v - m.x * P;
+    } else {
+      w += m.w * cross(v2(r.z, r.w), P);
+      v = v + m.z * P;
+    }
   }
-  if (info.x & 0x100) a.vel[bd.x] = make_float4(va.x, va.y, wa, 0.0f);
-  if (info.x & 0x200) a.vel[bd.y] = make_float4(vb.x, vb.y, wb, 0.0f);
 }
-__global__ void __launch_bounds__(kBlock) k_warm_start(int begin, int end, VelArgs a) {
-  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < end) warm_start_one(i, a);
+static constexpr int kWarmSort = 8;
+__global__ void __launch_bounds__(kBlock) k_warm_start_bodies(int nb, const int* __restrict__ btype,
+                                                              const int* __restrict__ off, const int* __restrict__ adj,
+                                                              const int* __restrict__ inv_order, VelArgs a) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || btype[b] != 2) return;
+  const int s0 = off[b], e0 = off[b + 1], n = e0 - s0;
+  if (n == 0) return;
+  const float4 vb = a.vel[b];
+  V2 v = v2(vb.x, vb.y);
+  float w = vb.z;
+  if (n <= kWarmSort) {
+    // up to kWarmSort constraints: an odd-even transposition network sorts them in registers
+    int k[kWarmSort];
+#pragma unroll
+    for (int t = 0; t < kWarmSort; ++t) k[t] = t < n ? inv_order[adj[s0 + t]] : INT_MAX;
+#pragma unroll
+    for (int r = 0; r < kWarmSort; ++r)
+#pragma unroll
+      for (int t = r & 1; t + 1 < kWarmSort; t += 2) {
+        const int lo = min(k[t], k[t + 1]), hi = max(k[t], k[t + 1]);
+        k[t] = lo;
+        k[t + 1] = hi;
+      }
+#pragma unroll
+    for (int t = 0; t < kWarmSort; ++t)
+      if (k[t] != INT_MAX) warm_apply(b, k[t], a, v, w);
+  } else {
+    // more: the next larger index each time
+    int last = -1;
+    for (int c = 0; c < n; ++c) {
+      int next = INT_MAX;
+      for (int e = s0; e < e0; ++e) {
+        const int i = inv_order[adj[e]];
+        if (i > last && i < next) next = i;
+      }
+      warm_apply(b, next, a, v, w);
+      last = next;
+    }
+  }
+  a.vel[b] = make_float4(v.x, v.y, w, 0.0f);
 }
 
 // B2contactSolver::solve_velocity_constraints for one constraint
@@ -742,7 +785,7 @@ struct PosArgs {
   const float4 *vm, *plp, *pgeo, *plc;
   const float2* prad;
   float4* pos;
-  int* minsep;
+  int* violated;  // per island root: a contact of the island is below -3 linear slop this iteration
   const int* done;
 };
 
@@ -816,22 +859,10 @@ __device__ inline void solve_position_one(int i, bool live, int begin, const Pos
   }
   if (active && (info.x & 0x100)) a.pos[bd.x] = make_float4(cA.x, cA.y, aA, 0.0f);
   if (active && (info.x & 0x200)) a.pos[bd.y] = make_float4(cB.x, cB.y, aB, 0.0f);
-  // Island minimum separation. A pile is one island, so a plain per-thread atomic would
-  // serialise the whole colour on one address: lanes of a wave first reduce per island and one
-  // lane per island and wave does the atomic.
-  int key = float_flip(minSep);
-  const int isl = info.w;
-  unsigned long long pending = __ballot(minSep < 0.0f);
-  while (pending) {
-    const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-    const int lisl = __shfl(isl, leader, 64);
-    const bool mine = (pending >> (threadIdx.x & 63)) & 1ull && isl == lisl;
-    int v = mine ? key : 0x7fffffff;
-    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
-    if ((threadIdx.x & 63) == leader)
-      __hip_atomic_fetch_min(&a.minsep[lisl], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    pending &= ~__ballot(mine);
-  }
+  // The island may stop iterating only if no contact's separation is below -3 linear slop
+  // (minSeparation >= -3 * slop over the island's contacts, b2_contact_solver_private.rs); a
+  // violating constraint marks its island with a plain store (every writer stores the same value).
+  if (active && !(minSep >= -3.0f * kLinearSlop)) a.violated[info.w] = 1;
 }
 __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, PosArgs a) {
   int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -842,28 +873,24 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
 // (src/private/dynamics/b2_island_private.rs:257-273). A finished island's flag holds the number
 // of iterations it ran (the adjoint's tape reads it).
 __device__ inline void iteration_end_one(int b, int iteration, const int* __restrict__ btype,
-                                         const int* __restrict__ parent, int* __restrict__ minsep, int* __restrict__ done,
-                                         int* __restrict__ active) {
+                                         const int* __restrict__ parent, int* __restrict__ violated,
+                                         int* __restrict__ done, int* __restrict__ active) {
   if (btype[b] == 0 || parent[b] != b || done[b]) return;
-  float ms = float_unflip(__hip_atomic_load(&minsep[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  if (ms >= -3.0f * kLinearSlop) done[b] = iteration + 1;
+  if (!violated[b]) done[b] = iteration + 1;
   else atomicAdd(active, 1);
-  minsep[b] = float_flip(0.0f);
+  violated[b] = 0;
 }
 __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
-                                                                   const int* __restrict__ parent, int* __restrict__ minsep,
-                                                                   int* __restrict__ done, int* __restrict__ active) {
+                                                                   const int* __restrict__ parent,
+                                                                   int* __restrict__ violated, int* __restrict__ done,
+                                                                   int* __restrict__ active) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) iteration_end_one(b, iteration, btype, parent, minsep, done, active);
+  if (b < nb) iteration_end_one(b, iteration, btype, parent, violated, done, active);
 }
 
 // ---- graph-mode variants: colour ranges and counts read from device memory -----------------
 // A captured graph replays with new colour offsets (copied before each launch) and a fixed
 // grid; the loops stride over whatever the colour holds this step.
-__global__ void __launch_bounds__(kBlock) k_warm_start_g(int c, const int* __restrict__ off, VelArgs a) {
-  const int b0 = off[c], b1 = off[c + 1];
-  for (int i = b0 + blockIdx.x * blockDim.x + threadIdx.x; i < b1; i += gridDim.x * blockDim.x) warm_start_one(i, a);
-}
 __global__ void __launch_bounds__(kBlock) k_solve_velocity_g(int c, const int* __restrict__ off, int block_solve,
                                                              VelArgs a) {
   const int b0 = off[c], b1 = off[c + 1];
@@ -902,8 +929,8 @@ static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& v
   int* cnt = w->counters.p;
   hipStream_t st = w->stream;
   if (k.warm)
-    for (int col = 0; col < k.ncolors; ++col)
-      hipLaunchKernelGGL(k_warm_start_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, va);
+    hipLaunchKernelGGL(k_warm_start_bodies, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p,
+                       s.adj_off.p, s.adj.p, s.inv_order.p, va);
   for (int it = 0; it < k.vel_iters; ++it)
     for (int col = 0; col < k.ncolors; ++col)
       hipLaunchKernelGGL(k_solve_velocity_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, k.block, va);
@@ -915,7 +942,7 @@ static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& v
     for (int col = 0; col < k.ncolors; ++col)
       hipLaunchKernelGGL(k_solve_position_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, pa);
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, it,
-                       w->b.type.p, s.parent.p, s.isl_minsep.p, s.isl_done.p, cnt + 3);
+                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3);
   }
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p, w->b.pos.p,
                      w->b.mass.p, w->b.xf.p);
@@ -974,7 +1001,7 @@ static int reserve_solver(World* w, int nc) {
       (st = s.vms.reserve(n)) || (st = s.vbias.reserve(n)) || (st = s.K.reserve(n)) || (st = s.NM.reserve(n)) ||
       (st = s.lam.reserve(n)) || (st = s.plp.reserve(n)) || (st = s.pgeo.reserve(n)) || (st = s.plc.reserve(n)) ||
       (st = s.prad.reserve(n)) || (st = s.adj.reserve(2 * n)) || (st = s.color_state.reserve(n)) ||
-      (st = s.order.reserve(n)))
+      (st = s.order.reserve(n)) || (st = s.inv_order.reserve(n)))
     return st;
   return B2G_OK;
 }
@@ -993,7 +1020,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   const int n_pairs = w->n_pairs;
   if ((st = reserve_solver(w, n_pairs))) return st;
   if ((st = s.deg.reserve(nb + 1)) || (st = s.adj_off.reserve(nb + 1)) || (st = s.adj_cursor.reserve(nb)) ||
-      (st = s.parent.reserve(nb)) || (st = s.isl_minsep.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
+      (st = s.parent.reserve(nb)) || (st = s.isl_violated.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
       (st = s.color_count.reserve(kMaxColors + 1)) || (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
       (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(std::max(n_pairs, 1))))
     return st;
@@ -1023,7 +1050,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     hipLaunchKernelGGL(k_union, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p, s.parent.p);
   }
   hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
-                     s.isl_minsep.p, s.isl_done.p);
+                     s.isl_violated.p, s.isl_done.p);
   int ncolors = 0, nc = 0;
   if (n_pairs > 0) {
     if ((st = exclusive_scan(w, s.deg.p, s.adj_off.p, nb + 1, nullptr))) return st;
@@ -1086,8 +1113,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
     B2G_HIP(hipMemsetAsync(s.color_cursor.p, 0, sizeof(int) * kMaxColors, w->stream));
     hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
-                       s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.touch_list.p,
-                       c.colour.p);
+                       s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.inv_order.p,
+                       s.touch_list.p, c.colour.p);
     InitArgs ia;
     ia.n = nc;
     ia.order = s.order.p;
@@ -1128,7 +1155,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   }
   VelArgs va{s.k_bodies.p, s.k_info.p, s.vnf.p, s.vm.p, s.vr0.p, s.vr1.p, s.vms.p, s.vbias.p, s.K.p, s.NM.p, s.lam.p,
              w->b.vel.p};
-  PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_minsep.p,
+  PosArgs pa{s.k_bodies.p, s.k_info.p, s.vm.p, s.plp.p, s.pgeo.p, s.plc.p, s.prad.p, w->b.pos.p, s.isl_violated.p,
              s.isl_done.p};
   const std::vector<int>& co = w->h_color_off;
   // (graph nodes cannot carry the timing events of a timed solve kernel)
@@ -1155,8 +1182,9 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     k.block = w->block_solve;
     k.dt = dt;
     const void* ptrs[] = {va.bodies, va.info, va.vnf, va.vm, va.vr0, va.vr1, va.vms, va.vbias, va.K, va.NM, va.lam,
-                          va.vel, pa.plp, pa.pgeo, pa.plc, pa.prad, pa.pos, pa.minsep, pa.done, s.color_off.p, cnt,
-                          c.pt0.p, c.pt1.p, s.k_contact.p, w->b.type.p, s.parent.p, w->b.mass.p, w->b.xf.p};
+                          va.vel, pa.plp, pa.pgeo, pa.plc, pa.prad, pa.pos, pa.violated, pa.done, s.color_off.p, cnt,
+                          c.pt0.p, c.pt1.p, s.k_contact.p, w->b.type.p, s.parent.p, w->b.mass.p, w->b.xf.p,
+                          s.adj_off.p, s.adj.p, s.inv_order.p};
     static_assert(sizeof(ptrs) / sizeof(ptrs[0]) <= 32, "graph key pointers");
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) k.ptr[i] = ptrs[i];
     if ((st = solve_tail_graph(w, k, va, pa))) return st;
@@ -1164,11 +1192,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     return B2G_OK;
   }
   if (nc > 0 && w->warm_starting)
-    for (int col = 0; col < ncolors; ++col) {
-      int b0 = co[col], b1 = co[col + 1];
-      launch_timed(w, B2G_KERNEL_WARM_START, b1 - b0, k_warm_start, dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock),
-                   b0, b1, va);
-    }
+    launch_timed(w, B2G_KERNEL_WARM_START, nc, k_warm_start_bodies, dim3(blocks_for(nb, kBlock)), dim3(kBlock), nb,
+                 w->b.type.p, s.adj_off.p, s.adj.p, s.inv_order.p, va);
   for (int it = 0; it < vel_iters && nc > 0; ++it)
     for (int col = 0; col < ncolors; ++col) {
       int b0 = co[col], b1 = co[col + 1];
@@ -1188,7 +1213,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                    dim3(kBlock), b0, b1, pa);
     }
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
-                       w->b.type.p, s.parent.p, s.isl_minsep.p, s.isl_done.p, cnt + 3);
+                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3);
     ++w->pos_iters_run;
   }
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p, w->b.pos.p,
