@@ -420,7 +420,7 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
   World* w = &h->w;
   w->gravity = v2(gx, gy);
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
-      w->counters.reserve(16) != B2G_OK ||
+      w->counters.reserve(4 + 128 + 1) != B2G_OK ||
       hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     set_error("world allocation failed");
     delete h;
@@ -448,7 +448,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
   rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);
   rel(s.plp); rel(s.pgeo); rel(s.plc); rel(s.prad); rel(s.deg); rel(s.adj_off); rel(s.adj_cursor); rel(s.adj);
-  rel(s.color_state); rel(s.color_count); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.inv_order); rel(s.parent);
+  rel(s.color_state); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.inv_order); rel(s.parent);
   rel(s.isl_violated); rel(s.isl_done);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes); rel(bp.nodes4);

@@ -359,6 +359,7 @@ struct TreeView {
   const int4* leafrec;         // sorted leaf records
   const int* loose_off;        // per world: range of big static proxies kept out of the tree
   const int* loose;
+  int n_loose;                 // big static proxies over all worlds
   int batched;
 };
 
@@ -380,6 +381,9 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const float4* _
   __shared__ int stack[kStack][kTravBlock];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nq) return;
+  // proxies outside the tree emit nothing: their counts are cleared here
+  if (!OVERFLOW_PASS)
+    for (int e = k; e < tv.n_loose; e += nq) count[tv.loose[e]] = 0;
   const int4 qr = tv.leafrec[k];
   const int q = qr.x;
   if (OVERFLOW_PASS && count[q] <= kSlots) return;
@@ -457,11 +461,14 @@ __device__ inline int find_in_segment(int a, int b, int old_np, const int* __res
     if (old_pairs[i].y == b) return i;
   return -1;
 }
+// (also clears the new set's solve colours: the solve of this set writes those of its touching
+// contacts)
 __global__ void __launch_bounds__(kBlock) k_merge(int n, const int2* __restrict__ pairs, int old_np,
                                                   const int* __restrict__ old_off, const int2* __restrict__ old_pairs,
-                                                  int* __restrict__ old_index) {
+                                                  int* __restrict__ old_index, int* __restrict__ colour) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  colour[k] = -1;
   int2 pr = pairs[k];
   int found = find_in_segment(pr.x, pr.y, old_np, old_off, old_pairs);
   if (found < 0) found = find_in_segment(pr.y, pr.x, old_np, old_off, old_pairs);
@@ -518,9 +525,10 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     }
   }
   PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p};
-  TreeView tv{nt, bp.nodes4.p, bp.leafrec.p, w->f.loose_off.p, w->f.loose.p, wbits > 0};
+  TreeView tv{nt, bp.nodes4.p, bp.leafrec.p, w->f.loose_off.p, w->f.loose.p, np - nt, wbits > 0};
   // tree proxies walk in Morton order; proxies outside the tree are static and emit nothing
-  if (nt < np) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
+  // (the walk clears their counts; with no tree proxies there is no walk)
+  if (nt == 0 && np > 0) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
   if (nt > 0) {
     launch_timed(w, B2G_KERNEL_TRAVERSE, nt, k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock),
                  nt, fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr, flags);
@@ -583,12 +591,10 @@ int phase_broadphase(World* w) {
   }
   if ((st = find_pairs(w, w->f.fat.p, w->np, dst, &n))) return st;
   if ((st = reserve_contacts(dst, n))) return st;
-  // no solve colours yet (the solve of this set writes those of its touching contacts)
-  if (n > 0) B2G_HIP(hipMemsetAsync(dst.colour.p, 0xff, sizeof(int) * n, w->stream));
   if (n > 0) {
     int old_np = w->set_np[w->cur];  // proxies when the old set was built (0: no old set)
     hipLaunchKernelGGL(k_merge, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, dst.pair.p, old_np,
-                       old.off.p, old.pair.p, dst.old_index.p);
+                       old.off.p, old.pair.p, dst.old_index.p, dst.colour.p);
     B2G_HIP(hipGetLastError());
   }
   w->set_np[next] = w->np;

@@ -129,7 +129,7 @@ struct SolverArrays {
   // colouring
   DevArray<int> deg, adj_off, adj_cursor, adj;  // body -> constraints (dynamic bodies only)
   DevArray<int> color_state;                    // (round << 8) | colour, or -1
-  DevArray<int> color_count, color_off, color_cursor;  // color_off: launch offsets, then per colour id
+  DevArray<int> color_off, color_cursor;  // color_off: launch offsets, then per colour id
                                                        // its base offset and its launch index
   DevArray<int> prev_colour;                    // colour of the contact's predecessor, or -1
   DevArray<int> order;                          // colour-sorted constraint -> compacted index
@@ -254,7 +254,7 @@ struct World {
   int set_np[2] = {0, 0};  // proxy count when each contact set was built (0: never built)
   SolverArrays s;
   BroadphaseArrays bp;
-  DevArray<int> counters;   // small device counters
+  DevArray<int> counters;   // small device counters: 4 solver counters, then the colour histogram
   int* h_counters = nullptr;  // pinned host mirror (4 counters, 128 colour counts, last round)
   int n_touch = 0, n_colors = 0, n_islands = 0;
   int color_rounds = 16;       // colouring rounds launched before the first readback

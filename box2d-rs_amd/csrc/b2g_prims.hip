@@ -91,10 +91,41 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int* __restrict
   }
 }
 
+// Scan of a short array by one block in one launch: thread t owns a contiguous chunk, sums it,
+// the block scans the chunk sums, and each thread writes its chunk's prefixes (reading each
+// element before writing it, so in and out may alias).
+static constexpr int kOneBlock = 1024;
+static constexpr int kOneBlockMax = 4 * kOneBlock;  // longer arrays: the three-launch scan is faster
+__global__ void __launch_bounds__(kOneBlock) k_scan_one_block(const int* in, int* out, int n, int* __restrict__ total) {
+  __shared__ int wsum[kOneBlock / 64];
+  const int chunk = (n + kOneBlock - 1) / kOneBlock;
+  const int b0 = min(n, int(threadIdx.x) * chunk), b1 = min(n, b0 + chunk);
+  int s = 0;
+  for (int i = b0; i < b1; ++i) s += in[i];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int inc = wave_inclusive(s);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int base = 0;
+  for (int v = 0; v < wid; ++v) base += wsum[v];
+  int ex = base + inc - s;
+  if (threadIdx.x == kOneBlock - 1 && total) *total = ex + s;
+  for (int i = b0; i < b1; ++i) {
+    const int v = in[i];
+    out[i] = ex;
+    ex += v;
+  }
+}
+
 // out[i] = sum of in[0..i); *total_dev (if given) = sum of all. in and out may alias.
 int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev) {
   if (n <= 0) {
     if (total_dev) B2G_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), w->stream));
+    return B2G_OK;
+  }
+  if (n <= kOneBlockMax) {
+    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(kOneBlock), 0, w->stream, in, out, n, total_dev);
+    B2G_HIP(hipGetLastError());
     return B2G_OK;
   }
   int nblocks = blocks_for(n, kScanTile);

@@ -141,16 +141,30 @@ __global__ void __launch_bounds__(kBlock) k_fill_adjacency(const int* __restrict
   if (btype[b.y] == 2) adj[off[b.y] + atomicAdd(&cursor[b.y], 1)] = j;
 }
 
-__global__ void __launch_bounds__(kBlock) k_iota(int n, int* __restrict__ a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = i;
+// per body: its own island root, no constraints yet
+__global__ void __launch_bounds__(kBlock) k_solver_body_init(int nb, int* __restrict__ parent, int* __restrict__ deg,
+                                                             int* __restrict__ cursor) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) {
+    parent[b] = b;
+    deg[b] = 0;
+    cursor[b] = 0;
+  }
+  if (b == 0) deg[nb] = 0;
 }
 
-// per-step solver counters: [0] touching, [1] uncoloured, [2] colour overflow, [3] active islands
-__global__ void k_solver_counters_init(int* c) {
-  c[1] = c[0];
-  c[2] = 0;
-  c[3] = 0;
+// Per-step solver counters: [0] touching, [1] uncoloured, [2] colour overflow, [3] active
+// islands, then the colour histogram (kMaxColors counts and the last colouring round); also
+// clears the colour scatter cursors.
+__global__ void __launch_bounds__(kBlock) k_solver_counters_init(int* c, int* __restrict__ cursor) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    c[1] = c[0];
+    c[2] = 0;
+    c[3] = 0;
+  }
+  for (int i = t; i <= kMaxColors; i += blockDim.x) c[4 + i] = 0;
+  for (int i = t; i < kMaxColors; i += blockDim.x) cursor[i] = 0;
 }
 
 // Find with path halving for the union kernel, where other threads link roots with atomicCAS
@@ -1021,11 +1035,11 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   if ((st = reserve_solver(w, n_pairs))) return st;
   if ((st = s.deg.reserve(nb + 1)) || (st = s.adj_off.reserve(nb + 1)) || (st = s.adj_cursor.reserve(nb)) ||
       (st = s.parent.reserve(nb)) || (st = s.isl_violated.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
-      (st = s.color_count.reserve(kMaxColors + 1)) || (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
+      (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
       (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(std::max(n_pairs, 1))))
     return st;
   if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
-  hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(1), 0, w->stream, cnt);
+  hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(kBlock), 0, w->stream, cnt, s.color_cursor.p);
   const int gp = blocks_for(std::max(n_pairs, 1), kBlock);
   // Colour reuse (persisting contacts keep their colour) lets the colour count creep up as new
   // contacts take first-fit colours around kept ones; the step colours from scratch again once
@@ -1038,9 +1052,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     hipLaunchKernelGGL(k_compact, dim3(gp), dim3(kBlock), 0, w->stream, n_pairs, s.touch_flag.p, s.touch_scan.p,
                        s.touch_list.p);
   // constraint bodies, degrees, islands
-  B2G_HIP(hipMemsetAsync(s.deg.p, 0, sizeof(int) * (nb + 1), w->stream));
-  B2G_HIP(hipMemsetAsync(s.adj_cursor.p, 0, sizeof(int) * nb, w->stream));
-  hipLaunchKernelGGL(k_iota, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p);
+  hipLaunchKernelGGL(k_solver_body_init, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
+                     s.deg.p, s.adj_cursor.p);
   if (n_pairs > 0) {
     // colours of the previous step's contacts (the old set's solve wrote them)
     const int* old_colour = reuse ? w->c[1 - w->cur].colour.p : nullptr;
@@ -1070,10 +1083,10 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
         launch_timed(w, B2G_KERNEL_COLOR_ROUND, n_pairs, k_color_round, dim3(gp), dim3(kBlock), cnt, round,
                      s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
       }
-      B2G_HIP(hipMemsetAsync(s.color_count.p, 0, sizeof(int) * (kMaxColors + 1), w->stream));
-      hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, s.color_count.p);
-      B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, 4 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
-      B2G_HIP(hipMemcpyAsync(w->h_counters + 4, s.color_count.p, sizeof(int) * (kMaxColors + 1), hipMemcpyDeviceToHost,
+      // the histogram was cleared with the counters; a later batch recounts from zero
+      if (round > batch + 1) B2G_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(int) * (kMaxColors + 1), w->stream));
+      hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, cnt + 4);
+      B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, (4 + kMaxColors + 1) * sizeof(int), hipMemcpyDeviceToHost,
                              w->stream));
       B2G_HIP(hipStreamSynchronize(w->stream));
       if ((st = kt_harvest(w))) return st;
@@ -1111,7 +1124,6 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
     for (int i = 0; i <= kMaxColors; ++i) offs[i] = i <= ncolors ? w->h_color_off[i] : nc;
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
-    B2G_HIP(hipMemsetAsync(s.color_cursor.p, 0, sizeof(int) * kMaxColors, w->stream));
     hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
                        s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.inv_order.p,
                        s.touch_list.p, c.colour.p);
