@@ -502,10 +502,12 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   int* bnd = reinterpret_cast<int*>(bp.bounds.p);
   int* flags = bnd + 4;  // [0] stack overflow, [1] proxies with more pairs than slots
   hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, w->stream, bnd);
-  // world bits above Morton bits: 30 Morton bits for one world, fewer for many
+  // world bits above Morton bits: 12 bits per axis (4096 cells across the scene's extent, finer
+  // than any body on the scenes here; equal keys keep proxy order), fewer for many worlds, so one
+  // world sorts in three 8-bit passes
   int wbits = 0;
   while ((1 << wbits) < w->n_worlds) ++wbits;
-  const int mbits = std::min(15, (32 - wbits) / 2);
+  const int mbits = std::min(12, (32 - wbits) / 2);
   const int key_bits = wbits + 2 * mbits;
   const LeafInputs li{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p};
   if (nt > 0) {

@@ -55,22 +55,20 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const int* __restric
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-// exclusive scan of the block sums by one block, chunk by chunk
-__global__ void __launch_bounds__(kScanBlock) k_scan_sums(int* __restrict__ sums, int nblocks, int* __restrict__ total) {
-  int carry = 0;
-  for (int base = 0; base < nblocks; base += kScanBlock) {
-    int idx = base + threadIdx.x;
-    int x = idx < nblocks ? sums[idx] : 0;
-    int tot;
-    int ex = block_exclusive(x, &tot);
-    if (idx < nblocks) sums[idx] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
-}
-
 __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int* __restrict__ in, int* __restrict__ out, int n,
-                                                          const int* __restrict__ sums) {
+                                                          const int* __restrict__ sums, int nblocks,
+                                                          int* __restrict__ total) {
+  // the block's offset: the sum of the block sums before it (at most a few hundred), summed by
+  // the block itself instead of in a separate launch
+  __shared__ int wpre[kScanBlock / 64];
+  int pre = 0;
+  for (int i = threadIdx.x; i < blockIdx.x; i += kScanBlock) pre += sums[i];
+  pre = __shfl(wave_inclusive(pre), 63, 64);
+  if ((threadIdx.x & 63) == 0) wpre[threadIdx.x >> 6] = pre;
+  __syncthreads();
+  pre = 0;
+#pragma unroll
+  for (int w = 0; w < kScanBlock / 64; ++w) pre += wpre[w];
   // each thread owns kScanItems consecutive items so the block covers a contiguous tile
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -82,7 +80,8 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int* __restrict
     s += v[i];
   }
   int tot;
-  int ex = block_exclusive(s, &tot) + sums[blockIdx.x];
+  int ex = block_exclusive(s, &tot) + pre;
+  if (total && blockIdx.x == nblocks - 1 && threadIdx.x == 0) *total = pre + tot;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     int idx = base + i;
@@ -133,8 +132,8 @@ int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev) {
   if (st) return st;
   int* sums = w->bp.scan_tmp.p;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, w->stream, sums, nblocks, total_dev);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, out, n, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, out, n, sums, nblocks,
+                     total_dev);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }
