@@ -204,7 +204,7 @@ struct AdjCtx {
   const float4 *bmass, *props, *force;
   float gx, gy;
   const int4* fshape;
-  const float2* pool;
+  const ShapeV* shapes;
   const float4* fmat;
   // tape
   const int4* slot;
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(kBlock) k_af_geom(AdjCtx a) {
   const long long s = a.off + i;
   const int4 sl = a.slot[s];
   const int4 mt = a.meta[s];
-  const ShapeV sa = load_shape(a.fshape[sl.z], a.pool), sb = load_shape(a.fshape[sl.w], a.pool);
+  const ShapeV &sa = a.shapes[sl.z], &sb = a.shapes[sl.w];
   const float4 A = a.bmass[sl.x], B = a.bmass[sl.y];
   const float lcA[2] = {A.z, A.w}, lcB[2] = {B.z, B.w};
   int hdr = 0;
@@ -625,7 +625,7 @@ __global__ void __launch_bounds__(kBlock) k_ab_setup(AdjCtx a) {
     atomicAdd(&a.fric_grad[sl.w], g * 0.5f * fa / mu);
   }
   // geometry = collide(qA, qB)
-  const ShapeV sa = load_shape(a.fshape[sl.z], a.pool), sb = load_shape(a.fshape[sl.w], a.pool);
+  const ShapeV &sa = a.shapes[sl.z], &sb = a.shapes[sl.w];
   const float4 A = a.bmass[sl.x], B = a.bmass[sl.y];
   const float lcA[2] = {A.z, A.w}, lcB[2] = {B.z, B.w};
   float x[6], yb[8], xb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -688,7 +688,7 @@ static AdjCtx make_ctx(World* w, AdjBuffers& B) {
   a.gx = w->gravity.x;
   a.gy = w->gravity.y;
   a.fshape = w->f.shape.p;
-  a.pool = w->f.pool.p;
+  a.shapes = w->f.shapev.p;
   a.fmat = w->f.mat.p;
   a.slot = t.slot.p;
   a.meta = t.meta.p;

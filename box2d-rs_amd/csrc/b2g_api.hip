@@ -193,19 +193,27 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_
     w->h_shapes.push_back(s);
     w->h_fbody.push_back(b);
     w->h_fdensity.push_back(fd.density);
-    int off = int(w->h_pool.size());
-    if (s.type == kCircle) {
-      w->h_pool.push_back(make_float2(s.p.x, s.p.y));
-    } else if (s.type == kPolygon) {
-      for (int i = 0; i < s.count; ++i) w->h_pool.push_back(make_float2(s.v[i].x, s.v[i].y));
-      for (int i = 0; i < s.count; ++i) w->h_pool.push_back(make_float2(s.n[i].x, s.n[i].y));
-      w->h_pool.push_back(make_float2(s.p.x, s.p.y));
-    } else {
-      for (int i = 0; i < 4; ++i) w->h_pool.push_back(make_float2(s.v[i].x, s.v[i].y));
+    {
+      // the shape as the device kernels read it (fields a type does not use stay zero)
+      ShapeV r;
+      std::memset(&r, 0, sizeof(r));
+      r.type = s.type;
+      r.count = s.count;
+      r.one_sided = s.one_sided;
+      r.radius = s.radius;
+      if (s.type == kCircle) {
+        r.p = s.p;
+      } else if (s.type == kPolygon) {
+        for (int i = 0; i < s.count; ++i) { r.v[i] = s.v[i]; r.n[i] = s.n[i]; }
+        r.p = s.p;
+      } else {
+        for (int i = 0; i < 4; ++i) r.v[i] = s.v[i];
+      }
+      w->h_shapev.push_back(r);
     }
     int radius_bits;
     std::memcpy(&radius_bits, &s.radius, 4);
-    w->h_fshape.push_back(make_int4(s.type | (s.count << 8) | (s.one_sided << 16), off, radius_bits, 0));
+    w->h_fshape.push_back(make_int4(s.type | (s.count << 8) | (s.one_sided << 16), 0, radius_bits, 0));
     w->h_fmat.push_back(make_float4(fd.friction, fd.restitution, fd.restitution_threshold, 0.0f));
     w->h_ffilter.push_back(make_int2(int(fd.category_bits) | (int(fd.mask_bits) << 16), fd.group_index));
     w->h_fworld.push_back(w->h_bworld[b]);
@@ -296,7 +304,7 @@ int upload_world(World* w) {
       (st = up(w->f.body, w->h_fbody, w->stream)) || (st = up(w->f.shape, w->h_fshape, w->stream)) ||
       (st = up(w->f.mat, w->h_fmat, w->stream)) || (st = up(w->f.filter, w->h_ffilter, w->stream)) ||
       (st = up(w->f.aabb, w->h_faabb, w->stream)) || (st = up(w->f.fat, w->h_ffat, w->stream)) ||
-      (st = up(w->f.pool, w->h_pool, w->stream)) || (st = up(w->f.world, w->h_fworld, w->stream)) ||
+      (st = up(w->f.shapev, w->h_shapev, w->stream)) || (st = up(w->f.world, w->h_fworld, w->stream)) ||
       (st = w->f.moved.reserve(std::max(w->np, 1))))
     return st;
   // Big static proxies (ground, container walls: an AABB side over kLooseExtent) stay out of
@@ -440,7 +448,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force);
   rel(w->b.type);
   rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
-  rel(w->f.pool); rel(w->f.world); rel(w->f.tree_list); rel(w->f.loose_off); rel(w->f.loose);
+  rel(w->f.shapev); rel(w->f.world); rel(w->f.tree_list); rel(w->f.loose_off); rel(w->f.loose);
   for (auto& c : w->c) {
     rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids); rel(c.colour);
   }

@@ -25,7 +25,7 @@ static constexpr int kBlock = 256;
 
 // ---- fixture synchronisation --------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __restrict__ fbody,
-                                                        const int4* __restrict__ fshape, const float2* __restrict__ pool,
+                                                        const ShapeV* __restrict__ shapes,
                                                         const int* __restrict__ btype, const float4* __restrict__ pos0,
                                                         const float4* __restrict__ bmass, const float4* __restrict__ bxf,
                                                         float4* __restrict__ faabb, float4* __restrict__ ffat,
@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
   if (i >= np) return;
   int b = fbody[i];
   if (btype[b] == 0) { moved[i] = 0; return; }
-  ShapeV s = load_shape(fshape[i], pool);
+  const ShapeV& s = shapes[i];
   float4 p0 = pos0[b];
   float4 ms = bmass[b];
   Xf xf1;
@@ -572,7 +572,7 @@ static int reserve_contacts(ContactArrays& c, int n) {
 int phase_synchronize(World* w) {
   if (w->np == 0) return B2G_OK;
   hipLaunchKernelGGL(k_synchronize, dim3(blocks_for(w->np, kBlock)), dim3(kBlock), 0, w->stream, w->np, w->f.body.p,
-                     w->f.shape.p, w->f.pool.p, w->b.type.p, w->b.pos0.p, w->b.mass.p, w->b.xf.p, w->f.aabb.p,
+                     w->f.shapev.p, w->b.type.p, w->b.pos0.p, w->b.mass.p, w->b.xf.p, w->f.aabb.p,
                      w->f.fat.p, w->f.moved.p);
   B2G_HIP(hipGetLastError());
   return B2G_OK;

@@ -53,7 +53,7 @@ struct DevArray {
   }
 };
 
-// Host-side shape after construction (hull, box, ...): the pool layout written to the device.
+// Host-side shape after construction (hull, box, ...); the device keeps it as a ShapeV record.
 struct HostShape {
   int type = kCircle;
   int count = 0;
@@ -83,7 +83,7 @@ struct BodyArrays {
 // also the order of the reference proxy ids).
 struct ProxyArrays {
   DevArray<int> body;
-  DevArray<int4> shape;     // type | count << 8 | one_sided << 16, pool offset, radius bits, 0
+  DevArray<int4> shape;     // type | count << 8 | one_sided << 16, 0, radius bits, 0
   DevArray<float4> mat;     // friction, restitution, restitution threshold, 0
   DevArray<int2> filter;    // category | mask << 16, group
   DevArray<float4> aabb;    // swept AABB (proxy.aabb)
@@ -93,7 +93,7 @@ struct ProxyArrays {
   DevArray<int> tree_list;  // proxies kept in the BVH
   DevArray<int> loose_off;  // per world: range in `loose` of its big static proxies
   DevArray<int> loose;      // big static proxies (ground, walls) tested directly, not in the BVH
-  DevArray<float2> pool;    // shape vertex pool
+  DevArray<ShapeV> shapev;  // per proxy: its shape as the collide functions read it (by reference)
 };
 
 // Contact set: pairs grouped by their emitting proxy, plus manifolds.
@@ -236,7 +236,7 @@ struct World {
   // host sources of asynchronous uploads must outlive the copy: kept here, not on the stack
   std::vector<int> h_tree, h_loose_off, h_loose, h_color_off_dev;
   int n_worlds = 1;
-  std::vector<float2> h_pool;
+  std::vector<ShapeV> h_shapev;
   std::vector<HostShape> h_shapes;
   int nb = 0, np = 0;          // bodies, proxies on the host
   int dev_nb = 0, dev_np = 0;  // uploaded
@@ -340,29 +340,7 @@ __host__ __device__ inline int float_flip(float f) {
 }
 __host__ __device__ inline float float_unflip(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
 
-// device shape loader
-__device__ inline ShapeV load_shape(int4 h, const float2* __restrict__ pool) {
-  ShapeV s;
-  s.type = h.x & 0xff;
-  s.count = (h.x >> 8) & 0xff;
-  s.one_sided = (h.x >> 16) & 0xff;
-  s.radius = __int_as_float(h.z);
-  const float2* q = pool + h.y;
-  if (s.type == kCircle) {
-    s.p = v2(q[0].x, q[0].y);
-  } else if (s.type == kPolygon) {
-    for (int i = 0; i < s.count; ++i) {
-      s.v[i] = v2(q[i].x, q[i].y);
-      s.n[i] = v2(q[s.count + i].x, q[s.count + i].y);
-    }
-    s.p = v2(q[2 * s.count].x, q[2 * s.count].y);
-  } else {
-    for (int i = 0; i < 4; ++i) s.v[i] = v2(q[i].x, q[i].y);
-    s.p = v2(0.0f, 0.0f);
-  }
-  return s;
-}
-
+// body transform record (p.x, p.y, sin, cos) -> Xf
 __device__ inline Xf xf_of(float4 t) {
   Xf x;
   x.p = v2(t.x, t.y);

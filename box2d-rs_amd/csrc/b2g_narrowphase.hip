@@ -26,7 +26,7 @@ __device__ inline void store_manifold(int k, const Manifold& m, const float imp[
 
 __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restrict__ pairs,
                                                     const int* __restrict__ old_index, const int* __restrict__ fbody,
-                                                    const int4* __restrict__ fshape, const float2* __restrict__ pool,
+                                                    const int4* __restrict__ fshape, const ShapeV* __restrict__ shapes,
                                                     const float4* __restrict__ bxf,
                                                     // previous manifolds
                                                     const int* __restrict__ ohdr, const float4* __restrict__ opt0,
@@ -45,9 +45,9 @@ __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restric
   int4 hq = fshape[lo], hp = fshape[hi];
   bool swap = registry_swap(hq.x & 0xff, hp.x & 0xff) == 1;
   int fa = swap ? hi : lo, fb = swap ? lo : hi;
-  int4 ha = swap ? hp : hq, hb = swap ? hq : hp;
-  ShapeV sa = load_shape(ha, pool);
-  ShapeV sb = load_shape(hb, pool);
+  // shapes are read in place from their records (no private copy of the vertex arrays)
+  const ShapeV& sa = shapes[fa];
+  const ShapeV& sb = shapes[fb];
   Xf xa = xf_of(bxf[fbody[fa]]);
   Xf xb = xf_of(bxf[fbody[fb]]);
   Manifold m;
@@ -79,7 +79,7 @@ int phase_collide(World* w) {
   ContactArrays& o = w->c[1 - w->cur];
   launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), n, c.pair.p,
                c.old_index.p,
-               w->f.body.p, w->f.shape.p, w->f.pool.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p, o.ids.p, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
+               w->f.body.p, w->f.shape.p, w->f.shapev.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p, o.ids.p, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }
@@ -87,15 +87,14 @@ int phase_collide(World* w) {
 // ---- stateless entry point ----------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_collide_given(int n, const int2* __restrict__ pairs,
                                                           const float4* __restrict__ xa4, const float4* __restrict__ xb4,
-                                                          const int4* __restrict__ fshape,
-                                                          const float2* __restrict__ pool, int* __restrict__ hdr,
+                                                          const ShapeV* __restrict__ shapes, int* __restrict__ hdr,
                                                           float4* __restrict__ geo, float4* __restrict__ pt0,
                                                           float4* __restrict__ pt1, uint2* __restrict__ ids) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int2 pr = pairs[k];
-  ShapeV sa = load_shape(fshape[pr.x], pool);
-  ShapeV sb = load_shape(fshape[pr.y], pool);
+  const ShapeV& sa = shapes[pr.x];
+  const ShapeV& sb = shapes[pr.y];
   Manifold m;
   evaluate(&m, sa, xf_of(xa4[k]), sb, xf_of(xb4[k]));
   float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -133,7 +132,7 @@ int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xf
             hipMemcpyAsync(db.p, xfb, sizeof(float4) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess;
   if (ok) {
     hipLaunchKernelGGL(k_collide_given, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, dp.p, da.p, db.p,
-                       w->f.shape.p, w->f.pool.p, hdr.p, geo.p, p0.p, p1.p, ids.p);
+                       w->f.shapev.p, hdr.p, geo.p, p0.p, p1.p, ids.p);
     ok = hipGetLastError() == hipSuccess &&
          hipMemcpyAsync(hh.data(), hdr.p, sizeof(int) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
          hipMemcpyAsync(hg.data(), geo.p, sizeof(float4) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
