@@ -534,6 +534,12 @@ class B2body:
         m = self._world._world.mass()[self.index]
         return float(m[2] + m[0] * (m[4] * m[4] + m[5] * m[5]))
 
+    def is_awake(self):
+        return bool(self._state()["awake"] != 0)
+
+    def get_sleep_time(self):
+        return float(self._state()["sleep_time"])
+
 
 class B2world:
     """``B2world`` (src/b2_world.rs) stepping on the GPU.
@@ -548,25 +554,36 @@ class B2world:
         self._fixtures = []
         self._n_flushed_bodies = 0
         self._cache = None
-        self._world.set_flags(True, False, False, True)
+        # B2world::new defaults: warm starting and sleeping on (continuous physics is not on the
+        # GPU path and stays off)
+        self._warm_starting = True
+        self._allow_sleep = True
+        self._apply_flags()
 
     @classmethod
     def new(cls, gravity):
         return cls(gravity)
 
     def set_warm_starting(self, flag):
-        self._flags(warm_starting=flag)
+        self._warm_starting = bool(flag)
+        self._apply_flags()
 
     def set_continuous_physics(self, flag):
         if flag:
             raise B2gError(ERR_UNSUPPORTED, "continuous physics (TOI) is not on the GPU path")
 
     def set_allow_sleeping(self, flag):
-        if flag:
-            raise B2gError(ERR_UNSUPPORTED, "sleeping islands are not on the GPU path")
+        # B2world::set_allow_sleeping (src/private/dynamics/b2_world.rs:342-353): switching it off
+        # wakes every body
+        self._flush()
+        self._allow_sleep = bool(flag)
+        self._apply_flags()
 
-    def _flags(self, warm_starting=True):
-        self._world.set_flags(warm_starting, False, False, True)
+    def get_allow_sleeping(self):
+        return self._allow_sleep
+
+    def _apply_flags(self):
+        self._world.set_flags(self._warm_starting, self._allow_sleep, False, True)
 
     def create_body(self, body_def):
         self._bodies.append(body_def._def())

@@ -51,6 +51,13 @@ __global__ void __launch_bounds__(kBlock) k_carry(int n, const int* __restrict__
   }
 }
 
+__global__ void k_wake_all(int nb, const int* __restrict__ btype, int* __restrict__ awake, float* __restrict__ sleep_time) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || btype[b] == 0) return;
+  awake[b] = 1;
+  sleep_time[b] = 0.0f;
+}
+
 static int carry_manifolds(World* w) {
   if (w->manifolds_valid[w->cur]) return B2G_OK;
   int n = w->n_pairs;
@@ -176,6 +183,12 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_
     w->h_pos.push_back(p);
     w->h_pos0.push_back(p);
     w->h_vel.push_back(make_float4(d.linear_velocity[0], d.linear_velocity[1], d.angular_velocity, 0.0f));
+    // B2body::new: awake unless the def says otherwise; static bodies are never awake
+    const int awake = d.type != B2G_STATIC && d.awake ? 1 : 0;
+    if (d.type != B2G_STATIC && !awake) w->created_asleep = true;
+    w->h_awake.push_back(awake);
+    w->h_auto_sleep.push_back(d.allow_sleep ? 1 : 0);
+    w->h_sleep_time.push_back(0.0f);
     w->h_mass.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     w->h_props.push_back(make_float4(0.0f, d.gravity_scale, d.linear_damping, d.angular_damping));
     w->h_I.push_back(0.0f);
@@ -259,12 +272,18 @@ static int down(std::vector<T>& h, const DevArray<T>& d, size_t n, hipStream_t s
 static int download_state(World* w) {
   if (w->dev_nb == 0 && w->dev_np == 0) return B2G_OK;
   std::vector<float4> pos, vel, pos0, xf, fat, faabb;
+  std::vector<int> awake;
+  std::vector<float> sleep_time;
   int st;
   if ((st = down(pos, w->b.pos, w->dev_nb, w->stream)) || (st = down(vel, w->b.vel, w->dev_nb, w->stream)) ||
       (st = down(pos0, w->b.pos0, w->dev_nb, w->stream)) || (st = down(xf, w->b.xf, w->dev_nb, w->stream)) ||
-      (st = down(fat, w->f.fat, w->dev_np, w->stream)) || (st = down(faabb, w->f.aabb, w->dev_np, w->stream)))
+      (st = down(fat, w->f.fat, w->dev_np, w->stream)) || (st = down(faabb, w->f.aabb, w->dev_np, w->stream)) ||
+      (st = down(awake, w->b.awake, w->dev_nb, w->stream)) ||
+      (st = down(sleep_time, w->b.sleep_time, w->dev_nb, w->stream)))
     return st;
   B2G_HIP(hipStreamSynchronize(w->stream));
+  std::copy(awake.begin(), awake.end(), w->h_awake.begin());
+  std::copy(sleep_time.begin(), sleep_time.end(), w->h_sleep_time.begin());
   std::copy(pos.begin(), pos.end(), w->h_pos.begin());
   std::copy(vel.begin(), vel.end(), w->h_vel.begin());
   std::copy(pos0.begin(), pos0.end(), w->h_pos0.begin());
@@ -305,6 +324,8 @@ int upload_world(World* w) {
       (st = up(w->f.mat, w->h_fmat, w->stream)) || (st = up(w->f.filter, w->h_ffilter, w->stream)) ||
       (st = up(w->f.aabb, w->h_faabb, w->stream)) || (st = up(w->f.fat, w->h_ffat, w->stream)) ||
       (st = up(w->f.shapev, w->h_shapev, w->stream)) || (st = up(w->f.world, w->h_fworld, w->stream)) ||
+      (st = up(w->b.awake, w->h_awake, w->stream)) || (st = up(w->b.sleep_time, w->h_sleep_time, w->stream)) ||
+      (st = up(w->b.auto_sleep, w->h_auto_sleep, w->stream)) || (st = up(w->b.wake, std::vector<int>(w->nb, 0), w->stream)) ||
       (st = w->f.moved.reserve(std::max(w->np, 1))))
     return st;
   // Big static proxies (ground, container walls: an AABB side over kLooseExtent) stay out of
@@ -445,7 +466,7 @@ int b2g_world_destroy(b2g_world* h) {
   World* w = &h->w;
   (void)hipStreamSynchronize(w->stream);
   auto rel = [](auto& a) { a.release(); };
-  rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force);
+  rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force); rel(w->b.awake); rel(w->b.sleep_time); rel(w->b.auto_sleep); rel(w->b.wake);
   rel(w->b.type);
   rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
   rel(w->f.shapev); rel(w->f.world); rel(w->f.tree_list); rel(w->f.loose_off); rel(w->f.loose);
@@ -456,7 +477,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
   rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);
   rel(s.plp); rel(s.pgeo); rel(s.plc); rel(s.prad); rel(s.deg); rel(s.adj_off); rel(s.adj_cursor); rel(s.adj);
-  rel(s.color_state); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.inv_order); rel(s.parent);
+  rel(s.color_state); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.inv_order); rel(s.parent); rel(s.isl_awake); rel(s.isl_restless);
   rel(s.isl_violated); rel(s.isl_done);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes); rel(bp.nodes4);
@@ -476,11 +497,26 @@ int b2g_world_destroy(b2g_world* h) {
 int b2g_world_set_flags(b2g_world* h, int32_t warm_starting, int32_t allow_sleep, int32_t continuous,
                         int32_t block_solve) {
   if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
-  if (allow_sleep || continuous) {
-    set_error("allow_sleep and continuous physics are not supported on the GPU path");
+  if (continuous) {
+    set_error("continuous physics is not supported on the GPU path");
     return B2G_ERR_UNSUPPORTED;
   }
-  h->w.warm_starting = warm_starting != 0;
+  if (allow_sleep && h->w.tape.active) {
+    set_error("allow_sleep cannot be switched on while a tape is recording");
+    return B2G_ERR_UNSUPPORTED;
+  }
+  World* w = &h->w;
+  if (w->allow_sleep && !allow_sleep && w->nb > 0) {
+    // B2world::set_allow_sleeping(false) wakes every body (src/private/dynamics/b2_world.rs:342-353)
+    int st;
+    if ((st = ensure_uploaded(w))) return st;
+    hipLaunchKernelGGL(k_wake_all, dim3(blocks_for(w->nb, 256)), dim3(256), 0, w->stream, w->nb, w->b.type.p,
+                       w->b.awake.p, w->b.sleep_time.p);
+    B2G_HIP(hipGetLastError());
+    B2G_HIP(hipStreamSynchronize(w->stream));
+  }
+  w->allow_sleep = allow_sleep != 0;
+  w->warm_starting = warm_starting != 0;
   h->w.block_solve = block_solve != 0;
   return B2G_OK;
 }
@@ -582,8 +618,11 @@ int b2g_world_get_body_states(b2g_world* h, b2g_body_state* out, int32_t n) {
   int st;
   if ((st = ensure_uploaded(w))) return st;
   std::vector<float4> pos, vel, pos0, xf;
+  std::vector<int> awake;
+  std::vector<float> sleep_time;
   if ((st = down(pos, w->b.pos, n, w->stream)) || (st = down(vel, w->b.vel, n, w->stream)) ||
-      (st = down(pos0, w->b.pos0, n, w->stream)) || (st = down(xf, w->b.xf, n, w->stream)))
+      (st = down(pos0, w->b.pos0, n, w->stream)) || (st = down(xf, w->b.xf, n, w->stream)) ||
+      (st = down(awake, w->b.awake, n, w->stream)) || (st = down(sleep_time, w->b.sleep_time, n, w->stream)))
     return st;
   B2G_HIP(hipStreamSynchronize(w->stream));
   for (int i = 0; i < n; ++i) {
@@ -592,8 +631,8 @@ int b2g_world_get_body_states(b2g_world* h, b2g_body_state* out, int32_t n) {
     s.c[0] = pos[i].x; s.c[1] = pos[i].y; s.a = pos[i].z;
     s.v[0] = vel[i].x; s.v[1] = vel[i].y; s.w = vel[i].z;
     s.c0[0] = pos0[i].x; s.c0[1] = pos0[i].y; s.a0 = pos0[i].z;
-    s.sleep_time = 0.0f;
-    s.awake = w->host_type[i] != B2G_STATIC ? 1.0f : 0.0f;
+    s.sleep_time = sleep_time[i];
+    s.awake = awake[i] ? 1.0f : 0.0f;
     s.pad = 0.0f;
   }
   return B2G_OK;
@@ -605,21 +644,30 @@ int b2g_world_set_body_states(b2g_world* h, const b2g_body_state* in, int32_t n)
   int st;
   if ((st = ensure_uploaded(w))) return st;
   std::vector<float4> pos(n), vel(n), pos0(n), xf(n);
+  std::vector<int> awake(n);
+  std::vector<float> sleep_time(n);
   for (int i = 0; i < n; ++i) {
     const b2g_body_state& s = in[i];
     pos[i] = make_float4(s.c[0], s.c[1], s.a, 0.0f);
     vel[i] = make_float4(s.v[0], s.v[1], s.w, 0.0f);
     pos0[i] = make_float4(s.c0[0], s.c0[1], s.a0, 0.0f);
     xf[i] = make_float4(s.p[0], s.p[1], s.q[0], s.q[1]);
+    const bool non_static = w->host_type[i] != B2G_STATIC;
+    awake[i] = non_static && s.awake != 0.0f ? 1 : 0;
+    sleep_time[i] = non_static ? s.sleep_time : 0.0f;
+    if (non_static && !awake[i]) w->created_asleep = true;
   }
   if ((st = up(w->b.pos, pos, w->stream)) || (st = up(w->b.vel, vel, w->stream)) ||
-      (st = up(w->b.pos0, pos0, w->stream)) || (st = up(w->b.xf, xf, w->stream)))
+      (st = up(w->b.pos0, pos0, w->stream)) || (st = up(w->b.xf, xf, w->stream)) ||
+      (st = up(w->b.awake, awake, w->stream)) || (st = up(w->b.sleep_time, sleep_time, w->stream)))
     return st;
   B2G_HIP(hipStreamSynchronize(w->stream));
   w->h_pos = pos;
   w->h_vel = vel;
   w->h_pos0 = pos0;
   w->h_xf = xf;
+  w->h_awake = awake;
+  w->h_sleep_time = sleep_time;
   return B2G_OK;
 }
 
@@ -794,6 +842,11 @@ int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* 
 int b2g_tape_begin(b2g_world* h, int32_t checkpoint_every) {
   if (!h || checkpoint_every < 1) { set_error("b2g_tape_begin: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;
+  if (w->sleep_possible()) {
+    // the frozen model replays every body every step: no sleeping bodies on a tape
+    set_error("b2g_tape_begin: the adjoint needs allow_sleep off and every body awake");
+    return B2G_ERR_UNSUPPORTED;
+  }
   int st;
   if ((st = ensure_uploaded(w))) return st;
   B2G_HIP(hipStreamSynchronize(w->stream));

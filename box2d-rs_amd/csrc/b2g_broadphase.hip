@@ -29,11 +29,12 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
                                                         const int* __restrict__ btype, const float4* __restrict__ pos0,
                                                         const float4* __restrict__ bmass, const float4* __restrict__ bxf,
                                                         float4* __restrict__ faabb, float4* __restrict__ ffat,
-                                                        int* __restrict__ moved) {
+                                                        int* __restrict__ moved, IslandView iv) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int b = fbody[i];
-  if (btype[b] == 0) { moved[i] = 0; return; }
+  // only bodies of this step's islands are synchronised (src/private/dynamics/b2_world.rs:482-500)
+  if (btype[b] == 0 || iv.asleep(b)) { moved[i] = 0; return; }
   const ShapeV& s = shapes[i];
   float4 p0 = pos0[b];
   float4 ms = bmass[b];
@@ -573,7 +574,8 @@ int phase_synchronize(World* w) {
   if (w->np == 0) return B2G_OK;
   hipLaunchKernelGGL(k_synchronize, dim3(blocks_for(w->np, kBlock)), dim3(kBlock), 0, w->stream, w->np, w->f.body.p,
                      w->f.shapev.p, w->b.type.p, w->b.pos0.p, w->b.mass.p, w->b.xf.p, w->f.aabb.p,
-                     w->f.fat.p, w->f.moved.p);
+                     w->f.fat.p, w->f.moved.p,
+                     IslandView{w->s.parent.p, w->sleep_possible() ? w->s.isl_awake.p : nullptr});
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }
@@ -588,6 +590,7 @@ int phase_broadphase(World* w) {
   int st;
   if (w->np == 0) {
     w->cur = next;
+    w->n_pairs_old = w->n_pairs;
     w->n_pairs = 0;
     return B2G_OK;
   }
@@ -601,6 +604,7 @@ int phase_broadphase(World* w) {
   }
   w->set_np[next] = w->np;
   w->cur = next;
+  w->n_pairs_old = w->n_pairs;
   w->n_pairs = n;
   return B2G_OK;
 }

@@ -77,6 +77,11 @@ struct BodyArrays {
   DevArray<float4> props;  // mass, gravity_scale, linear_damping, angular_damping
   DevArray<float4> force;  // force.x, force.y, torque, 0
   DevArray<int> type;      // 0 static, 1 kinematic, 2 dynamic
+  // sleep (used when the world can have sleeping bodies)
+  DevArray<int> awake;       // 1 awake, 0 asleep (static bodies: 0)
+  DevArray<float> sleep_time;
+  DevArray<int> auto_sleep;  // the body def's allow_sleep
+  DevArray<int> wake;        // wake request from the collide phase (touching changed, touching contact gone)
 };
 
 // Device-side proxy arrays (one proxy per fixture; index = fixture creation order, which is
@@ -136,6 +141,7 @@ struct SolverArrays {
   DevArray<int> inv_order;                      // compacted index -> colour-sorted constraint
   // islands
   DevArray<int> parent, isl_violated, isl_done;
+  DevArray<int> isl_awake, isl_restless;  // per island root: has an awake body; a body is not ready to sleep
 };
 
 // One internal LBVH node as a walk sees it: both child boxes, the children (internal node
@@ -219,6 +225,10 @@ struct World {
   // settings
   V2 gravity;
   bool warm_starting = true, allow_sleep = false, continuous = false, block_solve = true;
+  // sleep_possible: allow_sleep is on or a body was created asleep; otherwise every non-static body
+  // is awake at every step and the sleep bookkeeping is skipped
+  bool created_asleep = false;
+  bool sleep_possible() const { return allow_sleep || created_asleep; }
   float inv_dt0 = 0.0f;
   // host mirrors of static data
   std::vector<b2g_body_def> body_defs;
@@ -237,6 +247,8 @@ struct World {
   std::vector<int> h_tree, h_loose_off, h_loose, h_color_off_dev;
   int n_worlds = 1;
   std::vector<ShapeV> h_shapev;
+  std::vector<int> h_awake, h_auto_sleep;
+  std::vector<float> h_sleep_time;
   std::vector<HostShape> h_shapes;
   int nb = 0, np = 0;          // bodies, proxies on the host
   int dev_nb = 0, dev_np = 0;  // uploaded
@@ -251,6 +263,7 @@ struct World {
   ContactArrays c[2];  // current and previous contact sets
   int cur = 0;         // index of the current contact set in c[]
   int n_pairs = 0;     // pairs in the current set
+  int n_pairs_old = 0; // pairs in the previous set
   int set_np[2] = {0, 0};  // proxy count when each contact set was built (0: never built)
   SolverArrays s;
   BroadphaseArrays bp;
@@ -339,6 +352,14 @@ __host__ __device__ inline int float_flip(float f) {
   return i >= 0 ? i : i ^ 0x7fffffff;
 }
 __host__ __device__ inline float float_unflip(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+// Which bodies this step's solve left alone: an island (connected component of the touching
+// contacts through non-static bodies) is solved only when one of its bodies is awake.
+struct IslandView {
+  const int* parent;     // body -> island root
+  const int* isl_awake;  // per root; null: every island is awake
+  __device__ bool asleep(int b) const { return isl_awake && !isl_awake[parent[b]]; }
+};
 
 // body transform record (p.x, p.y, sin, cos) -> Xf
 __device__ inline Xf xf_of(float4 t) {

@@ -29,6 +29,10 @@ constexpr float kPi = 3.14159265358979323846f;
 constexpr float kAabbExtension = 0.1f;
 constexpr float kAabbMultiplier = 4.0f;
 constexpr float kLinearSlop = 0.005f;
+// sleep thresholds (src/b2_common.rs:85-91; length units per metre = 1)
+constexpr float kTimeToSleep = 0.5f;
+constexpr float kLinearSleepTolerance = 0.01f;
+constexpr float kAngularSleepTolerance = 2.0f / 180.0f * kPi;
 constexpr float kPolygonRadius = 2.0f * 0.005f;
 constexpr float kMaxLinearCorrection = 0.2f;
 constexpr float kMaxTranslation = 2.0f;

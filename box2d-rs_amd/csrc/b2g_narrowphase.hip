@@ -24,6 +24,20 @@ __device__ inline void store_manifold(int k, const Manifold& m, const float imp[
   ids[k] = make_uint2(m.id[0], m.id[1]);
 }
 
+// Sleep (when the world can have sleeping bodies: awake non-null). A contact none of whose bodies
+// is active (awake and not static) is not updated: it keeps its manifold and impulses
+// (src/private/dynamics/b2_contact_manager.rs:104-114). A contact whose touching state changes
+// asks to wake both bodies (src/private/dynamics/b2_contact.rs:203-206); the requests are applied
+// after the pass, so whether a contact is updated depends on the awake states at the start of
+// the pass (the reference's sequential pass lets a body woken by an earlier contact of its list
+// update later ones).
+struct SleepView {
+  const int* awake;     // null: every non-static body is awake
+  const int* btype;
+  int* wake;
+  const float4* ogeo;   // previous manifold geometry (copied for contacts that are not updated)
+};
+
 __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restrict__ pairs,
                                                     const int* __restrict__ old_index, const int* __restrict__ fbody,
                                                     const int4* __restrict__ fshape, const ShapeV* __restrict__ shapes,
@@ -31,6 +45,7 @@ __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restric
                                                     // previous manifolds
                                                     const int* __restrict__ ohdr, const float4* __restrict__ opt0,
                                                     const float4* __restrict__ opt1, const uint2* __restrict__ oids,
+                                                    SleepView sv,
                                                     // outputs
                                                     int2* __restrict__ ab, int* __restrict__ hdr,
                                                     float4* __restrict__ geo, float4* __restrict__ pt0,
@@ -45,41 +60,108 @@ __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restric
   int4 hq = fshape[lo], hp = fshape[hi];
   bool swap = registry_swap(hq.x & 0xff, hp.x & 0xff) == 1;
   int fa = swap ? hi : lo, fb = swap ? lo : hi;
+  const int bA = fbody[fa], bB = fbody[fb];
+  int oi = old_index[k];
+  ab[k] = make_int2(fa, fb);
+  if (sv.awake) {
+    const bool active_a = sv.btype[bA] != 0 && sv.awake[bA], active_b = sv.btype[bB] != 0 && sv.awake[bB];
+    if (!active_a && !active_b) {
+      if (oi >= 0) {
+        hdr[k] = ohdr[oi];
+        geo[k] = sv.ogeo[oi];
+        pt0[k] = opt0[oi];
+        pt1[k] = opt1[oi];
+        ids[k] = oids[oi];
+        touching[k] = (ohdr[oi] >> 8) > 0 ? 1 : 0;
+      } else {
+        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        hdr[k] = 0;
+        geo[k] = z;
+        pt0[k] = z;
+        pt1[k] = z;
+        ids[k] = make_uint2(0u, 0u);
+        touching[k] = 0;
+      }
+      return;
+    }
+  }
   // shapes are read in place from their records (no private copy of the vertex arrays)
   const ShapeV& sa = shapes[fa];
   const ShapeV& sb = shapes[fb];
-  Xf xa = xf_of(bxf[fbody[fa]]);
-  Xf xb = xf_of(bxf[fbody[fb]]);
+  Xf xa = xf_of(bxf[bA]);
+  Xf xb = xf_of(bxf[bB]);
   Manifold m;
   evaluate(&m, sa, xa, sb, xb);
   float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  int oi = old_index[k];
-  if (oi >= 0 && m.count > 0) {
-    int ocount = ohdr[oi] >> 8;
-    if (ocount > 0) {
-      uint2 oid = oids[oi];
-      float4 o0 = opt0[oi], o1 = opt1[oi];
-      for (int i = 0; i < m.count; ++i) {
-        if (m.id[i] == oid.x) { imp[2 * i] = o0.z; imp[2 * i + 1] = o0.w; }
-        else if (ocount > 1 && m.id[i] == oid.y) { imp[2 * i] = o1.z; imp[2 * i + 1] = o1.w; }
-      }
+  const int ocount = oi >= 0 ? ohdr[oi] >> 8 : 0;
+  if (m.count > 0 && ocount > 0) {
+    uint2 oid = oids[oi];
+    float4 o0 = opt0[oi], o1 = opt1[oi];
+    for (int i = 0; i < m.count; ++i) {
+      if (m.id[i] == oid.x) { imp[2 * i] = o0.z; imp[2 * i + 1] = o0.w; }
+      else if (ocount > 1 && m.id[i] == oid.y) { imp[2 * i] = o1.z; imp[2 * i + 1] = o1.w; }
     }
   }
-  ab[k] = make_int2(fa, fb);
   store_manifold(k, m, imp, hdr, geo, pt0, pt1, ids);
   touching[k] = m.count > 0 ? 1 : 0;
+  if (sv.awake && (m.count > 0) != (ocount > 0)) {
+    if (sv.btype[bA] != 0) sv.wake[bA] = 1;
+    if (sv.btype[bB] != 0) sv.wake[bB] = 1;
+  }
+}
+
+// b2_contact_destroy wakes both bodies of a touching contact (src/private/dynamics/b2_contact.rs:
+// 33-45); the collide pass destroys a contact with an active body whose fat AABBs stopped
+// overlapping (src/private/dynamics/b2_contact_manager.rs:117-127). Those are the previous set's
+// pairs that are not in the current set.
+__global__ void __launch_bounds__(kBlock) k_wake_destroyed(int n_old, const int2* __restrict__ old_pairs,
+                                                           const int* __restrict__ ohdr, const int* __restrict__ fbody,
+                                                           const float4* __restrict__ fat, SleepView sv) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_old || (ohdr[k] >> 8) == 0) return;
+  const int2 pr = old_pairs[k];
+  const int bA = fbody[pr.x], bB = fbody[pr.y];
+  const bool active_a = sv.btype[bA] != 0 && sv.awake[bA], active_b = sv.btype[bB] != 0 && sv.awake[bB];
+  if (!active_a && !active_b) return;
+  const float4 fa = fat[pr.x], fb = fat[pr.y];
+  Aabb a, b;
+  a.lo = v2(fa.x, fa.y);
+  a.hi = v2(fa.z, fa.w);
+  b.lo = v2(fb.x, fb.y);
+  b.hi = v2(fb.z, fb.w);
+  if (overlap(a, b)) return;
+  if (sv.btype[bA] != 0) sv.wake[bA] = 1;
+  if (sv.btype[bB] != 0) sv.wake[bB] = 1;
+}
+
+// B2body::set_awake(true) for the bodies asked to wake (src/private/dynamics/b2_body.rs:783-801)
+__global__ void __launch_bounds__(kBlock) k_apply_wakes(int nb, int* __restrict__ wake, int* __restrict__ awake,
+                                                        float* __restrict__ sleep_time) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || !wake[b]) return;
+  wake[b] = 0;
+  awake[b] = 1;
+  sleep_time[b] = 0.0f;
 }
 
 int phase_collide(World* w) {
   int n = w->n_pairs;
   int st;
   if ((st = w->s.touch_flag.reserve(std::max(n, 1)))) return st;
-  if (n == 0) return B2G_OK;
   ContactArrays& c = w->c[w->cur];
   ContactArrays& o = w->c[1 - w->cur];
-  launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), n, c.pair.p,
-               c.old_index.p,
-               w->f.body.p, w->f.shape.p, w->f.shapev.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p, o.ids.p, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
+  const bool sleepy = w->sleep_possible();
+  SleepView sv{sleepy ? w->b.awake.p : nullptr, w->b.type.p, w->b.wake.p, o.geo.p};
+  if (sleepy && w->n_pairs_old > 0)
+    hipLaunchKernelGGL(k_wake_destroyed, dim3(blocks_for(w->n_pairs_old, kBlock)), dim3(kBlock), 0, w->stream,
+                       w->n_pairs_old, o.pair.p, o.hdr.p, w->f.body.p, w->f.fat.p, sv);
+  if (n > 0)
+    launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), n, c.pair.p,
+                 c.old_index.p, w->f.body.p, w->f.shape.p, w->f.shapev.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p,
+                 o.ids.p, sv, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
+  if (sleepy && w->nb > 0)
+    hipLaunchKernelGGL(k_apply_wakes, dim3(blocks_for(w->nb, kBlock)), dim3(kBlock), 0, w->stream, w->nb,
+                       w->b.wake.p, w->b.awake.p, w->b.sleep_time.p);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }

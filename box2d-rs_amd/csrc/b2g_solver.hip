@@ -23,16 +23,21 @@ static constexpr int kMaxColors = 128;
 
 // ---- integration --------------------------------------------------------------------------
 // src/private/dynamics/b2_island_private.rs:135-170
+// Islands with no awake body are not solved: their bodies keep their state. Bodies of awake
+// islands are set awake (the island DFS of src/private/dynamics/b2_world.rs:437-447, which does
+// not reset the sleep time).
+
 __global__ void __launch_bounds__(kBlock) k_integrate_velocities(int nb, const int* __restrict__ btype,
                                                                  const float4* __restrict__ pos, float4* __restrict__ vel,
                                                                  float4* __restrict__ pos0, const float4* __restrict__ props,
                                                                  const float4* __restrict__ bmass,
                                                                  const float4* __restrict__ force, float gx, float gy,
-                                                                 float h) {
+                                                                 float h, IslandView iv, int* __restrict__ awake) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   int t = btype[b];
-  if (t == 0) return;
+  if (t == 0 || iv.asleep(b)) return;
+  if (awake) awake[b] = 1;
   pos0[b] = pos[b];
   if (t != 2) return;
   float4 v = vel[b];
@@ -54,8 +59,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate_velocities(int nb, const i
 
 // src/private/dynamics/b2_island_private.rs:221-252
 __device__ inline void integrate_position_one(int b, const int* __restrict__ btype, float4* __restrict__ pos,
-                                              float4* __restrict__ vel, float h) {
-  if (btype[b] == 0) return;
+                                              float4* __restrict__ vel, float h, const IslandView& iv) {
+  if (btype[b] == 0 || iv.asleep(b)) return;
   float4 p = pos[b], v = vel[b];
   V2 tr = h * v2(v.x, v.y);
   if (dot(tr, tr) > kMaxTranslation * kMaxTranslation) {
@@ -75,26 +80,65 @@ __device__ inline void integrate_position_one(int b, const int* __restrict__ bty
   vel[b] = v;
 }
 __global__ void __launch_bounds__(kBlock) k_integrate_positions(int nb, const int* __restrict__ btype,
-                                                                float4* __restrict__ pos, float4* __restrict__ vel, float h) {
+                                                                float4* __restrict__ pos, float4* __restrict__ vel, float h,
+                                                                IslandView iv) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) integrate_position_one(b, btype, pos, vel, h);
+  if (b < nb) integrate_position_one(b, btype, pos, vel, h, iv);
 }
 
 // copy back + synchronize_transform (src/private/dynamics/b2_island_private.rs:276-285,
 // src/b2_body.rs:974-977). Static bodies keep the transform computed at creation (their
 // angle never changes, so the reference recomputes the same value).
 __device__ inline void finalize_one(int b, const int* __restrict__ btype, const float4* __restrict__ pos,
-                                    const float4* __restrict__ bmass, float4* __restrict__ xf) {
-  if (btype[b] == 0) return;
+                                    const float4* __restrict__ bmass, float4* __restrict__ xf, const IslandView& iv) {
+  if (btype[b] == 0 || iv.asleep(b)) return;
   float4 p = pos[b], ms = bmass[b];
   Rot q = rot(p.z);
   V2 o = v2(p.x, p.y) - mul(q, v2(ms.z, ms.w));
   xf[b] = make_float4(o.x, o.y, q.s, q.c);
 }
 __global__ void __launch_bounds__(kBlock) k_finalize(int nb, const int* __restrict__ btype, const float4* __restrict__ pos,
-                                                     const float4* __restrict__ bmass, float4* __restrict__ xf) {
+                                                     const float4* __restrict__ bmass, float4* __restrict__ xf,
+                                                     IslandView iv) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) finalize_one(b, btype, pos, bmass, xf);
+  if (b < nb) finalize_one(b, btype, pos, bmass, xf, iv);
+}
+
+// Island sleep (src/private/dynamics/b2_island_private.rs:291-327), in two passes over the bodies
+// of awake islands: each body's sleep time is reset when it may not sleep or moves faster than
+// the tolerances, else advanced by h; an island whose bodies all reached kTimeToSleep (no body
+// marks it restless) and whose position iterations converged goes to sleep.
+__global__ void __launch_bounds__(kBlock) k_sleep_time(int nb, const int* __restrict__ btype,
+                                                       const int* __restrict__ auto_sleep, const float4* __restrict__ vel,
+                                                       float* __restrict__ sleep_time, IslandView iv,
+                                                       int* __restrict__ restless, float h) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || btype[b] == 0 || iv.asleep(b)) return;
+  const float lin_tol_sqr = kLinearSleepTolerance * kLinearSleepTolerance;
+  const float ang_tol_sqr = kAngularSleepTolerance * kAngularSleepTolerance;
+  const float4 v = vel[b];
+  float t;
+  if (!auto_sleep[b] || v.z * v.z > ang_tol_sqr || v.x * v.x + v.y * v.y > lin_tol_sqr) {
+    t = 0.0f;
+  } else {
+    t = sleep_time[b] + h;
+  }
+  sleep_time[b] = t;
+  if (!(t >= kTimeToSleep)) restless[iv.parent[b]] = 1;
+}
+__global__ void __launch_bounds__(kBlock) k_sleep_apply(int nb, const int* __restrict__ btype, IslandView iv,
+                                                        const int* __restrict__ restless, const int* __restrict__ done,
+                                                        int* __restrict__ awake, float* __restrict__ sleep_time,
+                                                        float4* __restrict__ vel, float4* __restrict__ force) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || btype[b] == 0 || iv.asleep(b)) return;
+  const int r = iv.parent[b];
+  if (restless[r] || !done[r]) return;
+  // B2body::set_awake(false)
+  awake[b] = 0;
+  sleep_time[b] = 0.0f;
+  vel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---- touching contacts, islands, colouring -----------------------------------------------------
@@ -221,7 +265,8 @@ __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, c
 }
 
 __global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* __restrict__ violated,
-                                                     int* __restrict__ done) {
+                                                     int* __restrict__ done, int* __restrict__ isl_awake,
+                                                     int* __restrict__ restless) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   // Read-only walk to the root; each thread writes only its own entry. (Halving writes here
@@ -231,6 +276,19 @@ __global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* _
   parent[b] = r;
   violated[b] = 0;
   done[b] = 0;
+  if (isl_awake) {
+    isl_awake[b] = 0;
+    restless[b] = 0;
+  }
+}
+
+// An island is solved when one of its bodies is awake (the reference seeds its island search
+// only at awake bodies: src/private/dynamics/b2_world.rs:389-400).
+__global__ void __launch_bounds__(kBlock) k_island_awake(int nb, const int* __restrict__ btype,
+                                                         const int* __restrict__ awake, const int* __restrict__ parent,
+                                                         int* __restrict__ isl_awake) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb && btype[b] != 0 && awake[b]) isl_awake[parent[b]] = 1;
 }
 
 __device__ inline unsigned long long priority(int j) {
@@ -311,13 +369,20 @@ __device__ inline int color_pick(int j, int round, const int2* __restrict__ bodi
 // in other ways). Reads only prev_colour, so the outcome does not depend on the launch order.
 // Colours at or above `cap` are not kept: those contacts are coloured again first-fit, which
 // pulls the colour count back down as the contact graph changes.
+// Constraints of sleeping islands are left out of the solve (state -2, never coloured).
 __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                        const int* __restrict__ btype, const int* __restrict__ off,
                                                        const int* __restrict__ adj, const int* __restrict__ prev_colour,
-                                                       int cap, int* __restrict__ state, int* __restrict__ remaining) {
+                                                       int cap, IslandView iv, int* __restrict__ state,
+                                                       int* __restrict__ remaining) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  bool kept = false;
-  if (j < *ncp) {
+  bool kept = false, left_out = false;
+  if (j < *ncp && iv.isl_awake) {
+    const int2 b = bodies[j];
+    left_out = iv.asleep(btype[b.x] != 0 ? b.x : b.y);
+    if (left_out) state[j] = -2;
+  }
+  if (j < *ncp && !left_out) {
     const int pc = prev_colour[j];
     if (pc >= 0 && pc < cap) {
       kept = true;
@@ -349,7 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ n
     }
     state[j] = kept ? pc : -1;  // round 0
   }
-  const unsigned long long m = __ballot(kept);
+  const unsigned long long m = __ballot(kept || left_out);
   if (m && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1) atomicSub(remaining, __popcll(m));
 }
 
@@ -407,7 +472,7 @@ __global__ void __launch_bounds__(kBlock) k_color_scatter(int n, const int* __re
   __syncthreads();
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int c = -1, r = 0;
-  if (j < n) {
+  if (j < n && state[j] >= 0) {  // left-out constraints (sleeping islands) have no colour
     c = state[j] & 0xff;
     r = atomicAdd(&h[c], 1);
   }
@@ -594,9 +659,10 @@ __device__ inline void warm_apply(int b, int i, const VelArgs& a, V2& v, float& 
 static constexpr int kWarmSort = 8;
 __global__ void __launch_bounds__(kBlock) k_warm_start_bodies(int nb, const int* __restrict__ btype,
                                                               const int* __restrict__ off, const int* __restrict__ adj,
-                                                              const int* __restrict__ inv_order, VelArgs a) {
+                                                              const int* __restrict__ inv_order, VelArgs a,
+                                                              IslandView iv) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb || btype[b] != 2) return;
+  if (b >= nb || btype[b] != 2 || iv.asleep(b)) return;
   const int s0 = off[b], e0 = off[b + 1], n = e0 - s0;
   if (n == 0) return;
   const float4 vb = a.vel[b];
@@ -888,8 +954,9 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
 // of iterations it ran (the adjoint's tape reads it).
 __device__ inline void iteration_end_one(int b, int iteration, const int* __restrict__ btype,
                                          const int* __restrict__ parent, int* __restrict__ violated,
-                                         int* __restrict__ done, int* __restrict__ active) {
-  if (btype[b] == 0 || parent[b] != b || done[b]) return;
+                                         int* __restrict__ done, int* __restrict__ active,
+                                         const int* __restrict__ isl_awake) {
+  if (btype[b] == 0 || parent[b] != b || done[b] || (isl_awake && !isl_awake[b])) return;
   if (!violated[b]) done[b] = iteration + 1;
   else atomicAdd(active, 1);
   violated[b] = 0;
@@ -897,9 +964,10 @@ __device__ inline void iteration_end_one(int b, int iteration, const int* __rest
 __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
                                                                    const int* __restrict__ parent,
                                                                    int* __restrict__ violated, int* __restrict__ done,
-                                                                   int* __restrict__ active) {
+                                                                   int* __restrict__ active,
+                                                                   const int* __restrict__ isl_awake) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) iteration_end_one(b, iteration, btype, parent, violated, done, active);
+  if (b < nb) iteration_end_one(b, iteration, btype, parent, violated, done, active, isl_awake);
 }
 
 // ---- graph-mode variants: colour ranges and counts read from device memory -----------------
@@ -944,22 +1012,23 @@ static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& v
   hipStream_t st = w->stream;
   if (k.warm)
     hipLaunchKernelGGL(k_warm_start_bodies, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p,
-                       s.adj_off.p, s.adj.p, s.inv_order.p, va);
+                       s.adj_off.p, s.adj.p, s.inv_order.p, va, IslandView{s.parent.p, nullptr});
   for (int it = 0; it < k.vel_iters; ++it)
     for (int col = 0; col < k.ncolors; ++col)
       hipLaunchKernelGGL(k_solve_velocity_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, k.block, va);
   hipLaunchKernelGGL(k_store_impulses_g, dim3(k.grid_nc), dim3(kBlock), 0, st, cnt + 0, s.k_contact.p, s.k_info.p,
                      s.lam.p, c.pt0.p, c.pt1.p);
+  const IslandView all_awake{s.parent.p, nullptr};
   hipLaunchKernelGGL(k_integrate_positions, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p,
-                     w->b.pos.p, w->b.vel.p, k.dt);
+                     w->b.pos.p, w->b.vel.p, k.dt, all_awake);
   for (int it = 0; it < k.pos_iters; ++it) {
     for (int col = 0; col < k.ncolors; ++col)
       hipLaunchKernelGGL(k_solve_position_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, pa);
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, it,
-                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3);
+                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, nullptr);
   }
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p, w->b.pos.p,
-                     w->b.mass.p, w->b.xf.p);
+                     w->b.mass.p, w->b.xf.p, all_awake);
 }
 
 void release_solve_graphs(World* w) {
@@ -1027,14 +1096,14 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   ContactArrays& c = w->c[w->cur];
   int st;
   int* cnt = w->counters.p;  // [0] touching, [1] uncoloured, [2] colour overflow, [3] active islands
-  hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
-                     w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p, w->gravity.x,
-                     w->gravity.y, dt);
+  // sleeping bodies: islands without an awake body are left out of the solve
+  const bool sleepy = w->sleep_possible();
   // touching contacts: compacted on the device; the constraint arrays are sized for all pairs
   const int n_pairs = w->n_pairs;
   if ((st = reserve_solver(w, n_pairs))) return st;
   if ((st = s.deg.reserve(nb + 1)) || (st = s.adj_off.reserve(nb + 1)) || (st = s.adj_cursor.reserve(nb)) ||
       (st = s.parent.reserve(nb)) || (st = s.isl_violated.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
+      (sleepy && ((st = s.isl_awake.reserve(nb)) || (st = s.isl_restless.reserve(nb)))) ||
       (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
       (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(std::max(n_pairs, 1))))
     return st;
@@ -1063,8 +1132,15 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     hipLaunchKernelGGL(k_union, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p, s.parent.p);
   }
   hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
-                     s.isl_violated.p, s.isl_done.p);
-  int ncolors = 0, nc = 0;
+                     s.isl_violated.p, s.isl_done.p, sleepy ? s.isl_awake.p : nullptr, s.isl_restless.p);
+  const IslandView iv{s.parent.p, sleepy ? s.isl_awake.p : nullptr};
+  if (sleepy)
+    hipLaunchKernelGGL(k_island_awake, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
+                       w->b.awake.p, s.parent.p, s.isl_awake.p);
+  hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
+                     w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p, w->gravity.x,
+                     w->gravity.y, dt, iv, sleepy ? w->b.awake.p : nullptr);
+  int ncolors = 0, nc = 0, n_touching = 0;
   if (n_pairs > 0) {
     if ((st = exclusive_scan(w, s.deg.p, s.adj_off.p, nb + 1, nullptr))) return st;
     hipLaunchKernelGGL(k_fill_adjacency, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
@@ -1073,7 +1149,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // others, a first batch without waiting (rounds after the last constraint is coloured find
     // nothing to do), then more only if some remain
     hipLaunchKernelGGL(k_color_keep, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
-                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, s.color_state.p, cnt + 1);
+                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, iv, s.color_state.p, cnt + 1);
     // first batch: from scratch the configured count; with reuse one more than the rounds the
     // last reusing step needed (the batch size does not change the colouring)
     int round = 1;
@@ -1098,7 +1174,9 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
     w->color_rounds_used = round - 1;
     if (reuse) w->reuse_rounds = w->h_counters[4 + kMaxColors];
-    nc = w->h_counters[0];
+    // constraints to solve: the coloured ones (touching contacts of awake islands)
+    n_touching = w->h_counters[0];
+    for (int i = 0; i < kMaxColors; ++i) nc += w->h_counters[4 + i];
     if (w->h_counters[2]) {
       set_error("colouring needs more than 127 colours (a dynamic body with too many contacts)");
       return B2G_ERR_CAPACITY;
@@ -1124,7 +1202,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
     for (int i = 0; i <= kMaxColors; ++i) offs[i] = i <= ncolors ? w->h_color_off[i] : nc;
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
-    hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc,
+    hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream, n_touching,
                        s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.inv_order.p,
                        s.touch_list.p, c.colour.p);
     InitArgs ia;
@@ -1173,7 +1251,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   // (graph nodes cannot carry the timing events of a timed solve kernel)
   const bool timed_tail = w->kt.kernel == B2G_KERNEL_WARM_START || w->kt.kernel == B2G_KERNEL_SOLVE_VELOCITY ||
                           w->kt.kernel == B2G_KERNEL_SOLVE_POSITION;
-  if (nc > 0 && w->solver_graphs && !timed_tail) {
+  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy) {
     // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
     int maxc = 0;
     for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);
@@ -1205,7 +1283,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   }
   if (nc > 0 && w->warm_starting)
     launch_timed(w, B2G_KERNEL_WARM_START, nc, k_warm_start_bodies, dim3(blocks_for(nb, kBlock)), dim3(kBlock), nb,
-                 w->b.type.p, s.adj_off.p, s.adj.p, s.inv_order.p, va);
+                 w->b.type.p, s.adj_off.p, s.adj.p, s.inv_order.p, va, iv);
   for (int it = 0; it < vel_iters && nc > 0; ++it)
     for (int col = 0; col < ncolors; ++col) {
       int b0 = co[col], b1 = co[col + 1];
@@ -1216,7 +1294,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     hipLaunchKernelGGL(k_store_impulses, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_contact.p,
                        s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p);
   hipLaunchKernelGGL(k_integrate_positions, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
-                     w->b.pos.p, w->b.vel.p, dt);
+                     w->b.pos.p, w->b.vel.p, dt, iv);
   w->pos_iters_run = 0;
   for (int it = 0; it < pos_iters; ++it) {
     for (int col = 0; col < ncolors; ++col) {
@@ -1225,11 +1303,17 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                    dim3(kBlock), b0, b1, pa);
     }
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
-                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3);
+                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake);
     ++w->pos_iters_run;
   }
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p, w->b.pos.p,
-                     w->b.mass.p, w->b.xf.p);
+                     w->b.mass.p, w->b.xf.p, iv);
+  if (w->allow_sleep) {
+    hipLaunchKernelGGL(k_sleep_time, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
+                       w->b.auto_sleep.p, w->b.vel.p, w->b.sleep_time.p, iv, s.isl_restless.p, dt);
+    hipLaunchKernelGGL(k_sleep_apply, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p, iv,
+                       s.isl_restless.p, s.isl_done.p, w->b.awake.p, w->b.sleep_time.p, w->b.vel.p, w->b.force.p);
+  }
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }

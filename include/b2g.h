@@ -113,8 +113,13 @@ int b2g_world_create(float gravity_x, float gravity_y, b2g_world** out);
 int b2g_world_destroy(b2g_world* world);
 
 /* B2world::set_warm_starting / set_allow_sleeping / set_continuous_physics
- * (src/b2_world.rs:164-185) and G_BLOCK_SOLVE (src/b2_contact.rs:25). allow_sleep and
- * continuous must be 0: sleeping islands and TOI sub-stepping are not on the GPU path. */
+ * (src/b2_world.rs:164-185) and G_BLOCK_SOLVE (src/b2_contact.rs:25). continuous must be 0:
+ * TOI sub-stepping is not on the GPU path. allow_sleep works as in the reference (island sleep
+ * after B2_TIME_TO_SLEEP below the sleep tolerances, waking on touch changes; switching it off
+ * wakes every body), with one difference: whether a contact is updated in the collide phase
+ * depends on its bodies' awake states at the start of the phase, where the reference's
+ * sequential pass lets a body woken by an earlier contact of its list update later ones. A
+ * world with sleeping bodies cannot record an adjoint tape. */
 int b2g_world_set_flags(b2g_world* world, int32_t warm_starting, int32_t allow_sleep, int32_t continuous,
                         int32_t block_solve);
 

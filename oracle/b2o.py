@@ -84,7 +84,7 @@ class Oracle:
         L.b2o_rot.argtypes = [I, P, P]
         L.b2o_brute_pairs.argtypes = [I, P, ctypes.c_longlong, P]
         L.b2o_brute_pairs.restype = ctypes.c_longlong
-        L.b2o_solver_replay.argtypes = [I, P, P, P, I, P, P, P, P, P, P, P]
+        L.b2o_solver_replay.argtypes = [I, P, P, P, I, P, P, P, P, P, P, P, P, I]
         L.b2o_solver_replay.restype = I
 
     # -- conversion helpers ---------------------------------------------------------------
@@ -157,7 +157,8 @@ class Oracle:
         return out[:k]
 
     def solver_replay(self, body_type, body_props, state, contact_bodies, contact_params, manifolds, order,
-                      dt, dt_ratio, gravity, vel_iters=8, pos_iters=3, warm_starting=True, block_solve=True):
+                      dt, dt_ratio, gravity, vel_iters=8, pos_iters=3, warm_starting=True, block_solve=True,
+                      body_flags=None, allow_sleep=False):
         nb = len(body_type)
         nc = len(contact_bodies)
         bt = np.ascontiguousarray(body_type, np.int32)
@@ -170,8 +171,10 @@ class Oracle:
         sp = np.array([dt, dt_ratio, gravity[0], gravity[1]], self.real)
         it = np.array([vel_iters, pos_iters, int(warm_starting), int(block_solve)], np.int32)
         isl = np.zeros(nb, np.int32)
+        fl = None if body_flags is None else np.ascontiguousarray(body_flags, np.int32)
         n_islands = self.lib.b2o_solver_replay(nb, _ptr(bt), _ptr(bp), _ptr(st), nc, _ptr(cb), _ptr(cp), _ptr(mf),
-                                               _ptr(od), _ptr(sp), _ptr(it), _ptr(isl))
+                                               _ptr(od), _ptr(sp), _ptr(it), _ptr(isl),
+                                               None if fl is None else _ptr(fl), int(allow_sleep))
         return st, mf, isl, n_islands
 
     def world(self, gravity=(0.0, -10.0)):

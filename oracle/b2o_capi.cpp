@@ -424,12 +424,17 @@ long long b2o_brute_pairs(int n, const real* fat, long long cap, int32_t* out) {
 //   contacts: body pair, (friction, restitution, threshold, tangent_speed, radiusA, radiusB),
 //   manifold in/out (impulses stored back).
 // step = (dt, dt_ratio, gx, gy); iters = (velocity, position, warm_starting, block_solve).
+// Sleep: body_flags (null: every body awake, may sleep) = awake | allow_sleep << 1 per body, with
+// the sleep time in state; an island is solved only if one of its bodies is awake (all of them
+// are then set awake, src/private/dynamics/b2_world.rs:389-447), and with allow_sleep the island
+// sleep rule of solve_island runs (b2_island_private.rs:291-327).
 // Returns the number of islands.
 int b2o_solver_replay(int nb, const int32_t* body_type, const real* body_props, B2oBodyState* state,
                       int nc, const int32_t* contact_bodies, const real* contact_params, B2oManifold* manifolds,
-                      const int32_t* order, const real* step_params, const int32_t* iters, int32_t* island_of_body) {
+                      const int32_t* order, const real* step_params, const int32_t* iters, int32_t* island_of_body,
+                      const int32_t* body_flags, int32_t allow_sleep) {
   World w(Vec2(step_params[2], step_params[3]));
-  w.allow_sleep = false;
+  w.allow_sleep = allow_sleep != 0;
   w.block_solve = iters[3] != 0;
   std::vector<Body> bodies(nb);
   for (int i = 0; i < nb; ++i) {
@@ -445,6 +450,15 @@ int b2o_solver_replay(int nb, const int32_t* body_type, const real* body_props, 
     b.sweep.a = state[i].a;
     b.v = Vec2(state[i].v[0], state[i].v[1]);
     b.w = state[i].w;
+    // (bodies of islands that are not solved keep these)
+    b.sweep.c0 = Vec2(state[i].c0[0], state[i].c0[1]);
+    b.sweep.a0 = state[i].a0;
+    b.xf.p = Vec2(state[i].p[0], state[i].p[1]);
+    b.xf.q.s = state[i].q[0];
+    b.xf.q.c = state[i].q[1];
+    b.awake = b.type != kStatic && (body_flags ? (body_flags[i] & 1) != 0 : true);
+    b.auto_sleep = body_flags ? (body_flags[i] & 2) != 0 : true;
+    b.sleep_time = state[i].sleep_time;
   }
   std::vector<Fixture> fx(2 * nc);
   std::vector<Contact> cs(nc);
@@ -500,6 +514,10 @@ int b2o_solver_replay(int nb, const int32_t* body_type, const real* body_props, 
       for (Body* b : {c->fa->body, c->fb->body})
         if (b->type == kStatic && std::find(statics.begin(), statics.end(), b) == statics.end()) statics.push_back(b);
     }
+    bool any_awake = false;
+    for (Body* b : isl.bodies) any_awake = any_awake || b->awake;
+    if (!any_awake) continue;
+    for (Body* b : isl.bodies) b->awake = true;
     for (Body* b : statics) isl.bodies.push_back(b);
     for (size_t i = 0; i < isl.bodies.size(); ++i) isl.bodies[i]->island_index = int(i);
     solve_island(&w, isl, st);
@@ -512,6 +530,8 @@ int b2o_solver_replay(int nb, const int32_t* body_type, const real* body_props, 
     s.v[0] = b.v.x; s.v[1] = b.v.y; s.w = b.w;
     s.c0[0] = b.sweep.c0.x; s.c0[1] = b.sweep.c0.y; s.a0 = b.sweep.a0;
     s.p[0] = b.xf.p.x; s.p[1] = b.xf.p.y; s.q[0] = b.xf.q.s; s.q[1] = b.xf.q.c;
+    s.sleep_time = b.sleep_time;
+    s.awake = b.awake ? real(1) : real(0);
   }
   for (int k = 0; k < nc; ++k) manifold_to_rec(cs[k].manifold, manifolds + k);
   return int(islands.size());
