@@ -11,7 +11,10 @@ Closed forms, brute force and invariants:
   free fall (symplectic Euler), Pade damping, circle-circle manifold, box/circle mass,
   resting-box impulse = m g h, hull vs an independent monotone-chain hull, SAT depth vs an
   independent f64 separating-axis computation, 2-point block LCP vs exhaustive enumeration,
-  linear momentum conservation and non-negative normal impulses.
+  linear momentum conservation and non-negative normal impulses; sleep: a box placed at rest
+  sleeps in the step whose f32 sum of h reaches the 0.5 s time to sleep, with zero velocity and
+  the sleep time reset, never when the world or one body of its island forbids sleeping, and a
+  stack sleeps as one island and stays frozen.
 Each check is chosen so that a plausible slip (dropped term, wrong sign, swapped operand,
 wrong index) fails it.
 """
@@ -40,6 +43,7 @@ def _world(oracle, bodies, fixtures, gravity=(0.0, -10.0)):
         bd[i]["angular_velocity"] = b.get("w", 0)
         bd[i]["linear_damping"] = b.get("ld", 0)
         bd[i]["angular_damping"] = b.get("ad", 0)
+        bd[i]["allow_sleep"] = b.get("allow_sleep", 1)
     sd = np.zeros(len(fixtures), dt["shape_def"])
     fd = np.zeros(len(fixtures), dt["fixture_def"])
     fb = np.zeros(len(fixtures), np.int32)
@@ -443,3 +447,85 @@ def test_normal_impulses_non_negative(oracle):
     valid = np.arange(2)[None, :] < pc[:, None]
     assert valid.sum() > 150
     assert np.all(ni[valid] >= 0)
+
+
+# ---- sleep (src/private/dynamics/b2_island_private.rs:291-327, src/b2_common.rs:85-91) ----
+def _resting_boxes(oracle, n_boxes=1, allow=None, world_sleep=True):
+    """Boxes stacked at their resting heights on a static ground box (top at y = 0.5): box i
+    at y = 1.015 + 1.015 i (half 0.5, two polygon skins 0.02, minus the 0.005 slop)."""
+    allow = allow or [1] * n_boxes
+    bodies = [{"type": 0, "position": (0, 0)}] + [{"position": (0, 1.015 * (i + 1)), "allow_sleep": allow[i]}
+                                                  for i in range(n_boxes)]
+    fixtures = [(0, _box(5, 0.5), 0.0, 0.6)] + [(i + 1, _box(0.5, 0.5), 1.0, 0.6) for i in range(n_boxes)]
+    w, _ = _world(oracle, bodies, fixtures)
+    w.set_flags(warm_starting=True, allow_sleep=world_sleep, block_solve=True)
+    return w
+
+
+def _steps_to_time_to_sleep(h=np.float32(1.0 / 60.0)):
+    t, n = np.float32(0.0), 0
+    while not t >= np.float32(0.5):
+        t = np.float32(t + h)
+        n += 1
+    return n
+
+
+def test_resting_box_sleeps_after_time_to_sleep(oracle):
+    """A box placed at rest stays below the sleep tolerances from the first step, so its sleep
+    time grows by h each step (f32 sum) and it sleeps in the step that brings it to 0.5 s,
+    with zero velocity and the sleep time reset."""
+    w = _resting_boxes(oracle)
+    n = _steps_to_time_to_sleep()
+    for k in range(1, n + 1):
+        w.step(1.0 / 60.0, 8, 3)
+        if k < n:
+            assert w.bodies()[1]["awake"] == 1
+    b = w.bodies()[1]
+    assert b["awake"] == 0 and b["sleep_time"] == 0 and b["w"] == 0 and np.all(b["v"] == 0)
+
+
+def test_sleep_time_sums_h_in_float(oracle):
+    w = _resting_boxes(oracle)
+    h = np.float32(1.0 / 60.0)
+    t = np.float32(0.0)
+    for _ in range(10):
+        w.step(1.0 / 60.0, 8, 3)
+        t = np.float32(t + h)
+        assert w.bodies()[1]["sleep_time"] == t
+
+
+def test_no_sleep_when_the_world_forbids_it(oracle):
+    w = _resting_boxes(oracle, world_sleep=False)
+    for _ in range(3 * _steps_to_time_to_sleep()):
+        w.step(1.0 / 60.0, 8, 3)
+    b = w.bodies()[1]
+    assert b["awake"] == 1 and b["sleep_time"] == 0
+
+
+def test_stack_sleeps_as_one_island_and_stays_frozen(oracle):
+    """Two stacked boxes form one island: both fall asleep in the same step; asleep, nothing
+    moves."""
+    w = _resting_boxes(oracle, n_boxes=2)
+    n = _steps_to_time_to_sleep()
+    for _ in range(n - 1):
+        w.step(1.0 / 60.0, 8, 3)
+    assert np.all(w.bodies()["awake"][1:] == 1)
+    w.step(1.0 / 60.0, 8, 3)
+    assert np.all(w.bodies()["awake"][1:] == 0)
+    before = w.bodies()
+    for _ in range(60):
+        w.step(1.0 / 60.0, 8, 3)
+    after = w.bodies()
+    for f in ("c", "a", "v", "w", "p", "q"):
+        np.testing.assert_array_equal(before[f], after[f])
+
+
+def test_a_body_that_may_not_sleep_keeps_its_island_awake(oracle):
+    """The island sleeps only when every body reached the time to sleep; a body whose def
+    forbids sleeping resets its sleep time every step, so its whole stack stays awake."""
+    w = _resting_boxes(oracle, n_boxes=2, allow=[1, 0])
+    for _ in range(3 * _steps_to_time_to_sleep()):
+        w.step(1.0 / 60.0, 8, 3)
+    b = w.bodies()
+    assert np.all(b["awake"][1:] == 1)
+    assert b["sleep_time"][2] == 0 and b["sleep_time"][1] > 0.5
