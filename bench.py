@@ -97,6 +97,11 @@ def timed_steps(w, steps, kernel, dt=1.0 / 60.0):
     return t1 - t0, totals
 
 
+def awake_fraction(w, body_defs):
+    dyn = body_defs["type"] == 2
+    return float((w.body_states()["awake"][dyn] != 0).mean())
+
+
 def base_record(args, metric, value, unit, elapsed, steps, workload, extra):
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": 1000.0 * elapsed / max(steps, 1), "higher_is_better": True, "scaling": "weak",
@@ -107,7 +112,7 @@ def base_record(args, metric, value, unit, elapsed, steps, workload, extra):
 def run_large_world(args, b2g, scenes):
     scene = scenes.large_world(n=args.bodies, seed=1)
     w = b2g.World()
-    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
+    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
     w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     for _ in range(args.warmup):
@@ -118,8 +123,11 @@ def run_large_world(args, b2g, scenes):
     n = int(scene.meta["dynamic"])
     out = base_record(args, "body-steps/s (64k-body world)", n * steps / elapsed, "body-steps/s", elapsed, steps,
                       "large_world: %d dynamic bodies, 400x400 m box, dt 1/60, 8/3 iterations, warm start on, "
-                      "sleep and continuous off" % n,
+                      "%s, continuous off" % (n, "sleep on" if args.sleep else "sleep off"),
                       {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    if args.sleep:
+        out["config"]["sleep"] = True
+        out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
     out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux,
                                traffic_key=args.bodies == 65536)
     if not args.no_cpu:
@@ -132,6 +140,7 @@ def run_batched(args, b2g, scenes):
     the same worlds stepped one at a time as separate worlds (a sample of them)."""
     batch = scenes.batched_worlds(4096)
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
     w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
     w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
     for _ in range(args.warmup):
@@ -145,12 +154,16 @@ def run_batched(args, b2g, scenes):
                       % batch.meta["dynamic"], {"worlds": nw, "bodies": int(batch.meta["dynamic"]),
                                                 "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
     out["body_steps_per_s"] = batch.meta["dynamic"] * steps / elapsed
+    if args.sleep:
+        out["config"]["sleep"] = True
+        out["config"]["awake_after"] = awake_fraction(w, batch.body_defs)
     # plain per-world stepping: a sample of the same worlds, each its own world, stepped in turn
     sample = list(range(0, nw, nw // 32))
     per_world = []
     for wi in sample:
         sc = scenes.world_of(batch, wi)
         one = b2g.World()
+        one.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
         one.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
         for _ in range(args.warmup):
             one.step(1.0 / 60.0, 8, 3)
@@ -181,6 +194,9 @@ def run_stress(args, b2g, scenes):
     out = base_record(args, "body-steps/s (1.18M-body stress world)", n * steps / elapsed, "body-steps/s", elapsed,
                       steps, "stress_world: 1,048,576 mixed bodies + 16,384 towers x 8 boxes, 1600x1600 m box",
                       {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    if args.sleep:
+        out["config"]["sleep"] = True
+        out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
     out["pairs_per_s"] = k_aux / (k_ms / 1000.0) if k_ms else None
     out["manifolds_per_s"] = c_items / (c_ms / 1000.0) if c_ms else None
     out["solver_body_iterations_per_s"] = n * (8 + 3) / (p["solve"] / 1000.0) if p["solve"] else None
@@ -228,6 +244,8 @@ def main():
                                                             "adjoint 120)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--bodies", type=int, default=65536)
+    ap.add_argument("--sleep", action="store_true",
+                    help="allow_sleep on (large_world, batched): islands at rest sleep and are skipped")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint"))
     ap.add_argument("--roofline-kernel", default="traverse", choices=sorted(UNIT_BYTES))
