@@ -211,25 +211,17 @@ __global__ void __launch_bounds__(kBlock) k_solver_counters_init(int* c, int* __
   for (int i = t; i < kMaxColors; i += blockDim.x) cursor[i] = 0;
 }
 
-// Find with path halving for the union kernel, where other threads link roots with atomicCAS
-// at the same time. Every write to parent[] is a CAS (the halving replaces a parent by its own
-// parent only if it is still that parent), so no plain store can be written back over a
-// concurrent link. The first look-up reads plainly (a stale parent is still an ancestor, and a
-// root found from a stale read fails the linking CAS); after a failed CAS the reads are
-// coherent (agent scope), so the retry sees the other thread's link.
-template <bool COHERENT>
+// Find for the union kernel, where other threads link roots with atomicCAS at the same time.
+// Reads are plain (cached, and on a multi-XCD part possibly stale within the launch): a parent
+// pointer only ever moves to a node of higher rank, so a stale parent is still an ancestor and
+// the walk only climbs. A root found from stale reads fails the linking CAS, which returns the
+// node's true parent; the walk continues from there, so each failed CAS climbs at least one true
+// level and no coherent (device-scope) load is needed. parent[] is written only by that CAS.
 __device__ inline int uf_read(int* parent, int x) {
-  return COHERENT ? __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : parent[x];
+  return __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // a plain cached load
 }
-template <bool COHERENT>
-__device__ inline int uf_find_union(int* parent, int x) {
-  int p = uf_read<COHERENT>(parent, x);
-  while (p != x) {
-    const int g = uf_read<COHERENT>(parent, p);
-    if (g != p) atomicCAS(&parent[x], p, g);
-    x = p;
-    p = g;
-  }
+__device__ inline int uf_find(int* parent, int x) {
+  for (int p = uf_read(parent, x); p != x; p = uf_read(parent, x)) x = p;
   return x;
 }
 // Linking order: the root with the lower (hash, index) goes under the other. A bijective hash
@@ -254,13 +246,17 @@ __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, c
   if (j >= *ncp) return;
   int2 b = bodies[j];
   if (btype[b.x] == 0 || btype[b.y] == 0) return;
-  int a = uf_find_union<false>(parent, b.x), c = uf_find_union<false>(parent, b.y);
+  int a = uf_find(parent, b.x), c = uf_find(parent, b.y);
   for (;;) {
     if (a == c) return;
-    int lo = uf_below(a, c) ? a : c, hi = lo == a ? c : a;
-    if (atomicCAS(&parent[lo], lo, hi) == lo) return;
-    a = uf_find_union<true>(parent, lo);
-    c = uf_find_union<true>(parent, hi);
+    const bool a_low = uf_below(a, c);
+    const int lo = a_low ? a : c, hi = a_low ? c : a;
+    const int old = atomicCAS(&parent[lo], lo, hi);
+    if (old == lo) return;
+    // lo was linked meanwhile: climb from its true parent
+    const int up = uf_find(parent, old);
+    if (a_low) a = up;
+    else c = up;
   }
 }
 
@@ -634,29 +630,43 @@ struct VelArgs {
 // colour order (ascending colour-sorted index), the order in which a sweep colour by colour
 // applies them to it, with the same operations, so the velocities are the same bit for bit (two
 // constraints of one colour never share a dynamic body). One launch instead of one per colour.
-__device__ inline void warm_apply(int b, int i, const VelArgs& a, V2& v, float& w) {
-  const int2 bd = a.bodies[i];
-  const int4 info = a.info[i];
-  const float4 m = a.vm[i];
-  const float4 nf = a.vnf[i];
-  const float4 lam = a.lam[i];
-  const V2 normal = v2(nf.x, nf.y);
+// one constraint's warm-start data as seen from body b (its side, point count, masses, normal,
+// impulses and anchors)
+struct WarmRec {
+  float4 m, nf, lam, r0, r1;
+  int side_a, count;
+};
+__device__ inline WarmRec warm_load(int b, int i, const VelArgs& a) {
+  WarmRec r;
+  r.side_a = a.bodies[i].x == b;
+  r.count = a.info[i].z;
+  r.m = a.vm[i];
+  r.nf = a.vnf[i];
+  r.lam = a.lam[i];
+  r.r0 = a.vr0[i];
+  r.r1 = a.vr1[i];
+  return r;
+}
+__device__ inline void warm_apply(const WarmRec& c, V2& v, float& w) {
+  const V2 normal = v2(c.nf.x, c.nf.y);
   const V2 tangent = cross(normal, 1.0f);
-  const bool side_a = bd.x == b;
-  for (int p = 0; p < info.z; ++p) {
-    const float4 r = p == 0 ? a.vr0[i] : a.vr1[i];
-    const float ni = p == 0 ? lam.x : lam.z, ti = p == 0 ? lam.y : lam.w;
+  for (int p = 0; p < c.count; ++p) {
+    const float4 r = p == 0 ? c.r0 : c.r1;
+    const float ni = p == 0 ? c.lam.x : c.lam.z, ti = p == 0 ? c.lam.y : c.lam.w;
     const V2 P = ni * normal + ti * tangent;
-    if (side_a) {
-      w -= m.y * cross(v2(r.x, r.y), P);
-      v = v - m.x * P;
+    if (c.side_a) {
+      w -= c.m.y * cross(v2(r.x, r.y), P);
+      v = v - c.m.x * P;
     } else {
-      w += m.w * cross(v2(r.z, r.w), P);
-      v = v + m.z * P;
+      w += c.m.w * cross(v2(r.z, r.w), P);
+      v = v + c.m.z * P;
     }
   }
 }
-static constexpr int kWarmSort = 8;
+// Up to kWarmSort constraints per body are sorted in registers (odd-even transposition network)
+// and their records loaded kWarmBatch at a time with independent loads before they are applied
+// in order; a body with more constraints takes the next larger index each time.
+static constexpr int kWarmSort = 16, kWarmBatch = 8;
 __global__ void __launch_bounds__(kBlock) k_warm_start_bodies(int nb, const int* __restrict__ btype,
                                                               const int* __restrict__ off, const int* __restrict__ adj,
                                                               const int* __restrict__ inv_order, VelArgs a,
@@ -669,7 +679,6 @@ __global__ void __launch_bounds__(kBlock) k_warm_start_bodies(int nb, const int*
   V2 v = v2(vb.x, vb.y);
   float w = vb.z;
   if (n <= kWarmSort) {
-    // up to kWarmSort constraints: an odd-even transposition network sorts them in registers
     int k[kWarmSort];
 #pragma unroll
     for (int t = 0; t < kWarmSort; ++t) k[t] = t < n ? inv_order[adj[s0 + t]] : INT_MAX;
@@ -682,10 +691,16 @@ __global__ void __launch_bounds__(kBlock) k_warm_start_bodies(int nb, const int*
         k[t + 1] = hi;
       }
 #pragma unroll
-    for (int t = 0; t < kWarmSort; ++t)
-      if (k[t] != INT_MAX) warm_apply(b, k[t], a, v, w);
+    for (int h = 0; h < kWarmSort; h += kWarmBatch) {
+      if (h >= n) break;
+      WarmRec rec[kWarmBatch];
+#pragma unroll
+      for (int t = 0; t < kWarmBatch; ++t) rec[t] = warm_load(b, k[h + t] != INT_MAX ? k[h + t] : k[0], a);
+#pragma unroll
+      for (int t = 0; t < kWarmBatch; ++t)
+        if (k[h + t] != INT_MAX) warm_apply(rec[t], v, w);
+    }
   } else {
-    // more: the next larger index each time
     int last = -1;
     for (int c = 0; c < n; ++c) {
       int next = INT_MAX;
@@ -693,7 +708,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_start_bodies(int nb, const int*
         const int i = inv_order[adj[e]];
         if (i > last && i < next) next = i;
       }
-      warm_apply(b, next, a, v, w);
+      warm_apply(warm_load(b, next, a), v, w);
       last = next;
     }
   }
