@@ -849,6 +849,20 @@ __global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, i
   if (i < end) solve_velocity_one(i, block_solve, a);
 }
 
+// The small last colours of first-fit colouring (each at most kTailMax constraints) are solved by
+// one workgroup in one launch: colour after colour, with a workgroup barrier between them, which
+// is the same sequence as one launch per colour (the barrier makes a colour's velocity writes
+// visible to the workgroup's threads before the next colour reads them).
+static constexpr int kTailBlock = 1024, kTailMax = 512;
+__global__ void __launch_bounds__(kTailBlock) k_solve_velocity_tail(int c0, int c1, const int* __restrict__ off,
+                                                                    int block_solve, VelArgs a) {
+  for (int c = c0; c < c1; ++c) {
+    const int b0 = off[c], b1 = off[c + 1];
+    for (int i = b0 + threadIdx.x; i < b1; i += kTailBlock) solve_velocity_one(i, block_solve, a);
+    __syncthreads();
+  }
+}
+
 // store_impulses (src/private/dynamics/b2_contact_solver_private.rs:585-598)
 __device__ inline void store_one(int i, const int* __restrict__ kc, const int4* __restrict__ info,
                                  const float4* __restrict__ lam, float4* __restrict__ pt0, float4* __restrict__ pt1) {
@@ -962,6 +976,14 @@ __device__ inline void solve_position_one(int i, bool live, int begin, const Pos
 __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, PosArgs a) {
   int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   solve_position_one(i, i < end, begin, a);
+}
+__global__ void __launch_bounds__(kTailBlock) k_solve_position_tail(int c0, int c1, const int* __restrict__ off,
+                                                                    PosArgs a) {
+  for (int c = c0; c < c1; ++c) {
+    const int b0 = off[c], b1 = off[c + 1];
+    for (int i = b0 + threadIdx.x; i < b1; i += kTailBlock) solve_position_one(i, true, b0, a);
+    __syncthreads();
+  }
 }
 
 // end of a position iteration: islands whose error is small stop iterating
@@ -1299,12 +1321,24 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   if (nc > 0 && w->warm_starting)
     launch_timed(w, B2G_KERNEL_WARM_START, nc, k_warm_start_bodies, dim3(blocks_for(nb, kBlock)), dim3(kBlock), nb,
                  w->b.type.p, s.adj_off.p, s.adj.p, s.inv_order.p, va, iv);
-  for (int it = 0; it < vel_iters && nc > 0; ++it)
-    for (int col = 0; col < ncolors; ++col) {
+  // colours [tail, ncolors): the small last colours, one launch per iteration (when at least two);
+  // co holds ncolors + 1 offsets only when there are constraints
+  int tail = ncolors;
+  if (nc > 0) {
+    while (tail > 0 && co[tail] - co[tail - 1] <= kTailMax) --tail;
+    if (ncolors - tail < 2) tail = ncolors;
+  }
+  const int tail_items = nc > 0 ? co[ncolors] - co[tail] : 0;
+  for (int it = 0; it < vel_iters && nc > 0; ++it) {
+    for (int col = 0; col < tail; ++col) {
       int b0 = co[col], b1 = co[col + 1];
       launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)),
                    dim3(kBlock), b0, b1, int(w->block_solve), va);
     }
+    if (tail < ncolors)
+      launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, tail_items, k_solve_velocity_tail, dim3(1), dim3(kTailBlock), tail,
+                   ncolors, s.color_off.p, int(w->block_solve), va);
+  }
   if (nc > 0)
     hipLaunchKernelGGL(k_store_impulses, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_contact.p,
                        s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p);
@@ -1312,11 +1346,14 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                      w->b.pos.p, w->b.vel.p, dt, iv);
   w->pos_iters_run = 0;
   for (int it = 0; it < pos_iters; ++it) {
-    for (int col = 0; col < ncolors; ++col) {
+    for (int col = 0; col < tail; ++col) {
       int b0 = co[col], b1 = co[col + 1];
       launch_timed(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0, k_solve_position, dim3(blocks_for(b1 - b0, kBlock)),
                    dim3(kBlock), b0, b1, pa);
     }
+    if (tail < ncolors)
+      launch_timed(w, B2G_KERNEL_SOLVE_POSITION, tail_items, k_solve_position_tail, dim3(1), dim3(kTailBlock), tail,
+                   ncolors, s.color_off.p, pa);
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
                        w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake);
     ++w->pos_iters_run;
