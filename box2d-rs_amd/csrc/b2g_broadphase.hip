@@ -333,7 +333,9 @@ struct PairFilter {
   }
 };
 
-static constexpr int kTravBlock = 64;
+static constexpr int kTravBlock = 64;                  // one wave: 16 queries of 4 lanes
+static constexpr int kTravLanes = 4;                   // lanes per query, one per slot of a 4-wide node
+static constexpr int kTravQueries = kTravBlock / kTravLanes;
 static constexpr int kStack = 64;
 static constexpr int kSlots = 64;
 
@@ -364,26 +366,32 @@ struct TreeView {
   int batched;
 };
 
-// One tree walk per tree proxy, thread k = sorted leaf k (neighbouring threads walk
-// neighbouring paths). The emitter of a pair of tree proxies is the one with the lower sorted
-// position, so a walk skips every child whose leaves all sit at or before its own position:
-// the last leaf under each slot is in the node record. A visited node is the 4-wide record of a
-// binary node (its grandchildren's boxes), so the walk descends two levels per 128-byte load, and
-// a leaf is tested from its parent's record without a load of its own. Then the world's big static proxies (outside the tree) are tested
-// directly; every tree proxy emits its pairs with them. First pass: the first kSlots pairs go
-// to the query's slots and the count is recorded. The overflow pass re-walks only queries with
-// more than kSlots pairs and writes their whole segment; the walk order is deterministic, so
-// both passes list the pairs in the same order.
+// One tree walk per tree proxy, walked by 4 lanes: query k = sorted leaf k (neighbouring groups
+// walk neighbouring paths), lane s of the group tests slot s of each visited node. The emitter of
+// a pair of tree proxies is the one with the lower sorted position, so a walk skips every child
+// whose leaves all sit at or before its own position: the last leaf under each slot is in the
+// node record. A visited node is the 4-wide record of a binary node (its grandchildren's boxes),
+// so the walk descends two levels per 128-byte record, read as four 32-byte slot loads by the
+// group, and a leaf is tested from its parent's record without a load of its own. The group's
+// stack lives in LDS; its depth, the pair count and the order of emitted pairs (slot order within
+// a node) are the same in the four lanes, kept in step by ballots over the wave. Then the world's
+// big static proxies (outside the tree) are tested directly, four at a time; every tree proxy
+// emits its pairs with them. First pass: the first kSlots pairs go to the query's slots and the
+// count is recorded. The overflow pass re-walks only queries with more than kSlots pairs and
+// writes their whole segment; the walk order is deterministic, so both passes list the pairs in
+// the same order.
 template <bool OVERFLOW_PASS>
 __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const float4* __restrict__ fat, TreeView tv,
                                                          PairFilter pf, int* __restrict__ count,
                                                          int* __restrict__ slots, const int* __restrict__ off,
                                                          int2* __restrict__ pairs, int* __restrict__ flags) {
-  __shared__ int stack[kStack][kTravBlock];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int stack[kStack][kTravQueries];
+  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / kTravLanes;
+  const int g = threadIdx.x / kTravLanes;  // the query's stack column (and lane group of the wave)
+  const int s = threadIdx.x % kTravLanes;  // the node slot this lane tests
   if (k >= nq) return;
   // proxies outside the tree emit nothing: their counts are cleared here
-  if (!OVERFLOW_PASS)
+  if (!OVERFLOW_PASS && s == 0)
     for (int e = k; e < tv.n_loose; e += nq) count[tv.loose[e]] = 0;
   const int4 qr = tv.leafrec[k];
   const int q = qr.x;
@@ -393,49 +401,67 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const float4* _
   qb.lo = v2(qa.x, qa.y);
   qb.hi = v2(qa.z, qa.w);
   const int qw = pf.world[q];
-  int c = 0;
+  const unsigned below = (1u << s) - 1u;  // this lane's lower slots in the group mask
+  int c = 0;                              // pairs emitted so far (same in the group's lanes)
   const int out = OVERFLOW_PASS ? off[q] : 0;
-  auto emit = [&](int p) {
-    if (OVERFLOW_PASS) pairs[out + c] = make_int2(q, p);
-    else if (c < kSlots) slots[size_t(c) * nq + k] = p;
-    ++c;
+  // emit the group's hits in slot order
+  auto emit = [&](bool hit, int p) {
+    const unsigned gm = unsigned(__ballot(hit) >> (g * kTravLanes)) & 0xfu;
+    if (hit) {
+      const int r = c + __popc(gm & below);
+      if (OVERFLOW_PASS) pairs[out + r] = make_int2(q, p);
+      else if (r < kSlots) slots[size_t(r) * nq + k] = p;
+    }
+    c += __popc(gm);
   };
   if (tv.n > 1) {
-    int sp = 0;
-    stack[sp++][threadIdx.x] = 0;
+    if (s == 0) stack[0][g] = 0;
+    int sp = 1;
     while (sp > 0) {
-      const int node = stack[--sp][threadIdx.x];
-      const Bvh4Node nd = tv.nodes[node];
-      // last slot first: internal slots are pushed so that the leftmost is walked first
-#pragma unroll
-      for (int s = 3; s >= 0; --s) {
-        if (nd.last[s] <= k) continue;
-        if (tv.batched && (qw < nd.wlo[s] || qw > nd.whi[s])) continue;
-        const float4 cb = nd.box[s];
-        Aabb b;
-        b.lo = v2(cb.x, cb.y);
-        b.hi = v2(cb.z, cb.w);
-        if (!overlap(b, qb)) continue;
-        const int ch = nd.link[s];
-        if (ch < 0) {
-          const int4 pr = tv.leafrec[~ch];
-          if (leaf_pair_ok(qr, pr)) emit(pr.x);
-        } else {
-          if (sp == kStack) { atomicExch(&flags[0], 1); sp = 0; break; }
-          stack[sp++][threadIdx.x] = ch;
-        }
+      const Bvh4Node* nd = tv.nodes + stack[--sp][g];
+      const float4 cb = nd->box[s];
+      const int ch = nd->link[s];
+      bool hit = nd->last[s] > k;
+      if (tv.batched) hit = hit && qw >= nd->wlo[s] && qw <= nd->whi[s];
+      Aabb b;
+      b.lo = v2(cb.x, cb.y);
+      b.hi = v2(cb.z, cb.w);
+      hit = hit && overlap(b, qb);
+      bool ok = false;
+      int p = 0;
+      if (hit && ch < 0) {
+        const int4 pr = tv.leafrec[~ch];
+        ok = leaf_pair_ok(qr, pr);
+        p = pr.x;
       }
+      emit(ok, p);
+      // internal children: pushed so that the lowest slot is walked first
+      const bool inner = hit && ch >= 0;
+      const unsigned im = unsigned(__ballot(inner) >> (g * kTravLanes)) & 0xfu;
+      const int n_in = __popc(im);
+      if (sp + n_in > kStack) {
+        if (s == 0) atomicExch(&flags[0], 1);
+        break;
+      }
+      if (inner) stack[sp + n_in - 1 - __popc(im & below)][g] = ch;
+      sp += n_in;
     }
   }
-  for (int e = tv.loose_off[qw], end = tv.loose_off[qw + 1]; e < end; ++e) {
-    const int p = tv.loose[e];
-    float4 nb = fat[p];
-    Aabb b;
-    b.lo = v2(nb.x, nb.y);
-    b.hi = v2(nb.z, nb.w);
-    if (overlap(b, qb) && pf.ok(q, p)) emit(p);
+  for (int e0 = tv.loose_off[qw], end = tv.loose_off[qw + 1]; e0 < end; e0 += kTravLanes) {
+    const int e = e0 + s;
+    bool ok = false;
+    int p = 0;
+    if (e < end) {
+      p = tv.loose[e];
+      const float4 nb = fat[p];
+      Aabb b;
+      b.lo = v2(nb.x, nb.y);
+      b.hi = v2(nb.z, nb.w);
+      ok = overlap(b, qb) && pf.ok(q, p);
+    }
+    emit(ok, p);
   }
-  if (!OVERFLOW_PASS) {
+  if (!OVERFLOW_PASS && s == 0) {
     count[q] = c;
     if (c > kSlots) atomicAdd(&flags[1], 1);
   }
@@ -533,7 +559,8 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   // (the walk clears their counts; with no tree proxies there is no walk)
   if (nt == 0 && np > 0) B2G_HIP(hipMemsetAsync(bp.count.p, 0, sizeof(int) * np, w->stream));
   if (nt > 0) {
-    launch_timed(w, B2G_KERNEL_TRAVERSE, nt, k_traverse<false>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock),
+    launch_timed(w, B2G_KERNEL_TRAVERSE, nt, k_traverse<false>, dim3(blocks_for(nt * kTravLanes, kTravBlock)),
+                 dim3(kTravBlock),
                  nt, fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr, flags);
   }
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
@@ -553,7 +580,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     hipLaunchKernelGGL(k_pairs_from_slots, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, w->stream, nt,
                        bp.leafrec.p, bp.count.p, bp.slots.p, dst.off.p, dst.pair.p);
   if (h[2] > 0)
-    hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(nt, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt, fat,
+    hipLaunchKernelGGL(k_traverse<true>, dim3(blocks_for(nt * kTravLanes, kTravBlock)), dim3(kTravBlock), 0, w->stream, nt, fat,
                        tv, pf, bp.count.p, bp.slots.p, dst.off.p, dst.pair.p, flags);
   B2G_HIP(hipGetLastError());
   *npairs = total;

@@ -27,34 +27,43 @@ static constexpr int kMaxColors = 128;
 // islands are set awake (the island DFS of src/private/dynamics/b2_world.rs:437-447, which does
 // not reset the sleep time).
 
-__global__ void __launch_bounds__(kBlock) k_integrate_velocities(int nb, const int* __restrict__ btype,
-                                                                 const float4* __restrict__ pos, float4* __restrict__ vel,
-                                                                 float4* __restrict__ pos0, const float4* __restrict__ props,
-                                                                 const float4* __restrict__ bmass,
-                                                                 const float4* __restrict__ force, float gx, float gy,
-                                                                 float h, IslandView iv, int* __restrict__ awake) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  int t = btype[b];
+struct IntegrateArgs {
+  const int* btype;
+  const float4* pos;
+  float4* vel;
+  float4* pos0;
+  const float4* props;
+  const float4* bmass;
+  const float4* force;
+  float gx, gy, h;
+};
+__device__ inline void integrate_velocity_one(int b, const IntegrateArgs& a, const IslandView& iv, int* awake) {
+  int t = a.btype[b];
   if (t == 0 || iv.asleep(b)) return;
   if (awake) awake[b] = 1;
-  pos0[b] = pos[b];
+  a.pos0[b] = a.pos[b];
   if (t != 2) return;
-  float4 v = vel[b];
-  float4 pr = props[b];  // mass, gravity scale, linear damping, angular damping
-  float4 ms = bmass[b];
-  float4 f = force[b];
+  float4 v = a.vel[b];
+  float4 pr = a.props[b];  // mass, gravity scale, linear damping, angular damping
+  float4 ms = a.bmass[b];
+  float4 f = a.force[b];
+  const float h = a.h;
   float s = h * ms.x;
   float gm = pr.y * pr.x;
-  v.x += s * (gm * gx + f.x);
-  v.y += s * (gm * gy + f.y);
+  v.x += s * (gm * a.gx + f.x);
+  v.y += s * (gm * a.gy + f.y);
   v.z += h * ms.y * f.z;
   float dl = 1.0f / (1.0f + h * pr.z);
   float da = 1.0f / (1.0f + h * pr.w);
   v.x *= dl;
   v.y *= dl;
   v.z *= da;
-  vel[b] = v;
+  a.vel[b] = v;
+}
+__global__ void __launch_bounds__(kBlock) k_integrate_velocities(int nb, IntegrateArgs a, IslandView iv,
+                                                                 int* __restrict__ awake) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) integrate_velocity_one(b, a, iv, awake);
 }
 
 // src/private/dynamics/b2_island_private.rs:221-252
@@ -142,26 +151,35 @@ __global__ void __launch_bounds__(kBlock) k_sleep_apply(int nb, const int* __res
 }
 
 // ---- touching contacts, islands, colouring -----------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_compact(int n, const int* __restrict__ flag, const int* __restrict__ scan,
-                                                    int* __restrict__ list) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  if (flag[k]) list[scan[k]] = k;
+// per-step solver counters: [0] touching (written by the scan), [1] uncoloured, [2] colour
+// overflow, [3] active islands, then the colour histogram; and the colour scatter cursors
+__device__ inline void counters_init(int t, int* c, int* __restrict__ cursor) {
+  if (t == 0) {
+    c[1] = c[0];
+    c[2] = 0;
+    c[3] = 0;
+  }
+  for (int i = t; i <= kMaxColors; i += kBlock) c[4 + i] = 0;
+  for (int i = t; i < kMaxColors; i += kBlock) cursor[i] = 0;
 }
 
-// Kernels over the touching contacts read their count from device memory (*ncp) so that the
-// host does not wait for the compaction; they are launched over the pair count, an upper bound.
-// Also looks up the solve colour of the contact's predecessor in the previous contact set
-// (old_colour null: no reuse).
-__global__ void __launch_bounds__(kBlock) k_constraint_bodies(const int* __restrict__ ncp, const int* __restrict__ list,
-                                                              const int2* __restrict__ ab, const int* __restrict__ fbody,
-                                                              const int* __restrict__ btype, int2* __restrict__ bodies,
-                                                              int* __restrict__ deg, const int* __restrict__ old_index,
-                                                              const int* __restrict__ old_colour,
-                                                              int* __restrict__ prev_colour) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= *ncp) return;
-  const int k = list[j];
+// One thread per pair of the contact set: a touching pair becomes constraint j = its rank among
+// the touching pairs (the exclusive scan of the touching flags), with its bodies, its share of
+// the bodies' degrees and the solve colour of its predecessor in the previous contact set
+// (old_colour null: no reuse). Block 0 also initialises the per-step solver counters.
+__global__ void __launch_bounds__(kBlock) k_touching_setup(int n, const int* __restrict__ flag,
+                                                           const int* __restrict__ scan, int* __restrict__ list,
+                                                           const int2* __restrict__ ab, const int* __restrict__ fbody,
+                                                           const int* __restrict__ btype, int2* __restrict__ bodies,
+                                                           int* __restrict__ deg, const int* __restrict__ old_index,
+                                                           const int* __restrict__ old_colour,
+                                                           int* __restrict__ prev_colour, int* counters,
+                                                           int* __restrict__ cursor) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0) counters_init(threadIdx.x, counters, cursor);
+  if (k >= n || !flag[k]) return;
+  const int j = scan[k];
+  list[j] = k;
   int2 f = ab[k];
   int ba = fbody[f.x], bb = fbody[f.y];
   bodies[j] = make_int2(ba, bb);
@@ -175,6 +193,8 @@ __global__ void __launch_bounds__(kBlock) k_constraint_bodies(const int* __restr
   prev_colour[j] = pc >= 0 && pc < kMaxColors ? pc : -1;
 }
 
+// Kernels over the touching contacts read their count from device memory (*ncp) so that the
+// host does not wait for the compaction; they are launched over the pair count, an upper bound.
 __global__ void __launch_bounds__(kBlock) k_fill_adjacency(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                            const int* __restrict__ btype, const int* __restrict__ off,
                                                            int* __restrict__ cursor, int* __restrict__ adj) {
@@ -185,14 +205,16 @@ __global__ void __launch_bounds__(kBlock) k_fill_adjacency(const int* __restrict
   if (btype[b.y] == 2) adj[off[b.y] + atomicAdd(&cursor[b.y], 1)] = j;
 }
 
-// per body: its own island root, no constraints yet
+// per body: its own island root, no constraints yet; when every island is awake (no sleeping
+// bodies possible) also the velocity integration, which then does not wait for the islands
 __global__ void __launch_bounds__(kBlock) k_solver_body_init(int nb, int* __restrict__ parent, int* __restrict__ deg,
-                                                             int* __restrict__ cursor) {
+                                                             int* __restrict__ cursor, int integrate, IntegrateArgs ia) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     parent[b] = b;
     deg[b] = 0;
     cursor[b] = 0;
+    if (integrate) integrate_velocity_one(b, ia, IslandView{parent, nullptr}, nullptr);
   }
   if (b == 0) deg[nb] = 0;
 }
@@ -201,14 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_solver_body_init(int nb, int* __rest
 // islands, then the colour histogram (kMaxColors counts and the last colouring round); also
 // clears the colour scatter cursors.
 __global__ void __launch_bounds__(kBlock) k_solver_counters_init(int* c, int* __restrict__ cursor) {
-  const int t = threadIdx.x;
-  if (t == 0) {
-    c[1] = c[0];
-    c[2] = 0;
-    c[3] = 0;
-  }
-  for (int i = t; i <= kMaxColors; i += blockDim.x) c[4 + i] = 0;
-  for (int i = t; i < kMaxColors; i += blockDim.x) cursor[i] = 0;
+  counters_init(threadIdx.x, c, cursor);
 }
 
 // Find for the union kernel, where other threads link roots with atomicCAS at the same time.
@@ -886,6 +901,18 @@ __global__ void __launch_bounds__(kBlock) k_store_impulses(int n, const int* __r
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) store_one(i, kc, info, lam, pt0, pt1);
 }
+// the impulse store (per constraint) and the position integration (per body) in one launch:
+// neither reads what the other writes
+__global__ void __launch_bounds__(kBlock) k_store_and_integrate(int nc, const int* __restrict__ kc,
+                                                                const int4* __restrict__ info,
+                                                                const float4* __restrict__ lam, float4* __restrict__ pt0,
+                                                                float4* __restrict__ pt1, int nb,
+                                                                const int* __restrict__ btype, float4* __restrict__ pos,
+                                                                float4* __restrict__ vel, float h, IslandView iv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nc) store_one(t, kc, info, lam, pt0, pt1);
+  if (t < nb) integrate_position_one(t, btype, pos, vel, h, iv);
+}
 
 // ---- position passes ----------------------------------------------------------------------------
 struct PosArgs {
@@ -997,6 +1024,24 @@ __device__ inline void iteration_end_one(int b, int iteration, const int* __rest
   if (!violated[b]) done[b] = iteration + 1;
   else atomicAdd(active, 1);
   violated[b] = 0;
+}
+// the last position iteration's end and the copy back in one launch (both per body; finalize
+// reads only positions, the iteration end only island flags)
+struct IterationEndArgs {
+  int iteration;  // -1: no iteration ends here
+  const int* parent;
+  int* violated;
+  int* done;
+  int* active;
+  const int* isl_awake;
+};
+__global__ void __launch_bounds__(kBlock) k_finalize_end(int nb, const int* __restrict__ btype,
+                                                         const float4* __restrict__ pos, const float4* __restrict__ bmass,
+                                                         float4* __restrict__ xf, IslandView iv, IterationEndArgs e) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (e.iteration >= 0) iteration_end_one(b, e.iteration, btype, e.parent, e.violated, e.done, e.active, e.isl_awake);
+  finalize_one(b, btype, pos, bmass, xf, iv);
 }
 __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
                                                                    const int* __restrict__ parent,
@@ -1144,8 +1189,15 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
       (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(std::max(n_pairs, 1))))
     return st;
+  const IntegrateArgs ia{w->b.type.p, w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p,
+                         w->gravity.x, w->gravity.y, dt};
+  // per body: island roots, degrees, adjacency cursors and (no sleeping possible) the velocity
+  // integration
+  hipLaunchKernelGGL(k_solver_body_init, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
+                     s.deg.p, s.adj_cursor.p, int(!sleepy), ia);
   if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
-  hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(kBlock), 0, w->stream, cnt, s.color_cursor.p);
+  if (n_pairs == 0)
+    hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(kBlock), 0, w->stream, cnt, s.color_cursor.p);
   const int gp = blocks_for(std::max(n_pairs, 1), kBlock);
   // Colour reuse (persisting contacts keep their colour) lets the colour count creep up as new
   // contacts take first-fit colours around kept ones; the step colours from scratch again once
@@ -1154,29 +1206,24 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   constexpr int kColorSlack = 2, kColorRefresh = 32;
   const bool reuse = w->color_reuse && w->scratch_colors > 0 && w->n_colors <= w->scratch_colors + kColorSlack &&
                      w->steps_since_scratch < kColorRefresh;
-  if (n_pairs > 0)
-    hipLaunchKernelGGL(k_compact, dim3(gp), dim3(kBlock), 0, w->stream, n_pairs, s.touch_flag.p, s.touch_scan.p,
-                       s.touch_list.p);
-  // constraint bodies, degrees, islands
-  hipLaunchKernelGGL(k_solver_body_init, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
-                     s.deg.p, s.adj_cursor.p);
+  // constraints (touching contacts): list, bodies, degrees, previous colours; then islands
   if (n_pairs > 0) {
     // colours of the previous step's contacts (the old set's solve wrote them)
     const int* old_colour = reuse ? w->c[1 - w->cur].colour.p : nullptr;
-    hipLaunchKernelGGL(k_constraint_bodies, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.touch_list.p, c.ab.p,
-                       w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p, c.old_index.p, old_colour,
-                       s.prev_colour.p);
+    hipLaunchKernelGGL(k_touching_setup, dim3(gp), dim3(kBlock), 0, w->stream, n_pairs, s.touch_flag.p,
+                       s.touch_scan.p, s.touch_list.p, c.ab.p, w->f.body.p, w->b.type.p, s.k_bodies.p, s.deg.p,
+                       c.old_index.p, old_colour, s.prev_colour.p, cnt, s.color_cursor.p);
     hipLaunchKernelGGL(k_union, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p, s.parent.p);
   }
   hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
                      s.isl_violated.p, s.isl_done.p, sleepy ? s.isl_awake.p : nullptr, s.isl_restless.p);
   const IslandView iv{s.parent.p, sleepy ? s.isl_awake.p : nullptr};
-  if (sleepy)
+  if (sleepy) {
     hipLaunchKernelGGL(k_island_awake, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
                        w->b.awake.p, s.parent.p, s.isl_awake.p);
-  hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
-                     w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p, w->gravity.x,
-                     w->gravity.y, dt, iv, sleepy ? w->b.awake.p : nullptr);
+    hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, ia, iv,
+                       w->b.awake.p);
+  }
   int ncolors = 0, nc = 0, n_touching = 0;
   if (n_pairs > 0) {
     if ((st = exclusive_scan(w, s.deg.p, s.adj_off.p, nb + 1, nullptr))) return st;
@@ -1339,11 +1386,9 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, tail_items, k_solve_velocity_tail, dim3(1), dim3(kTailBlock), tail,
                    ncolors, s.color_off.p, int(w->block_solve), va);
   }
-  if (nc > 0)
-    hipLaunchKernelGGL(k_store_impulses, dim3(blocks_for(nc, kBlock)), dim3(kBlock), 0, w->stream, nc, s.k_contact.p,
-                       s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p);
-  hipLaunchKernelGGL(k_integrate_positions, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
-                     w->b.pos.p, w->b.vel.p, dt, iv);
+  hipLaunchKernelGGL(k_store_and_integrate, dim3(blocks_for(std::max(nc, nb), kBlock)), dim3(kBlock), 0, w->stream, nc,
+                     s.k_contact.p, s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p, nb, w->b.type.p, w->b.pos.p, w->b.vel.p, dt,
+                     iv);
   w->pos_iters_run = 0;
   for (int it = 0; it < pos_iters; ++it) {
     for (int col = 0; col < tail; ++col) {
@@ -1354,12 +1399,15 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     if (tail < ncolors)
       launch_timed(w, B2G_KERNEL_SOLVE_POSITION, tail_items, k_solve_position_tail, dim3(1), dim3(kTailBlock), tail,
                    ncolors, s.color_off.p, pa);
-    hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
-                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake);
+    // (the last iteration's end runs with the copy back below)
+    if (it + 1 < pos_iters)
+      hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
+                         w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake);
     ++w->pos_iters_run;
   }
-  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p, w->b.pos.p,
-                     w->b.mass.p, w->b.xf.p, iv);
+  const IterationEndArgs last_end{pos_iters - 1, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake};
+  hipLaunchKernelGGL(k_finalize_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
+                     w->b.pos.p, w->b.mass.p, w->b.xf.p, iv, last_end);
   if (w->allow_sleep) {
     hipLaunchKernelGGL(k_sleep_time, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
                        w->b.auto_sleep.p, w->b.vel.p, w->b.sleep_time.p, iv, s.isl_restless.p, dt);
