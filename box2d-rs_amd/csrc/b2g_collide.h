@@ -71,20 +71,38 @@ struct TClipV {
   uint32_t id;
 };
 
+// b2_clip_segment_to_line (src/private/collision/b2_collision.rs): the points behind the plane
+// in input order, then the crossing point when the two lie on opposite sides. Written with
+// selects only (no run-time index into a local array, which would live in scratch memory); when
+// both points are kept the product d0 * d1 is not negative, so at most two points come out.
 template <class T>
 B2G_HD int clip_segment(TClipV<T> out[2], const TClipV<T> in[2], TV2<T> normal, T offset, int vertexIndexA) {
-  int count = 0;
-  T d0 = dot(normal, in[0].v) - offset;
-  T d1 = dot(normal, in[1].v) - offset;
-  if (d0 <= T(0.0)) out[count++] = in[0];
-  if (d1 <= T(0.0)) out[count++] = in[1];
-  if (d0 * d1 < T(0.0)) {
+  const T d0 = dot(normal, in[0].v) - offset;
+  const T d1 = dot(normal, in[1].v) - offset;
+  const bool keep0 = d0 <= T(0.0), keep1 = d1 <= T(0.0), crossing = d0 * d1 < T(0.0);
+  TClipV<T> mid = in[0];
+  if (crossing) {
     T interp = d0 / (d0 - d1);
-    out[count].v = in[0].v + interp * (in[1].v - in[0].v);
-    out[count].id = feature(uint32_t(vertexIndexA), (in[0].id >> 8) & 0xffu, kVertex, kFace);
-    ++count;
+    mid.v = in[0].v + interp * (in[1].v - in[0].v);
+    mid.id = feature(uint32_t(vertexIndexA), (in[0].id >> 8) & 0xffu, kVertex, kFace);
   }
-  return count;
+  const int kept = int(keep0) + int(keep1);
+  const TClipV<T> first = keep0 ? in[0] : in[1];
+  out[0] = kept > 0 ? first : mid;
+  out[1] = kept > 1 ? in[1] : mid;
+  return kept + int(crossing);
+}
+
+// manifold point pc (0 or 1) without a run-time index into the manifold
+template <class T>
+B2G_HD void set_point(TManifold<T>* m, int pc, TV2<T> p, uint32_t id) {
+  if (pc == 0) {
+    m->lp[0] = p;
+    m->id[0] = id;
+  } else {
+    m->lp[1] = p;
+    m->id[1] = id;
+  }
 }
 
 template <class T>
@@ -224,8 +242,7 @@ B2G_HD void collide_polygons(TManifold<T>* m, const ShapeV& pA, TXf<T> xfA, cons
   for (int i = 0; i < 2; ++i) {
     T separation = dot(normal, cp2[i].v) - frontOffset;
     if (separation <= totalRadius) {
-      m->lp[pc] = mulT(xf2, cp2[i].v);
-      m->id[pc] = flip ? feature_swap(cp2[i].id) : cp2[i].id;
+      set_point(m, pc, mulT(xf2, cp2[i].v), flip ? feature_swap(cp2[i].id) : cp2[i].id);
       ++pc;
     }
   }
@@ -287,12 +304,11 @@ B2G_HD void collide_edge_polygon(TManifold<T>* m, const ShapeV& edge, TXf<T> xfA
   T offset1 = dot(normal1, centroidB - v1);
   bool oneSided = edge.one_sided != 0;
   if (oneSided && offset1 < T(0.0)) return;
-  TV2<T> tvx[8], tn[8];
-  int tc = poly.count;
-  for (int i = 0; i < tc; ++i) {
-    tvx[i] = mul(xf, tv<T>(poly.v[i]));
-    tn[i] = mul(xf.q, tv<T>(poly.n[i]));
-  }
+  // the polygon in the edge's frame, recomputed where used (a local array indexed at run time
+  // would live in scratch memory)
+  const int tc = poly.count;
+  auto tvx = [&](int i) { return mul(xf, tv<T>(poly.v[i])); };
+  auto tn = [&](int i) { return mul(xf.q, tv<T>(poly.n[i])); };
   T radius = T(poly.radius) + T(edge.radius);
   // edge separation (src/private/collision/b2_collide_edge.rs:172-201)
   int eIndex = -1;
@@ -303,7 +319,7 @@ B2G_HD void collide_edge_polygon(TManifold<T>* m, const ShapeV& edge, TXf<T> xfA
     for (int j = 0; j < 2; ++j) {
       T sj = T(kMaxFloat);
       for (int i = 0; i < tc; ++i) {
-        T si = dot(axes[j], tvx[i] - v1);
+        T si = dot(axes[j], tvx(i) - v1);
         if (si < sj) sj = si;
       }
       if (sj > eSep) { eIndex = j; eSep = sj; eNormal = axes[j]; }
@@ -315,9 +331,9 @@ B2G_HD void collide_edge_polygon(TManifold<T>* m, const ShapeV& edge, TXf<T> xfA
   T pSep = T(-kMaxFloat);
   TV2<T> pNormal = v2(T(0.0), T(0.0));
   for (int i = 0; i < tc; ++i) {
-    TV2<T> n = -tn[i];
-    T s1 = dot(n, tvx[i] - v1);
-    T s2 = dot(n, tvx[i] - v2_);
+    TV2<T> n = -tn(i);
+    T s1 = dot(n, tvx(i) - v1);
+    T s2 = dot(n, tvx(i) - v2_);
     T s = bmin(s1, s2);
     if (s > pSep) { pType = 2; pIndex = i; pSep = s; pNormal = n; }
   }
@@ -362,15 +378,15 @@ B2G_HD void collide_edge_polygon(TManifold<T>* m, const ShapeV& edge, TXf<T> xfA
   if (prType == 1) {
     m->type = kFaceA;
     int bestIndex = 0;
-    T bestValue = dot(prNormal, tn[0]);
+    T bestValue = dot(prNormal, tn(0));
     for (int i = 1; i < tc; ++i) {
-      T value = dot(prNormal, tn[i]);
+      T value = dot(prNormal, tn(i));
       if (value < bestValue) { bestValue = value; bestIndex = i; }
     }
     int i1 = bestIndex, i2 = i1 + 1 < tc ? i1 + 1 : 0;
-    clip[0].v = tvx[i1];
+    clip[0].v = tvx(i1);
     clip[0].id = feature(0, uint32_t(i1), kFace, kVertex);
-    clip[1].v = tvx[i2];
+    clip[1].v = tvx(i2);
     clip[1].id = feature(0, uint32_t(i2), kFace, kVertex);
     ri1 = 0; ri2 = 1; rv1 = v1; rv2 = v2_; rnormal = prNormal;
     sn1 = -edge1; sn2 = edge1;
@@ -382,8 +398,8 @@ B2G_HD void collide_edge_polygon(TManifold<T>* m, const ShapeV& edge, TXf<T> xfA
     clip[1].id = feature(0, uint32_t(prIndex), kVertex, kFace);
     ri1 = prIndex;
     ri2 = ri1 + 1 < tc ? ri1 + 1 : 0;
-    rv1 = tvx[ri1]; rv2 = tvx[ri2];
-    rnormal = tn[ri1];
+    rv1 = tvx(ri1); rv2 = tvx(ri2);
+    rnormal = tn(ri1);
     sn1 = v2(rnormal.y, -rnormal.x);
     sn2 = -sn1;
   }
@@ -405,13 +421,8 @@ B2G_HD void collide_edge_polygon(TManifold<T>* m, const ShapeV& edge, TXf<T> xfA
   for (int i = 0; i < 2; ++i) {
     T separation = dot(rnormal, c2[i].v - rv1);
     if (separation <= radius) {
-      if (prType == 1) {
-        m->lp[pc] = mulT(xf, c2[i].v);
-        m->id[pc] = c2[i].id;
-      } else {
-        m->lp[pc] = c2[i].v;
-        m->id[pc] = feature_swap(c2[i].id);
-      }
+      if (prType == 1) set_point(m, pc, mulT(xf, c2[i].v), c2[i].id);
+      else set_point(m, pc, c2[i].v, feature_swap(c2[i].id));
       ++pc;
     }
   }

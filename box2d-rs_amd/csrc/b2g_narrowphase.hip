@@ -38,7 +38,15 @@ struct SleepView {
   const float4* ogeo;   // previous manifold geometry (copied for contacts that are not updated)
 };
 
-__global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restrict__ pairs,
+// The shape-pair kinds of a step's pairs are interleaved (the pair list is in tree order), and a
+// wave holding several kinds runs each kind's code in turn. So a workgroup first sorts its pairs
+// by kind in LDS (a counting sort over the 16 shape-type pairs) and each thread then evaluates
+// the pair at its sorted position: most waves hold one kind. Outputs still go to the pair's own
+// index, so the thread that evaluates a pair does not change any result.
+static constexpr int kCollideBlock = 256;
+static constexpr int kKinds = 17;  // 4 x 4 shape-type pairs (lower type first), and past-the-end
+
+__global__ void __launch_bounds__(kCollideBlock) k_collide(int n, const int2* __restrict__ pairs,
                                                     const int* __restrict__ old_index, const int* __restrict__ fbody,
                                                     const int4* __restrict__ fshape, const ShapeV* __restrict__ shapes,
                                                     const float4* __restrict__ bxf,
@@ -51,7 +59,25 @@ __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restric
                                                     float4* __restrict__ geo, float4* __restrict__ pt0,
                                                     float4* __restrict__ pt1, uint2* __restrict__ ids,
                                                     int* __restrict__ touching) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int kind_count[kKinds];
+  __shared__ int sorted[kCollideBlock];
+  const int t = threadIdx.x;
+  if (t < kKinds) kind_count[t] = 0;
+  __syncthreads();
+  int k = blockIdx.x * kCollideBlock + t;
+  int kind = kKinds - 1;
+  if (k < n) {
+    const int2 p = pairs[k];
+    const int ta = fshape[p.x].x & 3, tb = fshape[p.y].x & 3;
+    kind = ta < tb ? ta * 4 + tb : tb * 4 + ta;
+  }
+  const int r = atomicAdd(&kind_count[kind], 1);
+  __syncthreads();
+  int base = r;
+  for (int j = 0; j < kind; ++j) base += kind_count[j];
+  sorted[base] = k;
+  __syncthreads();
+  k = sorted[t];
   if (k >= n) return;
   int2 pr = pairs[k];
   // the reference pair is (lower proxy id, higher proxy id); b2_contact_create then lets the
@@ -95,9 +121,11 @@ __global__ void __launch_bounds__(kBlock) k_collide(int n, const int2* __restric
   float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   const int ocount = oi >= 0 ? ohdr[oi] >> 8 : 0;
   if (m.count > 0 && ocount > 0) {
-    uint2 oid = oids[oi];
-    float4 o0 = opt0[oi], o1 = opt1[oi];
-    for (int i = 0; i < m.count; ++i) {
+    const uint2 oid = oids[oi];
+    const float4 o0 = opt0[oi], o1 = opt1[oi];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (i >= m.count) break;
       if (m.id[i] == oid.x) { imp[2 * i] = o0.z; imp[2 * i + 1] = o0.w; }
       else if (ocount > 1 && m.id[i] == oid.y) { imp[2 * i] = o1.z; imp[2 * i + 1] = o1.w; }
     }
@@ -156,7 +184,7 @@ int phase_collide(World* w) {
     hipLaunchKernelGGL(k_wake_destroyed, dim3(blocks_for(w->n_pairs_old, kBlock)), dim3(kBlock), 0, w->stream,
                        w->n_pairs_old, o.pair.p, o.hdr.p, w->f.body.p, w->f.fat.p, sv);
   if (n > 0)
-    launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kBlock)), dim3(kBlock), n, c.pair.p,
+    launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kCollideBlock)), dim3(kCollideBlock), n, c.pair.p,
                  c.old_index.p, w->f.body.p, w->f.shape.p, w->f.shapev.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p,
                  o.ids.p, sv, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
   if (sleepy && w->nb > 0)
