@@ -32,14 +32,16 @@ B2G_HD uint32_t feature_swap(uint32_t id) {
 
 // A shape in registers. Polygon: v/n hold vertices and normals, p the centroid. Circle: p is
 // the centre. Edge: v[0]=vertex1, v[1]=vertex2, v[2]=vertex0, v[3]=vertex3.
+static constexpr int kMaxPolygonVertices = 8;  // B2_MAX_POLYGON_VERTICES, src/b2_settings.rs
+
 struct ShapeV {
   int type;
   int count;
   int one_sided;
   float radius;
   V2 p;
-  V2 v[8];
-  V2 n[8];
+  V2 v[kMaxPolygonVertices];
+  V2 n[kMaxPolygonVertices];
 };
 
 template <class T>
@@ -171,12 +173,19 @@ B2G_HD T find_max_separation(int* edgeIndex, const ShapeV& p1, TXf<T> xf1, const
   TXf<T> xf = mulT(xf2, xf1);
   int best = 0;
   T maxSep = T(-kMaxFloat);
+  // p2's vertices are read once into registers (fixed indices after unrolling)
+  const int n2 = p2.count;
+  TV2<T> w2[kMaxPolygonVertices];
+#pragma unroll
+  for (int j = 0; j < kMaxPolygonVertices; ++j) w2[j] = j < n2 ? tv<T>(p2.v[j]) : v2(T(0.0), T(0.0));
   for (int i = 0; i < p1.count; ++i) {
     TV2<T> n = mul(xf.q, tv<T>(p1.n[i]));
     TV2<T> v1 = mul(xf, tv<T>(p1.v[i]));
     T si = T(kMaxFloat);
-    for (int j = 0; j < p2.count; ++j) {
-      T sij = dot(n, tv<T>(p2.v[j]) - v1);
+#pragma unroll
+    for (int j = 0; j < kMaxPolygonVertices; ++j) {
+      if (j >= n2) break;
+      T sij = dot(n, w2[j] - v1);
       if (sij < si) si = sij;
     }
     if (si > maxSep) { maxSep = si; best = i; }
