@@ -129,9 +129,17 @@ B2G_HD void collide_polygon_circle(TManifold<T>* m, const ShapeV& poly, TXf<T> x
   int ni = 0;
   T separation = T(-kMaxFloat);
   T radius = T(poly.radius) + T(circ.radius);
-  int vc = poly.count;
-  for (int i = 0; i < vc; ++i) {
-    T s = dot(tv<T>(poly.n[i]), cl - tv<T>(poly.v[i]));
+  const int vc = poly.count;
+  TV2<T> pn[kMaxPolygonVertices], pv[kMaxPolygonVertices];  // read once, fixed indices after unrolling
+#pragma unroll
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
+    pn[i] = i < vc ? tv<T>(poly.n[i]) : v2(T(0.0), T(0.0));
+    pv[i] = i < vc ? tv<T>(poly.v[i]) : v2(T(0.0), T(0.0));
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
+    if (i >= vc) break;
+    T s = dot(pn[i], cl - pv[i]);
     if (s > radius) return;
     if (s > separation) { separation = s; ni = i; }
   }
@@ -178,9 +186,18 @@ B2G_HD T find_max_separation(int* edgeIndex, const ShapeV& p1, TXf<T> xf1, const
   TV2<T> w2[kMaxPolygonVertices];
 #pragma unroll
   for (int j = 0; j < kMaxPolygonVertices; ++j) w2[j] = j < n2 ? tv<T>(p2.v[j]) : v2(T(0.0), T(0.0));
-  for (int i = 0; i < p1.count; ++i) {
-    TV2<T> n = mul(xf.q, tv<T>(p1.n[i]));
-    TV2<T> v1 = mul(xf, tv<T>(p1.v[i]));
+  const int n1 = p1.count;
+  TV2<T> pn[kMaxPolygonVertices], pv[kMaxPolygonVertices];
+#pragma unroll
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
+    pn[i] = i < n1 ? tv<T>(p1.n[i]) : v2(T(0.0), T(0.0));
+    pv[i] = i < n1 ? tv<T>(p1.v[i]) : v2(T(0.0), T(0.0));
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
+    if (i >= n1) break;
+    TV2<T> n = mul(xf.q, pn[i]);
+    TV2<T> v1 = mul(xf, pv[i]);
     T si = T(kMaxFloat);
 #pragma unroll
     for (int j = 0; j < kMaxPolygonVertices; ++j) {
