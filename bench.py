@@ -18,26 +18,49 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "box2d-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Roofline kernel: k_traverse, the LBVH pair search, the longest launch of a step (one launch,
-# under a tenth of a millisecond on this workload). The solver's per-colour kernels take more
-# time in total but each launch lasts 4-11 us, so an event pair around one of them mostly
-# measures its dispatch.
-# Algorithmic HBM bytes of one traversal launch (f32/i32 sizes, each read or written once): per
-# query its packed leaf record (proxy, body, filter bits, group/shape type/body type: 16 B), its
-# fat AABB (16 B), its world (4 B) and its pair count (4 B): 40 B; per pair found the partner's
+# Roofline kernel: k_collide, the narrowphase, the longest single launch of a step (one launch of
+# about 45 us on this workload). The solver's per-colour kernels take more time in total but each
+# launch lasts 2-10 us, so an event pair around one of them mostly measures its dispatch.
+# Algorithmic HBM bytes of one narrowphase launch (f32/i32 sizes, each read or written once): per
+# candidate pair 120 B (pair, fixture and body records, the two transforms, the previous
+# manifold's ids and impulses, the new manifold) plus both shapes: circle 12 B, polygon 12 + 16 n
+# (n vertices and normals), edge 36 B. The shape bytes per pair are averaged over the contact
+# list read back before and after the timed steps.
+# k_traverse (--roofline-kernel traverse), the LBVH pair search: per query its packed leaf record
+# (16 B), fat AABB (16 B), world (4 B) and pair count (4 B): 40 B; per pair found the partner's
 # box (16 B, held in its parent's node record) and leaf record (16 B) and the slot write (4 B):
 # 36 B. Internal-node reads of the walk are not counted: they are the search's overhead above
 # the algorithmic minimum.
 QUERY_BYTES, PAIR_BYTES = 40, 36
-UNIT_BYTES = {"traverse": QUERY_BYTES, "solve_position": 136, "solve_velocity": 224, "warm_start": 128,
-              "init_constraints": 404}
-UNIT_NAME = {"traverse": "proxy query", "solve_velocity": "constraint-iteration",
+COLLIDE_PAIR_BYTES = 120
+UNIT_BYTES = {"traverse": QUERY_BYTES, "collide": COLLIDE_PAIR_BYTES, "solve_position": 136, "solve_velocity": 224,
+              "warm_start": 128, "init_constraints": 404}
+UNIT_NAME = {"traverse": "proxy query", "collide": "candidate pair", "solve_velocity": "constraint-iteration",
              "solve_position": "constraint-iteration", "warm_start": "constraint", "init_constraints": "constraint"}
+
+
+def shape_bytes(shape_defs):
+    """Per fixture: the narrowphase's algorithmic bytes of its shape (circle 12, polygon 12 + 16 n,
+    edge 36; a box is a 4-gon, a polygon's hull keeps all of the scene's points)."""
+    t, n = shape_defs["type"], shape_defs["count"]
+    out = np.full(len(shape_defs), 36, np.int64)
+    out[t == 0] = 12
+    out[t == 2] = 12 + 16 * n[t == 2]
+    out[t == 4] = 12 + 16 * 4
+    return out
+
+
+def pair_shape_bytes(w, sbytes):
+    """Mean shape bytes per candidate pair of the world's current contact list."""
+    fab, _, _ = w.contacts()
+    return float((sbytes[fab[:, 0]] + sbytes[fab[:, 1]]).mean()) if len(fab) else 0.0
 
 
 def cpu_baseline(scene, seconds=15.0, max_steps=200):
@@ -68,18 +91,24 @@ def measured_traffic(kernel):
         return json.load(f).get("bytes_per_launch")
 
 
-def roofline(kernel, k_ms, k_launches, k_items, k_aux, traffic_key=None):
+def roofline(kernel, k_ms, k_launches, k_items, k_aux, traffic_key=None, pair_shape=None):
     bytes_per_launch = UNIT_BYTES[kernel] * k_items / max(k_launches, 1)
     if kernel == "traverse":
         bytes_per_launch += PAIR_BYTES * k_aux / max(k_launches, 1)
+    if kernel == "collide":
+        bytes_per_launch += sum(pair_shape) / len(pair_shape) * k_items / max(k_launches, 1)
     avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel) if traffic_key else None,
             "kernel": "k_" + kernel, "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_launch": bytes_per_launch, "unit_bytes": UNIT_BYTES[kernel],
-            "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[kernel],
-            "pairs_per_launch": k_aux / max(k_launches, 1)}
+            "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[kernel]}
+    if kernel == "traverse":
+        out["pairs_per_launch"] = k_aux / max(k_launches, 1)
+    if kernel == "collide":
+        out["shape_bytes_per_pair"] = list(pair_shape)
+    return out
 
 
 def timed_steps(w, steps, kernel, dt=1.0 / 60.0):
@@ -118,7 +147,11 @@ def run_large_world(args, b2g, scenes):
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     steps = args.steps if args.steps is not None else 600
+    sbytes = shape_bytes(scene.shape_defs)
+    pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
     elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    if pair_shape:
+        pair_shape.append(pair_shape_bytes(w, sbytes))
     p = w.profile()
     n = int(scene.meta["dynamic"])
     out = base_record(args, "body-steps/s (64k-body world)", n * steps / elapsed, "body-steps/s", elapsed, steps,
@@ -129,7 +162,7 @@ def run_large_world(args, b2g, scenes):
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
     out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux,
-                               traffic_key=args.bodies == 65536)
+                               traffic_key=args.bodies == 65536, pair_shape=pair_shape)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     return out
@@ -146,7 +179,11 @@ def run_batched(args, b2g, scenes):
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     steps = args.steps if args.steps is not None else 300
+    sbytes = shape_bytes(batch.shape_defs)
+    pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
     elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    if pair_shape:
+        pair_shape.append(pair_shape_bytes(w, sbytes))
     nw = batch.meta["worlds"]
     p = w.profile()
     out = base_record(args, "world-steps/s (4096 batched worlds)", nw * steps / elapsed, "world-steps/s", elapsed,
@@ -172,7 +209,7 @@ def run_batched(args, b2g, scenes):
     plain = 1.0 / (sum(per_world) / len(per_world))
     out["plain_per_world"] = {"value": plain, "unit": "world-steps/s", "sample_worlds": len(sample),
                               "ratio": out["value"] / plain}
-    out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux)
+    out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux, pair_shape=pair_shape)
     return out
 
 
@@ -248,7 +285,7 @@ def main():
                     help="allow_sleep on (large_world, batched): islands at rest sleep and are skipped")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint"))
-    ap.add_argument("--roofline-kernel", default="traverse", choices=sorted(UNIT_BYTES))
+    ap.add_argument("--roofline-kernel", default="collide", choices=sorted(UNIT_BYTES))
     ap.add_argument("--solver-graph", type=int, default=0, choices=(0, 1),
                     help="replay the solve's per-colour launches from a captured graph (default 0: direct "
                          "launches are faster on MI355X; timing a solve kernel launches the solve directly)")

@@ -95,16 +95,15 @@ B2G_HD int clip_segment(TClipV<T> out[2], const TClipV<T> in[2], TV2<T> normal, 
   return kept + int(crossing);
 }
 
-// manifold point pc (0 or 1) without a run-time index into the manifold
+// manifold point pc (0 or 1) without a run-time index into the manifold: both slots are stored
+// with selected values (a branch per slot would be merged into a store to a selected address)
 template <class T>
 B2G_HD void set_point(TManifold<T>* m, int pc, TV2<T> p, uint32_t id) {
-  if (pc == 0) {
-    m->lp[0] = p;
-    m->id[0] = id;
-  } else {
-    m->lp[1] = p;
-    m->id[1] = id;
-  }
+  const bool first = pc == 0;
+  m->lp[0] = first ? p : m->lp[0];
+  m->id[0] = first ? id : m->id[0];
+  m->lp[1] = first ? m->lp[1] : p;
+  m->id[1] = first ? m->id[1] : id;
 }
 
 template <class T>

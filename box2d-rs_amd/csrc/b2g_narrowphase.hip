@@ -14,9 +14,8 @@ namespace b2g {
 
 static constexpr int kBlock = 128;
 
-__device__ inline void store_manifold(int k, const Manifold& m, const float imp[4], int* __restrict__ hdr,
-                                      float4* __restrict__ geo, float4* __restrict__ pt0, float4* __restrict__ pt1,
-                                      uint2* __restrict__ ids) {
+__device__ inline void store_manifold(int k, const Manifold& m, const float imp[4], int* hdr, float4* geo,
+                                      float4* pt0, float4* pt1, uint2* ids) {
   hdr[k] = m.type | (m.count << 8);
   geo[k] = make_float4(m.local_normal.x, m.local_normal.y, m.local_point.x, m.local_point.y);
   pt0[k] = make_float4(m.lp[0].x, m.lp[0].y, imp[0], imp[1]);
@@ -41,10 +40,18 @@ struct SleepView {
 // The shape-pair kinds of a step's pairs are interleaved (the pair list is in tree order), and a
 // wave holding several kinds runs each kind's code in turn. So a workgroup first sorts its pairs
 // by kind in LDS (a counting sort over the 16 shape-type pairs) and each thread then evaluates
-// the pair at its sorted position: most waves hold one kind. Outputs still go to the pair's own
-// index, so the thread that evaluates a pair does not change any result.
+// the pair at its sorted position: most waves hold one kind. A thread's results go to the pair's
+// slot in an LDS copy of the workgroup's output range, which is then written out in pair order
+// (coalesced rows), so the thread that evaluates a pair does not change any result.
 static constexpr int kCollideBlock = 256;
 static constexpr int kKinds = 17;  // 4 x 4 shape-type pairs (lower type first), and past-the-end
+
+struct CollideStage {
+  float4 geo[kCollideBlock], pt0[kCollideBlock], pt1[kCollideBlock];
+  int2 ab[kCollideBlock];
+  uint2 ids[kCollideBlock];
+  int hdr[kCollideBlock], touching[kCollideBlock];
+};
 
 __global__ void __launch_bounds__(kCollideBlock) k_collide(int n, const int2* __restrict__ pairs,
                                                     const int* __restrict__ old_index, const int* __restrict__ fbody,
@@ -61,80 +68,97 @@ __global__ void __launch_bounds__(kCollideBlock) k_collide(int n, const int2* __
                                                     int* __restrict__ touching) {
   __shared__ int kind_count[kKinds];
   __shared__ int sorted[kCollideBlock];
+  __shared__ CollideStage st;
   const int t = threadIdx.x;
+  const int base = blockIdx.x * kCollideBlock;
   if (t < kKinds) kind_count[t] = 0;
   __syncthreads();
-  int k = blockIdx.x * kCollideBlock + t;
   int kind = kKinds - 1;
-  if (k < n) {
-    const int2 p = pairs[k];
+  if (base + t < n) {
+    const int2 p = pairs[base + t];
     const int ta = fshape[p.x].x & 3, tb = fshape[p.y].x & 3;
     kind = ta < tb ? ta * 4 + tb : tb * 4 + ta;
   }
   const int r = atomicAdd(&kind_count[kind], 1);
   __syncthreads();
-  int base = r;
-  for (int j = 0; j < kind; ++j) base += kind_count[j];
-  sorted[base] = k;
+  int pos = r;
+  for (int j = 0; j < kind; ++j) pos += kind_count[j];
+  sorted[pos] = t;
   __syncthreads();
-  k = sorted[t];
-  if (k >= n) return;
-  int2 pr = pairs[k];
-  // the reference pair is (lower proxy id, higher proxy id); b2_contact_create then lets the
-  // registry decide which fixture is A
-  int lo = pr.x < pr.y ? pr.x : pr.y, hi = pr.x < pr.y ? pr.y : pr.x;
-  int4 hq = fshape[lo], hp = fshape[hi];
-  bool swap = registry_swap(hq.x & 0xff, hp.x & 0xff) == 1;
-  int fa = swap ? hi : lo, fb = swap ? lo : hi;
-  const int bA = fbody[fa], bB = fbody[fb];
-  int oi = old_index[k];
-  ab[k] = make_int2(fa, fb);
-  if (sv.awake) {
-    const bool active_a = sv.btype[bA] != 0 && sv.awake[bA], active_b = sv.btype[bB] != 0 && sv.awake[bB];
-    if (!active_a && !active_b) {
+  const int s = sorted[t];  // the pair this thread evaluates: base + s
+  const int k = base + s;
+  if (k < n) {
+    const int2 pr = pairs[k];
+    // the reference pair is (lower proxy id, higher proxy id); b2_contact_create then lets the
+    // registry decide which fixture is A
+    const int lo = pr.x < pr.y ? pr.x : pr.y, hi = pr.x < pr.y ? pr.y : pr.x;
+    const int4 hq = fshape[lo], hp = fshape[hi];
+    const bool swap = registry_swap(hq.x & 0xff, hp.x & 0xff) == 1;
+    const int fa = swap ? hi : lo, fb = swap ? lo : hi;
+    const int bA = fbody[fa], bB = fbody[fb];
+    const int oi = old_index[k];
+    st.ab[s] = make_int2(fa, fb);
+    bool frozen = false;
+    if (sv.awake) {
+      const bool active_a = sv.btype[bA] != 0 && sv.awake[bA], active_b = sv.btype[bB] != 0 && sv.awake[bB];
+      frozen = !active_a && !active_b;
+    }
+    if (frozen) {
       if (oi >= 0) {
-        hdr[k] = ohdr[oi];
-        geo[k] = sv.ogeo[oi];
-        pt0[k] = opt0[oi];
-        pt1[k] = opt1[oi];
-        ids[k] = oids[oi];
-        touching[k] = (ohdr[oi] >> 8) > 0 ? 1 : 0;
+        const int oh = ohdr[oi];
+        st.hdr[s] = oh;
+        st.geo[s] = sv.ogeo[oi];
+        st.pt0[s] = opt0[oi];
+        st.pt1[s] = opt1[oi];
+        st.ids[s] = oids[oi];
+        st.touching[s] = (oh >> 8) > 0 ? 1 : 0;
       } else {
         const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        hdr[k] = 0;
-        geo[k] = z;
-        pt0[k] = z;
-        pt1[k] = z;
-        ids[k] = make_uint2(0u, 0u);
-        touching[k] = 0;
+        st.hdr[s] = 0;
+        st.geo[s] = z;
+        st.pt0[s] = z;
+        st.pt1[s] = z;
+        st.ids[s] = make_uint2(0u, 0u);
+        st.touching[s] = 0;
       }
-      return;
-    }
-  }
-  // shapes are read in place from their records (no private copy of the vertex arrays)
-  const ShapeV& sa = shapes[fa];
-  const ShapeV& sb = shapes[fb];
-  Xf xa = xf_of(bxf[bA]);
-  Xf xb = xf_of(bxf[bB]);
-  Manifold m;
-  evaluate(&m, sa, xa, sb, xb);
-  float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  const int ocount = oi >= 0 ? ohdr[oi] >> 8 : 0;
-  if (m.count > 0 && ocount > 0) {
-    const uint2 oid = oids[oi];
-    const float4 o0 = opt0[oi], o1 = opt1[oi];
+    } else {
+      // shapes are read in place from their records (no private copy of the vertex arrays)
+      const ShapeV& sa = shapes[fa];
+      const ShapeV& sb = shapes[fb];
+      Xf xa = xf_of(bxf[bA]);
+      Xf xb = xf_of(bxf[bB]);
+      Manifold m;
+      evaluate(&m, sa, xa, sb, xb);
+      float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      const int ocount = oi >= 0 ? ohdr[oi] >> 8 : 0;
+      if (m.count > 0 && ocount > 0) {
+        const uint2 oid = oids[oi];
+        const float4 o0 = opt0[oi], o1 = opt1[oi];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      if (i >= m.count) break;
-      if (m.id[i] == oid.x) { imp[2 * i] = o0.z; imp[2 * i + 1] = o0.w; }
-      else if (ocount > 1 && m.id[i] == oid.y) { imp[2 * i] = o1.z; imp[2 * i + 1] = o1.w; }
+        for (int i = 0; i < 2; ++i) {
+          if (i >= m.count) break;
+          if (m.id[i] == oid.x) { imp[2 * i] = o0.z; imp[2 * i + 1] = o0.w; }
+          else if (ocount > 1 && m.id[i] == oid.y) { imp[2 * i] = o1.z; imp[2 * i + 1] = o1.w; }
+        }
+      }
+      store_manifold(s, m, imp, st.hdr, st.geo, st.pt0, st.pt1, st.ids);
+      st.touching[s] = m.count > 0 ? 1 : 0;
+      if (sv.awake && (m.count > 0) != (ocount > 0)) {
+        if (sv.btype[bA] != 0) sv.wake[bA] = 1;
+        if (sv.btype[bB] != 0) sv.wake[bB] = 1;
+      }
     }
   }
-  store_manifold(k, m, imp, hdr, geo, pt0, pt1, ids);
-  touching[k] = m.count > 0 ? 1 : 0;
-  if (sv.awake && (m.count > 0) != (ocount > 0)) {
-    if (sv.btype[bA] != 0) sv.wake[bA] = 1;
-    if (sv.btype[bB] != 0) sv.wake[bB] = 1;
+  __syncthreads();
+  const int ko = base + t;
+  if (ko < n) {
+    ab[ko] = st.ab[t];
+    hdr[ko] = st.hdr[t];
+    geo[ko] = st.geo[t];
+    pt0[ko] = st.pt0[t];
+    pt1[ko] = st.pt1[t];
+    ids[ko] = st.ids[t];
+    touching[ko] = st.touching[t];
   }
 }
 
