@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libb2g.so")
 
 OK, ERR_INVALID_ARGUMENT, ERR_CAPACITY, ERR_DEVICE, ERR_DEGENERATE, ERR_UNSUPPORTED = range(6)
 B2_STATIC_BODY, B2_KINEMATIC_BODY, B2_DYNAMIC_BODY = 0, 1, 2
-SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX = 0, 1, 2, 4
+SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX, SHAPE_POLYGON_EXACT = 0, 1, 2, 4, 5
 PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
 # kernels b2g_world_time_kernel can time (B2G_KERNEL_*)
 KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2, "init_constraints": 3,
@@ -35,7 +35,7 @@ BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), 
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
                      ("angular_velocity", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),
                      ("gravity_scale", "<f4"), ("pad", "<f4")])
-SHAPE_DEF = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", "<f4"), ("params", "<f4", 20)])
+SHAPE_DEF = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", "<f4"), ("params", "<f4", 36)])
 FIXTURE_DEF = np.dtype([("friction", "<f4"), ("restitution", "<f4"), ("restitution_threshold", "<f4"),
                         ("density", "<f4"), ("is_sensor", "<i4"), ("category", "<u2"), ("mask", "<u2"),
                         ("group", "<i2"), ("pad0", "<i2"), ("pad1", "<i4")])
@@ -46,7 +46,7 @@ POINT = np.dtype([("local_point", "<f4", 2), ("normal_impulse", "<f4"), ("tangen
                   ("pad", "<i4")])
 MANIFOLD = np.dtype([("type", "<i4"), ("point_count", "<i4"), ("local_normal", "<f4", 2),
                      ("local_point", "<f4", 2), ("points", POINT, 2)])
-assert BODY_DEF.itemsize == 56 and SHAPE_DEF.itemsize == 96 and FIXTURE_DEF.itemsize == 32
+assert BODY_DEF.itemsize == 56 and SHAPE_DEF.itemsize == 160 and FIXTURE_DEF.itemsize == 32
 assert BODY_STATE.itemsize == 64 and MANIFOLD.itemsize == 72
 
 
@@ -56,6 +56,11 @@ class B2gError(RuntimeError):
     def __init__(self, status, message):
         super().__init__("b2g error %d: %s" % (status, message))
         self.status = status
+
+
+class _SolverState(ctypes.Structure):
+    _fields_ = [("inv_dt0", ctypes.c_float), ("n_colours", ctypes.c_int32), ("scratch_colours", ctypes.c_int32),
+                ("steps_since_scratch", ctypes.c_int32)]
 
 
 class _Profile(ctypes.Structure):
@@ -105,6 +110,12 @@ def _load():
         "b2g_tape_backward": ([P, P, P, P], I),
         "b2g_tape_export": ([P, I, I, P, P, P, I, P, P, P], I),
         "b2g_world_kernel_time": ([P, P, P, P, P], I),
+        "b2g_world_get_shapes": ([P, P, I], I),
+        "b2g_world_get_contact_colours": ([P, I, P, P], I),
+        "b2g_world_set_contacts": ([P, I, P, P, P], I),
+        "b2g_world_get_destroyed_contacts": ([P, I, P, P, P], I),
+        "b2g_world_get_solver_state": ([P, P], I),
+        "b2g_world_set_solver_state": ([P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -149,6 +160,10 @@ class World:
         _check(_lib.b2g_world_create(float(gravity[0]), float(gravity[1]), ctypes.byref(h)))
         self._h = h
         self.gravity = (float(gravity[0]), float(gravity[1]))
+        self.flags = {"warm_starting": True, "allow_sleep": False, "continuous": False, "block_solve": True}
+        # host copies of the definitions handed over (the serde world writer reads them)
+        self._body_defs, self._fixture_body, self._fixture_defs = [], [], []
+        self._n_bodies, self._batched = 0, False
 
     def close(self):
         if getattr(self, "_h", None):
@@ -161,6 +176,8 @@ class World:
     def set_flags(self, warm_starting=True, allow_sleep=False, continuous=False, block_solve=True):
         _check(_lib.b2g_world_set_flags(self._h, int(warm_starting), int(allow_sleep), int(continuous),
                                         int(block_solve)))
+        self.flags = {"warm_starting": bool(warm_starting), "allow_sleep": bool(allow_sleep),
+                      "continuous": bool(continuous), "block_solve": bool(block_solve)}
 
     def add(self, body_defs, fixture_body, shape_defs, fixture_defs):
         bd = _as(body_defs, BODY_DEF)
@@ -169,7 +186,17 @@ class World:
         fb = np.ascontiguousarray(fixture_body, np.int32)
         proxies = np.zeros(len(fb), np.int32)
         _check(_lib.b2g_world_add(self._h, len(bd), _ptr(bd), len(fb), _ptr(fb), _ptr(sd), _ptr(fd), _ptr(proxies)))
+        self._body_defs.append(bd.copy())
+        self._fixture_body.append(fb + self._n_bodies)
+        self._fixture_defs.append(fd.copy())
+        self._n_bodies += len(bd)
         return proxies
+
+    def definitions(self):
+        """(body_defs, fixture_body, fixture_defs) handed to add(), concatenated (fixture_body
+        holds world-wide body indices)."""
+        cat = lambda parts, dt: np.concatenate(parts) if parts else np.zeros(0, dt)
+        return (cat(self._body_defs, BODY_DEF), cat(self._fixture_body, np.int32), cat(self._fixture_defs, FIXTURE_DEF))
 
     def add_batched(self, body_defs, body_world, fixture_body, shape_defs, fixture_defs):
         """Bodies of many independent worlds (world id per body) stepped together."""
@@ -181,6 +208,7 @@ class World:
         proxies = np.zeros(len(fb), np.int32)
         _check(_lib.b2g_world_add_batched(self._h, len(bd), _ptr(bd), _ptr(bw), len(fb), _ptr(fb), _ptr(sd), _ptr(fd),
                                           _ptr(proxies)))
+        self._batched = True
         return proxies
 
     @property
@@ -243,6 +271,50 @@ class World:
         _check(_lib.b2g_world_get_contacts(self._h, n, _ptr(fab), _ptr(flags), _ptr(man), ctypes.byref(cnt)))
         k = min(cnt.value, n)
         return fab[:k], flags[:k], man[:k]
+
+    def shapes(self):
+        """Every fixture's shape as the device holds it (SHAPE_DEF; polygons as SHAPE_POLYGON_EXACT)."""
+        out = np.zeros(self.fixture_count, SHAPE_DEF)
+        _check(_lib.b2g_world_get_shapes(self._h, _ptr(out), len(out)))
+        return out
+
+    def contact_colours(self):
+        """Per contact of contacts(): the solve colour its predecessor had (-1: none)."""
+        n = self.contact_count
+        out = np.full(max(n, 1), -1, np.int32)
+        cnt = ctypes.c_int32()
+        _check(_lib.b2g_world_get_contact_colours(self._h, n, _ptr(out), ctypes.byref(cnt)))
+        return out[:min(cnt.value, n)]
+
+    def destroyed_contacts(self):
+        """Touching contacts of the last solve that left the contact set (the next collide
+        wakes their bodies): (fixtures_ab, manifolds)."""
+        cnt = ctypes.c_int32()
+        _check(_lib.b2g_world_get_destroyed_contacts(self._h, 0, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        fab = np.zeros((max(n, 1), 2), np.int32)
+        man = np.zeros(max(n, 1), MANIFOLD)
+        _check(_lib.b2g_world_get_destroyed_contacts(self._h, n, _ptr(fab), _ptr(man), ctypes.byref(cnt)))
+        return fab[:n], man[:n]
+
+    def set_contacts(self, fixtures_ab, manifolds, colours=None):
+        """Rebuild the contact set from the current fat AABBs and give its contacts the given
+        manifolds (with impulses) and colours, matched by fixtures (A, B)."""
+        fab = np.ascontiguousarray(fixtures_ab, np.int32).reshape(-1, 2)
+        man = np.ascontiguousarray(manifolds, MANIFOLD)
+        col = None if colours is None else np.ascontiguousarray(colours, np.int32)
+        if len(man) != len(fab) or (col is not None and len(col) != len(fab)):
+            raise ValueError("set_contacts: fixtures, manifolds and colours differ in length")
+        _check(_lib.b2g_world_set_contacts(self._h, len(fab), _ptr(fab), _ptr(man), _ptr(col)))
+
+    def solver_state(self):
+        st = _SolverState()
+        _check(_lib.b2g_world_get_solver_state(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in _SolverState._fields_}
+
+    def set_solver_state(self, state):
+        st = _SolverState(**{name: state[name] for name, _ in _SolverState._fields_})
+        _check(_lib.b2g_world_set_solver_state(self._h, ctypes.byref(st)))
 
     def solver_order(self):
         n = ctypes.c_int32()

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <unordered_map>
+
 #include "b2g_internal.h"
 
 struct b2g_world {
@@ -391,6 +393,7 @@ static int run_phase(World* w, int phase, float dt, int vi, int pi) {
     }
     if ((st = phase_collide(w))) return st;
     w->manifolds_valid[w->cur] = true;
+    w->collide_pending = false;
     return B2G_OK;
   }
   if (phase == B2G_PHASE_SOLVE) {
@@ -405,6 +408,7 @@ static int run_phase(World* w, int phase, float dt, int vi, int pi) {
       if ((st = carry_manifolds(w))) return st;
       if ((st = phase_broadphase(w))) return st;
       w->manifolds_valid[w->cur] = false;
+      w->collide_pending = true;
       w->inv_dt0 = 1.0f / dt;
     }
     return B2G_OK;
@@ -745,6 +749,240 @@ int b2g_world_get_contacts(b2g_world* h, int32_t cap, int32_t* fixtures_ab, int3
       mf.points[1].id = ids[i].y; mf.points[1].pad = 0;
     }
   }
+  return B2G_OK;
+}
+
+int b2g_world_get_shapes(b2g_world* h, b2g_shape_def* out, int32_t n) {
+  if (!h || !out || n != h->w.np) { set_error("b2g_world_get_shapes: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  for (int f = 0; f < n; ++f) {
+    const HostShape& s = w->h_shapes[f];
+    b2g_shape_def& d = out[f];
+    std::memset(&d, 0, sizeof(d));
+    d.radius = s.radius;
+    if (s.type == kCircle) {
+      d.type = B2G_SHAPE_CIRCLE;
+      d.params[0] = s.p.x;
+      d.params[1] = s.p.y;
+    } else if (s.type == kEdge) {
+      d.type = B2G_SHAPE_EDGE;
+      d.flags = s.one_sided ? 1 : 0;
+      for (int i = 0; i < 4; ++i) {
+        d.params[2 * i] = s.v[i].x;
+        d.params[2 * i + 1] = s.v[i].y;
+      }
+    } else {
+      d.type = B2G_SHAPE_POLYGON_EXACT;
+      d.count = s.count;
+      for (int i = 0; i < s.count; ++i) {
+        d.params[2 * i] = s.v[i].x;
+        d.params[2 * i + 1] = s.v[i].y;
+        d.params[16 + 2 * i] = s.n[i].x;
+        d.params[16 + 2 * i + 1] = s.n[i].y;
+      }
+      d.params[32] = s.p.x;
+      d.params[33] = s.p.y;
+    }
+  }
+  return B2G_OK;
+}
+
+int b2g_world_get_contact_colours(b2g_world* h, int32_t cap, int32_t* colours, int32_t* count) {
+  if (!h || !count || cap < 0 || (cap > 0 && !colours)) {
+    set_error("b2g_world_get_contact_colours: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  const int n = w->n_pairs;
+  *count = n;
+  const int m = std::min(cap, n);
+  if (m == 0) return B2G_OK;
+  std::fill(colours, colours + m, -1);
+  // the previous set's colours, read through each contact's predecessor (as the next step's
+  // colouring reads them)
+  const int n_old = w->set_np[1 - w->cur] > 0 ? w->n_pairs_old : 0;
+  if (n_old == 0 || w->need_broadphase) return B2G_OK;
+  std::vector<int> oi, oc;
+  if ((st = down(oi, w->c[w->cur].old_index, m, w->stream)) || (st = down(oc, w->c[1 - w->cur].colour, n_old, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  for (int k = 0; k < m; ++k)
+    if (oi[k] >= 0 && oi[k] < n_old) colours[k] = oc[oi[k]];
+  return B2G_OK;
+}
+
+int b2g_world_get_destroyed_contacts(b2g_world* h, int32_t cap, int32_t* fixtures_ab, b2g_manifold* manifolds,
+                                     int32_t* count) {
+  if (!h || !count || cap < 0 || (cap > 0 && (!fixtures_ab || !manifolds))) {
+    set_error("b2g_world_get_destroyed_contacts: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  *count = 0;
+  // before the next collide only: afterwards the predecessor set is that collide's input
+  if (w->need_broadphase || !w->collide_pending || w->set_np[1 - w->cur] == 0) return B2G_OK;
+  const int n_old = w->n_pairs_old, np = w->n_pairs;
+  if (n_old == 0) return B2G_OK;
+  const ContactArrays& c = w->c[w->cur];
+  const ContactArrays& o = w->c[1 - w->cur];
+  std::vector<int> oi, hdr;
+  std::vector<int2> ab;
+  std::vector<float4> geo, p0, p1;
+  std::vector<uint2> ids;
+  if ((st = down(oi, c.old_index, np, w->stream)) || (st = down(hdr, o.hdr, n_old, w->stream)) ||
+      (st = down(ab, o.ab, n_old, w->stream)) || (st = down(geo, o.geo, n_old, w->stream)) ||
+      (st = down(p0, o.pt0, n_old, w->stream)) || (st = down(p1, o.pt1, n_old, w->stream)) ||
+      (st = down(ids, o.ids, n_old, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  std::vector<char> kept(n_old, 0);
+  for (int k = 0; k < np; ++k)
+    if (oi[k] >= 0 && oi[k] < n_old) kept[oi[k]] = 1;
+  int m = 0;
+  for (int k = 0; k < n_old; ++k) {
+    if (kept[k] || (hdr[k] >> 8) == 0) continue;
+    if (m < cap) {
+      fixtures_ab[2 * m] = ab[k].x;
+      fixtures_ab[2 * m + 1] = ab[k].y;
+      b2g_manifold& mf = manifolds[m];
+      std::memset(&mf, 0, sizeof(mf));
+      mf.type = hdr[k] & 0xff;
+      mf.point_count = hdr[k] >> 8;
+      mf.local_normal[0] = geo[k].x; mf.local_normal[1] = geo[k].y;
+      mf.local_point[0] = geo[k].z; mf.local_point[1] = geo[k].w;
+      mf.points[0].local_point[0] = p0[k].x; mf.points[0].local_point[1] = p0[k].y;
+      mf.points[0].normal_impulse = p0[k].z; mf.points[0].tangent_impulse = p0[k].w;
+      mf.points[0].id = ids[k].x;
+      mf.points[1].local_point[0] = p1[k].x; mf.points[1].local_point[1] = p1[k].y;
+      mf.points[1].normal_impulse = p1[k].z; mf.points[1].tangent_impulse = p1[k].w;
+      mf.points[1].id = ids[k].y;
+    }
+    ++m;
+  }
+  *count = m;
+  return B2G_OK;
+}
+
+int b2g_world_set_contacts(b2g_world* h, int32_t n, const int32_t* fixtures_ab, const b2g_manifold* manifolds,
+                           const int32_t* colours) {
+  if (!h || n < 0 || (n > 0 && (!fixtures_ab || !manifolds))) {
+    set_error("b2g_world_set_contacts: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  if (w->tape.active) { set_error("b2g_world_set_contacts: a tape is being recorded"); return B2G_ERR_INVALID_ARGUMENT; }
+  // the pair set of the current fat AABBs becomes the current set ...
+  if ((st = phase_broadphase(w))) return st;
+  w->need_broadphase = false;
+  const int np = w->n_pairs;
+  ContactArrays& c = w->c[w->cur];
+  ContactArrays& o = w->c[1 - w->cur];
+  std::vector<int2> pairs;
+  if ((st = down(pairs, c.pair, np, w->stream))) return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  // ... and the checkpoint becomes its predecessor set: the same pairs, with the checkpoint's
+  // manifolds and colours, each contact its own predecessor
+  std::unordered_map<long long, int> index;
+  index.reserve(size_t(np) * 2 + 1);
+  std::vector<int2> ab(np);
+  for (int k = 0; k < np; ++k) {
+    const int lo = std::min(pairs[k].x, pairs[k].y), hi = std::max(pairs[k].x, pairs[k].y);
+    const bool swap = registry_swap(w->h_fshape[lo].x & 0xff, w->h_fshape[hi].x & 0xff) == 1;
+    ab[k] = swap ? make_int2(hi, lo) : make_int2(lo, hi);
+    index[(long long)ab[k].x << 32 | (unsigned)ab[k].y] = k;
+  }
+  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  std::vector<int> hdr(np, 0), col(np, -1), oi(np);
+  std::vector<float4> geo(np, z), p0(np, z), p1(np, z);
+  std::vector<uint2> ids(np, make_uint2(0u, 0u));
+  // entries that are not pairs of the rebuilt set are contacts the next collide destroys: they
+  // follow the set's own pairs in the predecessor set (so that their bodies are woken)
+  int n_old = np;
+  for (int i = 0; i < n; ++i) {
+    const int fa = fixtures_ab[2 * i], fb = fixtures_ab[2 * i + 1];
+    if (fa < 0 || fb < 0 || fa >= w->np || fb >= w->np || fa == fb) {
+      set_error("b2g_world_set_contacts: contact " + std::to_string(i) + " has invalid fixtures");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+    const b2g_manifold& m = manifolds[i];
+    if (m.point_count < 0 || m.point_count > 2 || m.type < 0 || m.type > 2) {
+      set_error("b2g_world_set_contacts: bad manifold " + std::to_string(i));
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+    auto it = index.find((long long)fa << 32 | (unsigned)fb);
+    int k;
+    if (it != index.end()) {
+      k = it->second;
+    } else {
+      k = n_old++;
+      ab.push_back(make_int2(fa, fb));
+      hdr.push_back(0);
+      col.push_back(-1);
+      geo.push_back(z);
+      p0.push_back(z);
+      p1.push_back(z);
+      ids.push_back(make_uint2(0u, 0u));
+      pairs.push_back(make_int2(fa, fb));
+    }
+    hdr[k] = m.type | (m.point_count << 8);
+    geo[k] = make_float4(m.local_normal[0], m.local_normal[1], m.local_point[0], m.local_point[1]);
+    p0[k] = make_float4(m.points[0].local_point[0], m.points[0].local_point[1], m.points[0].normal_impulse,
+                        m.points[0].tangent_impulse);
+    p1[k] = make_float4(m.points[1].local_point[0], m.points[1].local_point[1], m.points[1].normal_impulse,
+                        m.points[1].tangent_impulse);
+    ids[k] = make_uint2(m.points[0].id, m.points[1].id);
+    if (k < np) col[k] = colours ? colours[i] : -1;
+  }
+  for (int k = 0; k < np; ++k) oi[k] = k;
+  const int cap = std::max(n_old, 1);
+  if ((st = o.pair.reserve(cap)) || (st = o.ab.reserve(cap)) || (st = o.hdr.reserve(cap)) ||
+      (st = o.geo.reserve(cap)) || (st = o.pt0.reserve(cap)) || (st = o.pt1.reserve(cap)) ||
+      (st = o.ids.reserve(cap)) || (st = o.colour.reserve(cap)) || (st = o.old_index.reserve(cap)) ||
+      (st = o.off.reserve(w->np + 1)) || (st = c.old_index.reserve(std::max(np, 1))))
+    return st;
+  if (n_old > 0) {
+    if ((st = up(o.pair, pairs, w->stream)) || (st = up(o.ab, ab, w->stream)) || (st = up(o.hdr, hdr, w->stream)) ||
+        (st = up(o.geo, geo, w->stream)) || (st = up(o.pt0, p0, w->stream)) || (st = up(o.pt1, p1, w->stream)) ||
+        (st = up(o.ids, ids, w->stream)) || (st = up(o.colour, col, w->stream)))
+      return st;
+  }
+  if (np > 0 && (st = up(c.old_index, oi, w->stream))) return st;
+  B2G_HIP(hipMemcpyAsync(o.off.p, c.off.p, sizeof(int) * (w->np + 1), hipMemcpyDeviceToDevice, w->stream));
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  w->set_np[1 - w->cur] = w->np;
+  w->n_pairs_old = n_old;
+  w->manifolds_valid[1 - w->cur] = true;
+  w->manifolds_valid[w->cur] = false;
+  w->collide_pending = true;
+  return B2G_OK;
+}
+
+int b2g_world_get_solver_state(b2g_world* h, b2g_solver_state* out) {
+  if (!h || !out) { set_error("b2g_world_get_solver_state: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  const World* w = &h->w;
+  out->inv_dt0 = w->inv_dt0;
+  out->n_colours = w->n_colors;
+  out->scratch_colours = w->scratch_colors;
+  out->steps_since_scratch = w->steps_since_scratch;
+  return B2G_OK;
+}
+
+int b2g_world_set_solver_state(b2g_world* h, const b2g_solver_state* in) {
+  if (!h || !in || in->n_colours < 0 || in->scratch_colours < 0 || in->steps_since_scratch < 0) {
+    set_error("b2g_world_set_solver_state: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  w->inv_dt0 = in->inv_dt0;
+  w->n_colors = in->n_colours;
+  w->scratch_colors = in->scratch_colours;
+  w->steps_since_scratch = in->steps_since_scratch;
   return B2G_OK;
 }
 

@@ -136,6 +136,21 @@ int build_host_shape(const b2g_shape_def& d, HostShape* s) {
       s->p = host_centroid(s->v, h);
       return B2G_OK;
     }
+    case B2G_SHAPE_POLYGON_EXACT: {
+      // a stored B2polygonShape taken as it is (the serde polygon record,
+      // src/shapes/serialize_b2_polygon_shape.rs:40-90): no hull, normals and centroid given
+      const int n = d.count;
+      if (n < 3 || n > 8) { host_box(s, 1.0f, 1.0f); return B2G_ERR_DEGENERATE; }
+      s->type = kPolygon;
+      s->radius = d.radius;
+      s->count = n;
+      for (int i = 0; i < n; ++i) {
+        s->v[i] = v2(d.params[2 * i], d.params[2 * i + 1]);
+        s->n[i] = v2(d.params[16 + 2 * i], d.params[16 + 2 * i + 1]);
+      }
+      s->p = v2(d.params[32], d.params[33]);
+      return B2G_OK;
+    }
     default:
       return B2G_ERR_UNSUPPORTED;
   }

@@ -253,6 +253,7 @@ struct World {
   int nb = 0, np = 0;          // bodies, proxies on the host
   int dev_nb = 0, dev_np = 0;  // uploaded
   bool need_broadphase = false;  // new fixtures: find pairs before the next collide
+  bool collide_pending = false;  // the current set has not been through a collide yet
   bool dirty = false;            // host items not yet on the device
   bool manifolds_valid[2] = {true, true};  // set went through a narrowphase (or a carry)
   bool profile_pending = false;

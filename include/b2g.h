@@ -37,6 +37,10 @@ extern "C" {
 #define B2G_SHAPE_EDGE 1    /* params = v1, v2 (two-sided) [+ v0, v3 when flags bit0]     */
 #define B2G_SHAPE_POLYGON 2 /* count raw points in params, passed to the hull (set())     */
 #define B2G_SHAPE_BOX 4     /* params = hx, hy [, cx, cy, angle when flags bit0]          */
+/* A polygon as B2polygonShape stores it (the serde record of src/shapes/serialize_b2_polygon_shape.rs:
+ * m_vertices, m_normals, m_centroid), taken verbatim: count vertices in params[0..2*count),
+ * normals in params[16..16+2*count), centroid in params[32..34), radius = m_radius. */
+#define B2G_SHAPE_POLYGON_EXACT 5
 
 /* Body definition: B2bodyDef (src/b2_body.rs:32-100). 56 bytes. */
 typedef struct b2g_body_def {
@@ -57,7 +61,7 @@ typedef struct b2g_shape_def {
   int32_t count;
   int32_t flags;
   float radius;
-  float params[20];
+  float params[36];
 } b2g_shape_def;
 
 /* Fixture definition: B2fixtureDef + B2filter (src/b2_fixture.rs:14-58). 32 bytes. */
@@ -181,6 +185,41 @@ int b2g_world_set_fat_aabbs(b2g_world* world, const float* in, int32_t n);
  * *count receives the number of contacts; at most cap are written. */
 int b2g_world_get_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, int32_t* flags,
                            b2g_manifold* manifolds, int32_t* count);
+
+/* The shape of every fixture as the device holds it (after the hull, box and edge setters):
+ * circles as B2G_SHAPE_CIRCLE, edges as B2G_SHAPE_EDGE (flags bit0 = one-sided), polygons as
+ * B2G_SHAPE_POLYGON_EXACT. What the serde world writer records (src/serialize/serialize_b2_fixture.rs:23-52). */
+int b2g_world_get_shapes(b2g_world* world, b2g_shape_def* out, int32_t n);
+
+/* Checkpoint of the contact state between steps (the solver-state extension of the serde world
+ * format).
+ * get_contact_colours: for each contact of b2g_world_get_contacts, in the same order, the solve
+ * colour its predecessor had in the last solve (-1: none), which the next step's colouring may
+ * keep.
+ * get_destroyed_contacts: the touching contacts of the last solve that are not in the current set
+ * (their fat AABBs stopped overlapping); the next collide wakes their bodies when sleeping is
+ * possible (b2_contact_destroy, src/private/dynamics/b2_contact.rs:33-45). Empty once that collide
+ * has run.
+ * set_contacts: rebuilds the contact set from the current fat AABBs (the pair search of
+ * B2contactManager::find_new_contacts, src/b2_contact_manager.rs:35-45) and gives each contact
+ * the manifold (with impulses) and colour of the entry with the same fixtures A, B; contacts
+ * without an entry start empty, and entries whose fixtures are not a pair of the rebuilt set are
+ * taken as destroyed contacts. Call after b2g_world_set_body_states and b2g_world_set_fat_aabbs.
+ * colours may be null (all -1). */
+int b2g_world_get_contact_colours(b2g_world* world, int32_t cap, int32_t* colours, int32_t* count);
+int b2g_world_get_destroyed_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, b2g_manifold* manifolds,
+                                     int32_t* count);
+int b2g_world_set_contacts(b2g_world* world, int32_t n, const int32_t* fixtures_ab, const b2g_manifold* manifolds,
+                           const int32_t* colours);
+
+/* Step-to-step solver scalars: m_inv_dt0 of B2world (src/private/dynamics/b2_world.rs:924, 949: the
+ * warm-start ratio is dt * inv_dt0) and the colour-reuse counters of the coloured solve. */
+typedef struct b2g_solver_state {
+  float inv_dt0;
+  int32_t n_colours, scratch_colours, steps_since_scratch;
+} b2g_solver_state;
+int b2g_world_get_solver_state(b2g_world* world, b2g_solver_state* out);
+int b2g_world_set_solver_state(b2g_world* world, const b2g_solver_state* in);
 
 /* Solver constraint order of the last SOLVE phase: contact indices in solve order (colour by
  * colour) and colour offsets (n_colours + 1 entries). Used to replay the solve on the CPU. */

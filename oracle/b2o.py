@@ -23,7 +23,7 @@ def _dtypes(real):
                          ("angular_velocity", real), ("linear_damping", real), ("angular_damping", real),
                          ("gravity_scale", real), ("pad", real)], align=True)
     shape_def = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", real),
-                          ("params", real, 20)], align=True)
+                          ("params", real, 36)], align=True)
     fixture_def = np.dtype([("friction", real), ("restitution", real), ("restitution_threshold", real),
                             ("density", real), ("is_sensor", "<i4"), ("category", "<u2"), ("mask", "<u2"),
                             ("group", "<i2"), ("pad0", "<i2"), ("pad1", "<i4")], align=True)

@@ -28,13 +28,14 @@ struct B2oBodyDef {
 };
 
 // Shape kinds of B2oShapeDef::type
-enum { B2O_CIRCLE = 0, B2O_EDGE = 1, B2O_POLYGON = 2, B2O_BOX = 4 };
+enum { B2O_CIRCLE = 0, B2O_EDGE = 1, B2O_POLYGON = 2, B2O_BOX = 4, B2O_POLYGON_EXACT = 5 };
 struct B2oShapeDef {
   int32_t type;
   int32_t count;   // polygon: number of raw points passed to the hull
   int32_t flags;   // box: bit0 = centred/rotated (set_as_box_angle); edge: bit0 = one-sided
   real radius;     // circle radius
-  real params[20]; // circle: p; polygon: points; box: hx,hy,cx,cy,angle; edge: v1,v2,v0,v3
+  real params[36]; // circle: p; polygon: points; box: hx,hy,cx,cy,angle; edge: v1,v2,v0,v3;
+                   // exact polygon: vertices [0,16), normals [16,32), centroid [32,34)
 };
 
 struct B2oFixtureDef {
@@ -100,6 +101,20 @@ static void shape_from_def(const B2oShapeDef& d, Shape* s, bool* ok) {
       int n = std::min<int>(d.count, kMaxPolygonVertices);
       for (int i = 0; i < n; ++i) pts[i] = Vec2(d.params[2 * i], d.params[2 * i + 1]);
       *ok = polygon_set(s, pts, n);
+      break;
+    }
+    case B2O_POLYGON_EXACT: {
+      // a stored B2polygonShape (the serde record, src/shapes/serialize_b2_polygon_shape.rs:40-90)
+      const int n = d.count;
+      if (n < 3 || n > kMaxPolygonVertices) { *ok = false; break; }
+      s->type = kPolygon;
+      s->radius = d.radius;
+      s->count = n;
+      for (int i = 0; i < n; ++i) {
+        s->vertices[i] = Vec2(d.params[2 * i], d.params[2 * i + 1]);
+        s->normals[i] = Vec2(d.params[16 + 2 * i], d.params[16 + 2 * i + 1]);
+      }
+      s->centroid = Vec2(d.params[32], d.params[33]);
       break;
     }
     default: *ok = false;

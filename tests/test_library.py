@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     import b2g
     assert b2g.BODY_DEF.itemsize == 56
-    assert b2g.SHAPE_DEF.itemsize == 96
+    assert b2g.SHAPE_DEF.itemsize == 160
     assert b2g.FIXTURE_DEF.itemsize == 32
     assert b2g.BODY_STATE.itemsize == 64
     assert b2g.MANIFOLD.itemsize == 72

@@ -62,13 +62,13 @@ def _world(oracle, bodies, fixtures, gravity=(0.0, -10.0)):
 
 
 def _box(hx, hy):
-    p = np.zeros(20)
+    p = np.zeros(36)
     p[:2] = (hx, hy)
     return {"type": BOX, "params": p}
 
 
 def _circle(r, p=(0, 0)):
-    q = np.zeros(20)
+    q = np.zeros(36)
     q[:2] = p
     return {"type": CIRCLE, "radius": r, "params": q}
 

@@ -1,0 +1,158 @@
+"""The serde world format (CPU): layout, number round trips, exact polygons.
+
+* Layout: tests/golden/serde_world_small.json is written by hand from the reference's serde
+  definitions -- world members (src/serialize/serialize_b2_world.rs:131-179), body record
+  (src/serialize/serialize_b2_body.rs:57-70), B2bodyDef field order (src/b2_body.rs:89-148),
+  fixture record (src/serialize/serialize_b2_fixture.rs:23-52), B2filter (src/b2_fixture.rs:28-42),
+  shapes (src/shapes/b2_circle_shape.rs:10-16, src/shapes/b2_edge_shape.rs:14-28,
+  src/shapes/serialize_b2_polygon_shape.rs:40-90, base src/b2_shape.rs:42-50), enums by variant
+  name -- in serde_json's pretty layout (two-space indent), bodies newest first. The reader must
+  give the definitions it holds, and the writer must give the file back byte for byte. No file
+  written by the reference itself exists in its tree (its testbed writes them at run time,
+  examples/testbed/test_serialize.rs:22-54), so the byte layout against serde_json's own output
+  is parity unpinned; values are pinned (f32 text round trip below).
+* Numbers: every finite f32 bit pattern drawn at random survives text -> value exactly.
+* Exact polygons: a polygon stored as vertices, normals and centroid (what the format holds)
+  builds the same oracle shape and mass as the hull it came from (B2polygonShape::set,
+  src/private/collision/b2_polygon_shape.rs:96-211), bit for bit.
+* What the GPU path does not hold (joints, chains, disabled bodies) is refused.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from b2g import serialize as ser
+import b2g
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "serde_world_small.json")
+
+
+def test_golden_layout_reads_and_writes_back():
+    text = open(GOLDEN).read()
+    doc = ser.read_world(text)
+    bd, fb, sd, fd = doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"]
+    assert doc["gravity"] == (0.0, -10.0) and doc["checkpoint"] is None
+    # file order: the box (created last), the circle, the ground (created first)
+    assert list(bd["type"]) == [2, 2, 0]
+    np.testing.assert_array_equal(bd["position"], np.array([[0.25, 4.0], [1.5, 2.0], [0.0, 0.0]], np.float32))
+    np.testing.assert_array_equal(bd["angle"], np.float32([0.5, 0.0, 0.0]))
+    np.testing.assert_array_equal(bd["linear_velocity"][0], np.float32([1.0, 0.0]))
+    assert bd["angular_velocity"][0] == np.float32(-2.0)
+    assert bd["linear_damping"][1] == np.float32(0.1) and bd["angular_damping"][1] == np.float32(0.05)
+    assert list(bd["allow_sleep"]) == [1, 0, 1] and list(bd["awake"]) == [1, 1, 0]
+    assert list(bd["fixed_rotation"]) == [0, 1, 0] and bd["gravity_scale"][1] == np.float32(0.5)
+    assert list(fb) == [0, 1, 2]
+    assert list(sd["type"]) == [b2g.SHAPE_POLYGON_EXACT, b2g.SHAPE_CIRCLE, b2g.SHAPE_EDGE]
+    assert sd["count"][0] == 4 and sd["radius"][0] == np.float32(0.01) and sd["radius"][1] == np.float32(0.5)
+    np.testing.assert_array_equal(sd["params"][0][:8], np.float32([-0.5, -0.5, 0.5, -0.5, 0.5, 0.5, -0.5, 0.5]))
+    np.testing.assert_array_equal(sd["params"][0][16:24], np.float32([0, -1, 1, 0, 0, 1, -1, 0]))
+    np.testing.assert_array_equal(sd["params"][2][:4], np.float32([-10, 0, 10, 0]))
+    assert sd["flags"][2] == 0
+    np.testing.assert_array_equal(fd["friction"], np.float32([0.6, 0.2, 0.6]))
+    np.testing.assert_array_equal(fd["restitution"], np.float32([0.0, 0.3, 0.0]))
+    np.testing.assert_array_equal(fd["density"], np.float32([1.0, 2.5, 0.0]))
+    assert (fd["category"][1], fd["mask"][1], fd["group"][1]) == (2, 65533, -3)
+    # written back in creation order (the reverse of the file's), the text is the file's
+    order = np.arange(len(bd))[::-1]
+    remap = np.empty_like(order)
+    remap[order] = np.arange(len(order))
+    out = ser.defs_to_json(doc["gravity"], bd[order], remap[fb][::-1], sd[::-1], fd[::-1])
+    assert out == text
+
+
+def test_write_read_round_trip_is_exact():
+    from b2g import scenes
+    scene = scenes.large_world(n=256, seed=3)
+    # polygons as stored: build exact records from the oracle's hull of every polygon
+    import b2o
+    o = b2o.Oracle("f32")
+    shapes, status = o.make_shapes(scene.shape_defs)
+    assert np.all(status == 0)
+    sd = exact_shape_defs(scene.shape_defs, shapes)
+    text = ser.defs_to_json((0.0, -10.0), scene.body_defs, scene.fixture_body, sd, scene.fixture_defs)
+    doc = ser.read_world(text)
+    nb = len(scene.body_defs)
+    back = np.arange(nb)[::-1]
+    for f in ("type", "position", "angle", "linear_velocity", "angular_velocity", "linear_damping",
+              "angular_damping", "allow_sleep", "fixed_rotation", "gravity_scale"):
+        np.testing.assert_array_equal(doc["body_defs"][f][back], scene.body_defs[f], err_msg=f)
+    # fixtures come back newest first: map them to their creation index
+    fb = np.asarray(scene.fixture_body)
+    per_body = [np.flatnonzero(fb == b) for b in range(nb)]
+    file_order = np.concatenate([per_body[b][::-1] for b in range(nb)[::-1]])
+    for f in ("friction", "restitution", "restitution_threshold", "density", "category", "mask", "group"):
+        np.testing.assert_array_equal(doc["fixture_defs"][f], scene.fixture_defs[f][file_order], err_msg=f)
+    got = doc["shape_defs"]
+    want = sd[file_order]
+    for f in ("type", "count", "flags", "radius", "params"):
+        np.testing.assert_array_equal(got[f], want[f], err_msg=f)
+
+
+def exact_shape_defs(shape_defs, oracle_shapes):
+    """The shapes as the serde format stores them (polygons: hull vertices, normals, centroid)."""
+    out = np.zeros(len(shape_defs), b2g.SHAPE_DEF)
+    for i, (d, s) in enumerate(zip(shape_defs, oracle_shapes)):
+        out[i]["radius"] = s["radius"]
+        if int(s["type"]) == 0:
+            out[i]["type"] = b2g.SHAPE_CIRCLE
+            out[i]["params"][:2] = s["p"]
+        elif int(s["type"]) == 1:
+            out[i]["type"] = b2g.SHAPE_EDGE
+            out[i]["flags"] = int(s["one_sided"])
+            out[i]["params"][:8] = np.concatenate([s["v1"], s["v2"], s["v0"], s["v3"]])
+        else:
+            n = int(s["count"])
+            out[i]["type"] = b2g.SHAPE_POLYGON_EXACT
+            out[i]["count"] = n
+            out[i]["params"][:2 * n] = s["vertices"][:2 * n]
+            out[i]["params"][16:16 + 2 * n] = s["normals"][:2 * n]
+            out[i]["params"][32:34] = s["centroid"]
+    return out
+
+
+def test_exact_polygon_equals_its_hull(oracle):
+    from b2g import scenes
+    scene = scenes.large_world(n=512, seed=5)
+    poly = np.flatnonzero(np.isin(scene.shape_defs["type"], (b2g.SHAPE_POLYGON, b2g.SHAPE_BOX)))
+    src = scene.shape_defs[poly]
+    hull, st = oracle.make_shapes(src)
+    assert np.all(st == 0)
+    exact = exact_shape_defs(src, hull)
+    again, st2 = oracle.make_shapes(exact)
+    assert np.all(st2 == 0)
+    for f in hull.dtype.names:
+        np.testing.assert_array_equal(again[f], hull[f], err_msg=f)
+    np.testing.assert_array_equal(oracle.shape_mass(again, 1.0), oracle.shape_mass(hull, 1.0))
+
+
+def test_f32_text_round_trip():
+    rng = np.random.default_rng(11)
+    bits = rng.integers(0, 2 ** 32, 60000, dtype=np.uint64).astype(np.uint32)
+    vals = bits.view(np.float32)
+    vals = vals[np.isfinite(vals)]
+    for x in vals[:20000]:
+        y = ser._parse_f32(ser._f32_text(x))
+        assert np.float32(y).view(np.uint32) == np.float32(x).view(np.uint32), repr(x)
+    # decimal text read with correct rounding (ties to even): the exact midpoint between 1 and its
+    # successor 1 + 2^-23 goes to 1 (even), a hair below to 1, a hair above to the successor
+    one_up = np.nextafter(np.float32(1.0), np.float32(2.0))
+    assert ser._parse_f32("1.000000059604644775390625") == np.float32(1.0)
+    assert ser._parse_f32("1.0000000596046447") == np.float32(1.0)
+    assert ser._parse_f32("1.0000000596046448") == one_up
+    # the same above 1 + 2^-23 (odd): its midpoint with 1 + 2^-22 goes up to the even one
+    assert ser._parse_f32("1.000000178813934326171875") == np.float32(1.0 + 2.0 ** -22)
+
+
+@pytest.mark.parametrize("what", ["joint", "chain", "disabled"])
+def test_unsupported_members_are_refused(what):
+    doc = json.loads(open(GOLDEN).read())
+    if what == "joint":
+        doc["m_joints_list"] = [{"jtype": "EDistanceJoint", "joint_def": {}}]
+    elif what == "chain":
+        doc["m_bodies_list"][1]["m_fixture_list"][0]["m_shape_type"] = "EChain"
+    else:
+        doc["m_bodies_list"][0]["m_definition"]["enabled"] = False
+    with pytest.raises(b2g.B2gError):
+        ser.read_world(json.dumps(doc))
