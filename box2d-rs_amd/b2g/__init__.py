@@ -31,6 +31,15 @@ KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2
 # tuning knobs (B2G_TUNE_*)
 TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH, TUNE_COLOR_REUSE = 2, 3, 4
 
+# Joint definition (B2G_JOINT_*: B2jointType values, src/b2_joint.rs:46-58)
+JOINT_DISTANCE, JOINT_REVOLUTE = 1, 8
+JOINT_DEF = np.dtype([("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
+                      ("local_anchor_a", "<f4", 2), ("local_anchor_b", "<f4", 2), ("length", "<f4"),
+                      ("min_length", "<f4"), ("max_length", "<f4"), ("stiffness", "<f4"), ("damping", "<f4"),
+                      ("reference_angle", "<f4"), ("lower_angle", "<f4"), ("upper_angle", "<f4"),
+                      ("max_motor_torque", "<f4"), ("motor_speed", "<f4"), ("enable_limit", "<i4"),
+                      ("enable_motor", "<i4"), ("pad", "<i4", 2)])
+
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
                      ("angular_velocity", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),

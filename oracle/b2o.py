@@ -37,8 +37,15 @@ def _dtypes(real):
     shape = np.dtype([("type", "<i4"), ("count", "<i4"), ("one_sided", "<i4"), ("pad", "<i4"), ("radius", real),
                       ("p", real, 2), ("centroid", real, 2), ("v0", real, 2), ("v1", real, 2), ("v2", real, 2),
                       ("v3", real, 2), ("vertices", real, 16), ("normals", real, 16)], align=True)
+    joint_def = np.dtype([("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
+                          ("local_anchor_a", real, 2), ("local_anchor_b", real, 2), ("length", real),
+                          ("min_length", real), ("max_length", real), ("stiffness", real), ("damping", real),
+                          ("reference_angle", real), ("lower_angle", real), ("upper_angle", real),
+                          ("max_motor_torque", real), ("motor_speed", real), ("enable_limit", "<i4"),
+                          ("enable_motor", "<i4"), ("pad", "<i4", 2)], align=True)
+    joint_state = np.dtype([("impulse", real, 5), ("pad", real, 3)], align=True)
     return dict(body_def=body_def, shape_def=shape_def, fixture_def=fixture_def, body_state=body_state,
-                manifold=manifold, shape=shape)
+                manifold=manifold, shape=shape, joint_def=joint_def, joint_state=joint_state)
 
 
 def _ptr(a):
@@ -65,6 +72,8 @@ class Oracle:
         L.b2o_world_set_flags.argtypes = [P, I, I, I]
         L.b2o_world_add.argtypes = [P, I, P, I, P, P, P, P]
         L.b2o_world_step.argtypes = [P, R, I, I]
+        L.b2o_world_add_joints.argtypes = [P, I, P]
+        L.b2o_world_get_joints.argtypes = [P, P]
         L.b2o_world_collide.argtypes = [P]
         L.b2o_world_run.argtypes = [P, R, I, I, I]
         L.b2o_world_run.restype = ctypes.c_double
@@ -207,6 +216,19 @@ class OracleWorld:
         if st != 0:
             raise ValueError("degenerate polygon in shape definitions")
         return proxies
+
+    def add_joints(self, joint_defs):
+        jd = self.o.cast(joint_defs, "joint_def")
+        made = self.o.lib.b2o_world_add_joints(self.h, len(jd), _ptr(jd))
+        if made != len(jd):
+            raise ValueError("joint %d: bad bodies or type" % made)
+        self.n_joints = getattr(self, "n_joints", 0) + len(jd)
+
+    def joints(self):
+        out = np.zeros(getattr(self, "n_joints", 0), self.o.dt["joint_state"])
+        if len(out):
+            self.o.lib.b2o_world_get_joints(self.h, _ptr(out))
+        return out
 
     def step(self, dt=1.0 / 60.0, vel_iters=8, pos_iters=3, steps=1):
         if steps == 1:

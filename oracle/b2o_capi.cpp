@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "b2o_world.h"
+#include "b2o_joints.h"
 
 using namespace b2o;
 
@@ -36,6 +37,25 @@ struct B2oShapeDef {
   real radius;     // circle radius
   real params[36]; // circle: p; polygon: points; box: hx,hy,cx,cy,angle; edge: v1,v2,v0,v3;
                    // exact polygon: vertices [0,16), normals [16,32), centroid [32,34)
+};
+
+// Joint definition: B2distanceJointDef / B2revoluteJointDef (src/joints/b2_distance_joint.rs,
+// src/joints/b2_revolute_joint.rs) with the bodies as creation indices
+struct B2oJointDef {
+  int32_t type;  // 1 distance, 8 revolute (B2jointType)
+  int32_t body_a, body_b;
+  int32_t collide_connected;
+  real local_anchor_a[2], local_anchor_b[2];
+  real length, min_length, max_length, stiffness, damping;
+  real reference_angle, lower_angle, upper_angle, max_motor_torque, motor_speed;
+  int32_t enable_limit, enable_motor;
+  int32_t pad[2];
+};
+// Joint state: the warm-start impulses (distance: impulse, lower, upper; revolute: impulse.x,
+// impulse.y, motor, lower, upper)
+struct B2oJointState {
+  real impulse[5];
+  real pad[3];
 };
 
 struct B2oFixtureDef {
@@ -313,6 +333,63 @@ int b2o_world_add(void* wp, int n, const B2oBodyDef* bodies, int nf, const int32
     if (proxy_ids) proxy_ids[k] = fx->proxy_id;
   }
   return status;
+}
+
+// B2world::create_joint (src/private/dynamics/b2_world.rs:156-275) with b2_distance_joint_new
+// (src/private/dynamics/joints/b2_distance_joint.rs:44-78) / B2revoluteJoint::new
+// (src/joints/b2_revolute_joint.rs:253-287); returns the number created
+int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
+  World* w = static_cast<World*>(wp);
+  const int nb = int(w->bodies.size());
+  for (int i = 0; i < n; ++i) {
+    const B2oJointDef& d = defs[i];
+    if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) return i;
+    if (d.type != kDistanceJoint && d.type != kRevoluteJoint) return i;
+    Joint j;
+    j.type = d.type;
+    j.a = w->bodies[d.body_a];
+    j.b = w->bodies[d.body_b];
+    j.collide_connected = d.collide_connected != 0;
+    j.local_anchor_a = Vec2(d.local_anchor_a[0], d.local_anchor_a[1]);
+    j.local_anchor_b = Vec2(d.local_anchor_b[0], d.local_anchor_b[1]);
+    if (d.type == kDistanceJoint) {
+      j.length = bmax(d.length, kLinearSlop);
+      j.min_length = bmax(d.min_length, kLinearSlop);
+      j.max_length = bmax(d.max_length, j.min_length);
+      j.stiffness = d.stiffness;
+      j.damping = d.damping;
+    } else {
+      j.reference_angle = d.reference_angle;
+      j.lower_angle = d.lower_angle;
+      j.upper_angle = d.upper_angle;
+      j.max_motor_torque = d.max_motor_torque;
+      j.motor_speed = d.motor_speed;
+      j.enable_limit = d.enable_limit != 0;
+      j.enable_motor = d.enable_motor != 0;
+    }
+    w->create_joint(j);
+  }
+  return n;
+}
+
+void b2o_world_get_joints(void* wp, B2oJointState* out) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->joints.size(); ++i) {
+    const Joint* j = w->joints[i];
+    B2oJointState& o = out[i];
+    memset(&o, 0, sizeof(o));
+    if (j->type == kDistanceJoint) {
+      o.impulse[0] = j->impulse;
+      o.impulse[1] = j->lower_impulse;
+      o.impulse[2] = j->upper_impulse;
+    } else {
+      o.impulse[0] = j->impulse2.x;
+      o.impulse[1] = j->impulse2.y;
+      o.impulse[2] = j->motor_impulse;
+      o.impulse[3] = j->lower_impulse;
+      o.impulse[4] = j->upper_impulse;
+    }
+  }
 }
 
 void b2o_world_step(void* wp, real dt, int vi, int pi) { static_cast<World*>(wp)->step(dt, vi, pi); }

@@ -1,5 +1,6 @@
 // CPU oracle, part 2 implementation. TEST INFRASTRUCTURE ONLY (see b2o_core.h).
 #include "b2o_world.h"
+#include "b2o_joints.h"
 
 #include <chrono>
 
@@ -256,6 +257,7 @@ void Body::set_awake(bool flag) {
 
 World::~World() {
   for (Contact* c = contact_list; c;) { Contact* n = c->next; delete c; c = n; }
+  for (Joint* j : joints) delete j;
   for (Fixture* f : fixtures) delete f;
   for (Body* b : bodies) delete b;
 }
@@ -347,10 +349,38 @@ Fixture* World::create_fixture(Body* b, const Shape& s, real density, real frict
   return f;
 }
 
+// ---- joints: src/private/dynamics/b2_world.rs:156-275 ----------------------------------------
+Joint* World::create_joint(const Joint& def) {
+  Joint* j = new Joint(def);
+  j->index = int(joints.size());
+  joints.push_back(j);
+  // the bodies' joint lists (push_front)
+  j->edge_a.joint = j;
+  j->edge_a.other = j->b;
+  j->edge_a.prev = nullptr;
+  j->edge_a.next = j->a->joint_list;
+  if (j->a->joint_list) j->a->joint_list->prev = &j->edge_a;
+  j->a->joint_list = &j->edge_a;
+  j->edge_b.joint = j;
+  j->edge_b.other = j->a;
+  j->edge_b.prev = nullptr;
+  j->edge_b.next = j->b->joint_list;
+  if (j->b->joint_list) j->b->joint_list->prev = &j->edge_b;
+  j->b->joint_list = &j->edge_b;
+  // a joint that prevents collisions flags the contacts between its bodies for filtering
+  if (!j->collide_connected)
+    for (ContactEdge* e = j->b->contact_list; e; e = e->next)
+      if (e->other == j->a) e->contact->filter = true;
+  // (creating a joint does not wake the bodies)
+  return j;
+}
+
 // ---- contact manager ----------------------------------------------------------------------
 static bool should_collide_bodies(const Body* b, const Body* other) {
-  // src/private/dynamics/b2_body.rs:391-416 (no joints in this restatement)
+  // src/private/dynamics/b2_body.rs:391-416
   if (b->type != kDynamic && other->type != kDynamic) return false;
+  for (const JointEdge* je = b->joint_list; je; je = je->next)
+    if (je->other == other && !je->joint->collide_connected) return false;
   return true;
 }
 // src/private/dynamics/b2_world_callbacks.rs:6-18
@@ -442,6 +472,14 @@ void World::collide() {
     Fixture* fb = c->fb;
     Body* ba = fa->body;
     Body* bb = fb->body;
+    // flagged for filtering (src/private/dynamics/b2_contact_manager.rs:100-127)
+    if (c->filter) {
+      if (!should_collide_bodies(bb, ba) || !should_collide_fixtures(fa, fb)) {
+        to_destroy.push_back(c);
+        continue;
+      }
+      c->filter = false;
+    }
     bool activeA = ba->awake && ba->type != kStatic;
     bool activeB = bb->awake && bb->type != kStatic;
     if (!activeA && !activeB) continue;
@@ -893,7 +931,12 @@ bool solve_island(World* w, IslandIO& island, const TimeStep& step) {
   cs.init(scs, sbodies);
   cs.initialize_velocity_constraints(scs, positions.data(), velocities.data());
   if (step.warm_starting) cs.warm_start(velocities.data());
-  for (int i = 0; i < step.velocity_iterations; ++i) cs.solve_velocity_constraints(velocities.data());
+  const JointSolverData jd{step, positions.data(), velocities.data()};
+  for (Joint* j : island.joints) joint_init_velocity(j, jd);
+  for (int i = 0; i < step.velocity_iterations; ++i) {
+    for (Joint* j : island.joints) joint_solve_velocity(j, jd);
+    cs.solve_velocity_constraints(velocities.data());
+  }
   cs.store_impulses(scs);
   for (size_t i = 0; i < nb; ++i) {
     Vec2 c = positions[i].c;
@@ -920,7 +963,12 @@ bool solve_island(World* w, IslandIO& island, const TimeStep& step) {
   bool position_solved = false;
   for (int i = 0; i < step.position_iterations; ++i) {
     bool contacts_okay = cs.solve_position_constraints(positions.data());
-    if (contacts_okay) { position_solved = true; break; }
+    bool joints_okay = true;
+    for (Joint* j : island.joints) {
+      const bool joint_okay = joint_solve_position(j, jd);
+      joints_okay = joints_okay && joint_okay;
+    }
+    if (contacts_okay && joints_okay) { position_solved = true; break; }
   }
   for (size_t i = 0; i < nb; ++i) {
     Body* b = island.bodies[i];
@@ -954,6 +1002,7 @@ bool solve_island(World* w, IslandIO& island, const TimeStep& step) {
 void World::solve(const TimeStep& step) {
   for (Body* b : bodies) b->island_flag = false;
   for (Contact* c = contact_list; c; c = c->next) c->island_flag = false;
+  for (Joint* j : joints) j->island_flag = false;
   IslandIO island;
   std::vector<Body*> stack;
   stack.reserve(bodies.size());
@@ -964,6 +1013,7 @@ void World::solve(const TimeStep& step) {
     if (seed->type == kStatic) continue;
     island.bodies.clear();
     island.contacts.clear();
+    island.joints.clear();
     stack.clear();
     stack.push_back(seed);
     seed->island_flag = true;
@@ -982,6 +1032,17 @@ void World::solve(const TimeStep& step) {
         island.contacts.push_back(c);
         c->island_flag = true;
         Body* other = ce->other;
+        if (other->island_flag) continue;
+        stack.push_back(other);
+        other->island_flag = true;
+      }
+      for (JointEdge* je = b->joint_list; je; je = je->next) {
+        Joint* j = je->joint;
+        if (j->island_flag) continue;
+        Body* other = je->other;
+        if (!other->enabled) continue;
+        island.joints.push_back(j);
+        j->island_flag = true;
         if (other->island_flag) continue;
         stack.push_back(other);
         other->island_flag = true;

@@ -112,6 +112,16 @@ struct ContactEdge {
 
 struct Filter { uint16_t category = 1, mask = 0xFFFF; int16_t group = 0; };
 
+// Joints: B2jointType values (src/b2_joint.rs:46-58) of the two restated kinds
+enum JointType { kDistanceJoint = 1, kRevoluteJoint = 8 };
+struct Joint;
+// src/b2_joint.rs B2jointEdge; a body's list is push_front / newest-first
+struct JointEdge {
+  Body* other = nullptr;
+  Joint* joint = nullptr;
+  JointEdge *prev = nullptr, *next = nullptr;
+};
+
 struct Fixture {
   int index = -1;
   Body* body = nullptr;
@@ -139,6 +149,7 @@ struct Body {
   int island_index = -1;
   std::vector<Fixture*> fixtures;  // newest first (src/private/dynamics/b2_body.rs:183)
   ContactEdge* contact_list = nullptr;
+  JointEdge* joint_list = nullptr;
   void synchronize_transform() {
     xf.q.set(sweep.a);
     xf.p = sweep.c - mul(xf.q, sweep.local_center);
@@ -147,12 +158,37 @@ struct Body {
 };
 
 struct Contact {
-  bool island_flag = false, touching = false, enabled = true;
+  bool island_flag = false, touching = false, enabled = true, filter = false;
   Fixture *fa = nullptr, *fb = nullptr;
   Manifold manifold;
   real friction = 0, restitution = 0, threshold = 0, tangent_speed = 0;
   ContactEdge node_a, node_b;
   Contact *prev = nullptr, *next = nullptr;
+};
+
+// src/b2_joint.rs B2joint + the distance / revolute joint state
+// (src/joints/b2_distance_joint.rs, src/joints/b2_revolute_joint.rs)
+struct Joint {
+  int type = kRevoluteJoint;
+  int index = -1;
+  Body *a = nullptr, *b = nullptr;
+  bool collide_connected = false, island_flag = false;
+  JointEdge edge_a, edge_b;
+  Vec2 local_anchor_a, local_anchor_b;
+  // distance
+  real length = 1, min_length = 0, max_length = kMaxFloat, stiffness = 0, damping = 0;
+  real gamma = 0, bias = 0, impulse = 0, lower_impulse = 0, upper_impulse = 0, current_length = 0, soft_mass = 0;
+  Vec2 u;
+  // revolute
+  real reference_angle = 0, lower_angle = 0, upper_angle = 0, max_motor_torque = 0, motor_speed = 0;
+  bool enable_limit = false, enable_motor = false;
+  Vec2 impulse2;
+  real motor_impulse = 0, axial_mass = 0, angle = 0;
+  Mat22 K;
+  // solver temporaries
+  int index_a = 0, index_b = 0;
+  Vec2 local_center_a, local_center_b, rA, rB;
+  real inv_mass_a = 0, inv_mass_b = 0, inv_ia = 0, inv_ib = 0, mass = 0;
 };
 
 // ---- contact solver: src/private/dynamics/b2_contact_solver{,_private}.rs ----------------
@@ -213,6 +249,7 @@ struct World {
   Vec2 gravity;
   std::vector<Body*> bodies;       // creation order; iterated in reverse = push_front list order
   std::vector<Fixture*> fixtures;  // creation order
+  std::vector<Joint*> joints;      // creation order; iterated in reverse = push_front list order
   Contact* contact_list = nullptr;
   int contact_count = 0;
   BroadPhase bp;
@@ -228,6 +265,7 @@ struct World {
   Fixture* create_fixture(Body* b, const Shape& s, real density, real friction, real restitution, real threshold,
                           Filter filter, bool sensor);
   void reset_mass_data(Body* b);
+  Joint* create_joint(const Joint& def);
   void step(real dt, int vel_iters, int pos_iters);
   // contact manager: src/private/dynamics/b2_contact_manager.rs
   void collide();
@@ -245,6 +283,7 @@ struct IslandIO {
   // bodies of the island, in island-index order
   std::vector<Body*> bodies;
   std::vector<Contact*> contacts;
+  std::vector<Joint*> joints;
 };
 bool solve_island(World* w, IslandIO& island, const TimeStep& step);
 
