@@ -123,6 +123,8 @@ def _load():
         "b2g_world_get_contact_colours": ([P, I, P, P], I),
         "b2g_world_set_contacts": ([P, I, P, P, P], I),
         "b2g_world_get_destroyed_contacts": ([P, I, P, P, P], I),
+        "b2g_world_add_joints": ([P, I, P], I),
+        "b2g_world_get_joint_impulses": ([P, P, I], I),
         "b2g_world_get_solver_state": ([P, P], I),
         "b2g_world_set_solver_state": ([P, P], I),
     }
@@ -200,6 +202,23 @@ class World:
         self._fixture_defs.append(fd.copy())
         self._n_bodies += len(bd)
         return proxies
+
+    def add_joints(self, joint_defs):
+        """Distance and revolute joints (JOINT_DEF; bodies are creation indices)."""
+        jd = _as(joint_defs, JOINT_DEF)
+        _check(_lib.b2g_world_add_joints(self._h, len(jd), _ptr(jd)))
+        self._n_joints = getattr(self, "_n_joints", 0) + len(jd)
+
+    @property
+    def joint_count(self):
+        return getattr(self, "_n_joints", 0)
+
+    def joint_impulses(self):
+        """Warm-start impulses per joint: (n, 5) (distance: impulse, lower, upper; revolute:
+        impulse.x, impulse.y, motor, lower, upper)."""
+        out = np.zeros((max(self.joint_count, 1), 5), np.float32)
+        _check(_lib.b2g_world_get_joint_impulses(self._h, _ptr(out), self.joint_count))
+        return out[:self.joint_count]
 
     def definitions(self):
         """(body_defs, fixture_body, fixture_defs) handed to add(), concatenated (fixture_body

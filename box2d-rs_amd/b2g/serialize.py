@@ -209,6 +209,8 @@ def world_to_json(world, checkpoint=False):
     b2g_checkpoint extension is added."""
     if world._batched:
         raise B2gError(ERR_UNSUPPORTED, "serde world: a batch of worlds has no single-world record")
+    if world.joint_count:
+        raise B2gError(ERR_UNSUPPORTED, "serde world: joints are not written (their serde records are not restated)")
     bd, fbody, fd = world.definitions()
     return defs_to_json(world.gravity, bd, fbody, world.shapes(), fd, states=world.body_states(),
                         extension=(lambda per_body: _checkpoint(world, fbody, per_body)) if checkpoint else None)

@@ -297,8 +297,62 @@ static int download_state(World* w) {
 
 static bool host_is_static(const World* w, int f) { return w->host_type[w->h_fbody[f]] == B2G_STATIC; }
 
+// Joint rows, the joint colouring and the per-body lists of bodies a joint keeps from colliding.
+// Impulses of joints already on the device are kept; new joints start at zero.
+int upload_joints(World* w) {
+  const int nj = w->nj, nb = w->nb;
+  int st;
+  std::vector<int2> jb(nj);
+  std::vector<int4> jt(nj);
+  std::vector<float4> d0(nj), d1(nj), d2(nj);
+  std::vector<std::vector<int>> nc(nb);
+  for (int i = 0; i < nj; ++i) {
+    const b2g_joint_def& d = w->h_joints[i];
+    jb[i] = make_int2(d.body_a, d.body_b);
+    jt[i] = make_int4(d.type, (d.collide_connected ? 1 : 0) | (d.enable_limit ? 2 : 0) | (d.enable_motor ? 4 : 0), 0, 0);
+    d0[i] = make_float4(d.local_anchor_a[0], d.local_anchor_a[1], d.local_anchor_b[0], d.local_anchor_b[1]);
+    if (d.type == B2G_JOINT_DISTANCE) {
+      d1[i] = make_float4(d.length, d.min_length, d.max_length, d.stiffness);
+      d2[i] = make_float4(d.damping, 0.0f, 0.0f, 0.0f);
+    } else {
+      d1[i] = make_float4(d.reference_angle, d.lower_angle, d.upper_angle, d.max_motor_torque);
+      d2[i] = make_float4(d.motor_speed, 0.0f, 0.0f, 0.0f);
+    }
+    if (!d.collide_connected) {
+      nc[d.body_a].push_back(d.body_b);
+      nc[d.body_b].push_back(d.body_a);
+    }
+  }
+  std::vector<int> nc_off(nb + 1, 0), nc_list, nc_flag(nb, 0);
+  for (int b = 0; b < nb; ++b) {
+    nc_off[b + 1] = nc_off[b] + int(nc[b].size());
+    nc_flag[b] = nc[b].empty() ? 0 : 1;
+    nc_list.insert(nc_list.end(), nc[b].begin(), nc[b].end());
+  }
+  JointArrays& J = w->j;
+  // impulses: grow keeping the uploaded prefix
+  const int had = w->dev_nj;
+  if ((st = J.imp.reserve(std::max(nj, 1), true, had)) || (st = J.imp2.reserve(std::max(nj, 1), true, had)) ||
+      (st = J.r.reserve(std::max(nj, 1))) || (st = J.k.reserve(std::max(nj, 1))) || (st = J.k2.reserve(std::max(nj, 1))))
+    return st;
+  if (nj > had) {
+    B2G_HIP(hipMemsetAsync(J.imp.p + had, 0, sizeof(float4) * (nj - had), w->stream));
+    B2G_HIP(hipMemsetAsync(J.imp2.p + had, 0, sizeof(float) * (nj - had), w->stream));
+  }
+  if ((st = up(J.bodies, jb, w->stream)) || (st = up(J.type, jt, w->stream)) || (st = up(J.d0, d0, w->stream)) ||
+      (st = up(J.d1, d1, w->stream)) || (st = up(J.d2, d2, w->stream)) || (st = up(J.order, w->h_joint_order, w->stream)) ||
+      (st = up(J.nc_off, nc_off, w->stream)) || (st = up(J.nc_list, nc_list, w->stream)) ||
+      (st = up(J.nc_flag, nc_flag, w->stream)))
+    return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  w->dev_nj = nj;
+  w->n_nocollide = int(nc_list.size());
+  w->joints_dirty = false;
+  return B2G_OK;
+}
+
 int upload_world(World* w) {
-  if (!w->dirty) return B2G_OK;
+  if (!w->dirty) return w->joints_dirty ? upload_joints(w) : B2G_OK;
   int st;
   // arrays may be reallocated below (with copies on the default stream): let this world's
   // stream finish first
@@ -376,6 +430,7 @@ int upload_world(World* w) {
   w->dev_nb = w->nb;
   w->dev_np = w->np;
   w->dirty = false;
+  if (w->nj > 0 || w->joints_dirty) return upload_joints(w);
   return B2G_OK;
 }
 
@@ -752,6 +807,86 @@ int b2g_world_get_contacts(b2g_world* h, int32_t cap, int32_t* fixtures_ab, int3
   return B2G_OK;
 }
 
+int b2g_world_add_joints(b2g_world* h, int32_t n, const b2g_joint_def* defs) {
+  if (!h || n < 0 || (n > 0 && !defs)) { set_error("b2g_world_add_joints: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  World* w = &h->w;
+  if (w->tape.active) { set_error("b2g_world_add_joints: a tape is being recorded"); return B2G_ERR_UNSUPPORTED; }
+  for (int i = 0; i < n; ++i) {
+    const b2g_joint_def& d = defs[i];
+    if (d.type != B2G_JOINT_DISTANCE && d.type != B2G_JOINT_REVOLUTE) {
+      set_error("b2g_world_add_joints: joint " + std::to_string(i) + ": only distance and revolute joints");
+      return B2G_ERR_UNSUPPORTED;
+    }
+    if (d.body_a < 0 || d.body_a >= w->nb || d.body_b < 0 || d.body_b >= w->nb || d.body_a == d.body_b ||
+        w->h_bworld[d.body_a] != w->h_bworld[d.body_b]) {
+      set_error("b2g_world_add_joints: joint " + std::to_string(i) + ": bad bodies");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    b2g_joint_def d = defs[i];
+    if (d.type == B2G_JOINT_DISTANCE) {
+      // b2_distance_joint_new (src/private/dynamics/joints/b2_distance_joint.rs:44-78)
+      d.length = std::max(d.length, kLinearSlop);
+      d.min_length = std::max(d.min_length, kLinearSlop);
+      d.max_length = std::max(d.max_length, d.min_length);
+    }
+    w->h_joints.push_back(d);
+  }
+  w->nj = int(w->h_joints.size());
+  // colours: first fit over the dynamic bodies, joints in creation order
+  std::vector<unsigned long long> used(w->nb, 0ull);
+  std::vector<int> colour(w->nj);
+  int ncol = 0;
+  for (int i = 0; i < w->nj; ++i) {
+    const b2g_joint_def& d = w->h_joints[i];
+    const bool da = w->host_type[d.body_a] == B2G_DYNAMIC, db = w->host_type[d.body_b] == B2G_DYNAMIC;
+    const unsigned long long busy = (da ? used[d.body_a] : 0ull) | (db ? used[d.body_b] : 0ull);
+    int c = 0;
+    while (c < 64 && (busy >> c) & 1ull) ++c;
+    if (c == 64) { set_error("b2g_world_add_joints: a body has more than 63 joints"); return B2G_ERR_CAPACITY; }
+    colour[i] = c;
+    if (da) used[d.body_a] |= 1ull << c;
+    if (db) used[d.body_b] |= 1ull << c;
+    ncol = std::max(ncol, c + 1);
+  }
+  w->h_joint_colour_off.assign(ncol + 1, 0);
+  for (int i = 0; i < w->nj; ++i) ++w->h_joint_colour_off[colour[i] + 1];
+  for (int c = 0; c < ncol; ++c) w->h_joint_colour_off[c + 1] += w->h_joint_colour_off[c];
+  std::vector<int> cursor(w->h_joint_colour_off.begin(), w->h_joint_colour_off.end() - 1);
+  w->h_joint_order.assign(w->nj, 0);
+  for (int i = 0; i < w->nj; ++i) w->h_joint_order[cursor[colour[i]]++] = i;
+  w->joints_dirty = true;
+  // the next collide works on a contact set found with the joints' collision filter
+  // (create_joint flags the bodies' contacts for filtering, src/private/dynamics/b2_world.rs:257-271)
+  w->need_broadphase = true;
+  return B2G_OK;
+}
+
+int b2g_world_get_joint_impulses(b2g_world* h, float* out, int32_t n) {
+  if (!h || n != h->w.nj || (n > 0 && !out)) {
+    set_error("b2g_world_get_joint_impulses: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  std::vector<float4> imp;
+  std::vector<float> imp2;
+  if ((st = down(imp, w->j.imp, n, w->stream)) || (st = down(imp2, w->j.imp2, n, w->stream))) return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  for (int i = 0; i < n; ++i) {
+    float* o = out + 5 * i;
+    o[0] = imp[i].x;
+    o[1] = imp[i].y;
+    o[2] = imp[i].z;
+    o[3] = imp[i].w;
+    o[4] = imp2[i];
+    if (w->h_joints[i].type == B2G_JOINT_DISTANCE) o[3] = o[4] = 0.0f;
+  }
+  return B2G_OK;
+}
+
 int b2g_world_get_shapes(b2g_world* h, b2g_shape_def* out, int32_t n) {
   if (!h || !out || n != h->w.np) { set_error("b2g_world_get_shapes: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;
@@ -1080,6 +1215,10 @@ int b2g_world_kernel_time(b2g_world* h, double* ms, int64_t* launches, int64_t* 
 int b2g_tape_begin(b2g_world* h, int32_t checkpoint_every) {
   if (!h || checkpoint_every < 1) { set_error("b2g_tape_begin: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;
+  if (w->nj > 0) {
+    set_error("b2g_tape_begin: the adjoint does not hold joints");
+    return B2G_ERR_UNSUPPORTED;
+  }
   if (w->sleep_possible()) {
     // the frozen model replays every body every step: no sleeping bodies on a tape
     set_error("b2g_tape_begin: the adjoint needs allow_sleep off and every body awake");

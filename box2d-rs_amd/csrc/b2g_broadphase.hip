@@ -164,11 +164,16 @@ struct LeafInputs {
   const int* btype;
   const int2* filter;
   const int4* fshape;
+  const int* nc_flag;  // per body: joined by a joint that forbids collision (null: no such joint)
 };
+// leaf record: proxy, body, filter bits, group | shape type << 16 | body type << 24 | 1 << 30
+// when a joint keeps the body from colliding with some other body
 __device__ inline int4 leaf_record(int p, const LeafInputs& li) {
   const int b = li.fbody[p];
   const int2 f = li.filter[p];
-  return make_int4(p, b, f.x, (f.y & 0xffff) | ((li.fshape[p].x & 0xff) << 16) | ((li.btype[b] & 0xff) << 24));
+  const int nc = li.nc_flag ? li.nc_flag[b] : 0;
+  return make_int4(p, b, f.x,
+                   (f.y & 0xffff) | ((li.fshape[p].x & 0xff) << 16) | ((li.btype[b] & 0x3f) << 24) | (nc << 30));
 }
 
 __global__ void __launch_bounds__(kBlock) k_seg_reduce(Levels lv, int L0, const unsigned* __restrict__ vals,
@@ -307,18 +312,30 @@ __global__ void __launch_bounds__(kBlock) k_bvh4_collapse(int n_internal, const 
 }
 
 // ---- pair search --------------------------------------------------------------------------
+// B2body::should_collide's joint rule (src/private/dynamics/b2_body.rs:398-412): bodies joined by a
+// joint with collide_connected off do not collide
+__device__ inline bool joined_no_collide(int ba, int bb, const int* __restrict__ nc_off,
+                                         const int* __restrict__ nc_list) {
+  for (int i = nc_off[ba], e = nc_off[ba + 1]; i < e; ++i)
+    if (nc_list[i] == bb) return true;
+  return false;
+}
+
 struct PairFilter {
   const int* __restrict__ fbody;
   const int* __restrict__ btype;
   const int2* __restrict__ filter;
   const int4* __restrict__ fshape;
   const int* __restrict__ world;
+  const int* __restrict__ nc_off;   // null: no joint forbids a collision
+  const int* __restrict__ nc_list;
   __device__ bool ok(int q, int p) const {
     int bq = fbody[q], bp = fbody[p];
     if (bq == bp) return false;
     if (world[q] != world[p]) return false;  // batched worlds never interact
     // B2body::should_collide: at least one body dynamic (src/private/dynamics/b2_body.rs:391-416)
     if (btype[bq] != 2 && btype[bp] != 2) return false;
+    if (nc_off && joined_no_collide(bq, bp, nc_off, nc_list)) return false;
     // default contact filter (src/private/dynamics/b2_world_callbacks.rs:6-18)
     int2 fq = filter[q], fp = filter[p];
     if (fq.y == fp.y && fq.y != 0) {
@@ -342,9 +359,11 @@ static constexpr int kSlots = 64;
 // PairFilter::ok on two leaf records of one world (the node world ranges have already matched
 // the worlds): same body, at least one dynamic body, the default contact filter, and a contact
 // class for the shape pair.
-__device__ inline bool leaf_pair_ok(int4 a, int4 b) {
+__device__ inline bool leaf_pair_ok(int4 a, int4 b, const int* __restrict__ nc_off, const int* __restrict__ nc_list) {
   if (a.y == b.y) return false;
-  if (((a.w >> 24) & 0xff) != 2 && ((b.w >> 24) & 0xff) != 2) return false;
+  if (((a.w >> 24) & 0x3f) != 2 && ((b.w >> 24) & 0x3f) != 2) return false;
+  if (((a.w & b.w) >> 30) & 1)
+    if (joined_no_collide(a.y, b.y, nc_off, nc_list)) return false;
   const int ga = static_cast<short>(a.w & 0xffff), gb = static_cast<short>(b.w & 0xffff);
   if (ga == gb && ga != 0) {
     if (ga < 0) return false;
@@ -431,7 +450,7 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int nq, const float4* _
       int p = 0;
       if (hit && ch < 0) {
         const int4 pr = tv.leafrec[~ch];
-        ok = leaf_pair_ok(qr, pr);
+        ok = leaf_pair_ok(qr, pr, pf.nc_off, pf.nc_list);
         p = pr.x;
       }
       emit(ok, p);
@@ -536,7 +555,8 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   while ((1 << wbits) < w->n_worlds) ++wbits;
   const int mbits = std::min(12, (32 - wbits) / 2);
   const int key_bits = wbits + 2 * mbits;
-  const LeafInputs li{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p};
+  const bool nc = w->n_nocollide > 0;
+  const LeafInputs li{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, nc ? w->j.nc_flag.p : nullptr};
   if (nt > 0) {
     int gb = std::min(blocks_for(nt, kBlock), 1024);
     hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat, bnd);
@@ -553,7 +573,8 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
                          bp.nodes.p, bp.nodes4.p);
     }
   }
-  PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p};
+  PairFilter pf{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, w->f.world.p, nc ? w->j.nc_off.p : nullptr,
+                nc ? w->j.nc_list.p : nullptr};
   TreeView tv{nt, bp.nodes4.p, bp.leafrec.p, w->f.loose_off.p, w->f.loose.p, np - nt, wbits > 0};
   // tree proxies walk in Morton order; proxies outside the tree are static and emit nothing
   // (the walk clears their counts; with no tree proxies there is no walk)

@@ -221,6 +221,25 @@ struct Tape {
   int nb = 0;
 };
 
+// Joints (distance, revolute): definitions, persistent impulses and per-step solver rows,
+// indexed by joint creation order.
+struct JointArrays {
+  DevArray<int2> bodies;   // body A, body B
+  DevArray<int4> type;     // B2G_JOINT_*, flags (bit0 collide_connected, bit1 limit, bit2 motor)
+  DevArray<float4> d0;     // local anchor A, local anchor B
+  DevArray<float4> d1;     // distance: length, min, max, stiffness; revolute: reference, lower, upper, max torque
+  DevArray<float4> d2;     // distance: damping; revolute: motor speed
+  DevArray<float4> imp;    // distance: impulse, lower, upper; revolute: impulse.x, .y, motor, lower
+  DevArray<float> imp2;    // revolute: upper impulse
+  DevArray<float4> r;      // rA, rB of the step
+  DevArray<float4> k;      // distance: u, mass, soft mass; revolute: K.ex.x, K.ex.y (= K.ey.x), K.ey.y, axial mass
+  DevArray<float4> k2;     // distance: gamma, bias, current length; revolute: angle
+  DevArray<int> order;     // joints in colour order
+  DevArray<int> nc_off;    // per body: offsets into nc_list (nb + 1)
+  DevArray<int> nc_list;   // per body: bodies it is joined to by a joint that forbids their collision
+  DevArray<int> nc_flag;   // per body: 1 when its nc_list is not empty
+};
+
 struct World {
   // settings
   V2 gravity;
@@ -291,6 +310,13 @@ struct World {
   hipEvent_t ev[8] = {};
   KernelTimer kt;
   Tape tape;
+  // joints
+  int nj = 0, dev_nj = 0;
+  int n_nocollide = 0;  // entries of the per-body no-collide lists
+  std::vector<b2g_joint_def> h_joints;
+  std::vector<int> h_joint_order, h_joint_colour_off;
+  bool joints_dirty = false;
+  JointArrays j;
 };
 
 // adjoint tape hooks of b2g_world_step (b2g_adjoint.hip)
@@ -361,6 +387,34 @@ struct IslandView {
   const int* isl_awake;  // per root; null: every island is awake
   __device__ bool asleep(int b) const { return isl_awake && !isl_awake[parent[b]]; }
 };
+
+// Kernel arguments of the joint solve (b2g_joints.hip).
+struct JointArgs {
+  const int* order;
+  const int2* jb;
+  const int4* jt;
+  const float4 *jd0, *jd1, *jd2;
+  float4* jimp;
+  float* jimp2;
+  float4 *jr, *jk, *jk2;
+  const int* btype;
+  float4* pos;
+  float4* vel;
+  const float4* mass;
+  IslandView iv;
+  int* violated;
+  const int* done;
+  float dt, inv_dt, dt_ratio;
+  int warm;
+};
+
+// joint solve (b2g_joints.hip): phase 0 init + warm start, 1 velocity, 2 position; one launch
+// per joint colour
+void launch_joint_colours(World* w, int phase, const JointArgs& a);
+int upload_joints(World* w);
+// joints join islands (b2g_solver.hip)
+void launch_joint_union(World* w);
+
 
 // body transform record (p.x, p.y, sin, cos) -> Xf
 __device__ inline Xf xf_of(float4 t) {

@@ -1,0 +1,392 @@
+// b2g joints in the coloured island solve: distance and revolute joints.
+//
+// The reference solves an island's joints in island order before its contacts in every velocity
+// iteration, and after its contacts in every position iteration
+// (src/private/dynamics/b2_island_private.rs:196-275). Here the joints of a world are coloured
+// once, when they are created (no two joints of a colour share a dynamic body), and each colour is
+// one launch: joint colours, then contact colours, per velocity iteration; contact colours, then
+// joint colours, per position iteration. That is the sequence of one sequential order (joints by
+// colour, then contacts by colour). The per-joint arithmetic follows the reference's operation
+// order:
+//   distance joint  src/private/dynamics/joints/b2_distance_joint.rs:80-320
+//   revolute joint  src/private/dynamics/joints/b2_revolute_joint.rs:60-330
+//   B2Mat22::solve  src/b2_math.rs:278-293
+#include "b2g_internal.h"
+
+namespace b2g {
+
+static constexpr int kBlock = 256;
+constexpr float kAngularSlop = 2.0f / 180.0f * kPi;           // src/b2_common.rs:43
+constexpr float kMaxAngularCorrection = 8.0f / 180.0f * kPi;  // src/b2_common.rs:64
+
+__device__ inline V2 mat22_solve(float a11, float a21, float a12, float a22, V2 b) {
+  float det = a11 * a22 - a12 * a21;
+  if (det != 0.0f) det = 1.0f / det;
+  return v2(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
+}
+
+// a joint's two bodies as the solver holds them
+struct JointBodies {
+  int a, b;
+  bool dyn_a, dyn_b;
+  float mA, iA, mB, iB;
+  V2 lcA, lcB;
+};
+
+__device__ inline bool joint_bodies(int j, const JointArgs& a, JointBodies* jb, int* root) {
+  const int2 b = a.jb[j];
+  jb->a = b.x;
+  jb->b = b.y;
+  const int ta = a.btype[b.x], tb = a.btype[b.y];
+  jb->dyn_a = ta == 2;
+  jb->dyn_b = tb == 2;
+  const int anchor = ta != 0 ? b.x : b.y;  // a body of the island (not static)
+  if (a.iv.asleep(anchor)) return false;
+  *root = a.iv.parent[anchor];
+  const float4 ma = a.mass[b.x], mb = a.mass[b.y];
+  jb->mA = ma.x;
+  jb->iA = ma.y;
+  jb->lcA = v2(ma.z, ma.w);
+  jb->mB = mb.x;
+  jb->iB = mb.y;
+  jb->lcB = v2(mb.z, mb.w);
+  return true;
+}
+
+// ---- init: B2distanceJoint / B2revoluteJoint::init_velocity_constraints (with warm start) ----
+__device__ void joint_init_one(int j, const JointArgs& a) {
+  JointBodies B;
+  int root;
+  if (!joint_bodies(j, a, &B, &root)) return;
+  const int type = a.jt[j].x;
+  const float4 d0 = a.jd0[j], d1 = a.jd1[j], d2 = a.jd2[j];
+  const V2 laA = v2(d0.x, d0.y), laB = v2(d0.z, d0.w);
+  const float4 PA = a.pos[B.a], PB = a.pos[B.b];
+  const float4 VA = a.vel[B.a], VB = a.vel[B.b];
+  V2 vA = v2(VA.x, VA.y), vB = v2(VB.x, VB.y);
+  float wA = VA.z, wB = VB.z;
+  const Rot qA = rot(PA.z), qB = rot(PB.z);
+  const V2 rA = mul(qA, laA - B.lcA);
+  const V2 rB = mul(qB, laB - B.lcB);
+  float4 imp = a.jimp[j];
+  if (type == B2G_JOINT_DISTANCE) {
+    const float length = d1.x, min_length = d1.y, max_length = d1.z, stiffness = d1.w, damping = d2.x;
+    V2 u = ((v2(PB.x, PB.y) + rB) - v2(PA.x, PA.y)) - rA;
+    const float current = len(u);
+    float mass = 0.0f;
+    if (current > kLinearSlop) {
+      u = (1.0f / current) * u;
+    } else {
+      u = v2(0.0f, 0.0f);
+      imp = make_float4(0.0f, 0.0f, 0.0f, imp.w);
+    }
+    const float crAu = cross(rA, u), crBu = cross(rB, u);
+    float inv_mass = ((B.mA + B.iA * crAu * crAu) + B.mB) + B.iB * crBu * crBu;
+    mass = inv_mass != 0.0f ? 1.0f / inv_mass : 0.0f;
+    float gamma, bias, soft_mass;
+    if (stiffness > 0.0f && min_length < max_length) {
+      const float C = current - length;
+      const float h = a.dt;
+      gamma = h * (damping + h * stiffness);
+      gamma = gamma != 0.0f ? 1.0f / gamma : 0.0f;
+      bias = C * h * stiffness * gamma;
+      inv_mass += gamma;
+      soft_mass = inv_mass != 0.0f ? 1.0f / inv_mass : 0.0f;
+    } else {
+      gamma = 0.0f;
+      bias = 0.0f;
+      soft_mass = mass;
+    }
+    if (a.warm) {
+      imp.x *= a.dt_ratio;
+      imp.y *= a.dt_ratio;
+      imp.z *= a.dt_ratio;
+      const V2 P = ((imp.x + imp.y) - imp.z) * u;
+      vA = vA - B.mA * P;
+      wA -= B.iA * cross(rA, P);
+      vB = vB + B.mB * P;
+      wB += B.iB * cross(rB, P);
+    } else {
+      imp.x = 0.0f;
+    }
+    a.jk[j] = make_float4(u.x, u.y, mass, soft_mass);
+    a.jk2[j] = make_float4(gamma, bias, current, 0.0f);
+  } else {
+    const float reference_angle = d1.x;
+    const int flags = a.jt[j].y;
+    const bool limit = flags & 2, motor = flags & 4;
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const float kxx = ((mA + mB) + rA.y * rA.y * iA) + rB.y * rB.y * iB;
+    const float kyx = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+    const float kyy = ((mA + mB) + rA.x * rA.x * iA) + rB.x * rB.x * iB;
+    float axial = iA + iB;
+    bool fixed_rotation;
+    if (axial > 0.0f) {
+      axial = 1.0f / axial;
+      fixed_rotation = false;
+    } else {
+      fixed_rotation = true;
+    }
+    const float angle = (PB.z - PA.z) - reference_angle;
+    float upper = a.jimp2[j];
+    if (!limit || fixed_rotation) {
+      imp.w = 0.0f;
+      upper = 0.0f;
+    }
+    if (!motor || fixed_rotation) imp.z = 0.0f;
+    if (a.warm) {
+      imp.x *= a.dt_ratio;
+      imp.y *= a.dt_ratio;
+      imp.z *= a.dt_ratio;
+      imp.w *= a.dt_ratio;
+      upper *= a.dt_ratio;
+      const float axial_impulse = (imp.z + imp.w) - upper;
+      const V2 P = v2(imp.x, imp.y);
+      vA = vA - mA * P;
+      wA -= iA * (cross(rA, P) + axial_impulse);
+      vB = vB + mB * P;
+      wB += iB * (cross(rB, P) + axial_impulse);
+    } else {
+      imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      upper = 0.0f;
+    }
+    a.jimp2[j] = upper;
+    a.jk[j] = make_float4(kxx, kyx, kyy, axial);
+    a.jk2[j] = make_float4(angle, 0.0f, 0.0f, 0.0f);
+  }
+  a.jimp[j] = imp;
+  a.jr[j] = make_float4(rA.x, rA.y, rB.x, rB.y);
+  if (B.dyn_a) a.vel[B.a] = make_float4(vA.x, vA.y, wA, 0.0f);
+  if (B.dyn_b) a.vel[B.b] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// ---- velocity: solve_velocity_constraints ------------------------------------------------------
+__device__ inline void distance_push(float impulse, V2 u, V2 rA, V2 rB, const JointBodies& B, V2& vA, float& wA,
+                                     V2& vB, float& wB) {
+  const V2 P = impulse * u;
+  vA = vA - B.mA * P;
+  wA -= B.iA * cross(rA, P);
+  vB = vB + B.mB * P;
+  wB += B.iB * cross(rB, P);
+}
+
+__device__ void joint_velocity_one(int j, const JointArgs& a) {
+  JointBodies B;
+  int root;
+  if (!joint_bodies(j, a, &B, &root)) return;
+  const int type = a.jt[j].x;
+  const float4 VA = a.vel[B.a], VB = a.vel[B.b];
+  V2 vA = v2(VA.x, VA.y), vB = v2(VB.x, VB.y);
+  float wA = VA.z, wB = VB.z;
+  const float4 r = a.jr[j], k = a.jk[j], k2 = a.jk2[j];
+  const V2 rA = v2(r.x, r.y), rB = v2(r.z, r.w);
+  float4 imp = a.jimp[j];
+  if (type == B2G_JOINT_DISTANCE) {
+    const float4 d1 = a.jd1[j];
+    const float min_length = d1.y, max_length = d1.z, stiffness = d1.w;
+    const V2 u = v2(k.x, k.y);
+    const float mass = k.z, soft_mass = k.w, gamma = k2.x, bias = k2.y, current = k2.z;
+    if (min_length < max_length) {
+      if (stiffness > 0.0f) {
+        const V2 vpA = vA + cross(wA, rA), vpB = vB + cross(wB, rB);
+        const float Cdot = dot(u, vpB - vpA);
+        const float impulse = -soft_mass * ((Cdot + bias) + gamma * imp.x);
+        imp.x += impulse;
+        distance_push(impulse, u, rA, rB, B, vA, wA, vB, wB);
+      }
+      {
+        const float C = current - min_length;
+        const float b = bmax(0.0f, C) * a.inv_dt;
+        const V2 vpA = vA + cross(wA, rA), vpB = vB + cross(wB, rB);
+        const float Cdot = dot(u, vpB - vpA);
+        float impulse = -mass * (Cdot + b);
+        const float old = imp.y;
+        imp.y = bmax(0.0f, imp.y + impulse);
+        impulse = imp.y - old;
+        distance_push(impulse, u, rA, rB, B, vA, wA, vB, wB);
+      }
+      {
+        const float C = max_length - current;
+        const float b = bmax(0.0f, C) * a.inv_dt;
+        const V2 vpA = vA + cross(wA, rA), vpB = vB + cross(wB, rB);
+        const float Cdot = dot(u, vpA - vpB);
+        float impulse = -mass * (Cdot + b);
+        const float old = imp.z;
+        imp.z = bmax(0.0f, imp.z + impulse);
+        impulse = imp.z - old;
+        distance_push(-impulse, u, rA, rB, B, vA, wA, vB, wB);
+      }
+    } else {
+      const V2 vpA = vA + cross(wA, rA), vpB = vB + cross(wB, rB);
+      const float Cdot = dot(u, vpB - vpA);
+      const float impulse = -mass * Cdot;
+      imp.x += impulse;
+      distance_push(impulse, u, rA, rB, B, vA, wA, vB, wB);
+    }
+  } else {
+    const int flags = a.jt[j].y;
+    const bool limit = flags & 2, motor = flags & 4;
+    const float4 d1 = a.jd1[j], d2 = a.jd2[j];
+    const float lower_angle = d1.y, upper_angle = d1.z, max_torque = d1.w, motor_speed = d2.x;
+    const float iA = B.iA, iB = B.iB, mA = B.mA, mB = B.mB;
+    const float axial = k.w, angle = k2.x;
+    const bool fixed_rotation = iA + iB == 0.0f;
+    float upper = a.jimp2[j];
+    if (motor && !fixed_rotation) {
+      const float Cdot = (wB - wA) - motor_speed;
+      float impulse = -axial * Cdot;
+      const float old = imp.z;
+      const float max_impulse = a.dt * max_torque;
+      imp.z = bclamp(imp.z + impulse, -max_impulse, max_impulse);
+      impulse = imp.z - old;
+      wA -= iA * impulse;
+      wB += iB * impulse;
+    }
+    if (limit && !fixed_rotation) {
+      {
+        const float C = angle - lower_angle;
+        const float Cdot = wB - wA;
+        float impulse = -axial * (Cdot + bmax(C, 0.0f) * a.inv_dt);
+        const float old = imp.w;
+        imp.w = bmax(imp.w + impulse, 0.0f);
+        impulse = imp.w - old;
+        wA -= iA * impulse;
+        wB += iB * impulse;
+      }
+      {
+        const float C = upper_angle - angle;
+        const float Cdot = wA - wB;
+        float impulse = -axial * (Cdot + bmax(C, 0.0f) * a.inv_dt);
+        const float old = upper;
+        upper = bmax(upper + impulse, 0.0f);
+        impulse = upper - old;
+        wA += iA * impulse;
+        wB -= iB * impulse;
+      }
+    }
+    {
+      const V2 Cdot = ((vB + cross(wB, rB)) - vA) - cross(wA, rA);
+      const V2 impulse = mat22_solve(k.x, k.y, k.y, k.z, -Cdot);
+      imp.x += impulse.x;
+      imp.y += impulse.y;
+      vA = vA - mA * impulse;
+      wA -= iA * cross(rA, impulse);
+      vB = vB + mB * impulse;
+      wB += iB * cross(rB, impulse);
+    }
+    a.jimp2[j] = upper;
+  }
+  a.jimp[j] = imp;
+  if (B.dyn_a) a.vel[B.a] = make_float4(vA.x, vA.y, wA, 0.0f);
+  if (B.dyn_b) a.vel[B.b] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// ---- position: solve_position_constraints; a joint that is not within tolerance marks its
+// island (the island stops iterating only when its contacts and joints are all within it)
+__device__ void joint_position_one(int j, const JointArgs& a) {
+  JointBodies B;
+  int root;
+  if (!joint_bodies(j, a, &B, &root)) return;
+  if (a.done[root]) return;
+  const int type = a.jt[j].x;
+  const float4 d0 = a.jd0[j], d1 = a.jd1[j];
+  const V2 laA = v2(d0.x, d0.y), laB = v2(d0.z, d0.w);
+  const float4 PA = a.pos[B.a], PB = a.pos[B.b];
+  V2 cA = v2(PA.x, PA.y), cB = v2(PB.x, PB.y);
+  float aA = PA.z, aB = PB.z;
+  bool okay;
+  if (type == B2G_JOINT_DISTANCE) {
+    const float min_length = d1.y, max_length = d1.z;
+    const float mass = a.jk[j].z;
+    const Rot qA = rot(aA), qB = rot(aB);
+    const V2 rA = mul(qA, laA - B.lcA);
+    const V2 rB = mul(qB, laB - B.lcB);
+    V2 u = ((cB + rB) - cA) - rA;
+    const float length = normalize(&u);
+    float C;
+    bool apply = true;
+    if (min_length == max_length) C = length - min_length;
+    else if (length < min_length) C = length - min_length;
+    else if (max_length < length) C = length - max_length;
+    else { C = 0.0f; apply = false; }
+    if (apply) {
+      const float impulse = -mass * C;
+      const V2 P = impulse * u;
+      cA = cA - B.mA * P;
+      aA -= B.iA * cross(rA, P);
+      cB = cB + B.mB * P;
+      aB += B.iB * cross(rB, P);
+    }
+    okay = !apply || fabsf(C) < kLinearSlop;
+    if (!apply) return;
+  } else {
+    const int flags = a.jt[j].y;
+    const bool limit = flags & 2;
+    const float reference_angle = d1.x, lower_angle = d1.y, upper_angle = d1.z;
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const float axial = a.jk[j].w;
+    float angular_error = 0.0f;
+    const bool fixed_rotation = iA + iB == 0.0f;
+    if (limit && !fixed_rotation) {
+      const float angle = (aB - aA) - reference_angle;
+      float C = 0.0f;
+      if (fabsf(upper_angle - lower_angle) < 2.0f * kAngularSlop)
+        C = bclamp(angle - lower_angle, -kMaxAngularCorrection, kMaxAngularCorrection);
+      else if (angle <= lower_angle)
+        C = bclamp((angle - lower_angle) + kAngularSlop, -kMaxAngularCorrection, 0.0f);
+      else if (angle >= upper_angle)
+        C = bclamp((angle - upper_angle) - kAngularSlop, 0.0f, kMaxAngularCorrection);
+      const float limit_impulse = -axial * C;
+      aA -= iA * limit_impulse;
+      aB += iB * limit_impulse;
+      angular_error = fabsf(C);
+    }
+    const Rot qA = rot(aA), qB = rot(aB);
+    const V2 rA = mul(qA, laA - B.lcA);
+    const V2 rB = mul(qB, laB - B.lcB);
+    const V2 C = ((cB + rB) - cA) - rA;
+    const float position_error = len(C);
+    const float kxx = ((mA + mB) + iA * rA.y * rA.y) + iB * rB.y * rB.y;
+    const float kxy = -iA * rA.x * rA.y - iB * rB.x * rB.y;
+    const float kyy = ((mA + mB) + iA * rA.x * rA.x) + iB * rB.x * rB.x;
+    const V2 impulse = -mat22_solve(kxx, kxy, kxy, kyy, C);
+    cA = cA - mA * impulse;
+    aA -= iA * cross(rA, impulse);
+    cB = cB + mB * impulse;
+    aB += iB * cross(rB, impulse);
+    okay = position_error <= kLinearSlop && angular_error <= kAngularSlop;
+  }
+  if (B.dyn_a) a.pos[B.a] = make_float4(cA.x, cA.y, aA, 0.0f);
+  if (B.dyn_b) a.pos[B.b] = make_float4(cB.x, cB.y, aB, 0.0f);
+  if (!okay) a.violated[root] = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_joint_init(int begin, int end, JointArgs a) {
+  const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end) joint_init_one(a.order[i], a);
+}
+__global__ void __launch_bounds__(kBlock) k_joint_velocity(int begin, int end, JointArgs a) {
+  const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end) joint_velocity_one(a.order[i], a);
+}
+__global__ void __launch_bounds__(kBlock) k_joint_position(int begin, int end, JointArgs a) {
+  const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end) joint_position_one(a.order[i], a);
+}
+static int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_joint_colours(World* w, int phase, const JointArgs& a) {
+  const std::vector<int>& off = w->h_joint_colour_off;
+  for (size_t c = 0; c + 1 < off.size(); ++c) {
+    const int b0 = off[c], b1 = off[c + 1];
+    if (b1 <= b0) continue;
+    if (phase == 0)
+      hipLaunchKernelGGL(k_joint_init, dim3(blocks(b1 - b0)), dim3(kBlock), 0, w->stream, b0, b1, a);
+    else if (phase == 1)
+      hipLaunchKernelGGL(k_joint_velocity, dim3(blocks(b1 - b0)), dim3(kBlock), 0, w->stream, b0, b1, a);
+    else
+      hipLaunchKernelGGL(k_joint_position, dim3(blocks(b1 - b0)), dim3(kBlock), 0, w->stream, b0, b1, a);
+  }
+}
+
+}  // namespace b2g

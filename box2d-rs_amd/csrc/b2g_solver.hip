@@ -255,13 +255,9 @@ __device__ inline bool uf_below(int a, int c) {
   const unsigned ha = uf_rank(a), hc = uf_rank(c);
   return ha < hc || (ha == hc && a < c);
 }
-__global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, const int2* __restrict__ bodies,
-                                                  const int* __restrict__ btype, int* parent) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= *ncp) return;
-  int2 b = bodies[j];
-  if (btype[b.x] == 0 || btype[b.y] == 0) return;
-  int a = uf_find(parent, b.x), c = uf_find(parent, b.y);
+__device__ inline void union_bodies(int x, int y, const int* __restrict__ btype, int* parent) {
+  if (btype[x] == 0 || btype[y] == 0) return;
+  int a = uf_find(parent, x), c = uf_find(parent, y);
   for (;;) {
     if (a == c) return;
     const bool a_low = uf_below(a, c);
@@ -273,6 +269,26 @@ __global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, c
     if (a_low) a = up;
     else c = up;
   }
+}
+__global__ void __launch_bounds__(kBlock) k_union(const int* __restrict__ ncp, const int2* __restrict__ bodies,
+                                                  const int* __restrict__ btype, int* parent) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *ncp) return;
+  int2 b = bodies[j];
+  union_bodies(b.x, b.y, btype, parent);
+}
+// joints join islands like touching contacts (src/private/dynamics/b2_world.rs:466-490)
+__global__ void __launch_bounds__(kBlock) k_joint_union(int nj, const int2* __restrict__ jb,
+                                                        const int* __restrict__ btype, int* parent) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  const int2 b = jb[j];
+  union_bodies(b.x, b.y, btype, parent);
+}
+void launch_joint_union(World* w) {
+  if (w->nj == 0) return;
+  hipLaunchKernelGGL(k_joint_union, dim3(blocks_for(w->nj, kBlock)), dim3(kBlock), 0, w->stream, w->nj,
+                     w->j.bodies.p, w->b.type.p, w->s.parent.p);
 }
 
 __global__ void __launch_bounds__(kBlock) k_compress(int nb, int* parent, int* __restrict__ violated,
@@ -1215,6 +1231,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                        c.old_index.p, old_colour, s.prev_colour.p, cnt, s.color_cursor.p);
     hipLaunchKernelGGL(k_union, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p, s.parent.p);
   }
+  launch_joint_union(w);
   hipLaunchKernelGGL(k_compress, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
                      s.isl_violated.p, s.isl_done.p, sleepy ? s.isl_awake.p : nullptr, s.isl_restless.p);
   const IslandView iv{s.parent.p, sleepy ? s.isl_awake.p : nullptr};
@@ -1335,7 +1352,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   // (graph nodes cannot carry the timing events of a timed solve kernel)
   const bool timed_tail = w->kt.kernel == B2G_KERNEL_WARM_START || w->kt.kernel == B2G_KERNEL_SOLVE_VELOCITY ||
                           w->kt.kernel == B2G_KERNEL_SOLVE_POSITION;
-  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy) {
+  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0) {
     // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
     int maxc = 0;
     for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);
@@ -1368,6 +1385,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   if (nc > 0 && w->warm_starting)
     launch_timed(w, B2G_KERNEL_WARM_START, nc, k_warm_start_bodies, dim3(blocks_for(nb, kBlock)), dim3(kBlock), nb,
                  w->b.type.p, s.adj_off.p, s.adj.p, s.inv_order.p, va, iv);
+  // joints (b2g_joints.hip): initialised after the contacts' warm start, solved before the
+  // contacts in each velocity iteration and after them in each position iteration
+  const JointArgs ja{w->j.order.p, w->j.bodies.p, w->j.type.p, w->j.d0.p, w->j.d1.p, w->j.d2.p, w->j.imp.p,
+                     w->j.imp2.p, w->j.r.p, w->j.k.p, w->j.k2.p, w->b.type.p, w->b.pos.p, w->b.vel.p, w->b.mass.p,
+                     iv, s.isl_violated.p, s.isl_done.p, dt, 1.0f / dt, dt_ratio, int(w->warm_starting)};
+  if (w->nj > 0) launch_joint_colours(w, 0, ja);
   // colours [tail, ncolors): the small last colours, one launch per iteration (when at least two);
   // co holds ncolors + 1 offsets only when there are constraints
   int tail = ncolors;
@@ -1376,7 +1399,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     if (ncolors - tail < 2) tail = ncolors;
   }
   const int tail_items = nc > 0 ? co[ncolors] - co[tail] : 0;
-  for (int it = 0; it < vel_iters && nc > 0; ++it) {
+  for (int it = 0; it < vel_iters && (nc > 0 || w->nj > 0); ++it) {
+    if (w->nj > 0) launch_joint_colours(w, 1, ja);
     for (int col = 0; col < tail; ++col) {
       int b0 = co[col], b1 = co[col + 1];
       launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)),
@@ -1399,6 +1423,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     if (tail < ncolors)
       launch_timed(w, B2G_KERNEL_SOLVE_POSITION, tail_items, k_solve_position_tail, dim3(1), dim3(kTailBlock), tail,
                    ncolors, s.color_off.p, pa);
+    if (w->nj > 0) launch_joint_colours(w, 2, ja);
     // (the last iteration's end runs with the copy back below)
     if (it + 1 < pos_iters)
       hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,

@@ -74,6 +74,24 @@ typedef struct b2g_fixture_def {
   int32_t pad1;
 } b2g_fixture_def;
 
+/* Joint kinds of b2g_joint_def::type: B2jointType values (src/b2_joint.rs:46-58). */
+#define B2G_JOINT_DISTANCE 1
+#define B2G_JOINT_REVOLUTE 8
+
+/* Joint definition: B2jointDef (src/b2_joint.rs:149-164) with B2distanceJointDef
+ * (src/joints/b2_distance_joint.rs) or B2revoluteJointDef (src/joints/b2_revolute_joint.rs:41-73);
+ * the bodies are creation indices. 88 bytes. */
+typedef struct b2g_joint_def {
+  int32_t type;
+  int32_t body_a, body_b;
+  int32_t collide_connected;
+  float local_anchor_a[2], local_anchor_b[2];
+  float length, min_length, max_length, stiffness, damping;                       /* distance */
+  float reference_angle, lower_angle, upper_angle, max_motor_torque, motor_speed; /* revolute */
+  int32_t enable_limit, enable_motor;                                             /* revolute */
+  int32_t pad[2];
+} b2g_joint_def;
+
 /* Body state as B2body holds it: transform m_xf, sweep, velocities. 64 bytes. */
 typedef struct b2g_body_state {
   float p[2], q[2]; /* m_xf.p, (m_xf.q.s, m_xf.q.c) */
@@ -185,6 +203,17 @@ int b2g_world_set_fat_aabbs(b2g_world* world, const float* in, int32_t n);
  * *count receives the number of contacts; at most cap are written. */
 int b2g_world_get_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, int32_t* flags,
                            b2g_manifold* manifolds, int32_t* count);
+
+/* B2world::create_joint (src/private/dynamics/b2_world.rs:156-275) for n joints, with the
+ * constructors' clamping (distance: length, min_length >= linear slop, max_length >= min_length,
+ * src/private/dynamics/joints/b2_distance_joint.rs:44-78). Joints join islands, are solved before
+ * the contacts in each velocity iteration and after them in each position iteration, and a joint
+ * with collide_connected = 0 removes its bodies' contacts from the next step on. Kinds other than
+ * distance and revolute: B2G_ERR_UNSUPPORTED. Not with a tape (b2g_tape_begin refuses joints). */
+int b2g_world_add_joints(b2g_world* world, int32_t n, const b2g_joint_def* defs);
+/* Warm-start impulses of every joint, 5 floats each (distance: impulse, lower, upper, 0, 0;
+ * revolute: impulse.x, impulse.y, motor, lower, upper). */
+int b2g_world_get_joint_impulses(b2g_world* world, float* out, int32_t n);
 
 /* The shape of every fixture as the device holds it (after the hull, box and edge setters):
  * circles as B2G_SHAPE_CIRCLE, edges as B2G_SHAPE_EDGE (flags bit0 = one-sided), polygons as
