@@ -9,7 +9,8 @@ are resident in HBM before timing. Prints one JSON line.
 
 Other workloads (not the default line): --workload batched (4096 worlds, world-steps/s against
 the same worlds stepped one by one), stress (1.18M bodies: body-steps/s, pairs/s, manifolds/s,
-solver body-iterations/s), adjoint (1024 worlds x 64 bodies, 120 taped steps + reverse pass).
+solver body-iterations/s), adjoint (1024 worlds x 64 bodies, 120 taped steps + reverse pass),
+chains (4096 hanging chains of 16 links on revolute joints).
 """
 import argparse
 import json
@@ -63,14 +64,17 @@ def pair_shape_bytes(w, sbytes):
     return float((sbytes[fab[:, 0]] + sbytes[fab[:, 1]]).mean()) if len(fab) else 0.0
 
 
-def cpu_baseline(scene, seconds=15.0, max_steps=200):
+def cpu_baseline(scene, seconds=15.0, max_steps=200, joints=None):
     """The oracle (sequential C++ restatement of the reference step, 1 thread) on the same
     scene, timed on this host for a bounded number of steps from the initial state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import b2o
     oracle = b2o.Oracle("f32")
     w = oracle.world()
+    w.set_flags(warm_starting=True, allow_sleep=False, block_solve=True)  # the GPU run's flags
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    if joints is not None:
+        w.add_joints(joints)
     steps, elapsed = 0, 0.0
     while elapsed < seconds and steps < max_steps:
         elapsed += w.step(1.0 / 60.0, 8, 3, steps=2)
@@ -213,6 +217,36 @@ def run_batched(args, b2g, scenes):
     return out
 
 
+def run_chains(args, b2g, scenes):
+    """4096 hanging chains of 16 links (the testbed's chain: revolute joints, neighbouring links
+    overlapping with collide_connected off) swinging down side by side: 65,536 bodies, 65,536
+    joints."""
+    scene, joints = scenes.chains(4096, 16)
+    w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
+    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    w.add_joints(joints)
+    for _ in range(args.warmup):
+        w.step(1.0 / 60.0, 8, 3)
+    steps = args.steps if args.steps is not None else 300
+    sbytes = shape_bytes(scene.shape_defs)
+    pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
+    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    if pair_shape:
+        pair_shape.append(pair_shape_bytes(w, sbytes))
+    p = w.profile()
+    n = int(scene.meta["dynamic"])
+    out = base_record(args, "body-steps/s (4096 chains x 16 links)", n * steps / elapsed, "body-steps/s", elapsed,
+                      steps, "chains: 4096 chains x 16 links (revolute joints), %d bodies, %d joints, dt 1/60, 8/3 "
+                      "iterations, warm start on, %s" % (n, len(joints), "sleep on" if args.sleep else "sleep off"),
+                      {"bodies": n, "joints": len(joints),
+                       "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux, pair_shape=pair_shape)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, joints=joints)
+    return out
+
+
 def run_stress(args, b2g, scenes):
     """1,048,576 large-world bodies and 16,384 towers of 8 boxes in a 1600 x 1600 m box: body-steps/s,
     plus the pair search's pairs/s (traversal time), the narrowphase's manifolds/s (collide time)
@@ -284,7 +318,7 @@ def main():
     ap.add_argument("--sleep", action="store_true",
                     help="allow_sleep on (large_world, batched): islands at rest sleep and are skipped")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint"))
+    ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint", "chains"))
     ap.add_argument("--roofline-kernel", default="collide", choices=sorted(UNIT_BYTES))
     ap.add_argument("--solver-graph", type=int, default=0, choices=(0, 1),
                     help="replay the solve's per-colour launches from a captured graph (default 0: direct "
@@ -297,7 +331,7 @@ def main():
     import b2g
     from b2g import scenes
     run = {"large_world": run_large_world, "batched": run_batched, "stress": run_stress,
-           "adjoint": run_adjoint}[args.workload]
+           "adjoint": run_adjoint, "chains": run_chains}[args.workload]
     print(json.dumps(run(args, b2g, scenes)))
 
 

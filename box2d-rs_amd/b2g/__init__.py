@@ -641,6 +641,89 @@ class B2body:
         return float(self._state()["sleep_time"])
 
 
+def _local_point(body, p):
+    """B2body::get_local_point: mul_t(xf, p) in f32 (src/b2_body.rs)."""
+    st = body._state()
+    f = np.float32
+    dx, dy = f(p[0]) - f(st["p"][0]), f(p[1]) - f(st["p"][1])
+    s, c = f(st["q"][0]), f(st["q"][1])
+    return (float(c * dx + s * dy), float(-s * dx + c * dy))
+
+
+class B2distanceJointDef:
+    """``B2distanceJointDef`` (src/joints/b2_distance_joint.rs:11-60)."""
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.local_anchor_a = (0.0, 0.0)
+        self.local_anchor_b = (0.0, 0.0)
+        self.length, self.min_length, self.max_length = 1.0, 0.0, float(np.finfo(np.float32).max)
+        self.stiffness = self.damping = 0.0
+
+    def initialize(self, body_a, body_b, anchor_a, anchor_b):
+        """b2_distance_joint_def_initialize (src/private/dynamics/joints/b2_distance_joint.rs:23-40)."""
+        self.body_a, self.body_b = body_a, body_b
+        self.local_anchor_a = _local_point(body_a, anchor_a)
+        self.local_anchor_b = _local_point(body_b, anchor_b)
+        d = np.float32(anchor_b[0]) - np.float32(anchor_a[0]), np.float32(anchor_b[1]) - np.float32(anchor_a[1])
+        self.length = float(max(np.sqrt(d[0] * d[0] + d[1] * d[1]), np.float32(0.005)))
+        self.min_length = self.max_length = self.length
+
+    def _def(self):
+        d = np.zeros((), JOINT_DEF)
+        d["type"] = JOINT_DISTANCE
+        d["body_a"], d["body_b"] = self.body_a.index, self.body_b.index
+        d["collide_connected"] = int(self.collide_connected)
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["length"], d["min_length"], d["max_length"] = self.length, self.min_length, self.max_length
+        d["stiffness"], d["damping"] = self.stiffness, self.damping
+        return d
+
+
+class B2revoluteJointDef:
+    """``B2revoluteJointDef`` (src/joints/b2_revolute_joint.rs:10-95)."""
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.local_anchor_a = (0.0, 0.0)
+        self.local_anchor_b = (0.0, 0.0)
+        self.reference_angle = 0.0
+        self.enable_limit, self.lower_angle, self.upper_angle = False, 0.0, 0.0
+        self.enable_motor, self.motor_speed, self.max_motor_torque = False, 0.0, 0.0
+
+    def initialize(self, body_a, body_b, anchor):
+        """B2revoluteJointDef::initialize (src/joints/b2_revolute_joint.rs:76-82)."""
+        self.body_a, self.body_b = body_a, body_b
+        self.local_anchor_a = _local_point(body_a, anchor)
+        self.local_anchor_b = _local_point(body_b, anchor)
+        self.reference_angle = float(np.float32(body_b.get_angle()) - np.float32(body_a.get_angle()))
+
+    def _def(self):
+        d = np.zeros((), JOINT_DEF)
+        d["type"] = JOINT_REVOLUTE
+        d["body_a"], d["body_b"] = self.body_a.index, self.body_b.index
+        d["collide_connected"] = int(self.collide_connected)
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["reference_angle"] = self.reference_angle
+        d["enable_limit"], d["lower_angle"], d["upper_angle"] = int(self.enable_limit), self.lower_angle, self.upper_angle
+        d["enable_motor"], d["motor_speed"] = int(self.enable_motor), self.motor_speed
+        d["max_motor_torque"] = self.max_motor_torque
+        return d
+
+
+class B2joint:
+    """Handle on one joint of a :class:`B2world`."""
+
+    def __init__(self, world, index):
+        self._world = world
+        self.index = index
+
+    def get_impulses(self):
+        return self._world._world.joint_impulses()[self.index]
+
+
 class B2world:
     """``B2world`` (src/b2_world.rs) stepping on the GPU.
 
@@ -719,6 +802,13 @@ class B2world:
         if self._cache is None:
             self._cache = self._world.body_states()
         return self._cache
+
+    def create_joint(self, joint_def):
+        """B2world::create_joint (src/private/dynamics/b2_world.rs:156-275): distance and revolute."""
+        self._flush()
+        self._world.add_joints(np.array([joint_def._def()], JOINT_DEF))
+        self._cache = None
+        return B2joint(self, self._world.joint_count - 1)
 
     def get_body_count(self):
         self._flush()

@@ -22,7 +22,7 @@ from collections import namedtuple
 import numpy as np
 
 from . import BODY_DEF, FIXTURE_DEF, SHAPE_DEF, B2_DYNAMIC_BODY, B2_STATIC_BODY, SHAPE_BOX, SHAPE_CIRCLE, \
-    SHAPE_EDGE, SHAPE_POLYGON
+    SHAPE_EDGE, SHAPE_POLYGON, JOINT_DEF, JOINT_REVOLUTE
 
 Scene = namedtuple("Scene", "body_defs fixture_body shape_defs fixture_defs meta")
 BatchScene = namedtuple("BatchScene", "body_defs body_world fixture_body shape_defs fixture_defs meta")
@@ -335,3 +335,47 @@ def world_of(batch, wi):
     remap[bsel] = np.arange(len(bsel))
     return Scene(batch.body_defs[bsel], remap[batch.fixture_body[fsel]].astype(np.int32), batch.shape_defs[fsel],
                  batch.fixture_defs[fsel], {"name": "world %d of batch" % wi, "dynamic": int(batch.meta["sizes"][wi])})
+
+
+def chains(n_chains=4096, links=16, spacing=2.0, rows=32):
+    """Hanging chains of the testbed's chain test (examples/testbed/tests/chain.rs:66-104): each
+    chain is `links` boxes of half extents 0.6 x 0.125 (density 20, friction 0.2) 1 m apart,
+    hinged end to end at integer x by revolute joints with collide_connected off (neighbouring
+    links overlap), the first hinged to a static pin. Chains stand in a grid (rows of chains
+    `spacing` apart, each row links + 1 m below the last) over a ground box and start horizontal,
+    so they swing down. Returns (Scene, joint_defs)."""
+    per_row = (n_chains + rows - 1) // rows
+    row_h = links + 1.0
+    ground = _body_defs(1, B2_STATIC_BODY)
+    gsd = np.zeros(1, SHAPE_DEF)
+    gsd["type"] = SHAPE_BOX
+    gsd["flags"] = 1
+    gsd[0]["params"][:5] = (per_row * spacing + links + 10.0, 0.5, 0.0, -0.5, 0.0)
+    gfd = _fixture_defs(1, friction=0.6, density=0.0)
+    pins = _body_defs(n_chains, B2_STATIC_BODY)
+    n = n_chains * links
+    bd = _body_defs(n)
+    sd = np.zeros(n, SHAPE_DEF)
+    sd["type"] = SHAPE_BOX
+    sd["params"][:, 0] = 0.6
+    sd["params"][:, 1] = 0.125
+    fd = _fixture_defs(n, friction=0.2, density=20.0)
+    jd = np.zeros(n, JOINT_DEF)
+    jd["type"] = JOINT_REVOLUTE
+    base_pin = 1
+    base_link = 1 + n_chains
+    for c in range(n_chains):
+        x0 = (c % per_row) * spacing - 0.5 * per_row * spacing
+        y = 2.0 + (rows - 1 - c // per_row) * row_h + links
+        pins[c]["position"] = (x0, y)
+        for k in range(links):
+            i = c * links + k
+            bd[i]["position"] = (x0 + 0.5 + k, y)
+            jd[i]["body_a"] = base_pin + c if k == 0 else base_link + i - 1
+            jd[i]["body_b"] = base_link + i
+            jd[i]["local_anchor_a"] = (0.0, 0.0) if k == 0 else (0.5, 0.0)
+            jd[i]["local_anchor_b"] = (-0.5, 0.0)
+    parts = concat((ground, [0], gsd, gfd), (pins, np.zeros(0, np.int32), np.zeros(0, SHAPE_DEF),
+                                             np.zeros(0, FIXTURE_DEF)),
+                   (bd, np.arange(n), sd, fd))
+    return Scene(*parts, {"name": "chains", "dynamic": n, "chains": n_chains, "links": links}), jd

@@ -167,3 +167,27 @@ def test_chain_on_ground_is_deterministic():
         np.testing.assert_array_equal(runs[0][f], runs[1][f])
     assert np.all(np.isfinite(runs[0]["c"]))
     assert runs[0]["c"][1:17, 1].min() > -0.2  # the chain rests on the ground
+
+
+def test_reference_style_api():
+    world = b2g.B2world((0.0, -10.0))
+    ground = world.create_body(b2g.B2bodyDef(position=(0.0, 0.0)))
+    bob = world.create_body(b2g.B2bodyDef(body_type=b2g.B2_DYNAMIC_BODY, position=(2.0, 10.0)))
+    circle = b2g.B2circleShape(0.1)
+    bob.create_fixture_by_shape(circle, 1.0)
+    jd = b2g.B2revoluteJointDef()
+    jd.initialize(ground, bob, (0.0, 10.0))
+    world.create_joint(jd)
+    rod = world.create_body(b2g.B2bodyDef(body_type=b2g.B2_DYNAMIC_BODY, position=(5.0, 10.0)))
+    rod.create_fixture_by_shape(circle, 1.0)
+    dd = b2g.B2distanceJointDef()
+    dd.initialize(ground, rod, (3.0, 10.0), (5.0, 10.0))
+    world.create_joint(dd)
+    for _ in range(60):
+        world.step(1.0 / 60.0, 8, 3)
+    p = np.array(bob.get_position())
+    assert abs(np.linalg.norm(p - (0.0, 10.0)) - 2.0) < 0.01
+    assert p[1] < 9.0  # it swung down
+    q = np.array(rod.get_position())
+    assert abs(np.linalg.norm(q - (3.0, 10.0)) - 2.0) < 0.01
+
