@@ -32,7 +32,7 @@ KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2
 TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH, TUNE_COLOR_REUSE = 2, 3, 4
 
 # Joint definition (B2G_JOINT_*: B2jointType values, src/b2_joint.rs:46-58)
-JOINT_DISTANCE, JOINT_REVOLUTE = 1, 8
+JOINT_DISTANCE, JOINT_REVOLUTE, JOINT_WELD = 1, 8, 9
 JOINT_DEF = np.dtype([("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
                       ("local_anchor_a", "<f4", 2), ("local_anchor_b", "<f4", 2), ("length", "<f4"),
                       ("min_length", "<f4"), ("max_length", "<f4"), ("stiffness", "<f4"), ("damping", "<f4"),
@@ -710,6 +710,34 @@ class B2revoluteJointDef:
         d["enable_limit"], d["lower_angle"], d["upper_angle"] = int(self.enable_limit), self.lower_angle, self.upper_angle
         d["enable_motor"], d["motor_speed"] = int(self.enable_motor), self.motor_speed
         d["max_motor_torque"] = self.max_motor_torque
+        return d
+
+
+class B2weldJointDef:
+    """``B2weldJointDef`` (src/joints/b2_weld_joint.rs:10-60)."""
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.local_anchor_a = (0.0, 0.0)
+        self.local_anchor_b = (0.0, 0.0)
+        self.reference_angle = 0.0
+        self.stiffness = self.damping = 0.0
+
+    def initialize(self, body_a, body_b, anchor):
+        """B2weldJointDef::initialize (src/joints/b2_weld_joint.rs:54-60)."""
+        self.body_a, self.body_b = body_a, body_b
+        self.local_anchor_a = _local_point(body_a, anchor)
+        self.local_anchor_b = _local_point(body_b, anchor)
+        self.reference_angle = float(np.float32(body_b.get_angle()) - np.float32(body_a.get_angle()))
+
+    def _def(self):
+        d = np.zeros((), JOINT_DEF)
+        d["type"] = JOINT_WELD
+        d["body_a"], d["body_b"] = self.body_a.index, self.body_b.index
+        d["collide_connected"] = int(self.collide_connected)
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["reference_angle"], d["stiffness"], d["damping"] = self.reference_angle, self.stiffness, self.damping
         return d
 
 

@@ -1,4 +1,4 @@
-// b2g joints in the coloured island solve: distance and revolute joints.
+// b2g joints in the coloured island solve: distance, revolute and weld joints.
 //
 // The reference solves an island's joints in island order before its contacts in every velocity
 // iteration, and after its contacts in every position iteration
@@ -10,7 +10,9 @@
 // order:
 //   distance joint  src/private/dynamics/joints/b2_distance_joint.rs:80-320
 //   revolute joint  src/private/dynamics/joints/b2_revolute_joint.rs:60-330
-//   B2Mat22::solve  src/b2_math.rs:278-293
+//   weld joint      src/private/dynamics/joints/b2_weld_joint.rs:23-300
+//   B2Mat22::solve  src/b2_math.rs:278-293; B2Mat33 solve33 / solve22 / get_inverse22 /
+//   get_sym_inverse33 src/private/common/b2_math.rs:5-71; b2_mul_mat33 / b2_mul22 src/b2_math.rs:601-608
 #include "b2g_internal.h"
 
 namespace b2g {
@@ -23,6 +25,53 @@ __device__ inline V2 mat22_solve(float a11, float a21, float a12, float a22, V2 
   float det = a11 * a22 - a12 * a21;
   if (det != 0.0f) det = 1.0f / det;
   return v2(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
+}
+
+// ---- weld joint helpers: the symmetric 3x3 K and its inverses ------------------------------------
+struct Sym33 {
+  float m11, m12, m13, m22, m23, m33;  // columns ex = (m11, m12, m13), ey = (m12, m22, m23), ez = (m13, m23, m33)
+};
+struct F3 {
+  float x, y, z;
+};
+__device__ inline float dot3(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ inline F3 cross3(F3 a, F3 b) {
+  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ inline Sym33 weld_k(float mA, float mB, float iA, float iB, V2 rA, V2 rB) {
+  Sym33 k;
+  k.m11 = ((mA + mB) + rA.y * rA.y * iA) + rB.y * rB.y * iB;
+  k.m12 = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+  k.m13 = -rA.y * iA - rB.y * iB;
+  k.m22 = ((mA + mB) + rA.x * rA.x * iA) + rB.x * rB.x * iB;
+  k.m23 = rA.x * iA + rB.x * iB;
+  k.m33 = iA + iB;
+  return k;
+}
+__device__ inline Sym33 inverse22(const Sym33& k) {
+  const float a = k.m11, b = k.m12, c = k.m12, d = k.m22;
+  float det = a * d - b * c;
+  if (det != 0.0f) det = 1.0f / det;
+  return Sym33{det * d, -det * b, 0.0f, det * a, 0.0f, 0.0f};
+}
+__device__ inline Sym33 sym_inverse33(const Sym33& k) {
+  const F3 ex{k.m11, k.m12, k.m13}, ey{k.m12, k.m22, k.m23}, ez{k.m13, k.m23, k.m33};
+  float det = dot3(ex, cross3(ey, ez));
+  if (det != 0.0f) det = 1.0f / det;
+  const float a11 = k.m11, a12 = k.m12, a13 = k.m13, a22 = k.m22, a23 = k.m23, a33 = k.m33;
+  return Sym33{det * (a22 * a33 - a23 * a23), det * (a13 * a23 - a12 * a33), det * (a12 * a23 - a13 * a22),
+               det * (a11 * a33 - a13 * a13), det * (a13 * a12 - a11 * a23), det * (a11 * a22 - a12 * a12)};
+}
+__device__ inline F3 mul33(const Sym33& m, F3 v) {
+  return F3{(v.x * m.m11 + v.y * m.m12) + v.z * m.m13, (v.x * m.m12 + v.y * m.m22) + v.z * m.m23,
+            (v.x * m.m13 + v.y * m.m23) + v.z * m.m33};
+}
+__device__ inline V2 mul22(const Sym33& m, V2 v) { return v2(m.m11 * v.x + m.m12 * v.y, m.m12 * v.x + m.m22 * v.y); }
+__device__ inline F3 solve33(const Sym33& k, F3 b) {
+  const F3 ex{k.m11, k.m12, k.m13}, ey{k.m12, k.m22, k.m23}, ez{k.m13, k.m23, k.m33};
+  float det = dot3(ex, cross3(ey, ez));
+  if (det != 0.0f) det = 1.0f / det;
+  return F3{det * dot3(b, cross3(ey, ez)), det * dot3(ex, cross3(b, ez)), det * dot3(ex, cross3(ey, b))};
 }
 
 // a joint's two bodies as the solver holds them
@@ -111,6 +160,41 @@ __device__ void joint_init_one(int j, const JointArgs& a) {
     }
     a.jk[j] = make_float4(u.x, u.y, mass, soft_mass);
     a.jk2[j] = make_float4(gamma, bias, current, 0.0f);
+  } else if (type == B2G_JOINT_WELD) {
+    const float reference_angle = d1.x, stiffness = d1.y, damping = d1.z;
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const Sym33 K = weld_k(mA, mB, iA, iB, rA, rB);
+    Sym33 M;
+    float gamma = 0.0f, bias = 0.0f;
+    if (stiffness > 0.0f) {
+      M = inverse22(K);
+      float inv_m = iA + iB;
+      const float C = (PB.z - PA.z) - reference_angle;
+      const float h = a.dt;
+      gamma = h * (damping + h * stiffness);
+      gamma = gamma != 0.0f ? 1.0f / gamma : 0.0f;
+      bias = C * h * stiffness * gamma;
+      inv_m += gamma;
+      M.m33 = inv_m != 0.0f ? 1.0f / inv_m : 0.0f;
+    } else if (K.m33 == 0.0f) {
+      M = inverse22(K);
+    } else {
+      M = sym_inverse33(K);
+    }
+    if (a.warm) {
+      imp.x *= a.dt_ratio;
+      imp.y *= a.dt_ratio;
+      imp.z *= a.dt_ratio;
+      const V2 P = v2(imp.x, imp.y);
+      vA = vA - mA * P;
+      wA -= iA * (cross(rA, P) + imp.z);
+      vB = vB + mB * P;
+      wB += iB * (cross(rB, P) + imp.z);
+    } else {
+      imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    a.jk[j] = make_float4(M.m11, M.m12, M.m13, M.m22);
+    a.jk2[j] = make_float4(M.m23, M.m33, gamma, bias);
   } else {
     const float reference_angle = d1.x;
     const int flags = a.jt[j].y;
@@ -223,6 +307,40 @@ __device__ void joint_velocity_one(int j, const JointArgs& a) {
       imp.x += impulse;
       distance_push(impulse, u, rA, rB, B, vA, wA, vB, wB);
     }
+  } else if (type == B2G_JOINT_WELD) {
+    const float stiffness = a.jd1[j].y;
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const Sym33 M{k.x, k.y, k.z, k.w, k2.x, k2.y};
+    const float gamma = k2.z, bias = k2.w;
+    if (stiffness > 0.0f) {
+      const float Cdot2 = wB - wA;
+      const float impulse2 = -M.m33 * ((Cdot2 + bias) + gamma * imp.z);
+      imp.z += impulse2;
+      wA -= iA * impulse2;
+      wB += iB * impulse2;
+      const V2 Cdot1 = ((vB + cross(wB, rB)) - vA) - cross(wA, rA);
+      const V2 impulse1 = -mul22(M, Cdot1);
+      imp.x += impulse1.x;
+      imp.y += impulse1.y;
+      const V2 P = impulse1;
+      vA = vA - mA * P;
+      wA -= iA * cross(rA, P);
+      vB = vB + mB * P;
+      wB += iB * cross(rB, P);
+    } else {
+      const V2 Cdot1 = ((vB + cross(wB, rB)) - vA) - cross(wA, rA);
+      const float Cdot2 = wB - wA;
+      const F3 m = mul33(M, F3{Cdot1.x, Cdot1.y, Cdot2});
+      const F3 impulse{-m.x, -m.y, -m.z};
+      imp.x += impulse.x;
+      imp.y += impulse.y;
+      imp.z += impulse.z;
+      const V2 P = v2(impulse.x, impulse.y);
+      vA = vA - mA * P;
+      wA -= iA * (cross(rA, P) + impulse.z);
+      vB = vB + mB * P;
+      wB += iB * (cross(rB, P) + impulse.z);
+    }
   } else {
     const int flags = a.jt[j].y;
     const bool limit = flags & 2, motor = flags & 4;
@@ -319,6 +437,43 @@ __device__ void joint_position_one(int j, const JointArgs& a) {
     }
     okay = !apply || fabsf(C) < kLinearSlop;
     if (!apply) return;
+  } else if (type == B2G_JOINT_WELD) {
+    const float reference_angle = d1.x, stiffness = d1.y;
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const Rot qA = rot(aA), qB = rot(aB);
+    const V2 rA = mul(qA, laA - B.lcA);
+    const V2 rB = mul(qB, laB - B.lcB);
+    const Sym33 K = weld_k(mA, mB, iA, iB, rA, rB);
+    float position_error, angular_error;
+    if (stiffness > 0.0f) {
+      const V2 C1 = ((cB + rB) - cA) - rA;
+      position_error = len(C1);
+      angular_error = 0.0f;
+      const V2 P = -mat22_solve(K.m11, K.m12, K.m12, K.m22, C1);
+      cA = cA - mA * P;
+      aA -= iA * cross(rA, P);
+      cB = cB + mB * P;
+      aB += iB * cross(rB, P);
+    } else {
+      const V2 C1 = ((cB + rB) - cA) - rA;
+      const float C2 = (aB - aA) - reference_angle;
+      position_error = len(C1);
+      angular_error = fabsf(C2);
+      F3 impulse;
+      if (K.m33 > 0.0f) {
+        const F3 sol = solve33(K, F3{C1.x, C1.y, C2});
+        impulse = F3{-sol.x, -sol.y, -sol.z};
+      } else {
+        const V2 i2 = -mat22_solve(K.m11, K.m12, K.m12, K.m22, C1);
+        impulse = F3{i2.x, i2.y, 0.0f};
+      }
+      const V2 P = v2(impulse.x, impulse.y);
+      cA = cA - mA * P;
+      aA -= iA * (cross(rA, P) + impulse.z);
+      cB = cB + mB * P;
+      aB += iB * (cross(rB, P) + impulse.z);
+    }
+    okay = position_error <= kLinearSlop && angular_error <= kAngularSlop;
   } else {
     const int flags = a.jt[j].y;
     const bool limit = flags & 2;

@@ -344,7 +344,7 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
   for (int i = 0; i < n; ++i) {
     const B2oJointDef& d = defs[i];
     if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) return i;
-    if (d.type != kDistanceJoint && d.type != kRevoluteJoint) return i;
+    if (d.type != kDistanceJoint && d.type != kRevoluteJoint && d.type != kWeldJoint) return i;
     Joint j;
     j.type = d.type;
     j.a = w->bodies[d.body_a];
@@ -356,6 +356,11 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
       j.length = bmax(d.length, kLinearSlop);
       j.min_length = bmax(d.min_length, kLinearSlop);
       j.max_length = bmax(d.max_length, j.min_length);
+      j.stiffness = d.stiffness;
+      j.damping = d.damping;
+    } else if (d.type == kWeldJoint) {
+      // B2weldJoint::new (src/joints/b2_weld_joint.rs): reference angle, stiffness, damping
+      j.reference_angle = d.reference_angle;
       j.stiffness = d.stiffness;
       j.damping = d.damping;
     } else {
@@ -382,6 +387,10 @@ void b2o_world_get_joints(void* wp, B2oJointState* out) {
       o.impulse[0] = j->impulse;
       o.impulse[1] = j->lower_impulse;
       o.impulse[2] = j->upper_impulse;
+    } else if (j->type == kWeldJoint) {
+      o.impulse[0] = j->impulse3[0];
+      o.impulse[1] = j->impulse3[1];
+      o.impulse[2] = j->impulse3[2];
     } else {
       o.impulse[0] = j->impulse2.x;
       o.impulse[1] = j->impulse2.y;

@@ -1,9 +1,11 @@
-// Oracle (test infrastructure, never part of the product): the distance and revolute joint
-// solvers of the reference, restated in operation order.
+// Oracle (test infrastructure, never part of the product): the distance, revolute and weld
+// joint solvers of the reference, restated in operation order.
 //   distance joint  src/private/dynamics/joints/b2_distance_joint.rs:23-330
 //   revolute joint  src/private/dynamics/joints/b2_revolute_joint.rs:60-330, construction
 //                   src/joints/b2_revolute_joint.rs:253-287
-//   B2Mat22::solve  src/b2_math.rs:278-293
+//   weld joint      src/private/dynamics/joints/b2_weld_joint.rs:23-300
+//   B2Mat22::solve  src/b2_math.rs:278-293; B2Mat33 solve33 / solve22 / get_inverse22 /
+//   get_sym_inverse33 src/private/common/b2_math.rs:5-71, b2_mul_mat33 / b2_mul22 src/b2_math.rs:601-608
 #pragma once
 
 #include "b2o_world.h"
@@ -355,16 +357,254 @@ inline bool revolute_solve_position(Joint* j, const JointSolverData& d) {
   return position_error <= kLinearSlop && angular_error <= kAngularSlop;
 }
 
+// ---- 3x3 helpers: columns ex, ey, ez --------------------------------------------------------
+struct Vec3 {
+  real x = 0, y = 0, z = 0;
+  Vec3() = default;
+  Vec3(real x_, real y_, real z_) : x(x_), y(y_), z(z_) {}
+};
+struct Mat33 {
+  Vec3 ex, ey, ez;
+};
+inline real dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 cross3(Vec3 a, Vec3 b) {
+  return Vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline Vec3 solve33(const Mat33& k, Vec3 b) {
+  real det = dot3(k.ex, cross3(k.ey, k.ez));
+  if (det != 0) det = real(1.0) / det;
+  return Vec3(det * dot3(b, cross3(k.ey, k.ez)), det * dot3(k.ex, cross3(b, k.ez)), det * dot3(k.ex, cross3(k.ey, b)));
+}
+inline Vec2 solve22(const Mat33& k, Vec2 b) {
+  const real a11 = k.ex.x, a12 = k.ey.x, a21 = k.ex.y, a22 = k.ey.y;
+  real det = a11 * a22 - a12 * a21;
+  if (det != 0) det = real(1.0) / det;
+  return Vec2(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
+}
+inline void inverse22(const Mat33& k, Mat33* m) {
+  const real a = k.ex.x, b = k.ey.x, c = k.ex.y, d = k.ey.y;
+  real det = a * d - b * c;
+  if (det != 0) det = real(1.0) / det;
+  m->ex.x = det * d;
+  m->ey.x = -det * b;
+  m->ex.z = 0;
+  m->ex.y = -det * c;
+  m->ey.y = det * a;
+  m->ey.z = 0;
+  m->ez.x = 0;
+  m->ez.y = 0;
+  m->ez.z = 0;
+}
+inline void sym_inverse33(const Mat33& k, Mat33* m) {
+  real det = dot3(k.ex, cross3(k.ey, k.ez));
+  if (det != 0) det = real(1.0) / det;
+  const real a11 = k.ex.x, a12 = k.ey.x, a13 = k.ez.x, a22 = k.ey.y, a23 = k.ez.y, a33 = k.ez.z;
+  m->ex.x = det * (a22 * a33 - a23 * a23);
+  m->ex.y = det * (a13 * a23 - a12 * a33);
+  m->ex.z = det * (a12 * a23 - a13 * a22);
+  m->ey.x = m->ex.y;
+  m->ey.y = det * (a11 * a33 - a13 * a13);
+  m->ey.z = det * (a13 * a12 - a11 * a23);
+  m->ez.x = m->ex.z;
+  m->ez.y = m->ey.z;
+  m->ez.z = det * (a11 * a22 - a12 * a12);
+}
+// v.x * ex + v.y * ey + v.z * ez
+inline Vec3 mul33(const Mat33& a, Vec3 v) {
+  return Vec3(v.x * a.ex.x + v.y * a.ey.x + v.z * a.ez.x, v.x * a.ex.y + v.y * a.ey.y + v.z * a.ez.y,
+              v.x * a.ex.z + v.y * a.ey.z + v.z * a.ez.z);
+}
+inline Vec2 mul22(const Mat33& a, Vec2 v) { return Vec2(a.ex.x * v.x + a.ey.x * v.y, a.ex.y * v.x + a.ey.y * v.y); }
+inline Mat33 weld_k(real mA, real mB, real iA, real iB, Vec2 rA, Vec2 rB) {
+  Mat33 k;
+  k.ex.x = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+  k.ey.x = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+  k.ez.x = -rA.y * iA - rB.y * iB;
+  k.ex.y = k.ey.x;
+  k.ey.y = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+  k.ez.y = rA.x * iA + rB.x * iB;
+  k.ex.z = k.ez.x;
+  k.ey.z = k.ez.y;
+  k.ez.z = iA + iB;
+  return k;
+}
+inline Mat33 mass_of(const Joint* j) {
+  Mat33 m;
+  m.ex = Vec3(j->mass33[0][0], j->mass33[0][1], j->mass33[0][2]);
+  m.ey = Vec3(j->mass33[1][0], j->mass33[1][1], j->mass33[1][2]);
+  m.ez = Vec3(j->mass33[2][0], j->mass33[2][1], j->mass33[2][2]);
+  return m;
+}
+inline void set_mass(Joint* j, const Mat33& m) {
+  const Vec3* cols[3] = {&m.ex, &m.ey, &m.ez};
+  for (int c = 0; c < 3; ++c) {
+    j->mass33[c][0] = cols[c]->x;
+    j->mass33[c][1] = cols[c]->y;
+    j->mass33[c][2] = cols[c]->z;
+  }
+}
+
+// ---- weld joint ----------------------------------------------------------------------------
+inline void weld_init_velocity(Joint* j, const JointSolverData& d) {
+  joint_bind(j);
+  const real aA = d.positions[j->index_a].a;
+  const real aB = d.positions[j->index_b].a;
+  Vec2 vA = d.velocities[j->index_a].v;
+  real wA = d.velocities[j->index_a].w;
+  Vec2 vB = d.velocities[j->index_b].v;
+  real wB = d.velocities[j->index_b].w;
+  Rot qA, qB;
+  qA.set(aA);
+  qB.set(aB);
+  j->rA = mul(qA, j->local_anchor_a - j->local_center_a);
+  j->rB = mul(qB, j->local_anchor_b - j->local_center_b);
+  const real mA = j->inv_mass_a, mB = j->inv_mass_b, iA = j->inv_ia, iB = j->inv_ib;
+  const Mat33 K = weld_k(mA, mB, iA, iB, j->rA, j->rB);
+  Mat33 M;
+  if (j->stiffness > 0) {
+    inverse22(K, &M);
+    real inv_m = iA + iB;
+    const real C = aB - aA - j->reference_angle;
+    const real dmp = j->damping, k = j->stiffness, h = d.step.dt;
+    j->gamma = h * (dmp + h * k);
+    j->gamma = j->gamma != 0 ? real(1.0) / j->gamma : real(0);
+    j->bias = C * h * k * j->gamma;
+    inv_m += j->gamma;
+    M.ez.z = inv_m != 0 ? real(1.0) / inv_m : real(0);
+  } else if (K.ez.z == 0) {
+    inverse22(K, &M);
+    j->gamma = 0;
+    j->bias = 0;
+  } else {
+    sym_inverse33(K, &M);
+    j->gamma = 0;
+    j->bias = 0;
+  }
+  set_mass(j, M);
+  if (d.step.warm_starting) {
+    j->impulse3[0] *= d.step.dt_ratio;
+    j->impulse3[1] *= d.step.dt_ratio;
+    j->impulse3[2] *= d.step.dt_ratio;
+    const Vec2 P(j->impulse3[0], j->impulse3[1]);
+    vA -= mA * P;
+    wA -= iA * (cross(j->rA, P) + j->impulse3[2]);
+    vB += mB * P;
+    wB += iB * (cross(j->rB, P) + j->impulse3[2]);
+  } else {
+    j->impulse3[0] = j->impulse3[1] = j->impulse3[2] = 0;
+  }
+  d.velocities[j->index_a].v = vA;
+  d.velocities[j->index_a].w = wA;
+  d.velocities[j->index_b].v = vB;
+  d.velocities[j->index_b].w = wB;
+}
+
+inline void weld_solve_velocity(Joint* j, const JointSolverData& d) {
+  Vec2 vA = d.velocities[j->index_a].v;
+  real wA = d.velocities[j->index_a].w;
+  Vec2 vB = d.velocities[j->index_b].v;
+  real wB = d.velocities[j->index_b].w;
+  const real mA = j->inv_mass_a, mB = j->inv_mass_b, iA = j->inv_ia, iB = j->inv_ib;
+  const Mat33 M = mass_of(j);
+  if (j->stiffness > 0) {
+    const real Cdot2 = wB - wA;
+    const real impulse2 = -M.ez.z * (Cdot2 + j->bias + j->gamma * j->impulse3[2]);
+    j->impulse3[2] += impulse2;
+    wA -= iA * impulse2;
+    wB += iB * impulse2;
+    const Vec2 Cdot1 = vB + cross(wB, j->rB) - vA - cross(wA, j->rA);
+    const Vec2 impulse1 = -mul22(M, Cdot1);
+    j->impulse3[0] += impulse1.x;
+    j->impulse3[1] += impulse1.y;
+    const Vec2 P = impulse1;
+    vA -= mA * P;
+    wA -= iA * cross(j->rA, P);
+    vB += mB * P;
+    wB += iB * cross(j->rB, P);
+  } else {
+    const Vec2 Cdot1 = vB + cross(wB, j->rB) - vA - cross(wA, j->rA);
+    const real Cdot2 = wB - wA;
+    const Vec3 Cdot(Cdot1.x, Cdot1.y, Cdot2);
+    const Vec3 m = mul33(M, Cdot);
+    const Vec3 impulse(-m.x, -m.y, -m.z);
+    j->impulse3[0] += impulse.x;
+    j->impulse3[1] += impulse.y;
+    j->impulse3[2] += impulse.z;
+    const Vec2 P(impulse.x, impulse.y);
+    vA -= mA * P;
+    wA -= iA * (cross(j->rA, P) + impulse.z);
+    vB += mB * P;
+    wB += iB * (cross(j->rB, P) + impulse.z);
+  }
+  d.velocities[j->index_a].v = vA;
+  d.velocities[j->index_a].w = wA;
+  d.velocities[j->index_b].v = vB;
+  d.velocities[j->index_b].w = wB;
+}
+
+inline bool weld_solve_position(Joint* j, const JointSolverData& d) {
+  Vec2 cA = d.positions[j->index_a].c;
+  real aA = d.positions[j->index_a].a;
+  Vec2 cB = d.positions[j->index_b].c;
+  real aB = d.positions[j->index_b].a;
+  Rot qA, qB;
+  qA.set(aA);
+  qB.set(aB);
+  const real mA = j->inv_mass_a, mB = j->inv_mass_b, iA = j->inv_ia, iB = j->inv_ib;
+  const Vec2 rA = mul(qA, j->local_anchor_a - j->local_center_a);
+  const Vec2 rB = mul(qB, j->local_anchor_b - j->local_center_b);
+  real position_error, angular_error;
+  const Mat33 K = weld_k(mA, mB, iA, iB, rA, rB);
+  if (j->stiffness > 0) {
+    const Vec2 C1 = cB + rB - cA - rA;
+    position_error = C1.length();
+    angular_error = 0;
+    const Vec2 P = -solve22(K, C1);
+    cA -= mA * P;
+    aA -= iA * cross(rA, P);
+    cB += mB * P;
+    aB += iB * cross(rB, P);
+  } else {
+    const Vec2 C1 = cB + rB - cA - rA;
+    const real C2 = aB - aA - j->reference_angle;
+    position_error = C1.length();
+    angular_error = babs(C2);
+    const Vec3 C(C1.x, C1.y, C2);
+    Vec3 impulse;
+    if (K.ez.z > 0) {
+      const Vec3 s = solve33(K, C);
+      impulse = Vec3(-s.x, -s.y, -s.z);
+    } else {
+      const Vec2 i2 = -solve22(K, C1);
+      impulse = Vec3(i2.x, i2.y, 0);
+    }
+    const Vec2 P(impulse.x, impulse.y);
+    cA -= mA * P;
+    aA -= iA * (cross(rA, P) + impulse.z);
+    cB += mB * P;
+    aB += iB * (cross(rB, P) + impulse.z);
+  }
+  d.positions[j->index_a].c = cA;
+  d.positions[j->index_a].a = aA;
+  d.positions[j->index_b].c = cB;
+  d.positions[j->index_b].a = aB;
+  return position_error <= kLinearSlop && angular_error <= kAngularSlop;
+}
+
 inline void joint_init_velocity(Joint* j, const JointSolverData& d) {
   if (j->type == kDistanceJoint) distance_init_velocity(j, d);
+  else if (j->type == kWeldJoint) weld_init_velocity(j, d);
   else revolute_init_velocity(j, d);
 }
 inline void joint_solve_velocity(Joint* j, const JointSolverData& d) {
   if (j->type == kDistanceJoint) distance_solve_velocity(j, d);
+  else if (j->type == kWeldJoint) weld_solve_velocity(j, d);
   else revolute_solve_velocity(j, d);
 }
 inline bool joint_solve_position(Joint* j, const JointSolverData& d) {
-  return j->type == kDistanceJoint ? distance_solve_position(j, d) : revolute_solve_position(j, d);
+  if (j->type == kDistanceJoint) return distance_solve_position(j, d);
+  if (j->type == kWeldJoint) return weld_solve_position(j, d);
+  return revolute_solve_position(j, d);
 }
 
 }  // namespace b2o

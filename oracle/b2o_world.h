@@ -113,7 +113,7 @@ struct ContactEdge {
 struct Filter { uint16_t category = 1, mask = 0xFFFF; int16_t group = 0; };
 
 // Joints: B2jointType values (src/b2_joint.rs:46-58) of the two restated kinds
-enum JointType { kDistanceJoint = 1, kRevoluteJoint = 8 };
+enum JointType { kDistanceJoint = 1, kRevoluteJoint = 8, kWeldJoint = 9 };
 struct Joint;
 // src/b2_joint.rs B2jointEdge; a body's list is push_front / newest-first
 struct JointEdge {
@@ -185,6 +185,9 @@ struct Joint {
   Vec2 impulse2;
   real motor_impulse = 0, axial_mass = 0, angle = 0;
   Mat22 K;
+  // weld (src/joints/b2_weld_joint.rs): impulse (x, y, angular), the 3x3 mass (symmetric)
+  real impulse3[3] = {0, 0, 0};
+  real mass33[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // [column][row]
   // solver temporaries
   int index_a = 0, index_b = 0;
   Vec2 local_center_a, local_center_b, rA, rB;

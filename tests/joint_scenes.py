@@ -74,3 +74,28 @@ def chain(links=12, half=0.25, y=10.0, collide_connected=False):
         j["local_anchor_a"] = anchor - a_pos
         j["local_anchor_b"] = anchor - np.array(bd[1 + i]["position"], np.float64)
     return bd, fb, sd, _fixture(links), jd
+
+
+def welds(stiffness, half=(1.0, 0.1), spacing=4.0, omega=0.0):
+    """One static body and, per entry of stiffness, a box welded at its left end to a point of the
+    static body (a cantilever; stiffness 0 is the rigid weld), each its own island."""
+    n = len(stiffness)
+    bd = _bodies(1 + n)
+    bd[0]["type"] = 0
+    sd = np.zeros(n, b2g.SHAPE_DEF)
+    jd = np.zeros(n, b2g.JOINT_DEF)
+    fb = np.arange(1, n + 1, dtype=np.int32)
+    for i, k in enumerate(stiffness):
+        anchor = np.array([spacing * i, 10.0])
+        bd[1 + i]["type"] = 2
+        bd[1 + i]["position"] = anchor + np.array([half[0], 0.0])
+        bd[1 + i]["angular_velocity"] = omega
+        sd[i]["type"] = b2g.SHAPE_BOX
+        sd[i]["params"][:2] = half
+        j = jd[i]
+        j["type"] = b2g.JOINT_WELD
+        j["body_a"], j["body_b"] = 0, 1 + i
+        j["local_anchor_a"] = anchor
+        j["local_anchor_b"] = (-half[0], 0.0)
+        j["stiffness"] = k
+    return bd, fb, sd, _fixture(n), jd

@@ -5,6 +5,8 @@
   (device sin/cos may differ from glibc's by one ulp), bodies and joint impulses.
 * A hanging 12-link chain (joints in two colours against the reference's island order) stays
   within a band of the oracle's run over one second.
+* Welded cantilevers (rigid and soft, each alone in its island) agree with the oracle at every
+  step, as the pendulums do.
 * Motor and limit behave as the closed forms of the oracle's pins say; collide_connected filters
   pairs, also for a joint created between touching bodies; a pendulum at rest falls asleep with
   sleeping on; runs are deterministic.
@@ -13,7 +15,7 @@ import numpy as np
 import pytest
 
 import b2g
-from joint_scenes import chain, pendulums
+from joint_scenes import chain, pendulums, welds
 
 pytestmark = pytest.mark.gpu
 
@@ -190,4 +192,20 @@ def test_reference_style_api():
     assert p[1] < 9.0  # it swung down
     q = np.array(rod.get_position())
     assert abs(np.linalg.norm(q - (3.0, 10.0)) - 2.0) < 0.01
+
+
+def test_welds_match_oracle(oracle):
+    bd, fb, sd, fd, jd = welds([0.0, 5.0, 0.0, 40.0], omega=0.5)
+    jd[1]["damping"] = 0.3
+    bd[3]["angle"] = 0.2  # welded at an angle: the reference angle is 0, the weld pulls it back
+    g = _gpu(bd, fb, sd, fd, jd)
+    r = _cpu(oracle, bd, fb, sd, fd, jd)
+    for step in range(150):
+        g.step(DT, 8, 3)
+        r.step(DT, 8, 3)
+        _close(g.body_states(), r.bodies(), 1e-5, "step %d" % step)
+        gi = g.joint_impulses()[:, :3]
+        ri = np.asarray(r.joints()["impulse"], np.float64)[:, :3]
+        err = np.abs(gi - ri).max()
+        assert err <= 1e-5 * max(1.0, np.abs(ri).max()), "step %d: weld impulses differ by %g" % (step, err)
 

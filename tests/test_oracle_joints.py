@@ -15,7 +15,10 @@ src/private/dynamics/b2_body.rs:391-416) by what mechanics fixes:
   * limit: a hinged bar released horizontally stays within its angle limits (plus the angular
     slop);
   * collide_connected: overlapping jointed boxes get no contact unless the joint allows it, and
-    a joint created between touching bodies removes their contact at the next step.
+    a joint created between touching bodies removes their contact at the next step;
+  * weld: a rigid cantilever holds its angle and anchor within the slops under gravity; a soft
+    weld (angular stiffness k) swings about its anchor with period 2 pi sqrt(I_anchor / k)
+    (within 2%), I_anchor = I + m d^2 about the anchor.
 A slip such as a wrong sign in the block solve, a dropped inertia term or a swapped limit
 fails one of these.
 """
@@ -23,7 +26,7 @@ import numpy as np
 import pytest
 
 import b2g
-from joint_scenes import chain, pendulums
+from joint_scenes import chain, pendulums, welds
 
 DT = np.float32(1.0 / 60.0)
 G = 10.0
@@ -154,3 +157,38 @@ def test_joint_filters_an_existing_contact(oracle):
     w.add_joints(j2)
     w.step(DT, 8, 3)
     assert w.contact_count == 0
+
+
+def test_weld_cantilever_rigid(oracle):
+    bd, fb, sd, fd, jd = welds([0.0])
+    w = _oracle_world(oracle, bd, fb, sd, fd, jd)
+    slop = 2.0 / 180.0 * np.pi
+    for _ in range(120):
+        w.step(DT, 8, 3)
+        st = w.bodies()[1]
+        assert abs(float(st["a"])) < 2 * slop
+        # the welded end stays at the anchor
+        q = st["q"]
+        end = np.array(st["p"], np.float64) + np.array([q[1] * -1.0, q[0] * -1.0])
+        assert np.linalg.norm(end - (0.0, 10.0)) < 0.01
+
+
+def test_weld_soft_angular_period(oracle):
+    k = 2.0
+    half = (1.0, 0.1)
+    bd, fb, sd, fd, jd = welds([k], half=half, omega=0.2)
+    w = _oracle_world(oracle, bd, fb, sd, fd, jd, gravity=(0.0, 0.0))
+    m = fd["density"][0] * 4 * half[0] * half[1]
+    I = m * (4 * half[0] ** 2 + 4 * half[1] ** 2) / 12.0
+    T = 2 * np.pi * np.sqrt((I + m * half[0] ** 2) / k)
+    steps = int(3.2 * T / DT)
+    angles = []
+    for _ in range(steps):
+        w.step(DT, 8, 3)
+        angles.append(float(w.bodies()[1]["a"]))
+    t = np.arange(1, steps + 1) * float(DT)
+    zc = _zero_crossings(t, np.array(angles))
+    assert len(zc) >= 5
+    period = 2 * np.mean(np.diff(zc))
+    assert abs(period / T - 1) < 0.02, (period, T)
+
