@@ -32,13 +32,13 @@ KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2
 TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH, TUNE_COLOR_REUSE = 2, 3, 4
 
 # Joint definition (B2G_JOINT_*: B2jointType values, src/b2_joint.rs:46-58)
-JOINT_DISTANCE, JOINT_REVOLUTE, JOINT_WELD = 1, 8, 9
+JOINT_DISTANCE, JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_WELD = 1, 6, 8, 9
 JOINT_DEF = np.dtype([("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
                       ("local_anchor_a", "<f4", 2), ("local_anchor_b", "<f4", 2), ("length", "<f4"),
                       ("min_length", "<f4"), ("max_length", "<f4"), ("stiffness", "<f4"), ("damping", "<f4"),
                       ("reference_angle", "<f4"), ("lower_angle", "<f4"), ("upper_angle", "<f4"),
                       ("max_motor_torque", "<f4"), ("motor_speed", "<f4"), ("enable_limit", "<i4"),
-                      ("enable_motor", "<i4"), ("pad", "<i4", 2)])
+                      ("enable_motor", "<i4"), ("local_axis_a", "<f4", 2)])
 
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
@@ -738,6 +738,48 @@ class B2weldJointDef:
         d["collide_connected"] = int(self.collide_connected)
         d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
         d["reference_angle"], d["stiffness"], d["damping"] = self.reference_angle, self.stiffness, self.damping
+        return d
+
+
+class B2prismaticJointDef:
+    """``B2prismaticJointDef`` (src/joints/b2_prismatic_joint.rs): anchors, the axis in body A's
+    frame, reference angle, translation limits, motor (speed, max force)."""
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.local_anchor_a = (0.0, 0.0)
+        self.local_anchor_b = (0.0, 0.0)
+        self.local_axis_a = (1.0, 0.0)
+        self.reference_angle = 0.0
+        self.enable_limit, self.lower_translation, self.upper_translation = False, 0.0, 0.0
+        self.enable_motor, self.max_motor_force, self.motor_speed = False, 0.0, 0.0
+
+    def initialize(self, body_a, body_b, anchor, axis):
+        """B2prismaticJointDef::initialize (src/joints/b2_prismatic_joint.rs:75-90): anchors from the
+        bodies' poses, the axis as a local vector of body A, the reference angle from their angles."""
+        self.body_a, self.body_b = body_a, body_b
+        self.local_anchor_a = _local_point(body_a, anchor)
+        self.local_anchor_b = _local_point(body_b, anchor)
+        st = body_a._state()
+        f = np.float32
+        s, c = f(st["q"][0]), f(st["q"][1])
+        ax, ay = f(axis[0]), f(axis[1])
+        self.local_axis_a = (float(c * ax + s * ay), float(-s * ax + c * ay))  # get_local_vector
+        self.reference_angle = float(np.float32(body_b.get_angle()) - np.float32(body_a.get_angle()))
+
+    def _def(self):
+        d = np.zeros((), JOINT_DEF)
+        d["type"] = JOINT_PRISMATIC
+        d["body_a"], d["body_b"] = self.body_a.index, self.body_b.index
+        d["collide_connected"] = int(self.collide_connected)
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["local_axis_a"] = self.local_axis_a
+        d["reference_angle"] = self.reference_angle
+        d["enable_limit"], d["lower_angle"], d["upper_angle"] = (int(self.enable_limit), self.lower_translation,
+                                                                 self.upper_translation)
+        d["enable_motor"], d["motor_speed"], d["max_motor_torque"] = (int(self.enable_motor), self.motor_speed,
+                                                                       self.max_motor_force)
         return d
 
 

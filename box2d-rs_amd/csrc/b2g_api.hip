@@ -317,6 +317,12 @@ int upload_joints(World* w) {
     } else if (d.type == B2G_JOINT_WELD) {
       d1[i] = make_float4(d.reference_angle, d.stiffness, d.damping, 0.0f);
       d2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else if (d.type == B2G_JOINT_PRISMATIC) {
+      // the axis normalised as B2vec2::normalize does (src/b2_math.rs:82-92)
+      V2 axis = v2(d.local_axis_a[0], d.local_axis_a[1]);
+      normalize(&axis);
+      d1[i] = make_float4(d.reference_angle, d.lower_angle, d.upper_angle, d.max_motor_torque);
+      d2[i] = make_float4(d.motor_speed, axis.x, axis.y, 0.0f);
     } else {
       d1[i] = make_float4(d.reference_angle, d.lower_angle, d.upper_angle, d.max_motor_torque);
       d2[i] = make_float4(d.motor_speed, 0.0f, 0.0f, 0.0f);
@@ -816,8 +822,10 @@ int b2g_world_add_joints(b2g_world* h, int32_t n, const b2g_joint_def* defs) {
   if (w->tape.active) { set_error("b2g_world_add_joints: a tape is being recorded"); return B2G_ERR_UNSUPPORTED; }
   for (int i = 0; i < n; ++i) {
     const b2g_joint_def& d = defs[i];
-    if (d.type != B2G_JOINT_DISTANCE && d.type != B2G_JOINT_REVOLUTE && d.type != B2G_JOINT_WELD) {
-      set_error("b2g_world_add_joints: joint " + std::to_string(i) + ": only distance, revolute and weld joints");
+    if (d.type != B2G_JOINT_DISTANCE && d.type != B2G_JOINT_REVOLUTE && d.type != B2G_JOINT_WELD &&
+        d.type != B2G_JOINT_PRISMATIC) {
+      set_error("b2g_world_add_joints: joint " + std::to_string(i) +
+                ": only distance, revolute, weld and prismatic joints");
       return B2G_ERR_UNSUPPORTED;
     }
     if (d.body_a < 0 || d.body_a >= w->nb || d.body_b < 0 || d.body_b >= w->nb || d.body_a == d.body_b ||
@@ -885,7 +893,7 @@ int b2g_world_get_joint_impulses(b2g_world* h, float* out, int32_t n) {
     o[2] = imp[i].z;
     o[3] = imp[i].w;
     o[4] = imp2[i];
-    if (w->h_joints[i].type != B2G_JOINT_REVOLUTE) o[3] = o[4] = 0.0f;
+    if (w->h_joints[i].type != B2G_JOINT_REVOLUTE && w->h_joints[i].type != B2G_JOINT_PRISMATIC) o[3] = o[4] = 0.0f;
   }
   return B2G_OK;
 }

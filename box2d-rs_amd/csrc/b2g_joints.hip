@@ -1,4 +1,4 @@
-// b2g joints in the coloured island solve: distance, revolute and weld joints.
+// b2g joints in the coloured island solve: distance, revolute, weld and prismatic joints.
 //
 // The reference solves an island's joints in island order before its contacts in every velocity
 // iteration, and after its contacts in every position iteration
@@ -11,6 +11,8 @@
 //   distance joint  src/private/dynamics/joints/b2_distance_joint.rs:80-320
 //   revolute joint  src/private/dynamics/joints/b2_revolute_joint.rs:60-330
 //   weld joint      src/private/dynamics/joints/b2_weld_joint.rs:23-300
+//   prismatic joint src/private/dynamics/joints/b2_prismatic_joint.rs:40-460 (the perpendicular
+//                   axis is (-axis.y, axis.x): bit for bit what rotating cross(1, local axis) gives)
 //   B2Mat22::solve  src/b2_math.rs:278-293; B2Mat33 solve33 / solve22 / get_inverse22 /
 //   get_sym_inverse33 src/private/common/b2_math.rs:5-71; b2_mul_mat33 / b2_mul22 src/b2_math.rs:601-608
 #include "b2g_internal.h"
@@ -102,6 +104,14 @@ __device__ inline bool joint_bodies(int j, const JointArgs& a, JointBodies* jb, 
   return true;
 }
 
+__device__ inline void prismatic_push(const JointBodies& B, V2 P, float LA, float LB, V2& vA, float& wA, V2& vB,
+                                      float& wB) {
+  vA = vA - B.mA * P;
+  wA -= B.iA * LA;
+  vB = vB + B.mB * P;
+  wB += B.iB * LB;
+}
+
 // ---- init: B2distanceJoint / B2revoluteJoint::init_velocity_constraints (with warm start) ----
 __device__ void joint_init_one(int j, const JointArgs& a) {
   JointBodies B;
@@ -160,6 +170,54 @@ __device__ void joint_init_one(int j, const JointArgs& a) {
     }
     a.jk[j] = make_float4(u.x, u.y, mass, soft_mass);
     a.jk2[j] = make_float4(gamma, bias, current, 0.0f);
+  } else if (type == B2G_JOINT_PRISMATIC) {
+    const int flags = a.jt[j].y;
+    const bool limit = flags & 2, motor = flags & 4;
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const V2 cA = v2(PA.x, PA.y), cB = v2(PB.x, PB.y);
+    const V2 dd = ((cB - cA) + rB) - rA;
+    const V2 axis = mul(qA, v2(d2.y, d2.z));
+    const float a1 = cross(dd + rA, axis), a2 = cross(rB, axis);
+    float axial = ((mA + mB) + iA * a1 * a1) + iB * a2 * a2;
+    if (axial > 0.0f) axial = 1.0f / axial;
+    const V2 perp = v2(-axis.y, axis.x);
+    const float s1 = cross(dd + rA, perp), s2 = cross(rB, perp);
+    const float k11 = ((mA + mB) + iA * s1 * s1) + iB * s2 * s2;
+    const float k12 = iA * s1 + iB * s2;
+    float k22 = iA + iB;
+    if (k22 == 0.0f) k22 = 1.0f;
+    float translation = 0.0f;
+    float upper = a.jimp2[j];
+    if (limit) {
+      translation = dot(axis, dd);
+    } else {
+      imp.w = 0.0f;
+      upper = 0.0f;
+    }
+    if (!motor) imp.z = 0.0f;
+    if (a.warm) {
+      imp.x *= a.dt_ratio;
+      imp.y *= a.dt_ratio;
+      imp.z *= a.dt_ratio;
+      imp.w *= a.dt_ratio;
+      upper *= a.dt_ratio;
+      const float axial_impulse = (imp.z + imp.w) - upper;
+      const V2 P = imp.x * perp + axial_impulse * axis;
+      const float LA = (imp.x * s1 + imp.y) + axial_impulse * a1;
+      const float LB = (imp.x * s2 + imp.y) + axial_impulse * a2;
+      prismatic_push(B, P, LA, LB, vA, wA, vB, wB);
+    } else {
+      imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      upper = 0.0f;
+    }
+    a.jimp2[j] = upper;
+    a.jimp[j] = imp;
+    a.jr[j] = make_float4(axis.x, axis.y, s1, s2);
+    a.jk[j] = make_float4(k11, k12, k22, axial);
+    a.jk2[j] = make_float4(a1, a2, translation, 0.0f);
+    if (B.dyn_a) a.vel[B.a] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (B.dyn_b) a.vel[B.b] = make_float4(vB.x, vB.y, wB, 0.0f);
+    return;
   } else if (type == B2G_JOINT_WELD) {
     const float reference_angle = d1.x, stiffness = d1.y, damping = d1.z;
     const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
@@ -307,6 +365,60 @@ __device__ void joint_velocity_one(int j, const JointArgs& a) {
       imp.x += impulse;
       distance_push(impulse, u, rA, rB, B, vA, wA, vB, wB);
     }
+  } else if (type == B2G_JOINT_PRISMATIC) {
+    const int flags = a.jt[j].y;
+    const bool limit = flags & 2, motor = flags & 4;
+    const float4 d1 = a.jd1[j], d2 = a.jd2[j];
+    const float lower = d1.y, upper_t = d1.z, max_force = d1.w, motor_speed = d2.x;
+    const V2 axis = v2(r.x, r.y), perp = v2(-r.y, r.x);
+    const float s1 = r.z, s2 = r.w, k11 = k.x, k12 = k.y, k22 = k.z, axial = k.w;
+    const float a1 = k2.x, a2 = k2.y, translation = k2.z;
+    float upper = a.jimp2[j];
+    if (motor) {
+      const float Cdot = (dot(axis, vB - vA) + a2 * wB) - a1 * wA;
+      float impulse = axial * (motor_speed - Cdot);
+      const float old = imp.z;
+      const float max_impulse = a.dt * max_force;
+      imp.z = bclamp(imp.z + impulse, -max_impulse, max_impulse);
+      impulse = imp.z - old;
+      prismatic_push(B, impulse * axis, impulse * a1, impulse * a2, vA, wA, vB, wB);
+    }
+    if (limit) {
+      {
+        const float C = translation - lower;
+        const float Cdot = (dot(axis, vB - vA) + a2 * wB) - a1 * wA;
+        float impulse = -axial * (Cdot + bmax(C, 0.0f) * a.inv_dt);
+        const float old = imp.w;
+        imp.w = bmax(imp.w + impulse, 0.0f);
+        impulse = imp.w - old;
+        prismatic_push(B, impulse * axis, impulse * a1, impulse * a2, vA, wA, vB, wB);
+      }
+      {
+        const float C = upper_t - translation;
+        const float Cdot = (dot(axis, vA - vB) + a1 * wA) - a2 * wB;
+        float impulse = -axial * (Cdot + bmax(C, 0.0f) * a.inv_dt);
+        const float old = upper;
+        upper = bmax(upper + impulse, 0.0f);
+        impulse = upper - old;
+        const V2 P = impulse * axis;
+        const float LA = impulse * a1, LB = impulse * a2;
+        vA = vA + B.mA * P;
+        wA += B.iA * LA;
+        vB = vB - B.mB * P;
+        wB -= B.iB * LB;
+      }
+    }
+    {
+      const V2 Cdot = v2((dot(perp, vB - vA) + s2 * wB) - s1 * wA, wB - wA);
+      const V2 df = mat22_solve(k11, k12, k12, k22, -Cdot);
+      imp.x += df.x;
+      imp.y += df.y;
+      const V2 P = df.x * perp;
+      const float LA = df.x * s1 + df.y;
+      const float LB = df.x * s2 + df.y;
+      prismatic_push(B, P, LA, LB, vA, wA, vB, wB);
+    }
+    a.jimp2[j] = upper;
   } else if (type == B2G_JOINT_WELD) {
     const float stiffness = a.jd1[j].y;
     const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
@@ -437,6 +549,63 @@ __device__ void joint_position_one(int j, const JointArgs& a) {
     }
     okay = !apply || fabsf(C) < kLinearSlop;
     if (!apply) return;
+  } else if (type == B2G_JOINT_PRISMATIC) {
+    const int flags = a.jt[j].y;
+    const bool limit = flags & 2;
+    const float reference_angle = d1.x, lower = d1.y, upper = d1.z;
+    const float4 d2 = a.jd2[j];
+    const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;
+    const Rot qA = rot(aA), qB = rot(aB);
+    const V2 rA = mul(qA, laA - B.lcA);
+    const V2 rB = mul(qB, laB - B.lcB);
+    const V2 dd = ((cB + rB) - cA) - rA;
+    const V2 axis = mul(qA, v2(d2.y, d2.z));
+    const float a1 = cross(dd + rA, axis), a2 = cross(rB, axis);
+    const V2 perp = v2(-axis.y, axis.x);
+    const float s1 = cross(dd + rA, perp), s2 = cross(rB, perp);
+    const V2 C1 = v2(dot(perp, dd), (aB - aA) - reference_angle);
+    float linear_error = fabsf(C1.x);
+    const float angular_error = fabsf(C1.y);
+    bool active = false;
+    float C2 = 0.0f;
+    if (limit) {
+      const float translation = dot(axis, dd);
+      if (fabsf(upper - lower) < 2.0f * kLinearSlop) {
+        C2 = translation;
+        linear_error = bmax(linear_error, fabsf(translation));
+        active = true;
+      } else if (translation <= lower) {
+        C2 = bmin(translation - lower, 0.0f);
+        linear_error = bmax(linear_error, lower - translation);
+        active = true;
+      } else if (translation >= upper) {
+        C2 = bmax(translation - upper, 0.0f);
+        linear_error = bmax(linear_error, translation - upper);
+        active = true;
+      }
+    }
+    F3 impulse;
+    const float k11 = ((mA + mB) + iA * s1 * s1) + iB * s2 * s2;
+    const float k12 = iA * s1 + iB * s2;
+    float k22 = iA + iB;
+    if (k22 == 0.0f) k22 = 1.0f;
+    if (active) {
+      const float k13 = iA * s1 * a1 + iB * s2 * a2;
+      const float k23 = iA * a1 + iB * a2;
+      const float k33 = ((mA + mB) + iA * a1 * a1) + iB * a2 * a2;
+      impulse = solve33(Sym33{k11, k12, k13, k22, k23, k33}, F3{-C1.x, -C1.y, -C2});
+    } else {
+      const V2 i1 = mat22_solve(k11, k12, k12, k22, -C1);
+      impulse = F3{i1.x, i1.y, 0.0f};
+    }
+    const V2 P = impulse.x * perp + impulse.z * axis;
+    const float LA = (impulse.x * s1 + impulse.y) + impulse.z * a1;
+    const float LB = (impulse.x * s2 + impulse.y) + impulse.z * a2;
+    cA = cA - mA * P;
+    aA -= iA * LA;
+    cB = cB + mB * P;
+    aB += iB * LB;
+    okay = linear_error <= kLinearSlop && angular_error <= kAngularSlop;
   } else if (type == B2G_JOINT_WELD) {
     const float reference_angle = d1.x, stiffness = d1.y;
     const float mA = B.mA, mB = B.mB, iA = B.iA, iB = B.iB;

@@ -76,13 +76,17 @@ typedef struct b2g_fixture_def {
 
 /* Joint kinds of b2g_joint_def::type: B2jointType values (src/b2_joint.rs:46-58). */
 #define B2G_JOINT_DISTANCE 1
+#define B2G_JOINT_PRISMATIC 6
 #define B2G_JOINT_REVOLUTE 8
 #define B2G_JOINT_WELD 9
 
 /* Joint definition: B2jointDef (src/b2_joint.rs:149-164) with B2distanceJointDef
- * (src/joints/b2_distance_joint.rs), B2revoluteJointDef (src/joints/b2_revolute_joint.rs:41-73) or
- * B2weldJointDef (src/joints/b2_weld_joint.rs: local anchors, reference_angle, stiffness, damping);
- * the bodies are creation indices. 88 bytes. */
+ * (src/joints/b2_distance_joint.rs), B2revoluteJointDef (src/joints/b2_revolute_joint.rs:41-73),
+ * B2weldJointDef (src/joints/b2_weld_joint.rs: local anchors, reference_angle, stiffness, damping) or
+ * B2prismaticJointDef (src/joints/b2_prismatic_joint.rs: local anchors, local_axis_a,
+ * reference_angle, lower/upper translation in lower_angle/upper_angle, max motor force in
+ * max_motor_torque, motor_speed, enable_limit, enable_motor); the bodies are creation indices.
+ * 88 bytes. */
 typedef struct b2g_joint_def {
   int32_t type;
   int32_t body_a, body_b;
@@ -90,8 +94,8 @@ typedef struct b2g_joint_def {
   float local_anchor_a[2], local_anchor_b[2];
   float length, min_length, max_length, stiffness, damping;                       /* distance (weld: stiffness, damping) */
   float reference_angle, lower_angle, upper_angle, max_motor_torque, motor_speed; /* revolute (weld: reference_angle) */
-  int32_t enable_limit, enable_motor;                                             /* revolute */
-  int32_t pad[2];
+  int32_t enable_limit, enable_motor;                                             /* revolute, prismatic */
+  float local_axis_a[2];                                                          /* prismatic */
 } b2g_joint_def;
 
 /* Body state as B2body holds it: transform m_xf, sweep, velocities. 64 bytes. */
@@ -211,10 +215,10 @@ int b2g_world_get_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, 
  * src/private/dynamics/joints/b2_distance_joint.rs:44-78). Joints join islands, are solved before
  * the contacts in each velocity iteration and after them in each position iteration, and a joint
  * with collide_connected = 0 removes its bodies' contacts from the next step on. Kinds other than
- * distance, revolute and weld: B2G_ERR_UNSUPPORTED. Not with a tape (b2g_tape_begin refuses joints). */
+ * distance, revolute, weld and prismatic: B2G_ERR_UNSUPPORTED. Not with a tape (b2g_tape_begin refuses joints). */
 int b2g_world_add_joints(b2g_world* world, int32_t n, const b2g_joint_def* defs);
 /* Warm-start impulses of every joint, 5 floats each (distance: impulse, lower, upper, 0, 0;
- * revolute: impulse.x, impulse.y, motor, lower, upper; weld: impulse.x, .y, .z, 0, 0). */
+ * revolute and prismatic: impulse.x, impulse.y, motor, lower, upper; weld: impulse.x, .y, .z, 0, 0). */
 int b2g_world_get_joint_impulses(b2g_world* world, float* out, int32_t n);
 
 /* The shape of every fixture as the device holds it (after the hull, box and edge setters):

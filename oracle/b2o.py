@@ -42,7 +42,7 @@ def _dtypes(real):
                           ("min_length", real), ("max_length", real), ("stiffness", real), ("damping", real),
                           ("reference_angle", real), ("lower_angle", real), ("upper_angle", real),
                           ("max_motor_torque", real), ("motor_speed", real), ("enable_limit", "<i4"),
-                          ("enable_motor", "<i4"), ("pad", "<i4", 2)], align=True)
+                          ("enable_motor", "<i4"), ("local_axis_a", real, 2)], align=True)
     joint_state = np.dtype([("impulse", real, 5), ("pad", real, 3)], align=True)
     return dict(body_def=body_def, shape_def=shape_def, fixture_def=fixture_def, body_state=body_state,
                 manifold=manifold, shape=shape, joint_def=joint_def, joint_state=joint_state)

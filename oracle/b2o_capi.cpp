@@ -42,14 +42,14 @@ struct B2oShapeDef {
 // Joint definition: B2distanceJointDef / B2revoluteJointDef (src/joints/b2_distance_joint.rs,
 // src/joints/b2_revolute_joint.rs) with the bodies as creation indices
 struct B2oJointDef {
-  int32_t type;  // 1 distance, 8 revolute (B2jointType)
+  int32_t type;  // 1 distance, 6 prismatic, 8 revolute, 9 weld (B2jointType)
   int32_t body_a, body_b;
   int32_t collide_connected;
   real local_anchor_a[2], local_anchor_b[2];
   real length, min_length, max_length, stiffness, damping;
   real reference_angle, lower_angle, upper_angle, max_motor_torque, motor_speed;
   int32_t enable_limit, enable_motor;
-  int32_t pad[2];
+  real local_axis_a[2];  // prismatic
 };
 // Joint state: the warm-start impulses (distance: impulse, lower, upper; revolute: impulse.x,
 // impulse.y, motor, lower, upper)
@@ -344,7 +344,8 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
   for (int i = 0; i < n; ++i) {
     const B2oJointDef& d = defs[i];
     if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) return i;
-    if (d.type != kDistanceJoint && d.type != kRevoluteJoint && d.type != kWeldJoint) return i;
+    if (d.type != kDistanceJoint && d.type != kRevoluteJoint && d.type != kWeldJoint && d.type != kPrismaticJoint)
+      return i;
     Joint j;
     j.type = d.type;
     j.a = w->bodies[d.body_a];
@@ -358,6 +359,19 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
       j.max_length = bmax(d.max_length, j.min_length);
       j.stiffness = d.stiffness;
       j.damping = d.damping;
+    } else if (d.type == kPrismaticJoint) {
+      // b2_prismatic_joint::new (src/private/dynamics/joints/b2_prismatic_joint.rs:40-70): the axis
+      // normalised, the perpendicular cross(1, axis); limits as translations, motor as a force
+      j.local_xaxis_a = Vec2(d.local_axis_a[0], d.local_axis_a[1]);
+      j.local_xaxis_a.normalize();
+      j.local_yaxis_a = cross(real(1.0), j.local_xaxis_a);
+      j.reference_angle = d.reference_angle;
+      j.lower_translation = d.lower_angle;
+      j.upper_translation = d.upper_angle;
+      j.max_motor_force = d.max_motor_torque;
+      j.motor_speed = d.motor_speed;
+      j.enable_limit = d.enable_limit != 0;
+      j.enable_motor = d.enable_motor != 0;
     } else if (d.type == kWeldJoint) {
       // B2weldJoint::new (src/joints/b2_weld_joint.rs): reference angle, stiffness, damping
       j.reference_angle = d.reference_angle;
@@ -387,6 +401,12 @@ void b2o_world_get_joints(void* wp, B2oJointState* out) {
       o.impulse[0] = j->impulse;
       o.impulse[1] = j->lower_impulse;
       o.impulse[2] = j->upper_impulse;
+    } else if (j->type == kPrismaticJoint || j->type == kRevoluteJoint) {
+      o.impulse[0] = j->impulse2.x;
+      o.impulse[1] = j->impulse2.y;
+      o.impulse[2] = j->motor_impulse;
+      o.impulse[3] = j->lower_impulse;
+      o.impulse[4] = j->upper_impulse;
     } else if (j->type == kWeldJoint) {
       o.impulse[0] = j->impulse3[0];
       o.impulse[1] = j->impulse3[1];

@@ -1,9 +1,10 @@
-// Oracle (test infrastructure, never part of the product): the distance, revolute and weld
-// joint solvers of the reference, restated in operation order.
+// Oracle (test infrastructure, never part of the product): the distance, revolute, weld and
+// prismatic joint solvers of the reference, restated in operation order.
 //   distance joint  src/private/dynamics/joints/b2_distance_joint.rs:23-330
 //   revolute joint  src/private/dynamics/joints/b2_revolute_joint.rs:60-330, construction
 //                   src/joints/b2_revolute_joint.rs:253-287
 //   weld joint      src/private/dynamics/joints/b2_weld_joint.rs:23-300
+//   prismatic joint src/private/dynamics/joints/b2_prismatic_joint.rs:40-460
 //   B2Mat22::solve  src/b2_math.rs:278-293; B2Mat33 solve33 / solve22 / get_inverse22 /
 //   get_sym_inverse33 src/private/common/b2_math.rs:5-71, b2_mul_mat33 / b2_mul22 src/b2_math.rs:601-608
 #pragma once
@@ -591,19 +592,226 @@ inline bool weld_solve_position(Joint* j, const JointSolverData& d) {
   return position_error <= kLinearSlop && angular_error <= kAngularSlop;
 }
 
+// ---- prismatic joint ------------------------------------------------------------------------
+inline void prismatic_push(Joint* j, Vec2 P, real LA, real LB, Vec2& vA, real& wA, Vec2& vB, real& wB) {
+  vA -= j->inv_mass_a * P;
+  wA -= j->inv_ia * LA;
+  vB += j->inv_mass_b * P;
+  wB += j->inv_ib * LB;
+}
+
+inline void prismatic_init_velocity(Joint* j, const JointSolverData& d) {
+  joint_bind(j);
+  const Vec2 cA = d.positions[j->index_a].c;
+  const real aA = d.positions[j->index_a].a;
+  Vec2 vA = d.velocities[j->index_a].v;
+  real wA = d.velocities[j->index_a].w;
+  const Vec2 cB = d.positions[j->index_b].c;
+  const real aB = d.positions[j->index_b].a;
+  Vec2 vB = d.velocities[j->index_b].v;
+  real wB = d.velocities[j->index_b].w;
+  Rot qA, qB;
+  qA.set(aA);
+  qB.set(aB);
+  const Vec2 rA = mul(qA, j->local_anchor_a - j->local_center_a);
+  const Vec2 rB = mul(qB, j->local_anchor_b - j->local_center_b);
+  const Vec2 dd = (cB - cA) + rB - rA;
+  const real mA = j->inv_mass_a, mB = j->inv_mass_b, iA = j->inv_ia, iB = j->inv_ib;
+  {
+    j->axis = mul(qA, j->local_xaxis_a);
+    j->a1 = cross(dd + rA, j->axis);
+    j->a2 = cross(rB, j->axis);
+    j->axial_mass = mA + mB + iA * j->a1 * j->a1 + iB * j->a2 * j->a2;
+    if (j->axial_mass > 0) j->axial_mass = real(1.0) / j->axial_mass;
+  }
+  {
+    j->perp = mul(qA, j->local_yaxis_a);
+    j->s1 = cross(dd + rA, j->perp);
+    j->s2 = cross(rB, j->perp);
+    const real k11 = mA + mB + iA * j->s1 * j->s1 + iB * j->s2 * j->s2;
+    const real k12 = iA * j->s1 + iB * j->s2;
+    real k22 = iA + iB;
+    if (k22 == 0) k22 = 1;
+    j->K.ex = Vec2(k11, k12);
+    j->K.ey = Vec2(k12, k22);
+  }
+  if (j->enable_limit) {
+    j->translation = dot(j->axis, dd);
+  } else {
+    j->lower_impulse = 0;
+    j->upper_impulse = 0;
+  }
+  if (!j->enable_motor) j->motor_impulse = 0;
+  if (d.step.warm_starting) {
+    j->impulse2 *= d.step.dt_ratio;
+    j->motor_impulse *= d.step.dt_ratio;
+    j->lower_impulse *= d.step.dt_ratio;
+    j->upper_impulse *= d.step.dt_ratio;
+    const real axial_impulse = j->motor_impulse + j->lower_impulse - j->upper_impulse;
+    const Vec2 P = j->impulse2.x * j->perp + axial_impulse * j->axis;
+    const real LA = j->impulse2.x * j->s1 + j->impulse2.y + axial_impulse * j->a1;
+    const real LB = j->impulse2.x * j->s2 + j->impulse2.y + axial_impulse * j->a2;
+    prismatic_push(j, P, LA, LB, vA, wA, vB, wB);
+  } else {
+    j->impulse2 = Vec2(0, 0);
+    j->motor_impulse = 0;
+    j->lower_impulse = 0;
+    j->upper_impulse = 0;
+  }
+  d.velocities[j->index_a].v = vA;
+  d.velocities[j->index_a].w = wA;
+  d.velocities[j->index_b].v = vB;
+  d.velocities[j->index_b].w = wB;
+}
+
+inline void prismatic_solve_velocity(Joint* j, const JointSolverData& d) {
+  Vec2 vA = d.velocities[j->index_a].v;
+  real wA = d.velocities[j->index_a].w;
+  Vec2 vB = d.velocities[j->index_b].v;
+  real wB = d.velocities[j->index_b].w;
+  if (j->enable_motor) {
+    const real Cdot = dot(j->axis, vB - vA) + j->a2 * wB - j->a1 * wA;
+    real impulse = j->axial_mass * (j->motor_speed - Cdot);
+    const real old = j->motor_impulse;
+    const real max_impulse = d.step.dt * j->max_motor_force;
+    j->motor_impulse = bclamp(j->motor_impulse + impulse, -max_impulse, max_impulse);
+    impulse = j->motor_impulse - old;
+    prismatic_push(j, impulse * j->axis, impulse * j->a1, impulse * j->a2, vA, wA, vB, wB);
+  }
+  if (j->enable_limit) {
+    {
+      const real C = j->translation - j->lower_translation;
+      const real Cdot = dot(j->axis, vB - vA) + j->a2 * wB - j->a1 * wA;
+      real impulse = -j->axial_mass * (Cdot + bmax(C, real(0)) * d.step.inv_dt);
+      const real old = j->lower_impulse;
+      j->lower_impulse = bmax(j->lower_impulse + impulse, real(0));
+      impulse = j->lower_impulse - old;
+      prismatic_push(j, impulse * j->axis, impulse * j->a1, impulse * j->a2, vA, wA, vB, wB);
+    }
+    {
+      const real C = j->upper_translation - j->translation;
+      const real Cdot = dot(j->axis, vA - vB) + j->a1 * wA - j->a2 * wB;
+      real impulse = -j->axial_mass * (Cdot + bmax(C, real(0)) * d.step.inv_dt);
+      const real old = j->upper_impulse;
+      j->upper_impulse = bmax(j->upper_impulse + impulse, real(0));
+      impulse = j->upper_impulse - old;
+      const Vec2 P = impulse * j->axis;
+      const real LA = impulse * j->a1, LB = impulse * j->a2;
+      vA += j->inv_mass_a * P;
+      wA += j->inv_ia * LA;
+      vB -= j->inv_mass_b * P;
+      wB -= j->inv_ib * LB;
+    }
+  }
+  {
+    const Vec2 Cdot(dot(j->perp, vB - vA) + j->s2 * wB - j->s1 * wA, wB - wA);
+    const Vec2 df = mat22_solve(j->K, -Cdot);
+    j->impulse2 += df;
+    const Vec2 P = df.x * j->perp;
+    const real LA = df.x * j->s1 + df.y;
+    const real LB = df.x * j->s2 + df.y;
+    prismatic_push(j, P, LA, LB, vA, wA, vB, wB);
+  }
+  d.velocities[j->index_a].v = vA;
+  d.velocities[j->index_a].w = wA;
+  d.velocities[j->index_b].v = vB;
+  d.velocities[j->index_b].w = wB;
+}
+
+inline bool prismatic_solve_position(Joint* j, const JointSolverData& d) {
+  Vec2 cA = d.positions[j->index_a].c;
+  real aA = d.positions[j->index_a].a;
+  Vec2 cB = d.positions[j->index_b].c;
+  real aB = d.positions[j->index_b].a;
+  Rot qA, qB;
+  qA.set(aA);
+  qB.set(aB);
+  const real mA = j->inv_mass_a, mB = j->inv_mass_b, iA = j->inv_ia, iB = j->inv_ib;
+  const Vec2 rA = mul(qA, j->local_anchor_a - j->local_center_a);
+  const Vec2 rB = mul(qB, j->local_anchor_b - j->local_center_b);
+  const Vec2 dd = cB + rB - cA - rA;
+  const Vec2 axis = mul(qA, j->local_xaxis_a);
+  const real a1 = cross(dd + rA, axis);
+  const real a2 = cross(rB, axis);
+  const Vec2 perp = mul(qA, j->local_yaxis_a);
+  const real s1 = cross(dd + rA, perp);
+  const real s2 = cross(rB, perp);
+  Vec3 impulse;
+  const Vec2 C1(dot(perp, dd), aB - aA - j->reference_angle);
+  real linear_error = babs(C1.x);
+  const real angular_error = babs(C1.y);
+  bool active = false;
+  real C2 = 0;
+  if (j->enable_limit) {
+    const real translation = dot(axis, dd);
+    if (babs(j->upper_translation - j->lower_translation) < real(2.0) * kLinearSlop) {
+      C2 = translation;
+      linear_error = bmax(linear_error, babs(translation));
+      active = true;
+    } else if (translation <= j->lower_translation) {
+      C2 = bmin(translation - j->lower_translation, real(0));
+      linear_error = bmax(linear_error, j->lower_translation - translation);
+      active = true;
+    } else if (translation >= j->upper_translation) {
+      C2 = bmax(translation - j->upper_translation, real(0));
+      linear_error = bmax(linear_error, translation - j->upper_translation);
+      active = true;
+    }
+  }
+  if (active) {
+    const real k11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+    const real k12 = iA * s1 + iB * s2;
+    const real k13 = iA * s1 * a1 + iB * s2 * a2;
+    real k22 = iA + iB;
+    if (k22 == 0) k22 = 1;
+    const real k23 = iA * a1 + iB * a2;
+    const real k33 = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+    Mat33 K;
+    K.ex = Vec3(k11, k12, k13);
+    K.ey = Vec3(k12, k22, k23);
+    K.ez = Vec3(k13, k23, k33);
+    impulse = solve33(K, Vec3(-C1.x, -C1.y, -C2));
+  } else {
+    const real k11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+    const real k12 = iA * s1 + iB * s2;
+    real k22 = iA + iB;
+    if (k22 == 0) k22 = 1;
+    Mat22 K;
+    K.ex = Vec2(k11, k12);
+    K.ey = Vec2(k12, k22);
+    const Vec2 impulse1 = mat22_solve(K, -C1);
+    impulse = Vec3(impulse1.x, impulse1.y, 0);
+  }
+  const Vec2 P = impulse.x * perp + impulse.z * axis;
+  const real LA = impulse.x * s1 + impulse.y + impulse.z * a1;
+  const real LB = impulse.x * s2 + impulse.y + impulse.z * a2;
+  cA -= mA * P;
+  aA -= iA * LA;
+  cB += mB * P;
+  aB += iB * LB;
+  d.positions[j->index_a].c = cA;
+  d.positions[j->index_a].a = aA;
+  d.positions[j->index_b].c = cB;
+  d.positions[j->index_b].a = aB;
+  return linear_error <= kLinearSlop && angular_error <= kAngularSlop;
+}
+
 inline void joint_init_velocity(Joint* j, const JointSolverData& d) {
   if (j->type == kDistanceJoint) distance_init_velocity(j, d);
   else if (j->type == kWeldJoint) weld_init_velocity(j, d);
+  else if (j->type == kPrismaticJoint) prismatic_init_velocity(j, d);
   else revolute_init_velocity(j, d);
 }
 inline void joint_solve_velocity(Joint* j, const JointSolverData& d) {
   if (j->type == kDistanceJoint) distance_solve_velocity(j, d);
   else if (j->type == kWeldJoint) weld_solve_velocity(j, d);
+  else if (j->type == kPrismaticJoint) prismatic_solve_velocity(j, d);
   else revolute_solve_velocity(j, d);
 }
 inline bool joint_solve_position(Joint* j, const JointSolverData& d) {
   if (j->type == kDistanceJoint) return distance_solve_position(j, d);
   if (j->type == kWeldJoint) return weld_solve_position(j, d);
+  if (j->type == kPrismaticJoint) return prismatic_solve_position(j, d);
   return revolute_solve_position(j, d);
 }
 

@@ -113,7 +113,7 @@ struct ContactEdge {
 struct Filter { uint16_t category = 1, mask = 0xFFFF; int16_t group = 0; };
 
 // Joints: B2jointType values (src/b2_joint.rs:46-58) of the two restated kinds
-enum JointType { kDistanceJoint = 1, kRevoluteJoint = 8, kWeldJoint = 9 };
+enum JointType { kDistanceJoint = 1, kPrismaticJoint = 6, kRevoluteJoint = 8, kWeldJoint = 9 };
 struct Joint;
 // src/b2_joint.rs B2jointEdge; a body's list is push_front / newest-first
 struct JointEdge {
@@ -185,6 +185,10 @@ struct Joint {
   Vec2 impulse2;
   real motor_impulse = 0, axial_mass = 0, angle = 0;
   Mat22 K;
+  // prismatic (src/joints/b2_prismatic_joint.rs): axes in A's frame, translation limits, motor force
+  Vec2 local_xaxis_a, local_yaxis_a, axis, perp;
+  real lower_translation = 0, upper_translation = 0, max_motor_force = 0;
+  real s1 = 0, s2 = 0, a1 = 0, a2 = 0, translation = 0;
   // weld (src/joints/b2_weld_joint.rs): impulse (x, y, angular), the 3x3 mass (symmetric)
   real impulse3[3] = {0, 0, 0};
   real mass33[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // [column][row]

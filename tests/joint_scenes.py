@@ -99,3 +99,28 @@ def welds(stiffness, half=(1.0, 0.1), spacing=4.0, omega=0.0):
         j["local_anchor_b"] = (-half[0], 0.0)
         j["stiffness"] = k
     return bd, fb, sd, _fixture(n), jd
+
+
+def sliders(angles, half=(0.5, 0.25), spacing=4.0):
+    """One static body and, per entry of angles, a box on a prismatic joint whose axis (in the
+    static body's frame) makes that angle with the x axis, anchored at the box's centre (each its
+    own island)."""
+    n = len(angles)
+    bd = _bodies(1 + n)
+    bd[0]["type"] = 0
+    sd = np.zeros(n, b2g.SHAPE_DEF)
+    jd = np.zeros(n, b2g.JOINT_DEF)
+    fb = np.arange(1, n + 1, dtype=np.int32)
+    for i, ang in enumerate(angles):
+        anchor = np.array([spacing * i, 10.0])
+        bd[1 + i]["type"] = 2
+        bd[1 + i]["position"] = anchor
+        sd[i]["type"] = b2g.SHAPE_BOX
+        sd[i]["params"][:2] = half
+        j = jd[i]
+        j["type"] = b2g.JOINT_PRISMATIC
+        j["body_a"], j["body_b"] = 0, 1 + i
+        j["local_anchor_a"] = anchor
+        j["local_anchor_b"] = (0.0, 0.0)
+        j["local_axis_a"] = (np.cos(ang), np.sin(ang))
+    return bd, fb, sd, _fixture(n), jd

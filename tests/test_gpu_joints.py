@@ -5,8 +5,8 @@
   (device sin/cos may differ from glibc's by one ulp), bodies and joint impulses.
 * A hanging 12-link chain (joints in two colours against the reference's island order) stays
   within a band of the oracle's run over one second.
-* Welded cantilevers (rigid and soft, each alone in its island) agree with the oracle at every
-  step, as the pendulums do.
+* Welded cantilevers (rigid and soft) and prismatic sliders (inclined, limited, motorised), each
+  alone in its island, agree with the oracle at every step, as the pendulums do.
 * Motor and limit behave as the closed forms of the oracle's pins say; collide_connected filters
   pairs, also for a joint created between touching bodies; a pendulum at rest falls asleep with
   sleeping on; runs are deterministic.
@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import b2g
-from joint_scenes import chain, pendulums, welds
+from joint_scenes import chain, pendulums, sliders, welds
 
 pytestmark = pytest.mark.gpu
 
@@ -194,6 +194,29 @@ def test_reference_style_api():
     assert abs(np.linalg.norm(q - (3.0, 10.0)) - 2.0) < 0.01
 
 
+def test_reference_style_prismatic_and_weld():
+    world = b2g.B2world((0.0, -10.0))
+    ground = world.create_body(b2g.B2bodyDef())
+    box = b2g.B2polygonShape()
+    box.set_as_box(0.5, 0.25)
+    slider = world.create_body(b2g.B2bodyDef(body_type=b2g.B2_DYNAMIC_BODY, position=(0.0, 10.0)))
+    slider.create_fixture_by_shape(box, 1.0)
+    pj = b2g.B2prismaticJointDef()
+    pj.initialize(ground, slider, (0.0, 10.0), (1.0, 0.0))
+    pj.enable_motor, pj.motor_speed, pj.max_motor_force = True, 2.0, 1e4
+    world.create_joint(pj)
+    beam = world.create_body(b2g.B2bodyDef(body_type=b2g.B2_DYNAMIC_BODY, position=(6.0, 10.0)))
+    beam.create_fixture_by_shape(box, 1.0)
+    wj = b2g.B2weldJointDef()
+    wj.initialize(ground, beam, (5.5, 10.0))
+    world.create_joint(wj)
+    for _ in range(60):
+        world.step(1.0 / 60.0, 8, 3)
+    assert abs(slider.get_linear_velocity()[0] - 2.0) < 1e-4
+    assert abs(slider.get_position()[1] - 10.0) < 0.005
+    assert abs(beam.get_angle()) < 0.05 and abs(beam.get_position()[1] - 10.0) < 0.01
+
+
 def test_welds_match_oracle(oracle):
     bd, fb, sd, fd, jd = welds([0.0, 5.0, 0.0, 40.0], omega=0.5)
     jd[1]["damping"] = 0.3
@@ -208,4 +231,22 @@ def test_welds_match_oracle(oracle):
         ri = np.asarray(r.joints()["impulse"], np.float64)[:, :3]
         err = np.abs(gi - ri).max()
         assert err <= 1e-5 * max(1.0, np.abs(ri).max()), "step %d: weld impulses differ by %g" % (step, err)
+
+
+def test_sliders_match_oracle(oracle):
+    bd, fb, sd, fd, jd = sliders([-np.pi / 6, np.pi / 2 + 0.3, -np.pi / 4, 0.2])
+    bd[2]["angular_velocity"] = 1.0  # starts turning: the joint takes it back
+    jd[2]["enable_limit"], jd[2]["lower_angle"], jd[2]["upper_angle"] = 1, -1.0, 0.5
+    jd[3]["enable_motor"], jd[3]["motor_speed"], jd[3]["max_motor_torque"] = 1, -1.5, 30.0
+    jd[3]["reference_angle"] = 0.1
+    g = _gpu(bd, fb, sd, fd, jd)
+    r = _cpu(oracle, bd, fb, sd, fd, jd)
+    for step in range(150):
+        g.step(DT, 8, 3)
+        r.step(DT, 8, 3)
+        _close(g.body_states(), r.bodies(), 1e-5, "step %d" % step)
+        gi = g.joint_impulses()
+        ri = np.asarray(r.joints()["impulse"], np.float64)
+        err = np.abs(gi - ri).max()
+        assert err <= 1e-5 * max(1.0, np.abs(ri).max()), "step %d: slider impulses differ by %g" % (step, err)
 

@@ -18,7 +18,11 @@ src/private/dynamics/b2_body.rs:391-416) by what mechanics fixes:
     a joint created between touching bodies removes their contact at the next step;
   * weld: a rigid cantilever holds its angle and anchor within the slops under gravity; a soft
     weld (angular stiffness k) swings about its anchor with period 2 pi sqrt(I_anchor / k)
-    (within 2%), I_anchor = I + m d^2 about the anchor.
+    (within 2%), I_anchor = I + m d^2 about the anchor;
+  * prismatic: a box on an axis inclined by theta slides down it as the integrator's closed form
+    of frictionless acceleration g sin(theta) says (within 0.5%), without turning or leaving the
+    axis; its translation limits hold within the slop; a motor of ample force drives it along
+    the axis at the motor speed.
 A slip such as a wrong sign in the block solve, a dropped inertia term or a swapped limit
 fails one of these.
 """
@@ -26,7 +30,7 @@ import numpy as np
 import pytest
 
 import b2g
-from joint_scenes import chain, pendulums, welds
+from joint_scenes import chain, pendulums, sliders, welds
 
 DT = np.float32(1.0 / 60.0)
 G = 10.0
@@ -191,4 +195,40 @@ def test_weld_soft_angular_period(oracle):
     assert len(zc) >= 5
     period = 2 * np.mean(np.diff(zc))
     assert abs(period / T - 1) < 0.02, (period, T)
+
+
+def _slide(st, angle, anchor):
+    d = np.array(st["c"], np.float64) - anchor
+    return d @ np.array([np.cos(angle), np.sin(angle)]), d @ np.array([-np.sin(angle), np.cos(angle)])
+
+
+def test_prismatic_incline(oracle):
+    angles = [-np.pi / 6, np.pi / 2 + 0.3]
+    bd, fb, sd, fd, jd = sliders(angles)
+    w = _oracle_world(oracle, bd, fb, sd, fd, jd)
+    h = float(DT)
+    n = 60
+    for _ in range(n):
+        w.step(DT, 8, 3)
+    for i, ang in enumerate(angles):
+        st = w.bodies()[1 + i]
+        s_along, s_perp = _slide(st, ang, np.array([4.0 * i, 10.0]))
+        # symplectic Euler from rest under the axial component of gravity: s_n = h^2 a n (n + 1) / 2
+        expect = h * h * (-G * np.sin(ang)) * n * (n + 1) / 2
+        assert abs(s_along / expect - 1) < 0.005, (s_along, expect)
+        assert abs(s_perp) < 0.005 and abs(float(st["a"])) < 2.0 / 180.0 * np.pi
+
+
+def test_prismatic_limit_and_motor(oracle):
+    bd, fb, sd, fd, jd = sliders([-np.pi / 4, 0.0])
+    jd[0]["enable_limit"], jd[0]["lower_angle"], jd[0]["upper_angle"] = 1, -1.0, 0.5
+    jd[1]["enable_motor"], jd[1]["motor_speed"], jd[1]["max_motor_torque"] = 1, 1.5, 1e4
+    w = _oracle_world(oracle, bd, fb, sd, fd, jd)
+    for _ in range(120):
+        w.step(DT, 8, 3)
+        st = w.bodies()
+        s0, _ = _slide(st[1], -np.pi / 4, np.array([0.0, 10.0]))
+        assert -1.0 - 0.01 <= s0 <= 0.5 + 0.01, s0
+        assert abs(float(st[2]["v"][0]) - 1.5) < 1e-5
+    assert s0 > 0.5 - 0.02  # it slid down to the upper limit (the axis points down the slope)
 
