@@ -125,6 +125,7 @@ def _load():
         "b2g_world_get_destroyed_contacts": ([P, I, P, P, P], I),
         "b2g_world_add_joints": ([P, I, P], I),
         "b2g_world_get_joint_impulses": ([P, P, I], I),
+        "b2g_world_set_joint_impulses": ([P, P, I], I),
         "b2g_world_get_solver_state": ([P, P], I),
         "b2g_world_set_solver_state": ([P, P], I),
     }
@@ -208,6 +209,16 @@ class World:
         jd = _as(joint_defs, JOINT_DEF)
         _check(_lib.b2g_world_add_joints(self._h, len(jd), _ptr(jd)))
         self._n_joints = getattr(self, "_n_joints", 0) + len(jd)
+        self._joint_defs = getattr(self, "_joint_defs", []) + [jd.copy()]
+
+    def joint_definitions(self):
+        """The joint definitions handed to add_joints, in creation order."""
+        parts = getattr(self, "_joint_defs", [])
+        return np.concatenate(parts) if parts else np.zeros(0, JOINT_DEF)
+
+    def set_joint_impulses(self, impulses):
+        imp = np.ascontiguousarray(impulses, np.float32).reshape(-1, 5)
+        _check(_lib.b2g_world_set_joint_impulses(self._h, _ptr(imp), len(imp)))
 
     @property
     def joint_count(self):

@@ -36,19 +36,33 @@ reader rejects unknown members (its field enums have no catch-all variant,
 src/serialize/serialize_b2_world.rs:189-198), so only a file written without the extension is
 for the reference.
 
-Joints, chain shapes and disabled bodies are not part of the GPU path: reading them raises.
+Joints (distance, revolute, weld, prismatic) are ``{"jtype", "joint_def"}`` records in the world's
+joint list, newest first (src/serialize/serialize_b2_world.rs:58-129), each definition with its
+``base`` (``jtype``, ``user_data``, ``body_a``, ``body_b``, ``collide_connected``,
+src/serialize/serialize_b2_joint.rs:33-53) and the kind's fields in their serializer's order
+(src/joints/serialize/serialize_b2_*_joint.rs). A joint names its bodies by index: the writer
+gives a body's position in the written body list (serialize_b2_world.rs:135-138); the
+reference's reader looks that index up in the list of the world it has just built, which is in
+the opposite order (serialize_b2_joint.rs:88-150 over a newest-first list), so a round trip
+through the reference attaches a joint to the mirrored body. ``read_world`` follows the
+reference's reader by default (``joint_bodies="reference"``); ``joint_bodies="file"`` takes the
+index as the writer meant it. Other joint kinds, chain shapes and disabled bodies are not part
+of the GPU path: reading them raises.
 """
 import json
 from fractions import Fraction
 
 import numpy as np
 
-from . import (BODY_DEF, FIXTURE_DEF, MANIFOLD, SHAPE_DEF, SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON_EXACT, B2gError,
-               ERR_UNSUPPORTED, World)
+from . import (BODY_DEF, FIXTURE_DEF, JOINT_DEF, JOINT_DISTANCE, JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_WELD, MANIFOLD,
+               SHAPE_DEF, SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON_EXACT, B2gError, ERR_UNSUPPORTED, World)
 
 BODY_TYPES = ("B2StaticBody", "B2KinematicBody", "B2DynamicBody")          # src/b2_body.rs:25-31
 SHAPE_TYPES = ("ECircle", "EEdge", "EPolygon", "EChain")                  # src/b2_shape.rs:23-31
 MANIFOLD_TYPES = ("ECircles", "EFaceA", "EFaceB")                         # src/b2_collision.rs:63-67
+JOINT_TYPES = {JOINT_DISTANCE: "EDistanceJoint", JOINT_PRISMATIC: "EPrismaticJoint",  # src/b2_joint.rs:46-58
+               JOINT_REVOLUTE: "ERevoluteJoint", JOINT_WELD: "EWeldJoint"}
+LINEAR_SLOP = np.float32(0.005)
 
 
 # ---- f32 numbers ------------------------------------------------------------------------------
@@ -152,7 +166,44 @@ def _manifold_json(m):
             "m_type": MANIFOLD_TYPES[int(m["type"])], "point_count": int(m["point_count"])}
 
 
-def defs_to_json(gravity, body_defs, fixture_body, shape_defs, fixture_defs, states=None, extension=None):
+def _joint_json(d, nb):
+    """A joint record; body indices are positions in the written (newest-first) body list."""
+    t = int(d["type"])
+    base = {"jtype": JOINT_TYPES[t], "user_data": None, "body_a": nb - 1 - int(d["body_a"]),
+            "body_b": nb - 1 - int(d["body_b"]), "collide_connected": bool(d["collide_connected"])}
+    out = {"base": base, "local_anchor_a": _vec(*d["local_anchor_a"]), "local_anchor_b": _vec(*d["local_anchor_b"])}
+    if t == JOINT_DISTANCE:
+        # the joint's lengths as its constructor clamps them (b2_distance_joint.rs:44-78)
+        length = max(np.float32(d["length"]), LINEAR_SLOP)
+        lo = max(np.float32(d["min_length"]), LINEAR_SLOP)
+        hi = max(np.float32(d["max_length"]), lo)
+        out.update({"length": F32(length), "min_length": F32(lo), "max_length": F32(hi),
+                    "stiffness": F32(d["stiffness"]), "damping": F32(d["damping"])})
+    elif t == JOINT_REVOLUTE:
+        out.update({"reference_angle": F32(d["reference_angle"]), "enable_limit": bool(d["enable_limit"]),
+                    "lower_angle": F32(d["lower_angle"]), "upper_angle": F32(d["upper_angle"]),
+                    "enable_motor": bool(d["enable_motor"]), "motor_speed": F32(d["motor_speed"]),
+                    "max_motor_torque": F32(d["max_motor_torque"])})
+    elif t == JOINT_WELD:
+        out.update({"reference_angle": F32(d["reference_angle"]), "stiffness": F32(d["stiffness"]),
+                    "damping": F32(d["damping"])})
+    elif t == JOINT_PRISMATIC:
+        # the axis as the constructor normalises it (B2vec2::normalize, src/b2_math.rs:82-92)
+        ax = np.array(d["local_axis_a"], np.float32)
+        length = np.sqrt(ax[0] * ax[0] + ax[1] * ax[1])
+        if length >= np.finfo(np.float32).eps:
+            inv = np.float32(1.0) / length
+            ax = np.array([ax[0] * inv, ax[1] * inv], np.float32)
+        out.update({"local_axis_a": _vec(*ax), "reference_angle": F32(d["reference_angle"]),
+                    "enable_limit": bool(d["enable_limit"]), "lower_translation": F32(d["lower_angle"]),
+                    "upper_translation": F32(d["upper_angle"]), "enable_motor": bool(d["enable_motor"]),
+                    "motor_speed": F32(d["motor_speed"]), "max_motor_force": F32(d["max_motor_torque"])})
+    else:
+        raise B2gError(ERR_UNSUPPORTED, "serde world: joint kind %d" % t)
+    return {"jtype": JOINT_TYPES[t], "joint_def": out}
+
+
+def defs_to_json(gravity, body_defs, fixture_body, shape_defs, fixture_defs, states=None, extension=None, joints=None):
     """serde world JSON of bodies and fixtures given in creation order. shape_defs hold the shapes
     as stored (circles, edges, SHAPE_POLYGON_EXACT polygons); states (BODY_STATE, optional) give
     the current transform, velocities and awake flags, else the definitions' own."""
@@ -198,7 +249,8 @@ def defs_to_json(gravity, body_defs, fixture_body, shape_defs, fixture_defs, sta
                              "group_index": int(x["group"])},
                 "m_shape_type": kind, "m_shape": shape})
         bodies.append({"m_definition": definition, "m_fixture_list": fixtures})
-    out = {"m_gravity": _vec(*gravity), "m_bodies_list": bodies, "m_joints_list": [], "m_gear_joints_list": []}
+    jl = [] if joints is None else [_joint_json(joints[i], len(bd)) for i in reversed(range(len(joints)))]
+    out = {"m_gravity": _vec(*gravity), "m_bodies_list": bodies, "m_joints_list": jl, "m_gear_joints_list": []}
     if extension is not None:
         out["b2g_checkpoint"] = extension(per_body)
     return _dump(out) + "\n"
@@ -209,11 +261,10 @@ def world_to_json(world, checkpoint=False):
     b2g_checkpoint extension is added."""
     if world._batched:
         raise B2gError(ERR_UNSUPPORTED, "serde world: a batch of worlds has no single-world record")
-    if world.joint_count:
-        raise B2gError(ERR_UNSUPPORTED, "serde world: joints are not written (their serde records are not restated)")
     bd, fbody, fd = world.definitions()
     return defs_to_json(world.gravity, bd, fbody, world.shapes(), fd, states=world.body_states(),
-                        extension=(lambda per_body: _checkpoint(world, fbody, per_body)) if checkpoint else None)
+                        extension=(lambda per_body: _checkpoint(world, fbody, per_body)) if checkpoint else None,
+                        joints=world.joint_definitions() if world.joint_count else None)
 
 
 def _checkpoint(world, fbody, per_body):
@@ -239,6 +290,11 @@ def _checkpoint(world, fbody, per_body):
                       "manifold": _manifold_json(man[k])} for k in range(len(fab))],
         "destroyed": [{"fixture_a": int(dfab[k, 0]), "fixture_b": int(dfab[k, 1]), "manifold": _manifold_json(dman[k])}
                       for k in range(len(dfab))],
+        "joint_index": list(reversed(range(world.joint_count))),
+        # prismatic axes as given (the record holds the normalised axis; normalising it again could
+        # move a last bit)
+        "joint_axes": [_vec(*a) for a in world.joint_definitions()["local_axis_a"]] if world.joint_count else [],
+        "joint_impulses": [[F32(v) for v in row] for row in (world.joint_impulses() if world.joint_count else [])],
         "solver": {"inv_dt0": F32(solver["inv_dt0"]), "n_colours": int(solver["n_colours"]),
                    "scratch_colours": int(solver["scratch_colours"]),
                    "steps_since_scratch": int(solver["steps_since_scratch"])},
@@ -250,15 +306,55 @@ def _loads(text):
     return json.loads(text, parse_float=_parse_f32, parse_int=int)
 
 
-def read_world(text):
+def _read_joint(rec, nb, joint_bodies):
+    kind = rec["jtype"]
+    names = {v: k for k, v in JOINT_TYPES.items()}
+    if kind not in names:
+        raise B2gError(ERR_UNSUPPORTED, "serde world: %s is not on the GPU path" % kind)
+    d = rec["joint_def"]
+    base = d["base"]
+    j = np.zeros((), JOINT_DEF)
+    j["type"] = names[kind]
+    ia, ib = int(base["body_a"]), int(base["body_b"])
+    if joint_bodies == "reference":
+        # the reference's reader: the index into the newly built world's newest-first body list,
+        # whose bodies were created in file order
+        ia, ib = nb - 1 - ia, nb - 1 - ib
+    j["body_a"], j["body_b"] = ia, ib
+    j["collide_connected"] = int(bool(base["collide_connected"]))
+    j["local_anchor_a"] = _getvec(d["local_anchor_a"])
+    j["local_anchor_b"] = _getvec(d["local_anchor_b"])
+    f = _parse_f32_any
+    if kind == "EDistanceJoint":
+        for key in ("length", "min_length", "max_length", "stiffness", "damping"):
+            j[key] = f(d[key])
+    elif kind == "ERevoluteJoint":
+        for key in ("reference_angle", "lower_angle", "upper_angle", "motor_speed", "max_motor_torque"):
+            j[key] = f(d[key])
+        j["enable_limit"], j["enable_motor"] = int(bool(d["enable_limit"])), int(bool(d["enable_motor"]))
+    elif kind == "EWeldJoint":
+        for key in ("reference_angle", "stiffness", "damping"):
+            j[key] = f(d[key])
+    else:
+        j["local_axis_a"] = _getvec(d["local_axis_a"])
+        j["reference_angle"] = f(d["reference_angle"])
+        j["lower_angle"], j["upper_angle"] = f(d["lower_translation"]), f(d["upper_translation"])
+        j["motor_speed"], j["max_motor_torque"] = f(d["motor_speed"]), f(d["max_motor_force"])
+        j["enable_limit"], j["enable_motor"] = int(bool(d["enable_limit"])), int(bool(d["enable_motor"]))
+    return j
+
+
+def read_world(text, joint_bodies="reference"):
     """Parse serde world JSON -> dict with gravity, body_defs, fixture_body, shape_defs,
-    fixture_defs (in the order the reference's reader creates them: file order) and the
-    checkpoint extension (or None)."""
+    fixture_defs, joint_defs (in the order the reference's reader creates them: file order) and
+    the checkpoint extension (or None). joint_bodies: "reference" resolves a joint's body index
+    as the reference's reader does, "file" as the writer meant it (a position in the file's body
+    list)."""
+    if joint_bodies not in ("reference", "file"):
+        raise ValueError("joint_bodies: 'reference' or 'file'")
     doc = _loads(text)
-    for key in ("m_joints_list", "m_gear_joints_list"):
-        if doc.get(key):
-            raise B2gError(ERR_UNSUPPORTED, "serde world: joints are not on the GPU path (%d in %s)"
-                           % (len(doc[key]), key))
+    if doc.get("m_gear_joints_list"):
+        raise B2gError(ERR_UNSUPPORTED, "serde world: gear joints are not on the GPU path")
     bodies = doc.get("m_bodies_list", [])
     bd = np.zeros(len(bodies), BODY_DEF)
     fixture_body, sds, fds = [], [], []
@@ -293,8 +389,10 @@ def read_world(text):
             fixture_body.append(i)
     sd = np.array(sds, SHAPE_DEF) if sds else np.zeros(0, SHAPE_DEF)
     fd = np.array(fds, FIXTURE_DEF) if fds else np.zeros(0, FIXTURE_DEF)
+    jl = doc.get("m_joints_list", [])
+    jd = np.array([_read_joint(r, len(bodies), joint_bodies) for r in jl], JOINT_DEF) if jl else np.zeros(0, JOINT_DEF)
     return {"gravity": _getvec(doc["m_gravity"]), "body_defs": bd, "fixture_body": np.array(fixture_body, np.int32),
-            "shape_defs": sd, "fixture_defs": fd, "checkpoint": doc.get("b2g_checkpoint")}
+            "shape_defs": sd, "fixture_defs": fd, "joint_defs": jd, "checkpoint": doc.get("b2g_checkpoint")}
 
 
 def _read_shape(kind, s):
@@ -326,16 +424,20 @@ def _read_shape(kind, s):
     return out
 
 
-def world_from_json(text):
+def world_from_json(text, joint_bodies="reference"):
     """A GPU world from serde world JSON. Without the checkpoint extension: the reference's
-    reader (bodies and fixtures created in file order, fresh proxies and contacts). With it: the
-    written world's creation order and its full state, so that stepping continues exactly."""
-    doc = read_world(text)
+    reader (bodies, fixtures and joints created in file order, fresh proxies and contacts; joint
+    body indices resolved as joint_bodies says). With it: the written world's creation order and
+    its full state, so that stepping continues exactly."""
+    ck_probe = _loads(text).get("b2g_checkpoint")
+    doc = read_world(text, joint_bodies="file" if ck_probe is not None else joint_bodies)
     ck = doc["checkpoint"]
     w = World(doc["gravity"])
     if ck is None:
         w.set_flags(warm_starting=True, allow_sleep=True, continuous=False, block_solve=True)
         w.add(doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"])
+        if len(doc["joint_defs"]):
+            w.add_joints(doc["joint_defs"])
         return w
     fl = ck["flags"]
     w.set_flags(warm_starting=fl["warm_starting"], allow_sleep=fl["allow_sleep"], continuous=fl["continuous"],
@@ -354,6 +456,18 @@ def world_from_json(text):
     fd[fixture_index] = doc["fixture_defs"]
     fb[fixture_index] = body_index[doc["fixture_body"]]
     w.add(bd, fb, sd, fd)
+    jdf = doc["joint_defs"]
+    if len(jdf):
+        # joints in their creation order, bodies as creation indices
+        joint_index = np.asarray(ck["joint_index"], np.int64)
+        jd = np.zeros(len(jdf), JOINT_DEF)
+        jd[joint_index] = jdf
+        jd["body_a"] = body_index[jd["body_a"]]
+        jd["body_b"] = body_index[jd["body_b"]]
+        jd["local_axis_a"] = np.array([_getvec(a) for a in ck["joint_axes"]], np.float32).reshape(-1, 2)
+        w.add_joints(jd)
+        w.set_joint_impulses(np.array([[_parse_f32_any(v) for v in row] for row in ck["joint_impulses"]],
+                                      np.float32).reshape(-1, 5))
     st = w.body_states()
     for i, s in enumerate(ck["bodies"]):
         st[i]["p"] = _getvec(s["p"])

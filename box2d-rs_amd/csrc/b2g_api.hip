@@ -898,6 +898,26 @@ int b2g_world_get_joint_impulses(b2g_world* h, float* out, int32_t n) {
   return B2G_OK;
 }
 
+int b2g_world_set_joint_impulses(b2g_world* h, const float* in, int32_t n) {
+  if (!h || n != h->w.nj || (n > 0 && !in)) {
+    set_error("b2g_world_set_joint_impulses: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  std::vector<float4> imp(n);
+  std::vector<float> imp2(n);
+  for (int i = 0; i < n; ++i) {
+    const float* o = in + 5 * i;
+    imp[i] = make_float4(o[0], o[1], o[2], o[3]);
+    imp2[i] = o[4];
+  }
+  if ((st = up(w->j.imp, imp, w->stream)) || (st = up(w->j.imp2, imp2, w->stream))) return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  return B2G_OK;
+}
+
 int b2g_world_get_shapes(b2g_world* h, b2g_shape_def* out, int32_t n) {
   if (!h || !out || n != h->w.np) { set_error("b2g_world_get_shapes: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;

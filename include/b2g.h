@@ -220,6 +220,9 @@ int b2g_world_add_joints(b2g_world* world, int32_t n, const b2g_joint_def* defs)
 /* Warm-start impulses of every joint, 5 floats each (distance: impulse, lower, upper, 0, 0;
  * revolute and prismatic: impulse.x, impulse.y, motor, lower, upper; weld: impulse.x, .y, .z, 0, 0). */
 int b2g_world_get_joint_impulses(b2g_world* world, float* out, int32_t n);
+/* Overwrites them (same layout): the checkpoint extension of the serde world format restores a
+ * world's joints with their impulses. */
+int b2g_world_set_joint_impulses(b2g_world* world, const float* in, int32_t n);
 
 /* The shape of every fixture as the device holds it (after the hull, box and edge setters):
  * circles as B2G_SHAPE_CIRCLE, edges as B2G_SHAPE_EDGE (flags bit0 = one-sided), polygons as

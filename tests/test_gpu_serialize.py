@@ -8,7 +8,10 @@
   flags (src/serialize/serialize_b2_body.rs:37-56).
 * Checkpoint: a world written with the checkpoint extension after some steps and read back
   continues bit-identically with the written one (body states, contacts and impulses after more
-  steps), with sleeping off and on; written again right after reading, its text is the same.
+  steps), with sleeping off and on, and with joints (hanging chains; prismatic and weld joints);
+  written again right after reading, its text is the same.
+* Joints read from a file (indices as the writer meant them) step bit-identically with the same
+  joints created directly in file order.
 """
 import numpy as np
 import pytest
@@ -104,3 +107,70 @@ def test_checkpoint_continues_bit_identically(kind, sleep):
     np.testing.assert_array_equal(fa, fb_)
     np.testing.assert_array_equal(ta, tb)
     assert ma.tobytes() == mb.tobytes()
+
+
+def _joint_world(kind):
+    import b2g
+    from b2g import scenes
+    if kind == "chains":
+        scene, jd = scenes.chains(32, 8, spacing=1.5, rows=4)
+        return scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs, jd
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from joint_scenes import sliders, welds
+    a = sliders([-0.5, 0.3, 1.2])
+    b = welds([0.0, 5.0])
+    nb = len(a[0])
+    bd = np.concatenate([a[0], b[0]])
+    fb = np.concatenate([a[1], b[1] + nb]).astype(np.int32)
+    jd = np.concatenate([a[4], b[4]])
+    jd["body_a"][len(a[4]):] += nb
+    jd["body_b"][len(a[4]):] += nb
+    jd[0]["enable_limit"], jd[0]["lower_angle"], jd[0]["upper_angle"] = 1, -0.5, 0.4
+    jd[1]["enable_motor"], jd[1]["motor_speed"], jd[1]["max_motor_torque"] = 1, 0.7, 50.0
+    return bd, fb, np.concatenate([a[2], b[2]]), np.concatenate([a[3], b[3]]), jd
+
+
+@pytest.mark.parametrize("kind", ["chains", "mixed"])
+def test_checkpoint_with_joints(kind):
+    import b2g
+    bd, fb, sd, fd, jd = _joint_world(kind)
+    w1 = b2g.World()
+    w1.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
+    w1.add(bd, fb, sd, fd)
+    w1.add_joints(jd)
+    for _ in range(45):
+        w1.step(DT, 8, 3)
+    text = ser.world_to_json(w1, checkpoint=True)
+    w2 = ser.world_from_json(text)
+    assert ser.world_to_json(w2, checkpoint=True) == text
+    for _ in range(60):
+        w1.step(DT, 8, 3)
+        w2.step(DT, 8, 3)
+    _states_equal(w1.body_states(), w2.body_states(), "after 60 more steps")
+    np.testing.assert_array_equal(w1.joint_impulses(), w2.joint_impulses())
+
+
+def test_read_joints_step_like_direct_creation():
+    import b2g
+    bd, fb, sd, fd, jd = _joint_world("chains")
+    w0 = b2g.World()
+    w0.add(bd, fb, sd, fd)
+    w0.add_joints(jd)
+    text = ser.world_to_json(w0)
+    w1 = ser.world_from_json(text, joint_bodies="file")
+    doc = ser.read_world(text, joint_bodies="file")
+    w2 = b2g.World()
+    w2.set_flags(warm_starting=True, allow_sleep=True, continuous=False, block_solve=True)
+    w2.add(doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"])
+    w2.add_joints(doc["joint_defs"])
+    # the file's joints are the world's, newest first, on the same bodies
+    nb = len(bd)
+    back = jd[::-1]
+    np.testing.assert_array_equal(doc["joint_defs"]["body_a"], nb - 1 - back["body_a"])
+    np.testing.assert_array_equal(doc["joint_defs"]["body_b"], nb - 1 - back["body_b"])
+    for _ in range(60):
+        w1.step(DT, 8, 3)
+        w2.step(DT, 8, 3)
+    _states_equal(w1.body_states(), w2.body_states(), "read vs direct")
+

@@ -11,11 +11,17 @@
   written by the reference itself exists in its tree (its testbed writes them at run time,
   examples/testbed/test_serialize.rs:22-54), so the byte layout against serde_json's own output
   is parity unpinned; values are pinned (f32 text round trip below).
+* Joints: tests/golden/serde_world_joints.json (a distance and a revolute joint, written by hand
+  from src/serialize/serialize_b2_joint.rs:33-53 and src/joints/serialize/serialize_b2_distance_joint.rs:44-51,
+  serialize_b2_revolute_joint.rs:47-58) reads into the expected joint definitions and writes back
+  byte for byte; its body indices resolve as the reference's reader resolves them (mirrored in
+  the newest-first list of the world it builds, serialize_b2_joint.rs:88-150) or as the writer
+  meant them.
 * Numbers: every finite f32 bit pattern drawn at random survives text -> value exactly.
 * Exact polygons: a polygon stored as vertices, normals and centroid (what the format holds)
   builds the same oracle shape and mass as the hull it came from (B2polygonShape::set,
   src/private/collision/b2_polygon_shape.rs:96-211), bit for bit.
-* What the GPU path does not hold (joints, chains, disabled bodies) is refused.
+* What the GPU path does not hold (wheel and other joint kinds, chains, disabled bodies) is refused.
 """
 import json
 import os
@@ -27,6 +33,7 @@ from b2g import serialize as ser
 import b2g
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "serde_world_small.json")
+GOLDEN_JOINTS = os.path.join(os.path.dirname(__file__), "golden", "serde_world_joints.json")
 
 
 def test_golden_layout_reads_and_writes_back():
@@ -149,10 +156,40 @@ def test_f32_text_round_trip():
 def test_unsupported_members_are_refused(what):
     doc = json.loads(open(GOLDEN).read())
     if what == "joint":
-        doc["m_joints_list"] = [{"jtype": "EDistanceJoint", "joint_def": {}}]
+        doc["m_joints_list"] = [{"jtype": "EWheelJoint", "joint_def": {}}]
     elif what == "chain":
         doc["m_bodies_list"][1]["m_fixture_list"][0]["m_shape_type"] = "EChain"
     else:
         doc["m_bodies_list"][0]["m_definition"]["enabled"] = False
     with pytest.raises(b2g.B2gError):
         ser.read_world(json.dumps(doc))
+
+
+def test_golden_joints_read_and_write_back():
+    text = open(GOLDEN_JOINTS).read()
+    doc = ser.read_world(text, joint_bodies="file")
+    bd, fb, sd, fd, jd = doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"], doc["joint_defs"]
+    assert list(bd["type"]) == [2, 2, 0] and list(fb) == [0, 1]
+    assert list(jd["type"]) == [b2g.JOINT_DISTANCE, b2g.JOINT_REVOLUTE]
+    assert (jd["body_a"][0], jd["body_b"][0], jd["body_a"][1], jd["body_b"][1]) == (1, 0, 2, 1)
+    assert list(jd["collide_connected"]) == [0, 1]
+    np.testing.assert_array_equal(jd["length"][0], np.float32(2.828427))
+    assert (jd["min_length"][0], jd["max_length"][0]) == (np.float32(2.0), np.float32(3.5))
+    assert (jd["stiffness"][0], jd["damping"][0]) == (np.float32(4.0), np.float32(0.5))
+    np.testing.assert_array_equal(jd["local_anchor_b"][1], np.float32([-1.0, 0.0]))
+    assert jd["enable_limit"][1] == 1 and jd["enable_motor"][1] == 0
+    assert (jd["lower_angle"][1], jd["upper_angle"][1]) == (np.float32(-0.5), np.float32(0.25))
+    # the reference's reader: index i names the i-th body of the new world's newest-first list
+    ref = ser.read_world(text)["joint_defs"]
+    assert (ref["body_a"][0], ref["body_b"][0], ref["body_a"][1], ref["body_b"][1]) == (1, 2, 0, 1)
+    # written back in creation order (bodies, fixtures and joints the reverse of the file's)
+    n = len(bd)
+    order = np.arange(n)[::-1]
+    remap = np.empty_like(order)
+    remap[order] = np.arange(n)
+    jw = jd[::-1].copy()
+    jw["body_a"] = remap[jw["body_a"]]
+    jw["body_b"] = remap[jw["body_b"]]
+    out = ser.defs_to_json(doc["gravity"], bd[order], remap[fb][::-1], sd[::-1], fd[::-1], joints=jw)
+    assert out == text
+
