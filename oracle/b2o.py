@@ -89,6 +89,11 @@ class Oracle:
         L.b2o_shape_mass.argtypes = [P, R, P]
         L.b2o_shape_aabb.argtypes = [P, P, P]
         L.b2o_collide.argtypes = [I, P, P, P, P, P]
+        L.b2o_distance.argtypes = [I, P, P, P, P, I, P]
+        L.b2o_time_of_impact.argtypes = [I, P, P, P, P, R, P]
+        L.b2o_world_set_continuous.argtypes = [P, I]
+        L.b2o_world_toi_events.argtypes = [P]
+        L.b2o_world_toi_events.restype = I
         L.b2o_world_manifold.argtypes = [I, P, P, P, P, P, P]
         L.b2o_rot.argtypes = [I, P, P]
         L.b2o_brute_pairs.argtypes = [I, P, ctypes.c_longlong, P]
@@ -139,6 +144,31 @@ class Oracle:
         out = np.zeros(n, self.dt["manifold"])
         self.lib.b2o_collide(n, _ptr(sa), _ptr(xa), _ptr(sb), _ptr(xb), _ptr(out))
         return out
+
+    def distance(self, shapes_a, xf_a, shapes_b, xf_b, use_radii=False):
+        """GJK distance (src/private/collision/b2_distance.rs:404): rows of (distance, point_a,
+        point_b, iterations)."""
+        n = len(shapes_a)
+        xa = np.ascontiguousarray(xf_a, self.real).reshape(n, 4)
+        xb = np.ascontiguousarray(xf_b, self.real).reshape(n, 4)
+        sa = np.ascontiguousarray(shapes_a)
+        sb = np.ascontiguousarray(shapes_b)
+        out = np.zeros((n, 6), self.real)
+        self.lib.b2o_distance(n, _ptr(sa), _ptr(xa), _ptr(sb), _ptr(xb), int(use_radii), _ptr(out))
+        return out
+
+    def time_of_impact(self, shapes_a, sweep_a, shapes_b, sweep_b, t_max=1.0):
+        """Conservative advancement (src/private/collision/b2_time_of_impact.rs:232). A sweep row is
+        (local_center(2), c0(2), c(2), a0, a, alpha0). Returns (state, t) arrays; states as
+        B2toioutputState: 1 failed, 2 overlapped, 3 touching, 4 separated."""
+        n = len(shapes_a)
+        wa = np.ascontiguousarray(sweep_a, self.real).reshape(n, 9)
+        wb = np.ascontiguousarray(sweep_b, self.real).reshape(n, 9)
+        sa = np.ascontiguousarray(shapes_a)
+        sb = np.ascontiguousarray(shapes_b)
+        out = np.zeros((n, 2), self.real)
+        self.lib.b2o_time_of_impact(n, _ptr(sa), _ptr(wa), _ptr(sb), _ptr(wb), t_max, _ptr(out))
+        return out[:, 0].astype(np.int32), out[:, 1].copy()
 
     def world_manifold(self, manifolds, xf_a, ra, xf_b, rb):
         n = len(manifolds)
@@ -203,8 +233,14 @@ class OracleWorld:
             self.o.lib.b2o_world_free(self.h)
             self.h = None
 
-    def set_flags(self, warm_starting=True, allow_sleep=False, block_solve=True):
+    def set_flags(self, warm_starting=True, allow_sleep=False, block_solve=True, continuous=False):
         self.o.lib.b2o_world_set_flags(self.h, int(warm_starting), int(allow_sleep), int(block_solve))
+        self.o.lib.b2o_world_set_continuous(self.h, int(continuous))
+
+    @property
+    def toi_events(self):
+        """TOI events solved in the last step."""
+        return self.o.lib.b2o_world_toi_events(self.h)
 
     def add(self, body_defs, fixture_body, shape_defs, fixture_defs):
         bd = self.o.cast(body_defs, "body_def")

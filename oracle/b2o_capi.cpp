@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "b2o_world.h"
+#include "b2o_toi.h"
 #include "b2o_joints.h"
 
 using namespace b2o;
@@ -268,6 +269,59 @@ void b2o_collide(int n, const B2oShape* shapes_a, const real* xf_a, const B2oSha
   }
 }
 
+// GJK distance of n shape pairs at transforms (p.x, p.y, sin, cos), with or without radii.
+// out per pair: distance, point_a (2), point_b (2), iterations.
+void b2o_distance(int n, const B2oShape* shapes_a, const real* xf_a, const B2oShape* shapes_b, const real* xf_b,
+                  int use_radii, real* out) {
+  for (int i = 0; i < n; ++i) {
+    Shape a, b;
+    shape_from_rec(shapes_a[i], &a);
+    shape_from_rec(shapes_b[i], &b);
+    Transform ta, tb;
+    ta.p = Vec2(xf_a[4 * i], xf_a[4 * i + 1]); ta.q.s = xf_a[4 * i + 2]; ta.q.c = xf_a[4 * i + 3];
+    tb.p = Vec2(xf_b[4 * i], xf_b[4 * i + 1]); tb.q.s = xf_b[4 * i + 2]; tb.q.c = xf_b[4 * i + 3];
+    DistanceProxy pa, pb;
+    pa.set(a);
+    pb.set(b);
+    SimplexCache cache;
+    DistanceOutput d;
+    distance(&d, &cache, pa, ta, pb, tb, use_radii != 0);
+    real* o = out + 6 * i;
+    o[0] = d.distance;
+    o[1] = d.pa.x; o[2] = d.pa.y; o[3] = d.pb.x; o[4] = d.pb.y;
+    o[5] = real(d.iterations);
+  }
+}
+
+// Time of impact of n shape pairs over sweeps. A sweep is (local_center (2), c0 (2), c (2), a0, a, alpha0).
+// out per pair: state (B2toioutputState: 0 unknown, 1 failed, 2 overlapped, 3 touching, 4 separated), t.
+void b2o_time_of_impact(int n, const B2oShape* shapes_a, const real* sweep_a, const B2oShape* shapes_b,
+                        const real* sweep_b, real t_max, real* out) {
+  for (int i = 0; i < n; ++i) {
+    Shape a, b;
+    shape_from_rec(shapes_a[i], &a);
+    shape_from_rec(shapes_b[i], &b);
+    Sweep sw[2];
+    const real* src[2] = {sweep_a + 9 * i, sweep_b + 9 * i};
+    for (int k = 0; k < 2; ++k) {
+      const real* q = src[k];
+      sw[k].local_center = Vec2(q[0], q[1]);
+      sw[k].c0 = Vec2(q[2], q[3]);
+      sw[k].c = Vec2(q[4], q[5]);
+      sw[k].a0 = q[6];
+      sw[k].a = q[7];
+      sw[k].alpha0 = q[8];
+    }
+    DistanceProxy pa, pb;
+    pa.set(a);
+    pb.set(b);
+    ToiOutput o;
+    time_of_impact(&o, pa, pb, sw[0], sw[1], t_max);
+    out[2 * i] = real(o.state);
+    out[2 * i + 1] = o.t;
+  }
+}
+
 // World manifold of n manifolds: out per manifold = normal(2), points(4), separations(2).
 void b2o_world_manifold(int n, const B2oManifold* ms, const real* xf_a, real* ra, const real* xf_b, real* rb, real* out) {
   for (int i = 0; i < n; ++i) {
@@ -303,6 +357,8 @@ void b2o_world_set_flags(void* wp, int warm_starting, int allow_sleep, int block
   w->allow_sleep = allow_sleep != 0;
   w->block_solve = block_solve != 0;
 }
+void b2o_world_set_continuous(void* wp, int continuous) { static_cast<World*>(wp)->continuous = continuous != 0; }
+int b2o_world_toi_events(void* wp) { return static_cast<World*>(wp)->toi_events; }
 
 // Adds n bodies, then nf fixtures (fixture k on body fixture_body[k], in array order).
 // Body indices continue from the existing count. Writes each fixture's proxy id.

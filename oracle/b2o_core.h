@@ -41,6 +41,8 @@ static const real kMaxTranslationSquared = kMaxTranslation * kMaxTranslation;
 static const real kMaxRotation = real(0.5) * kPi;
 static const real kMaxRotationSquared = kMaxRotation * kMaxRotation;
 static const real kBaumgarte = real(0.2);
+static const real kToiBaumgarte = real(0.75);  // src/b2_common.rs:80
+static const int kMaxSubSteps = 8;             // src/b2_common.rs:51
 static const real kTimeToSleep = real(0.5);
 static const real kLinearSleepTolerance = real(0.01);
 static const real kAngularSleepTolerance = real(2.0) / real(180.0) * kPi;

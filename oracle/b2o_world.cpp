@@ -1,6 +1,7 @@
 // CPU oracle, part 2 implementation. TEST INFRASTRUCTURE ONLY (see b2o_core.h).
 #include "b2o_world.h"
 #include "b2o_joints.h"
+#include "b2o_toi.h"
 
 #include <chrono>
 
@@ -485,7 +486,18 @@ void World::collide() {
     if (!activeA && !activeB) continue;
     bool overlap = test_overlap(bp.tree.nodes[fa->proxy_id].aabb, bp.tree.nodes[fb->proxy_id].aabb);
     if (!overlap) { to_destroy.push_back(c); continue; }
-    // update
+    update_contact(c);
+  }
+  for (Contact* c : to_destroy) destroy_contact(c);
+}
+
+// B2contact::update (src/private/dynamics/b2_contact.rs:103-217)
+void World::update_contact(Contact* c) {
+  {
+    Fixture* fa = c->fa;
+    Fixture* fb = c->fb;
+    Body* ba = fa->body;
+    Body* bb = fb->body;
     c->enabled = true;
     Manifold old = c->manifold;
     bool was_touching = c->touching;
@@ -515,7 +527,6 @@ void World::collide() {
     }
     c->touching = touching;
   }
-  for (Contact* c : to_destroy) destroy_contact(c);
 }
 
 void World::find_new_contacts() {
@@ -890,6 +901,254 @@ bool ContactSolver::solve_position_constraints(Position* pos, real* min_sep_out)
   return minSeparation >= real(-3.0) * kLinearSlop;
 }
 
+// src/private/dynamics/b2_contact_solver_private.rs:733-822: only the two TOI bodies move
+bool ContactSolver::solve_toi_position_constraints(Position* pos, int toi_index_a, int toi_index_b) {
+  real minSeparation = 0;
+  for (size_t i = 0; i < pcs.size(); ++i) {
+    const PositionConstraint& pc = pcs[i];
+    int iA = pc.indexA, iB = pc.indexB;
+    Vec2 lcA = pc.localCenterA, lcB = pc.localCenterB;
+    real mA = 0, IA = 0, mB = 0, IB = 0;
+    if (iA == toi_index_a || iA == toi_index_b) { mA = pc.invMassA; IA = pc.invIA; }
+    if (iB == toi_index_a || iB == toi_index_b) { mB = pc.invMassB; IB = pc.invIB; }
+    Vec2 cA = pos[iA].c; real aA = pos[iA].a;
+    Vec2 cB = pos[iB].c; real aB = pos[iB].a;
+    for (int j = 0; j < pc.point_count; ++j) {
+      Transform xfA, xfB;
+      xfA.q.set(aA);
+      xfB.q.set(aB);
+      xfA.p = cA - mul(xfA.q, lcA);
+      xfB.p = cB - mul(xfB.q, lcB);
+      PositionSolverManifold psm;
+      psm.initialize(pc, xfA, xfB, j);
+      Vec2 normal = psm.normal, point = psm.point;
+      real separation = psm.separation;
+      Vec2 rA = point - cA, rB = point - cB;
+      minSeparation = bmin(minSeparation, separation);
+      real C = bclamp(kToiBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, real(0));
+      real rnA = cross(rA, normal), rnB = cross(rB, normal);
+      real K = mA + mB + IA * rnA * rnA + IB * rnB * rnB;
+      real impulse = K > 0 ? -C / K : real(0);
+      Vec2 P = impulse * normal;
+      cA -= mA * P;
+      aA -= IA * cross(rA, P);
+      cB += mB * P;
+      aB += IB * cross(rB, P);
+    }
+    pos[iA].c = cA; pos[iA].a = aA;
+    pos[iB].c = cB; pos[iB].a = aB;
+  }
+  return minSeparation >= real(-1.5) * kLinearSlop;
+}
+
+void Body::advance(real alpha) {
+  sweep_advance(&sweep, alpha);
+  sweep.c = sweep.c0;
+  sweep.a = sweep.a0;
+  xf.q.set(sweep.a);
+  xf.p = sweep.c - mul(xf.q, sweep.local_center);
+}
+
+// B2island::solve_toi (src/private/dynamics/b2_island_private.rs:330-458)
+static void solve_toi_island(World* w, IslandIO& island, const TimeStep& sub, int toi_a, int toi_b) {
+  size_t nb = island.bodies.size();
+  std::vector<Position> positions(nb);
+  std::vector<Velocity> velocities(nb);
+  std::vector<SolverBody> sbodies(nb);
+  for (size_t i = 0; i < nb; ++i) {
+    Body* b = island.bodies[i];
+    positions[i].c = b->sweep.c;
+    positions[i].a = b->sweep.a;
+    velocities[i].v = b->v;
+    velocities[i].w = b->w;
+    sbodies[i].inv_mass = b->inv_mass;
+    sbodies[i].inv_I = b->inv_I;
+    sbodies[i].local_center = b->sweep.local_center;
+  }
+  std::vector<SolverContact> scs(island.contacts.size());
+  for (size_t i = 0; i < island.contacts.size(); ++i) {
+    Contact* c = island.contacts[i];
+    scs[i] = SolverContact{&c->manifold, c->fa->body->island_index, c->fb->body->island_index, c->fa->shape.radius,
+                           c->fb->shape.radius, c->friction, c->restitution, c->threshold, c->tangent_speed};
+  }
+  ContactSolver cs;
+  cs.step = sub;
+  cs.block_solve = w->block_solve;
+  cs.init(scs, sbodies);
+  for (int i = 0; i < sub.position_iterations; ++i)
+    if (cs.solve_toi_position_constraints(positions.data(), toi_a, toi_b)) break;
+  island.bodies[toi_a]->sweep.c0 = positions[toi_a].c;
+  island.bodies[toi_a]->sweep.a0 = positions[toi_a].a;
+  island.bodies[toi_b]->sweep.c0 = positions[toi_b].c;
+  island.bodies[toi_b]->sweep.a0 = positions[toi_b].a;
+  cs.initialize_velocity_constraints(scs, positions.data(), velocities.data());
+  for (int i = 0; i < sub.velocity_iterations; ++i) cs.solve_velocity_constraints(velocities.data());
+  const real h = sub.dt;
+  for (size_t i = 0; i < nb; ++i) {
+    Vec2 c = positions[i].c;
+    real a = positions[i].a;
+    Vec2 v = velocities[i].v;
+    real wv = velocities[i].w;
+    Vec2 translation = h * v;
+    if (dot(translation, translation) > kMaxTranslationSquared) {
+      real ratio = kMaxTranslation / translation.length();
+      v *= ratio;
+    }
+    real rotation = h * wv;
+    if (rotation * rotation > kMaxRotationSquared) {
+      real ratio = kMaxRotation / babs(rotation);
+      wv *= ratio;
+    }
+    c += h * v;
+    a += h * wv;
+    Body* b = island.bodies[i];
+    b->sweep.c = c;
+    b->sweep.a = a;
+    b->v = v;
+    b->w = wv;
+    b->synchronize_transform();
+  }
+}
+
+// src/private/dynamics/b2_world.rs:534-901 (bullets are not part of this path: every dynamic body is a
+// non-bullet, so only contacts with a static or kinematic body get a time of impact)
+void World::solve_toi(const TimeStep& step) {
+  toi_events = 0;
+  for (Body* b : bodies) {
+    b->island_flag = false;
+    b->sweep.alpha0 = 0;
+  }
+  for (Contact* c = contact_list; c; c = c->next) {
+    c->toi_flag = false;
+    c->island_flag = false;
+    c->toi_count = 0;
+    c->toi = 1;
+  }
+  IslandIO island;
+  for (;;) {
+    Contact* min_contact = nullptr;
+    real min_alpha = 1;
+    for (Contact* c = contact_list; c; c = c->next) {
+      if (!c->enabled) continue;
+      if (c->toi_count > kMaxSubSteps) continue;
+      real alpha;
+      if (c->toi_flag) {
+        alpha = c->toi;
+      } else {
+        Fixture* fa = c->fa;
+        Fixture* fb = c->fb;
+        if (fa->sensor || fb->sensor) continue;
+        Body* ba = fa->body;
+        Body* bb = fb->body;
+        bool activeA = ba->awake && ba->type != kStatic;
+        bool activeB = bb->awake && bb->type != kStatic;
+        if (!activeA && !activeB) continue;
+        bool collideA = ba->type != kDynamic;
+        bool collideB = bb->type != kDynamic;
+        if (!collideA && !collideB) continue;
+        real alpha0 = ba->sweep.alpha0;
+        if (ba->sweep.alpha0 < bb->sweep.alpha0) {
+          alpha0 = bb->sweep.alpha0;
+          sweep_advance(&ba->sweep, alpha0);
+        } else if (bb->sweep.alpha0 < ba->sweep.alpha0) {
+          alpha0 = ba->sweep.alpha0;
+          sweep_advance(&bb->sweep, alpha0);
+        }
+        DistanceProxy pa, pb;
+        pa.set(fa->shape);
+        pb.set(fb->shape);
+        ToiOutput out;
+        time_of_impact(&out, pa, pb, ba->sweep, bb->sweep, real(1.0));
+        real beta = out.t;
+        if (out.state == kToiTouching) alpha = bmin(alpha0 + (real(1.0) - alpha0) * beta, real(1.0));
+        else alpha = 1;
+        c->toi = alpha;
+        c->toi_flag = true;
+      }
+      if (alpha < min_alpha) {
+        min_contact = c;
+        min_alpha = alpha;
+      }
+    }
+    if (!min_contact || real(1.0) - real(10.0) * kEpsilon < min_alpha) break;
+
+    Body* ba = min_contact->fa->body;
+    Body* bb = min_contact->fb->body;
+    Sweep backup1 = ba->sweep, backup2 = bb->sweep;
+    ba->advance(min_alpha);
+    bb->advance(min_alpha);
+    update_contact(min_contact);
+    min_contact->toi_flag = false;
+    ++min_contact->toi_count;
+    if (!min_contact->enabled || !min_contact->touching) {
+      min_contact->enabled = false;
+      ba->sweep = backup1;
+      bb->sweep = backup2;
+      ba->synchronize_transform();
+      bb->synchronize_transform();
+      continue;
+    }
+    ++toi_events;
+    ba->set_awake(true);
+    bb->set_awake(true);
+    island.bodies.clear();
+    island.contacts.clear();
+    island.joints.clear();
+    ba->island_index = 0;
+    island.bodies.push_back(ba);
+    bb->island_index = 1;
+    island.bodies.push_back(bb);
+    island.contacts.push_back(min_contact);
+    ba->island_flag = true;
+    bb->island_flag = true;
+    min_contact->island_flag = true;
+    Body* pair[2] = {ba, bb};
+    for (Body* body : pair) {
+      if (body->type != kDynamic) continue;
+      for (ContactEdge* ce = body->contact_list; ce; ce = ce->next) {
+        Contact* contact = ce->contact;
+        if (contact->island_flag) continue;
+        Body* other = ce->other;
+        if (other->type == kDynamic) continue;
+        if (contact->fa->sensor || contact->fb->sensor) continue;
+        Sweep backup = other->sweep;
+        if (!other->island_flag) other->advance(min_alpha);
+        update_contact(contact);
+        if (!contact->enabled || !contact->touching) {
+          other->sweep = backup;
+          other->synchronize_transform();
+          continue;
+        }
+        contact->island_flag = true;
+        island.contacts.push_back(contact);
+        if (other->island_flag) continue;
+        other->island_flag = true;
+        if (other->type != kStatic) other->set_awake(true);
+        other->island_index = int(island.bodies.size());
+        island.bodies.push_back(other);
+      }
+    }
+    TimeStep sub;
+    sub.dt = (real(1.0) - min_alpha) * step.dt;
+    sub.inv_dt = real(1.0) / sub.dt;
+    sub.dt_ratio = 1;
+    sub.position_iterations = 20;
+    sub.velocity_iterations = step.velocity_iterations;
+    sub.warm_starting = false;
+    solve_toi_island(this, island, sub, 0, 1);
+    for (Body* body : island.bodies) {
+      body->island_flag = false;
+      if (body->type != kDynamic) continue;
+      synchronize_fixtures(body);
+      for (ContactEdge* ce = body->contact_list; ce; ce = ce->next) {
+        ce->contact->toi_flag = false;
+        ce->contact->island_flag = false;
+      }
+    }
+    find_new_contacts();
+  }
+}
+
 // ---- island solve: src/private/dynamics/b2_island_private.rs:129-328 ---------------------
 bool solve_island(World* w, IslandIO& island, const TimeStep& step) {
   const real h = step.dt;
@@ -1063,8 +1322,7 @@ void World::solve(const TimeStep& step) {
   profile.broadphase = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// src/private/dynamics/b2_world.rs:903-959. Continuous collision (solve_toi) is not restated;
-// the oracle always runs with continuous physics off.
+// src/private/dynamics/b2_world.rs:903-959 (solve_toi when continuous physics is on; sub-stepping is off)
 void World::step(real dt, int vel_iters, int pos_iters) {
   auto t0 = std::chrono::steady_clock::now();
   if (new_contacts) {
@@ -1085,6 +1343,7 @@ void World::step(real dt, int vel_iters, int pos_iters) {
   if (st.dt > 0) solve(st);
   auto t3 = std::chrono::steady_clock::now();
   profile.solve = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  if (continuous && st.dt > 0) solve_toi(st);
   if (st.dt > 0) inv_dt0 = st.inv_dt;
   for (Body* b : bodies) { b->force.set_zero(); b->torque = 0; }
   profile.step = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -155,10 +155,16 @@ struct Body {
     xf.p = sweep.c - mul(xf.q, sweep.local_center);
   }
   void set_awake(bool flag);
+  // src/b2_body.rs:979-986: advance to a safe time without touching the broad-phase
+  void advance(real alpha);
 };
 
 struct Contact {
   bool island_flag = false, touching = false, enabled = true, filter = false;
+  // continuous collision: cached time of impact (src/private/dynamics/b2_contact.rs E_TOI_FLAG, m_toi, m_toi_count)
+  bool toi_flag = false;
+  int toi_count = 0;
+  real toi = 1;
   Fixture *fa = nullptr, *fb = nullptr;
   Manifold manifold;
   real friction = 0, restitution = 0, threshold = 0, tangent_speed = 0;
@@ -248,6 +254,7 @@ struct ContactSolver {
   void solve_velocity_constraints(Velocity* vel);
   void store_impulses(const std::vector<SolverContact>& contacts);
   bool solve_position_constraints(Position* pos, real* min_sep_out = nullptr);
+  bool solve_toi_position_constraints(Position* pos, int toi_index_a, int toi_index_b);
 };
 
 struct Profile { double step = 0, collide = 0, solve = 0, broadphase = 0; };
@@ -263,6 +270,7 @@ struct World {
   bool warm_starting = true, allow_sleep = true, continuous = false, block_solve = true;
   bool new_contacts = false;
   real inv_dt0 = 0;
+  int toi_events = 0;  // TOI events solved in the last step (a count for tests)
   Profile profile;
 
   explicit World(Vec2 g) : gravity(g) {}
@@ -280,6 +288,8 @@ struct World {
   void add_pair(int fixtureA, int fixtureB);
   void destroy_contact(Contact* c);
   void solve(const TimeStep& step);
+  void solve_toi(const TimeStep& step);
+  void update_contact(Contact* c);
   void synchronize_fixtures(Body* b);
 };
 
