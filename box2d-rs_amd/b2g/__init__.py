@@ -76,7 +76,7 @@ class _Profile(ctypes.Structure):
     _fields_ = [("step", ctypes.c_float), ("collide", ctypes.c_float), ("solve", ctypes.c_float),
                 ("broadphase", ctypes.c_float), ("pairs", ctypes.c_int32), ("touching", ctypes.c_int32),
                 ("colours", ctypes.c_int32), ("islands", ctypes.c_int32), ("position_iterations", ctypes.c_int32),
-                ("pad", ctypes.c_int32 * 3)]
+                ("toi_events", ctypes.c_int32), ("pad", ctypes.c_int32 * 2)]
 
 
 def _load():
@@ -818,10 +818,12 @@ class B2world:
         self._fixtures = []
         self._n_flushed_bodies = 0
         self._cache = None
-        # B2world::new defaults: warm starting and sleeping on (continuous physics is not on the
-        # GPU path and stays off)
+        # B2world::new defaults: warm starting and sleeping on. Continuous physics (on by default
+        # in the reference) is off until set_continuous_physics(True): the TOI pass is a
+        # sequential event loop that costs far more than the step on large worlds.
         self._warm_starting = True
         self._allow_sleep = True
+        self._continuous = False
         self._apply_flags()
 
     @classmethod
@@ -833,8 +835,8 @@ class B2world:
         self._apply_flags()
 
     def set_continuous_physics(self, flag):
-        if flag:
-            raise B2gError(ERR_UNSUPPORTED, "continuous physics (TOI) is not on the GPU path")
+        self._continuous = bool(flag)
+        self._apply_flags()
 
     def set_allow_sleeping(self, flag):
         # B2world::set_allow_sleeping (src/private/dynamics/b2_world.rs:342-353): switching it off
@@ -847,7 +849,7 @@ class B2world:
         return self._allow_sleep
 
     def _apply_flags(self):
-        self._world.set_flags(self._warm_starting, self._allow_sleep, False, True)
+        self._world.set_flags(self._warm_starting, self._allow_sleep, self._continuous, True)
 
     def create_body(self, body_def):
         self._bodies.append(body_def._def())

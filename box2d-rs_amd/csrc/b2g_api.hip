@@ -77,6 +77,8 @@ static int carry_manifolds(World* w) {
   return B2G_OK;
 }
 
+int carry_manifolds_for_toi(World* w) { return carry_manifolds(w); }
+
 // ---- host-side creation (create_body / create_fixture / reset_mass_data) --------------------
 // src/private/dynamics/b2_body.rs:15-89 (body), :292-349 (reset_mass_data)
 static void reset_mass(World* w, int b) {
@@ -541,6 +543,10 @@ int b2g_world_destroy(b2g_world* h) {
   for (auto& c : w->c) {
     rel(c.pair); rel(c.off); rel(c.old_index); rel(c.ab); rel(c.hdr); rel(c.geo); rel(c.pt0); rel(c.pt1); rel(c.ids); rel(c.colour);
   }
+  for (auto& t : w->toi_state) {
+    rel(t.toi); rel(t.count); rel(t.flags);
+  }
+  rel(w->toi_alpha0); rel(w->toi_sel); rel(w->toi_two); rel(w->toi_ctl);
   SolverArrays& s = w->s;
   rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
   rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);
@@ -565,8 +571,8 @@ int b2g_world_destroy(b2g_world* h) {
 int b2g_world_set_flags(b2g_world* h, int32_t warm_starting, int32_t allow_sleep, int32_t continuous,
                         int32_t block_solve) {
   if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
-  if (continuous) {
-    set_error("continuous physics is not supported on the GPU path");
+  if (continuous && h->w.tape.active) {
+    set_error("continuous physics cannot be switched on while a tape is recording");
     return B2G_ERR_UNSUPPORTED;
   }
   if (allow_sleep && h->w.tape.active) {
@@ -585,6 +591,7 @@ int b2g_world_set_flags(b2g_world* h, int32_t warm_starting, int32_t allow_sleep
   }
   w->allow_sleep = allow_sleep != 0;
   w->warm_starting = warm_starting != 0;
+  w->continuous = continuous != 0;
   h->w.block_solve = block_solve != 0;
   return B2G_OK;
 }
@@ -645,6 +652,8 @@ int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {
   if (taping && (st = tape_after_solve(w, dt, dt_ratio, vi, pi))) return st;
   (void)hipEventRecord(w->ev[2], w->stream);
   if ((st = run_phase(w, B2G_PHASE_FIND, dt, vi, pi))) return st;
+  w->toi_events = 0;
+  if (w->continuous && dt > 0.0f && (st = phase_toi(w, dt, vi))) return st;
   (void)hipEventRecord(w->ev[3], w->stream);
   w->profile_pending = true;
   return B2G_OK;
@@ -1188,6 +1197,7 @@ int b2g_world_get_profile(b2g_world* h, b2g_profile* out) {
   out->touching = w->n_touch;
   out->colours = w->n_colors;
   out->position_iterations = w->pos_iters_run;
+  out->toi_events = w->toi_events;
   return B2G_OK;
 }
 

@@ -618,14 +618,17 @@ static int reserve_contacts(ContactArrays& c, int n) {
   return B2G_OK;
 }
 
-int phase_synchronize(World* w) {
+int phase_synchronize_view(World* w, IslandView iv) {
   if (w->np == 0) return B2G_OK;
   hipLaunchKernelGGL(k_synchronize, dim3(blocks_for(w->np, kBlock)), dim3(kBlock), 0, w->stream, w->np, w->f.body.p,
                      w->f.shapev.p, w->b.type.p, w->b.pos0.p, w->b.mass.p, w->b.xf.p, w->f.aabb.p,
-                     w->f.fat.p, w->f.moved.p,
-                     IslandView{w->s.parent.p, w->sleep_possible() ? w->s.isl_awake.p : nullptr});
+                     w->f.fat.p, w->f.moved.p, iv);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
+}
+
+int phase_synchronize(World* w) {
+  return phase_synchronize_view(w, IslandView{w->s.parent.p, w->sleep_possible() ? w->s.isl_awake.p : nullptr});
 }
 
 // New contact set from the current fat AABBs; becomes the current set, the old one is kept

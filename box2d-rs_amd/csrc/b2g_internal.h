@@ -114,6 +114,13 @@ struct ContactArrays {
   DevArray<int> colour;      // solve colour (launch index) of a touching contact, else -1
 };
 
+// Continuous collision state of one contact set: cached time of impact (as a fraction of the
+// step), TOI count and flags (bit 0 cached, bit 1 disabled by a TOI event).
+struct ToiState {
+  DevArray<float> toi;
+  DevArray<int> count, flags;
+};
+
 // Per-step solver arrays (indexed by constraint, colour-sorted).
 struct SolverArrays {
   DevArray<int> touch_flag, touch_scan, touch_list;   // compaction of touching contacts
@@ -281,6 +288,11 @@ struct World {
   BodyArrays b;
   ProxyArrays f;
   ContactArrays c[2];  // current and previous contact sets
+  ToiState toi_state[2];              // per contact set, when continuous physics is on
+  DevArray<float> toi_alpha0;         // per body: the sweep's alpha0 during the TOI pass
+  DevArray<int> toi_sel, toi_two;     // bodies re-synchronised after a TOI event; {0, 1}
+  DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies
+  int toi_events = 0;                 // TOI events solved in the last step
   int cur = 0;         // index of the current contact set in c[]
   int n_pairs = 0;     // pairs in the current set
   int n_pairs_old = 0; // pairs in the previous set
@@ -361,6 +373,7 @@ int phase_broadphase(World* w);  // fat AABBs -> next contact set (pairs + old i
 int phase_collide(World* w);     // narrowphase on the current contact set
 int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters);
 int phase_synchronize(World* w); // swept + fat AABBs of non-static proxies
+int phase_toi(World* w, float dt, int vi);             // continuous collision after the solve
 void release_solve_graphs(World* w);
 // primitives
 int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev);
@@ -387,6 +400,7 @@ struct IslandView {
   const int* isl_awake;  // per root; null: every island is awake
   __device__ bool asleep(int b) const { return isl_awake && !isl_awake[parent[b]]; }
 };
+int phase_synchronize_view(World* w, IslandView iv);  // phase_synchronize for the bodies the view holds awake
 
 // Kernel arguments of the joint solve (b2g_joints.hip).
 struct JointArgs {

@@ -272,9 +272,12 @@ B2G_HD void solve_velocity_t(const TVelC<T>& c, const TContactConst<S>& k, T lam
 }
 
 // solve_position_constraints for one constraint, point by point; returns the smallest
-// separation seen (src/private/dynamics/b2_contact_solver_private.rs:600-730)
+// separation seen (src/private/dynamics/b2_contact_solver_private.rs:600-730). With the TOI
+// Baumgarte factor and the non-TOI bodies' masses zeroed it is solve_toiposition_constraints
+// (:733-822).
 template <class T, class S>
-B2G_HD T solve_position_t(const TContactConst<S>& k, const TGeom<T>& g, TPos<T>& A, TPos<T>& B) {
+B2G_HD T solve_position_t(const TContactConst<S>& k, const TGeom<T>& g, TPos<T>& A, TPos<T>& B,
+                          double baumgarte = kBaumgarteD) {
   const T mA = T(k.mA), iA = T(k.iA), mB = T(k.mB), iB = T(k.iB);
   const TV2<T> lcA = v2(T(k.lcA[0]), T(k.lcA[1])), lcB = v2(T(k.lcB[0]), T(k.lcB[1]));
   T minSep = T(0.0);
@@ -309,7 +312,7 @@ B2G_HD T solve_position_t(const TContactConst<S>& k, const TGeom<T>& g, TPos<T>&
     }
     TV2<T> rA = point - A.c, rB = point - B.c;
     minSep = bmin(minSep, separation);
-    T C = bclamp(T(kBaumgarteD) * (separation + T(kLinearSlopD)), -T(kMaxLinearCorrectionD), T(0.0));
+    T C = bclamp(T(baumgarte) * (separation + T(kLinearSlopD)), -T(kMaxLinearCorrectionD), T(0.0));
     T rnA = cross(rA, normal), rnB = cross(rB, normal);
     T K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
     T impulse = K > T(0.0) ? -C / K : T(0.0);

@@ -1,0 +1,1028 @@
+// Continuous collision (time of impact) on the device: the TOI pass that B2world::step runs after
+// the solve when continuous physics is on (src/private/dynamics/b2_world.rs:534-901).
+//
+// The reference loops: find the contact with the smallest time of impact over the contact list,
+// advance its two bodies to that time, update the contact, gather the static and kinematic
+// contacts of the two bodies into a small island, solve that island's positions (TOI Baumgarte,
+// 20 iterations) and velocities over the rest of the step, re-synchronise the bodies' proxies,
+// find new pairs, and repeat until no contact has a time of impact before the end of the step.
+//
+// Here:
+// * k_toi_first (grid): at the start of the pass every body's sweep starts at alpha0 = 0, so
+//   each contact's time of impact is independent of the others: one thread per contact.
+// * k_toi_min (grid): the smallest (alpha, contact index) over the contacts with a cached time of
+//   impact, as one 64-bit atomicMin of the float's bits (alpha >= 0) over the index. The
+//   reference takes the first contact of its list among equal times; the GPU's contact set is
+//   ordered by its own (deterministic) pair order instead of the reference's list order.
+// * k_toi_event (one workgroup): the event itself, sequential as in the reference: advance,
+//   update, island gathering over the contact set in set order (the two bodies' contact lists),
+//   the island's TOI solve, the new sweeps and velocities, and the TOI flags of the displaced
+//   bodies' contacts cleared.
+// * then the host re-synchronises proxies and rebuilds the contact set (find_new_contacts), and
+//   k_toi_rescan (one workgroup) recomputes the times of impact that are no longer cached, in set
+//   order, with the reference's sweep alignment (the body with the earlier alpha0 is advanced to
+//   the other's) done sequentially where it is needed.
+// Time of impact: GJK distance (src/private/collision/b2_distance.rs:404-530) and conservative
+// advancement with separating axes (src/private/collision/b2_time_of_impact.rs:17-463).
+#include "b2g_internal.h"
+#include "b2g_collide.h"
+#include "b2g_step_t.h"
+
+namespace b2g {
+
+namespace {
+
+constexpr int kToiBlock = 256;
+constexpr int kRescanBlock = 1024;
+constexpr int kEventBlock = 256;
+constexpr int kMaxToiBodies = 64;      // bodies of one TOI island (two plus static/kinematic neighbours)
+constexpr int kMaxToiContacts = 64;    // contacts of one TOI island
+constexpr int kMaxBodyContacts = 512;  // contacts of one body looked at when the island is gathered
+constexpr float kFltEpsilon = 1.1920928955078125e-7f;
+constexpr float kSlop = 0.005f;
+constexpr float kToiBaumgarte = 0.75f;  // src/b2_common.rs:80
+constexpr int kMaxSubSteps = 8;         // src/b2_common.rs:51
+constexpr float kTwoPi = 2.0f * 3.14159265359f;
+
+enum : int { kToiValid = 1, kToiDisabled = 2 };
+enum : int { kEventNotSolid = 0, kEventSolved = 1, kEventCapacity = 2 };
+
+struct SweepD {
+  V2 lc, c0, c;
+  float a0, a, alpha0;
+};
+
+__device__ inline Xf sweep_xf(const SweepD& s, float beta) {
+  Xf xf;
+  xf.p = (1.0f - beta) * s.c0 + beta * s.c;
+  const float angle = (1.0f - beta) * s.a0 + beta * s.a;
+  xf.q = rot(angle);
+  xf.p = xf.p - mul(xf.q, s.lc);
+  return xf;
+}
+
+// B2Sweep::advance (src/b2_math.rs:773-779)
+__device__ inline void sweep_advance(SweepD& s, float alpha) {
+  const float beta = (alpha - s.alpha0) / (1.0f - s.alpha0);
+  s.c0 = s.c0 + beta * (s.c - s.c0);
+  s.a0 = s.a0 + beta * (s.a - s.a0);
+  s.alpha0 = alpha;
+}
+
+// A shape as GJK sees it: its core points and radius.
+struct Proxy {
+  const V2* v;
+  int count;
+  float radius;
+  __device__ int support(V2 d) const {
+    int best = 0;
+    float best_value = dot(v[0], d);
+    for (int i = 1; i < count; ++i) {
+      const float value = dot(v[i], d);
+      if (value > best_value) {
+        best = i;
+        best_value = value;
+      }
+    }
+    return best;
+  }
+};
+
+__device__ inline Proxy proxy_of(const ShapeV& s) {
+  Proxy p;
+  if (s.type == kCircle) {
+    p.v = &s.p;
+    p.count = 1;
+  } else if (s.type == kPolygon) {
+    p.v = s.v;
+    p.count = s.count;
+  } else {
+    p.v = s.v;  // vertex1, vertex2
+    p.count = 2;
+  }
+  p.radius = s.radius;
+  return p;
+}
+
+struct Cache {
+  float metric;
+  int count;
+  int ia[3], ib[3];
+};
+
+struct SVert {
+  V2 wa, wb, w;
+  float a;
+  int ia, ib;
+};
+
+struct Simplex {
+  SVert v[3];
+  int count;
+
+  __device__ float metric() const {
+    if (count == 2) return len(v[0].w - v[1].w);
+    if (count == 3) return cross(v[1].w - v[0].w, v[2].w - v[0].w);
+    return 0.0f;
+  }
+  __device__ void vertex(int i, int ia, int ib, const Proxy& pa, const Xf& xa, const Proxy& pb, const Xf& xb) {
+    v[i].ia = ia;
+    v[i].ib = ib;
+    v[i].wa = mul(xa, pa.v[ia]);
+    v[i].wb = mul(xb, pb.v[ib]);
+    v[i].w = v[i].wb - v[i].wa;
+  }
+  __device__ void read_cache(const Cache& c, const Proxy& pa, const Xf& xa, const Proxy& pb, const Xf& xb) {
+    count = c.count;
+    for (int i = 0; i < count; ++i) {
+      vertex(i, c.ia[i], c.ib[i], pa, xa, pb, xb);
+      v[i].a = 0.0f;
+    }
+    if (count > 1) {
+      const float m1 = c.metric, m2 = metric();
+      if (m2 < 0.5f * m1 || 2.0f * m1 < m2 || m2 < kFltEpsilon) count = 0;
+    }
+    if (count == 0) {
+      vertex(0, 0, 0, pa, xa, pb, xb);
+      v[0].a = 1.0f;
+      count = 1;
+    }
+  }
+  __device__ void write_cache(Cache& c) const {
+    c.metric = metric();
+    c.count = count;
+    for (int i = 0; i < count; ++i) {
+      c.ia[i] = v[i].ia;
+      c.ib[i] = v[i].ib;
+    }
+  }
+  __device__ void solve2() {
+    const V2 w1 = v[0].w, w2 = v[1].w, e12 = w2 - w1;
+    const float d12_2 = -dot(w1, e12);
+    if (d12_2 <= 0.0f) {
+      v[0].a = 1.0f;
+      count = 1;
+      return;
+    }
+    const float d12_1 = dot(w2, e12);
+    if (d12_1 <= 0.0f) {
+      v[1].a = 1.0f;
+      count = 1;
+      v[0] = v[1];
+      return;
+    }
+    const float inv = 1.0f / (d12_1 + d12_2);
+    v[0].a = d12_1 * inv;
+    v[1].a = d12_2 * inv;
+    count = 2;
+  }
+  __device__ void solve3() {
+    const V2 w1 = v[0].w, w2 = v[1].w, w3 = v[2].w;
+    const V2 e12 = w2 - w1, e13 = w3 - w1, e23 = w3 - w2;
+    const float d12_1 = dot(w2, e12), d12_2 = -dot(w1, e12);
+    const float d13_1 = dot(w3, e13), d13_2 = -dot(w1, e13);
+    const float d23_1 = dot(w3, e23), d23_2 = -dot(w2, e23);
+    const float n123 = cross(e12, e13);
+    const float d123_1 = n123 * cross(w2, w3), d123_2 = n123 * cross(w3, w1), d123_3 = n123 * cross(w1, w2);
+    if (d12_2 <= 0.0f && d13_2 <= 0.0f) {
+      v[0].a = 1.0f;
+      count = 1;
+    } else if (d12_1 > 0.0f && d12_2 > 0.0f && d123_3 <= 0.0f) {
+      const float inv = 1.0f / (d12_1 + d12_2);
+      v[0].a = d12_1 * inv;
+      v[1].a = d12_2 * inv;
+      count = 2;
+    } else if (d13_1 > 0.0f && d13_2 > 0.0f && d123_2 <= 0.0f) {
+      const float inv = 1.0f / (d13_1 + d13_2);
+      v[0].a = d13_1 * inv;
+      v[2].a = d13_2 * inv;
+      count = 2;
+      v[1] = v[2];
+    } else if (d12_1 <= 0.0f && d23_2 <= 0.0f) {
+      v[1].a = 1.0f;
+      count = 1;
+      v[0] = v[1];
+    } else if (d13_1 <= 0.0f && d23_1 <= 0.0f) {
+      v[2].a = 1.0f;
+      count = 1;
+      v[0] = v[2];
+    } else if (d23_1 > 0.0f && d23_2 > 0.0f && d123_1 <= 0.0f) {
+      const float inv = 1.0f / (d23_1 + d23_2);
+      v[1].a = d23_1 * inv;
+      v[2].a = d23_2 * inv;
+      count = 2;
+      v[0] = v[2];
+    } else {
+      const float inv = 1.0f / (d123_1 + d123_2 + d123_3);
+      v[0].a = d123_1 * inv;
+      v[1].a = d123_2 * inv;
+      v[2].a = d123_3 * inv;
+      count = 3;
+    }
+  }
+};
+
+// GJK distance between the core shapes (no radii), with a simplex cache carried between calls.
+__device__ float gjk_distance(Cache& cache, const Proxy& pa, const Xf& xa, const Proxy& pb, const Xf& xb) {
+  Simplex s;
+  s.read_cache(cache, pa, xa, pb, xb);
+  int save_a[3], save_b[3];
+  for (int iter = 0; iter < 20;) {
+    const int save_count = s.count;
+    for (int i = 0; i < save_count; ++i) {
+      save_a[i] = s.v[i].ia;
+      save_b[i] = s.v[i].ib;
+    }
+    if (s.count == 2) s.solve2();
+    else if (s.count == 3) s.solve3();
+    if (s.count == 3) break;
+    V2 d;
+    if (s.count == 1) {
+      d = -s.v[0].w;
+    } else {
+      const V2 e12 = s.v[1].w - s.v[0].w;
+      d = cross(e12, -s.v[0].w) > 0.0f ? cross(1.0f, e12) : cross(e12, 1.0f);
+    }
+    if (dot(d, d) < kFltEpsilon * kFltEpsilon) break;
+    const int ia = pa.support(mulT(xa.q, -d));
+    const int ib = pb.support(mulT(xb.q, d));
+    s.vertex(s.count, ia, ib, pa, xa, pb, xb);
+    ++iter;
+    bool duplicate = false;
+    for (int i = 0; i < save_count; ++i)
+      if (ia == save_a[i] && ib == save_b[i]) duplicate = true;
+    if (duplicate) break;
+    ++s.count;
+  }
+  V2 wa, wb;
+  if (s.count == 1) {
+    wa = s.v[0].wa;
+    wb = s.v[0].wb;
+  } else if (s.count == 2) {
+    wa = s.v[0].a * s.v[0].wa + s.v[1].a * s.v[1].wa;
+    wb = s.v[0].a * s.v[0].wb + s.v[1].a * s.v[1].wb;
+  } else {
+    wa = s.v[0].a * s.v[0].wa + s.v[1].a * s.v[1].wa + s.v[2].a * s.v[2].wa;
+    wb = wa;
+  }
+  s.write_cache(cache);
+  return len(wa - wb);
+}
+
+struct Separation {
+  int type;  // 0 points, 1 face A, 2 face B
+  V2 local_point, axis;
+};
+
+__device__ Separation separation_init(const Cache& cache, const Proxy& pa, const SweepD& sa, const Proxy& pb,
+                                      const SweepD& sb, float t1) {
+  Separation f;
+  f.local_point = v2(0.0f, 0.0f);
+  const Xf xa = sweep_xf(sa, t1), xb = sweep_xf(sb, t1);
+  if (cache.count == 1) {
+    f.type = 0;
+    f.axis = mul(xb, pb.v[cache.ib[0]]) - mul(xa, pa.v[cache.ia[0]]);
+    normalize(&f.axis);
+  } else if (cache.ia[0] == cache.ia[1]) {
+    f.type = 2;
+    const V2 b1 = pb.v[cache.ib[0]], b2 = pb.v[cache.ib[1]];
+    f.axis = cross(b2 - b1, 1.0f);
+    normalize(&f.axis);
+    const V2 normal = mul(xb.q, f.axis);
+    f.local_point = 0.5f * (b1 + b2);
+    if (dot(mul(xa, pa.v[cache.ia[0]]) - mul(xb, f.local_point), normal) < 0.0f) f.axis = -f.axis;
+  } else {
+    f.type = 1;
+    const V2 a1 = pa.v[cache.ia[0]], a2 = pa.v[cache.ia[1]];
+    f.axis = cross(a2 - a1, 1.0f);
+    normalize(&f.axis);
+    const V2 normal = mul(xa.q, f.axis);
+    f.local_point = 0.5f * (a1 + a2);
+    if (dot(mul(xb, pb.v[cache.ib[0]]) - mul(xa, f.local_point), normal) < 0.0f) f.axis = -f.axis;
+  }
+  return f;
+}
+
+__device__ float separation_min(const Separation& f, int& ia, int& ib, const Proxy& pa, const SweepD& sa,
+                                const Proxy& pb, const SweepD& sb, float t) {
+  const Xf xa = sweep_xf(sa, t), xb = sweep_xf(sb, t);
+  if (f.type == 0) {
+    ia = pa.support(mulT(xa.q, f.axis));
+    ib = pb.support(mulT(xb.q, -f.axis));
+    return dot(mul(xb, pb.v[ib]) - mul(xa, pa.v[ia]), f.axis);
+  }
+  if (f.type == 1) {
+    const V2 normal = mul(xa.q, f.axis);
+    const V2 qa = mul(xa, f.local_point);
+    ia = -1;
+    ib = pb.support(mulT(xb.q, -normal));
+    return dot(mul(xb, pb.v[ib]) - qa, normal);
+  }
+  const V2 normal = mul(xb.q, f.axis);
+  const V2 qb = mul(xb, f.local_point);
+  ib = -1;
+  ia = pa.support(mulT(xa.q, -normal));
+  return dot(mul(xa, pa.v[ia]) - qb, normal);
+}
+
+__device__ float separation_eval(const Separation& f, int ia, int ib, const Proxy& pa, const SweepD& sa,
+                                 const Proxy& pb, const SweepD& sb, float t) {
+  const Xf xa = sweep_xf(sa, t), xb = sweep_xf(sb, t);
+  if (f.type == 0) return dot(mul(xb, pb.v[ib]) - mul(xa, pa.v[ia]), f.axis);
+  if (f.type == 1) return dot(mul(xb, pb.v[ib]) - mul(xa, f.local_point), mul(xa.q, f.axis));
+  return dot(mul(xa, pa.v[ia]) - mul(xb, f.local_point), mul(xb.q, f.axis));
+}
+
+// b2_time_of_impact with t_max = 1: returns t, *touching set for the touching state
+__device__ float time_of_impact(const Proxy& pa, const Proxy& pb, SweepD sa, SweepD sb, bool* touching) {
+  *touching = false;
+  // B2Sweep::normalize (src/b2_math.rs:782-787)
+  float d = kTwoPi * floorf(sa.a0 / kTwoPi);
+  sa.a0 -= d;
+  sa.a -= d;
+  d = kTwoPi * floorf(sb.a0 / kTwoPi);
+  sb.a0 -= d;
+  sb.a -= d;
+  const float target = bmax(kSlop, pa.radius + pb.radius - 3.0f * kSlop);
+  const float tolerance = 0.25f * kSlop;
+  float t1 = 0.0f;
+  Cache cache;
+  cache.count = 0;
+  cache.metric = 0.0f;
+  for (int iter = 0;;) {
+    const float dist = gjk_distance(cache, pa, sweep_xf(sa, t1), pb, sweep_xf(sb, t1));
+    if (dist <= 0.0f) return 0.0f;  // overlapped
+    if (dist < target + tolerance) {
+      *touching = true;
+      return t1;
+    }
+    const Separation f = separation_init(cache, pa, sa, pb, sb, t1);
+    float t2 = 1.0f;
+    for (int push_back = 0;;) {
+      int ia = 0, ib = 0;
+      float s2 = separation_min(f, ia, ib, pa, sa, pb, sb, t2);
+      if (s2 > target + tolerance) return 1.0f;  // separated
+      if (s2 > target - tolerance) {
+        t1 = t2;
+        break;
+      }
+      float s1 = separation_eval(f, ia, ib, pa, sa, pb, sb, t1);
+      if (s1 < target - tolerance) return t1;  // failed
+      if (s1 <= target + tolerance) {
+        *touching = true;
+        return t1;
+      }
+      float a1 = t1, a2 = t2;
+      for (int root = 0;;) {
+        float t;
+        if (root & 1) t = a1 + (target - s1) * (a2 - a1) / (s2 - s1);
+        else t = 0.5f * (a1 + a2);
+        ++root;
+        const float s = separation_eval(f, ia, ib, pa, sa, pb, sb, t);
+        if (fabsf(s - target) < tolerance) {
+          t2 = t;
+          break;
+        }
+        if (s > target) {
+          a1 = t;
+          s1 = s;
+        } else {
+          a2 = t;
+          s2 = s;
+        }
+        if (root == 50) break;
+      }
+      if (++push_back == kMaxPolygonVertices) break;
+    }
+    if (++iter == 20) return t1;  // failed
+  }
+}
+
+struct ToiArgs {
+  int n;                 // contacts in the current set
+  int nb;                // bodies
+  const int2* ab;        // contact fixtures (registry order)
+  int* hdr;              // manifolds of the current set
+  float4* geo;
+  float4* pt0;
+  float4* pt1;
+  uint2* ids;
+  float* toi;            // per contact: cached alpha, TOI count, flags
+  int* tcount;
+  int* tflags;
+  const int* fbody;      // fixture -> body
+  const ShapeV* shapes;  // per fixture
+  const float4* fmat;    // friction, restitution, threshold
+  const int4* fshape;    // radius bits in .z
+  const int* btype;
+  float4* pos;           // sweep c, a
+  float4* pos0;          // sweep c0, a0
+  float4* vel;
+  float4* xf;
+  const float4* bmass;   // inv_mass, inv_I, local centre
+  float* alpha0;
+  int* awake;            // null: every non-static body is awake
+  float* sleep_time;
+  unsigned long long* key;  // smallest (alpha bits << 32 | contact)
+  int* status;              // event result
+  int2* event_bodies;       // the event's two bodies
+  int block_solve;
+  int velocity_iterations;
+  float dt;
+};
+
+__device__ inline SweepD load_sweep(const ToiArgs& a, int b) {
+  SweepD s;
+  const float4 p = a.pos[b], p0 = a.pos0[b], m = a.bmass[b];
+  s.lc = v2(m.z, m.w);
+  s.c0 = v2(p0.x, p0.y);
+  s.a0 = p0.z;
+  s.c = v2(p.x, p.y);
+  s.a = p.z;
+  s.alpha0 = a.alpha0[b];
+  return s;
+}
+
+__device__ inline void store_sweep(const ToiArgs& a, int b, const SweepD& s) {
+  a.pos[b] = make_float4(s.c.x, s.c.y, s.a, 0.0f);
+  a.pos0[b] = make_float4(s.c0.x, s.c0.y, s.a0, 0.0f);
+  a.alpha0[b] = s.alpha0;
+  // synchronize_transform (src/b2_body.rs:974-977)
+  const Rot q = rot(s.a);
+  const V2 p = s.c - mul(q, s.lc);
+  a.xf[b] = make_float4(p.x, p.y, q.s, q.c);
+}
+
+__device__ inline bool is_awake(const ToiArgs& a, int b) { return a.btype[b] != 0 && (!a.awake || a.awake[b]); }
+
+__device__ inline void set_awake(const ToiArgs& a, int b) {
+  if (a.btype[b] == 0 || !a.awake) return;
+  a.awake[b] = 1;
+  a.sleep_time[b] = 0.0f;
+}
+
+// The reference's conditions for a contact to get a time of impact (src/private/dynamics/
+// b2_world.rs:580-621); bullets are not part of this path, so one body must be static or kinematic.
+__device__ inline bool toi_candidate(const ToiArgs& a, int k, int* bA, int* bB) {
+  const int f = a.tflags[k];
+  if ((f & kToiDisabled) || a.tcount[k] > kMaxSubSteps) return false;
+  const int2 fab = a.ab[k];
+  *bA = a.fbody[fab.x];
+  *bB = a.fbody[fab.y];
+  if (!is_awake(a, *bA) && !is_awake(a, *bB)) return false;
+  return a.btype[*bA] != 2 || a.btype[*bB] != 2;
+}
+
+__device__ inline float toi_alpha(const ToiArgs& a, int k, const SweepD& sa, const SweepD& sb, float alpha0) {
+  const int2 fab = a.ab[k];
+  bool touching;
+  const float beta = time_of_impact(proxy_of(a.shapes[fab.x]), proxy_of(a.shapes[fab.y]), sa, sb, &touching);
+  return touching ? bmin(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
+}
+
+// First scan of the pass: every alpha0 is 0 (src/private/dynamics/b2_world.rs:546-561).
+__global__ void __launch_bounds__(kToiBlock) k_toi_first(ToiArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int b = k; b < a.nb; b += gridDim.x * blockDim.x) a.alpha0[b] = 0.0f;
+  if (k >= a.n) return;
+  a.tcount[k] = 0;
+  a.tflags[k] = 0;
+  a.toi[k] = 1.0f;
+  int bA, bB;
+  if (!toi_candidate(a, k, &bA, &bB)) return;
+  SweepD sa = load_sweep(a, bA), sb = load_sweep(a, bB);
+  sa.alpha0 = 0.0f;
+  sb.alpha0 = 0.0f;
+  a.toi[k] = toi_alpha(a, k, sa, sb, 0.0f);
+  a.tflags[k] = kToiValid;
+}
+
+__global__ void __launch_bounds__(kToiBlock) k_toi_min(ToiArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  if (a.tflags[k] != kToiValid || a.tcount[k] > kMaxSubSteps) return;
+  const float alpha = a.toi[k];
+  if (!(alpha < 1.0f)) return;
+  const unsigned long long key =
+      (static_cast<unsigned long long>(__float_as_uint(alpha)) << 32) | static_cast<unsigned>(k);
+  atomicMin(a.key, key);
+}
+
+// Later scans: contacts whose time of impact is not cached, in set order. The sweep alignment
+// advances a body's sweep, which later contacts of the same body see, so it runs on one thread
+// in order; the times of impact of the chunk then run in parallel from the aligned sweeps.
+__global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) {
+  __shared__ int list[kRescanBlock];
+  __shared__ int count;
+  __shared__ float al0[kRescanBlock];
+  __shared__ SweepD snap[2][kRescanBlock / 4];
+  const int t = threadIdx.x;
+  for (int base = 0; base < a.n; base += kRescanBlock) {
+    const int k = base + t;
+    int bA = -1, bB = -1;
+    bool need = false;
+    if (k < a.n && !(a.tflags[k] & kToiValid)) need = toi_candidate(a, k, &bA, &bB);
+    if (t == 0) count = 0;
+    __syncthreads();
+    if (need) list[atomicAdd(&count, 1)] = k;
+    __syncthreads();
+    // set order within the chunk
+    if (t == 0) {
+      const int m = count;
+      for (int i = 1; i < m; ++i) {
+        const int x = list[i];
+        int j = i - 1;
+        while (j >= 0 && list[j] > x) {
+          list[j + 1] = list[j];
+          --j;
+        }
+        list[j + 1] = x;
+      }
+    }
+    __syncthreads();
+    const int m = count;
+    for (int c0 = 0; c0 < m; c0 += kRescanBlock / 4) {
+      const int cm = min(m - c0, kRescanBlock / 4);
+      if (t == 0) {
+        for (int i = 0; i < cm; ++i) {
+          const int kk = list[c0 + i];
+          const int2 fab = a.ab[kk];
+          const int ba = a.fbody[fab.x], bb = a.fbody[fab.y];
+          SweepD sa = load_sweep(a, ba), sb = load_sweep(a, bb);
+          float alpha0 = sa.alpha0;
+          if (sa.alpha0 < sb.alpha0) {
+            alpha0 = sb.alpha0;
+            sweep_advance(sa, alpha0);
+            a.pos0[ba] = make_float4(sa.c0.x, sa.c0.y, sa.a0, 0.0f);
+            a.alpha0[ba] = sa.alpha0;
+          } else if (sb.alpha0 < sa.alpha0) {
+            alpha0 = sa.alpha0;
+            sweep_advance(sb, alpha0);
+            a.pos0[bb] = make_float4(sb.c0.x, sb.c0.y, sb.a0, 0.0f);
+            a.alpha0[bb] = sb.alpha0;
+          }
+          snap[0][i] = sa;
+          snap[1][i] = sb;
+          al0[i] = alpha0;
+        }
+      }
+      __syncthreads();
+      if (t < cm) {
+        const int kk = list[c0 + t];
+        a.toi[kk] = toi_alpha(a, kk, snap[0][t], snap[1][t], al0[t]);
+        a.tflags[kk] = kToiValid;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// B2contact::update for one contact (src/private/dynamics/b2_contact.rs:103-217) with the bodies'
+// current transforms; returns whether it is touching.
+__device__ bool update_contact(const ToiArgs& a, int k) {
+  const int2 fab = a.ab[k];
+  const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
+  const Xf xa = xf_of(a.xf[bA]), xb = xf_of(a.xf[bB]);
+  Manifold m;
+  evaluate(&m, a.shapes[fab.x], xa, a.shapes[fab.y], xb);
+  const int ocount = a.hdr[k] >> 8;
+  const uint2 oid = a.ids[k];
+  const float4 o0 = a.pt0[k], o1 = a.pt1[k];
+  float imp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < m.count; ++i) {
+    if (ocount > 0 && m.id[i] == oid.x) {
+      imp[2 * i] = o0.z;
+      imp[2 * i + 1] = o0.w;
+    } else if (ocount > 1 && m.id[i] == oid.y) {
+      imp[2 * i] = o1.z;
+      imp[2 * i + 1] = o1.w;
+    }
+  }
+  a.hdr[k] = m.type | (m.count << 8);
+  a.geo[k] = make_float4(m.local_normal.x, m.local_normal.y, m.local_point.x, m.local_point.y);
+  a.pt0[k] = make_float4(m.lp[0].x, m.lp[0].y, imp[0], imp[1]);
+  a.pt1[k] = make_float4(m.lp[1].x, m.lp[1].y, imp[2], imp[3]);
+  a.ids[k] = make_uint2(m.id[0], m.id[1]);
+  const bool touching = m.count > 0;
+  if (touching != (ocount > 0)) {
+    set_awake(a, bA);
+    set_awake(a, bB);
+  }
+  a.tflags[k] &= ~kToiDisabled;  // update enables the contact
+  return touching;
+}
+
+struct EventShared {
+  int bodies[kMaxToiBodies];
+  SweepD sw[kMaxToiBodies];
+  int nbody;
+  int contacts[kMaxToiContacts];
+  int ncontact;
+  int cand[kMaxBodyContacts];
+  int ncand;
+  int overflow;
+  TPos<float> P[kMaxToiBodies];
+  TVel<float> V[kMaxToiBodies];
+  TVelC<float> vc[kMaxToiContacts];
+  float lam[kMaxToiContacts][4];
+};
+
+__device__ inline int island_slot(const EventShared& s, int b) {
+  for (int i = 0; i < s.nbody; ++i)
+    if (s.bodies[i] == b) return i;
+  return -1;
+}
+
+__device__ inline ContactConst contact_const(const ToiArgs& a, int k, const EventShared& s, bool toi_masses,
+                                             int* ia, int* ib, float* friction, TGeom<float>* g) {
+  const int2 fab = a.ab[k];
+  const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
+  *ia = island_slot(s, bA);
+  *ib = island_slot(s, bB);
+  const float4 matA = a.fmat[fab.x], matB = a.fmat[fab.y];
+  ContactConst c;
+  const int h = a.hdr[k];
+  c.type = h & 0xff;
+  c.count = h >> 8;
+  // mixing laws, src/b2_contact.rs:29-46
+  *friction = sqrtf(matA.x * matB.x);
+  c.restitution = matA.y > matB.y ? matA.y : matB.y;
+  c.threshold = matA.z < matB.z ? matA.z : matB.z;
+  c.rA = __int_as_float(a.fshape[fab.x].z);
+  c.rB = __int_as_float(a.fshape[fab.y].z);
+  const float4 mA = a.bmass[bA], mB = a.bmass[bB];
+  c.mA = mA.x;
+  c.iA = mA.y;
+  c.mB = mB.x;
+  c.iB = mB.y;
+  // the TOI position solve moves only the two TOI bodies (island slots 0 and 1)
+  if (toi_masses && *ia > 1) c.mA = c.iA = 0.0f;
+  if (toi_masses && *ib > 1) c.mB = c.iB = 0.0f;
+  c.lcA[0] = mA.z;
+  c.lcA[1] = mA.w;
+  c.lcB[0] = mB.z;
+  c.lcB[1] = mB.w;
+  const float4 gg = a.geo[k], p0 = a.pt0[k], p1 = a.pt1[k];
+  g->ln = v2(gg.x, gg.y);
+  g->lpt = v2(gg.z, gg.w);
+  g->lp[0] = v2(p0.x, p0.y);
+  g->lp[1] = v2(p1.x, p1.y);
+  return c;
+}
+
+// One TOI event (src/private/dynamics/b2_world.rs:683-898), one workgroup.
+__global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
+  __shared__ EventShared s;
+  __shared__ float min_alpha;
+  __shared__ int min_k, solid;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const unsigned long long key = *a.key;
+    min_k = static_cast<int>(key & 0xffffffffu);
+    min_alpha = __uint_as_float(static_cast<unsigned>(key >> 32));
+    s.overflow = 0;
+    const int k = min_k;
+    const int2 fab = a.ab[k];
+    const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
+    const SweepD backup1 = load_sweep(a, bA), backup2 = load_sweep(a, bB);
+    // B2body::advance (src/b2_body.rs:979-986)
+    SweepD sa = backup1, sb = backup2;
+    sweep_advance(sa, min_alpha);
+    sa.c = sa.c0;
+    sa.a = sa.a0;
+    store_sweep(a, bA, sa);
+    sweep_advance(sb, min_alpha);
+    sb.c = sb.c0;
+    sb.a = sb.a0;
+    store_sweep(a, bB, sb);
+    *a.event_bodies = make_int2(bA, bB);
+    const bool touching = update_contact(a, k);
+    a.tflags[k] &= ~kToiValid;
+    a.tcount[k] += 1;
+    solid = touching ? 1 : 0;
+    if (!touching) {
+      a.tflags[k] |= kToiDisabled;
+      store_sweep(a, bA, backup1);
+      store_sweep(a, bB, backup2);
+      *a.status = kEventNotSolid;
+    } else {
+      set_awake(a, bA);
+      set_awake(a, bB);
+      s.bodies[0] = bA;
+      s.sw[0] = sa;
+      s.bodies[1] = bB;
+      s.sw[1] = sb;
+      s.nbody = 2;
+      s.contacts[0] = k;
+      s.ncontact = 1;
+    }
+  }
+  __syncthreads();
+  if (!solid) return;
+  // gather the static and kinematic contacts of each dynamic TOI body, in set order
+  for (int side = 0; side < 2; ++side) {
+    const int body = s.bodies[side];
+    if (a.btype[body] != 2) continue;
+    if (t == 0) s.ncand = 0;
+    __syncthreads();
+    for (int base = 0; base < a.n; base += kEventBlock) {
+      const int k = base + t;
+      bool mine = false;
+      if (k < a.n) {
+        const int2 fab = a.ab[k];
+        mine = a.fbody[fab.x] == body || a.fbody[fab.y] == body;
+      }
+      const unsigned long long ballot = __ballot(mine);
+      // wave-ordered append: waves in turn, lanes by rank within the wave
+      for (int w = 0; w < kEventBlock / warpSize; ++w) {
+        if (t / warpSize == w && mine) {
+          const int lane = t % warpSize;
+          const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
+          const int slot = s.ncand + rank;
+          if (slot < kMaxBodyContacts) s.cand[slot] = k;
+          else s.overflow = 1;
+        }
+        __syncthreads();
+        if (t == w * warpSize) s.ncand += __popcll(ballot);
+        __syncthreads();
+      }
+    }
+    if (t == 0 && !s.overflow) {
+      for (int i = 0; i < s.ncand; ++i) {
+        const int k = s.cand[i];
+        bool in = false;
+        for (int j = 0; j < s.ncontact; ++j) in = in || s.contacts[j] == k;
+        if (in) continue;
+        const int2 fab = a.ab[k];
+        const int other = a.fbody[fab.x] == body ? a.fbody[fab.y] : a.fbody[fab.x];
+        if (a.btype[other] == 2) continue;
+        const int slot = island_slot(s, other);
+        const SweepD backup = load_sweep(a, other);
+        SweepD so = backup;
+        if (slot < 0) {
+          sweep_advance(so, min_alpha);
+          so.c = so.c0;
+          so.a = so.a0;
+          store_sweep(a, other, so);
+        }
+        if (!update_contact(a, k)) {
+          store_sweep(a, other, backup);
+          continue;
+        }
+        if (s.ncontact == kMaxToiContacts) {
+          s.overflow = 1;
+          break;
+        }
+        s.contacts[s.ncontact++] = k;
+        if (slot >= 0) continue;
+        if (s.nbody == kMaxToiBodies) {
+          s.overflow = 1;
+          break;
+        }
+        set_awake(a, other);
+        s.bodies[s.nbody] = other;
+        s.sw[s.nbody] = so;
+        ++s.nbody;
+      }
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  if (s.overflow) {
+    *a.status = kEventCapacity;
+    return;
+  }
+  // B2island::solve_toi (src/private/dynamics/b2_island_private.rs:330-458)
+  const float h = (1.0f - min_alpha) * a.dt;
+  for (int i = 0; i < s.nbody; ++i) {
+    const int b = s.bodies[i];
+    s.P[i].c = s.sw[i].c;
+    s.P[i].a = s.sw[i].a;
+    const float4 v = a.vel[b];
+    s.V[i].v = v2(v.x, v.y);
+    s.V[i].w = v.z;
+  }
+  for (int it = 0; it < 20; ++it) {
+    float min_sep = 0.0f;
+    for (int j = 0; j < s.ncontact; ++j) {
+      int ia, ib;
+      float fr;
+      TGeom<float> g;
+      const ContactConst c = contact_const(a, s.contacts[j], s, true, &ia, &ib, &fr, &g);
+      min_sep = bmin(min_sep, solve_position_t(c, g, s.P[ia], s.P[ib], double(kToiBaumgarte)));
+    }
+    if (min_sep >= -1.5f * kSlop) break;
+  }
+  s.sw[0].c0 = s.P[0].c;
+  s.sw[0].a0 = s.P[0].a;
+  s.sw[1].c0 = s.P[1].c;
+  s.sw[1].a0 = s.P[1].a;
+  for (int j = 0; j < s.ncontact; ++j) {
+    int ia, ib;
+    float fr;
+    TGeom<float> g;
+    const ContactConst c = contact_const(a, s.contacts[j], s, false, &ia, &ib, &fr, &g);
+    s.vc[j] = init_velocity_t(c, g, fr, s.P[ia], s.P[ib], s.V[ia], s.V[ib], a.block_solve != 0);
+    s.lam[j][0] = s.lam[j][1] = s.lam[j][2] = s.lam[j][3] = 0.0f;
+  }
+  for (int it = 0; it < a.velocity_iterations; ++it) {
+    for (int j = 0; j < s.ncontact; ++j) {
+      int ia, ib;
+      float fr;
+      TGeom<float> g;
+      const ContactConst c = contact_const(a, s.contacts[j], s, false, &ia, &ib, &fr, &g);
+      solve_velocity_t(s.vc[j], c, s.lam[j], s.V[ia], s.V[ib], a.block_solve != 0);
+    }
+  }
+  for (int i = 0; i < s.nbody; ++i) {
+    integrate_position_t(s.P[i], s.V[i], h);
+    const int b = s.bodies[i];
+    SweepD sw = s.sw[i];
+    sw.c = s.P[i].c;
+    sw.a = s.P[i].a;
+    store_sweep(a, b, sw);
+    a.vel[b] = make_float4(s.V[i].v.x, s.V[i].v.y, s.V[i].w, 0.0f);
+  }
+  *a.status = kEventSolved;
+}
+
+// Clear the TOI flags of every contact of the event's dynamic bodies (the island's other bodies
+// are static or kinematic; src/private/dynamics/b2_world.rs:871-891), and select those bodies for
+// the proxy synchronisation that follows (sel[b] = 1: a view in which only they are awake).
+__global__ void __launch_bounds__(kToiBlock) k_toi_invalidate(ToiArgs a, int* __restrict__ sel) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int2 eb = *a.event_bodies;
+  const int dA = a.btype[eb.x] == 2 ? eb.x : -1, dB = a.btype[eb.y] == 2 ? eb.y : -1;
+  if (k < a.nb) sel[k] = (k == dA || k == dB) ? 1 : 0;
+  if (k >= a.n) return;
+  const int2 fab = a.ab[k];
+  const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
+  if (bA == dA || bA == dB || bB == dA || bB == dB) a.tflags[k] &= ~kToiValid;
+}
+
+// TOI state of a rebuilt contact set from its predecessors (new pairs start uncached).
+__global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __restrict__ old_index,
+                                                         const float* __restrict__ otoi, const int* __restrict__ ocount,
+                                                         const int* __restrict__ oflags, float* __restrict__ toi,
+                                                         int* __restrict__ tcount, int* __restrict__ tflags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int o = old_index[k];
+  toi[k] = o >= 0 ? otoi[o] : 1.0f;
+  tcount[k] = o >= 0 ? ocount[o] : 0;
+  tflags[k] = o >= 0 ? oflags[o] : 0;
+}
+
+}  // namespace
+
+static ToiArgs toi_args(World* w, float dt, int vi) {
+  ContactArrays& c = w->c[w->cur];
+  ToiState& ts = w->toi_state[w->cur];
+  ToiArgs a;
+  a.n = w->n_pairs;
+  a.nb = w->nb;
+  a.ab = c.ab.p;
+  a.hdr = c.hdr.p;
+  a.geo = c.geo.p;
+  a.pt0 = c.pt0.p;
+  a.pt1 = c.pt1.p;
+  a.ids = c.ids.p;
+  a.toi = ts.toi.p;
+  a.tcount = ts.count.p;
+  a.tflags = ts.flags.p;
+  a.fbody = w->f.body.p;
+  a.shapes = w->f.shapev.p;
+  a.fmat = w->f.mat.p;
+  a.fshape = w->f.shape.p;
+  a.btype = w->b.type.p;
+  a.pos = w->b.pos.p;
+  a.pos0 = w->b.pos0.p;
+  a.vel = w->b.vel.p;
+  a.xf = w->b.xf.p;
+  a.bmass = w->b.mass.p;
+  a.alpha0 = w->toi_alpha0.p;
+  a.awake = w->sleep_possible() ? w->b.awake.p : nullptr;
+  a.sleep_time = w->b.sleep_time.p;
+  a.key = w->toi_ctl.p;
+  a.status = reinterpret_cast<int*>(w->toi_ctl.p + 1);
+  a.event_bodies = reinterpret_cast<int2*>(w->toi_ctl.p + 2);
+  a.block_solve = w->block_solve ? 1 : 0;
+  a.velocity_iterations = vi;
+  a.dt = dt;
+  return a;
+}
+
+static int toi_reserve(World* w) {
+  int st;
+  const size_t n = size_t(std::max(w->n_pairs, 1));
+  ToiState& ts = w->toi_state[w->cur];
+  if ((st = ts.toi.reserve(n)) || (st = ts.count.reserve(n)) || (st = ts.flags.reserve(n))) return st;
+  if ((st = w->toi_alpha0.reserve(std::max(w->nb, 1))) || (st = w->toi_sel.reserve(std::max(w->nb, 1))) ||
+      (st = w->toi_ctl.reserve(3)))
+    return st;
+  if (!w->toi_two.p) {
+    if ((st = w->toi_two.reserve(2))) return st;
+    const int two[2] = {0, 1};
+    B2G_HIP(hipMemcpyAsync(w->toi_two.p, two, sizeof(two), hipMemcpyHostToDevice, w->stream));
+    B2G_HIP(hipStreamSynchronize(w->stream));
+  }
+  return B2G_OK;
+}
+
+static int toi_read(World* w, unsigned long long* key, int* status) {
+  unsigned long long h[2];
+  B2G_HIP(hipMemcpyAsync(h, w->toi_ctl.p, sizeof(h), hipMemcpyDeviceToHost, w->stream));
+  B2G_HIP(hipStreamSynchronize(w->stream));
+  *key = h[0];
+  *status = static_cast<int>(h[1] & 0xffffffffu);
+  return B2G_OK;
+}
+
+static int toi_find_min(World* w, float dt, int vi) {
+  static const unsigned long long none = ~0ull;
+  B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &none, sizeof(none), hipMemcpyHostToDevice, w->stream));
+  if (w->n_pairs > 0) {
+    hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                       toi_args(w, dt, vi));
+    B2G_HIP(hipGetLastError());
+  }
+  return B2G_OK;
+}
+
+int carry_manifolds_for_toi(World* w);
+
+// find_new_contacts: a new contact set from the fat AABBs, its manifolds and TOI state carried
+static int toi_rebuild(World* w) {
+  int st;
+  if ((st = phase_broadphase(w))) return st;
+  w->manifolds_valid[w->cur] = false;
+  if ((st = carry_manifolds_for_toi(w))) return st;
+  if ((st = toi_reserve(w))) return st;
+  if (w->n_pairs > 0) {
+    ToiState& o = w->toi_state[1 - w->cur];
+    ToiState& c = w->toi_state[w->cur];
+    hipLaunchKernelGGL(k_toi_carry, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                       w->n_pairs, w->c[w->cur].old_index.p, o.toi.p, o.count.p, o.flags.p, c.toi.p, c.count.p,
+                       c.flags.p);
+    B2G_HIP(hipGetLastError());
+  }
+  return B2G_OK;
+}
+
+// The TOI pass after the solve (src/private/dynamics/b2_world.rs:534-901, step_complete, no
+// sub-stepping). The current contact set is the one find_new_contacts left.
+int phase_toi(World* w, float dt, int vi) {
+  int st;
+  w->toi_events = 0;
+  if ((st = carry_manifolds_for_toi(w))) return st;
+  if ((st = toi_reserve(w))) return st;
+  if (w->n_pairs > 0) {
+    hipLaunchKernelGGL(k_toi_first, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                       toi_args(w, dt, vi));
+    B2G_HIP(hipGetLastError());
+  }
+  // every event either disables its contact or raises its TOI count, so the loop ends; the
+  // bound is a guard against a logic error, not a limit of the reference
+  bool dirty = false;
+  const long long kMaxEvents = 4ll * (kMaxSubSteps + 2) * (static_cast<long long>(w->n_pairs) + w->nb + 16);
+  for (long long iter = 0; iter < kMaxEvents; ++iter) {
+    if ((st = toi_find_min(w, dt, vi))) return st;
+    unsigned long long key;
+    int status;
+    if ((st = toi_read(w, &key, &status))) return st;
+    const float alpha = __builtin_bit_cast(float, static_cast<unsigned>(key >> 32));
+    if (key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha) {
+      // manifolds that an event changed after the last rebuild become the next collide's
+      // predecessors through one more rebuild
+      return dirty ? toi_rebuild(w) : B2G_OK;
+    }
+    hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(kEventBlock), 0, w->stream, toi_args(w, dt, vi));
+    B2G_HIP(hipGetLastError());
+    if ((st = toi_read(w, &key, &status))) return st;
+    if (status == kEventCapacity) {
+      set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
+      return B2G_ERR_CAPACITY;
+    }
+    dirty = true;
+    if (status == kEventSolved) {
+      ++w->toi_events;
+      // the displaced bodies' contacts lose their cached times, their proxies are
+      // re-synchronised, and the contact set is rebuilt (find_new_contacts)
+      const int nmax = std::max(w->n_pairs, w->nb);
+      hipLaunchKernelGGL(k_toi_invalidate, dim3(blocks_for(nmax, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                         toi_args(w, dt, vi), w->toi_sel.p);
+      B2G_HIP(hipGetLastError());
+      if ((st = phase_synchronize_view(w, IslandView{w->toi_sel.p, w->toi_two.p}))) return st;
+      if ((st = toi_rebuild(w))) return st;
+      dirty = false;
+    }
+    if (w->n_pairs > 0) {
+      hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, toi_args(w, dt, vi));
+      B2G_HIP(hipGetLastError());
+    }
+  }
+  set_error("continuous collision: the TOI pass did not end");
+  return B2G_ERR_DEVICE;
+}
+
+}  // namespace b2g

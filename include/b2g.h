@@ -126,7 +126,8 @@ typedef struct b2g_manifold {
 typedef struct b2g_profile {
   float step, collide, solve, broadphase; /* milliseconds, device time */
   int32_t pairs, touching, colours, islands, position_iterations;
-  int32_t pad[3];
+  int32_t toi_events; /* continuous collision: TOI events solved in the step */
+  int32_t pad[2];
 } b2g_profile;
 
 typedef struct b2g_world b2g_world;
@@ -141,8 +142,10 @@ int b2g_world_create(float gravity_x, float gravity_y, b2g_world** out);
 int b2g_world_destroy(b2g_world* world);
 
 /* B2world::set_warm_starting / set_allow_sleeping / set_continuous_physics
- * (src/b2_world.rs:164-185) and G_BLOCK_SOLVE (src/b2_contact.rs:25). continuous must be 0:
- * TOI sub-stepping is not on the GPU path. allow_sleep works as in the reference (island sleep
+ * (src/b2_world.rs:164-185) and G_BLOCK_SOLVE (src/b2_contact.rs:25). continuous runs the TOI
+ * pass after each step's solve (src/private/dynamics/b2_world.rs:534-901; no sub-stepping, no
+ * bullets: times of impact against static and kinematic bodies; among equal times of impact the
+ * first contact of the GPU's contact-set order goes first). allow_sleep works as in the reference (island sleep
  * after B2_TIME_TO_SLEEP below the sleep tolerances, waking on touch changes; switching it off
  * wakes every body), with one difference: whether a contact is updated in the collide phase
  * depends on its bodies' awake states at the start of the phase, where the reference's
