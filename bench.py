@@ -10,7 +10,8 @@ are resident in HBM before timing. Prints one JSON line.
 Other workloads (not the default line): --workload batched (4096 worlds, world-steps/s against
 the same worlds stepped one by one), stress (1.18M bodies: body-steps/s, pairs/s, manifolds/s,
 solver body-iterations/s), adjoint (1024 worlds x 64 bodies, 120 taped steps + reverse pass),
-chains (4096 hanging chains of 16 links on revolute joints).
+chains (4096 hanging chains of 16 links on revolute joints). --continuous runs the large world
+with continuous physics (TOI) on.
 """
 import argparse
 import json
@@ -64,14 +65,15 @@ def pair_shape_bytes(w, sbytes):
     return float((sbytes[fab[:, 0]] + sbytes[fab[:, 1]]).mean()) if len(fab) else 0.0
 
 
-def cpu_baseline(scene, seconds=15.0, max_steps=200, joints=None):
+def cpu_baseline(scene, seconds=15.0, max_steps=200, joints=None, continuous=False):
     """The oracle (sequential C++ restatement of the reference step, 1 thread) on the same
     scene, timed on this host for a bounded number of steps from the initial state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import b2o
     oracle = b2o.Oracle("f32")
     w = oracle.world()
-    w.set_flags(warm_starting=True, allow_sleep=False, block_solve=True)  # the GPU run's flags
+    w.set_flags(warm_starting=True, allow_sleep=False, block_solve=True,  # the GPU run's flags
+                continuous=continuous)
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     if joints is not None:
         w.add_joints(joints)
@@ -145,12 +147,23 @@ def base_record(args, metric, value, unit, elapsed, steps, workload, extra):
 def run_large_world(args, b2g, scenes):
     scene = scenes.large_world(n=args.bodies, seed=1)
     w = b2g.World()
-    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
+    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=args.continuous, block_solve=True)
     w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     steps = args.steps if args.steps is not None else 600
+    events = 0
+    if args.continuous:
+        # TOI events over the timed steps, counted in an untimed replay of the same steps
+        probe = b2g.World()
+        probe.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=True, block_solve=True)
+        probe.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+        for i in range(args.warmup + steps):
+            probe.step(1.0 / 60.0, 8, 3)
+            if i >= args.warmup:
+                events += probe.profile()["toi_events"]
+        del probe
     sbytes = shape_bytes(scene.shape_defs)
     pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
     elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
@@ -160,15 +173,19 @@ def run_large_world(args, b2g, scenes):
     n = int(scene.meta["dynamic"])
     out = base_record(args, "body-steps/s (64k-body world)", n * steps / elapsed, "body-steps/s", elapsed, steps,
                       "large_world: %d dynamic bodies, 400x400 m box, dt 1/60, 8/3 iterations, warm start on, "
-                      "%s, continuous off" % (n, "sleep on" if args.sleep else "sleep off"),
+                      "%s, continuous %s" % (n, "sleep on" if args.sleep else "sleep off",
+                                             "on" if args.continuous else "off"),
                       {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    if args.continuous:
+        out["config"]["continuous"] = True
+        out["config"]["toi_events"] = events
     if args.sleep:
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
     out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux,
                                traffic_key=args.bodies == 65536, pair_shape=pair_shape)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene)
+        out["cpu_baseline"] = cpu_baseline(scene, continuous=args.continuous)
     return out
 
 
@@ -318,6 +335,8 @@ def main():
     ap.add_argument("--sleep", action="store_true",
                     help="allow_sleep on (large_world, batched): islands at rest sleep and are skipped")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--continuous", action="store_true",
+                    help="large_world with continuous physics (the TOI pass) on, the reference's default")
     ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint", "chains"))
     ap.add_argument("--roofline-kernel", default="collide", choices=sorted(UNIT_BYTES))
     ap.add_argument("--solver-graph", type=int, default=0, choices=(0, 1),

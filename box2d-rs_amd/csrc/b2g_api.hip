@@ -547,6 +547,7 @@ int b2g_world_destroy(b2g_world* h) {
     rel(t.toi); rel(t.count); rel(t.flags);
   }
   rel(w->toi_alpha0); rel(w->toi_sel); rel(w->toi_two); rel(w->toi_ctl);
+  rel(w->toi_cand); rel(w->toi_need); rel(w->toi_rank); rel(w->toi_snap);
   SolverArrays& s = w->s;
   rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
   rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);

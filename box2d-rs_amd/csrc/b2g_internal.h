@@ -291,7 +291,9 @@ struct World {
   ToiState toi_state[2];              // per contact set, when continuous physics is on
   DevArray<float> toi_alpha0;         // per body: the sweep's alpha0 during the TOI pass
   DevArray<int> toi_sel, toi_two;     // bodies re-synchronised after a TOI event; {0, 1}
-  DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies
+  DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies, counters
+  DevArray<int> toi_cand, toi_need, toi_rank;  // TOI event candidates; contacts to recompute, ranked
+  DevArray<float> toi_snap;                    // aligned sweeps of the contacts to recompute
   int toi_events = 0;                 // TOI events solved in the last step
   int cur = 0;         // index of the current contact set in c[]
   int n_pairs = 0;     // pairs in the current set

@@ -14,14 +14,16 @@
 //   impact, as one 64-bit atomicMin of the float's bits (alpha >= 0) over the index. The
 //   reference takes the first contact of its list among equal times; the GPU's contact set is
 //   ordered by its own (deterministic) pair order instead of the reference's list order.
-// * k_toi_event (one workgroup): the event itself, sequential as in the reference: advance,
-//   update, island gathering over the contact set in set order (the two bodies' contact lists),
-//   the island's TOI solve, the new sweeps and velocities, and the TOI flags of the displaced
-//   bodies' contacts cleared.
-// * then the host re-synchronises proxies and rebuilds the contact set (find_new_contacts), and
-//   k_toi_rescan (one workgroup) recomputes the times of impact that are no longer cached, in set
-//   order, with the reference's sweep alignment (the body with the earlier alpha0 is advanced to
-//   the other's) done sequentially where it is needed.
+// * k_toi_cand (grid) lists the contacts of the event's two bodies; k_toi_event (one workgroup)
+//   runs the event itself, sequential as in the reference: advance, update, island gathering over
+//   those contacts in set order (the two bodies' contact lists), the island's TOI solve, the new
+//   sweeps and velocities. k_toi_post clears the TOI flags of the displaced bodies' contacts.
+// * the host re-synchronises the displaced bodies' proxies and rebuilds the contact set
+//   (find_new_contacts) when one of them was re-fattened (only moved proxies can add pairs).
+// * k_toi_need (grid) lists the contacts whose time of impact is no longer cached, and
+//   k_toi_rescan (one workgroup) ranks them into set order, does the reference's sweep alignment
+//   (the body with the earlier alpha0 is advanced to the other's) on one thread in that order,
+//   and computes their times of impact in parallel.
 // Time of impact: GJK distance (src/private/collision/b2_distance.rs:404-530) and conservative
 // advancement with separating axes (src/private/collision/b2_time_of_impact.rs:17-463).
 #include "b2g_internal.h"
@@ -426,6 +428,13 @@ struct ToiArgs {
   unsigned long long* key;  // smallest (alpha bits << 32 | contact)
   int* status;              // event result
   int2* event_bodies;       // the event's two bodies
+  int* flags4;              // [0] any proxy re-fattened, [1] need count, [2] / [3] candidate counts
+  int* cand;                // contacts of the event's bodies: A's at [0, kMaxBodyContacts), B's after
+  int* need;                // contacts whose time of impact is recomputed (cap n)
+  int* rank;                // their set-order rank
+  float* snap;              // per need entry: both aligned sweeps and alpha0 (19 floats)
+  const int* moved;         // per proxy: re-fattened by the last synchronisation
+  int np;
   int block_solve;
   int velocity_iterations;
   float dt;
@@ -508,72 +517,90 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_min(ToiArgs a) {
   atomicMin(a.key, key);
 }
 
-// Later scans: contacts whose time of impact is not cached, in set order. The sweep alignment
-// advances a body's sweep, which later contacts of the same body see, so it runs on one thread
-// in order; the times of impact of the chunk then run in parallel from the aligned sweeps.
+// Later scans: contacts whose time of impact is not cached. k_toi_need lists them (any order),
+// k_toi_rescan ranks the list into set order, aligns the sweeps on one thread in that order (an
+// alignment advances a body's sweep, which later contacts of the same body see) and then computes
+// the times of impact in parallel from the aligned sweeps.
+__global__ void __launch_bounds__(kToiBlock) k_toi_need(ToiArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n || (a.tflags[k] & kToiValid)) return;
+  int bA, bB;
+  if (toi_candidate(a, k, &bA, &bB)) a.need[atomicAdd(&a.flags4[1], 1)] = k;
+}
+
 __global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) {
-  __shared__ int list[kRescanBlock];
-  __shared__ int count;
-  __shared__ float al0[kRescanBlock];
-  __shared__ SweepD snap[2][kRescanBlock / 4];
   const int t = threadIdx.x;
-  for (int base = 0; base < a.n; base += kRescanBlock) {
-    const int k = base + t;
-    int bA = -1, bB = -1;
-    bool need = false;
-    if (k < a.n && !(a.tflags[k] & kToiValid)) need = toi_candidate(a, k, &bA, &bB);
-    if (t == 0) count = 0;
-    __syncthreads();
-    if (need) list[atomicAdd(&count, 1)] = k;
-    __syncthreads();
-    // set order within the chunk
-    if (t == 0) {
-      const int m = count;
-      for (int i = 1; i < m; ++i) {
-        const int x = list[i];
-        int j = i - 1;
-        while (j >= 0 && list[j] > x) {
-          list[j + 1] = list[j];
-          --j;
-        }
-        list[j + 1] = x;
+  const int m = a.flags4[1];
+  for (int i = t; i < m; i += kRescanBlock) {
+    const int x = a.need[i];
+    int r = 0;
+    for (int j = 0; j < m; ++j) r += a.need[j] < x ? 1 : 0;
+    a.rank[r] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int i = 0; i < m; ++i) {
+      const int k = a.rank[i];
+      const int2 fab = a.ab[k];
+      const int ba = a.fbody[fab.x], bb = a.fbody[fab.y];
+      SweepD sa = load_sweep(a, ba), sb = load_sweep(a, bb);
+      float alpha0 = sa.alpha0;
+      if (sa.alpha0 < sb.alpha0) {
+        alpha0 = sb.alpha0;
+        sweep_advance(sa, alpha0);
+        a.pos0[ba] = make_float4(sa.c0.x, sa.c0.y, sa.a0, 0.0f);
+        a.alpha0[ba] = sa.alpha0;
+      } else if (sb.alpha0 < sa.alpha0) {
+        alpha0 = sa.alpha0;
+        sweep_advance(sb, alpha0);
+        a.pos0[bb] = make_float4(sb.c0.x, sb.c0.y, sb.a0, 0.0f);
+        a.alpha0[bb] = sb.alpha0;
       }
+      float* q = a.snap + 19 * i;
+      const SweepD* sw[2] = {&sa, &sb};
+      for (int e = 0; e < 2; ++e) {
+        const SweepD& x = *sw[e];
+        q[9 * e + 0] = x.lc.x; q[9 * e + 1] = x.lc.y; q[9 * e + 2] = x.c0.x; q[9 * e + 3] = x.c0.y;
+        q[9 * e + 4] = x.c.x; q[9 * e + 5] = x.c.y; q[9 * e + 6] = x.a0; q[9 * e + 7] = x.a; q[9 * e + 8] = x.alpha0;
+      }
+      q[18] = alpha0;
     }
-    __syncthreads();
-    const int m = count;
-    for (int c0 = 0; c0 < m; c0 += kRescanBlock / 4) {
-      const int cm = min(m - c0, kRescanBlock / 4);
-      if (t == 0) {
-        for (int i = 0; i < cm; ++i) {
-          const int kk = list[c0 + i];
-          const int2 fab = a.ab[kk];
-          const int ba = a.fbody[fab.x], bb = a.fbody[fab.y];
-          SweepD sa = load_sweep(a, ba), sb = load_sweep(a, bb);
-          float alpha0 = sa.alpha0;
-          if (sa.alpha0 < sb.alpha0) {
-            alpha0 = sb.alpha0;
-            sweep_advance(sa, alpha0);
-            a.pos0[ba] = make_float4(sa.c0.x, sa.c0.y, sa.a0, 0.0f);
-            a.alpha0[ba] = sa.alpha0;
-          } else if (sb.alpha0 < sa.alpha0) {
-            alpha0 = sa.alpha0;
-            sweep_advance(sb, alpha0);
-            a.pos0[bb] = make_float4(sb.c0.x, sb.c0.y, sb.a0, 0.0f);
-            a.alpha0[bb] = sb.alpha0;
-          }
-          snap[0][i] = sa;
-          snap[1][i] = sb;
-          al0[i] = alpha0;
-        }
-      }
-      __syncthreads();
-      if (t < cm) {
-        const int kk = list[c0 + t];
-        a.toi[kk] = toi_alpha(a, kk, snap[0][t], snap[1][t], al0[t]);
-        a.tflags[kk] = kToiValid;
-      }
-      __syncthreads();
+  }
+  __syncthreads();
+  for (int i = t; i < m; i += kRescanBlock) {
+    const int k = a.rank[i];
+    const float* q = a.snap + 19 * i;
+    SweepD sw[2];
+    for (int e = 0; e < 2; ++e) {
+      sw[e].lc = v2(q[9 * e + 0], q[9 * e + 1]);
+      sw[e].c0 = v2(q[9 * e + 2], q[9 * e + 3]);
+      sw[e].c = v2(q[9 * e + 4], q[9 * e + 5]);
+      sw[e].a0 = q[9 * e + 6];
+      sw[e].a = q[9 * e + 7];
+      sw[e].alpha0 = q[9 * e + 8];
     }
+    a.toi[k] = toi_alpha(a, k, sw[0], sw[1], q[18]);
+    a.tflags[k] = kToiValid;
+  }
+}
+
+// The contacts of the coming event's two bodies (dynamic ones only), in any order; the event
+// kernel sorts them into set order.
+__global__ void __launch_bounds__(kToiBlock) k_toi_cand(ToiArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const int km = static_cast<int>(*a.key & 0xffffffffu);
+  const int2 fm = a.ab[km];
+  const int bA = a.fbody[fm.x], bB = a.fbody[fm.y];
+  const int2 fab = a.ab[k];
+  const int x = a.fbody[fab.x], y = a.fbody[fab.y];
+  if (a.btype[bA] == 2 && (x == bA || y == bA)) {
+    const int slot = atomicAdd(&a.flags4[2], 1);
+    if (slot < kMaxBodyContacts) a.cand[slot] = k;
+  }
+  if (a.btype[bB] == 2 && (x == bB || y == bB)) {
+    const int slot = atomicAdd(&a.flags4[3], 1);
+    if (slot < kMaxBodyContacts) a.cand[kMaxBodyContacts + slot] = k;
   }
 }
 
@@ -719,75 +746,65 @@ __global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
   }
   __syncthreads();
   if (!solid) return;
+  if (t != 0) return;
   // gather the static and kinematic contacts of each dynamic TOI body, in set order
   for (int side = 0; side < 2; ++side) {
     const int body = s.bodies[side];
     if (a.btype[body] != 2) continue;
-    if (t == 0) s.ncand = 0;
-    __syncthreads();
-    for (int base = 0; base < a.n; base += kEventBlock) {
-      const int k = base + t;
-      bool mine = false;
-      if (k < a.n) {
-        const int2 fab = a.ab[k];
-        mine = a.fbody[fab.x] == body || a.fbody[fab.y] == body;
-      }
-      const unsigned long long ballot = __ballot(mine);
-      // wave-ordered append: waves in turn, lanes by rank within the wave
-      for (int w = 0; w < kEventBlock / warpSize; ++w) {
-        if (t / warpSize == w && mine) {
-          const int lane = t % warpSize;
-          const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
-          const int slot = s.ncand + rank;
-          if (slot < kMaxBodyContacts) s.cand[slot] = k;
-          else s.overflow = 1;
-        }
-        __syncthreads();
-        if (t == w * warpSize) s.ncand += __popcll(ballot);
-        __syncthreads();
-      }
+    const int nc = a.flags4[2 + side];
+    if (nc > kMaxBodyContacts) {
+      s.overflow = 1;
+      break;
     }
-    if (t == 0 && !s.overflow) {
-      for (int i = 0; i < s.ncand; ++i) {
-        const int k = s.cand[i];
-        bool in = false;
-        for (int j = 0; j < s.ncontact; ++j) in = in || s.contacts[j] == k;
-        if (in) continue;
-        const int2 fab = a.ab[k];
-        const int other = a.fbody[fab.x] == body ? a.fbody[fab.y] : a.fbody[fab.x];
-        if (a.btype[other] == 2) continue;
-        const int slot = island_slot(s, other);
-        const SweepD backup = load_sweep(a, other);
-        SweepD so = backup;
-        if (slot < 0) {
-          sweep_advance(so, min_alpha);
-          so.c = so.c0;
-          so.a = so.a0;
-          store_sweep(a, other, so);
-        }
-        if (!update_contact(a, k)) {
-          store_sweep(a, other, backup);
-          continue;
-        }
-        if (s.ncontact == kMaxToiContacts) {
-          s.overflow = 1;
-          break;
-        }
-        s.contacts[s.ncontact++] = k;
-        if (slot >= 0) continue;
-        if (s.nbody == kMaxToiBodies) {
-          s.overflow = 1;
-          break;
-        }
-        set_awake(a, other);
-        s.bodies[s.nbody] = other;
-        s.sw[s.nbody] = so;
-        ++s.nbody;
+    s.ncand = nc;
+    for (int i = 0; i < nc; ++i) s.cand[i] = a.cand[side * kMaxBodyContacts + i];
+    for (int i = 1; i < nc; ++i) {
+      const int x = s.cand[i];
+      int j = i - 1;
+      while (j >= 0 && s.cand[j] > x) {
+        s.cand[j + 1] = s.cand[j];
+        --j;
       }
+      s.cand[j + 1] = x;
     }
-    __syncthreads();
+    for (int i = 0; i < s.ncand; ++i) {
+      const int k = s.cand[i];
+      bool in = false;
+      for (int j = 0; j < s.ncontact; ++j) in = in || s.contacts[j] == k;
+      if (in) continue;
+      const int2 fab = a.ab[k];
+      const int other = a.fbody[fab.x] == body ? a.fbody[fab.y] : a.fbody[fab.x];
+      if (a.btype[other] == 2) continue;
+      const int slot = island_slot(s, other);
+      const SweepD backup = load_sweep(a, other);
+      SweepD so = backup;
+      if (slot < 0) {
+        sweep_advance(so, min_alpha);
+        so.c = so.c0;
+        so.a = so.a0;
+        store_sweep(a, other, so);
+      }
+      if (!update_contact(a, k)) {
+        store_sweep(a, other, backup);
+        continue;
+      }
+      if (s.ncontact == kMaxToiContacts) {
+        s.overflow = 1;
+        break;
+      }
+      s.contacts[s.ncontact++] = k;
+      if (slot >= 0) continue;
+      if (s.nbody == kMaxToiBodies) {
+        s.overflow = 1;
+        break;
+      }
+      set_awake(a, other);
+      s.bodies[s.nbody] = other;
+      s.sw[s.nbody] = so;
+      ++s.nbody;
+    }
+    if (s.overflow) break;
   }
-  if (t != 0) return;
   if (s.overflow) {
     *a.status = kEventCapacity;
     return;
@@ -846,18 +863,29 @@ __global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
   *a.status = kEventSolved;
 }
 
-// Clear the TOI flags of every contact of the event's dynamic bodies (the island's other bodies
-// are static or kinematic; src/private/dynamics/b2_world.rs:871-891), and select those bodies for
-// the proxy synchronisation that follows (sel[b] = 1: a view in which only they are awake).
-__global__ void __launch_bounds__(kToiBlock) k_toi_invalidate(ToiArgs a, int* __restrict__ sel) {
+// After a solved event: clear the TOI flags of every contact of the event's dynamic bodies (the
+// island's other bodies are static or kinematic; src/private/dynamics/b2_world.rs:871-891) and
+// select those bodies for the proxy synchronisation that follows (sel[b] = 1: a view in which
+// only they are awake). After an event that was not solid nothing is selected.
+__global__ void __launch_bounds__(kToiBlock) k_toi_post(ToiArgs a, int* __restrict__ sel) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool solved = *a.status == kEventSolved;
   const int2 eb = *a.event_bodies;
-  const int dA = a.btype[eb.x] == 2 ? eb.x : -1, dB = a.btype[eb.y] == 2 ? eb.y : -1;
+  const int dA = solved && a.btype[eb.x] == 2 ? eb.x : -1, dB = solved && a.btype[eb.y] == 2 ? eb.y : -1;
+  if (k == 0) {
+    a.flags4[0] = 0;
+    a.flags4[1] = 0;
+  }
   if (k < a.nb) sel[k] = (k == dA || k == dB) ? 1 : 0;
-  if (k >= a.n) return;
+  if (k >= a.n || !solved) return;
   const int2 fab = a.ab[k];
   const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
   if (bA == dA || bA == dB || bB == dA || bB == dB) a.tflags[k] &= ~kToiValid;
+}
+
+__global__ void __launch_bounds__(kToiBlock) k_toi_any_moved(ToiArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.np && a.moved[i]) a.flags4[0] = 1;
 }
 
 // TOI state of a rebuilt contact set from its predecessors (new pairs start uncached).
@@ -906,6 +934,13 @@ static ToiArgs toi_args(World* w, float dt, int vi) {
   a.key = w->toi_ctl.p;
   a.status = reinterpret_cast<int*>(w->toi_ctl.p + 1);
   a.event_bodies = reinterpret_cast<int2*>(w->toi_ctl.p + 2);
+  a.flags4 = reinterpret_cast<int*>(w->toi_ctl.p + 3);
+  a.cand = w->toi_cand.p;
+  a.need = w->toi_need.p;
+  a.rank = w->toi_rank.p;
+  a.snap = w->toi_snap.p;
+  a.moved = w->f.moved.p;
+  a.np = w->np;
   a.block_solve = w->block_solve ? 1 : 0;
   a.velocity_iterations = vi;
   a.dt = dt;
@@ -917,8 +952,11 @@ static int toi_reserve(World* w) {
   const size_t n = size_t(std::max(w->n_pairs, 1));
   ToiState& ts = w->toi_state[w->cur];
   if ((st = ts.toi.reserve(n)) || (st = ts.count.reserve(n)) || (st = ts.flags.reserve(n))) return st;
+  if ((st = w->toi_need.reserve(n)) || (st = w->toi_rank.reserve(n)) || (st = w->toi_snap.reserve(19 * n)) ||
+      (st = w->toi_cand.reserve(2 * kMaxBodyContacts)))
+    return st;
   if ((st = w->toi_alpha0.reserve(std::max(w->nb, 1))) || (st = w->toi_sel.reserve(std::max(w->nb, 1))) ||
-      (st = w->toi_ctl.reserve(3)))
+      (st = w->toi_ctl.reserve(5)))
     return st;
   if (!w->toi_two.p) {
     if ((st = w->toi_two.reserve(2))) return st;
@@ -929,23 +967,17 @@ static int toi_reserve(World* w) {
   return B2G_OK;
 }
 
-static int toi_read(World* w, unsigned long long* key, int* status) {
-  unsigned long long h[2];
-  B2G_HIP(hipMemcpyAsync(h, w->toi_ctl.p, sizeof(h), hipMemcpyDeviceToHost, w->stream));
-  B2G_HIP(hipStreamSynchronize(w->stream));
-  *key = h[0];
-  *status = static_cast<int>(h[1] & 0xffffffffu);
-  return B2G_OK;
-}
+// ctl: [0] key, [1] status, [2] event bodies, [3] / [4] flags4
+struct ToiCtl {
+  unsigned long long key;
+  int status, pad;
+  int bodies[2];
+  int flags4[4];
+};
 
-static int toi_find_min(World* w, float dt, int vi) {
-  static const unsigned long long none = ~0ull;
-  B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &none, sizeof(none), hipMemcpyHostToDevice, w->stream));
-  if (w->n_pairs > 0) {
-    hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                       toi_args(w, dt, vi));
-    B2G_HIP(hipGetLastError());
-  }
+static int toi_read(World* w, ToiCtl* out) {
+  B2G_HIP(hipMemcpyAsync(out, w->toi_ctl.p, sizeof(ToiCtl), hipMemcpyDeviceToHost, w->stream));
+  B2G_HIP(hipStreamSynchronize(w->stream));
   return B2G_OK;
 }
 
@@ -976,48 +1008,68 @@ int phase_toi(World* w, float dt, int vi) {
   w->toi_events = 0;
   if ((st = carry_manifolds_for_toi(w))) return st;
   if ((st = toi_reserve(w))) return st;
+  static const ToiCtl kReset = {~0ull, 0, 0, {0, 0}, {0, 0, 0, 0}};
+  B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
   if (w->n_pairs > 0) {
     hipLaunchKernelGGL(k_toi_first, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
                        toi_args(w, dt, vi));
     B2G_HIP(hipGetLastError());
   }
+  bool dirty = false;
   // every event either disables its contact or raises its TOI count, so the loop ends; the
   // bound is a guard against a logic error, not a limit of the reference
-  bool dirty = false;
   const long long kMaxEvents = 4ll * (kMaxSubSteps + 2) * (static_cast<long long>(w->n_pairs) + w->nb + 16);
   for (long long iter = 0; iter < kMaxEvents; ++iter) {
-    if ((st = toi_find_min(w, dt, vi))) return st;
-    unsigned long long key;
-    int status;
-    if ((st = toi_read(w, &key, &status))) return st;
-    const float alpha = __builtin_bit_cast(float, static_cast<unsigned>(key >> 32));
-    if (key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha) {
+    // the smallest time of impact; the candidate lists of its bodies
+    B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
+    if (w->n_pairs > 0) {
+      hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                         toi_args(w, dt, vi));
+      B2G_HIP(hipGetLastError());
+    }
+    ToiCtl ctl;
+    if ((st = toi_read(w, &ctl))) return st;
+    const float alpha = __builtin_bit_cast(float, static_cast<unsigned>(ctl.key >> 32));
+    if (ctl.key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha) {
       // manifolds that an event changed after the last rebuild become the next collide's
       // predecessors through one more rebuild
       return dirty ? toi_rebuild(w) : B2G_OK;
     }
-    hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(kEventBlock), 0, w->stream, toi_args(w, dt, vi));
+    const ToiArgs a = toi_args(w, dt, vi);
+    hipLaunchKernelGGL(k_toi_cand, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
     B2G_HIP(hipGetLastError());
-    if ((st = toi_read(w, &key, &status))) return st;
-    if (status == kEventCapacity) {
+    hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(kEventBlock), 0, w->stream, a);
+    B2G_HIP(hipGetLastError());
+    // flags cleared, bodies selected, proxies re-synchronised, re-fattening noted
+    const int nmax = std::max(w->n_pairs, w->nb);
+    hipLaunchKernelGGL(k_toi_post, dim3(blocks_for(nmax, kToiBlock)), dim3(kToiBlock), 0, w->stream, a,
+                       w->toi_sel.p);
+    B2G_HIP(hipGetLastError());
+    if ((st = phase_synchronize_view(w, IslandView{w->toi_sel.p, w->toi_two.p}))) return st;
+    if (w->np > 0) {
+      hipLaunchKernelGGL(k_toi_any_moved, dim3(blocks_for(w->np, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
+      B2G_HIP(hipGetLastError());
+    }
+    if ((st = toi_read(w, &ctl))) return st;
+    if (ctl.status == kEventCapacity) {
       set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
       return B2G_ERR_CAPACITY;
     }
     dirty = true;
-    if (status == kEventSolved) {
+    if (ctl.status == kEventSolved) {
       ++w->toi_events;
-      // the displaced bodies' contacts lose their cached times, their proxies are
-      // re-synchronised, and the contact set is rebuilt (find_new_contacts)
-      const int nmax = std::max(w->n_pairs, w->nb);
-      hipLaunchKernelGGL(k_toi_invalidate, dim3(blocks_for(nmax, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                         toi_args(w, dt, vi), w->toi_sel.p);
-      B2G_HIP(hipGetLastError());
-      if ((st = phase_synchronize_view(w, IslandView{w->toi_sel.p, w->toi_two.p}))) return st;
-      if ((st = toi_rebuild(w))) return st;
-      dirty = false;
+      // find_new_contacts: only re-fattened proxies can add pairs (the reference queries the moved
+      // proxies only), so the set is rebuilt only when a proxy was re-fattened
+      if (ctl.flags4[0]) {
+        if ((st = toi_rebuild(w))) return st;
+        dirty = false;
+      }
     }
     if (w->n_pairs > 0) {
-      hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, toi_args(w, dt, vi));
+      const ToiArgs b = toi_args(w, dt, vi);
+      hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b);
+      B2G_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, b);
       B2G_HIP(hipGetLastError());
     }
   }
