@@ -652,6 +652,12 @@ struct EventShared {
   TVel<float> V[kMaxToiBodies];
   TVelC<float> vc[kMaxToiContacts];
   float lam[kMaxToiContacts][4];
+  // per island contact, read once: constants for the position solve (TOI masses) and the
+  // velocity solve, manifold geometry, friction, island slots
+  ContactConst kp[kMaxToiContacts], kv[kMaxToiContacts];
+  TGeom<float> geom[kMaxToiContacts];
+  float fr[kMaxToiContacts];
+  int2 slot[kMaxToiContacts];
 };
 
 __device__ inline int island_slot(const EventShared& s, int b) {
@@ -819,15 +825,20 @@ __global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
     s.V[i].v = v2(v.x, v.y);
     s.V[i].w = v.z;
   }
+  for (int j = 0; j < s.ncontact; ++j) {
+    int ia, ib;
+    s.kv[j] = contact_const(a, s.contacts[j], s, false, &ia, &ib, &s.fr[j], &s.geom[j]);
+    s.kp[j] = s.kv[j];
+    // the TOI position solve moves only the two TOI bodies (island slots 0 and 1)
+    if (ia > 1) s.kp[j].mA = s.kp[j].iA = 0.0f;
+    if (ib > 1) s.kp[j].mB = s.kp[j].iB = 0.0f;
+    s.slot[j] = make_int2(ia, ib);
+  }
   for (int it = 0; it < 20; ++it) {
     float min_sep = 0.0f;
-    for (int j = 0; j < s.ncontact; ++j) {
-      int ia, ib;
-      float fr;
-      TGeom<float> g;
-      const ContactConst c = contact_const(a, s.contacts[j], s, true, &ia, &ib, &fr, &g);
-      min_sep = bmin(min_sep, solve_position_t(c, g, s.P[ia], s.P[ib], double(kToiBaumgarte)));
-    }
+    for (int j = 0; j < s.ncontact; ++j)
+      min_sep = bmin(min_sep, solve_position_t(s.kp[j], s.geom[j], s.P[s.slot[j].x], s.P[s.slot[j].y],
+                                               double(kToiBaumgarte)));
     if (min_sep >= -1.5f * kSlop) break;
   }
   s.sw[0].c0 = s.P[0].c;
@@ -835,22 +846,14 @@ __global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
   s.sw[1].c0 = s.P[1].c;
   s.sw[1].a0 = s.P[1].a;
   for (int j = 0; j < s.ncontact; ++j) {
-    int ia, ib;
-    float fr;
-    TGeom<float> g;
-    const ContactConst c = contact_const(a, s.contacts[j], s, false, &ia, &ib, &fr, &g);
-    s.vc[j] = init_velocity_t(c, g, fr, s.P[ia], s.P[ib], s.V[ia], s.V[ib], a.block_solve != 0);
+    const int2 sl = s.slot[j];
+    s.vc[j] = init_velocity_t(s.kv[j], s.geom[j], s.fr[j], s.P[sl.x], s.P[sl.y], s.V[sl.x], s.V[sl.y],
+                              a.block_solve != 0);
     s.lam[j][0] = s.lam[j][1] = s.lam[j][2] = s.lam[j][3] = 0.0f;
   }
-  for (int it = 0; it < a.velocity_iterations; ++it) {
-    for (int j = 0; j < s.ncontact; ++j) {
-      int ia, ib;
-      float fr;
-      TGeom<float> g;
-      const ContactConst c = contact_const(a, s.contacts[j], s, false, &ia, &ib, &fr, &g);
-      solve_velocity_t(s.vc[j], c, s.lam[j], s.V[ia], s.V[ib], a.block_solve != 0);
-    }
-  }
+  for (int it = 0; it < a.velocity_iterations; ++it)
+    for (int j = 0; j < s.ncontact; ++j)
+      solve_velocity_t(s.vc[j], s.kv[j], s.lam[j], s.V[s.slot[j].x], s.V[s.slot[j].y], a.block_solve != 0);
   for (int i = 0; i < s.nbody; ++i) {
     integrate_position_t(s.P[i], s.V[i], h);
     const int b = s.bodies[i];
