@@ -521,8 +521,11 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_min(ToiArgs a) {
 // k_toi_rescan ranks the list into set order, aligns the sweeps on one thread in that order (an
 // alignment advances a body's sweep, which later contacts of the same body see) and then computes
 // the times of impact in parallel from the aligned sweeps.
-__global__ void __launch_bounds__(kToiBlock) k_toi_need(ToiArgs a) {
+// wait_rebuild: list nothing when the last event re-fattened a proxy (the host rebuilds the
+// contact set first and lists again)
+__global__ void __launch_bounds__(kToiBlock) k_toi_need(ToiArgs a, int wait_rebuild) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wait_rebuild && a.flags4[0]) return;
   if (k >= a.n || (a.tflags[k] & kToiValid)) return;
   int bA, bB;
   if (toi_candidate(a, k, &bA, &bB)) a.need[atomicAdd(&a.flags4[1], 1)] = k;
@@ -581,6 +584,12 @@ __global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) {
     }
     a.toi[k] = toi_alpha(a, k, sw[0], sw[1], q[18]);
     a.tflags[k] = kToiValid;
+  }
+  // ready for the next minimum and the next event's candidate lists
+  if (t == 0) {
+    *a.key = ~0ull;
+    a.flags4[2] = 0;
+    a.flags4[3] = 0;
   }
 }
 
@@ -1018,33 +1027,63 @@ int phase_toi(World* w, float dt, int vi) {
                        toi_args(w, dt, vi));
     B2G_HIP(hipGetLastError());
   }
-  bool dirty = false;
+  if (w->n_pairs > 0) {
+    hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                       toi_args(w, dt, vi));
+    B2G_HIP(hipGetLastError());
+  }
+  bool dirty = false, pending = false;
   // every event either disables its contact or raises its TOI count, so the loop ends; the
   // bound is a guard against a logic error, not a limit of the reference
   const long long kMaxEvents = 4ll * (kMaxSubSteps + 2) * (static_cast<long long>(w->n_pairs) + w->nb + 16);
   for (long long iter = 0; iter < kMaxEvents; ++iter) {
-    // the smallest time of impact; the candidate lists of its bodies
-    B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
-    if (w->n_pairs > 0) {
-      hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                         toi_args(w, dt, vi));
-      B2G_HIP(hipGetLastError());
-    }
     ToiCtl ctl;
     if ((st = toi_read(w, &ctl))) return st;
+    if (pending) {
+      // the last event's outcome; without a re-fattened proxy the rescan and the minimum that
+      // followed it on the device are already the next ones
+      pending = false;
+      if (ctl.status == kEventCapacity) {
+        set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
+        return B2G_ERR_CAPACITY;
+      }
+      dirty = true;
+      if (ctl.status == kEventSolved) {
+        ++w->toi_events;
+        // find_new_contacts: only re-fattened proxies can add pairs (the reference queries the
+        // moved proxies only), so the set is rebuilt only when a proxy was re-fattened
+        if (ctl.flags4[0]) {
+          if ((st = toi_rebuild(w))) return st;
+          dirty = false;
+          if (w->n_pairs > 0) {
+            const ToiArgs b = toi_args(w, dt, vi);
+            hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b,
+                               0);
+            B2G_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, b);
+            B2G_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                               b);
+            B2G_HIP(hipGetLastError());
+          }
+          if ((st = toi_read(w, &ctl))) return st;
+        }
+      }
+    }
     const float alpha = __builtin_bit_cast(float, static_cast<unsigned>(ctl.key >> 32));
     if (ctl.key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha) {
       // manifolds that an event changed after the last rebuild become the next collide's
       // predecessors through one more rebuild
       return dirty ? toi_rebuild(w) : B2G_OK;
     }
+    // the event, the displaced bodies' flags and proxies, then (unless a proxy was re-fattened)
+    // the next scan and minimum, all without the host
     const ToiArgs a = toi_args(w, dt, vi);
+    const int nmax = std::max(w->n_pairs, w->nb);
     hipLaunchKernelGGL(k_toi_cand, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
     B2G_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(kEventBlock), 0, w->stream, a);
     B2G_HIP(hipGetLastError());
-    // flags cleared, bodies selected, proxies re-synchronised, re-fattening noted
-    const int nmax = std::max(w->n_pairs, w->nb);
     hipLaunchKernelGGL(k_toi_post, dim3(blocks_for(nmax, kToiBlock)), dim3(kToiBlock), 0, w->stream, a,
                        w->toi_sel.p);
     B2G_HIP(hipGetLastError());
@@ -1053,28 +1092,13 @@ int phase_toi(World* w, float dt, int vi) {
       hipLaunchKernelGGL(k_toi_any_moved, dim3(blocks_for(w->np, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
       B2G_HIP(hipGetLastError());
     }
-    if ((st = toi_read(w, &ctl))) return st;
-    if (ctl.status == kEventCapacity) {
-      set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
-      return B2G_ERR_CAPACITY;
-    }
-    dirty = true;
-    if (ctl.status == kEventSolved) {
-      ++w->toi_events;
-      // find_new_contacts: only re-fattened proxies can add pairs (the reference queries the moved
-      // proxies only), so the set is rebuilt only when a proxy was re-fattened
-      if (ctl.flags4[0]) {
-        if ((st = toi_rebuild(w))) return st;
-        dirty = false;
-      }
-    }
-    if (w->n_pairs > 0) {
-      const ToiArgs b = toi_args(w, dt, vi);
-      hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b);
-      B2G_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, b);
-      B2G_HIP(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a, 1);
+    B2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, a);
+    B2G_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
+    B2G_HIP(hipGetLastError());
+    pending = true;
   }
   set_error("continuous collision: the TOI pass did not end");
   return B2G_ERR_DEVICE;
