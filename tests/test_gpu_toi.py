@@ -8,9 +8,12 @@ same in the reference's list order and the GPU's contact-set order:
     and event count agree with the oracle's (src/private/dynamics/b2_world.rs:534-901) within 1e-4;
   * twelve boxes falling at 60 m/s onto a thin static ground at different heights and angles: the
     same, over 40 steps, and all of them end resting on the ground;
+  * fast spinning boxes onto a static edge and fast balls onto a kinematic platform moving up: the
+    same comparison over 45 steps, the boxes above the edge and the balls on the platform;
+  * fast spinning boxes onto a static edge and fast balls onto a kinematic platform moving up: the
+    same comparison over 45 steps, the boxes above the edge and the balls on the platform;
   * determinism: two runs of a 40-box rain with sleeping on are bit-identical; a 1024-body mixed
-    world runs 30 steps with continuous physics without error and matches the discrete run's
-    body count of resting bodies above the ground.
+    world runs 30 steps with continuous physics with finite states and nothing falling out.
 """
 import numpy as np
 import pytest
@@ -139,3 +142,45 @@ def test_large_mixed_world_with_continuous():
     dyn = sc.body_defs["type"] == 2
     # nothing falls out of the container either way
     assert (ys[1][dyn] > ys[0][dyn].min() - 1.0).all()
+
+
+def edge_kinematic_scene():
+    """Fast spinning boxes onto a static edge ground, and fast balls onto a kinematic platform
+    moving up: times of impact with edges and with a moving kinematic body."""
+    n = 6
+    bd = np.zeros(2 + 2 * n, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    bd[1]["type"] = 1  # kinematic platform
+    bd[1]["position"] = (40.0, 0.0)
+    bd[1]["linear_velocity"] = (0.0, 2.0)
+    for i in range(n):
+        b = bd[2 + i]
+        b["type"] = 2
+        b["position"] = (3.0 * i, 4.0 + 0.61 * i)
+        b["linear_velocity"] = (1.5 * (i - 2), -70.0)
+        b["angular_velocity"] = 9.0 - 3.0 * i
+        c = bd[2 + n + i]
+        c["type"] = 2
+        c["position"] = (34.0 + 2.5 * i, 5.0 + 0.53 * i)
+        c["linear_velocity"] = (0.0, -65.0)
+    sd = np.zeros(2 + 2 * n, b2g.SHAPE_DEF)
+    sd[0]["type"] = b2g.SHAPE_EDGE
+    sd[0]["params"][:4] = (-10.0, 0.0, 25.0, 0.0)
+    sd[1]["type"] = b2g.SHAPE_BOX
+    sd[1]["params"][:2] = (9.0, 0.05)
+    sd[2:2 + n]["type"] = b2g.SHAPE_BOX
+    sd[2:2 + n]["params"][:, :2] = (0.4, 0.15)
+    sd[2 + n:]["type"] = b2g.SHAPE_CIRCLE
+    sd[2 + n:]["radius"] = 0.12
+    return bd, np.arange(2 + 2 * n, dtype=np.int32), sd, _fixtures(2 + 2 * n), (0.0, -10.0)
+
+
+def test_edges_and_kinematic_bodies(oracle):
+    w, o = _pair(oracle, edge_kinematic_scene(), True)
+    st = _compare(w, o, 45)
+    n = 6
+    platform_y = float(st[1]["c"][1])
+    assert (st["c"][2:2 + n, 1] > 0.0).all(), st["c"][2:2 + n, 1]  # above the edge
+    assert (st["c"][2 + n:, 1] > platform_y).all(), (st["c"][2 + n:, 1], platform_y)  # on the platform
