@@ -1261,6 +1261,11 @@ int b2g_tape_begin(b2g_world* h, int32_t checkpoint_every) {
     set_error("b2g_tape_begin: the adjoint does not hold joints");
     return B2G_ERR_UNSUPPORTED;
   }
+  if (w->continuous) {
+    // TOI events move bodies outside the taped solve
+    set_error("b2g_tape_begin: the adjoint needs continuous physics off");
+    return B2G_ERR_UNSUPPORTED;
+  }
   if (w->sleep_possible()) {
     // the frozen model replays every body every step: no sleeping bodies on a tape
     set_error("b2g_tape_begin: the adjoint needs allow_sleep off and every body awake");

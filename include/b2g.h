@@ -288,7 +288,8 @@ int b2g_broadphase_pairs(b2g_world* world, const float* fat_aabbs, int64_t cap, 
  * position-iteration counts are then held fixed (the derivative of the branch the run took), and
  * the backward pass replays segments from body-state checkpoints taken every
  * checkpoint_every steps. While a tape is open, only b2g_world_step may advance the world and
- * no bodies may be added. */
+ * no bodies may be added. Worlds with joints, sleeping or continuous physics are refused
+ * (B2G_ERR_UNSUPPORTED). */
 int b2g_tape_begin(b2g_world* world, int32_t checkpoint_every);
 int b2g_tape_end(b2g_world* world);
 int b2g_tape_free(b2g_world* world);
