@@ -184,3 +184,13 @@ def test_edges_and_kinematic_bodies(oracle):
     platform_y = float(st[1]["c"][1])
     assert (st["c"][2:2 + n, 1] > 0.0).all(), st["c"][2:2 + n, 1]  # above the edge
     assert (st["c"][2 + n:, 1] > platform_y).all(), (st["c"][2 + n:, 1], platform_y)  # on the platform
+
+
+def test_tape_refuses_continuous_physics():
+    bd, fb, sd, fd, g = wall_scene()
+    w = b2g.World(g)
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=True, block_solve=True)
+    w.add(bd, fb, sd, fd)
+    with pytest.raises(b2g.B2gError):
+        w.tape_begin(4)
+    w.step(DT, 8, 3)  # the world still steps
