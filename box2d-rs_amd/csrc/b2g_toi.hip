@@ -1065,13 +1065,16 @@ int phase_toi(World* w, float dt, int vi) {
             hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
                                b);
             B2G_HIP(hipGetLastError());
+          } else {
+            // no pairs left: the minimum found before the rebuild named the old set
+            B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
           }
           if ((st = toi_read(w, &ctl))) return st;
         }
       }
     }
     const float alpha = __builtin_bit_cast(float, static_cast<unsigned>(ctl.key >> 32));
-    if (ctl.key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha) {
+    if (ctl.key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha || w->n_pairs == 0) {
       // manifolds that an event changed after the last rebuild become the next collide's
       // predecessors through one more rebuild
       return dirty ? toi_rebuild(w) : B2G_OK;
