@@ -29,7 +29,8 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
                                                         const int* __restrict__ btype, const float4* __restrict__ pos0,
                                                         const float4* __restrict__ bmass, const float4* __restrict__ bxf,
                                                         float4* __restrict__ faabb, float4* __restrict__ ffat,
-                                                        int* __restrict__ moved, IslandView iv) {
+                                                        int* __restrict__ moved, IslandView iv,
+                                                        int* __restrict__ any_moved) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int b = fbody[i];
@@ -67,6 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
   }
   ffat[i] = make_float4(fat.lo.x, fat.lo.y, fat.hi.x, fat.hi.y);
   moved[i] = 1;
+  if (any_moved) *any_moved = 1;  // every writer stores 1
 }
 
 // ---- scene bounds and Morton codes ---------------------------------------------------------
@@ -618,17 +620,18 @@ static int reserve_contacts(ContactArrays& c, int n) {
   return B2G_OK;
 }
 
-int phase_synchronize_view(World* w, IslandView iv) {
+int phase_synchronize_view(World* w, IslandView iv, int* any_moved) {
   if (w->np == 0) return B2G_OK;
   hipLaunchKernelGGL(k_synchronize, dim3(blocks_for(w->np, kBlock)), dim3(kBlock), 0, w->stream, w->np, w->f.body.p,
                      w->f.shapev.p, w->b.type.p, w->b.pos0.p, w->b.mass.p, w->b.xf.p, w->f.aabb.p,
-                     w->f.fat.p, w->f.moved.p, iv);
+                     w->f.fat.p, w->f.moved.p, iv, any_moved);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }
 
 int phase_synchronize(World* w) {
-  return phase_synchronize_view(w, IslandView{w->s.parent.p, w->sleep_possible() ? w->s.isl_awake.p : nullptr});
+  return phase_synchronize_view(w, IslandView{w->s.parent.p, w->sleep_possible() ? w->s.isl_awake.p : nullptr},
+                                nullptr);
 }
 
 // New contact set from the current fat AABBs; becomes the current set, the old one is kept

@@ -402,7 +402,9 @@ struct IslandView {
   const int* isl_awake;  // per root; null: every island is awake
   __device__ bool asleep(int b) const { return isl_awake && !isl_awake[parent[b]]; }
 };
-int phase_synchronize_view(World* w, IslandView iv);  // phase_synchronize for the bodies the view holds awake
+// phase_synchronize for the bodies the view holds awake; any_moved (optional) is set to 1 when a
+// proxy is re-fattened
+int phase_synchronize_view(World* w, IslandView iv, int* any_moved);
 
 // Kernel arguments of the joint solve (b2g_joints.hip).
 struct JointArgs {

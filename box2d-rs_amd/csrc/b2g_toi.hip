@@ -895,11 +895,6 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_post(ToiArgs a, int* __restri
   if (bA == dA || bA == dB || bB == dA || bB == dB) a.tflags[k] &= ~kToiValid;
 }
 
-__global__ void __launch_bounds__(kToiBlock) k_toi_any_moved(ToiArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.np && a.moved[i]) a.flags4[0] = 1;
-}
-
 // TOI state of a rebuilt contact set from its predecessors (new pairs start uncached).
 __global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __restrict__ old_index,
                                                          const float* __restrict__ otoi, const int* __restrict__ ocount,
@@ -1090,11 +1085,7 @@ int phase_toi(World* w, float dt, int vi) {
     hipLaunchKernelGGL(k_toi_post, dim3(blocks_for(nmax, kToiBlock)), dim3(kToiBlock), 0, w->stream, a,
                        w->toi_sel.p);
     B2G_HIP(hipGetLastError());
-    if ((st = phase_synchronize_view(w, IslandView{w->toi_sel.p, w->toi_two.p}))) return st;
-    if (w->np > 0) {
-      hipLaunchKernelGGL(k_toi_any_moved, dim3(blocks_for(w->np, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
-      B2G_HIP(hipGetLastError());
-    }
+    if ((st = phase_synchronize_view(w, IslandView{w->toi_sel.p, w->toi_two.p}, a.flags4))) return st;
     hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a, 1);
     B2G_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, a);
