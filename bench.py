@@ -5,13 +5,21 @@ iterations, synchronize + LBVH pair search) of the large-world scene: 65,536 dyn
 (50% circles, 30% boxes, 20% convex polygons) in a static 400 x 400 m box, seeded. Inputs
 are resident in HBM before timing. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps 600] [--warmup 10] [--bodies 65536] [--no-cpu]
+  python bench.py [--gpus 1] [--steps 600] [--warmup 10] [--bodies 65536] [--no-cpu] [--no-extras]
 
-Other workloads (not the default line): --workload batched (4096 worlds, world-steps/s against
-the same worlds stepped one by one), stress (1.18M bodies: body-steps/s, pairs/s, manifolds/s,
-solver body-iterations/s), adjoint (1024 worlds x 64 bodies, 120 taped steps + reverse pass),
-chains (4096 hanging chains of 16 links on revolute joints). --continuous runs the large world
-with continuous physics (TOI) on.
+The line's value covers the timed window (steps warmup+1 .. warmup+steps, named in config).
+Beside it the line carries:
+  full_run     all 600 steps of the same world from its initial state, timed as one block;
+  roofline     k_solve_velocity (the kernel with the largest share of a step) over the window;
+  cpu_baseline the oracle (one core) on the first steps of the same scene;
+  batched      4096 worlds of 16-1024 bodies, 300 steps, against the same worlds one by one;
+  adjoint      1024 worlds x 64 bodies: 120 taped steps and the reverse pass;
+  stress       1,048,576 mixed bodies + 16,384 towers of 8 boxes, 200 steps.
+--no-extras leaves out the last three.
+
+Other workloads as the main line: --workload batched | stress | adjoint | chains (4096 hanging
+chains of 16 links on revolute joints). --continuous runs the large world with continuous
+physics (TOI) on.
 """
 import argparse
 import json
@@ -27,10 +35,13 @@ sys.path.insert(0, os.path.join(ROOT, "box2d-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Roofline kernel: k_collide, the narrowphase, the longest single launch of a step (one launch of
-# about 45 us on this workload). The solver's per-colour kernels take more time in total but each
-# launch lasts 2-10 us, so an event pair around one of them mostly measures its dispatch.
-# Algorithmic HBM bytes of one narrowphase launch (f32/i32 sizes, each read or written once): per
+# Roofline kernel (default): k_solve_velocity, one velocity iteration over the constraints of one
+# colour: per constraint-iteration 224 B (the constraint's rows: bodies, point count, normal and
+# tangent masses, anchors, bias, block matrix and its inverse, impulses; both bodies' velocities
+# read and written). It is the kernel with the largest share of a step's time; its launches are
+# short (a few microseconds), so it is bound by launch and dependent-load latency more than by
+# bandwidth, which the fraction shows.
+# k_collide (--roofline-kernel collide): algorithmic HBM bytes of one narrowphase launch (f32/i32 sizes, each read or written once): per
 # candidate pair 120 B (pair, fixture and body records, the two transforms, the previous
 # manifold's ids and impulses, the new manifold) plus both shapes: circle 12 B, polygon 12 + 16 n
 # (n vertices and normals), edge 36 B. The shape bytes per pair are averaged over the contact
@@ -65,6 +76,24 @@ def pair_shape_bytes(w, sbytes):
     return float((sbytes[fab[:, 0]] + sbytes[fab[:, 1]]).mean()) if len(fab) else 0.0
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count()
+
+
 def cpu_baseline(scene, seconds=15.0, max_steps=200, joints=None, continuous=False):
     """The oracle (sequential C++ restatement of the reference step, 1 thread) on the same
     scene, timed on this host for a bounded number of steps from the initial state."""
@@ -83,21 +112,25 @@ def cpu_baseline(scene, seconds=15.0, max_steps=200, joints=None, continuous=Fal
         steps += 2
     n = int(scene.meta["dynamic"])
     return {"value": n * steps / elapsed, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": "first %d steps of the same %d-body scene, oracle f32 on %s" % (
-                steps, n, platform.processor() or platform.machine())}
+            "sample": "first %d steps of the same %d-body scene, oracle f32, one thread" % (steps, n),
+            "cpu_model": cpu_model(), "nproc": host_cores()}
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, window):
     """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary for this workload (calibrated against a copy kernel), or None when there is none."""
+    summary of this workload (calibrated against a copy kernel), when that summary covers the
+    same step window (first, last); otherwise None."""
     path = os.path.join(ROOT, "profiles", "traffic_k_%s_large_world_65536.json" % kernel)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        rec = json.load(f)
+    if list(rec.get("window", [])) != list(window):
+        return None
+    return rec.get("bytes_per_launch")
 
 
-def roofline(kernel, k_ms, k_launches, k_items, k_aux, traffic_key=None, pair_shape=None):
+def roofline(kernel, k_ms, k_launches, k_items, k_aux, window=None, pair_shape=None):
     bytes_per_launch = UNIT_BYTES[kernel] * k_items / max(k_launches, 1)
     if kernel == "traverse":
         bytes_per_launch += PAIR_BYTES * k_aux / max(k_launches, 1)
@@ -106,7 +139,7 @@ def roofline(kernel, k_ms, k_launches, k_items, k_aux, traffic_key=None, pair_sh
     avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel) if traffic_key else None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, window) if window else None,
             "kernel": "k_" + kernel, "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_launch": bytes_per_launch, "unit_bytes": UNIT_BYTES[kernel],
             "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[kernel]}
@@ -144,20 +177,45 @@ def base_record(args, metric, value, unit, elapsed, steps, workload, extra):
             "config": dict({"workload": workload}, **extra)}
 
 
+def large_world_flags(args):
+    return dict(warm_starting=True, allow_sleep=args.sleep, continuous=args.continuous, block_solve=True)
+
+
+def full_run(args, b2g, scene, steps=600):
+    """All `steps` steps of a fresh copy of the world from its initial state, timed as one block
+    (inputs uploaded before the clock starts), with the roofline kernel over the same block."""
+    w = b2g.World()
+    w.set_flags(**large_world_flags(args))
+    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
+    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    p = w.profile()
+    n = int(scene.meta["dynamic"])
+    out = {"steps": steps, "value": n * steps / elapsed, "unit": "body-steps/s", "ms_per_step": 1000.0 * elapsed / steps,
+           "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}}
+    if args.roofline_kernel != "collide":
+        r = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux)
+        out["roofline"] = {k: r[k] for k in ("kernel", "achieved", "frac", "launches", "avg_launch_us",
+                                             "bytes_per_launch", "items_per_launch")}
+    w.close()
+    return out
+
+
 def run_large_world(args, b2g, scenes):
     scene = scenes.large_world(n=args.bodies, seed=1)
     w = b2g.World()
-    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=args.continuous, block_solve=True)
+    w.set_flags(**large_world_flags(args))
     w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
+    p0 = w.profile()
     steps = args.steps if args.steps is not None else 600
     events = 0
     if args.continuous:
         # TOI events over the timed steps, counted in an untimed replay of the same steps
         probe = b2g.World()
-        probe.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=True, block_solve=True)
+        probe.set_flags(**large_world_flags(args))
         probe.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
         for i in range(args.warmup + steps):
             probe.step(1.0 / 60.0, 8, 3)
@@ -171,11 +229,17 @@ def run_large_world(args, b2g, scenes):
         pair_shape.append(pair_shape_bytes(w, sbytes))
     p = w.profile()
     n = int(scene.meta["dynamic"])
+    window = (args.warmup + 1, args.warmup + steps)
     out = base_record(args, "body-steps/s (64k-body world)", n * steps / elapsed, "body-steps/s", elapsed, steps,
                       "large_world: %d dynamic bodies, 400x400 m box, dt 1/60, 8/3 iterations, warm start on, "
                       "%s, continuous %s" % (n, "sleep on" if args.sleep else "sleep off",
                                              "on" if args.continuous else "off"),
-                      {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+                      {"bodies": n, "window": {"first_step": window[0], "last_step": window[1]},
+                       "touching": {"before_window": p0["touching"], "last_step": p["touching"]},
+                       "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
+    if args.roofline_kernel in ("solve_velocity", "solve_position"):
+        iters = 8 if args.roofline_kernel == "solve_velocity" else 3
+        out["config"]["touching"]["window_mean"] = k_items / (iters * steps)
     if args.continuous:
         out["config"]["continuous"] = True
         out["config"]["toi_events"] = events
@@ -183,7 +247,10 @@ def run_large_world(args, b2g, scenes):
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
     out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux,
-                               traffic_key=args.bodies == 65536, pair_shape=pair_shape)
+                               window=window if args.bodies == 65536 else None, pair_shape=pair_shape)
+    w.close()
+    if not args.no_full_run and args.bodies == 65536 and not args.continuous:
+        out["full_run"] = full_run(args, b2g, scene)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, continuous=args.continuous)
     return out
@@ -270,11 +337,12 @@ def run_stress(args, b2g, scenes):
     and the solver's body-iterations/s (solve phase time)."""
     scene = scenes.stress_world()
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
     w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
     w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
-    steps = args.steps if args.steps is not None else 40
+    steps = args.steps if args.steps is not None else 200
     elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, "traverse")
     n = int(scene.meta["dynamic"])
     _, (c_ms, c_launches, c_items, _) = timed_steps(w, 5, "collide")
@@ -285,6 +353,7 @@ def run_stress(args, b2g, scenes):
     if args.sleep:
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
+    out["pairs_per_step"] = k_aux / max(k_launches, 1)
     out["pairs_per_s"] = k_aux / (k_ms / 1000.0) if k_ms else None
     out["manifolds_per_s"] = c_items / (c_ms / 1000.0) if c_ms else None
     out["solver_body_iterations_per_s"] = n * (8 + 3) / (p["solve"] / 1000.0) if p["solve"] else None
@@ -298,6 +367,7 @@ def run_adjoint(args, b2g, scenes):
     import numpy as np
     batch = scenes.differentiable_worlds(1024, 64)
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)  # a tape needs these
     w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
     w.body_states()
     steps = args.steps if args.steps is not None else 120
@@ -335,10 +405,14 @@ def main():
     ap.add_argument("--sleep", action="store_true",
                     help="allow_sleep on (large_world, batched): islands at rest sleep and are skipped")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="large_world: leave out the batched, adjoint and stress summaries")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="large_world: leave out the 600-step run from the initial state")
     ap.add_argument("--continuous", action="store_true",
                     help="large_world with continuous physics (the TOI pass) on, the reference's default")
     ap.add_argument("--workload", default="large_world", choices=("large_world", "batched", "stress", "adjoint", "chains"))
-    ap.add_argument("--roofline-kernel", default="collide", choices=sorted(UNIT_BYTES))
+    ap.add_argument("--roofline-kernel", default="solve_velocity", choices=sorted(UNIT_BYTES))
     ap.add_argument("--solver-graph", type=int, default=0, choices=(0, 1),
                     help="replay the solve's per-colour launches from a captured graph (default 0: direct "
                          "launches are faster on MI355X; timing a solve kernel launches the solve directly)")
@@ -351,7 +425,37 @@ def main():
     from b2g import scenes
     run = {"large_world": run_large_world, "batched": run_batched, "stress": run_stress,
            "adjoint": run_adjoint, "chains": run_chains}[args.workload]
-    print(json.dumps(run(args, b2g, scenes)))
+    out = run(args, b2g, scenes)
+    if args.workload == "large_world" and not args.no_extras:
+        out.update(extras(args, b2g, scenes))
+    print(json.dumps(out))
+
+
+def extras(args, b2g, scenes):
+    """Summaries of the batched, adjoint and stress workloads at their own default sizes and
+    step counts (each timed as its own block, after its own warm-up)."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.roofline_kernel = None, 10, "solve_velocity"
+    out = {}
+    b = run_batched(sub, b2g, scenes)
+    out["batched"] = {"metric": b["metric"], "value": b["value"], "unit": b["unit"], "steps": b["steps"],
+                      "ms_per_step": b["ms_per_step"], "worlds": b["config"]["worlds"],
+                      "bodies": b["config"]["bodies"], "body_steps_per_s": b["body_steps_per_s"],
+                      "plain_per_world": b["plain_per_world"],
+                      "roofline": {k: b["roofline"][k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
+    a = run_adjoint(sub, b2g, scenes)
+    out["adjoint"] = {"metric": a["metric"], "value": a["value"], "unit": a["unit"], "steps": a["steps"],
+                      "forward_ms": a["forward_ms"], "backward_ms": a["backward_ms"],
+                      "worlds": a["config"]["worlds"], "recorded_contacts": a["config"]["slots"]}
+    sub.roofline_kernel = "traverse"
+    st = run_stress(sub, b2g, scenes)
+    out["stress"] = {"metric": st["metric"], "value": st["value"], "unit": st["unit"], "steps": st["steps"],
+                     "ms_per_step": st["ms_per_step"], "bodies": st["config"]["bodies"],
+                     "pairs_per_step": st["pairs_per_step"], "pairs_per_s": st["pairs_per_s"],
+                     "manifolds_per_s": st["manifolds_per_s"],
+                     "solver_body_iterations_per_s": st["solver_body_iterations_per_s"],
+                     "last_step": st["config"]["last_step"]}
+    return out
 
 
 if __name__ == "__main__":

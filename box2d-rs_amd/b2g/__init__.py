@@ -108,6 +108,7 @@ def _load():
         "b2g_world_get_solver_order": ([P, I, P, P, I, P, P], I),
         "b2g_world_get_profile": ([P, P], I),
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
+        "b2g_world_set_fixture_frictions": ([P, P, I], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
         "b2g_world_time_kernel": ([P, I], I),
         "b2g_world_tune": ([P, I, I], I),
@@ -172,7 +173,8 @@ class World:
         _check(_lib.b2g_world_create(float(gravity[0]), float(gravity[1]), ctypes.byref(h)))
         self._h = h
         self.gravity = (float(gravity[0]), float(gravity[1]))
-        self.flags = {"warm_starting": True, "allow_sleep": False, "continuous": False, "block_solve": True}
+        # the library's defaults, those of B2world::new
+        self.flags = {"warm_starting": True, "allow_sleep": True, "continuous": True, "block_solve": True}
         # host copies of the definitions handed over (the serde world writer reads them)
         self._body_defs, self._fixture_body, self._fixture_defs = [], [], []
         self._n_bodies, self._batched = 0, False
@@ -185,7 +187,7 @@ class World:
     def __del__(self):
         self.close()
 
-    def set_flags(self, warm_starting=True, allow_sleep=False, continuous=False, block_solve=True):
+    def set_flags(self, warm_starting=True, allow_sleep=True, continuous=True, block_solve=True):
         _check(_lib.b2g_world_set_flags(self._h, int(warm_starting), int(allow_sleep), int(continuous),
                                         int(block_solve)))
         self.flags = {"warm_starting": bool(warm_starting), "allow_sleep": bool(allow_sleep),
@@ -431,6 +433,11 @@ class World:
         _check(_lib.b2g_tape_export(self._h, int(step), n.value, _ptr(slots), _ptr(imp), _ptr(offs), len(offs),
                                     ctypes.byref(n), ctypes.byref(nc), _ptr(params)))
         return slots[:n.value], imp[:n.value], offs, params
+
+    def set_fixture_frictions(self, friction):
+        """B2fixture::set_friction for every fixture (array of the fixture count)."""
+        fr = np.ascontiguousarray(friction, np.float32)
+        _check(_lib.b2g_world_set_fixture_frictions(self._h, _ptr(fr), len(fr)))
 
     def collide_pairs(self, pairs, xf_a, xf_b):
         pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
@@ -818,12 +825,11 @@ class B2world:
         self._fixtures = []
         self._n_flushed_bodies = 0
         self._cache = None
-        # B2world::new defaults: warm starting and sleeping on. Continuous physics (on by default
-        # in the reference) is off until set_continuous_physics(True): the TOI pass is a
-        # sequential event loop that costs far more than the step on large worlds.
+        # B2world::new defaults (src/private/dynamics/b2_world.rs:37-43): warm starting, sleeping
+        # and continuous physics on
         self._warm_starting = True
         self._allow_sleep = True
-        self._continuous = False
+        self._continuous = True
         self._apply_flags()
 
     @classmethod

@@ -433,8 +433,7 @@ def world_from_json(text, joint_bodies="reference"):
     doc = read_world(text, joint_bodies="file" if ck_probe is not None else joint_bodies)
     ck = doc["checkpoint"]
     w = World(doc["gravity"])
-    if ck is None:
-        w.set_flags(warm_starting=True, allow_sleep=True, continuous=False, block_solve=True)
+    if ck is None:  # the reference's reader makes a B2world::new world: the library's defaults
         w.add(doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"])
         if len(doc["joint_defs"]):
             w.add_joints(doc["joint_defs"])

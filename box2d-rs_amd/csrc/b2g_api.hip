@@ -515,9 +515,15 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
     set_error("no HIP device available");
     return B2G_ERR_DEVICE;
   }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("hipGetDevice failed");
+    return B2G_ERR_DEVICE;
+  }
   b2g_world* h = new b2g_world();
   World* w = &h->w;
   w->gravity = v2(gx, gy);
+  w->device = dev;
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
       w->counters.reserve(4 + 128 + 1) != B2G_OK ||
       hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
@@ -548,6 +554,12 @@ int b2g_world_destroy(b2g_world* h) {
   }
   rel(w->toi_alpha0); rel(w->toi_sel); rel(w->toi_two); rel(w->toi_ctl);
   rel(w->toi_cand); rel(w->toi_need); rel(w->toi_rank); rel(w->toi_snap);
+  rel(w->toi_succ); rel(w->toi_drop_flag); rel(w->toi_drop_pos); rel(w->toi_drop_total);
+  {
+    ContactArrays& d = w->toi_dropped;
+    rel(d.pair); rel(d.off); rel(d.old_index); rel(d.ab); rel(d.hdr); rel(d.geo); rel(d.pt0); rel(d.pt1); rel(d.ids);
+    rel(d.colour);
+  }
   SolverArrays& s = w->s;
   rel(s.touch_flag); rel(s.touch_scan); rel(s.touch_list); rel(s.k_contact); rel(s.k_bodies); rel(s.k_info);
   rel(s.vnf); rel(s.vm); rel(s.vr0); rel(s.vr1); rel(s.vms); rel(s.vbias); rel(s.K); rel(s.NM); rel(s.lam);
@@ -746,6 +758,29 @@ int b2g_world_set_body_states(b2g_world* h, const b2g_body_state* in, int32_t n)
   w->h_xf = xf;
   w->h_awake = awake;
   w->h_sleep_time = sleep_time;
+  return B2G_OK;
+}
+
+int b2g_world_set_fixture_frictions(b2g_world* h, const float* friction, int32_t n) {
+  if (!h || !friction || n != h->w.np) {
+    set_error("b2g_world_set_fixture_frictions: invalid argument (n must be the fixture count)");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  if (w->tape.active) {
+    set_error("b2g_world_set_fixture_frictions: a tape is recording");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  for (int i = 0; i < n; ++i)
+    if (!(friction[i] >= 0.0f)) {
+      set_error("b2g_world_set_fixture_frictions: frictions must be non-negative");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  for (int i = 0; i < n; ++i) w->h_fmat[i].x = friction[i];
+  if ((st = up(w->f.mat, w->h_fmat, w->stream))) return st;
+  B2G_HIP(hipStreamSynchronize(w->stream));
   return B2G_OK;
 }
 

@@ -250,7 +250,10 @@ struct JointArrays {
 struct World {
   // settings
   V2 gravity;
-  bool warm_starting = true, allow_sleep = false, continuous = false, block_solve = true;
+  // B2world::new (src/private/dynamics/b2_world.rs:37-43): warm starting, continuous physics and
+  // sleeping on
+  bool warm_starting = true, allow_sleep = true, continuous = true, block_solve = true;
+  int device = 0;  // the HIP device current at creation: the stream's device
   // sleep_possible: allow_sleep is on or a body was created asleep; otherwise every non-static body
   // is awake at every step and the sleep bookkeeping is skipped
   bool created_asleep = false;
@@ -289,6 +292,10 @@ struct World {
   ProxyArrays f;
   ContactArrays c[2];  // current and previous contact sets
   ToiState toi_state[2];              // per contact set, when continuous physics is on
+  // touching contacts a TOI rebuild drops from the predecessor role (flags, scan, and the
+  // gathered entries appended to the new predecessor set; see toi_rebuild)
+  DevArray<int> toi_succ, toi_drop_flag, toi_drop_pos, toi_drop_total;
+  ContactArrays toi_dropped;
   DevArray<float> toi_alpha0;         // per body: the sweep's alpha0 during the TOI pass
   DevArray<int> toi_sel, toi_two;     // bodies re-synchronised after a TOI event; {0, 1}
   DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies, counters

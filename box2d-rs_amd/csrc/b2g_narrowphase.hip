@@ -235,21 +235,28 @@ __global__ void __launch_bounds__(kBlock) k_collide_given(int n, const int2* __r
   store_manifold(k, m, imp, hdr, geo, pt0, pt1, ids);
 }
 
-int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
-                          b2g_manifold* out) {
-  if (n <= 0) return B2G_OK;
+// the fixture indices are in range and every pair is in registry order
+static int check_given_pairs(const World* w, int n, const int32_t* pairs, const char* who) {
   for (int i = 0; i < 2 * n; ++i)
     if (pairs[i] < 0 || pairs[i] >= w->np) {
-      set_error("b2g_collide_pairs: fixture index out of range");
+      set_error(std::string(who) + ": fixture index out of range");
       return B2G_ERR_INVALID_ARGUMENT;
     }
   for (int i = 0; i < n; ++i) {
     int ta = w->h_fshape[pairs[2 * i]].x & 0xff, tb = w->h_fshape[pairs[2 * i + 1]].x & 0xff;
     if (registry_swap(ta, tb) != 0) {
-      set_error("b2g_collide_pairs: pair is not in registry order (A must be the primary shape type)");
+      set_error(std::string(who) + ": pair is not in registry order (A must be the primary shape type)");
       return B2G_ERR_INVALID_ARGUMENT;
     }
   }
+  return B2G_OK;
+}
+
+int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
+                          b2g_manifold* out) {
+  if (n <= 0) return B2G_OK;
+  int st0 = check_given_pairs(w, n, pairs, "b2g_collide_pairs");
+  if (st0) return st0;
   DevArray<int2> dp;
   DevArray<float4> da, db, geo, p0, p1;
   DevArray<int> hdr;

@@ -1129,7 +1129,11 @@ static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& v
                      w->b.mass.p, w->b.xf.p, all_awake);
 }
 
+// An executable graph is destroyed only once the stream has finished every launch of it: the
+// host runs ahead of the device, so the last replay of a graph can still be queued or running,
+// and destroying it then frees the runtime's copies of its kernel arguments under the launch.
 void release_solve_graphs(World* w) {
+  if (!w->graphs.empty()) (void)hipStreamSynchronize(w->stream);
   for (World::CachedGraph& g : w->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   w->graphs.clear();
@@ -1142,10 +1146,11 @@ static int solve_tail_graph(World* w, const SolveGraphKey& k, const VelArgs& va,
   for (World::CachedGraph& g : w->graphs)
     if (g.key.size() == sizeof(k) && std::memcmp(g.key.data(), kb, sizeof(k)) == 0) hit = &g;
   if (!hit) {
-    if (w->graphs.size() >= 16) {  // evict the least recently used
+    if (w->graphs.size() >= 16) {  // evict the least recently used, once its launches are done
       size_t lru = 0;
       for (size_t i = 1; i < w->graphs.size(); ++i)
         if (w->graphs[i].last_use < w->graphs[lru].last_use) lru = i;
+      B2G_HIP(hipStreamSynchronize(w->stream));
       (void)hipGraphExecDestroy(w->graphs[lru].exec);
       w->graphs.erase(w->graphs.begin() + lru);
     }

@@ -991,9 +991,115 @@ static int toi_read(World* w, ToiCtl* out) {
 int carry_manifolds_for_toi(World* w);
 
 // find_new_contacts: a new contact set from the fat AABBs, its manifolds and TOI state carried
+__global__ void __launch_bounds__(kToiBlock) k_clear(int n, int* __restrict__ a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = 0;
+}
+
+// Touching contacts of the predecessor set with no successor in the current set: the next collide
+// destroys them and wakes their bodies (src/private/dynamics/b2_contact.rs:33-45), and
+// b2g_world_get_destroyed_contacts reports them. A TOI rebuild makes the current set the
+// predecessor and overwrites the old predecessor, so these entries are gathered first and appended
+// to the new predecessor after its own pairs (where nothing takes them as a predecessor).
+__global__ void __launch_bounds__(kToiBlock) k_mark_successors(int n, const int* __restrict__ old_index, int n_old,
+                                                               int* __restrict__ succ) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int o = old_index[k];
+  if (o >= 0 && o < n_old) succ[o] = 1;
+}
+
+__global__ void __launch_bounds__(kToiBlock) k_dropped_flags(int n_old, const int* __restrict__ hdr,
+                                                             const int* __restrict__ succ, int* __restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_old) return;
+  flag[k] = (hdr[k] >> 8) > 0 && !succ[k] ? 1 : 0;
+}
+
+struct ContactView {
+  int2* pair;
+  int2* ab;
+  int* hdr;
+  float4 *geo, *pt0, *pt1;
+  uint2* ids;
+  int* colour;
+};
+
+__global__ void __launch_bounds__(kToiBlock) k_move_dropped(int n, const int* __restrict__ flag,
+                                                            const int* __restrict__ pos, ContactView src,
+                                                            ContactView dst, int dst_off) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || (flag && !flag[k])) return;
+  const int d = dst_off + (pos ? pos[k] : k);
+  dst.pair[d] = src.pair[k];
+  dst.ab[d] = src.ab[k];
+  dst.hdr[d] = src.hdr[k];
+  dst.geo[d] = src.geo[k];
+  dst.pt0[d] = src.pt0[k];
+  dst.pt1[d] = src.pt1[k];
+  dst.ids[d] = src.ids[k];
+  if (dst.colour) dst.colour[d] = -1;
+}
+
+static ContactView view_of(ContactArrays& c) {
+  return ContactView{c.pair.p, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, c.colour.p};
+}
+
+// the flagged entries of src (flag and pos its exclusive scan; null: all n) copied densely to dst
+// from dst_off
+static int move_dropped(World* w, int n, const int* flag, const int* pos, ContactArrays& src, ContactArrays& dst,
+                        int dst_off) {
+  hipLaunchKernelGGL(k_move_dropped, dim3(blocks_for(n, kToiBlock)), dim3(kToiBlock), 0, w->stream, n, flag, pos,
+                     view_of(src), view_of(dst), dst_off);
+  B2G_HIP(hipGetLastError());
+  return B2G_OK;
+}
+
+static int reserve_contacts(ContactArrays& c, int n, int keep) {
+  const size_t m = std::max(n, 1), k = std::max(keep, 0);
+  int st;
+  if ((st = c.pair.reserve(m, true, k)) || (st = c.ab.reserve(m, true, k)) || (st = c.hdr.reserve(m, true, k)) ||
+      (st = c.geo.reserve(m, true, k)) || (st = c.pt0.reserve(m, true, k)) || (st = c.pt1.reserve(m, true, k)) ||
+      (st = c.ids.reserve(m, true, k)) || (st = c.colour.reserve(m, true, k)))
+    return st;
+  return B2G_OK;
+}
+
 static int toi_rebuild(World* w) {
   int st;
+  // the predecessor's dropped touching contacts (its extra entries included) before it is overwritten
+  const int n_old = w->n_pairs_old, n_cur = w->n_pairs;
+  int dropped = 0;
+  if (n_old > 0 && w->manifolds_valid[1 - w->cur]) {
+    ContactArrays& pred = w->c[1 - w->cur];
+    if ((st = w->toi_succ.reserve(n_old)) || (st = w->toi_drop_flag.reserve(n_old)) ||
+        (st = w->toi_drop_pos.reserve(n_old)) || (st = w->toi_drop_total.reserve(1)))
+      return st;
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n_old, kToiBlock)), dim3(kToiBlock), 0, w->stream, n_old,
+                       w->toi_succ.p);
+    if (n_cur > 0)
+      hipLaunchKernelGGL(k_mark_successors, dim3(blocks_for(n_cur, kToiBlock)), dim3(kToiBlock), 0, w->stream, n_cur,
+                         w->c[w->cur].old_index.p, n_old, w->toi_succ.p);
+    hipLaunchKernelGGL(k_dropped_flags, dim3(blocks_for(n_old, kToiBlock)), dim3(kToiBlock), 0, w->stream, n_old,
+                       pred.hdr.p, w->toi_succ.p, w->toi_drop_flag.p);
+    B2G_HIP(hipGetLastError());
+    if ((st = exclusive_scan(w, w->toi_drop_flag.p, w->toi_drop_pos.p, n_old, w->toi_drop_total.p))) return st;
+    B2G_HIP(hipMemcpyAsync(&dropped, w->toi_drop_total.p, sizeof(int), hipMemcpyDeviceToHost, w->stream));
+    B2G_HIP(hipStreamSynchronize(w->stream));
+    if (dropped > 0) {
+      if ((st = reserve_contacts(w->toi_dropped, dropped, 0))) return st;
+      if ((st = move_dropped(w, n_old, w->toi_drop_flag.p, w->toi_drop_pos.p, pred, w->toi_dropped, 0))) return st;
+    }
+  }
   if ((st = phase_broadphase(w))) return st;
+  if (dropped > 0) {
+    // appended to the new predecessor (the set the TOI pass started from) after its own pairs
+    ContactArrays& pred = w->c[1 - w->cur];
+    const int base = w->n_pairs_old;
+    if ((st = reserve_contacts(pred, base + dropped, base))) return st;
+    if ((st = move_dropped(w, dropped, nullptr, nullptr, w->toi_dropped, pred, base))) return st;
+    w->n_pairs_old = base + dropped;
+  }
   w->manifolds_valid[w->cur] = false;
   if ((st = carry_manifolds_for_toi(w))) return st;
   if ((st = toi_reserve(w))) return st;

@@ -137,7 +137,8 @@ const char* b2g_last_error(void);
 /* Library version string. */
 const char* b2g_version(void);
 
-/* B2world::new(gravity) -- src/b2_world.rs:22-30. */
+/* B2world::new(gravity) -- src/b2_world.rs:22-30, with its defaults: warm starting, sleeping and
+ * continuous physics on (src/private/dynamics/b2_world.rs:37-43); block solve on. */
 int b2g_world_create(float gravity_x, float gravity_y, b2g_world** out);
 int b2g_world_destroy(b2g_world* world);
 
@@ -199,6 +200,11 @@ int b2g_world_get_body_states(b2g_world* world, b2g_body_state* out, int32_t n);
 int b2g_world_set_body_states(b2g_world* world, const b2g_body_state* in, int32_t n);
 /* Mass data per body: mass, inv_mass, I, inv_I, local_center.x, local_center.y. */
 int b2g_world_get_mass(b2g_world* world, float* out, int32_t n);
+/* B2fixture::set_friction (src/b2_fixture.rs:260-264) for every fixture at once (n = fixture
+ * count, frictions >= 0). One difference: a contact's friction is mixed from its fixtures at each
+ * step here, so existing contacts take the new values at the next step, where the reference keeps
+ * a contact's mixed friction until B2contact::reset_friction. Refused while a tape records. */
+int b2g_world_set_fixture_frictions(b2g_world* world, const float* friction, int32_t n);
 
 /* Fat AABB of every proxy (B2broadPhase::get_fat_aabb, src/b2_broad_phase.rs) as
  * lower.x, lower.y, upper.x, upper.y. */

@@ -130,3 +130,102 @@ def oracle_structure(oracle, scene, steps, dt=1.0 / 60.0, pos_iters=3):
         total += len(touch)
         ow.step(dt, 8, pos_iters)
     return out
+
+
+def tape_structure(w):
+    """The recorded structure of every step of a GPU tape: [(slots, lam0, params)] with the
+    b2g_tape_export layouts (carry sources numbered over the whole batch)."""
+    out = []
+    for k in range(w.tape_info()[0]):
+        slots, imp, offs, params = w.tape_export(k)
+        out.append((np.asarray(slots).reshape(-1, 10), np.asarray(imp)[:, :4], np.asarray(params)))
+    return out
+
+
+def world_slice(batch, structure, wi):
+    """World wi's part of a batch's recorded structure, with bodies, fixtures and carry sources
+    renumbered to the world's own (the order of scenes.world_of): [(slots, lam0, params)]."""
+    bsel = np.nonzero(batch.body_world == wi)[0]
+    bmap = np.full(len(batch.body_world), -1, np.int64)
+    bmap[bsel] = np.arange(len(bsel))
+    fsel = np.nonzero(np.isin(batch.fixture_body, bsel))[0]
+    fmap = np.full(len(batch.fixture_body), -1, np.int64)
+    fmap[fsel] = np.arange(len(fsel))
+    g2l = []  # per recorded slot of the batch: its number in the world's structure, or -1
+    out, nloc = [], 0
+    for slots, lam0, params in structure:
+        keep = bmap[slots[:, 0]] >= 0
+        loc = np.full(len(slots), -1, np.int64)
+        loc[keep] = nloc + np.arange(int(keep.sum()))
+        s = slots[keep].astype(np.int64).copy()
+        s[:, 0], s[:, 1] = bmap[s[:, 0]], bmap[s[:, 1]]
+        s[:, 2], s[:, 3] = fmap[s[:, 2]], fmap[s[:, 3]]
+        assert np.all(s[:, :4] >= 0), "a slot joins two worlds"
+        prev = np.concatenate(g2l) if g2l else np.zeros(0, np.int64)
+        for col in (5, 6):
+            src = s[:, col]
+            has = src >= 0
+            to = prev[src[has] // 2]
+            assert np.all(to >= 0), "a carry source lies in another world"
+            src[has] = 2 * to + src[has] % 2
+        out.append((s.astype(np.int32), lam0[keep], params))
+        g2l.append(loc)
+        nloc += int(keep.sum())
+    return out
+
+
+def fd_check_world(oracle32, batch, wi, mass, state0, structure, g, gf, rng, h=1e-3, n_other=5, n_friction=4):
+    """GPU gradients of world wi against central differences of the host f64 frozen model on the
+    GPU's recorded structure, for the loss |c_body0 - goal|^2: body 0's initial velocity (3
+    components), n_other random initial-velocity components of other bodies and n_friction
+    random fixture frictions. Returns a dict: 'branch' False when the f64 model leaves the
+    recorded branches (its gradient is then of another function); else the relative error
+    max |g - fd| / max |fd| over the checked parameters, and the disagreement of the central
+    differences at h and h / 10 (a clamp switching inside the step shows there)."""
+    from b2g import scenes
+    sc = scenes.world_of(batch, wi)
+    bsel = np.nonzero(batch.body_world == wi)[0]
+    fsel = np.nonzero(np.isin(batch.fixture_body, bsel))[0]
+    host = HostAdjoint(oracle32, sc, mass[bsel])
+    for sl, l0, pr in world_slice(batch, structure, wi):
+        host.add_step(sl, l0, pr)
+    s0 = np.asarray(state0, np.float64)[bsel]
+    out, mism = host.forward(s0)
+    body0 = int(batch.meta["body0"][wi] - bsel[0])
+    goal = batch.meta["goal"][wi]
+    if mism:
+        return {"world": int(wi), "branch": False, "mismatches": int(mism)}
+
+    def loss(s, fr=None):
+        o, m = host.forward(s, fr)
+        d = o[body0, 0:2] - goal
+        return float(d @ d), m
+
+    dyn = np.nonzero(sc.body_defs["type"] == 2)[0]
+    params = [("s", body0, j) for j in (3, 4, 5)]
+    others = [("s", int(b), j) for b in dyn if b != body0 for j in (3, 4, 5)]
+    params += [others[i] for i in rng.choice(len(others), n_other, replace=False)]
+    params += [("f", int(f), 0) for f in rng.choice(len(fsel), n_friction, replace=False)]
+    rows = []
+    branch = True
+    for kind, i, j in params:
+        fd = []
+        for step in (h, h / 10):
+            if kind == "s":
+                sp, sm = s0.copy(), s0.copy()
+                sp[i, j] += step
+                sm[i, j] -= step
+                (lp, mp), (lm, mm) = loss(sp), loss(sm)
+            else:
+                fp, fm = host.friction.copy(), host.friction.copy()
+                fp[i] += step
+                fm[i] -= step
+                (lp, mp), (lm, mm) = loss(s0, fp), loss(s0, fm)
+            branch &= mp == 0 and mm == 0
+            fd.append((lp - lm) / (2 * step))
+        gg = float(g[bsel][i, j]) if kind == "s" else float(gf[fsel][i])
+        rows.append((gg, fd[0], fd[1]))
+    rows = np.array(rows)
+    scale = max(float(np.abs(rows[:, 1]).max()), 1e-12)
+    return {"world": int(wi), "branch": bool(branch), "rel_err": float(np.abs(rows[:, 0] - rows[:, 1]).max() / scale),
+            "fd_steps_disagree": float(np.abs(rows[:, 1] - rows[:, 2]).max() / scale), "scale": scale}

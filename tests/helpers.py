@@ -125,3 +125,69 @@ def glibc():
 
 def to_numpy_f32(a):
     return np.asarray(a, np.float32)
+
+
+def overlap_pairs(fat, fixture_body, body_type, big=8.0):
+    """Exact set of fixture pairs (lower, higher index) whose fat AABBs overlap as closed boxes
+    (b2_test_overlap, src/b2_collision.rs:355-368), on different bodies with at least one dynamic
+    (add_pair's filters for the default fixture filter), found with a uniform grid instead of
+    all N^2 tests. Boxes with a side over `big` metres (container walls) are tested against every
+    box directly. Returns an int64 array of keys lower << 32 | higher, sorted."""
+    fat = np.asarray(fat, np.float64).reshape(-1, 4)
+    n = len(fat)
+    size = np.maximum(fat[:, 2] - fat[:, 0], fat[:, 3] - fat[:, 1])
+    large = np.nonzero(size > big)[0]
+    small = np.nonzero(size <= big)[0]
+    keys = []
+
+    def overlap(a, b):
+        return ~((fat[b, 0] > fat[a, 2]) | (fat[b, 1] > fat[a, 3]) | (fat[a, 0] > fat[b, 2]) | (fat[a, 1] > fat[b, 3]))
+
+    if len(small):
+        cs = float(size[small].max()) * 1.0001
+        f = fat[small]
+        x0, y0 = np.floor(f[:, 0] / cs).astype(np.int64), np.floor(f[:, 1] / cs).astype(np.int64)
+        x1, y1 = np.floor(f[:, 2] / cs).astype(np.int64), np.floor(f[:, 3] / cs).astype(np.int64)
+        ox, oy = x0.min(), y0.min()
+        span = int(max(x1.max() - ox, y1.max() - oy)) + 2
+        cell, box = [], []
+        for dx in (0, 1):
+            for dy in (0, 1):  # a box covers at most 2 x 2 cells of side >= its size
+                m = (x0 + dx <= x1) & (y0 + dy <= y1)
+                cell.append((x0[m] + dx - ox) * span + (y0[m] + dy - oy))
+                box.append(small[m])
+        cell, box = np.concatenate(cell), np.concatenate(box)
+        order = np.lexsort((box, cell))
+        cell, box = cell[order], box[order]
+        d = 1
+        while True:
+            same = cell[d:] == cell[:-d]
+            if not same.any():
+                break
+            a, b = box[:-d][same], box[d:][same]
+            c = cell[:-d][same]
+            ok = overlap(a, b)
+            a, b, c = a[ok], b[ok], c[ok]
+            # count a pair only in the cell holding the lower corner of the two boxes' overlap
+            ref = ((np.floor(np.maximum(fat[a, 0], fat[b, 0]) / cs).astype(np.int64) - ox) * span +
+                   (np.floor(np.maximum(fat[a, 1], fat[b, 1]) / cs).astype(np.int64) - oy))
+            a, b = a[c == ref], b[c == ref]
+            keys.append(np.minimum(a, b).astype(np.int64) << 32 | np.maximum(a, b))
+            d += 1
+    for g in large:
+        other = np.arange(n)
+        other = other[other != g]
+        ok = overlap(np.full(len(other), g), other)
+        o = other[ok]
+        keys.append(np.minimum(o, g).astype(np.int64) << 32 | np.maximum(o, g))
+    keys = np.unique(np.concatenate(keys)) if keys else np.zeros(0, np.int64)
+    body = np.asarray(fixture_body)
+    lo, hi = keys >> 32, keys & 0xFFFFFFFF
+    bt = np.asarray(body_type)[body]
+    keep = (body[lo] != body[hi]) & ((bt[lo] == 2) | (bt[hi] == 2))
+    return keys[keep]
+
+
+def pair_keys(pairs):
+    pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
+    return np.unique(np.minimum(pairs[:, 0], pairs[:, 1]) << 32 | np.maximum(pairs[:, 0], pairs[:, 1]))

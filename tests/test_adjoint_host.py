@@ -102,3 +102,40 @@ def test_host_adjoint_friction_and_velocity_matter(oracle, oracle64):
     g, gf, _ = model.backward(s0, cot)
     assert np.abs(g[body0, 3:5]).max() > 1e-3
     assert np.abs(gf).max() > 0.0
+
+
+def test_world_slice_recovers_each_world(oracle64):
+    """world_slice (which takes one world's part out of a batch's recorded structure) returns, for
+    a batch assembled from three single-world structures, each of those structures exactly."""
+    from b2g import scenes
+    from adjoint_model import world_slice
+    batch = scenes.differentiable_worlds(3, 10)
+    per_world = [oracle_structure(oracle64, scenes.world_of(batch, wi), 12) for wi in range(3)]
+    b_off = [int(np.nonzero(batch.body_world == wi)[0][0]) for wi in range(3)]
+    f_off = [int(np.nonzero(np.isin(batch.fixture_body, np.nonzero(batch.body_world == wi)[0]))[0][0])
+             for wi in range(3)]
+    merged, seen = [], [0, 0, 0]
+    total = 0
+    starts = [[] for _ in range(3)]  # per world: the batch number of each of its slots
+    for k in range(12):
+        rows, lams = [], []
+        for wi in range(3):
+            slots, lam0, params = per_world[wi][k]
+            s = slots.astype(np.int64).copy()
+            s[:, 0:2] += b_off[wi]
+            s[:, 2:4] += f_off[wi]
+            for col in (5, 6):
+                has = s[:, col] >= 0
+                s[has, col] = 2 * np.asarray(starts[wi])[s[has, col] // 2] + s[has, col] % 2
+            rows.append(s)
+            lams.append(lam0)
+        for wi in range(3):
+            n = len(per_world[wi][k][0])
+            starts[wi].extend(range(total, total + n))
+            total += n
+        merged.append((np.concatenate(rows).astype(np.int32), np.concatenate(lams), per_world[0][k][2]))
+    for wi in range(3):
+        got = world_slice(batch, merged, wi)
+        for k in range(12):
+            np.testing.assert_array_equal(got[k][0], per_world[wi][k][0])
+            np.testing.assert_array_equal(got[k][1], per_world[wi][k][1])

@@ -23,6 +23,7 @@ def _run(n_worlds, n, steps, every=16):
     from b2g import scenes
     batch = scenes.differentiable_worlds(n_worlds, n)
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)  # a tape needs continuous physics off
     w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
     s0 = w.body_states()
     w.tape_begin(every)

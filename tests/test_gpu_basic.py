@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_hello_world_gpu():
-    """tests/test.rs:99-101 on the GPU path (continuous physics off)."""
+    """tests/test.rs:99-101 on the GPU path, through the reference-style API with B2world::new's
+    defaults (warm starting, sleeping and continuous physics on)."""
     import b2g
     world = b2g.B2world((0.0, -10.0))
     ground = world.create_body(b2g.B2bodyDef(position=(0.0, -10.0)))
@@ -44,6 +45,31 @@ def _shape_world(sd_list):
     w = b2g.World()
     w.add(bd, np.arange(n, dtype=np.int32), sd, fd)
     return w
+
+
+def test_hello_world_c_abi_defaults_match_oracle(oracle):
+    """tests/test.rs:26-102 through b2g_world_create's defaults (no set_flags call), which are
+    B2world::new's (src/private/dynamics/b2_world.rs:37-43): each of the 60 steps agrees with the
+    oracle world run with warm starting, sleeping and continuous physics on, and the box ends
+    at rest at y = 1.01."""
+    import b2g
+    from b2g import scenes
+    sc = scenes.hello_world()
+    w = b2g.World()
+    assert w.flags == {"warm_starting": True, "allow_sleep": True, "continuous": True, "block_solve": True}
+    w.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
+    ow = oracle.world()
+    ow.set_flags(warm_starting=True, allow_sleep=True, block_solve=True, continuous=True)
+    ow.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
+    for k in range(60):
+        w.step(1.0 / 60.0, 6, 2)
+        ow.step(1.0 / 60.0, 6, 2)
+        g, o = w.body_states()[1], ow.bodies()[1]
+        assert w.profile()["toi_events"] == ow.toi_events, "step %d" % k
+        for f in ("c", "a", "v", "w"):
+            np.testing.assert_allclose(g[f], o[f], rtol=1e-5, atol=1e-5, err_msg="step %d %s" % (k, f))
+        assert g["awake"] == o["awake"], "step %d" % k
+    assert abs(g["p"][1] - 1.01) < 0.01 and abs(g["p"][0]) < 0.01
 
 
 @pytest.mark.parametrize("kinds", PAIR_KINDS, ids=["-".join(k) for k in PAIR_KINDS])
@@ -126,6 +152,7 @@ def test_gpu_pyramid_tracks_oracle(oracle):
     from b2g import scenes
     sc = scenes.pyramid()
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)  # the oracle world's flags
     w.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
     ow = oracle.world()
     ow.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)

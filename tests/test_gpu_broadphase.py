@@ -26,6 +26,7 @@ def test_synchronize_matches_oracle(oracle):
     ow = oracle.world()
     ow.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)  # the oracle's
     w.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
     moved_total = 0
     for k in range(1, 41):
@@ -116,6 +117,7 @@ def test_device_rotation_matches_glibc(oracle):
     fd["density"] = 1
     fd["category"], fd["mask"] = 1, 0xFFFF
     w = b2g.World(gravity=(0.0, 0.0))
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
     w.add(bd, np.arange(n, dtype=np.int32), sd, fd)
     w.step(DT, 8, 3)
     q = w.body_states()["q"]
@@ -126,3 +128,77 @@ def test_device_rotation_matches_glibc(oracle):
     ulp = np.spacing(np.abs(ref).astype(np.float32))
     assert np.all(np.abs(q - ref) <= ulp)
     assert mismatch <= n // 20, "%d of %d rotations differ from glibc" % (mismatch, n)
+
+
+def _filtered_scene(seed=31):
+    """A heap of the large-world mix in a box with non-default fixture filters: categories of one
+    of four bits, masks of random bit subsets, groups 0 (most), +-1 and +-2; a tenth of the bodies
+    carry a second fixture (a small circle beside them, so same-body pairs arise) and a few are
+    kinematic."""
+    import b2g
+    from b2g import scenes
+    sc = scenes.heap_world(700, seed=seed, half_extent=14.0)
+    rng = np.random.default_rng(seed)
+    bd = sc.body_defs.copy()
+    dyn = np.nonzero(bd["type"] == 2)[0]
+    kin = rng.choice(dyn, 12, replace=False)
+    bd["type"][kin] = b2g.B2_KINEMATIC_BODY
+    bd["linear_velocity"][kin] = rng.normal(0, 1, (len(kin), 2)).astype(np.float32)
+    extra = rng.choice(dyn, len(dyn) // 10, replace=False)
+    esd = np.zeros(len(extra), b2g.SHAPE_DEF)
+    esd["type"] = b2g.SHAPE_CIRCLE
+    esd["radius"] = 0.2
+    esd["params"][:, :2] = np.array([0.35, 0.0], np.float32)
+    efd = np.zeros(len(extra), b2g.FIXTURE_DEF)
+    efd["density"], efd["friction"] = 1.0, 0.6
+    sd = np.concatenate([sc.shape_defs, esd])
+    fd = np.concatenate([sc.fixture_defs, efd])
+    fb = np.concatenate([np.asarray(sc.fixture_body), extra.astype(np.int32)])
+    nf = len(fd)
+    fd["category"] = (1 << rng.integers(0, 4, nf)).astype(np.uint16)
+    fd["mask"] = np.where(rng.random(nf) < 0.3, 0xFFFF, rng.integers(0, 16, nf)).astype(np.uint16)
+    fd["group"] = np.where(rng.random(nf) < 0.8, 0, rng.choice([-2, -1, 1, 2], nf)).astype(np.int16)
+    return bd, fb, sd, fd
+
+
+def test_filters_and_registry_orientation_match_oracle(oracle):
+    """add_pair's filters (src/private/dynamics/b2_contact_manager.rs:178-300 with the default
+    filter of src/private/dynamics/b2_world_callbacks.rs:6-18 and should_collide of
+    src/private/dynamics/b2_body.rs:391-416: same body, no dynamic body, group, category and mask)
+    and the registry's fixture A/B orientation (src/private/dynamics/b2_contact_registers.rs:67-103):
+    after the pair search of a step from the oracle's state and fat AABBs, the GPU's contacts are
+    the oracle's, as ordered (fixture A, fixture B) pairs."""
+    import b2g
+    bd, fb, sd, fd = _filtered_scene()
+    ow = oracle.world()
+    ow.add(bd, fb, sd, fd)
+    w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)  # the oracle's
+    w.add(bd, fb, sd, fd)
+    checked = 0
+    for k in range(1, 31):
+        fat_prev, _ = ow.proxies()
+        ow.step(DT, 8, 3)
+        if k % 10:
+            continue
+        ow.collide()  # the contact list of the next collide: overlapping and filter-passing pairs
+        ofab, _, _, _ = ow.contacts()
+        w.set_body_states(ow.bodies())
+        w.set_fat_aabbs(fat_prev)
+        w.step_phase(b2g.PHASE_FIND, DT, 8, 3)
+        gfab, _, _ = w.contacts()
+        og = set(map(tuple, ofab.tolist()))
+        gg = set(map(tuple, gfab.tolist()))
+        assert len(gg) == len(gfab)
+        assert gg == og, "step %d: %d GPU-only, %d oracle-only pairs" % (k, len(gg - og), len(og - gg))
+        checked += len(gg)
+    # the filters and the registry were exercised: some overlapping pairs were filtered out, and
+    # some pairs are in swapped (higher, lower) order
+    fat, _ = ow.proxies()
+    body = np.asarray(fb)
+    overlap = oracle.brute_pairs(fat)
+    keep = (body[overlap[:, 0]] != body[overlap[:, 1]]) & (
+        (bd["type"][body[overlap[:, 0]]] == 2) | (bd["type"][body[overlap[:, 1]]] == 2))
+    assert len(og) < int(keep.sum())
+    assert any(a > b for a, b in og) and any(a < b for a, b in og)
+    assert checked > 1000

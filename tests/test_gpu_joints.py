@@ -32,7 +32,7 @@ def _gpu(bd, fb, sd, fd, jd, gravity=(0.0, -10.0), sleep=False):
 
 def _cpu(oracle, bd, fb, sd, fd, jd, gravity=(0.0, -10.0), sleep=False):
     w = oracle.world(gravity)
-    w.set_flags(warm_starting=True, allow_sleep=sleep, block_solve=True)
+    w.set_flags(warm_starting=True, allow_sleep=sleep, block_solve=True, continuous=False)
     w.add(bd, fb, sd, fd)
     w.add_joints(jd)
     return w

@@ -59,8 +59,7 @@ def test_read_world_steps_like_direct_creation(kind):
     fb = np.asarray(scene.fixture_body)
     per_body = [np.flatnonzero(fb == b) for b in range(nb)]
     forder = np.concatenate([per_body[b][::-1] for b in order])
-    w2 = b2g.World()
-    w2.set_flags(warm_starting=True, allow_sleep=True, continuous=False, block_solve=True)
+    w2 = b2g.World()  # the reader's world: B2world::new's flags, the library's defaults
     w2.add(scene.body_defs[order], remap[fb[forder]].astype(np.int32), scene.shape_defs[forder],
            scene.fixture_defs[forder])
     assert w1.shapes().tobytes() == w2.shapes().tobytes()
@@ -160,8 +159,7 @@ def test_read_joints_step_like_direct_creation():
     text = ser.world_to_json(w0)
     w1 = ser.world_from_json(text, joint_bodies="file")
     doc = ser.read_world(text, joint_bodies="file")
-    w2 = b2g.World()
-    w2.set_flags(warm_starting=True, allow_sleep=True, continuous=False, block_solve=True)
+    w2 = b2g.World()  # the reader's world: B2world::new's flags, the library's defaults
     w2.add(doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"])
     w2.add_joints(doc["joint_defs"])
     # the file's joints are the world's, newest first, on the same bodies

@@ -18,8 +18,10 @@ POLY_RADIUS = np.float32(2.0) * np.float32(0.005)
 
 
 def _world(scene):
+    """A GPU world with the oracle replay's flags: sleeping and continuous physics off."""
     import b2g
     w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
     if hasattr(scene, "body_world"):
         w.add_batched(scene.body_defs, scene.body_world, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
     else:
@@ -152,6 +154,7 @@ def test_linear_momentum_conserved_without_gravity():
     from b2g import scenes
     sc = scenes.large_world(n=900, seed=8)  # a 36 m grid in the middle of a 400 m box
     w = b2g.World(gravity=(0.0, 0.0))
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
     w.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
     mass = w.mass()[:, 0].astype(np.float64)
 

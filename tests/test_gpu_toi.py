@@ -10,8 +10,12 @@ same in the reference's list order and the GPU's contact-set order:
     same, over 40 steps, and all of them end resting on the ground;
   * fast spinning boxes onto a static edge and fast balls onto a kinematic platform moving up: the
     same comparison over 45 steps, the boxes above the edge and the balls on the platform;
-  * fast spinning boxes onto a static edge and fast balls onto a kinematic platform moving up: the
-    same comparison over 45 steps, the boxes above the edge and the balls on the platform;
+  * sleeping and continuous physics on together: fast boxes landing among bouncing balls whose
+    ground contacts separate in steps with TOI events; awake flags and sleep times compared
+    exactly with the oracle's at every step;
+  * fast boxes dropped into a static corner (a TOI island touching two static fixtures) and fast
+    balls landing at distinct times on one moving kinematic platform (events sharing a kinematic
+    body);
   * determinism: two runs of a 40-box rain with sleeping on are bit-identical; a 1024-body mixed
     world runs 30 steps with continuous physics with finite states and nothing falling out.
 """
@@ -212,3 +216,113 @@ def test_reference_style_api_set_continuous_physics(continuous):
         world.step(1.0 / 60.0, 8, 3)
     x = float(ball.get_position().x)
     assert (x < 4.9) if continuous else (x > 5.05), x
+
+
+def _compare_sleep(w, o, steps, tol=1e-4):
+    """Per step: body states within tol, awake flags and sleep times exactly, TOI event counts
+    exactly; returns (states, steps with TOI events and a destroyed touching contact)."""
+    both = 0
+    for step in range(steps):
+        w.step(DT, 8, 3)
+        o.step(DT, 8, 3)
+        assert w.profile()["toi_events"] == o.toi_events, step
+        g, r = w.body_states(), o.bodies()
+        for f in ("c", "a", "v", "w"):
+            np.testing.assert_allclose(g[f], r[f], rtol=0, atol=tol, err_msg="step %d %s" % (step, f))
+        np.testing.assert_array_equal(g["awake"], r["awake"], err_msg="step %d awake" % step)
+        np.testing.assert_array_equal(g["sleep_time"], r["sleep_time"], err_msg="step %d sleep_time" % step)
+        if o.toi_events and len(w.destroyed_contacts()[0]):
+            both += 1
+    return w.body_states(), both
+
+
+def bounce_rain_scene(n_fast=24, n_balls=10):
+    """Fast boxes (60 m/s) landing on a thin static ground one after another, among balls with
+    restitution that bounce on the same ground (their touching contacts with it separate and are
+    destroyed in steps that also have TOI events), with sleeping on: the balls and boxes come to
+    rest and fall asleep."""
+    n = n_fast + n_balls
+    bd = np.zeros(1 + n, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    bd[1:]["type"] = 2
+    xs = np.arange(n) * 1.1 - 0.55 * n
+    bd[1:1 + n_fast]["position"] = np.stack([xs[:n_fast], 2.0 + 0.45 * np.arange(n_fast)], 1)
+    bd[1:1 + n_fast]["linear_velocity"] = (0.0, -60.0)
+    bd[1 + n_fast:]["position"] = np.stack([xs[n_fast:], 0.35 + 0.07 * np.arange(n_balls)], 1)
+    bd[1 + n_fast:]["linear_velocity"] = np.stack([np.zeros(n_balls), -3.0 - 0.2 * np.arange(n_balls)], 1)
+    sd = np.zeros(1 + n, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_BOX
+    sd[0]["params"][:2] = (0.6 * n + 2.0, 0.05)
+    sd[1:1 + n_fast]["params"][:, :2] = (0.25, 0.15)
+    sd[1 + n_fast:]["type"] = b2g.SHAPE_CIRCLE
+    sd[1 + n_fast:]["radius"] = 0.2
+    fd = _fixtures(1 + n)
+    fd["restitution"][1 + n_fast:] = 0.6
+    return bd, np.arange(1 + n, dtype=np.int32), sd, fd, (0.0, -10.0)
+
+
+def test_sleep_with_continuous_matches_oracle(oracle):
+    """Sleeping and continuous physics on together (B2world::new's defaults): every step's states,
+    awake flags, sleep times and TOI event counts agree with the oracle, including steps where a
+    touching contact is destroyed in a step with TOI events (the destroy wakes both bodies and
+    resets their sleep times, src/private/dynamics/b2_contact.rs:33-45), and the scene ends
+    asleep."""
+    w, o = _pair(oracle, bounce_rain_scene(), True, sleep=True)
+    st, both = _compare_sleep(w, o, 150)
+    assert both >= 1
+    assert (st["awake"][1:] == 0).sum() > 0
+
+
+def corner_scene():
+    """Fast boxes dropped into a corner of two static boxes (a floor and a wall) while sliding
+    along the wall: in their TOI events the body touches both static fixtures."""
+    n = 5
+    bd = np.zeros(2 + n, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    bd[1]["position"] = (-0.35, 3.0)  # wall, its right face at x = -0.3
+    bd[2:]["type"] = 2
+    bd[2:]["position"] = np.stack([np.full(n, -0.3 + 0.2 + 0.001), 2.0 + 1.3 * np.arange(n)], 1)
+    bd[2:]["linear_velocity"] = np.stack([-0.5 * np.ones(n), -55.0 - 3.0 * np.arange(n)], 1)
+    sd = np.zeros(2 + n, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_BOX
+    sd[0]["params"][:2] = (6.0, 0.05)
+    sd[1]["params"][:2] = (0.05, 3.0)
+    sd[2:]["params"][:, :2] = (0.2, 0.2)
+    return bd, np.arange(2 + n, dtype=np.int32), sd, _fixtures(2 + n, 0.3), (0.0, -10.0)
+
+
+def test_toi_island_touching_several_static_fixtures(oracle):
+    w, o = _pair(oracle, corner_scene(), True)
+    st = _compare(w, o, 50)
+    assert (st["c"][2:, 1] > 0.05).all() and (st["c"][2:, 0] > -0.3).all(), st["c"][2:]
+
+
+def shared_platform_scene():
+    """Two groups of fast balls landing at distinct times on one kinematic platform moving up
+    and sideways: their TOI events share the platform, whose sweep is advanced by each event."""
+    n = 6
+    bd = np.zeros(1 + n, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    bd[0]["type"] = 1
+    bd[0]["linear_velocity"] = (0.7, 1.5)
+    bd[1:]["type"] = 2
+    bd[1:]["position"] = np.stack([-3.0 + 1.2 * np.arange(n), 2.5 + 0.83 * np.arange(n)], 1)
+    bd[1:]["linear_velocity"] = (0.0, -70.0)
+    sd = np.zeros(1 + n, b2g.SHAPE_DEF)
+    sd[0]["type"] = b2g.SHAPE_BOX
+    sd[0]["params"][:2] = (6.0, 0.05)
+    sd[1:]["type"] = b2g.SHAPE_CIRCLE
+    sd[1:]["radius"] = 0.1
+    return bd, np.arange(1 + n, dtype=np.int32), sd, _fixtures(1 + n), (0.0, -10.0)
+
+
+def test_events_sharing_a_kinematic_platform(oracle):
+    w, o = _pair(oracle, shared_platform_scene(), True)
+    st = _compare(w, o, 40)
+    assert (st["c"][1:, 1] > st[0]["c"][1]).all(), (st["c"][1:, 1], st[0]["c"][1])

@@ -74,10 +74,11 @@ def _circle(r, p=(0, 0)):
 
 
 def test_hello_world_assert(oracle):
-    """tests/test.rs:26-102 (continuous physics off in the oracle)."""
+    """tests/test.rs:26-102, with B2world::new's defaults: warm starting, sleeping and continuous
+    physics on."""
     w, _ = _world(oracle, [{"type": 0, "position": (0, -10)}, {"type": 2, "position": (0, 4)}],
                   [(0, _box(50, 10), 0.0, 0.2), (1, _box(1, 1), 1.0, 0.3)])
-    w.set_flags(True, True, True)  # the test runs with sleeping allowed (the default)
+    w.set_flags(warm_starting=True, allow_sleep=True, block_solve=True, continuous=True)
     for _ in range(60):
         w.step(1.0 / 60.0, 6, 2)
     b = w.bodies()[1]
