@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libb2g.so")
 OK, ERR_INVALID_ARGUMENT, ERR_CAPACITY, ERR_DEVICE, ERR_DEGENERATE, ERR_UNSUPPORTED = range(6)
 B2_STATIC_BODY, B2_KINEMATIC_BODY, B2_DYNAMIC_BODY = 0, 1, 2
 SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX, SHAPE_POLYGON_EXACT = 0, 1, 2, 4, 5
+NOALLOC_MAX_PAIRS = 65536  # B2G_NOALLOC_MAX_PAIRS
 PHASE_COLLIDE, PHASE_SOLVE, PHASE_FIND = 0, 1, 2
 # kernels b2g_world_time_kernel can time (B2G_KERNEL_*)
 KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2, "init_constraints": 3,
@@ -109,6 +110,7 @@ def _load():
         "b2g_world_get_profile": ([P, P], I),
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
         "b2g_world_set_fixture_frictions": ([P, P, I], I),
+        "b2g_collide_pairs_noalloc": ([P, I, P, P, P, P], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
         "b2g_world_time_kernel": ([P, I], I),
         "b2g_world_tune": ([P, I, I], I),
@@ -446,6 +448,17 @@ class World:
         xb = np.ascontiguousarray(xf_b, np.float32).reshape(n, 4)
         out = np.zeros(max(n, 1), MANIFOLD)
         _check(_lib.b2g_collide_pairs(self._h, n, _ptr(pairs), _ptr(xa), _ptr(xb), _ptr(out)))
+        return out[:n]
+
+    def collide_pairs_noalloc(self, pairs, xf_a, xf_b):
+        """b2g_collide_pairs_noalloc: the same as collide_pairs with the device staging in the
+        library's fixed scratch area (no device allocation); at most NOALLOC_MAX_PAIRS pairs."""
+        pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        n = len(pairs)
+        xa = np.ascontiguousarray(xf_a, np.float32).reshape(n, 4)
+        xb = np.ascontiguousarray(xf_b, np.float32).reshape(n, 4)
+        out = np.zeros(max(n, 1), MANIFOLD)
+        _check(_lib.b2g_collide_pairs_noalloc(self._h, n, _ptr(pairs), _ptr(xa), _ptr(xb), _ptr(out)))
         return out[:n]
 
     def broadphase_pairs(self, fat):

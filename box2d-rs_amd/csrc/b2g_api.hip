@@ -520,6 +520,8 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
     set_error("hipGetDevice failed");
     return B2G_ERR_DEVICE;
   }
+  int st0 = noalloc_scratch_init(dev);  // the no-allocation entry point's scratch on this device
+  if (st0) return st0;
   b2g_world* h = new b2g_world();
   World* w = &h->w;
   w->gravity = v2(gx, gy);
@@ -1421,6 +1423,21 @@ int b2g_collide_pairs(b2g_world* h, int32_t n, const int32_t* pairs, const float
   int st;
   if ((st = ensure_uploaded(&h->w))) return st;
   return collide_fixture_pairs(&h->w, n, pairs, xf_a, xf_b, out);
+}
+
+int b2g_collide_pairs_noalloc(b2g_world* h, int32_t n, const int32_t* pairs, const float* xf_a, const float* xf_b,
+                              b2g_manifold* out) {
+  if (!h || n < 0 || (n > 0 && (!pairs || !xf_a || !xf_b || !out))) {
+    set_error("b2g_collide_pairs_noalloc: invalid argument");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  // uploading a world allocates: the world must already be on the device
+  if (h->w.dirty || h->w.joints_dirty) {
+    set_error("b2g_collide_pairs_noalloc: the world has bodies or joints not yet on the device (step it or call "
+              "b2g_collide_pairs once first)");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  return collide_fixture_pairs_noalloc(&h->w, n, pairs, xf_a, xf_b, out);
 }
 
 int b2g_broadphase_pairs(b2g_world* h, const float* fat_aabbs, int64_t cap, int32_t* pairs, int64_t* count) {

@@ -390,6 +390,10 @@ int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits);
 // stateless collide of fixture pairs under given transforms (parity entry point)
 int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
                           b2g_manifold* out);
+int collide_fixture_pairs_noalloc(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
+                                  b2g_manifold* out);
+// looks up the address of the no-allocation scratch array on `device` (which must be current)
+int noalloc_scratch_init(int device);
 int broadphase_pairs_for(World* w, const float* fat, int64_t cap, int32_t* out, int64_t* count);
 
 // launch geometry helpers

@@ -8,6 +8,8 @@
 // normal/tangent impulses of the previous manifold of the same pair by contact-feature id.
 // Manifolds are stored SoA (header, geometry, two points, ids) so the writes of a wave are
 // coalesced.
+#include <mutex>
+
 #include "b2g_internal.h"
 
 namespace b2g {
@@ -299,6 +301,107 @@ int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xf
     m.points[1].local_point[0] = h1[i].x; m.points[1].local_point[1] = h1[i].y;
     m.points[1].normal_impulse = h1[i].z; m.points[1].tangent_impulse = h1[i].w;
     m.points[1].id = hi[i].y; m.points[1].pad = 0;
+  }
+  return B2G_OK;
+}
+
+// ---- the same pairs without allocating ------------------------------------------------------
+// b2g_collide_pairs_noalloc stages the pairs, both transforms and the manifold records of at most
+// B2G_NOALLOC_MAX_PAIRS pairs in one file-scope device array instead of per-call allocations.
+// Each device holds its own copy of the array for the life of the process. In host code the
+// array's name is not a device address: the address of each device's copy is looked up once
+// (noalloc_scratch_init, at world creation, with that device current) and only that pointer is
+// passed to kernels and copies. Every kernel writes each record it owns in full, so nothing has
+// to be cleared between calls; calls that overlap on one device share the array and race.
+constexpr int kNoallocPairs = B2G_NOALLOC_MAX_PAIRS;
+constexpr size_t kNoallocXaOff = sizeof(int2) * kNoallocPairs;
+constexpr size_t kNoallocXbOff = kNoallocXaOff + sizeof(float4) * kNoallocPairs;
+constexpr size_t kNoallocOutOff = kNoallocXbOff + sizeof(float4) * kNoallocPairs;
+constexpr size_t kNoallocBytes = kNoallocOutOff + sizeof(b2g_manifold) * kNoallocPairs;
+static_assert(kNoallocXaOff % 256 == 0 && kNoallocXbOff % 256 == 0 && kNoallocOutOff % 256 == 0,
+              "scratch regions start on 256-byte boundaries");
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device " + std::to_string(device) + " is outside the scratch table");
+    return B2G_ERR_DEVICE;
+  }
+  if (g_scratch_ptr[device]) return B2G_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error("hipGetSymbolAddress(scratch) failed on device " + std::to_string(device) + ": " +
+              hipGetErrorString(e));
+    return B2G_ERR_DEVICE;
+  }
+  g_scratch_ptr[device] = p;
+  return B2G_OK;
+}
+
+// manifold records in the C-ABI layout, so one copy returns them
+__global__ void __launch_bounds__(kBlock) k_collide_given_records(int n, const int2* __restrict__ pairs,
+                                                                  const float4* __restrict__ xa4,
+                                                                  const float4* __restrict__ xb4,
+                                                                  const ShapeV* __restrict__ shapes,
+                                                                  b2g_manifold* __restrict__ out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int2 pr = pairs[k];
+  Manifold m;
+  evaluate(&m, shapes[pr.x], xf_of(xa4[k]), shapes[pr.y], xf_of(xb4[k]));
+  b2g_manifold r;
+  r.type = m.type;
+  r.point_count = m.count;
+  r.local_normal[0] = m.local_normal.x; r.local_normal[1] = m.local_normal.y;
+  r.local_point[0] = m.local_point.x; r.local_point[1] = m.local_point.y;
+  for (int p = 0; p < 2; ++p) {
+    r.points[p].local_point[0] = m.lp[p].x; r.points[p].local_point[1] = m.lp[p].y;
+    r.points[p].normal_impulse = 0.0f; r.points[p].tangent_impulse = 0.0f;
+    r.points[p].id = m.id[p]; r.points[p].pad = 0;
+  }
+  out[k] = r;
+}
+
+int collide_fixture_pairs_noalloc(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,
+                                  b2g_manifold* out) {
+  if (n > kNoallocPairs) {
+    set_error("b2g_collide_pairs_noalloc: " + std::to_string(n) + " pairs exceed the scratch area's " +
+              std::to_string(kNoallocPairs));
+    return B2G_ERR_CAPACITY;
+  }
+  if (n <= 0) return B2G_OK;
+  int st = check_given_pairs(w, n, pairs, "b2g_collide_pairs_noalloc");
+  if (st) return st;
+  void* base = (w->device >= 0 && w->device < kMaxDevices) ? g_scratch_ptr[w->device] : nullptr;
+  if (!base) {
+    set_error("b2g_collide_pairs_noalloc: no scratch address for device " + std::to_string(w->device));
+    return B2G_ERR_DEVICE;
+  }
+  unsigned char* b = static_cast<unsigned char*>(base);
+  int2* dp = reinterpret_cast<int2*>(b);
+  float4* da = reinterpret_cast<float4*>(b + kNoallocXaOff);
+  float4* db = reinterpret_cast<float4*>(b + kNoallocXbOff);
+  b2g_manifold* dout = reinterpret_cast<b2g_manifold*>(b + kNoallocOutOff);
+  bool ok = hipMemcpyAsync(dp, pairs, sizeof(int2) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess &&
+            hipMemcpyAsync(da, xfa, sizeof(float4) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess &&
+            hipMemcpyAsync(db, xfb, sizeof(float4) * n, hipMemcpyHostToDevice, w->stream) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_collide_given_records, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, dp, da,
+                       db, w->f.shapev.p, dout);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(out, dout, sizeof(b2g_manifold) * n, hipMemcpyDeviceToHost, w->stream) == hipSuccess &&
+         hipStreamSynchronize(w->stream) == hipSuccess;
+  }
+  if (!ok) {
+    set_error("b2g_collide_pairs_noalloc: device error");
+    return B2G_ERR_DEVICE;
   }
   return B2G_OK;
 }

@@ -28,6 +28,7 @@ pub const B2G_JOINT_WELD: i32 = 9;
 pub const B2G_PHASE_COLLIDE: i32 = 0;
 pub const B2G_PHASE_SOLVE: i32 = 1;
 pub const B2G_PHASE_FIND: i32 = 2;
+pub const B2G_NOALLOC_MAX_PAIRS: i32 = 65536;
 pub const B2G_TUNE_COLOR_ROUNDS: i32 = 2;
 pub const B2G_TUNE_SOLVER_GRAPH: i32 = 3;
 pub const B2G_TUNE_COLOR_REUSE: i32 = 4;
@@ -128,6 +129,8 @@ extern "C" {
     pub fn b2g_world_get_profile(w: *mut b2g_world, out: *mut b2g_profile) -> c_int;
     pub fn b2g_collide_pairs(w: *mut b2g_world, n: i32, pairs: *const i32, xf_a: *const f32, xf_b: *const f32,
                              out: *mut b2g_manifold) -> c_int;
+    pub fn b2g_collide_pairs_noalloc(w: *mut b2g_world, n: i32, pairs: *const i32, xf_a: *const f32,
+                                     xf_b: *const f32, out: *mut b2g_manifold) -> c_int;
     pub fn b2g_broadphase_pairs(w: *mut b2g_world, fat_aabbs: *const f32, cap: i64, pairs: *mut i32,
                                 count: *mut i64) -> c_int;
     pub fn b2g_tape_begin(w: *mut b2g_world, checkpoint_every: i32) -> c_int;

@@ -282,6 +282,19 @@ int b2g_world_get_profile(b2g_world* world, b2g_profile* out);
 int b2g_collide_pairs(b2g_world* world, int32_t n, const int32_t* pairs, const float* xf_a, const float* xf_b,
                       b2g_manifold* out);
 
+/* No counterpart in the reference: b2g_collide_pairs for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and results as b2g_collide_pairs, but its device staging (pairs, both
+ * transforms, manifold records) lives in a fixed per-device scratch area of
+ * B2G_NOALLOC_MAX_PAIRS pairs (about 7 MB, held by each device the library runs on until the
+ * process ends) instead of per-call allocations. n above the maximum: B2G_ERR_CAPACITY. The
+ * world must already be on the device (stepped, or queried once through an allocating entry
+ * point): otherwise B2G_ERR_INVALID_ARGUMENT. Calls that overlap on one device share the
+ * scratch area and race. */
+#define B2G_NOALLOC_MAX_PAIRS 65536
+int b2g_collide_pairs_noalloc(b2g_world* world, int32_t n, const int32_t* pairs, const float* xf_a,
+                              const float* xf_b, b2g_manifold* out);
+
 /* Stateless broadphase: B2broadPhase::update_pairs pair search (src/b2_broad_phase.rs:200-249)
  * over the given fat AABBs of this world's proxies (n = fixture count, 4 floats each), with the
  * world's filters. Writes at most cap pairs (lower, higher fixture index) and the total count. */
