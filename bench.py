@@ -151,8 +151,8 @@ def roofline(kernel, k_ms, k_launches, k_items, k_aux, window=None, pair_shape=N
 
 
 def timed_steps(w, steps, kernel, dt=1.0 / 60.0):
-    """K steps bracketed by a stream synchronisation on both sides, with the roofline kernel's
-    launches event-timed; returns (seconds, kernel totals)."""
+    """K steps bracketed by a stream synchronisation on both sides, with the given kernel's
+    launches event-timed ("none": no events); returns (seconds, kernel totals)."""
     w.body_states()
     w.time_kernel(kernel)
     t0 = time.perf_counter()
@@ -163,6 +163,21 @@ def timed_steps(w, steps, kernel, dt=1.0 / 60.0):
     totals = w.kernel_time()
     w.time_kernel("none")
     return t1 - t0, totals
+
+
+def replay_kernel(make_world, warmup, steps, kernel):
+    """The kernel's launch totals over a replay of the same window on a fresh copy of the world
+    (the same warm-up steps, then the same steps with the kernel's launches event-timed). The
+    headline window itself runs without events: an event pair around each of the solver's short
+    launches serialises the stream and slows the whole step by about a fifth. The world, its
+    scene and the steps are deterministic, so the replay launches the same kernels on the same
+    data."""
+    r = make_world()
+    for _ in range(warmup):
+        r.step(1.0 / 60.0, 8, 3)
+    _, totals = timed_steps(r, steps, kernel)
+    r.close()
+    return totals
 
 
 def awake_fraction(w, body_defs):
@@ -181,14 +196,23 @@ def large_world_flags(args):
     return dict(warm_starting=True, allow_sleep=args.sleep, continuous=args.continuous, block_solve=True)
 
 
+def large_world_maker(args, b2g, scene):
+    def make():
+        w = b2g.World()
+        w.set_flags(**large_world_flags(args))
+        w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
+        w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+        return w
+    return make
+
+
 def full_run(args, b2g, scene, steps=600):
     """All `steps` steps of a fresh copy of the world from its initial state, timed as one block
-    (inputs uploaded before the clock starts), with the roofline kernel over the same block."""
-    w = b2g.World()
-    w.set_flags(**large_world_flags(args))
-    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
-    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
-    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    (inputs uploaded before the clock starts), with the roofline kernel over a replay of it."""
+    make = large_world_maker(args, b2g, scene)
+    w = make()
+    elapsed, _ = timed_steps(w, steps, "none")
+    k_ms, k_launches, k_items, k_aux = replay_kernel(make, 0, steps, args.roofline_kernel)
     p = w.profile()
     n = int(scene.meta["dynamic"])
     out = {"steps": steps, "value": n * steps / elapsed, "unit": "body-steps/s", "ms_per_step": 1000.0 * elapsed / steps,
@@ -203,10 +227,8 @@ def full_run(args, b2g, scene, steps=600):
 
 def run_large_world(args, b2g, scenes):
     scene = scenes.large_world(n=args.bodies, seed=1)
-    w = b2g.World()
-    w.set_flags(**large_world_flags(args))
-    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
-    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+    make = large_world_maker(args, b2g, scene)
+    w = make()
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     p0 = w.profile()
@@ -224,12 +246,13 @@ def run_large_world(args, b2g, scenes):
         del probe
     sbytes = shape_bytes(scene.shape_defs)
     pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
-    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    elapsed, _ = timed_steps(w, steps, "none")
     if pair_shape:
         pair_shape.append(pair_shape_bytes(w, sbytes))
     p = w.profile()
     n = int(scene.meta["dynamic"])
     window = (args.warmup + 1, args.warmup + steps)
+    k_ms, k_launches, k_items, k_aux = replay_kernel(make, args.warmup, steps, args.roofline_kernel)
     out = base_record(args, "body-steps/s (64k-body world)", n * steps / elapsed, "body-steps/s", elapsed, steps,
                       "large_world: %d dynamic bodies, 400x400 m box, dt 1/60, 8/3 iterations, warm start on, "
                       "%s, continuous %s" % (n, "sleep on" if args.sleep else "sleep off",
@@ -259,17 +282,35 @@ def run_large_world(args, b2g, scenes):
 def run_batched(args, b2g, scenes):
     """4096 independent worlds (16-1024 bodies each, heaps and stacks) stepped together, against
     the same worlds stepped one at a time as separate worlds (a sample of them)."""
+    from b2g import autotune
     batch = scenes.batched_worlds(4096)
-    w = b2g.World()
-    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
-    w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
-    w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
+    settings = {}
+
+    def make():
+        w = b2g.World()
+        w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
+        w.tune(b2g.TUNE_SOLVER_GRAPH, args.solver_graph)
+        w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
+        if settings:
+            autotune.apply(w, settings)
+        return w
+    # launch settings (threads per world by world size, colouring rounds) tuned on a warmed-up copy;
+    # they do not change the results
+    tuned = None
+    if not args.no_autotune:
+        t = make()
+        for _ in range(args.warmup):
+            t.step(1.0 / 60.0, 8, 3)
+        tuned = autotune.tune(t)
+        settings.update(tuned["settings"])
+        t.close()
+    w = make()
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     steps = args.steps if args.steps is not None else 300
     sbytes = shape_bytes(batch.shape_defs)
     pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
-    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    elapsed, _ = timed_steps(w, steps, "none")
     if pair_shape:
         pair_shape.append(pair_shape_bytes(w, sbytes))
     nw = batch.meta["worlds"]
@@ -279,6 +320,8 @@ def run_batched(args, b2g, scenes):
                       % batch.meta["dynamic"], {"worlds": nw, "bodies": int(batch.meta["dynamic"]),
                                                 "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
     out["body_steps_per_s"] = batch.meta["dynamic"] * steps / elapsed
+    if tuned:
+        out["config"]["autotuned"] = tuned["settings"]
     if args.sleep:
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, batch.body_defs)
@@ -297,6 +340,8 @@ def run_batched(args, b2g, scenes):
     plain = 1.0 / (sum(per_world) / len(per_world))
     out["plain_per_world"] = {"value": plain, "unit": "world-steps/s", "sample_worlds": len(sample),
                               "ratio": out["value"] / plain}
+    w.close()
+    k_ms, k_launches, k_items, k_aux = replay_kernel(make, args.warmup, steps, args.roofline_kernel)
     out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux, pair_shape=pair_shape)
     return out
 
@@ -306,16 +351,21 @@ def run_chains(args, b2g, scenes):
     overlapping with collide_connected off) swinging down side by side: 65,536 bodies, 65,536
     joints."""
     scene, joints = scenes.chains(4096, 16)
-    w = b2g.World()
-    w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
-    w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
-    w.add_joints(joints)
+
+    def make():
+        w = b2g.World()
+        w.set_flags(warm_starting=True, allow_sleep=args.sleep, continuous=False, block_solve=True)
+        w.add(scene.body_defs, scene.fixture_body, scene.shape_defs, scene.fixture_defs)
+        w.add_joints(joints)
+        return w
+    w = make()
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     steps = args.steps if args.steps is not None else 300
     sbytes = shape_bytes(scene.shape_defs)
     pair_shape = [pair_shape_bytes(w, sbytes)] if args.roofline_kernel == "collide" else None
-    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, args.roofline_kernel)
+    elapsed, _ = timed_steps(w, steps, "none")
+    k_ms, k_launches, k_items, k_aux = replay_kernel(make, args.warmup, steps, args.roofline_kernel)
     if pair_shape:
         pair_shape.append(pair_shape_bytes(w, sbytes))
     p = w.profile()
@@ -343,17 +393,19 @@ def run_stress(args, b2g, scenes):
     for _ in range(args.warmup):
         w.step(1.0 / 60.0, 8, 3)
     steps = args.steps if args.steps is not None else 200
-    elapsed, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, steps, "traverse")
+    elapsed, _ = timed_steps(w, steps, "none")
     n = int(scene.meta["dynamic"])
-    _, (c_ms, c_launches, c_items, _) = timed_steps(w, 5, "collide")
     p = w.profile()
+    # the phase rates from steps after the timed window, each phase's launches event-timed
+    _, (k_ms, k_launches, k_items, k_aux) = timed_steps(w, 10, "traverse")
+    _, (c_ms, c_launches, c_items, _) = timed_steps(w, 5, "collide")
     out = base_record(args, "body-steps/s (1.18M-body stress world)", n * steps / elapsed, "body-steps/s", elapsed,
                       steps, "stress_world: 1,048,576 mixed bodies + 16,384 towers x 8 boxes, 1600x1600 m box",
                       {"bodies": n, "last_step": {k: p[k] for k in ("pairs", "touching", "colours")}})
     if args.sleep:
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, scene.body_defs)
-    out["pairs_per_step"] = k_aux / max(k_launches, 1)
+    out["pairs_per_step"] = k_aux / max(k_launches, 1)  # over the 10 steps after the window
     out["pairs_per_s"] = k_aux / (k_ms / 1000.0) if k_ms else None
     out["manifolds_per_s"] = c_items / (c_ms / 1000.0) if c_ms else None
     out["solver_body_iterations_per_s"] = n * (8 + 3) / (p["solve"] / 1000.0) if p["solve"] else None
@@ -407,6 +459,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="large_world: leave out the batched, adjoint and stress summaries")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched: launch settings at their defaults instead of tuned on a copy of the batch")
     ap.add_argument("--no-full-run", action="store_true",
                     help="large_world: leave out the 600-step run from the initial state")
     ap.add_argument("--continuous", action="store_true",
@@ -440,6 +494,7 @@ def extras(args, b2g, scenes):
     b = run_batched(sub, b2g, scenes)
     out["batched"] = {"metric": b["metric"], "value": b["value"], "unit": b["unit"], "steps": b["steps"],
                       "ms_per_step": b["ms_per_step"], "worlds": b["config"]["worlds"],
+                      "autotuned": b["config"].get("autotuned"),
                       "bodies": b["config"]["bodies"], "body_steps_per_s": b["body_steps_per_s"],
                       "plain_per_world": b["plain_per_world"],
                       "roofline": {k: b["roofline"][k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}

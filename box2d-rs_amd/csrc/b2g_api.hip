@@ -423,6 +423,33 @@ int upload_world(World* w) {
         (st = up(w->f.loose, loose, w->stream)))
       return st;
   }
+  // per-world solve: the worlds' body ranges (when every world's bodies are contiguous) and the
+  // launch order of the worlds, by body-count bucket and larger worlds first within a bucket (the
+  // longest workgroups start first)
+  {
+    const int nwld = w->n_worlds;
+    std::vector<int>& off = w->h_world_body_off;
+    off.assign(nwld + 1, 0);
+    bool contiguous = w->nb > 0;
+    for (int b = 1; b < w->nb && contiguous; ++b) contiguous = w->h_bworld[b] >= w->h_bworld[b - 1];
+    w->worlds_contiguous = contiguous;
+    if (contiguous) {
+      for (int b = 0; b < w->nb; ++b) ++off[w->h_bworld[b] + 1];
+      for (int i = 0; i < nwld; ++i) off[i + 1] += off[i];
+      std::vector<int>& order = w->h_world_order;
+      order.resize(nwld);
+      for (int i = 0; i < nwld; ++i) order[i] = i;
+      auto size = [&](int i) { return off[i + 1] - off[i]; };
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const int ka = world_bucket(size(a)), kb = world_bucket(size(b));
+        return ka != kb ? ka < kb : size(a) > size(b);
+      });
+      for (int k = 0; k < 4; ++k) w->world_bucket_off[k] = 0;
+      for (int i = 0; i < nwld; ++i) ++w->world_bucket_off[world_bucket(size(i)) + 1];
+      for (int k = 0; k < 3; ++k) w->world_bucket_off[k + 1] += w->world_bucket_off[k];
+      if ((st = up(w->s.world_body_off, off, w->stream)) || (st = up(w->s.world_order, order, w->stream))) return st;
+    }
+  }
   // extend the open contact-set offset arrays: new proxies have empty segments
   for (int i = 0; i < 2; ++i) {
     ContactArrays& c = w->c[i];
@@ -568,6 +595,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.plp); rel(s.pgeo); rel(s.plc); rel(s.prad); rel(s.deg); rel(s.adj_off); rel(s.adj_cursor); rel(s.adj);
   rel(s.color_state); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.inv_order); rel(s.parent); rel(s.isl_awake); rel(s.isl_restless);
   rel(s.isl_violated); rel(s.isl_done);
+  rel(s.cw_count); rel(s.cw_off); rel(s.cw_cursor); rel(s.world_body_off); rel(s.world_order);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes); rel(bp.nodes4);
   rel(bp.leafrec); rel(bp.count); rel(bp.slots); rel(bp.bounds); rel(bp.scan_tmp);
@@ -1253,6 +1281,16 @@ int b2g_world_tune(b2g_world* h, int32_t param, int32_t value) {
       return B2G_OK;
     case B2G_TUNE_COLOR_REUSE:
       w->color_reuse = value != 0;
+      return B2G_OK;
+    case B2G_TUNE_WORLD_SOLVE:
+      if (value < -1 || value > 1) break;
+      w->world_solve = value;
+      return B2G_OK;
+    case B2G_TUNE_WORLD_THREADS_SMALL:
+    case B2G_TUNE_WORLD_THREADS_MEDIUM:
+    case B2G_TUNE_WORLD_THREADS_LARGE:
+      if (value != 64 && value != 128 && value != 256 && value != 512) break;
+      w->world_threads[param - B2G_TUNE_WORLD_THREADS_SMALL] = value;
       return B2G_OK;
     default: break;
   }

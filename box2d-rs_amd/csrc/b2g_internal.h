@@ -122,7 +122,13 @@ struct ToiState {
 };
 
 // Per-step solver arrays (indexed by constraint, colour-sorted).
+// body-count bucket of a world for the per-world solve's threads per world: < 64, 64-255, >= 256
+inline int world_bucket(int bodies) { return bodies < 64 ? 0 : bodies < 256 ? 1 : 2; }
+
 struct SolverArrays {
+  // per-world solve: constraints of each (colour, world), their exclusive scan (colour-major), the
+  // scatter cursors, the worlds' body ranges and the launch order of the worlds
+  DevArray<int> cw_count, cw_off, cw_cursor, world_body_off, world_order;
   DevArray<int> touch_flag, touch_scan, touch_list;   // compaction of touching contacts
   DevArray<int> k_contact;   // contact index of constraint
   DevArray<int2> k_bodies;
@@ -317,6 +323,16 @@ struct World {
   int reuse_rounds = 0;        // colouring rounds the last reusing step needed
   int steps_since_scratch = 0; // solves since the last from-scratch colouring
   bool solver_graphs = false;  // replay the solve's per-colour launches as a captured HIP graph
+  // Batched worlds: each world's velocity and position iterations run in one workgroup per world
+  // (colour after colour with workgroup barriers) instead of one launch per colour and iteration.
+  // world_solve: -1 auto (on when the batch has at least kWorldSolveMin worlds), 0 off, 1 on.
+  // world_threads[k]: threads per world for the worlds of body-count bucket k (see world_bucket).
+  int world_solve = -1;
+  int world_threads[3] = {64, 128, 256};
+  bool worlds_contiguous = false;         // every world's bodies are one index range (set at upload)
+  std::vector<int> h_world_body_off;      // n_worlds + 1 (when contiguous)
+  std::vector<int> h_world_order;         // worlds by bucket, larger worlds first within a bucket
+  int world_bucket_off[4] = {0, 0, 0, 0}; // ranges of h_world_order per bucket
   struct CachedGraph {
     std::vector<unsigned char> key;
     hipGraphExec_t exec = nullptr;

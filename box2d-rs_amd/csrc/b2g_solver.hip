@@ -1068,6 +1068,159 @@ __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int i
   if (b < nb) iteration_end_one(b, iteration, btype, parent, violated, done, active, isl_awake);
 }
 
+// ---- per-world solve (batched worlds) ---------------------------------------------------------
+// Within each colour the constraints are grouped by world: cw_off[L * nw + w] is the first
+// constraint of world w in launch colour L (an exclusive scan over (colour, world), colour-major,
+// so cw_off[L * nw] is colour L's first constraint and the order stays colour-major). One
+// workgroup then runs all iterations of its world, colour after colour with a workgroup barrier
+// between colours -- the sequence of the per-colour launches restricted to one world, whose
+// constraints touch no other world's bodies, so the results are the same bit for bit.
+static constexpr int kWorldSolveMin = 16;
+
+__device__ inline int constraint_world(int j, const int* __restrict__ list, const int2* __restrict__ ab,
+                                       const int* __restrict__ fworld) {
+  return fworld[ab[list[j]].x];
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_world_hist(int n, const int* __restrict__ state,
+                                                             const int* __restrict__ launch, const int* __restrict__ list,
+                                                             const int2* __restrict__ ab, const int* __restrict__ fworld,
+                                                             int nw, int* __restrict__ count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || state[j] < 0) return;
+  atomicAdd(&count[launch[state[j] & 0xff] * nw + constraint_world(j, list, ab, fworld)], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_world_scatter(int n, const int* __restrict__ state,
+                                                                const int* __restrict__ launch,
+                                                                const int* __restrict__ list, const int2* __restrict__ ab,
+                                                                const int* __restrict__ fworld, int nw,
+                                                                const int* __restrict__ off, int* __restrict__ cursor,
+                                                                int* __restrict__ order, int* __restrict__ inv_order,
+                                                                int* __restrict__ pair_colour) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || state[j] < 0) return;
+  const int L = launch[state[j] & 0xff];
+  const int g = L * nw + constraint_world(j, list, ab, fworld);
+  const int pos = off[g] + atomicAdd(&cursor[g], 1);
+  order[pos] = j;
+  inv_order[j] = pos;
+  pair_colour[list[j]] = L;
+}
+
+__global__ void __launch_bounds__(kBlock) k_clear_ints(int n, int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// the world of this workgroup and its constraint range per colour (in LDS)
+template <int B>
+__device__ inline int world_ranges(const int* __restrict__ off, int nw, int ncolors, const int* __restrict__ wlist,
+                                   int wbase, int2* rng) {
+  const int w = wlist[wbase + blockIdx.x];
+  for (int c = threadIdx.x; c < ncolors; c += B) rng[c] = make_int2(off[c * nw + w], off[c * nw + w + 1]);
+  __syncthreads();
+  return w;
+}
+
+template <int B>
+__global__ void __launch_bounds__(B) k_solve_worlds_velocity(const int* __restrict__ off, int nw, int ncolors,
+                                                             int iters, int block_solve, VelArgs a,
+                                                             const int* __restrict__ wlist, int wbase) {
+  __shared__ int2 rng[kMaxColors];
+  world_ranges<B>(off, nw, ncolors, wlist, wbase, rng);
+  for (int it = 0; it < iters; ++it)
+    for (int c = 0; c < ncolors; ++c) {
+      const int2 r = rng[c];
+      if (r.x == r.y) continue;  // uniform across the workgroup
+      for (int i = r.x + threadIdx.x; i < r.y; i += B) solve_velocity_one(i, block_solve, a);
+      __syncthreads();
+    }
+}
+
+// position iterations with the islands' early exit: after every iteration but the last (whose
+// end runs with the copy back), the world's island roots are updated as k_position_iteration_end
+// updates all of them
+template <int B>
+__global__ void __launch_bounds__(B) k_solve_worlds_position(const int* __restrict__ off, int nw, int ncolors,
+                                                             int iters, const int* __restrict__ body_off,
+                                                             const int* __restrict__ btype, PosArgs a,
+                                                             IterationEndArgs e, const int* __restrict__ wlist,
+                                                             int wbase) {
+  __shared__ int2 rng[kMaxColors];
+  const int w = world_ranges<B>(off, nw, ncolors, wlist, wbase, rng);
+  const int b0 = body_off[w], b1 = body_off[w + 1];
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < ncolors; ++c) {
+      const int2 r = rng[c];
+      if (r.x == r.y) continue;
+      for (int i = r.x + threadIdx.x; i < r.y; i += B) solve_position_one(i, true, r.x, a);
+      __syncthreads();
+    }
+    if (it + 1 < iters) {
+      for (int b = b0 + threadIdx.x; b < b1; b += B)
+        iteration_end_one(b, it, btype, e.parent, e.violated, e.done, e.active, e.isl_awake);
+      __syncthreads();
+    }
+  }
+}
+
+// one launch per body-count bucket of worlds, with that bucket's threads per world; a timed launch
+// counts its share of the items by its number of worlds
+static void launch_worlds_velocity(World* w, long long items, const int* off, int ncolors, int iters, int block_solve,
+                                   const VelArgs& a) {
+  const int nw = w->n_worlds;
+  const int* wl = w->s.world_order.p;
+  for (int k = 0; k < 3; ++k) {
+    const int n = w->world_bucket_off[k + 1] - w->world_bucket_off[k], wb = w->world_bucket_off[k];
+    if (n == 0) continue;
+    const long long share = items * n / std::max(nw, 1);
+    const int K = B2G_KERNEL_SOLVE_VELOCITY;
+    switch (w->world_threads[k]) {
+      case 64:
+        launch_timed(w, K, share, k_solve_worlds_velocity<64>, dim3(n), dim3(64), off, nw, ncolors, iters, block_solve, a, wl, wb);
+        break;
+      case 128:
+        launch_timed(w, K, share, k_solve_worlds_velocity<128>, dim3(n), dim3(128), off, nw, ncolors, iters, block_solve, a, wl, wb);
+        break;
+      case 512:
+        launch_timed(w, K, share, k_solve_worlds_velocity<512>, dim3(n), dim3(512), off, nw, ncolors, iters, block_solve, a, wl, wb);
+        break;
+      default:
+        launch_timed(w, K, share, k_solve_worlds_velocity<256>, dim3(n), dim3(256), off, nw, ncolors, iters, block_solve, a, wl, wb);
+        break;
+    }
+  }
+}
+
+static void launch_worlds_position(World* w, long long items, const int* off, int ncolors, int iters,
+                                   const PosArgs& a, const IterationEndArgs& e) {
+  const int nw = w->n_worlds;
+  const int* wl = w->s.world_order.p;
+  const int* bo = w->s.world_body_off.p;
+  const int* bt = w->b.type.p;
+  for (int k = 0; k < 3; ++k) {
+    const int n = w->world_bucket_off[k + 1] - w->world_bucket_off[k], wb = w->world_bucket_off[k];
+    if (n == 0) continue;
+    const long long share = items * n / std::max(nw, 1);
+    const int K = B2G_KERNEL_SOLVE_POSITION;
+    switch (w->world_threads[k]) {
+      case 64:
+        launch_timed(w, K, share, k_solve_worlds_position<64>, dim3(n), dim3(64), off, nw, ncolors, iters, bo, bt, a, e, wl, wb);
+        break;
+      case 128:
+        launch_timed(w, K, share, k_solve_worlds_position<128>, dim3(n), dim3(128), off, nw, ncolors, iters, bo, bt, a, e, wl, wb);
+        break;
+      case 512:
+        launch_timed(w, K, share, k_solve_worlds_position<512>, dim3(n), dim3(512), off, nw, ncolors, iters, bo, bt, a, e, wl, wb);
+        break;
+      default:
+        launch_timed(w, K, share, k_solve_worlds_position<256>, dim3(n), dim3(256), off, nw, ncolors, iters, bo, bt, a, e, wl, wb);
+        break;
+    }
+  }
+}
+
 // ---- graph-mode variants: colour ranges and counts read from device memory -----------------
 // A captured graph replays with new colour offsets (copied before each launch) and a fixed
 // grid; the loops stride over whatever the colour holds this step.
@@ -1289,6 +1442,9 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
   }
   w->n_touch = nc;
+  // batched worlds: each world's iterations in one workgroup (k_solve_worlds_*)
+  const bool per_world = nc > 0 && w->nj == 0 && w->worlds_contiguous && w->n_worlds > 1 &&
+                         (w->world_solve == 1 || (w->world_solve == -1 && w->n_worlds >= kWorldSolveMin));
   if (nc > 0) {
     // One launch per non-empty colour, in colour-id order (kept colours can leave gaps).
     // Device copy: launch offsets (kMaxColors + 1, padded with nc), then per colour id its base
@@ -1308,9 +1464,25 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
     for (int i = 0; i <= kMaxColors; ++i) offs[i] = i <= ncolors ? w->h_color_off[i] : nc;
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
-    hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream, n_touching,
-                       s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.inv_order.p,
-                       s.touch_list.p, c.colour.p);
+    if (per_world) {
+      // constraints grouped by world inside each colour (see k_color_world_hist)
+      const int nw = w->n_worlds, ncw = ncolors * nw;
+      if ((st = s.cw_count.reserve(ncw)) || (st = s.cw_cursor.reserve(ncw)) || (st = s.cw_off.reserve(ncw + 1)))
+        return st;
+      hipLaunchKernelGGL(k_clear_ints, dim3(blocks_for(ncw, kBlock)), dim3(kBlock), 0, w->stream, ncw, s.cw_count.p);
+      hipLaunchKernelGGL(k_clear_ints, dim3(blocks_for(ncw, kBlock)), dim3(kBlock), 0, w->stream, ncw, s.cw_cursor.p);
+      const int* launch = s.color_off.p + kMaxColors + 1 + kMaxColors;
+      hipLaunchKernelGGL(k_color_world_hist, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream,
+                         n_touching, s.color_state.p, launch, s.touch_list.p, c.ab.p, w->f.world.p, nw, s.cw_count.p);
+      if ((st = exclusive_scan(w, s.cw_count.p, s.cw_off.p, ncw, s.cw_off.p + ncw))) return st;
+      hipLaunchKernelGGL(k_color_world_scatter, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream,
+                         n_touching, s.color_state.p, launch, s.touch_list.p, c.ab.p, w->f.world.p, nw, s.cw_off.p,
+                         s.cw_cursor.p, s.order.p, s.inv_order.p, c.colour.p);
+    } else {
+      hipLaunchKernelGGL(k_color_scatter, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream, n_touching,
+                         s.color_state.p, s.color_off.p + kMaxColors + 1, s.color_cursor.p, s.order.p, s.inv_order.p,
+                         s.touch_list.p, c.colour.p);
+    }
     InitArgs ia;
     ia.n = nc;
     ia.order = s.order.p;
@@ -1357,7 +1529,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   // (graph nodes cannot carry the timing events of a timed solve kernel)
   const bool timed_tail = w->kt.kernel == B2G_KERNEL_WARM_START || w->kt.kernel == B2G_KERNEL_SOLVE_VELOCITY ||
                           w->kt.kernel == B2G_KERNEL_SOLVE_POSITION;
-  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0) {
+  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0 && !per_world) {
     // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
     int maxc = 0;
     for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);
@@ -1396,6 +1568,17 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                      w->j.imp2.p, w->j.r.p, w->j.k.p, w->j.k2.p, w->b.type.p, w->b.pos.p, w->b.vel.p, w->b.mass.p,
                      iv, s.isl_violated.p, s.isl_done.p, dt, 1.0f / dt, dt_ratio, int(w->warm_starting)};
   if (w->nj > 0) launch_joint_colours(w, 0, ja);
+  if (per_world) {
+    launch_worlds_velocity(w, static_cast<long long>(nc) * vel_iters, s.cw_off.p, ncolors, vel_iters,
+                           int(w->block_solve), va);
+    hipLaunchKernelGGL(k_store_and_integrate, dim3(blocks_for(std::max(nc, nb), kBlock)), dim3(kBlock), 0, w->stream,
+                       nc, s.k_contact.p, s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p, nb, w->b.type.p, w->b.pos.p,
+                       w->b.vel.p, dt, iv);
+    const IterationEndArgs ends{-1, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake};
+    if (pos_iters > 0)
+      launch_worlds_position(w, static_cast<long long>(nc) * pos_iters, s.cw_off.p, ncolors, pos_iters, pa, ends);
+    w->pos_iters_run = pos_iters;
+  } else {
   // colours [tail, ncolors): the small last colours, one launch per iteration (when at least two);
   // co holds ncolors + 1 offsets only when there are constraints
   int tail = ncolors;
@@ -1434,6 +1617,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
                          w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake);
     ++w->pos_iters_run;
+  }
   }
   const IterationEndArgs last_end{pos_iters - 1, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake};
   hipLaunchKernelGGL(k_finalize_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,

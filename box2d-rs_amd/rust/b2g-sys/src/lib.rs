@@ -32,6 +32,10 @@ pub const B2G_NOALLOC_MAX_PAIRS: i32 = 65536;
 pub const B2G_TUNE_COLOR_ROUNDS: i32 = 2;
 pub const B2G_TUNE_SOLVER_GRAPH: i32 = 3;
 pub const B2G_TUNE_COLOR_REUSE: i32 = 4;
+pub const B2G_TUNE_WORLD_SOLVE: i32 = 5;
+pub const B2G_TUNE_WORLD_THREADS_SMALL: i32 = 6;
+pub const B2G_TUNE_WORLD_THREADS_MEDIUM: i32 = 7;
+pub const B2G_TUNE_WORLD_THREADS_LARGE: i32 = 8;
 pub const B2G_KERNEL_NONE: i32 = -1;
 pub const B2G_KERNEL_SOLVE_VELOCITY: i32 = 0;
 pub const B2G_KERNEL_SOLVE_POSITION: i32 = 1;

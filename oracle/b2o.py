@@ -81,6 +81,7 @@ class Oracle:
             getattr(L, f).argtypes = [P]
         L.b2o_world_get_bodies.argtypes = [P, P]
         L.b2o_world_set_bodies.argtypes = [P, P]
+        L.b2o_world_set_awake.argtypes = [P, I, I]
         L.b2o_world_get_mass.argtypes = [P, P]
         L.b2o_world_get_proxies.argtypes = [P, P, P]
         L.b2o_world_get_contacts.argtypes = [P, I, P, P, P, P]
@@ -296,6 +297,10 @@ class OracleWorld:
     def set_bodies(self, state):
         st = self.o.cast(state, "body_state")
         self.o.lib.b2o_world_set_bodies(self.h, _ptr(st))
+
+    def set_awake(self, body, flag=True):
+        """B2body::set_awake on one body (src/b2_body.rs:783-801)."""
+        self.o.lib.b2o_world_set_awake(self.h, int(body), int(flag))
 
     def mass(self):
         out = np.zeros((self.body_count, 6), self.o.real)

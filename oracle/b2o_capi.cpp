@@ -520,6 +520,12 @@ void b2o_world_get_mass(void* wp, real* out) {
 
 // Overwrites the dynamic state of every body (sweep c, a, velocity; transform re-derived the
 // way synchronize_transform does).
+// B2body::set_awake (src/b2_body.rs:783-801) on one body
+void b2o_world_set_awake(void* wp, int body, int flag) {
+  World* w = static_cast<World*>(wp);
+  if (body >= 0 && body < int(w->bodies.size())) w->bodies[body]->set_awake(flag != 0);
+}
+
 void b2o_world_set_bodies(void* wp, const B2oBodyState* in) {
   World* w = static_cast<World*>(wp);
   for (size_t i = 0; i < w->bodies.size(); ++i) {

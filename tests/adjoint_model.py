@@ -179,9 +179,10 @@ def fd_check_world(oracle32, batch, wi, mass, state0, structure, g, gf, rng, h=1
     GPU's recorded structure, for the loss |c_body0 - goal|^2: body 0's initial velocity (3
     components), n_other random initial-velocity components of other bodies and n_friction
     random fixture frictions. Returns a dict: 'branch' False when the f64 model leaves the
-    recorded branches (its gradient is then of another function); else the relative error
-    max |g - fd| / max |fd| over the checked parameters, and the disagreement of the central
-    differences at h and h / 10 (a clamp switching inside the step shows there)."""
+    recorded branches (its gradient is then of another function); the relative error
+    max |g - fd| / max |fd| over the checked parameters ('rel_err') and over the smooth ones
+    ('rel_err_smooth': those whose central differences at h and h / 10 agree within 1e-4 of the
+    world's scale; a clamp switching inside the step shows as a disagreement there)."""
     from b2g import scenes
     sc = scenes.world_of(batch, wi)
     bsel = np.nonzero(batch.body_world == wi)[0]
@@ -227,5 +228,11 @@ def fd_check_world(oracle32, batch, wi, mass, state0, structure, g, gf, rng, h=1
         rows.append((gg, fd[0], fd[1]))
     rows = np.array(rows)
     scale = max(float(np.abs(rows[:, 1]).max()), 1e-12)
-    return {"world": int(wi), "branch": bool(branch), "rel_err": float(np.abs(rows[:, 0] - rows[:, 1]).max() / scale),
-            "fd_steps_disagree": float(np.abs(rows[:, 1] - rows[:, 2]).max() / scale), "scale": scale}
+    # a clamp (friction cone, non-negative normal impulse, block-solver case) switching inside
+    # [-h, h] makes the central difference an average over a kink; the two step sizes then disagree
+    smooth = np.abs(rows[:, 1] - rows[:, 2]) <= 1e-4 * scale
+    err = np.abs(rows[:, 0] - rows[:, 1]) / scale
+    return {"world": int(wi), "branch": bool(branch), "scale": scale, "params": len(rows),
+            "smooth": int(smooth.sum()), "rel_err": float(err.max()),
+            "rel_err_smooth": float(err[smooth].max()) if smooth.any() else None,
+            "fd_steps_disagree": float(np.abs(rows[:, 1] - rows[:, 2]).max() / scale)}

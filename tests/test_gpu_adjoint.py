@@ -52,7 +52,11 @@ def test_gpu_tape_replay_reproduces_forward():
 
 
 def test_gpu_adjoint_matches_host_f64(oracle, oracle64):
-    batch, w, s0 = _run(8, 24, 60)
+    """30 taped steps of 8 worlds of 24 bodies (checkpoints every 16): the GPU's f32 gradients
+    against the host f64 reverse pass on the GPU's recorded structure, within 1e-3 of each
+    world's gradient scale. Worlds whose f64 re-simulation leaves the recorded manifold features
+    are counted, printed and left out (at most 2 of 8)."""
+    batch, w, s0 = _run(8, 24, 30)
     final = state6(w.body_states())
     cot = _cotangent(batch, final)
     g, gf = w.tape_backward(cot.astype(np.float32))
@@ -66,8 +70,9 @@ def test_gpu_adjoint_matches_host_f64(oracle, oracle64):
     # The f64 model recomputes every manifold from its own states. Where f32 and f64 runs fall on
     # different sides of a feature change (a contact near a corner), the model follows another
     # branch than the recording for that slot, and its world's gradients are of another function.
-    # Such worlds are left out; they are rare.
+    # Such worlds are left out and counted.
     skip = set(np.unique(wid[host.mismatch_bodies()]).tolist())
+    print("f64 re-simulation left the recorded branches in %d of 8 worlds: %s" % (len(skip), sorted(skip)))
     assert len(skip) <= 2, "f64 replay left the recorded branches in %d of 8 worlds" % len(skip)
     hg, hgf, _ = host.backward(hs0, _cotangent(batch, out))
     dyn = np.nonzero(batch.body_defs["type"] == 2)[0]
@@ -79,8 +84,8 @@ def test_gpu_adjoint_matches_host_f64(oracle, oracle64):
         bsel = dyn[wid[dyn] == wi]
         a, b = g[bsel][:, 3:6].astype(np.float64), hg[bsel][:, 3:6]
         scale = max(1e-6, np.abs(b).max())
-        assert np.abs(a - b).max() <= 2e-2 * scale, "world %d: velocity gradients differ" % wi
+        assert np.abs(a - b).max() <= 1e-3 * scale, "world %d: velocity gradients differ" % wi
         fsel = np.nonzero(fworld == wi)[0]
         fa, fb = gf[fsel].astype(np.float64), hgf[fsel]
         fscale = max(1e-6, np.abs(fb).max())
-        assert np.abs(fa - fb).max() <= 2e-2 * fscale + 1e-6, "world %d: friction gradients differ" % wi
+        assert np.abs(fa - fb).max() <= 1e-3 * fscale + 1e-6, "world %d: friction gradients differ" % wi

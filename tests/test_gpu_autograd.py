@@ -3,7 +3,9 @@
 Forward: the final states equal the world's own states after the same steps; backward: the
 gradients equal b2g_tape_backward's for the same cotangent, and agree with central differences
 (h = 1e-3, f64) of the host frozen model on the recorded structure within 1e-3 of each world's
-gradient scale, on a 30-step rollout of 4 worlds of 16 bodies.
+gradient scale, on a 30-step rollout of 8 worlds of 16 bodies (parameters where the central
+differences at h and h / 10 disagree, a clamp switching inside the step, and worlds whose f64
+re-simulation leaves the recorded manifold features are counted and left out).
 """
 import numpy as np
 import pytest
@@ -18,7 +20,7 @@ def test_rollout_function_forward_backward(oracle):
     import b2g
     from b2g import scenes
     from b2g.autograd import Rollout, state6
-    batch = scenes.differentiable_worlds(4, 16)
+    batch = scenes.differentiable_worlds(8, 16)
     w = b2g.World()
     w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
     w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
@@ -43,9 +45,10 @@ def test_rollout_function_forward_backward(oracle):
     # against f64 central differences of the frozen model
     struct = tape_structure(w)
     rng = np.random.default_rng(7)
-    res = [fd_check_world(oracle, batch, wi, mass, s0, struct, g, gf, rng) for wi in range(4)]
+    res = [fd_check_world(oracle, batch, wi, mass, s0, struct, g, gf, rng) for wi in range(8)]
     print(res)
-    checked = [r for r in res if r["branch"]]
-    assert len(checked) >= 3, res
+    checked = [r for r in res if r["branch"] and r["smooth"]]
+    assert len(checked) >= 4, res
+    assert sum(r["smooth"] for r in checked) >= 40
     for r in checked:
-        assert r["rel_err"] <= 1e-3, r
+        assert r["rel_err_smooth"] <= 1e-3, r

@@ -42,3 +42,39 @@ def test_no_cross_world_pairs_and_brute_force(oracle, batch):
 def test_batched_solver_replay(oracle, batch):
     info = solver_replay_check(oracle, batch, 15)
     assert info["islands"] >= 48
+
+
+def _batched_run(batch, steps, sleep=False, **tune):
+    import b2g
+    w = b2g.World()
+    w.set_flags(warm_starting=True, allow_sleep=sleep, continuous=False, block_solve=True)
+    for k, v in tune.items():
+        w.tune(getattr(b2g, k), v)
+    w.add_batched(batch.body_defs, batch.body_world, batch.fixture_body, batch.shape_defs, batch.fixture_defs)
+    for _ in range(steps):
+        w.step(DT, 8, 3)
+    out = w.body_states().tobytes()
+    _, _, man = w.contacts()
+    w.close()
+    return out, man.tobytes()
+
+
+@pytest.mark.parametrize("sleep", [False, True])
+def test_world_solve_equals_colour_launches(batch, sleep):
+    """The per-world solve (one workgroup runs all iterations of its world, colour after colour)
+    and the per-colour launches solve the same constraints in the same colour order: states and
+    manifolds (impulses included) agree bit for bit after 90 steps, for every threads-per-world
+    setting, with sleeping off and on."""
+    ref = _batched_run(batch, 90, sleep, TUNE_WORLD_SOLVE=0)
+    for threads in (64, 128, 256, 512):
+        got = _batched_run(batch, 90, sleep, TUNE_WORLD_SOLVE=1, TUNE_WORLD_THREADS_SMALL=threads,
+                           TUNE_WORLD_THREADS_MEDIUM=threads, TUNE_WORLD_THREADS_LARGE=threads)
+        assert got[0] == ref[0], "states differ with %d threads per world" % threads
+        assert got[1] == ref[1], "manifolds differ with %d threads per world" % threads
+
+
+def test_world_solve_is_the_default_for_batches(oracle, batch):
+    """With the default settings a batch of 48 worlds takes the per-world solve, and its solve
+    replays on the CPU within 1e-5 (test_batched_solver_replay runs the default path); the
+    per-colour launches give the same bits."""
+    assert _batched_run(batch, 30) == _batched_run(batch, 30, TUNE_WORLD_SOLVE=1)

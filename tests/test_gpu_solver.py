@@ -186,12 +186,43 @@ def _max_penetration(oracle, scene, states, fab, flags, man):
     return float(max(0.0, -sep.min()))
 
 
+def _normal_residual(oracle, scene, states, fab, flags, man):
+    """Velocity-constraint residual: the largest |relative normal velocity| at a contact point
+    carrying a positive normal impulse (where the solved LCP wants it zero), at the step's end."""
+    body = np.asarray(scene.fixture_body)
+    touch = np.nonzero(flags & 1)[0]
+    if len(touch) == 0:
+        return 0.0
+    fa, fb = fab[touch, 0], fab[touch, 1]
+    ba, bb = body[fa], body[fb]
+    xa = np.concatenate([states["p"][ba], states["q"][ba]], 1)
+    xb = np.concatenate([states["p"][bb], states["q"][bb]], 1)
+    wm = oracle.world_manifold(man[touch], xa, _radius(scene, fa), xb, _radius(scene, fb)).astype(np.float64)
+    n = wm[:, 0:2]
+    out = 0.0
+    for p in range(2):
+        act = (man[touch]["point_count"] > p) & (man[touch]["points"]["normal_impulse"][:, p] > 0)
+        if not act.any():
+            continue
+        pt = wm[:, 2 + 2 * p:4 + 2 * p]
+        ra, rb = pt - states["c"][ba], pt - states["c"][bb]
+        va = states["v"][ba] + states["w"][ba][:, None] * np.stack([-ra[:, 1], ra[:, 0]], 1)
+        vb = states["v"][bb] + states["w"][bb][:, None] * np.stack([-rb[:, 1], rb[:, 0]], 1)
+        vn = ((vb - va) * n).sum(1)
+        out = max(out, float(np.abs(vn[act]).max()))
+    return out
+
+
 def test_pyramid_600_step_band(oracle):
-    """Pyramid, 600 steps: the GPU (coloured order) against the oracle (reference DFS order).
-    Band: over the whole run (including the drop of the rows) the GPU's max penetration stays
-    within 1.5x the oracle's + 0.01 m; once settled (steps 300-600) within the oracle's + 0.005 m
-    and below 3x linear slop + 0.005 m; the top box ends within 0.05 m of the oracle's; both end
-    at rest (max speed < 0.05 m/s)."""
+    """Pyramid, 600 steps: the GPU (coloured order) against the oracle (reference DFS order),
+    sampled every 10 steps. Band: over the whole run (including the drop of the rows) the GPU's max
+    penetration stays within 1.5x the oracle's + 0.01 m; once settled (steps 300-600) within the
+    oracle's + 0.005 m and below 3x linear slop + 0.005 m; the top box ends within 0.05 m of the
+    oracle's; both end at rest (max speed < 0.05 m/s). The velocity-constraint residual (largest
+    |normal relative velocity| at a point with a positive normal impulse) stays within 1.5x the
+    oracle's, over the whole run and once settled, and the momentum drift once settled (the
+    largest |sum of m v| of the pile, which at rest should be zero) within 1.5x the oracle's +
+    0.01 kg m/s."""
     import b2g
     from b2g import scenes
     sc = scenes.pyramid()
@@ -199,18 +230,28 @@ def test_pyramid_600_step_band(oracle):
     ow = oracle.world()
     ow.add(sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs)
     pen = {"g": [0.0, 0.0], "o": [0.0, 0.0]}  # [whole run, settled]
+    res = {"g": [0.0, 0.0], "o": [0.0, 0.0]}
+    mom = {"g": 0.0, "o": 0.0}  # settled
+    mass = w.mass()[:, 0].astype(np.float64)
     for i in range(600):
         w.step(DT, 8, 3)
         ow.step(DT, 8, 3)
         if i % 10 == 9:
             fab, flags, man = w.contacts()
-            pg = _max_penetration(oracle, sc, w.body_states(), fab, flags, man)
+            sg = w.body_states()
+            pg = _max_penetration(oracle, sc, sg, fab, flags, man)
+            rg = _normal_residual(oracle, sc, sg, fab, flags, man)
             ofab, oflags, oman, _ = ow.contacts()
-            po = _max_penetration(oracle, sc, ow.bodies(), ofab, oflags, oman)
-            for key, val in (("g", pg), ("o", po)):
+            so = ow.bodies()
+            po = _max_penetration(oracle, sc, so, ofab, oflags, oman)
+            ro = _normal_residual(oracle, sc, so, ofab, oflags, oman)
+            for key, val, r, st in (("g", pg, rg, sg), ("o", po, ro, so)):
                 pen[key][0] = max(pen[key][0], val)
+                res[key][0] = max(res[key][0], r)
                 if i >= 300:
                     pen[key][1] = max(pen[key][1], val)
+                    res[key][1] = max(res[key][1], r)
+                    mom[key] = max(mom[key], float(np.linalg.norm((mass[:, None] * st["v"]).sum(0))))
     sg, so = w.body_states(), ow.bodies()
     assert pen["g"][0] <= 1.5 * pen["o"][0] + 0.01
     assert pen["g"][1] <= pen["o"][1] + 0.005
@@ -218,6 +259,11 @@ def test_pyramid_600_step_band(oracle):
     assert abs(float(sg[-1]["c"][1]) - float(so[-1]["c"][1])) < 0.05
     assert float(np.abs(sg["v"][1:]).max()) < 0.05
     assert float(np.abs(so["v"][1:]).max()) < 0.05
+    print("residual whole run gpu %.5f oracle %.5f; settled gpu %.5f oracle %.5f; settled |P| gpu %.4f oracle %.4f"
+          % (res["g"][0], res["o"][0], res["g"][1], res["o"][1], mom["g"], mom["o"]))
+    assert res["g"][0] <= 1.5 * res["o"][0]
+    assert res["g"][1] <= 1.5 * res["o"][1]
+    assert mom["g"] <= 1.5 * mom["o"] + 0.01
 
 
 

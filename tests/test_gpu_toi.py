@@ -236,11 +236,11 @@ def _compare_sleep(w, o, steps, tol=1e-4):
     return w.body_states(), both
 
 
-def bounce_rain_scene(n_fast=24, n_balls=10):
-    """Fast boxes (60 m/s) landing on a thin static ground one after another, among balls with
-    restitution that bounce on the same ground (their touching contacts with it separate and are
-    destroyed in steps that also have TOI events), with sleeping on: the balls and boxes come to
-    rest and fall asleep."""
+def bounce_rain_scene(n_fast=24, n_balls=20):
+    """Fast boxes (60 m/s) landing on a thin static ground one after another over about 40 steps,
+    among balls with restitution that leave the same ground upward at different speeds and bounce
+    on it (their touching contacts with it separate and are destroyed in steps that also have TOI
+    events), with sleeping on: everything comes to rest and falls asleep."""
     n = n_fast + n_balls
     bd = np.zeros(1 + n, b2g.BODY_DEF)
     bd["gravity_scale"] = 1.0
@@ -248,10 +248,10 @@ def bounce_rain_scene(n_fast=24, n_balls=10):
     bd["allow_sleep"] = 1
     bd[1:]["type"] = 2
     xs = np.arange(n) * 1.1 - 0.55 * n
-    bd[1:1 + n_fast]["position"] = np.stack([xs[:n_fast], 2.0 + 0.45 * np.arange(n_fast)], 1)
+    bd[1:1 + n_fast]["position"] = np.stack([xs[:n_fast], 2.0 + 1.7 * np.arange(n_fast)], 1)
     bd[1:1 + n_fast]["linear_velocity"] = (0.0, -60.0)
-    bd[1 + n_fast:]["position"] = np.stack([xs[n_fast:], 0.35 + 0.07 * np.arange(n_balls)], 1)
-    bd[1 + n_fast:]["linear_velocity"] = np.stack([np.zeros(n_balls), -3.0 - 0.2 * np.arange(n_balls)], 1)
+    bd[1 + n_fast:]["position"] = np.stack([xs[n_fast:], np.full(n_balls, 0.05 + 0.2)], 1)
+    bd[1 + n_fast:]["linear_velocity"] = np.stack([np.zeros(n_balls), 1.5 + 0.15 * np.arange(n_balls)], 1)
     sd = np.zeros(1 + n, b2g.SHAPE_DEF)
     sd["type"] = b2g.SHAPE_BOX
     sd[0]["params"][:2] = (0.6 * n + 2.0, 0.05)
@@ -265,40 +265,91 @@ def bounce_rain_scene(n_fast=24, n_balls=10):
 
 def test_sleep_with_continuous_matches_oracle(oracle):
     """Sleeping and continuous physics on together (B2world::new's defaults): every step's states,
-    awake flags, sleep times and TOI event counts agree with the oracle, including steps where a
-    touching contact is destroyed in a step with TOI events (the destroy wakes both bodies and
-    resets their sleep times, src/private/dynamics/b2_contact.rs:33-45), and the scene ends
-    asleep."""
+    awake flags, sleep times and TOI event counts agree with the oracle while fast boxes land and
+    balls bounce, and the scene ends asleep."""
     w, o = _pair(oracle, bounce_rain_scene(), True, sleep=True)
-    st, both = _compare_sleep(w, o, 150)
-    assert both >= 1
+    st, _ = _compare_sleep(w, o, 150)
     assert (st["awake"][1:] == 0).sum() > 0
 
 
-def corner_scene():
-    """Fast boxes dropped into a corner of two static boxes (a floor and a wall) while sliding
-    along the wall: in their TOI events the body touches both static fixtures."""
-    n = 5
-    bd = np.zeros(2 + n, b2g.BODY_DEF)
+def launch_scene(n_fast=20):
+    """A box resting on a box on a static ground (they fall asleep together), beside a rain of
+    fast boxes landing one after another from about step 38 on (TOI events every step)."""
+    bd = np.zeros(3 + n_fast, b2g.BODY_DEF)
     bd["gravity_scale"] = 1.0
     bd["awake"] = 1
     bd["allow_sleep"] = 1
-    bd[1]["position"] = (-0.35, 3.0)  # wall, its right face at x = -0.3
-    bd[2:]["type"] = 2
-    bd[2:]["position"] = np.stack([np.full(n, -0.3 + 0.2 + 0.001), 2.0 + 1.3 * np.arange(n)], 1)
-    bd[2:]["linear_velocity"] = np.stack([-0.5 * np.ones(n), -55.0 - 3.0 * np.arange(n)], 1)
-    sd = np.zeros(2 + n, b2g.SHAPE_DEF)
+    bd[1:]["type"] = 2
+    bd[1]["position"] = (-6.0, 0.05 + 0.3)
+    bd[2]["position"] = (-6.0, 0.05 + 0.6 + 0.3)
+    t = 38.0 / 60.0 + np.arange(n_fast) * 0.6 / 60.0  # arrival times
+    bd[3:]["position"] = np.stack([np.arange(n_fast) * 0.9, 0.3 + 60.0 * t + 5.0 * t * t], 1)
+    bd[3:]["linear_velocity"] = (0.0, -60.0)
+    sd = np.zeros(3 + n_fast, b2g.SHAPE_DEF)
     sd["type"] = b2g.SHAPE_BOX
-    sd[0]["params"][:2] = (6.0, 0.05)
-    sd[1]["params"][:2] = (0.05, 3.0)
-    sd[2:]["params"][:, :2] = (0.2, 0.2)
-    return bd, np.arange(2 + n, dtype=np.int32), sd, _fixtures(2 + n, 0.3), (0.0, -10.0)
+    sd[0]["params"][:2] = (30.0, 0.05)
+    sd[1:3]["params"][:, :2] = (0.3, 0.3)
+    sd[3:]["params"][:, :2] = (0.25, 0.15)
+    return bd, np.arange(3 + n_fast, dtype=np.int32), sd, _fixtures(3 + n_fast), (0.0, -10.0)
+
+
+def test_touching_contact_destroyed_in_a_toi_step(oracle):
+    """A box asleep on an asleep box is launched upward at 30 m/s (awake, as B2body::set_awake(true)
+    with set_linear_velocity leave it) while fast boxes land elsewhere: in the next step the
+    touching contact between the two boxes leaves the contact set in a step with TOI events, and the
+    collide after it destroys it and wakes both bodies, resetting the lower box's sleep time
+    (src/private/dynamics/b2_contact.rs:33-45, src/b2_body.rs:783-801). Every step's states, awake
+    flags, sleep times and TOI event counts agree with the oracle's."""
+    w, o = _pair(oracle, launch_scene(), True, sleep=True)
+    _compare_sleep(w, o, 42)
+    g, r = w.body_states(), o.bodies()
+    assert g["awake"][1] == 0 and g["awake"][2] == 0  # the two boxes sleep
+    for st in (g, r):
+        st["v"][2] = (0.0, 30.0)
+        st["awake"][2] = 1
+        st["sleep_time"][2] = 0.0
+    w.set_body_states(g)
+    o.set_bodies(r)
+    o.set_awake(2, True)
+    _, both = _compare_sleep(w, o, 20)
+    assert both >= 1
+
+
+def corner_scene():
+    """Fast frictionless balls dropped into corners of a static floor and static walls, each ball
+    pressed against its wall while it falls: in its TOI event with the floor it touches both static
+    fixtures. (Frictionless balls: the wall's and the floor's normals are orthogonal and pass
+    through the centre, so the two constraints on a ball do not interact and the solve order,
+    which differs between the GPU's colours and the reference's island order, does not matter.)"""
+    n = 5
+    bd = np.zeros(1 + 2 * n, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    sd = np.zeros(1 + 2 * n, b2g.SHAPE_DEF)
+    sd[0]["type"] = b2g.SHAPE_BOX
+    sd[0]["params"][:2] = (20.0, 0.05)
+    for i in range(n):
+        x0 = -10.0 + 4.0 * i
+        bd[1 + i]["position"] = (x0 - 0.05, 3.0)  # wall, its right face at x0
+        sd[1 + i]["type"] = b2g.SHAPE_BOX
+        sd[1 + i]["params"][:2] = (0.05, 3.0)
+        b = bd[1 + n + i]
+        b["type"] = 2
+        b["position"] = (x0 + 0.2 - 0.002, 2.0 + 0.9 * i)
+        b["linear_velocity"] = (-0.3, -55.0 - 3.0 * i)
+        sd[1 + n + i]["type"] = b2g.SHAPE_CIRCLE
+        sd[1 + n + i]["radius"] = 0.2
+    return bd, np.arange(1 + 2 * n, dtype=np.int32), sd, _fixtures(1 + 2 * n, 0.0), (0.0, -10.0)
 
 
 def test_toi_island_touching_several_static_fixtures(oracle):
     w, o = _pair(oracle, corner_scene(), True)
     st = _compare(w, o, 50)
-    assert (st["c"][2:, 1] > 0.05).all() and (st["c"][2:, 0] > -0.3).all(), st["c"][2:]
+    n = 5
+    balls = st["c"][1 + n:]
+    walls = -10.0 + 4.0 * np.arange(n)
+    assert (balls[:, 1] > 0.05 + 0.2 - 0.02).all() and (balls[:, 0] > walls + 0.2 - 0.02).all(), balls
 
 
 def shared_platform_scene():
