@@ -326,7 +326,7 @@ def run_batched(args, b2g, scenes):
         out["config"]["sleep"] = True
         out["config"]["awake_after"] = awake_fraction(w, batch.body_defs)
     # plain per-world stepping: a sample of the same worlds, each its own world, stepped in turn
-    sample = list(range(0, nw, nw // 32))
+    sample = list(range(0, nw, nw // 32)) if not args.no_plain else []
     per_world = []
     for wi in sample:
         sc = scenes.world_of(batch, wi)
@@ -337,9 +337,10 @@ def run_batched(args, b2g, scenes):
             one.step(1.0 / 60.0, 8, 3)
         t, _ = timed_steps(one, 20, "none")
         per_world.append(t / 20)
-    plain = 1.0 / (sum(per_world) / len(per_world))
-    out["plain_per_world"] = {"value": plain, "unit": "world-steps/s", "sample_worlds": len(sample),
-                              "ratio": out["value"] / plain}
+    if per_world:
+        plain = 1.0 / (sum(per_world) / len(per_world))
+        out["plain_per_world"] = {"value": plain, "unit": "world-steps/s", "sample_worlds": len(sample),
+                                  "ratio": out["value"] / plain}
     w.close()
     k_ms, k_launches, k_items, k_aux = replay_kernel(make, args.warmup, steps, args.roofline_kernel)
     out["roofline"] = roofline(args.roofline_kernel, k_ms, k_launches, k_items, k_aux, pair_shape=pair_shape)
@@ -459,6 +460,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="large_world: leave out the batched, adjoint and stress summaries")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched: leave out the one-world-at-a-time comparison")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: launch settings at their defaults instead of tuned on a copy of the batch")
     ap.add_argument("--no-full-run", action="store_true",
@@ -496,7 +499,7 @@ def extras(args, b2g, scenes):
                       "ms_per_step": b["ms_per_step"], "worlds": b["config"]["worlds"],
                       "autotuned": b["config"].get("autotuned"),
                       "bodies": b["config"]["bodies"], "body_steps_per_s": b["body_steps_per_s"],
-                      "plain_per_world": b["plain_per_world"],
+                      "plain_per_world": b.get("plain_per_world"),
                       "roofline": {k: b["roofline"][k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
     a = run_adjoint(sub, b2g, scenes)
     out["adjoint"] = {"metric": a["metric"], "value": a["value"], "unit": a["unit"], "steps": a["steps"],

@@ -21,9 +21,9 @@ file lists them newest first, and a reader creates them in file order (so a worl
 its lists in the opposite order). Polygons are taken as stored (vertices, normals, centroid:
 ``SHAPE_POLYGON_EXACT``), not re-hulled.
 
-Numbers are written as the shortest decimal that reads back as the same f32 and read with
-correct rounding to f32, so values survive a round trip exactly; the text is not promised to be
-byte-identical to serde_json's (which formats floats with ryu).
+Numbers are written as serde_json writes an f32 (ryu's shortest round-trip digits in ryu's f32
+layout, see _f32_text) and read with correct rounding to f32, so values survive a round trip
+exactly and the text is what serde_json would write for the same values.
 
 Checkpoint extension (a ``"b2g_checkpoint"`` member of the world object): the state a GPU world
 needs to continue exactly as the written one would: every body's full state (transform, sweep
@@ -71,11 +71,33 @@ class F32(float):
 
 
 def _f32_text(x):
+    """An f32 as serde_json writes it: ryu's shortest decimal digits that read back as the same
+    f32, laid out as ryu's f32 formatter does (the ryu crate, pretty::format32): with the digits d
+    and the power of ten kk such that 10^(kk-1) <= |x| < 10^kk, plain notation when
+    -6 < kk <= 13 ("1234.5", "0.001234", integral values with ".0"), otherwise d.ddd followed by
+    "e" and the exponent kk - 1 without a plus sign or leading zeros ("1e-7", "1.5e13")."""
     v = np.float32(x)
     if not np.isfinite(v):
         raise ValueError("serde world: non-finite number %r" % x)
-    t = str(v)
-    return t if ("." in t or "e" in t or "inf" in t or "nan" in t) else t + ".0"
+    sign = "-" if np.signbit(v) else ""
+    if v == 0:
+        return sign + "0.0"
+    mant, exp = np.format_float_scientific(abs(v), unique=True, trim="-").split("e")
+    digits = mant.replace(".", "")
+    length = len(digits)
+    kk = int(exp) + 1
+    k = kk - length  # |x| = digits x 10^k
+    if 0 <= k and kk <= 13:
+        text = digits + "0" * (kk - length) + ".0"
+    elif 0 < kk <= 13:
+        text = digits[:kk] + "." + digits[kk:]
+    elif -6 < kk <= 0:
+        text = "0." + "0" * (-kk) + digits
+    elif length == 1:
+        text = digits + "e" + str(kk - 1)
+    else:
+        text = digits[0] + "." + digits[1:] + "e" + str(kk - 1)
+    return sign + text
 
 
 def _parse_f32(text):

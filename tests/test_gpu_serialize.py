@@ -12,6 +12,11 @@
   written again right after reading, its text is the same.
 * Joints read from a file (indices as the writer meant them) step bit-identically with the same
   joints created directly in file order.
+* Against the oracle: scenes written as serde JSON (the reference's hello_world box, a fast ball
+  at a thin wall) and read back step on the GPU like the oracle stepping the same file's world with
+  B2world::new's flags (warm starting, sleeping and continuous physics on), each step within 1e-5
+  (1e-4 with TOI events, the tolerance of the continuous-collision parity tests), awake flags and
+  sleep times exactly.
 """
 import numpy as np
 import pytest
@@ -172,3 +177,37 @@ def test_read_joints_step_like_direct_creation():
         w2.step(DT, 8, 3)
     _states_equal(w1.body_states(), w2.body_states(), "read vs direct")
 
+
+
+def _oracle_from_doc(oracle, doc):
+    ow = oracle.world(doc["gravity"])
+    ow.set_flags(warm_starting=True, allow_sleep=True, block_solve=True, continuous=True)
+    ow.add(doc["body_defs"], doc["fixture_body"], doc["shape_defs"], doc["fixture_defs"])
+    return ow
+
+
+@pytest.mark.parametrize("kind", ["hello_world", "wall"])
+def test_read_world_steps_like_the_oracle(oracle, kind):
+    import b2g
+    from b2g import scenes
+    from test_gpu_toi import wall_scene
+    if kind == "hello_world":
+        sc = scenes.hello_world()
+        bd, fb, sd, fd, g = sc.body_defs, sc.fixture_body, sc.shape_defs, sc.fixture_defs, (0.0, -10.0)
+    else:
+        bd, fb, sd, fd, g = wall_scene()
+    src = b2g.World(g)
+    src.add(bd, fb, sd, fd)
+    text = ser.world_to_json(src)
+    w = ser.world_from_json(text)
+    assert w.flags == {"warm_starting": True, "allow_sleep": True, "continuous": True, "block_solve": True}
+    ow = _oracle_from_doc(oracle, ser.read_world(text))
+    tol = 1e-5 if kind == "hello_world" else 1e-4
+    for k in range(90):
+        w.step(DT, 8, 3)
+        ow.step(DT, 8, 3)
+        gs, os_ = w.body_states(), ow.bodies()
+        for f in ("c", "a", "v", "w"):
+            np.testing.assert_allclose(gs[f], os_[f], rtol=0, atol=tol, err_msg="step %d %s" % (k, f))
+        np.testing.assert_array_equal(gs["awake"], os_["awake"], err_msg="step %d" % k)
+        np.testing.assert_array_equal(gs["sleep_time"], os_["sleep_time"], err_msg="step %d" % k)

@@ -13,6 +13,9 @@ same in the reference's list order and the GPU's contact-set order:
   * sleeping and continuous physics on together: fast boxes landing among bouncing balls whose
     ground contacts separate in steps with TOI events; awake flags and sleep times compared
     exactly with the oracle's at every step;
+  * a touching contact whose fat AABBs separate in a TOI sub-step (an asleep box launched into a
+    ceiling off the slab it touches): the next collide wakes both bodies, as the reference's
+    destroy does, and the slab's sleep time is reset as the oracle's;
   * fast boxes dropped into a static corner (a TOI island touching two static fixtures) and fast
     balls landing at distinct times on one moving kinematic platform (events sharing a kinematic
     body);
@@ -273,46 +276,62 @@ def test_sleep_with_continuous_matches_oracle(oracle):
 
 
 def launch_scene(n_fast=20):
-    """A box resting on a box on a static ground (they fall asleep together), beside a rain of
-    fast boxes landing one after another from about step 38 on (TOI events every step)."""
-    bd = np.zeros(3 + n_fast, b2g.BODY_DEF)
+    """A box floating with its core 0.018 m above a floating slab (no gravity on either: their
+    manifold touches, no impulse flows, both fall asleep), a thin static ceiling 0.5 m above the
+    box, and a rain of fast boxes landing on a static ground one after another from about step 38
+    on (TOI events every step). Each floating body has one constraint, so the solve order, which
+    differs between the GPU's colours and the reference's island order, does not matter."""
+    m = 4 + n_fast
+    bd = np.zeros(m, b2g.BODY_DEF)
     bd["gravity_scale"] = 1.0
     bd["awake"] = 1
     bd["allow_sleep"] = 1
-    bd[1:]["type"] = 2
-    bd[1]["position"] = (-6.0, 0.05 + 0.3)
-    bd[2]["position"] = (-6.0, 0.05 + 0.6 + 0.3)
+    bd[1]["position"] = (-6.0, 0.918 + 0.3 + 0.5 + 0.05)  # static ceiling
+    bd[2:]["type"] = 2
+    bd[2]["position"] = (-6.0, 0.5)  # slab, half height 0.1
+    bd[3]["position"] = (-6.0, 0.918)  # box, half height 0.3
+    bd[2:4]["gravity_scale"] = 0.0
     t = 38.0 / 60.0 + np.arange(n_fast) * 0.6 / 60.0  # arrival times
-    bd[3:]["position"] = np.stack([np.arange(n_fast) * 0.9, 0.3 + 60.0 * t + 5.0 * t * t], 1)
-    bd[3:]["linear_velocity"] = (0.0, -60.0)
-    sd = np.zeros(3 + n_fast, b2g.SHAPE_DEF)
+    bd[4:]["position"] = np.stack([np.arange(n_fast) * 0.9, 0.3 + 60.0 * t + 5.0 * t * t], 1)
+    bd[4:]["linear_velocity"] = (0.0, -60.0)
+    sd = np.zeros(m, b2g.SHAPE_DEF)
     sd["type"] = b2g.SHAPE_BOX
     sd[0]["params"][:2] = (30.0, 0.05)
-    sd[1:3]["params"][:, :2] = (0.3, 0.3)
-    sd[3:]["params"][:, :2] = (0.25, 0.15)
-    return bd, np.arange(3 + n_fast, dtype=np.int32), sd, _fixtures(3 + n_fast), (0.0, -10.0)
+    sd[1]["params"][:2] = (1.0, 0.05)
+    sd[2]["params"][:2] = (0.5, 0.1)
+    sd[3]["params"][:2] = (0.3, 0.3)
+    sd[4:]["params"][:, :2] = (0.25, 0.15)
+    return bd, np.arange(m, dtype=np.int32), sd, _fixtures(m), (0.0, -10.0)
 
 
 def test_touching_contact_destroyed_in_a_toi_step(oracle):
-    """A box asleep on an asleep box is launched upward at 30 m/s (awake, as B2body::set_awake(true)
-    with set_linear_velocity leave it) while fast boxes land elsewhere: in the next step the
-    touching contact between the two boxes leaves the contact set in a step with TOI events, and the
-    collide after it destroys it and wakes both bodies, resetting the lower box's sleep time
-    (src/private/dynamics/b2_contact.rs:33-45, src/b2_body.rs:783-801). Every step's states, awake
-    flags, sleep times and TOI event counts agree with the oracle's."""
+    """The asleep box is launched upward at 60 m/s (awake, as B2body::set_awake(true) with
+    set_linear_velocity leave it). In the next step it hits the ceiling in a TOI event, whose
+    sub-step moves its fat AABB off the slab's while their contact, not updated by the event (both
+    bodies dynamic), still touches: the GPU's rebuild of the contact set after the events drops it
+    and hands it to the next collide, which wakes both bodies, as the reference's collide destroys
+    it (src/private/dynamics/b2_contact_manager.rs:146-153, src/private/dynamics/b2_contact.rs:33-45)
+    and its set_awake(true) resets the slab's sleep time (src/b2_body.rs:783-801). Every step's
+    states, awake flags, sleep times and TOI event counts agree with the oracle's."""
     w, o = _pair(oracle, launch_scene(), True, sleep=True)
     _compare_sleep(w, o, 42)
     g, r = w.body_states(), o.bodies()
-    assert g["awake"][1] == 0 and g["awake"][2] == 0  # the two boxes sleep
+    assert (g["awake"][2:4] == 0).all()  # slab and box sleep
     for st in (g, r):
-        st["v"][2] = (0.0, 30.0)
-        st["awake"][2] = 1
-        st["sleep_time"][2] = 0.0
+        st["v"][3] = (0.0, 60.0)
+        st["awake"][3] = 1
+        st["sleep_time"][3] = 0.0
     w.set_body_states(g)
     o.set_bodies(r)
-    o.set_awake(2, True)
-    _, both = _compare_sleep(w, o, 20)
-    assert both >= 1
+    o.set_awake(3, True)
+    _compare_sleep(w, o, 1)
+    assert o.toi_events >= 1
+    fab, _ = w.destroyed_contacts()
+    assert (2, 3) in {tuple(sorted(map(int, p))) for p in fab}, fab
+    _compare_sleep(w, o, 1)
+    # the slab's sleep time: reset by the wake, then one step of rest (2 h without the wake)
+    assert w.body_states()["sleep_time"][2] == DT
+    _compare_sleep(w, o, 10)
 
 
 def corner_scene():

@@ -24,6 +24,7 @@
 * What the GPU path does not hold (wheel and other joint kinds, chains, disabled bodies) is refused.
 """
 import json
+import re
 import os
 
 import numpy as np
@@ -193,3 +194,39 @@ def test_golden_joints_read_and_write_back():
     out = ser.defs_to_json(doc["gravity"], bd[order], remap[fb][::-1], sd[::-1], fd[::-1], joints=jw)
     assert out == text
 
+
+
+def test_f32_text_is_serde_json_layout():
+    """serde_json writes an f32 with ryu: the shortest decimal digits that read back as the same
+    f32, in plain notation when -6 < kk <= 13 (10^(kk-1) <= |x| < 10^kk; integral values end in
+    ".0"), else as d.ddd e(kk-1) with no plus sign and no leading exponent zeros."""
+    cases = [(1.0, "1.0"), (0.5, "0.5"), (100.0, "100.0"), (1e10, "10000000000.0"), (1e13, "1e13"),
+             (1.5e13, "1.5e13"), (123456789.0, "123456790.0"), (0.001234, "0.001234"), (1e-5, "0.00001"),
+             (1e-6, "0.000001"), (1e-7, "1e-7"), (3.4028235e38, "3.4028235e38"), (1.17549435e-38, "1.1754944e-38"),
+             (-0.0, "-0.0"), (0.0, "0.0"), (0.1, "0.1"), (-2.5, "-2.5"), (9.999999e12, "9999999000000.0")]
+    for x, want in cases:
+        assert ser._f32_text(x) == want, (x, ser._f32_text(x), want)
+    rng = np.random.default_rng(12)
+    bits = rng.integers(0, 2 ** 32, 40000, dtype=np.uint64).astype(np.uint32)
+    vals = bits.view(np.float32)
+    vals = np.concatenate([vals[np.isfinite(vals)][:8000], rng.uniform(-50, 50, 4000).astype(np.float32),
+                           (10.0 ** rng.uniform(-9, 16, 4000)).astype(np.float32)])
+    plain = re.compile(r"^-?(0\.0*[1-9][0-9]*|[1-9][0-9]*\.[0-9]+)$")
+    sci = re.compile(r"^-?[1-9](\.[0-9]*[1-9])?e-?[1-9][0-9]?$")
+    for x in vals:
+        t = ser._f32_text(x)
+        assert np.float32(float(t)).view(np.uint32) == x.view(np.uint32), (x, t)
+        if x == 0:
+            continue
+        kk = int(np.floor(np.log10(abs(float(np.float32(float(t))))))) + 1
+        if -6 < kk <= 13:
+            assert plain.match(t), (x, t)
+        else:
+            assert sci.match(t), (x, t)
+        # shortest: no decimal with one significant digit fewer reads back as x
+        mant = t.lstrip("-").split("e")[0].replace(".", "").strip("0")
+        n = len(mant)
+        if n > 1:
+            for cand in ("%.*e" % (n - 2, float(x)), "%.*e" % (n - 2, np.nextafter(float(x), 0.0)),
+                         "%.*e" % (n - 2, np.nextafter(float(x), np.inf))):
+                assert np.float32(float(cand)).view(np.uint32) != x.view(np.uint32), (x, t, cand)
