@@ -40,7 +40,11 @@ HBM_PEAK_GBS = 8000.0
 # tangent masses, anchors, bias, block matrix and its inverse, impulses; both bodies' velocities
 # read and written). It is the kernel with the largest share of a step's time; its launches are
 # short (a few microseconds), so it is bound by launch and dependent-load latency more than by
-# bandwidth, which the fraction shows.
+# bandwidth, which the fraction shows. In steps where small islands hold most constraints the
+# velocity iterations run as one launch of k_solve_bins_velocity (all 8 iterations of every bin of
+# islands); its algorithmic bytes are counted the same way (224 B per constraint-iteration), while
+# its HBM traffic is far lower: it reads each constraint and body once and keeps them on chip
+# (registers and LDS) across the iterations.
 # k_collide (--roofline-kernel collide): algorithmic HBM bytes of one narrowphase launch (f32/i32 sizes, each read or written once): per
 # candidate pair 120 B (pair, fixture and body records, the two transforms, the previous
 # manifold's ids and impulses, the new manifold) plus both shapes: circle 12 B, polygon 12 + 16 n
@@ -140,7 +144,7 @@ def roofline(kernel, k_ms, k_launches, k_items, k_aux, window=None, pair_shape=N
     achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, window) if window else None,
-            "kernel": "k_" + kernel, "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
+            "kernel": solver_kernel_name(kernel), "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_launch": bytes_per_launch, "unit_bytes": UNIT_BYTES[kernel],
             "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[kernel]}
     if kernel == "traverse":
@@ -175,9 +179,32 @@ def replay_kernel(make_world, warmup, steps, kernel):
     r = make_world()
     for _ in range(warmup):
         r.step(1.0 / 60.0, 8, 3)
-    _, totals = timed_steps(r, steps, kernel)
+    r.body_states()
+    r.time_kernel(kernel)
+    bin_steps = 0
+    for _ in range(steps):
+        r.step(1.0 / 60.0, 8, 3)
+        bin_steps += r.profile()["island_bins"] > 0
+    totals = r.kernel_time()
+    r.time_kernel("none")
     r.close()
+    LAST_REPLAY["bin_steps"], LAST_REPLAY["steps"] = bin_steps, steps
     return totals
+
+
+# steps of the last replay whose solve ran the island bins (names the roofline's kernel)
+LAST_REPLAY = {"bin_steps": 0, "steps": 0}
+
+
+def solver_kernel_name(kernel):
+    if kernel not in ("solve_velocity", "solve_position"):
+        return "k_" + kernel
+    b, n = LAST_REPLAY["bin_steps"], LAST_REPLAY["steps"]
+    if b == 0:
+        return "k_" + kernel
+    if b == n:
+        return "k_solve_bins_" + kernel.split("_")[1]
+    return "k_%s + k_solve_bins_%s (bins in %d of %d steps)" % (kernel, kernel.split("_")[1], b, n)
 
 
 def awake_fraction(w, body_defs):

@@ -32,6 +32,7 @@ KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2
 # tuning knobs (B2G_TUNE_*)
 TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH, TUNE_COLOR_REUSE = 2, 3, 4
 TUNE_WORLD_SOLVE, TUNE_WORLD_THREADS_SMALL, TUNE_WORLD_THREADS_MEDIUM, TUNE_WORLD_THREADS_LARGE = 5, 6, 7, 8
+TUNE_ISLAND_SOLVE, TUNE_ISLAND_THREADS = 9, 10
 
 # Joint definition (B2G_JOINT_*: B2jointType values, src/b2_joint.rs:46-58)
 JOINT_DISTANCE, JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_WELD = 1, 6, 8, 9
@@ -78,7 +79,7 @@ class _Profile(ctypes.Structure):
     _fields_ = [("step", ctypes.c_float), ("collide", ctypes.c_float), ("solve", ctypes.c_float),
                 ("broadphase", ctypes.c_float), ("pairs", ctypes.c_int32), ("touching", ctypes.c_int32),
                 ("colours", ctypes.c_int32), ("islands", ctypes.c_int32), ("position_iterations", ctypes.c_int32),
-                ("toi_events", ctypes.c_int32), ("pad", ctypes.c_int32 * 2)]
+                ("toi_events", ctypes.c_int32), ("island_bins", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 def _load():

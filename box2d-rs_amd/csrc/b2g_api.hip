@@ -554,7 +554,7 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
   w->gravity = v2(gx, gy);
   w->device = dev;
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
-      w->counters.reserve(4 + 128 + 1) != B2G_OK ||
+      w->counters.reserve(4 + 128 + 2) != B2G_OK ||
       hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     set_error("world allocation failed");
     delete h;
@@ -596,6 +596,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.color_state); rel(s.color_off); rel(s.color_cursor); rel(s.prev_colour); rel(s.order); rel(s.inv_order); rel(s.parent); rel(s.isl_awake); rel(s.isl_restless);
   rel(s.isl_violated); rel(s.isl_done);
   rel(s.cw_count); rel(s.cw_off); rel(s.cw_cursor); rel(s.world_body_off); rel(s.world_order);
+  rel(s.isl_count); rel(s.isl_small); rel(s.isl_start); rel(s.croot);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes); rel(bp.nodes4);
   rel(bp.leafrec); rel(bp.count); rel(bp.slots); rel(bp.bounds); rel(bp.scan_tmp);
@@ -1264,6 +1265,7 @@ int b2g_world_get_profile(b2g_world* h, b2g_profile* out) {
   out->colours = w->n_colors;
   out->position_iterations = w->pos_iters_run;
   out->toi_events = w->toi_events;
+  out->island_bins = w->n_bins;
   return B2G_OK;
 }
 
@@ -1291,6 +1293,14 @@ int b2g_world_tune(b2g_world* h, int32_t param, int32_t value) {
     case B2G_TUNE_WORLD_THREADS_LARGE:
       if (value != 64 && value != 128 && value != 256 && value != 512) break;
       w->world_threads[param - B2G_TUNE_WORLD_THREADS_SMALL] = value;
+      return B2G_OK;
+    case B2G_TUNE_ISLAND_SOLVE:
+      if (value < -1 || value > 1) break;
+      w->island_solve = value;
+      return B2G_OK;
+    case B2G_TUNE_ISLAND_THREADS:
+      if (value != 64 && value != 128 && value != 256 && value != 512) break;
+      w->bin_threads = value;
       return B2G_OK;
     default: break;
   }

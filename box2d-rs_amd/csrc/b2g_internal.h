@@ -129,6 +129,9 @@ struct SolverArrays {
   // per-world solve: constraints of each (colour, world), their exclusive scan (colour-major), the
   // scatter cursors, the worlds' body ranges and the launch order of the worlds
   DevArray<int> cw_count, cw_off, cw_cursor, world_body_off, world_order;
+  // island solve: constraints per island root, the small islands' counts and their exclusive scan,
+  // each constraint's island root
+  DevArray<int> isl_count, isl_small, isl_start, croot;
   DevArray<int> touch_flag, touch_scan, touch_list;   // compaction of touching contacts
   DevArray<int> k_contact;   // contact index of constraint
   DevArray<int2> k_bodies;
@@ -329,6 +332,11 @@ struct World {
   // world_threads[k]: threads per world for the worlds of body-count bucket k (see world_bucket).
   int world_solve = -1;
   int world_threads[3] = {64, 128, 256};
+  // island_solve: -1 auto (on unless the per-world solve is), 0 off, 1 on; bin_threads: threads per
+  // bin (islands of at most bin_threads / 2 constraints are binned)
+  int island_solve = -1;
+  int bin_threads = 256;
+  int n_bins = 0;  // bins of the last solve
   bool worlds_contiguous = false;         // every world's bodies are one index range (set at upload)
   std::vector<int> h_world_body_off;      // n_worlds + 1 (when contiguous)
   std::vector<int> h_world_order;         // worlds by bucket, larger worlds first within a bucket

@@ -20,6 +20,9 @@ namespace b2g {
 
 static constexpr int kBlock = 256;
 static constexpr int kMaxColors = 128;
+// solver counter after the colour histogram and the last colouring round: constraints in islands
+// too large for the island solve's bins
+static constexpr int kBigIslandSlot = 4 + kMaxColors + 1;
 
 // ---- integration --------------------------------------------------------------------------
 // src/private/dynamics/b2_island_private.rs:135-170
@@ -158,6 +161,7 @@ __device__ inline void counters_init(int t, int* c, int* __restrict__ cursor) {
     c[1] = c[0];
     c[2] = 0;
     c[3] = 0;
+    c[kBigIslandSlot] = 0;
   }
   for (int i = t; i <= kMaxColors; i += kBlock) c[4 + i] = 0;
   for (int i = t; i < kMaxColors; i += kBlock) cursor[i] = 0;
@@ -208,12 +212,14 @@ __global__ void __launch_bounds__(kBlock) k_fill_adjacency(const int* __restrict
 // per body: its own island root, no constraints yet; when every island is awake (no sleeping
 // bodies possible) also the velocity integration, which then does not wait for the islands
 __global__ void __launch_bounds__(kBlock) k_solver_body_init(int nb, int* __restrict__ parent, int* __restrict__ deg,
-                                                             int* __restrict__ cursor, int integrate, IntegrateArgs ia) {
+                                                             int* __restrict__ cursor, int integrate, IntegrateArgs ia,
+                                                             int* __restrict__ isl_count) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     parent[b] = b;
     deg[b] = 0;
     cursor[b] = 0;
+    if (isl_count) isl_count[b] = 0;
     if (integrate) integrate_velocity_one(b, ia, IslandView{parent, nullptr}, nullptr);
   }
   if (b == 0) deg[nb] = 0;
@@ -397,17 +403,52 @@ __device__ inline int color_pick(int j, int round, const int2* __restrict__ bodi
 // Colours at or above `cap` are not kept: those contacts are coloured again first-fit, which
 // pulls the colour count back down as the contact graph changes.
 // Constraints of sleeping islands are left out of the solve (state -2, never coloured).
+// counts[key] += 1 for every thread's key (-1: none), aggregated per block in an LDS table so
+// that a key shared by many threads (the root of a large island) takes one global atomic per
+// block. Called by every thread of a kBlock-thread block.
+__device__ inline void block_count_keys(int key, int* __restrict__ counts) {
+  constexpr int H = 2 * kBlock, LG = 9;  // at most kBlock keys: the table stays at most half full
+  static_assert((1 << LG) == H, "table size");
+  __shared__ int tkey[H];
+  __shared__ int tcnt[H];
+  for (int e = threadIdx.x; e < H; e += kBlock) {
+    tkey[e] = -1;
+    tcnt[e] = 0;
+  }
+  __syncthreads();
+  if (key >= 0) {
+    int h = static_cast<int>((static_cast<unsigned>(key) * 2654435761u) >> (32 - LG));
+    for (;;) {
+      const int prev = atomicCAS(&tkey[h], -1, key);
+      if (prev == -1 || prev == key) break;
+      h = (h + 1) & (H - 1);
+    }
+    atomicAdd(&tcnt[h], 1);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < H; e += kBlock)
+    if (tcnt[e]) atomicAdd(&counts[tkey[e]], tcnt[e]);
+}
+
 __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                        const int* __restrict__ btype, const int* __restrict__ off,
                                                        const int* __restrict__ adj, const int* __restrict__ prev_colour,
                                                        int cap, IslandView iv, int* __restrict__ state,
-                                                       int* __restrict__ remaining) {
+                                                       int* __restrict__ remaining, int* __restrict__ isl_count,
+                                                       int* __restrict__ croot) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   bool kept = false, left_out = false;
   if (j < *ncp && iv.isl_awake) {
     const int2 b = bodies[j];
     left_out = iv.asleep(btype[b.x] != 0 ? b.x : b.y);
     if (left_out) state[j] = -2;
+  }
+  // each constraint's island (the island solve's bins are made from these and the counts below)
+  int root = -1;
+  if (j < *ncp && !left_out && isl_count) {
+    const int2 b = bodies[j];
+    root = iv.parent[btype[b.x] != 0 ? b.x : b.y];
+    croot[j] = root;
   }
   if (j < *ncp && !left_out) {
     const int pc = prev_colour[j];
@@ -443,6 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ n
   }
   const unsigned long long m = __ballot(kept || left_out);
   if (m && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1) atomicSub(remaining, __popcll(m));
+  if (isl_count) block_count_keys(root, isl_count);  // (uniform: every thread of the block calls it)
 }
 
 __global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ ncp, int round,
@@ -757,29 +799,21 @@ __device__ inline void apply_normal_pair(V2& va, float& wa, V2& vb, float& wb, f
   wb += iB * (cross(r1b, P1) + cross(r2b, P2));
 }
 
-__device__ inline void solve_velocity_one(int i, int block_solve, const VelArgs& a) {
-  int2 bd = a.bodies[i];
-  int4 info = a.info[i];
-  float4 m = a.vm[i];
-  float4 nf = a.vnf[i];
-  float4 lam = a.lam[i];
-  float4 ms = a.vms[i];
-  float4 r0 = a.vr0[i];
-  float4 vA = a.vel[bd.x], vB = a.vel[bd.y];
+// The constraint's records and the velocities (v.x, v.y, w, 0) of its two bodies in, its new
+// impulses and velocities out. kload(K, NM) supplies the block solver's K and normal mass; it is
+// called on that path only.
+template <class KLoad>
+__device__ inline void velocity_core(const float4 m, const float4 nf, float4& lam, const float4 ms, const float4 r0,
+                                     const float4 r1, const int vcount, const float2 bias, const int block_solve,
+                                     KLoad kload, float4& vA, float4& vB) {
   const float mA = m.x, iA = m.y, mB = m.z, iB = m.w;
   V2 va = v2(vA.x, vA.y), vb = v2(vB.x, vB.y);
   float wa = vA.z, wb = vB.z;
   V2 normal = v2(nf.x, nf.y);
   V2 tangent = cross(normal, 1.0f);
   const float friction = nf.z;
-  const int vcount = info.z;
   V2 r1a = v2(r0.x, r0.y), r1b = v2(r0.z, r0.w);
-  V2 r2a = v2(0.0f, 0.0f), r2b = v2(0.0f, 0.0f);
-  if (vcount == 2) {
-    float4 r1 = a.vr1[i];
-    r2a = v2(r1.x, r1.y);
-    r2b = v2(r1.z, r1.w);
-  }
+  V2 r2a = v2(r1.x, r1.y), r2b = v2(r1.z, r1.w);
   // tangent constraints
   for (int p = 0; p < vcount; ++p) {
     V2 ra = p == 0 ? r1a : r2a, rb = p == 0 ? r1b : r2b;
@@ -799,7 +833,6 @@ __device__ inline void solve_velocity_one(int i, int block_solve, const VelArgs&
     vb = vb + mB * P;
     wb += iB * cross(rb, P);
   }
-  float2 bias = a.vbias[i];
   if (vcount == 1 || !block_solve) {
     for (int p = 0; p < vcount; ++p) {
       V2 ra = p == 0 ? r1a : r2a, rb = p == 0 ? r1b : r2b;
@@ -819,7 +852,8 @@ __device__ inline void solve_velocity_one(int i, int block_solve, const VelArgs&
       wb += iB * cross(rb, P);
     }
   } else {
-    float4 K = a.K[i], NM = a.NM[i];  // (ex.x, ex.y, ey.x, ey.y)
+    float4 K, NM;  // (ex.x, ex.y, ey.x, ey.y)
+    kload(K, NM);
     V2 aa = v2(lam.x, lam.z);
     V2 dv1 = ((vb + cross(wb, r1b)) - va) - cross(wa, r1a);
     V2 dv2 = ((vb + cross(wb, r2b)) - va) - cross(wa, r2a);
@@ -871,9 +905,32 @@ __device__ inline void solve_velocity_one(int i, int block_solve, const VelArgs&
       break;  // no solution: give up
     }
   }
+  vA = make_float4(va.x, va.y, wa, 0.0f);
+  vB = make_float4(vb.x, vb.y, wb, 0.0f);
+}
+
+__device__ inline void solve_velocity_one(int i, int block_solve, const VelArgs& a) {
+  const int2 bd = a.bodies[i];
+  const int4 info = a.info[i];
+  const float4 m = a.vm[i];
+  const float4 nf = a.vnf[i];
+  float4 lam = a.lam[i];
+  const float4 ms = a.vms[i];
+  const float4 r0 = a.vr0[i];
+  float4 vA = a.vel[bd.x], vB = a.vel[bd.y];
+  const int vcount = info.z;
+  float4 r1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (vcount == 2) r1 = a.vr1[i];
+  const float2 bias = a.vbias[i];
+  velocity_core(m, nf, lam, ms, r0, r1, vcount, bias, block_solve,
+                [&](float4& K, float4& NM) {
+                  K = a.K[i];
+                  NM = a.NM[i];
+                },
+                vA, vB);
   a.lam[i] = lam;
-  if (info.x & 0x100) a.vel[bd.x] = make_float4(va.x, va.y, wa, 0.0f);
-  if (info.x & 0x200) a.vel[bd.y] = make_float4(vb.x, vb.y, wb, 0.0f);
+  if (info.x & 0x100) a.vel[bd.x] = vA;
+  if (info.x & 0x200) a.vel[bd.y] = vB;
 }
 __global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, int block_solve, VelArgs a) {
   int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -885,10 +942,12 @@ __global__ void __launch_bounds__(kBlock) k_solve_velocity(int begin, int end, i
 // is the same sequence as one launch per colour (the barrier makes a colour's velocity writes
 // visible to the workgroup's threads before the next colour reads them).
 static constexpr int kTailBlock = 1024, kTailMax = 512;
+// off holds ng ranges per colour; the tail solves the last one (ng 1: whole colours; with island
+// bins, the constraints of the islands too large for a bin)
 __global__ void __launch_bounds__(kTailBlock) k_solve_velocity_tail(int c0, int c1, const int* __restrict__ off,
-                                                                    int block_solve, VelArgs a) {
+                                                                    int ng, int block_solve, VelArgs a) {
   for (int c = c0; c < c1; ++c) {
-    const int b0 = off[c], b1 = off[c + 1];
+    const int b0 = off[c * ng + ng - 1], b1 = off[(c + 1) * ng];
     for (int i = b0 + threadIdx.x; i < b1; i += kTailBlock) solve_velocity_one(i, block_solve, a);
     __syncthreads();
   }
@@ -938,33 +997,17 @@ struct PosArgs {
   const float2* prad;
   float4* pos;
   int* violated;  // per island root: a contact of the island is below -3 linear slop this iteration
-  const int* done;
+  int* done;      // per island root: iterations run once it stopped (read by the per-colour kernels)
 };
 
-// B2contactSolver::solve_position_constraints for one constraint
-// (src/private/dynamics/b2_contact_solver_private.rs:600-730)
-// Called by every lane of a wave (the island reduction at the end uses the whole wave); lanes
-// past the colour's end pass live = false and stay inactive.
-__device__ inline void solve_position_one(int i, bool live, int begin, const PosArgs& a) {
-  // Inactive lanes stay in the wave (minSep = 0) for the island reduction at the end; they
-  // read the colour's first constraint and write nothing. All loads are issued up front: the
-  // constraint's records, then the body positions and the island flag (written by earlier
-  // launches only) side by side.
-  const int ii = live ? i : begin;
-  int4 info = a.info[ii];
-  const int2 bd = a.bodies[ii];
-  float4 m = a.vm[ii];
-  float4 lp = a.plp[ii], g = a.pgeo[ii], lc = a.plc[ii];
-  float2 rad = a.prad[ii];
-  float4 PA = a.pos[bd.x], PB = a.pos[bd.y];
-  const bool active = live && a.done[info.w] == 0;
-  if (!active) info.y = 0;
+// The constraint's position records and the positions (c.x, c.y, angle, 0) of its two bodies in,
+// the corrected positions out; returns the smallest separation seen (0 for count 0).
+__device__ inline float position_core(const int type, const int count, const float4 m, const float4 lp,
+                                      const float4 g, const float4 lc, const float2 rad, float4& PA, float4& PB) {
   const float mA = m.x, iA = m.y, mB = m.z, iB = m.w;
   V2 lcA = v2(lc.x, lc.y), lcB = v2(lc.z, lc.w);
   V2 cA = v2(PA.x, PA.y), cB = v2(PB.x, PB.y);
   float aA = PA.z, aB = PB.z;
-  const int type = info.x & 0xff;
-  const int count = info.y;
   V2 ln = v2(g.x, g.y), lpt = v2(g.z, g.w);
   float minSep = 0.0f;
   for (int p = 0; p < count; ++p) {
@@ -1009,8 +1052,29 @@ __device__ inline void solve_position_one(int i, bool live, int begin, const Pos
     cB = cB + mB * P;
     aB += iB * cross(rB, P);
   }
-  if (active && (info.x & 0x100)) a.pos[bd.x] = make_float4(cA.x, cA.y, aA, 0.0f);
-  if (active && (info.x & 0x200)) a.pos[bd.y] = make_float4(cB.x, cB.y, aB, 0.0f);
+  PA = make_float4(cA.x, cA.y, aA, 0.0f);
+  PB = make_float4(cB.x, cB.y, aB, 0.0f);
+  return minSep;
+}
+
+// B2contactSolver::solve_position_constraints for one constraint
+// (src/private/dynamics/b2_contact_solver_private.rs:600-730)
+// Lanes past the colour's end pass live = false: they read the colour's first constraint and
+// write nothing. All loads are issued up front: the constraint's records, then the body positions
+// and the island flag (written by earlier launches only) side by side.
+__device__ inline void solve_position_one(int i, bool live, int begin, const PosArgs& a) {
+  const int ii = live ? i : begin;
+  int4 info = a.info[ii];
+  const int2 bd = a.bodies[ii];
+  const float4 m = a.vm[ii];
+  const float4 lp = a.plp[ii], g = a.pgeo[ii], lc = a.plc[ii];
+  const float2 rad = a.prad[ii];
+  float4 PA = a.pos[bd.x], PB = a.pos[bd.y];
+  const bool active = live && a.done[info.w] == 0;
+  if (!active) info.y = 0;
+  const float minSep = position_core(info.x & 0xff, info.y, m, lp, g, lc, rad, PA, PB);
+  if (active && (info.x & 0x100)) a.pos[bd.x] = PA;
+  if (active && (info.x & 0x200)) a.pos[bd.y] = PB;
   // The island may stop iterating only if no contact's separation is below -3 linear slop
   // (minSeparation >= -3 * slop over the island's contacts, b2_contact_solver_private.rs); a
   // violating constraint marks its island with a plain store (every writer stores the same value).
@@ -1021,14 +1085,20 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
   solve_position_one(i, i < end, begin, a);
 }
 __global__ void __launch_bounds__(kTailBlock) k_solve_position_tail(int c0, int c1, const int* __restrict__ off,
-                                                                    PosArgs a) {
+                                                                    int ng, PosArgs a) {
   for (int c = c0; c < c1; ++c) {
-    const int b0 = off[c], b1 = off[c + 1];
+    const int b0 = off[c * ng + ng - 1], b1 = off[(c + 1) * ng];
     for (int i = b0 + threadIdx.x; i < b1; i += kTailBlock) solve_position_one(i, true, b0, a);
     __syncthreads();
   }
 }
 
+__device__ inline void iteration_end_root(int r, int iteration, int* __restrict__ violated, int* __restrict__ done,
+                                          int* __restrict__ active) {
+  if (!violated[r]) done[r] = iteration + 1;
+  else atomicAdd(active, 1);
+  violated[r] = 0;
+}
 // end of a position iteration: islands whose error is small stop iterating
 // (src/private/dynamics/b2_island_private.rs:257-273). A finished island's flag holds the number
 // of iterations it ran (the adjoint's tape reads it).
@@ -1037,9 +1107,7 @@ __device__ inline void iteration_end_one(int b, int iteration, const int* __rest
                                          int* __restrict__ done, int* __restrict__ active,
                                          const int* __restrict__ isl_awake) {
   if (btype[b] == 0 || parent[b] != b || done[b] || (isl_awake && !isl_awake[b])) return;
-  if (!violated[b]) done[b] = iteration + 1;
-  else atomicAdd(active, 1);
-  violated[b] = 0;
+  iteration_end_root(b, iteration, violated, done, active);
 }
 // the last position iteration's end and the copy back in one launch (both per body; finalize
 // reads only positions, the iteration end only island flags)
@@ -1059,13 +1127,18 @@ __global__ void __launch_bounds__(kBlock) k_finalize_end(int nb, const int* __re
   if (e.iteration >= 0) iteration_end_one(b, e.iteration, btype, e.parent, e.violated, e.done, e.active, e.isl_awake);
   finalize_one(b, btype, pos, bmass, xf, iv);
 }
+// binned (optional): islands of at most bin_cap constraints, whose iteration ends the island
+// solve's position kernel has run, are left out
 __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
                                                                    const int* __restrict__ parent,
                                                                    int* __restrict__ violated, int* __restrict__ done,
                                                                    int* __restrict__ active,
-                                                                   const int* __restrict__ isl_awake) {
+                                                                   const int* __restrict__ isl_awake,
+                                                                   const int* __restrict__ binned, int bin_cap) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) iteration_end_one(b, iteration, btype, parent, violated, done, active, isl_awake);
+  if (b >= nb) return;
+  if (binned && binned[b] > 0 && binned[b] <= bin_cap) return;
+  iteration_end_one(b, iteration, btype, parent, violated, done, active, isl_awake);
 }
 
 // ---- per-world solve (batched worlds) ---------------------------------------------------------
@@ -1221,6 +1294,346 @@ static void launch_worlds_position(World* w, long long items, const int* off, in
   }
 }
 
+// ---- island solve (island bins) ---------------------------------------------------------------
+// Islands share no dynamic body, so any interleaving of their constraints that keeps each
+// island's colour order is the same Gauss-Seidel sequence as the per-colour launches. Whole
+// islands of at most W = T/2 constraints are packed into bins: island r (its root) goes to bin
+// start(r) / W, where start is the exclusive scan of the small islands' counts in root order, so a
+// bin holds fewer than 2W = T constraints. One workgroup of T threads solves a bin through all
+// velocity (or position) iterations, colour after colour with a workgroup barrier between them;
+// each thread holds one constraint in registers and the bin's bodies sit in an LDS hash table
+// (body id -> slot), loaded once and written back once. Islands of more than W constraints keep
+// the per-colour launches (their constraints are the last group of every colour).
+static constexpr int kBinHashPerThread = 4;  // body slots per thread (a bin has < 2T bodies)
+
+// slot of body (or island) id `key` in a linear-probing table of H = 2^lg entries (-1 = empty);
+// the table never fills (at most H/2 keys), so the probe ends
+__device__ inline int bin_slot(int* keys, int lg, int key) {
+  const int mask = (1 << lg) - 1;
+  int h = static_cast<int>((static_cast<unsigned>(key) * 2654435761u) >> (32 - lg));
+  for (;;) {
+    const int prev = atomicCAS(&keys[h], -1, key);
+    if (prev == -1 || prev == key) return h;
+    h = (h + 1) & mask;
+  }
+}
+
+// this thread's constraint: the t-th of its bin in colour order (-1: none); rng holds the bin's
+// range per colour
+__device__ inline int bin_constraint(const int2* rng, int ncolors, int t, int* colour) {
+  int base = 0;
+  for (int c = 0; c < ncolors; ++c) {
+    const int2 r = rng[c];
+    const int n = r.y - r.x;
+    if (t >= base && t < base + n) {
+      *colour = c;
+      return r.x + t - base;
+    }
+    base += n;
+  }
+  *colour = -1;
+  return -1;
+}
+
+template <int T>
+__device__ inline void bin_ranges(const int* __restrict__ off, int ng, int ncolors, int2* rng) {
+  const int g = blockIdx.x;
+  for (int c = threadIdx.x; c < ncolors; c += T) rng[c] = make_int2(off[c * ng + g], off[c * ng + g + 1]);
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+template <int T>
+__global__ void __launch_bounds__(T) k_solve_bins_velocity(const int* __restrict__ off, int ng, int ncolors, int iters,
+                                                           int block_solve, VelArgs a) {
+  constexpr int H = kBinHashPerThread * T, LG = ilog2(H);
+  __shared__ int2 rng[kMaxColors];
+  __shared__ int keys[H];
+  __shared__ unsigned char dirty[H];
+  __shared__ float4 bv[H];
+  bin_ranges<T>(off, ng, ncolors, rng);
+  for (int e = threadIdx.x; e < H; e += T) {
+    keys[e] = -1;
+    dirty[e] = 0;
+  }
+  __syncthreads();
+  int mc;
+  const int i = bin_constraint(rng, ncolors, threadIdx.x, &mc);
+  int2 bd = make_int2(0, 0);
+  int4 info = make_int4(0, 0, 0, 0);
+  float4 m = {}, nf = {}, lam = {}, ms = {}, r0 = {}, r1 = {}, K = {}, NM = {};
+  float2 bias = {};
+  int sa = 0, sb = 0;
+  if (i >= 0) {
+    bd = a.bodies[i];
+    info = a.info[i];
+    m = a.vm[i];
+    nf = a.vnf[i];
+    lam = a.lam[i];
+    ms = a.vms[i];
+    r0 = a.vr0[i];
+    if (info.z == 2) r1 = a.vr1[i];
+    bias = a.vbias[i];
+    if (block_solve && info.z == 2) {
+      K = a.K[i];
+      NM = a.NM[i];
+    }
+    sa = bin_slot(keys, LG, bd.x);
+    sb = bin_slot(keys, LG, bd.y);
+    if (info.x & 0x100) dirty[sa] = 1;
+    if (info.x & 0x200) dirty[sb] = 1;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < H; e += T) {
+    const int k = keys[e];
+    if (k >= 0) bv[e] = a.vel[k];
+  }
+  __syncthreads();
+  for (int it = 0; it < iters; ++it)
+    for (int c = 0; c < ncolors; ++c) {
+      if (rng[c].x == rng[c].y) continue;  // uniform across the workgroup
+      if (mc == c) {
+        float4 vA = bv[sa], vB = bv[sb];
+        velocity_core(m, nf, lam, ms, r0, r1, info.z, bias, block_solve,
+                      [&](float4& k1, float4& k2) {
+                        k1 = K;
+                        k2 = NM;
+                      },
+                      vA, vB);
+        if (info.x & 0x100) bv[sa] = vA;
+        if (info.x & 0x200) bv[sb] = vB;
+      }
+      __syncthreads();
+    }
+  if (i >= 0) a.lam[i] = lam;
+  for (int e = threadIdx.x; e < H; e += T)
+    if (dirty[e]) a.vel[keys[e]] = bv[e];
+}
+
+// Position iterations with each island's early exit: the islands' flags (iterations run,
+// violated) sit in a second table keyed by island root; after every iteration but the last
+// (whose end runs with the copy back) each island of the bin ends its iteration as
+// k_position_iteration_end ends it. The flags are written back for that last end.
+template <int T>
+__global__ void __launch_bounds__(T) k_solve_bins_position(const int* __restrict__ off, int ng, int ncolors, int iters,
+                                                           PosArgs a) {
+  constexpr int H = kBinHashPerThread * T, LG = ilog2(H);
+  constexpr int HR = 2 * T, LGR = ilog2(HR);  // island table: fewer than T roots
+  __shared__ int2 rng[kMaxColors];
+  __shared__ int keys[H];
+  __shared__ unsigned char dirty[H];
+  __shared__ float4 bp[H];
+  __shared__ int rkeys[HR];
+  __shared__ int rdone[HR];
+  __shared__ unsigned char rviol[HR];
+  bin_ranges<T>(off, ng, ncolors, rng);
+  for (int e = threadIdx.x; e < H; e += T) {
+    keys[e] = -1;
+    dirty[e] = 0;
+  }
+  for (int e = threadIdx.x; e < HR; e += T) rkeys[e] = -1;
+  __syncthreads();
+  int mc;
+  const int i = bin_constraint(rng, ncolors, threadIdx.x, &mc);
+  int2 bd = make_int2(0, 0);
+  int4 info = make_int4(0, 0, 0, 0);
+  float4 m = {}, lp = {}, g = {}, lc = {};
+  float2 rad = {};
+  int sa = 0, sb = 0, sr = 0;
+  if (i >= 0) {
+    bd = a.bodies[i];
+    info = a.info[i];
+    m = a.vm[i];
+    lp = a.plp[i];
+    g = a.pgeo[i];
+    lc = a.plc[i];
+    rad = a.prad[i];
+    sa = bin_slot(keys, LG, bd.x);
+    sb = bin_slot(keys, LG, bd.y);
+    sr = bin_slot(rkeys, LGR, info.w);
+    if (info.x & 0x100) dirty[sa] = 1;
+    if (info.x & 0x200) dirty[sb] = 1;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < H; e += T) {
+    const int k = keys[e];
+    if (k >= 0) bp[e] = a.pos[k];
+  }
+  for (int e = threadIdx.x; e < HR; e += T) {
+    const int r = rkeys[e];
+    if (r >= 0) {
+      rdone[e] = a.done[r];
+      rviol[e] = a.violated[r] != 0;
+    }
+  }
+  __syncthreads();
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < ncolors; ++c) {
+      if (rng[c].x == rng[c].y) continue;
+      if (mc == c && rdone[sr] == 0) {
+        float4 PA = bp[sa], PB = bp[sb];
+        const float minSep = position_core(info.x & 0xff, info.y, m, lp, g, lc, rad, PA, PB);
+        if (info.x & 0x100) bp[sa] = PA;
+        if (info.x & 0x200) bp[sb] = PB;
+        if (!(minSep >= -3.0f * kLinearSlop)) rviol[sr] = 1;
+      }
+      __syncthreads();
+    }
+    if (it + 1 < iters) {
+      for (int e = threadIdx.x; e < HR; e += T)
+        if (rkeys[e] >= 0 && rdone[e] == 0) {
+          if (!rviol[e]) rdone[e] = it + 1;
+          rviol[e] = 0;
+        }
+      __syncthreads();
+    }
+  }
+  for (int e = threadIdx.x; e < H; e += T)
+    if (dirty[e]) a.pos[keys[e]] = bp[e];
+  for (int e = threadIdx.x; e < HR; e += T) {
+    const int r = rkeys[e];
+    if (r >= 0) {
+      a.done[r] = rdone[e];
+      a.violated[r] = rviol[e];
+    }
+  }
+}
+
+// Small islands' counts (at most cap constraints) for the bin scan; larger islands' constraints
+// are counted into *big.
+__global__ void __launch_bounds__(kBlock) k_island_small(int nb, const int* __restrict__ parent,
+                                                         const int* __restrict__ isl_count, int cap,
+                                                         int* __restrict__ small, int* __restrict__ big) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = 0;
+  if (b < nb && parent[b] == b) n = isl_count[b];
+  if (b < nb) small[b] = n <= cap ? n : 0;
+  if (n > cap) atomicAdd(big, n);
+}
+
+// (colour, bin) histogram and scatter of the constraints. A small island's constraints go to its
+// bin (isl_start / cap); a large island's to the last group of their colour, whose few counters
+// would see one global atomic per constraint: those are counted and ranked in LDS first and take
+// one global atomic per colour and block, as k_color_hist and k_color_scatter do.
+__global__ void __launch_bounds__(kBlock) k_color_bin_hist(int n, const int* __restrict__ state,
+                                                           const int* __restrict__ launch, const int* __restrict__ croot,
+                                                           const int* __restrict__ isl_count,
+                                                           const int* __restrict__ isl_start, int cap, int nbins,
+                                                           int* __restrict__ count) {
+  __shared__ int hb[kMaxColors];
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) hb[c] = 0;
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ng = nbins + 1;
+  if (j < n && state[j] >= 0) {
+    const int L = launch[state[j] & 0xff], r = croot[j];
+    if (isl_count[r] > cap) atomicAdd(&hb[L], 1);
+    else atomicAdd(&count[L * ng + isl_start[r] / cap], 1);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
+    if (hb[c]) atomicAdd(&count[c * ng + nbins], hb[c]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_bin_scatter(int n, const int* __restrict__ state,
+                                                              const int* __restrict__ launch,
+                                                              const int* __restrict__ croot,
+                                                              const int* __restrict__ isl_count,
+                                                              const int* __restrict__ isl_start, int cap, int nbins,
+                                                              const int* __restrict__ off, int* __restrict__ cursor,
+                                                              const int* __restrict__ list, int* __restrict__ order,
+                                                              int* __restrict__ inv_order, int* __restrict__ pair_colour) {
+  __shared__ int hb[kMaxColors];
+  __shared__ int base[kMaxColors];
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) hb[c] = 0;
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ng = nbins + 1;
+  int L = -1, pos = 0, rank = -1;
+  if (j < n && state[j] >= 0) {
+    L = launch[state[j] & 0xff];
+    const int r = croot[j];
+    if (isl_count[r] > cap) {
+      rank = atomicAdd(&hb[L], 1);
+    } else {
+      const int g = L * ng + isl_start[r] / cap;
+      pos = off[g] + atomicAdd(&cursor[g], 1);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
+    if (hb[c]) base[c] = off[c * ng + nbins] + atomicAdd(&cursor[c * ng + nbins], hb[c]);
+  __syncthreads();
+  if (L >= 0) {
+    if (rank >= 0) pos = base[L] + rank;
+    order[pos] = j;
+    inv_order[j] = pos;
+    pair_colour[list[j]] = L;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_clear_two(int n, int* __restrict__ a, int* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
+// per-colour launches of the large islands' constraints (the last group of each colour); the
+// range is read from the device, the grid is sized for the whole colour
+__global__ void __launch_bounds__(kBlock) k_solve_velocity_group(int c, const int* __restrict__ off, int ng,
+                                                                 int block_solve, VelArgs a) {
+  const int i = off[c * ng + ng - 1] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < off[(c + 1) * ng]) solve_velocity_one(i, block_solve, a);
+}
+__global__ void __launch_bounds__(kBlock) k_solve_position_group(int c, const int* __restrict__ off, int ng,
+                                                                 PosArgs a) {
+  const int b0 = off[c * ng + ng - 1], b1 = off[(c + 1) * ng];
+  if (b0 >= b1) return;
+  const int i = b0 + blockIdx.x * blockDim.x + threadIdx.x;
+  solve_position_one(i, i < b1, b0, a);
+}
+
+static int bin_threads_ok(int t) { return t == 64 || t == 128 || t == 256 || t == 512; }
+
+static void launch_bins_velocity(World* w, long long items, int nbins, const int* off, int ncolors, int iters,
+                                 int block_solve, const VelArgs& a) {
+  const int ng = nbins + 1, K = B2G_KERNEL_SOLVE_VELOCITY;
+  switch (w->bin_threads) {
+    case 64:
+      launch_timed(w, K, items, k_solve_bins_velocity<64>, dim3(nbins), dim3(64), off, ng, ncolors, iters, block_solve, a);
+      break;
+    case 128:
+      launch_timed(w, K, items, k_solve_bins_velocity<128>, dim3(nbins), dim3(128), off, ng, ncolors, iters, block_solve, a);
+      break;
+    case 512:
+      launch_timed(w, K, items, k_solve_bins_velocity<512>, dim3(nbins), dim3(512), off, ng, ncolors, iters, block_solve, a);
+      break;
+    default:
+      launch_timed(w, K, items, k_solve_bins_velocity<256>, dim3(nbins), dim3(256), off, ng, ncolors, iters, block_solve, a);
+      break;
+  }
+}
+static void launch_bins_position(World* w, long long items, int nbins, const int* off, int ncolors, int iters,
+                                 const PosArgs& a) {
+  const int ng = nbins + 1, K = B2G_KERNEL_SOLVE_POSITION;
+  switch (w->bin_threads) {
+    case 64:
+      launch_timed(w, K, items, k_solve_bins_position<64>, dim3(nbins), dim3(64), off, ng, ncolors, iters, a);
+      break;
+    case 128:
+      launch_timed(w, K, items, k_solve_bins_position<128>, dim3(nbins), dim3(128), off, ng, ncolors, iters, a);
+      break;
+    case 512:
+      launch_timed(w, K, items, k_solve_bins_position<512>, dim3(nbins), dim3(512), off, ng, ncolors, iters, a);
+      break;
+    default:
+      launch_timed(w, K, items, k_solve_bins_position<256>, dim3(nbins), dim3(256), off, ng, ncolors, iters, a);
+      break;
+  }
+}
+
 // ---- graph-mode variants: colour ranges and counts read from device memory -----------------
 // A captured graph replays with new colour offsets (copied before each launch) and a fixed
 // grid; the loops stride over whatever the colour holds this step.
@@ -1276,7 +1689,7 @@ static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& v
     for (int col = 0; col < k.ncolors; ++col)
       hipLaunchKernelGGL(k_solve_position_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, pa);
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, it,
-                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, nullptr);
+                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, nullptr, nullptr, 0);
   }
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p, w->b.pos.p,
                      w->b.mass.p, w->b.xf.p, all_awake);
@@ -1365,10 +1778,19 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     return st;
   const IntegrateArgs ia{w->b.type.p, w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p,
                          w->gravity.x, w->gravity.y, dt};
-  // per body: island roots, degrees, adjacency cursors and (no sleeping possible) the velocity
-  // integration
+  // batched worlds: each world's iterations in one workgroup (k_solve_worlds_*); otherwise small
+  // islands packed into bins (k_solve_bins_*); neither with joints
+  const bool want_world = w->nj == 0 && w->worlds_contiguous && w->n_worlds > 1 &&
+                          (w->world_solve == 1 || (w->world_solve == -1 && w->n_worlds >= kWorldSolveMin));
+  const bool want_bins = w->nj == 0 && (w->island_solve == 1 || (w->island_solve == -1 && !want_world));
+  const int bin_cap = w->bin_threads / 2;  // islands of at most bin_cap constraints are binned
+  if (want_bins && ((st = s.isl_count.reserve(nb)) || (st = s.isl_small.reserve(nb)) ||
+                    (st = s.isl_start.reserve(nb)) || (st = s.croot.reserve(std::max(n_pairs, 1)))))
+    return st;
+  // per body: island roots, degrees, adjacency cursors, island counts and (no sleeping possible)
+  // the velocity integration
   hipLaunchKernelGGL(k_solver_body_init, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
-                     s.deg.p, s.adj_cursor.p, int(!sleepy), ia);
+                     s.deg.p, s.adj_cursor.p, int(!sleepy), ia, want_bins ? s.isl_count.p : nullptr);
   if ((st = exclusive_scan(w, s.touch_flag.p, s.touch_scan.p, n_pairs, cnt + 0))) return st;
   if (n_pairs == 0)
     hipLaunchKernelGGL(k_solver_counters_init, dim3(1), dim3(kBlock), 0, w->stream, cnt, s.color_cursor.p);
@@ -1408,7 +1830,15 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // others, a first batch without waiting (rounds after the last constraint is coloured find
     // nothing to do), then more only if some remain
     hipLaunchKernelGGL(k_color_keep, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
-                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, iv, s.color_state.p, cnt + 1);
+                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, iv, s.color_state.p, cnt + 1,
+                       want_bins ? s.isl_count.p : nullptr, s.croot.p);
+    if (want_bins) {
+      // the small islands' counts and their scan (bin of island r: isl_start[r] / bin_cap); the
+      // large islands' constraint count comes back with the colouring's counters
+      hipLaunchKernelGGL(k_island_small, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, s.parent.p,
+                         s.isl_count.p, bin_cap, s.isl_small.p, cnt + kBigIslandSlot);
+      if ((st = exclusive_scan(w, s.isl_small.p, s.isl_start.p, nb, nullptr))) return st;
+    }
     // first batch: from scratch the configured count; with reuse one more than the rounds the
     // last reusing step needed (the batch size does not change the colouring)
     int round = 1;
@@ -1421,7 +1851,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       // the histogram was cleared with the counters; a later batch recounts from zero
       if (round > batch + 1) B2G_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(int) * (kMaxColors + 1), w->stream));
       hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, cnt + 4);
-      B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, (4 + kMaxColors + 1) * sizeof(int), hipMemcpyDeviceToHost,
+      B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, (kBigIslandSlot + 1) * sizeof(int), hipMemcpyDeviceToHost,
                              w->stream));
       B2G_HIP(hipStreamSynchronize(w->stream));
       if ((st = kt_harvest(w))) return st;
@@ -1442,9 +1872,14 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
   }
   w->n_touch = nc;
-  // batched worlds: each world's iterations in one workgroup (k_solve_worlds_*)
-  const bool per_world = nc > 0 && w->nj == 0 && w->worlds_contiguous && w->n_worlds > 1 &&
-                         (w->world_solve == 1 || (w->world_solve == -1 && w->n_worlds >= kWorldSolveMin));
+  const int n_big = want_bins && nc > 0 ? w->h_counters[kBigIslandSlot] : 0;
+  const int n_small = want_bins ? nc - n_big : 0;
+  // auto: bins only where they take at least half of the constraints (the per-colour launches the
+  // large islands need take the few small ones at no extra launch)
+  const bool bins = n_small > 0 && (w->island_solve == 1 || n_small >= n_big);
+  const int nbins = bins ? (n_small + bin_cap - 1) / bin_cap : 0;
+  w->n_bins = nbins;
+  const bool per_world = nc > 0 && want_world && !want_bins;
   if (nc > 0) {
     // One launch per non-empty colour, in colour-id order (kept colours can leave gaps).
     // Device copy: launch offsets (kMaxColors + 1, padded with nc), then per colour id its base
@@ -1464,7 +1899,22 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     }
     for (int i = 0; i <= kMaxColors; ++i) offs[i] = i <= ncolors ? w->h_color_off[i] : nc;
     B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
-    if (per_world) {
+    if (bins) {
+      // constraints grouped by bin inside each colour, the large islands' last (see k_color_bin_hist)
+      const int ng = nbins + 1, ncw = ncolors * ng;
+      if ((st = s.cw_count.reserve(ncw)) || (st = s.cw_cursor.reserve(ncw)) || (st = s.cw_off.reserve(ncw + 1)))
+        return st;
+      hipLaunchKernelGGL(k_clear_two, dim3(blocks_for(ncw, kBlock)), dim3(kBlock), 0, w->stream, ncw, s.cw_count.p,
+                         s.cw_cursor.p);
+      const int* launch = s.color_off.p + kMaxColors + 1 + kMaxColors;
+      hipLaunchKernelGGL(k_color_bin_hist, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream,
+                         n_touching, s.color_state.p, launch, s.croot.p, s.isl_count.p, s.isl_start.p, bin_cap, nbins,
+                         s.cw_count.p);
+      if ((st = exclusive_scan(w, s.cw_count.p, s.cw_off.p, ncw, s.cw_off.p + ncw))) return st;
+      hipLaunchKernelGGL(k_color_bin_scatter, dim3(blocks_for(n_touching, kBlock)), dim3(kBlock), 0, w->stream,
+                         n_touching, s.color_state.p, launch, s.croot.p, s.isl_count.p, s.isl_start.p, bin_cap, nbins,
+                         s.cw_off.p, s.cw_cursor.p, s.touch_list.p, s.order.p, s.inv_order.p, c.colour.p);
+    } else if (per_world) {
       // constraints grouped by world inside each colour (see k_color_world_hist)
       const int nw = w->n_worlds, ncw = ncolors * nw;
       if ((st = s.cw_count.reserve(ncw)) || (st = s.cw_cursor.reserve(ncw)) || (st = s.cw_off.reserve(ncw + 1)))
@@ -1529,7 +1979,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   // (graph nodes cannot carry the timing events of a timed solve kernel)
   const bool timed_tail = w->kt.kernel == B2G_KERNEL_WARM_START || w->kt.kernel == B2G_KERNEL_SOLVE_VELOCITY ||
                           w->kt.kernel == B2G_KERNEL_SOLVE_POSITION;
-  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0 && !per_world) {
+  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0 && !per_world && !bins) {
     // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
     int maxc = 0;
     for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);
@@ -1579,6 +2029,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       launch_worlds_position(w, static_cast<long long>(nc) * pos_iters, s.cw_off.p, ncolors, pos_iters, pa, ends);
     w->pos_iters_run = pos_iters;
   } else {
+  // Per-colour launches: of all constraints, or with island bins of the large islands' (the last
+  // of the ng groups of every colour), the bins' kernels solving the rest; the two sets share no
+  // body, so they may run in either order.
+  const int ng = bins ? nbins + 1 : 1;
+  const int* goff = bins ? s.cw_off.p : s.color_off.p;
+  const bool per_colour = !bins || n_big > 0;
   // colours [tail, ncolors): the small last colours, one launch per iteration (when at least two);
   // co holds ncolors + 1 offsets only when there are constraints
   int tail = ncolors;
@@ -1587,35 +2043,52 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     if (ncolors - tail < 2) tail = ncolors;
   }
   const int tail_items = nc > 0 ? co[ncolors] - co[tail] : 0;
-  for (int it = 0; it < vel_iters && (nc > 0 || w->nj > 0); ++it) {
+  if (bins && vel_iters > 0)
+    launch_bins_velocity(w, static_cast<long long>(n_small) * vel_iters, nbins, s.cw_off.p, ncolors, vel_iters,
+                         int(w->block_solve), va);
+  for (int it = 0; it < vel_iters && per_colour && (nc > 0 || w->nj > 0); ++it) {
     if (w->nj > 0) launch_joint_colours(w, 1, ja);
     for (int col = 0; col < tail; ++col) {
       int b0 = co[col], b1 = co[col + 1];
-      launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)),
-                   dim3(kBlock), b0, b1, int(w->block_solve), va);
+      if (bins)
+        launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity_group, dim3(blocks_for(b1 - b0, kBlock)),
+                     dim3(kBlock), col, goff, ng, int(w->block_solve), va);
+      else
+        launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)),
+                     dim3(kBlock), b0, b1, int(w->block_solve), va);
     }
     if (tail < ncolors)
       launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, tail_items, k_solve_velocity_tail, dim3(1), dim3(kTailBlock), tail,
-                   ncolors, s.color_off.p, int(w->block_solve), va);
+                   ncolors, goff, ng, int(w->block_solve), va);
   }
   hipLaunchKernelGGL(k_store_and_integrate, dim3(blocks_for(std::max(nc, nb), kBlock)), dim3(kBlock), 0, w->stream, nc,
                      s.k_contact.p, s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p, nb, w->b.type.p, w->b.pos.p, w->b.vel.p, dt,
                      iv);
+  if (bins && pos_iters > 0)
+    launch_bins_position(w, static_cast<long long>(n_small) * pos_iters, nbins, s.cw_off.p, ncolors, pos_iters, pa);
   w->pos_iters_run = 0;
   for (int it = 0; it < pos_iters; ++it) {
-    for (int col = 0; col < tail; ++col) {
-      int b0 = co[col], b1 = co[col + 1];
-      launch_timed(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0, k_solve_position, dim3(blocks_for(b1 - b0, kBlock)),
-                   dim3(kBlock), b0, b1, pa);
+    if (per_colour) {
+      for (int col = 0; col < tail; ++col) {
+        int b0 = co[col], b1 = co[col + 1];
+        if (bins)
+          launch_timed(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0, k_solve_position_group,
+                       dim3(blocks_for(b1 - b0, kBlock)), dim3(kBlock), col, goff, ng, pa);
+        else
+          launch_timed(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0, k_solve_position, dim3(blocks_for(b1 - b0, kBlock)),
+                       dim3(kBlock), b0, b1, pa);
+      }
+      if (tail < ncolors)
+        launch_timed(w, B2G_KERNEL_SOLVE_POSITION, tail_items, k_solve_position_tail, dim3(1), dim3(kTailBlock), tail,
+                     ncolors, goff, ng, pa);
     }
-    if (tail < ncolors)
-      launch_timed(w, B2G_KERNEL_SOLVE_POSITION, tail_items, k_solve_position_tail, dim3(1), dim3(kTailBlock), tail,
-                   ncolors, s.color_off.p, pa);
     if (w->nj > 0) launch_joint_colours(w, 2, ja);
-    // (the last iteration's end runs with the copy back below)
-    if (it + 1 < pos_iters)
+    // (the last iteration's end runs with the copy back below; the binned islands' ends run in
+    // their kernel)
+    if (it + 1 < pos_iters && per_colour)
       hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
-                         w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake);
+                         w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake,
+                         bins ? s.isl_count.p : nullptr, bin_cap);
     ++w->pos_iters_run;
   }
   }

@@ -36,6 +36,8 @@ pub const B2G_TUNE_WORLD_SOLVE: i32 = 5;
 pub const B2G_TUNE_WORLD_THREADS_SMALL: i32 = 6;
 pub const B2G_TUNE_WORLD_THREADS_MEDIUM: i32 = 7;
 pub const B2G_TUNE_WORLD_THREADS_LARGE: i32 = 8;
+pub const B2G_TUNE_ISLAND_SOLVE: i32 = 9;
+pub const B2G_TUNE_ISLAND_THREADS: i32 = 10;
 pub const B2G_KERNEL_NONE: i32 = -1;
 pub const B2G_KERNEL_SOLVE_VELOCITY: i32 = 0;
 pub const B2G_KERNEL_SOLVE_POSITION: i32 = 1;
@@ -86,7 +88,7 @@ pub struct b2g_manifold {
 pub struct b2g_profile {
     pub step: f32, pub collide: f32, pub solve: f32, pub broadphase: f32,
     pub pairs: i32, pub touching: i32, pub colours: i32, pub islands: i32, pub position_iterations: i32,
-    pub toi_events: i32, pub pad: [i32; 2],
+    pub toi_events: i32, pub island_bins: i32, pub pad: i32,
 }
 #[repr(C)] #[derive(Clone, Copy, Default)]
 pub struct b2g_solver_state { pub inv_dt0: f32, pub n_colours: i32, pub scratch_colours: i32, pub steps_since_scratch: i32 }

@@ -127,7 +127,8 @@ typedef struct b2g_profile {
   float step, collide, solve, broadphase; /* milliseconds, device time */
   int32_t pairs, touching, colours, islands, position_iterations;
   int32_t toi_events; /* continuous collision: TOI events solved in the step */
-  int32_t pad[2];
+  int32_t island_bins; /* island solve: bins of the last solve (0: per-colour or per-world launches) */
+  int32_t pad;
 } b2g_profile;
 
 typedef struct b2g_world b2g_world;
@@ -363,7 +364,17 @@ enum {
  *   B2G_TUNE_WORLD_THREADS_SMALL / _MEDIUM / _LARGE
  *                           threads per world (64, 128, 256 or 512) in that solve for worlds of
  *                           fewer than 64 bodies / 64-255 bodies / 256 or more (defaults 64, 128,
- *                           256; box2d-rs_amd/b2g/autotune.py sweeps them). */
+ *                           256; box2d-rs_amd/b2g/autotune.py sweeps them).
+ *   B2G_TUNE_ISLAND_SOLVE   1: islands of at most B2G_TUNE_ISLAND_THREADS / 2 constraints are packed
+ *                           into bins, and one workgroup per bin runs all velocity (and all position)
+ *                           iterations of its islands, colour after colour with workgroup barriers,
+ *                           its constraints in registers and its bodies in LDS (two launches per step
+ *                           instead of one per colour and iteration); larger islands keep the
+ *                           per-colour launches. 0: per-colour launches only. -1 (default): bins
+ *                           in steps whose small islands hold at least half of the constraints, unless
+ *                           the per-world solve is on. Islands share no dynamic body, so the results
+ *                           are the same bit for bit.
+ *   B2G_TUNE_ISLAND_THREADS threads per bin of that solve: 64, 128, 256 (default) or 512. */
 #define B2G_TUNE_COLOR_ROUNDS 2
 #define B2G_TUNE_SOLVER_GRAPH 3
 #define B2G_TUNE_COLOR_REUSE 4
@@ -371,6 +382,8 @@ enum {
 #define B2G_TUNE_WORLD_THREADS_SMALL 6
 #define B2G_TUNE_WORLD_THREADS_MEDIUM 7
 #define B2G_TUNE_WORLD_THREADS_LARGE 8
+#define B2G_TUNE_ISLAND_SOLVE 9
+#define B2G_TUNE_ISLAND_THREADS 10
 int b2g_world_tune(b2g_world* world, int32_t param, int32_t value);
 
 /* Measurement hook with no counterpart in the reference (its B2Profile times whole phases on

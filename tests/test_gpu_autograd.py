@@ -1,7 +1,8 @@
 """The torch.autograd.Function wrapper of a taped rollout (b2g.autograd.Rollout).
 
 Forward: the final states equal the world's own states after the same steps; backward: the
-gradients equal b2g_tape_backward's for the same cotangent, and agree with central differences
+gradients equal b2g_tape_backward's for the same cotangent (to rounding: the reverse pass sums
+adjoints with float atomics), and agree with central differences
 (h = 1e-3, f64) of the host frozen model on the recorded structure within 1e-3 of each world's
 gradient scale, on a 30-step rollout of 8 worlds of 16 bodies (parameters where the central
 differences at h and h / 10 disagree, a clamp switching inside the step, and worlds whose f64
@@ -40,8 +41,12 @@ def test_rollout_function_forward_backward(oracle):
     cot = np.zeros((len(s0), 6), np.float32)
     cot[batch.meta["body0"], 0:2] = 2.0 * (final.detach().numpy()[batch.meta["body0"], 0:2] - batch.meta["goal"])
     g, gf = w.tape_backward(cot)
-    np.testing.assert_allclose(state0.grad.numpy(), g, rtol=1e-6, atol=1e-6)
-    np.testing.assert_allclose(fric.grad.numpy(), gf, rtol=1e-6, atol=1e-6)
+    # Two backward passes over one tape: the reverse pass sums per-body and per-fixture adjoints
+    # with float atomics, whose order varies between launches, so they agree to rounding (a few
+    # ulps of the largest gradient), not bit for bit.
+    for got, ref in ((state0.grad.numpy(), g), (fric.grad.numpy(), gf)):
+        scale = max(1e-6, float(np.abs(ref).max()))
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * scale)
     # against f64 central differences of the frozen model
     struct = tape_structure(w)
     rng = np.random.default_rng(7)

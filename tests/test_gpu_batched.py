@@ -65,12 +65,16 @@ def test_world_solve_equals_colour_launches(batch, sleep):
     and the per-colour launches solve the same constraints in the same colour order: states and
     manifolds (impulses included) agree bit for bit after 90 steps, for every threads-per-world
     setting, with sleeping off and on."""
-    ref = _batched_run(batch, 90, sleep, TUNE_WORLD_SOLVE=0)
+    ref = _batched_run(batch, 90, sleep, TUNE_WORLD_SOLVE=0, TUNE_ISLAND_SOLVE=0)
     for threads in (64, 128, 256, 512):
         got = _batched_run(batch, 90, sleep, TUNE_WORLD_SOLVE=1, TUNE_WORLD_THREADS_SMALL=threads,
                            TUNE_WORLD_THREADS_MEDIUM=threads, TUNE_WORLD_THREADS_LARGE=threads)
         assert got[0] == ref[0], "states differ with %d threads per world" % threads
         assert got[1] == ref[1], "manifolds differ with %d threads per world" % threads
+    # the island solve (bins of whole islands, which never cross worlds) gives the same bits
+    for threads in (64, 256):
+        assert _batched_run(batch, 90, sleep, TUNE_WORLD_SOLVE=0, TUNE_ISLAND_SOLVE=1,
+                            TUNE_ISLAND_THREADS=threads) == ref, "island solve, %d threads per bin" % threads
 
 
 def test_world_solve_is_the_default_for_batches(oracle, batch):

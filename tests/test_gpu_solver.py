@@ -278,6 +278,7 @@ def test_solver_graph_matches_direct_launches():
     runs = []
     for graph in (0, 1):
         w = _world(sc)
+        w.tune(b2g.TUNE_ISLAND_SOLVE, 0)  # graph replay is a mode of the per-colour launches
         w.tune(b2g.TUNE_SOLVER_GRAPH, graph)
         colours = set()
         for _ in range(80):
