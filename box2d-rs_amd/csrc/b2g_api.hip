@@ -583,6 +583,9 @@ int b2g_world_destroy(b2g_world* h) {
   }
   rel(w->toi_alpha0); rel(w->toi_sel); rel(w->toi_two); rel(w->toi_ctl);
   rel(w->toi_cand); rel(w->toi_need); rel(w->toi_rank); rel(w->toi_snap);
+  rel(w->toi_cadj_off); rel(w->toi_cadj); rel(w->toi_cadj_cur); rel(w->toi_bfix_off); rel(w->toi_bfix);
+  rel(w->toi_bfix_cur); rel(w->toi_cl); rel(w->toi_in_list);
+  w->toi_bfix_np = -1;
   rel(w->toi_succ); rel(w->toi_drop_flag); rel(w->toi_drop_pos); rel(w->toi_drop_total);
   {
     ContactArrays& d = w->toi_dropped;

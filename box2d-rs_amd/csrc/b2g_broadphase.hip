@@ -36,37 +36,7 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
   int b = fbody[i];
   // only bodies of this step's islands are synchronised (src/private/dynamics/b2_world.rs:482-500)
   if (btype[b] == 0 || iv.asleep(b)) { moved[i] = 0; return; }
-  const ShapeV& s = shapes[i];
-  float4 p0 = pos0[b];
-  float4 ms = bmass[b];
-  Xf xf1;
-  xf1.q = rot(p0.z);
-  xf1.p = v2(p0.x, p0.y) - mul(xf1.q, v2(ms.z, ms.w));
-  Xf xf2 = xf_of(bxf[b]);
-  Aabb a1 = shape_aabb(s, xf1);
-  Aabb a2 = shape_aabb(s, xf2);
-  Aabb a = combine(a1, a2);
-  faabb[i] = make_float4(a.lo.x, a.lo.y, a.hi.x, a.hi.y);
-  V2 disp = center(a2) - center(a1);
-  // B2dynamicTree::move_proxy
-  const V2 r = v2(kAabbExtension, kAabbExtension);
-  Aabb fat;
-  fat.lo = a.lo - r;
-  fat.hi = a.hi + r;
-  V2 d = kAabbMultiplier * disp;
-  if (d.x < 0.0f) fat.lo.x += d.x; else fat.hi.x += d.x;
-  if (d.y < 0.0f) fat.lo.y += d.y; else fat.hi.y += d.y;
-  float4 t = ffat[i];
-  Aabb tree;
-  tree.lo = v2(t.x, t.y);
-  tree.hi = v2(t.z, t.w);
-  if (contains(tree, a)) {
-    Aabb huge;
-    huge.lo = fat.lo - 4.0f * r;
-    huge.hi = fat.hi + 4.0f * r;
-    if (contains(huge, tree)) { moved[i] = 0; return; }
-  }
-  ffat[i] = make_float4(fat.lo.x, fat.lo.y, fat.hi.x, fat.hi.y);
+  if (!sync_proxy(i, b, shapes[i], pos0, bmass, bxf, faabb, ffat)) { moved[i] = 0; return; }
   moved[i] = 1;
   if (any_moved) *any_moved = 1;  // every writer stores 1
 }

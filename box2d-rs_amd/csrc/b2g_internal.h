@@ -310,6 +310,12 @@ struct World {
   DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies, counters
   DevArray<int> toi_cand, toi_need, toi_rank;  // TOI event candidates; contacts to recompute, ranked
   DevArray<float> toi_snap;                    // aligned sweeps of the contacts to recompute
+  // device event loop: contacts per non-static body of the current set, fixtures per body (built
+  // when the fixture count changes), the event candidates (contacts with a cached time of impact
+  // below 1) and their membership flags
+  DevArray<int> toi_cadj_off, toi_cadj, toi_cadj_cur, toi_bfix_off, toi_bfix, toi_bfix_cur;
+  DevArray<int> toi_cl, toi_in_list;
+  int toi_bfix_np = -1;
   int toi_events = 0;                 // TOI events solved in the last step
   int cur = 0;         // index of the current contact set in c[]
   int n_pairs = 0;     // pairs in the current set
@@ -476,6 +482,46 @@ __device__ inline Xf xf_of(float4 t) {
   x.q.s = t.z;
   x.q.c = t.w;
   return x;
+}
+
+// B2fixture::synchronize + B2dynamicTree::move_proxy for proxy i of body b
+// (src/private/dynamics/b2_fixture.rs:147-173, src/private/collision/b2_dynamic_tree.rs:109-168):
+// the swept AABB from the sweep's start (pos0) to the current transform, and the fat AABB
+// re-fattened (with the displacement prediction) when it no longer contains the swept one or has
+// grown too large; returns whether it was re-fattened.
+__device__ inline bool sync_proxy(int i, int b, const ShapeV& s, const float4* __restrict__ pos0,
+                                  const float4* __restrict__ bmass, const float4* __restrict__ bxf,
+                                  float4* __restrict__ faabb, float4* __restrict__ ffat) {
+  const float4 p0 = pos0[b];
+  const float4 ms = bmass[b];
+  Xf xf1;
+  xf1.q = rot(p0.z);
+  xf1.p = v2(p0.x, p0.y) - mul(xf1.q, v2(ms.z, ms.w));
+  const Xf xf2 = xf_of(bxf[b]);
+  const Aabb a1 = shape_aabb(s, xf1);
+  const Aabb a2 = shape_aabb(s, xf2);
+  const Aabb a = combine(a1, a2);
+  faabb[i] = make_float4(a.lo.x, a.lo.y, a.hi.x, a.hi.y);
+  const V2 disp = center(a2) - center(a1);
+  const V2 r = v2(kAabbExtension, kAabbExtension);
+  Aabb fat;
+  fat.lo = a.lo - r;
+  fat.hi = a.hi + r;
+  const V2 d = kAabbMultiplier * disp;
+  if (d.x < 0.0f) fat.lo.x += d.x; else fat.hi.x += d.x;
+  if (d.y < 0.0f) fat.lo.y += d.y; else fat.hi.y += d.y;
+  const float4 t = ffat[i];
+  Aabb tree;
+  tree.lo = v2(t.x, t.y);
+  tree.hi = v2(t.z, t.w);
+  if (contains(tree, a)) {
+    Aabb huge;
+    huge.lo = fat.lo - 4.0f * r;
+    huge.hi = fat.hi + 4.0f * r;
+    if (contains(huge, tree)) return false;
+  }
+  ffat[i] = make_float4(fat.lo.x, fat.lo.y, fat.hi.x, fat.hi.y);
+  return true;
 }
 
 }  // namespace b2g

@@ -433,7 +433,7 @@ struct ToiArgs {
   int* need;                // contacts whose time of impact is recomputed (cap n)
   int* rank;                // their set-order rank
   float* snap;              // per need entry: both aligned sweeps and alpha0 (19 floats)
-  const int* moved;         // per proxy: re-fattened by the last synchronisation
+  int* moved;               // per proxy: re-fattened by the last synchronisation
   int np;
   int block_solve;
   int velocity_iterations;
@@ -531,10 +531,10 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_need(ToiArgs a, int wait_rebu
   if (toi_candidate(a, k, &bA, &bB)) a.need[atomicAdd(&a.flags4[1], 1)] = k;
 }
 
-__global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) {
+__device__ void toi_rescan_wg(const ToiArgs& a) {
   const int t = threadIdx.x;
   const int m = a.flags4[1];
-  for (int i = t; i < m; i += kRescanBlock) {
+  for (int i = t; i < m; i += blockDim.x) {
     const int x = a.need[i];
     int r = 0;
     for (int j = 0; j < m; ++j) r += a.need[j] < x ? 1 : 0;
@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) {
     }
   }
   __syncthreads();
-  for (int i = t; i < m; i += kRescanBlock) {
+  for (int i = t; i < m; i += blockDim.x) {
     const int k = a.rank[i];
     const float* q = a.snap + 19 * i;
     SweepD sw[2];
@@ -592,6 +592,7 @@ __global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) {
     a.flags4[3] = 0;
   }
 }
+__global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) { toi_rescan_wg(a); }
 
 // The contacts of the coming event's two bodies (dynamic ones only), in any order; the event
 // kernel sorts them into set order.
@@ -712,13 +713,21 @@ __device__ inline ContactConst contact_const(const ToiArgs& a, int k, const Even
   return c;
 }
 
-// One TOI event (src/private/dynamics/b2_world.rs:683-898), one workgroup.
-__global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
-  __shared__ EventShared s;
-  __shared__ float min_alpha;
-  __shared__ int min_k, solid;
+struct EventCtl {
+  float min_alpha;
+  int min_k, solid;
+};
+
+__device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alpha);
+
+// One TOI event (src/private/dynamics/b2_world.rs:683-898), called by a whole workgroup (the
+// event itself runs on its first thread, as sequential as the reference's).
+__device__ void toi_event(const ToiArgs& a, EventShared& s, EventCtl& e) {
   const int t = threadIdx.x;
   if (t == 0) {
+    float& min_alpha = e.min_alpha;
+    int& min_k = e.min_k;
+    int& solid = e.solid;
     const unsigned long long key = *a.key;
     min_k = static_cast<int>(key & 0xffffffffu);
     min_alpha = __uint_as_float(static_cast<unsigned>(key >> 32));
@@ -760,8 +769,13 @@ __global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
     }
   }
   __syncthreads();
-  if (!solid) return;
-  if (t != 0) return;
+  if (e.solid && t == 0) toi_event_island(a, s, e.min_alpha);
+  __syncthreads();
+}
+
+// the solid event's island: its static and kinematic contacts gathered, its TOI solve, the new
+// sweeps and velocities (first thread only)
+__device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alpha) {
   // gather the static and kinematic contacts of each dynamic TOI body, in set order
   for (int side = 0; side < 2; ++side) {
     const int body = s.bodies[side];
@@ -875,6 +889,12 @@ __global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
   *a.status = kEventSolved;
 }
 
+__global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
+  __shared__ EventShared s;
+  __shared__ EventCtl e;
+  toi_event(a, s, e);
+}
+
 // After a solved event: clear the TOI flags of every contact of the event's dynamic bodies (the
 // island's other bodies are static or kinematic; src/private/dynamics/b2_world.rs:871-891) and
 // select those bodies for the proxy synchronisation that follows (sel[b] = 1: a view in which
@@ -906,6 +926,221 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __res
   toi[k] = o >= 0 ? otoi[o] : 1.0f;
   tcount[k] = o >= 0 ? ocount[o] : 0;
   tflags[k] = o >= 0 ? oflags[o] : 0;
+}
+
+// ---- device event loop ---------------------------------------------------------------------
+// The pass's events run in one persistent workgroup (k_toi_loop) instead of one host round trip
+// per event: per event it takes the smallest cached time of impact over the candidate list (the
+// contacts whose cached time is below 1; the list only grows, stale entries are skipped), lists
+// the event bodies' contacts from the per-body contact lists, runs the event (toi_event), clears
+// the TOI flags of the displaced bodies' contacts, re-synchronises their proxies, recomputes the
+// times of impact of the contacts that lost theirs (toi_rescan_wg) and adds those below 1 to the
+// list. It returns to the host only when no event is left, when an event re-fattened a proxy
+// (the host rebuilds the contact set, as find_new_contacts would add pairs), on a capacity error,
+// or after max_events events. The kernels it replaces (k_toi_min, k_toi_cand, k_toi_post, the
+// proxy synchronisation, k_toi_need) scanned every contact or proxy per event; the lists make
+// each event's work proportional to the event's own contacts.
+constexpr int kLoopBlock = 512;  // (512 threads leave the event code 256 registers per thread)
+constexpr int kToiQueued = 4;  // TOI flag: listed for recomputation in this event
+enum : int { kLoopDone = 0, kLoopRebuild = 1, kLoopCapacity = 2, kLoopMore = 3 };
+
+struct LoopArgs {
+  const int* cadj_off;  // contacts per non-static body of the current set
+  const int* cadj;
+  const int* bfix_off;  // fixtures per body
+  const int* bfix;
+  float4* faabb;
+  float4* ffat;
+  int* cl;         // event candidates
+  int* cl_count;
+  int* in_list;    // per contact: in cl
+  int* state;      // loop exit state
+  int* events;     // [0] events run, [1] events solved
+  int max_events;
+};
+
+// contacts per non-static body (the adjacency the loop reads)
+__global__ void __launch_bounds__(kToiBlock) k_cadj_count(int n, const int2* __restrict__ ab,
+                                                          const int* __restrict__ fbody, const int* __restrict__ btype,
+                                                          int* __restrict__ deg) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int2 f = ab[k];
+  const int x = fbody[f.x], y = fbody[f.y];
+  if (btype[x] != 0) atomicAdd(&deg[x], 1);
+  if (btype[y] != 0) atomicAdd(&deg[y], 1);
+}
+__global__ void __launch_bounds__(kToiBlock) k_cadj_fill(int n, const int2* __restrict__ ab,
+                                                         const int* __restrict__ fbody, const int* __restrict__ btype,
+                                                         const int* __restrict__ off, int* __restrict__ cursor,
+                                                         int* __restrict__ adj) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int2 f = ab[k];
+  const int x = fbody[f.x], y = fbody[f.y];
+  if (btype[x] != 0) adj[off[x] + atomicAdd(&cursor[x], 1)] = k;
+  if (btype[y] != 0) adj[off[y] + atomicAdd(&cursor[y], 1)] = k;
+}
+// fixtures per body
+__global__ void __launch_bounds__(kToiBlock) k_bfix_count(int np, const int* __restrict__ fbody, int* __restrict__ deg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) atomicAdd(&deg[fbody[i]], 1);
+}
+__global__ void __launch_bounds__(kToiBlock) k_bfix_fill(int np, const int* __restrict__ fbody,
+                                                         const int* __restrict__ off, int* __restrict__ cursor,
+                                                         int* __restrict__ fix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int b = fbody[i];
+  fix[off[b] + atomicAdd(&cursor[b], 1)] = i;
+}
+// the candidate list from scratch: every contact with a cached time of impact below 1
+__global__ void __launch_bounds__(kToiBlock) k_toi_list(ToiArgs a, LoopArgs L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const bool c = a.tflags[k] == kToiValid && a.tcount[k] <= kMaxSubSteps && a.toi[k] < 1.0f;
+  L.in_list[k] = c ? 1 : 0;
+  if (c) L.cl[atomicAdd(L.cl_count, 1)] = k;
+}
+
+__device__ inline unsigned long long toi_key(float alpha, int k) {
+  return (static_cast<unsigned long long>(__float_as_uint(alpha)) << 32) | static_cast<unsigned>(k);
+}
+
+__device__ inline unsigned long long wave_min_u64(unsigned long long x) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned lo = __shfl_xor(static_cast<unsigned>(x), d, 64);
+    const unsigned hi = __shfl_xor(static_cast<unsigned>(x >> 32), d, 64);
+    const unsigned long long y = (static_cast<unsigned long long>(hi) << 32) | lo;
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+// contacts of non-static body b whose time of impact is to be recomputed: not cached, a TOI
+// candidate, not yet listed in this event
+__device__ inline void need_of_body(const ToiArgs& a, const LoopArgs& L, int b) {
+  const int e0 = L.cadj_off[b], e1 = L.cadj_off[b + 1];
+  for (int e = e0 + threadIdx.x; e < e1; e += kLoopBlock) {
+    const int k = L.cadj[e];
+    if (a.tflags[k] & (kToiValid | kToiQueued)) continue;
+    int bA, bB;
+    if (!toi_candidate(a, k, &bA, &bB)) continue;
+    if (atomicOr(&a.tflags[k], kToiQueued) & kToiQueued) continue;
+    a.need[atomicAdd(&a.flags4[1], 1)] = k;
+  }
+}
+
+__global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) {
+  __shared__ EventShared s;
+  __shared__ EventCtl e;
+  __shared__ unsigned long long wmin[kLoopBlock / 64];
+  __shared__ int stop, nbody, bodies[kMaxToiBodies];
+  const int t = threadIdx.x;
+  for (int iter = 0;; ++iter) {
+    // the smallest (time of impact, contact) over the candidates
+    unsigned long long best = ~0ull;
+    const int nl = *L.cl_count;
+    for (int i = t; i < nl; i += kLoopBlock) {
+      const int k = L.cl[i];
+      if (a.tflags[k] != kToiValid || a.tcount[k] > kMaxSubSteps) continue;
+      const float alpha = a.toi[k];
+      if (alpha < 1.0f) {
+        const unsigned long long key = toi_key(alpha, k);
+        best = key < best ? key : best;
+      }
+    }
+    best = wave_min_u64(best);
+    if ((t & 63) == 0) wmin[t >> 6] = best;
+    __syncthreads();
+    if (t == 0) {
+      for (int w = 1; w < kLoopBlock / 64; ++w) best = wmin[w] < best ? wmin[w] : best;
+      const float alpha = __uint_as_float(static_cast<unsigned>(best >> 32));
+      stop = 0;
+      if (best == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha) stop = kLoopDone + 1;
+      else if (iter >= L.max_events) stop = kLoopMore + 1;
+      if (stop) *L.state = stop - 1;
+      *a.key = best;
+      a.flags4[0] = 0;
+      a.flags4[1] = 0;
+      a.flags4[2] = 0;
+      a.flags4[3] = 0;
+    }
+    __syncthreads();
+    if (stop) break;
+    // the contacts of the event's dynamic bodies (any order; the event sorts them into set order)
+    const unsigned long long key = *a.key;
+    const int km = static_cast<int>(key & 0xffffffffu);
+    const int2 fm = a.ab[km];
+    const int ev[2] = {a.fbody[fm.x], a.fbody[fm.y]};
+    for (int side = 0; side < 2; ++side) {
+      const int b = ev[side];
+      if (a.btype[b] != 2) continue;
+      const int e0 = L.cadj_off[b], e1 = L.cadj_off[b + 1];
+      for (int i = t; i < e1 - e0 && i < kMaxBodyContacts; i += kLoopBlock)
+        a.cand[side * kMaxBodyContacts + i] = L.cadj[e0 + i];
+      if (t == 0) a.flags4[2 + side] = e1 - e0;
+    }
+    __syncthreads();
+    toi_event(a, s, e);
+    const int status = *a.status;
+    if (status == kEventCapacity) {
+      if (t == 0) *L.state = kLoopCapacity;
+      break;
+    }
+    // the bodies whose contacts may need a new time of impact: the island's non-static bodies
+    // after a solved event, the event's two otherwise (the contact update may have woken them)
+    if (t == 0) {
+      int n = 0;
+      if (status == kEventSolved) {
+        for (int i = 0; i < s.nbody; ++i)
+          if (a.btype[s.bodies[i]] != 0) bodies[n++] = s.bodies[i];
+      } else {
+        for (int i = 0; i < 2; ++i)
+          if (a.btype[ev[i]] != 0) bodies[n++] = ev[i];
+      }
+      nbody = n;
+      L.events[0] += 1;
+      if (status == kEventSolved) L.events[1] += 1;
+    }
+    __syncthreads();
+    if (status == kEventSolved) {
+      // clear the TOI flags of every contact of the event's dynamic bodies
+      // (src/private/dynamics/b2_world.rs:871-891), then re-synchronise their proxies
+      for (int side = 0; side < 2; ++side) {
+        const int b = ev[side];
+        if (a.btype[b] != 2) continue;
+        const int e0 = L.cadj_off[b], e1 = L.cadj_off[b + 1];
+        for (int i = e0 + t; i < e1; i += kLoopBlock) a.tflags[L.cadj[i]] &= ~kToiValid;
+        const int f0 = L.bfix_off[b], f1 = L.bfix_off[b + 1];
+        for (int i = f0 + t; i < f1; i += kLoopBlock) {
+          const int f = L.bfix[i];
+          const bool refat = sync_proxy(f, b, a.shapes[f], a.pos0, a.bmass, a.xf, L.faabb, L.ffat);
+          a.moved[f] = refat ? 1 : 0;
+          if (refat) a.flags4[0] = 1;  // every writer stores 1
+        }
+      }
+      __syncthreads();
+      if (a.flags4[0]) {
+        // a re-fattened proxy can add pairs: the host rebuilds the contact set first
+        if (t == 0) *L.state = kLoopRebuild;
+        break;
+      }
+    }
+    for (int i = 0; i < nbody; ++i) need_of_body(a, L, bodies[i]);
+    __syncthreads();
+    const int m = a.flags4[1];
+    toi_rescan_wg(a);
+    __syncthreads();
+    for (int i = t; i < m; i += kLoopBlock) {
+      const int k = a.rank[i];
+      if (a.toi[k] < 1.0f && !L.in_list[k]) {
+        L.in_list[k] = 1;
+        L.cl[atomicAdd(L.cl_count, 1)] = k;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -963,7 +1198,7 @@ static int toi_reserve(World* w) {
       (st = w->toi_cand.reserve(2 * kMaxBodyContacts)))
     return st;
   if ((st = w->toi_alpha0.reserve(std::max(w->nb, 1))) || (st = w->toi_sel.reserve(std::max(w->nb, 1))) ||
-      (st = w->toi_ctl.reserve(5)))
+      (st = w->toi_ctl.reserve(7)) || (st = w->toi_cl.reserve(n)) || (st = w->toi_in_list.reserve(n)))
     return st;
   if (!w->toi_two.p) {
     if ((st = w->toi_two.reserve(2))) return st;
@@ -974,13 +1209,16 @@ static int toi_reserve(World* w) {
   return B2G_OK;
 }
 
-// ctl: [0] key, [1] status, [2] event bodies, [3] / [4] flags4
+// ctl: [0] key, [1] status and candidate count, [2] event bodies, [3] / [4] flags4, [5] loop
+// state and events run, [6] events solved
 struct ToiCtl {
   unsigned long long key;
-  int status, pad;
+  int status, cl_count;
   int bodies[2];
   int flags4[4];
+  int loop_state, events, solved, pad;
 };
+static_assert(sizeof(ToiCtl) == 7 * sizeof(unsigned long long), "TOI control block");
 
 static int toi_read(World* w, ToiCtl* out) {
   B2G_HIP(hipMemcpyAsync(out, w->toi_ctl.p, sizeof(ToiCtl), hipMemcpyDeviceToHost, w->stream));
@@ -1114,94 +1352,140 @@ static int toi_rebuild(World* w) {
   return B2G_OK;
 }
 
+// fixtures per body (rebuilt when the fixture count changed)
+static int toi_body_fixtures(World* w) {
+  if (w->toi_bfix_np == w->np) return B2G_OK;
+  int st;
+  const int nb = w->nb, np = w->np;
+  if ((st = w->toi_bfix_off.reserve(nb + 1)) || (st = w->toi_bfix_cur.reserve(nb + 1)) ||
+      (st = w->toi_bfix.reserve(std::max(np, 1))))
+    return st;
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(nb + 1, kToiBlock)), dim3(kToiBlock), 0, w->stream, nb + 1,
+                     w->toi_bfix_cur.p);
+  if (np > 0)
+    hipLaunchKernelGGL(k_bfix_count, dim3(blocks_for(np, kToiBlock)), dim3(kToiBlock), 0, w->stream, np, w->f.body.p,
+                       w->toi_bfix_cur.p);
+  if ((st = exclusive_scan(w, w->toi_bfix_cur.p, w->toi_bfix_off.p, nb + 1, nullptr))) return st;
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(nb + 1, kToiBlock)), dim3(kToiBlock), 0, w->stream, nb + 1,
+                     w->toi_bfix_cur.p);
+  if (np > 0)
+    hipLaunchKernelGGL(k_bfix_fill, dim3(blocks_for(np, kToiBlock)), dim3(kToiBlock), 0, w->stream, np, w->f.body.p,
+                       w->toi_bfix_off.p, w->toi_bfix_cur.p, w->toi_bfix.p);
+  B2G_HIP(hipGetLastError());
+  w->toi_bfix_np = np;
+  return B2G_OK;
+}
+
+static LoopArgs loop_args(World* w) {
+  LoopArgs L;
+  L.cadj_off = w->toi_cadj_off.p;
+  L.cadj = w->toi_cadj.p;
+  L.bfix_off = w->toi_bfix_off.p;
+  L.bfix = w->toi_bfix.p;
+  L.faabb = w->f.aabb.p;
+  L.ffat = w->f.fat.p;
+  L.cl = w->toi_cl.p;
+  int* ctl = reinterpret_cast<int*>(w->toi_ctl.p);
+  L.cl_count = ctl + 3;
+  L.in_list = w->toi_in_list.p;
+  L.state = ctl + 10;
+  L.events = ctl + 11;
+  L.max_events = 1 << 16;
+  return L;
+}
+
+// the loop's per-set structures for the current contact set: contacts per non-static body and
+// the candidate list
+static int toi_loop_setup(World* w, float dt, int vi) {
+  int st;
+  const int nb = w->nb, n = w->n_pairs;
+  if ((st = w->toi_cadj_off.reserve(nb + 1)) || (st = w->toi_cadj_cur.reserve(nb + 1)) ||
+      (st = w->toi_cadj.reserve(std::max(2 * n, 1))) || (st = w->toi_cl.reserve(std::max(n, 1))) ||
+      (st = w->toi_in_list.reserve(std::max(n, 1))))
+    return st;
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(nb + 1, kToiBlock)), dim3(kToiBlock), 0, w->stream, nb + 1,
+                     w->toi_cadj_cur.p);
+  const ToiArgs a = toi_args(w, dt, vi);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cadj_count, dim3(blocks_for(n, kToiBlock)), dim3(kToiBlock), 0, w->stream, n, a.ab, a.fbody,
+                       a.btype, w->toi_cadj_cur.p);
+  if ((st = exclusive_scan(w, w->toi_cadj_cur.p, w->toi_cadj_off.p, nb + 1, nullptr))) return st;
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(nb + 1, kToiBlock)), dim3(kToiBlock), 0, w->stream, nb + 1,
+                     w->toi_cadj_cur.p);
+  const LoopArgs L = loop_args(w);
+  hipLaunchKernelGGL(k_clear, dim3(1), dim3(kToiBlock), 0, w->stream, 1, L.cl_count);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cadj_fill, dim3(blocks_for(n, kToiBlock)), dim3(kToiBlock), 0, w->stream, n, a.ab, a.fbody,
+                       a.btype, w->toi_cadj_off.p, w->toi_cadj_cur.p, w->toi_cadj.p);
+    hipLaunchKernelGGL(k_toi_list, dim3(blocks_for(n, kToiBlock)), dim3(kToiBlock), 0, w->stream, a, L);
+  }
+  B2G_HIP(hipGetLastError());
+  return B2G_OK;
+}
+
 // The TOI pass after the solve (src/private/dynamics/b2_world.rs:534-901, step_complete, no
-// sub-stepping). The current contact set is the one find_new_contacts left.
+// sub-stepping). The current contact set is the one find_new_contacts left. The events run on
+// the device (k_toi_loop); the host rebuilds the contact set when an event re-fattened a proxy.
 int phase_toi(World* w, float dt, int vi) {
   int st;
   w->toi_events = 0;
   if ((st = carry_manifolds_for_toi(w))) return st;
   if ((st = toi_reserve(w))) return st;
-  static const ToiCtl kReset = {~0ull, 0, 0, {0, 0}, {0, 0, 0, 0}};
+  if ((st = toi_body_fixtures(w))) return st;
+  static const ToiCtl kReset = {~0ull, 0, 0, {0, 0}, {0, 0, 0, 0}, 0, 0, 0, 0};
   B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
   if (w->n_pairs > 0) {
     hipLaunchKernelGGL(k_toi_first, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
                        toi_args(w, dt, vi));
     B2G_HIP(hipGetLastError());
   }
-  if (w->n_pairs > 0) {
-    hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                       toi_args(w, dt, vi));
-    B2G_HIP(hipGetLastError());
-  }
-  bool dirty = false, pending = false;
-  // every event either disables its contact or raises its TOI count, so the loop ends; the
-  // bound is a guard against a logic error, not a limit of the reference
+  if ((st = toi_loop_setup(w, dt, vi))) return st;
+  bool dirty = false;
+  long long total = 0;
+  // every event either disables its contact or raises its TOI count, so the pass ends; the bound
+  // is a guard against a logic error, not a limit of the reference
   const long long kMaxEvents = 4ll * (kMaxSubSteps + 2) * (static_cast<long long>(w->n_pairs) + w->nb + 16);
-  for (long long iter = 0; iter < kMaxEvents; ++iter) {
+  for (;;) {
+    if (w->n_pairs == 0) return dirty ? toi_rebuild(w) : B2G_OK;
+    hipLaunchKernelGGL(k_toi_loop, dim3(1), dim3(kLoopBlock), 0, w->stream, toi_args(w, dt, vi), loop_args(w));
+    B2G_HIP(hipGetLastError());
     ToiCtl ctl;
     if ((st = toi_read(w, &ctl))) return st;
-    if (pending) {
-      // the last event's outcome; without a re-fattened proxy the rescan and the minimum that
-      // followed it on the device are already the next ones
-      pending = false;
-      if (ctl.status == kEventCapacity) {
-        set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
-        return B2G_ERR_CAPACITY;
-      }
-      dirty = true;
-      if (ctl.status == kEventSolved) {
-        ++w->toi_events;
-        // find_new_contacts: only re-fattened proxies can add pairs (the reference queries the
-        // moved proxies only), so the set is rebuilt only when a proxy was re-fattened
-        if (ctl.flags4[0]) {
-          if ((st = toi_rebuild(w))) return st;
-          dirty = false;
-          if (w->n_pairs > 0) {
-            const ToiArgs b = toi_args(w, dt, vi);
-            hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b,
-                               0);
-            B2G_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, b);
-            B2G_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                               b);
-            B2G_HIP(hipGetLastError());
-          } else {
-            // no pairs left: the minimum found before the rebuild named the old set
-            B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
-          }
-          if ((st = toi_read(w, &ctl))) return st;
-        }
-      }
+    w->toi_events += ctl.solved;
+    total += ctl.events;
+    if (ctl.events > 0) dirty = true;
+    if (ctl.loop_state == kLoopCapacity) {
+      set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
+      return B2G_ERR_CAPACITY;
     }
-    const float alpha = __builtin_bit_cast(float, static_cast<unsigned>(ctl.key >> 32));
-    if (ctl.key == ~0ull || 1.0f - 10.0f * kFltEpsilon < alpha || w->n_pairs == 0) {
+    if (total > kMaxEvents) {
+      set_error("continuous collision: the TOI pass did not end");
+      return B2G_ERR_DEVICE;
+    }
+    // the next launch counts its own events
+    static const int kZero[3] = {kLoopDone, 0, 0};
+    B2G_HIP(hipMemcpyAsync(reinterpret_cast<int*>(w->toi_ctl.p) + 10, kZero, sizeof(kZero), hipMemcpyHostToDevice,
+                           w->stream));
+    if (ctl.loop_state == kLoopDone) {
       // manifolds that an event changed after the last rebuild become the next collide's
       // predecessors through one more rebuild
       return dirty ? toi_rebuild(w) : B2G_OK;
     }
-    // the event, the displaced bodies' flags and proxies, then (unless a proxy was re-fattened)
-    // the next scan and minimum, all without the host
-    const ToiArgs a = toi_args(w, dt, vi);
-    const int nmax = std::max(w->n_pairs, w->nb);
-    hipLaunchKernelGGL(k_toi_cand, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
-    B2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_toi_event, dim3(1), dim3(kEventBlock), 0, w->stream, a);
-    B2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_toi_post, dim3(blocks_for(nmax, kToiBlock)), dim3(kToiBlock), 0, w->stream, a,
-                       w->toi_sel.p);
-    B2G_HIP(hipGetLastError());
-    if ((st = phase_synchronize_view(w, IslandView{w->toi_sel.p, w->toi_two.p}, a.flags4))) return st;
-    hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a, 1);
-    B2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, a);
-    B2G_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_toi_min, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, a);
-    B2G_HIP(hipGetLastError());
-    pending = true;
+    if (ctl.loop_state == kLoopRebuild) {
+      // find_new_contacts: only re-fattened proxies can add pairs (the reference queries the
+      // moved proxies only), so the set is rebuilt only when a proxy was re-fattened
+      if ((st = toi_rebuild(w))) return st;
+      dirty = false;
+      if (w->n_pairs > 0) {
+        const ToiArgs b = toi_args(w, dt, vi);
+        hipLaunchKernelGGL(k_clear, dim3(1), dim3(kToiBlock), 0, w->stream, 1, b.flags4 + 1);
+        hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b, 0);
+        hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, b);
+        B2G_HIP(hipGetLastError());
+        if ((st = toi_loop_setup(w, dt, vi))) return st;
+      }
+    }
   }
-  set_error("continuous collision: the TOI pass did not end");
-  return B2G_ERR_DEVICE;
 }
 
 }  // namespace b2g
