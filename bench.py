@@ -120,11 +120,11 @@ def cpu_baseline(scene, seconds=15.0, max_steps=200, joints=None, continuous=Fal
             "cpu_model": cpu_model(), "nproc": host_cores()}
 
 
-def measured_traffic(kernel, window):
-    """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary of this workload (calibrated against a copy kernel), when that summary covers the
-    same step window (first, last); otherwise None."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_%s_large_world_65536.json" % kernel)
+def measured_traffic(name, window):
+    """HBM bytes per launch of the named kernel (k_...) from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a copy kernel), when that summary covers
+    the same step window (first, last); otherwise None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_large_world_65536.json" % name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -142,9 +142,10 @@ def roofline(kernel, k_ms, k_launches, k_items, k_aux, window=None, pair_shape=N
         bytes_per_launch += sum(pair_shape) / len(pair_shape) * k_items / max(k_launches, 1)
     avg_launch_s = k_ms / 1000.0 / max(k_launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if k_launches else 0.0
+    name = solver_kernel_name(kernel)
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, window) if window else None,
-            "kernel": solver_kernel_name(kernel), "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, window) if window else None,
+            "kernel": name, "launches": k_launches, "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_launch": bytes_per_launch, "unit_bytes": UNIT_BYTES[kernel],
             "items_per_launch": k_items / max(k_launches, 1), "item": UNIT_NAME[kernel]}
     if kernel == "traverse":

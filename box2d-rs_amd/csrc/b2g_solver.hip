@@ -430,6 +430,37 @@ __device__ inline void block_count_keys(int key, int* __restrict__ counts) {
     if (tcnt[e]) atomicAdd(&counts[tkey[e]], tcnt[e]);
 }
 
+// Block-aggregated scatter: every thread with a key >= 0 gets a position in its key's range,
+// off[key] + (elements of that key placed before it), with one global atomic on cursor[key] per
+// distinct key in the block (ranks inside the block from LDS atomics; the order among equal keys
+// is free). Called by every thread of a kBlock-thread block; returns -1 for key < 0.
+__device__ inline int block_place_keys(int key, const int* __restrict__ off, int* __restrict__ cursor) {
+  constexpr int H = 2 * kBlock, LG = 9;
+  __shared__ int tkey[H];
+  __shared__ int tcnt[H];
+  __shared__ int tbase[H];
+  for (int e = threadIdx.x; e < H; e += kBlock) {
+    tkey[e] = -1;
+    tcnt[e] = 0;
+  }
+  __syncthreads();
+  int h = 0, rank = 0;
+  if (key >= 0) {
+    h = static_cast<int>((static_cast<unsigned>(key) * 2654435761u) >> (32 - LG));
+    for (;;) {
+      const int prev = atomicCAS(&tkey[h], -1, key);
+      if (prev == -1 || prev == key) break;
+      h = (h + 1) & (H - 1);
+    }
+    rank = atomicAdd(&tcnt[h], 1);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < H; e += kBlock)
+    if (tcnt[e]) tbase[e] = off[tkey[e]] + atomicAdd(&cursor[tkey[e]], tcnt[e]);
+  __syncthreads();
+  return key >= 0 ? tbase[h] + rank : -1;
+}
+
 __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ ncp, const int2* __restrict__ bodies,
                                                        const int* __restrict__ btype, const int* __restrict__ off,
                                                        const int* __restrict__ adj, const int* __restrict__ prev_colour,
@@ -1160,8 +1191,8 @@ __global__ void __launch_bounds__(kBlock) k_color_world_hist(int n, const int* _
                                                              const int2* __restrict__ ab, const int* __restrict__ fworld,
                                                              int nw, int* __restrict__ count) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || state[j] < 0) return;
-  atomicAdd(&count[launch[state[j] & 0xff] * nw + constraint_world(j, list, ab, fworld)], 1);
+  const int g = j < n && state[j] >= 0 ? launch[state[j] & 0xff] * nw + constraint_world(j, list, ab, fworld) : -1;
+  block_count_keys(g, count);
 }
 
 __global__ void __launch_bounds__(kBlock) k_color_world_scatter(int n, const int* __restrict__ state,
@@ -1172,13 +1203,14 @@ __global__ void __launch_bounds__(kBlock) k_color_world_scatter(int n, const int
                                                                 int* __restrict__ order, int* __restrict__ inv_order,
                                                                 int* __restrict__ pair_colour) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || state[j] < 0) return;
-  const int L = launch[state[j] & 0xff];
-  const int g = L * nw + constraint_world(j, list, ab, fworld);
-  const int pos = off[g] + atomicAdd(&cursor[g], 1);
-  order[pos] = j;
-  inv_order[j] = pos;
-  pair_colour[list[j]] = L;
+  const int L = j < n && state[j] >= 0 ? launch[state[j] & 0xff] : -1;
+  const int g = L >= 0 ? L * nw + constraint_world(j, list, ab, fworld) : -1;
+  const int pos = block_place_keys(g, off, cursor);
+  if (g >= 0) {
+    order[pos] = j;
+    inv_order[j] = pos;
+    pair_colour[list[j]] = L;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_clear_ints(int n, int* __restrict__ a) {
@@ -1515,24 +1547,21 @@ __global__ void __launch_bounds__(kBlock) k_island_small(int nb, const int* __re
 // bin (isl_start / cap); a large island's to the last group of their colour, whose few counters
 // would see one global atomic per constraint: those are counted and ranked in LDS first and take
 // one global atomic per colour and block, as k_color_hist and k_color_scatter do.
+__device__ inline int constraint_group(int j, const int* __restrict__ state, const int* __restrict__ launch,
+                                       const int* __restrict__ croot, const int* __restrict__ isl_count,
+                                       const int* __restrict__ isl_start, int cap, int nbins) {
+  const int L = launch[state[j] & 0xff], r = croot[j];
+  return L * (nbins + 1) + (isl_count[r] > cap ? nbins : isl_start[r] / cap);
+}
+
 __global__ void __launch_bounds__(kBlock) k_color_bin_hist(int n, const int* __restrict__ state,
                                                            const int* __restrict__ launch, const int* __restrict__ croot,
                                                            const int* __restrict__ isl_count,
                                                            const int* __restrict__ isl_start, int cap, int nbins,
                                                            int* __restrict__ count) {
-  __shared__ int hb[kMaxColors];
-  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) hb[c] = 0;
-  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ng = nbins + 1;
-  if (j < n && state[j] >= 0) {
-    const int L = launch[state[j] & 0xff], r = croot[j];
-    if (isl_count[r] > cap) atomicAdd(&hb[L], 1);
-    else atomicAdd(&count[L * ng + isl_start[r] / cap], 1);
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
-    if (hb[c]) atomicAdd(&count[c * ng + nbins], hb[c]);
+  const int g = j < n && state[j] >= 0 ? constraint_group(j, state, launch, croot, isl_count, isl_start, cap, nbins) : -1;
+  block_count_keys(g, count);
 }
 
 __global__ void __launch_bounds__(kBlock) k_color_bin_scatter(int n, const int* __restrict__ state,
@@ -1543,32 +1572,13 @@ __global__ void __launch_bounds__(kBlock) k_color_bin_scatter(int n, const int* 
                                                               const int* __restrict__ off, int* __restrict__ cursor,
                                                               const int* __restrict__ list, int* __restrict__ order,
                                                               int* __restrict__ inv_order, int* __restrict__ pair_colour) {
-  __shared__ int hb[kMaxColors];
-  __shared__ int base[kMaxColors];
-  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) hb[c] = 0;
-  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ng = nbins + 1;
-  int L = -1, pos = 0, rank = -1;
-  if (j < n && state[j] >= 0) {
-    L = launch[state[j] & 0xff];
-    const int r = croot[j];
-    if (isl_count[r] > cap) {
-      rank = atomicAdd(&hb[L], 1);
-    } else {
-      const int g = L * ng + isl_start[r] / cap;
-      pos = off[g] + atomicAdd(&cursor[g], 1);
-    }
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
-    if (hb[c]) base[c] = off[c * ng + nbins] + atomicAdd(&cursor[c * ng + nbins], hb[c]);
-  __syncthreads();
-  if (L >= 0) {
-    if (rank >= 0) pos = base[L] + rank;
+  const int g = j < n && state[j] >= 0 ? constraint_group(j, state, launch, croot, isl_count, isl_start, cap, nbins) : -1;
+  const int pos = block_place_keys(g, off, cursor);
+  if (g >= 0) {
     order[pos] = j;
     inv_order[j] = pos;
-    pair_colour[list[j]] = L;
+    pair_colour[list[j]] = g / (nbins + 1);
   }
 }
 
