@@ -155,10 +155,10 @@ __global__ void __launch_bounds__(kBlock) k_sleep_apply(int nb, const int* __res
 
 // ---- touching contacts, islands, colouring -----------------------------------------------------
 // per-step solver counters: [0] touching (written by the scan), [1] uncoloured, [2] colour
-// overflow, [3] active islands, then the colour histogram; and the colour scatter cursors
+// overflow, [3] unused, then the colour histogram; and the colour scatter cursors
 __device__ inline void counters_init(int t, int* c, int* __restrict__ cursor) {
   if (t == 0) {
-    c[1] = c[0];
+    c[1] = 0;
     c[2] = 0;
     c[3] = 0;
     c[kBigIslandSlot] = 0;
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_solver_body_init(int nb, int* __rest
   if (b == 0) deg[nb] = 0;
 }
 
-// Per-step solver counters: [0] touching, [1] uncoloured, [2] colour overflow, [3] active
+// Per-step solver counters: [0] touching, [1] uncoloured, [2] colour overflow, [3] unused
 // islands, then the colour histogram (kMaxColors counts and the last colouring round); also
 // clears the colour scatter cursors.
 __global__ void __launch_bounds__(kBlock) k_solver_counters_init(int* c, int* __restrict__ cursor) {
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ n
                                                        const int* __restrict__ btype, const int* __restrict__ off,
                                                        const int* __restrict__ adj, const int* __restrict__ prev_colour,
                                                        int cap, IslandView iv, int* __restrict__ state,
-                                                       int* __restrict__ remaining, int* __restrict__ isl_count,
+                                                       int* __restrict__ isl_count,
                                                        int* __restrict__ croot) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   bool kept = false, left_out = false;
@@ -513,15 +513,13 @@ __global__ void __launch_bounds__(kBlock) k_color_keep(const int* __restrict__ n
     }
     state[j] = kept ? pc : -1;  // round 0
   }
-  const unsigned long long m = __ballot(kept || left_out);
-  if (m && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1) atomicSub(remaining, __popcll(m));
   if (isl_count) block_count_keys(root, isl_count);  // (uniform: every thread of the block calls it)
 }
 
 __global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ ncp, int round,
                                                         const int2* __restrict__ bodies, const int* __restrict__ btype,
                                                         const int* __restrict__ off, const int* __restrict__ adj,
-                                                        int* state, int* __restrict__ remaining,
+                                                        int* state,
                                                         int* __restrict__ overflow) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   int c = -1;
@@ -529,9 +527,6 @@ __global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ 
     c = color_pick(j, round, bodies, btype, off, adj, state, overflow);
     if (c >= 0) __hip_atomic_store(&state[j], (round << 8) | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
-  // one decrement of the shared counter per wave
-  const unsigned long long m = __ballot(c >= 0);
-  if (m && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1) atomicSub(remaining, __popcll(m));
 }
 
 // Colour histogram and scatter with block-aggregated atomics: ranks inside a block come from
@@ -539,25 +534,33 @@ __global__ void __launch_bounds__(kBlock) k_color_round(const int* __restrict__ 
 // atomic per block instead of one per constraint. The order inside a colour is free (its
 // constraints are independent), so no stable ranking is needed.
 // count[kMaxColors]: the last round in which a constraint was coloured
+// *uncoloured: constraints still without a colour (state -1), which the host reads to decide
+// whether more colouring rounds are needed
 __global__ void __launch_bounds__(kBlock) k_color_hist(const int* __restrict__ ncp, const int* __restrict__ state,
-                                                       int* __restrict__ count) {
+                                                       int* __restrict__ count, int* __restrict__ uncoloured) {
   __shared__ int h[kMaxColors];
-  __shared__ int last_round;
+  __shared__ int last_round, unc;
   for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x) h[c] = 0;
-  if (threadIdx.x == 0) last_round = 0;
+  if (threadIdx.x == 0) {
+    last_round = 0;
+    unc = 0;
+  }
   __syncthreads();
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < *ncp) {
     const int st = state[j];
-    if (st >= 0) {  // uncoloured constraints (-1) are not counted
+    if (st >= 0) {
       atomicAdd(&h[st & 0xff], 1);
       atomicMax(&last_round, st >> 8);
+    } else if (st == -1) {
+      atomicAdd(&unc, 1);
     }
   }
   __syncthreads();
   for (int c = threadIdx.x; c < kMaxColors; c += blockDim.x)
     if (h[c]) atomicAdd(&count[c], h[c]);
   if (threadIdx.x == 0 && last_round > 0) atomicMax(&count[kMaxColors], last_round);
+  if (threadIdx.x == 0 && unc > 0) atomicAdd(uncoloured, unc);
 }
 
 // map: per colour id its base offset in the colour order ([0, kMaxColors)) and its launch
@@ -1124,10 +1127,8 @@ __global__ void __launch_bounds__(kTailBlock) k_solve_position_tail(int c0, int 
   }
 }
 
-__device__ inline void iteration_end_root(int r, int iteration, int* __restrict__ violated, int* __restrict__ done,
-                                          int* __restrict__ active) {
+__device__ inline void iteration_end_root(int r, int iteration, int* __restrict__ violated, int* __restrict__ done) {
   if (!violated[r]) done[r] = iteration + 1;
-  else atomicAdd(active, 1);
   violated[r] = 0;
 }
 // end of a position iteration: islands whose error is small stop iterating
@@ -1135,10 +1136,9 @@ __device__ inline void iteration_end_root(int r, int iteration, int* __restrict_
 // of iterations it ran (the adjoint's tape reads it).
 __device__ inline void iteration_end_one(int b, int iteration, const int* __restrict__ btype,
                                          const int* __restrict__ parent, int* __restrict__ violated,
-                                         int* __restrict__ done, int* __restrict__ active,
-                                         const int* __restrict__ isl_awake) {
+                                         int* __restrict__ done, const int* __restrict__ isl_awake) {
   if (btype[b] == 0 || parent[b] != b || done[b] || (isl_awake && !isl_awake[b])) return;
-  iteration_end_root(b, iteration, violated, done, active);
+  iteration_end_root(b, iteration, violated, done);
 }
 // the last position iteration's end and the copy back in one launch (both per body; finalize
 // reads only positions, the iteration end only island flags)
@@ -1147,7 +1147,6 @@ struct IterationEndArgs {
   const int* parent;
   int* violated;
   int* done;
-  int* active;
   const int* isl_awake;
 };
 __global__ void __launch_bounds__(kBlock) k_finalize_end(int nb, const int* __restrict__ btype,
@@ -1155,7 +1154,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize_end(int nb, const int* __re
                                                          float4* __restrict__ xf, IslandView iv, IterationEndArgs e) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
-  if (e.iteration >= 0) iteration_end_one(b, e.iteration, btype, e.parent, e.violated, e.done, e.active, e.isl_awake);
+  if (e.iteration >= 0) iteration_end_one(b, e.iteration, btype, e.parent, e.violated, e.done, e.isl_awake);
   finalize_one(b, btype, pos, bmass, xf, iv);
 }
 // binned (optional): islands of at most bin_cap constraints, whose iteration ends the island
@@ -1163,13 +1162,12 @@ __global__ void __launch_bounds__(kBlock) k_finalize_end(int nb, const int* __re
 __global__ void __launch_bounds__(kBlock) k_position_iteration_end(int nb, int iteration, const int* __restrict__ btype,
                                                                    const int* __restrict__ parent,
                                                                    int* __restrict__ violated, int* __restrict__ done,
-                                                                   int* __restrict__ active,
                                                                    const int* __restrict__ isl_awake,
                                                                    const int* __restrict__ binned, int bin_cap) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   if (binned && binned[b] > 0 && binned[b] <= bin_cap) return;
-  iteration_end_one(b, iteration, btype, parent, violated, done, active, isl_awake);
+  iteration_end_one(b, iteration, btype, parent, violated, done, isl_awake);
 }
 
 // ---- per-world solve (batched worlds) ---------------------------------------------------------
@@ -1264,7 +1262,7 @@ __global__ void __launch_bounds__(B) k_solve_worlds_position(const int* __restri
     }
     if (it + 1 < iters) {
       for (int b = b0 + threadIdx.x; b < b1; b += B)
-        iteration_end_one(b, it, btype, e.parent, e.violated, e.done, e.active, e.isl_awake);
+        iteration_end_one(b, it, btype, e.parent, e.violated, e.done, e.isl_awake);
       __syncthreads();
     }
   }
@@ -1699,7 +1697,7 @@ static void launch_solve_tail(World* w, const SolveGraphKey& k, const VelArgs& v
     for (int col = 0; col < k.ncolors; ++col)
       hipLaunchKernelGGL(k_solve_position_g, dim3(k.grid_col), dim3(kBlock), 0, st, col, s.color_off.p, pa);
     hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, it,
-                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, nullptr, nullptr, 0);
+                       w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, nullptr, nullptr, 0);
   }
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(k.nb, kBlock)), dim3(kBlock), 0, st, k.nb, w->b.type.p, w->b.pos.p,
                      w->b.mass.p, w->b.xf.p, all_awake);
@@ -1774,7 +1772,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   SolverArrays& s = w->s;
   ContactArrays& c = w->c[w->cur];
   int st;
-  int* cnt = w->counters.p;  // [0] touching, [1] uncoloured, [2] colour overflow, [3] active islands
+  int* cnt = w->counters.p;  // [0] touching, [1] uncoloured, [2] colour overflow, [3] unused
   // sleeping bodies: islands without an awake body are left out of the solve
   const bool sleepy = w->sleep_possible();
   // touching contacts: compacted on the device; the constraint arrays are sized for all pairs
@@ -1840,7 +1838,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // others, a first batch without waiting (rounds after the last constraint is coloured find
     // nothing to do), then more only if some remain
     hipLaunchKernelGGL(k_color_keep, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.k_bodies.p, w->b.type.p,
-                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, iv, s.color_state.p, cnt + 1,
+                       s.adj_off.p, s.adj.p, s.prev_colour.p, w->scratch_colors, iv, s.color_state.p,
                        want_bins ? s.isl_count.p : nullptr, s.croot.p);
     if (want_bins) {
       // the small islands' counts and their scan (bin of island r: isl_start[r] / bin_cap); the
@@ -1856,11 +1854,14 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       int batch = round > 1 ? 4 : reuse ? std::max(2, std::min(w->color_rounds, w->reuse_rounds + 1)) : w->color_rounds;
       for (int r = 0; r < batch; ++r, ++round) {
         launch_timed(w, B2G_KERNEL_COLOR_ROUND, n_pairs, k_color_round, dim3(gp), dim3(kBlock), cnt, round,
-                     s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 1, cnt + 2);
+                     s.k_bodies.p, w->b.type.p, s.adj_off.p, s.adj.p, s.color_state.p, cnt + 2);
       }
       // the histogram was cleared with the counters; a later batch recounts from zero
-      if (round > batch + 1) B2G_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(int) * (kMaxColors + 1), w->stream));
-      hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, cnt + 4);
+      if (round > batch + 1) {
+        B2G_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(int) * (kMaxColors + 1), w->stream));
+        B2G_HIP(hipMemsetAsync(cnt + 1, 0, sizeof(int), w->stream));
+      }
+      hipLaunchKernelGGL(k_color_hist, dim3(gp), dim3(kBlock), 0, w->stream, cnt, s.color_state.p, cnt + 4, cnt + 1);
       B2G_HIP(hipMemcpyAsync(w->h_counters, cnt, (kBigIslandSlot + 1) * sizeof(int), hipMemcpyDeviceToHost,
                              w->stream));
       B2G_HIP(hipStreamSynchronize(w->stream));
@@ -2034,7 +2035,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     hipLaunchKernelGGL(k_store_and_integrate, dim3(blocks_for(std::max(nc, nb), kBlock)), dim3(kBlock), 0, w->stream,
                        nc, s.k_contact.p, s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p, nb, w->b.type.p, w->b.pos.p,
                        w->b.vel.p, dt, iv);
-    const IterationEndArgs ends{-1, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake};
+    const IterationEndArgs ends{-1, s.parent.p, s.isl_violated.p, s.isl_done.p, iv.isl_awake};
     if (pos_iters > 0)
       launch_worlds_position(w, static_cast<long long>(nc) * pos_iters, s.cw_off.p, ncolors, pos_iters, pa, ends);
     w->pos_iters_run = pos_iters;
@@ -2097,12 +2098,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // their kernel)
     if (it + 1 < pos_iters && per_colour)
       hipLaunchKernelGGL(k_position_iteration_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, it,
-                         w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake,
+                         w->b.type.p, s.parent.p, s.isl_violated.p, s.isl_done.p, iv.isl_awake,
                          bins ? s.isl_count.p : nullptr, bin_cap);
     ++w->pos_iters_run;
   }
   }
-  const IterationEndArgs last_end{pos_iters - 1, s.parent.p, s.isl_violated.p, s.isl_done.p, cnt + 3, iv.isl_awake};
+  const IterationEndArgs last_end{pos_iters - 1, s.parent.p, s.isl_violated.p, s.isl_done.p, iv.isl_awake};
   hipLaunchKernelGGL(k_finalize_end, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, w->stream, nb, w->b.type.p,
                      w->b.pos.p, w->b.mass.p, w->b.xf.p, iv, last_end);
   if (w->allow_sleep) {
