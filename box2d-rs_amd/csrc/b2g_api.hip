@@ -602,7 +602,7 @@ int b2g_world_destroy(b2g_world* h) {
   rel(s.isl_count); rel(s.isl_small); rel(s.isl_start); rel(s.croot);
   BroadphaseArrays& bp = w->bp;
   rel(bp.keys); rel(bp.keys_alt); rel(bp.vals); rel(bp.vals_alt); rel(bp.hist); rel(bp.seg); rel(bp.nodes); rel(bp.nodes4);
-  rel(bp.leafrec); rel(bp.count); rel(bp.slots); rel(bp.bounds); rel(bp.scan_tmp);
+  rel(bp.leafrec); rel(bp.count); rel(bp.slots); rel(bp.bounds); rel(bp.scan_tmp); rel(bp.scan_state); rel(bp.scan_ctl); bp.scan_epoch = 0; bp.scan_ctl_ready = false;
   rel(w->counters);
   if (w->h_counters) (void)hipHostFree(w->h_counters);
   for (int i = 0; i < 8; ++i)

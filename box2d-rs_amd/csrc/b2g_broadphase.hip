@@ -71,13 +71,14 @@ __global__ void __launch_bounds__(kBlock) k_bounds(int n, const int* __restrict_
   }
 }
 
-__global__ void k_bounds_init(int* out) {
+__global__ void k_bounds_init(int* out, const int* __restrict__ scan_ctl) {
   out[0] = float_flip(kMaxFloat);
   out[1] = float_flip(kMaxFloat);
   out[2] = float_flip(-kMaxFloat);
   out[3] = float_flip(-kMaxFloat);
   out[4] = 0;  // traversal stack overflow
   out[5] = 0;  // proxies whose pairs overflow their slots
+  out[6] = scan_ctl ? scan_ctl[1] : 0;  // a single-pass scan gave up waiting (read back with the flags)
 }
 
 __device__ inline unsigned spread15(unsigned x) {
@@ -519,7 +520,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
     return st;
   int* bnd = reinterpret_cast<int*>(bp.bounds.p);
   int* flags = bnd + 4;  // [0] stack overflow, [1] proxies with more pairs than slots
-  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, w->stream, bnd);
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, w->stream, bnd, static_cast<const int*>(bp.scan_ctl.p));
   // world bits above Morton bits: 12 bits per axis (4096 cells across the scene's extent, finer
   // than any body on the scenes here; equal keys keep proxy order), fewer for many worlds, so one
   // world sorts in three 8-bit passes
@@ -557,14 +558,18 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
                  nt, fat, tv, pf, bp.count.p, bp.slots.p, nullptr, nullptr, flags);
   }
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np))) return st;
-  int h[3];
+  int h[4];
   B2G_HIP(hipMemcpyAsync(&h[0], dst.off.p + np, sizeof(int), hipMemcpyDeviceToHost, w->stream));
-  B2G_HIP(hipMemcpyAsync(&h[1], flags, 2 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
+  B2G_HIP(hipMemcpyAsync(&h[1], flags, 3 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
   B2G_HIP(hipStreamSynchronize(w->stream));
   if ((st = kt_harvest(w))) return st;
   if (h[1]) {
     set_error("broadphase: BVH traversal stack overflow");
     return B2G_ERR_CAPACITY;
+  }
+  if (h[3]) {
+    set_error("device scan: a tile waited too long for its predecessor");
+    return B2G_ERR_DEVICE;
   }
   int total = h[0];
   kt_aux(w, B2G_KERNEL_TRAVERSE, total);

@@ -191,6 +191,11 @@ struct BroadphaseArrays {
   DevArray<int> slots;           // first pairs of each query, column-major (slot j of query k at j * n + k)
   DevArray<float4> bounds;       // scene bounds and flags
   DevArray<int> scan_tmp;        // block sums for scans
+  // single-pass scan: per tile its published (epoch, flag, sum) word; [0] tile ticket, [1] error
+  DevArray<unsigned long long> scan_state;
+  DevArray<int> scan_ctl;
+  unsigned scan_epoch = 0;
+  bool scan_ctl_ready = false;
 };
 
 struct StepProfile {

@@ -90,6 +90,93 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int* __restrict
   }
 }
 
+// Single-pass scan with decoupled look-back: each block takes the next tile in launch order
+// from a ticket counter, scans it, publishes its sum (flag A), sums the published values of the
+// tiles before it, walking back until one that carries its inclusive prefix (flag P), then
+// publishes its own prefix and writes its outputs. A tile only waits for tiles with smaller
+// tickets, which are already running and publish their sums before they look back themselves,
+// so the wait always ends. Published words carry the call's epoch in their top bits, so the
+// states never need clearing between calls. The spin is bounded: past the bound the block sets
+// the error flag (ctl[1], read back with the broadphase's counters) instead of hanging.
+static constexpr unsigned long long kScanFlagA = 1ull << 32, kScanFlagP = 2ull << 32;
+static constexpr int kScanSpinBound = 1 << 22;
+__global__ void __launch_bounds__(kScanBlock) k_scan_lookback(const int* in, int* out, int n, int* __restrict__ total,
+                                                             unsigned long long* __restrict__ state,
+                                                             int* __restrict__ ctl, unsigned epoch, int ntiles) {
+  __shared__ int tile_s, prefix_s;
+  if (threadIdx.x == 0) tile_s = atomicAdd(&ctl[0], 1);
+  __syncthreads();
+  const int tile = tile_s;
+  const int base = tile * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int idx = base + i;
+    v[i] = idx < n ? in[idx] : 0;
+    sum += v[i];
+  }
+  int agg;
+  int ex = block_exclusive(sum, &agg);
+  // the first wave publishes the tile's sum and looks back 64 tiles at a time: lane l reads tile
+  // (j - l); the window is summed up to the nearest tile that holds its inclusive prefix (or all
+  // 64 when none does and all are published), and read again while one of those is unpublished
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const unsigned long long tag = static_cast<unsigned long long>(epoch) << 34;
+    int prefix = 0;
+    if (tile > 0) {
+      if (lane == 0)
+        __hip_atomic_store(&state[tile], tag | kScanFlagA | static_cast<unsigned>(agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      int j = tile - 1, spins = 0;
+      for (;;) {
+        const int idx = j - lane;
+        unsigned long long word = 0;
+        bool valid = true, is_p = true;  // before tile 0: an empty prefix
+        if (idx >= 0) {
+          word = __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          valid = (word >> 34) == epoch && (word & (3ull << 32)) != 0;
+          is_p = valid && (word & (3ull << 32)) == kScanFlagP;
+        }
+        const unsigned long long pmask = __ballot(is_p), vmask = __ballot(valid);
+        const int first_p = pmask ? __ffsll(static_cast<long long>(pmask)) - 1 : 64;
+        const unsigned long long need = first_p >= 63 ? ~0ull : ((1ull << (first_p + 1)) - 1);
+        if ((vmask & need) != need) {
+          if (++spins > kScanSpinBound) {
+            if (lane == 0) ctl[1] = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        int val = (lane <= first_p && idx >= 0) ? static_cast<int>(static_cast<unsigned>(word)) : 0;
+        for (int d = 32; d >= 1; d >>= 1) val += __shfl_xor(val, d, 64);
+        prefix += val;
+        if (first_p < 64) break;
+        j -= 64;
+      }
+    }
+    if (lane == 0) {
+      __hip_atomic_store(&state[tile], tag | kScanFlagP | static_cast<unsigned>(prefix + agg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      prefix_s = prefix;
+      if (tile == ntiles - 1) {
+        if (total) *total = prefix + agg;
+        ctl[0] = 0;  // every ticket is taken: ready for the next call
+      }
+    }
+  }
+  __syncthreads();
+  ex += prefix_s;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int idx = base + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+}
+
 // Scan of a short array by one block in one launch: thread t owns a contiguous chunk, sums it,
 // the block scans the chunk sums, and each thread writes its chunk's prefixes (reading each
 // element before writing it, so in and out may alias).
@@ -127,13 +214,23 @@ int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev) {
     B2G_HIP(hipGetLastError());
     return B2G_OK;
   }
-  int nblocks = blocks_for(n, kScanTile);
-  int st = w->bp.scan_tmp.reserve(nblocks);
-  if (st) return st;
-  int* sums = w->bp.scan_tmp.p;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, n, sums);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, out, n, sums, nblocks,
-                     total_dev);
+  const int nblocks = blocks_for(n, kScanTile);
+  BroadphaseArrays& bp = w->bp;
+  int st;
+  // the tile states are cleared when (re)allocated and when the epoch wraps; epoch 0 is never used
+  const bool fresh = bp.scan_state.cap < size_t(nblocks);
+  if ((st = bp.scan_state.reserve(nblocks)) || (st = bp.scan_ctl.reserve(2))) return st;
+  if (!bp.scan_ctl_ready) {
+    B2G_HIP(hipMemsetAsync(bp.scan_ctl.p, 0, 2 * sizeof(int), w->stream));
+    bp.scan_ctl_ready = true;
+  }
+  if (fresh || bp.scan_epoch + 1 >= (1u << 30)) {
+    B2G_HIP(hipMemsetAsync(bp.scan_state.p, 0, sizeof(unsigned long long) * bp.scan_state.cap, w->stream));
+    bp.scan_epoch = 0;
+  }
+  ++bp.scan_epoch;
+  hipLaunchKernelGGL(k_scan_lookback, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, out, n, total_dev,
+                     bp.scan_state.p, bp.scan_ctl.p, bp.scan_epoch, nblocks);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }
