@@ -420,7 +420,7 @@ int phase_synchronize(World* w); // swept + fat AABBs of non-static proxies
 int phase_toi(World* w, float dt, int vi);             // continuous collision after the solve
 void release_solve_graphs(World* w);
 // primitives
-int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev);
+int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev, int* total2 = nullptr);
 int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits);
 // stateless collide of fixture pairs under given transforms (parity entry point)
 int collide_fixture_pairs(World* w, int n, const int32_t* pairs, const float* xfa, const float* xfb,

@@ -102,7 +102,8 @@ static constexpr unsigned long long kScanFlagA = 1ull << 32, kScanFlagP = 2ull <
 static constexpr int kScanSpinBound = 1 << 22;
 __global__ void __launch_bounds__(kScanBlock) k_scan_lookback(const int* in, int* out, int n, int* __restrict__ total,
                                                              unsigned long long* __restrict__ state,
-                                                             int* __restrict__ ctl, unsigned epoch, int ntiles) {
+                                                             int* __restrict__ ctl, unsigned epoch, int ntiles,
+                                                             int* __restrict__ total2) {
   __shared__ int tile_s, prefix_s;
   if (threadIdx.x == 0) tile_s = atomicAdd(&ctl[0], 1);
   __syncthreads();
@@ -163,6 +164,7 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_lookback(const int* in, int
       prefix_s = prefix;
       if (tile == ntiles - 1) {
         if (total) *total = prefix + agg;
+        if (total2) *total2 = prefix + agg;
         ctl[0] = 0;  // every ticket is taken: ready for the next call
       }
     }
@@ -182,7 +184,8 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_lookback(const int* in, int
 // element before writing it, so in and out may alias).
 static constexpr int kOneBlock = 1024;
 static constexpr int kOneBlockMax = 4 * kOneBlock;  // longer arrays: the three-launch scan is faster
-__global__ void __launch_bounds__(kOneBlock) k_scan_one_block(const int* in, int* out, int n, int* __restrict__ total) {
+__global__ void __launch_bounds__(kOneBlock) k_scan_one_block(const int* in, int* out, int n, int* __restrict__ total,
+                                                               int* __restrict__ total2) {
   __shared__ int wsum[kOneBlock / 64];
   const int chunk = (n + kOneBlock - 1) / kOneBlock;
   const int b0 = min(n, int(threadIdx.x) * chunk), b1 = min(n, b0 + chunk);
@@ -196,6 +199,7 @@ __global__ void __launch_bounds__(kOneBlock) k_scan_one_block(const int* in, int
   for (int v = 0; v < wid; ++v) base += wsum[v];
   int ex = base + inc - s;
   if (threadIdx.x == kOneBlock - 1 && total) *total = ex + s;
+  if (threadIdx.x == kOneBlock - 1 && total2) *total2 = ex + s;
   for (int i = b0; i < b1; ++i) {
     const int v = in[i];
     out[i] = ex;
@@ -203,14 +207,15 @@ __global__ void __launch_bounds__(kOneBlock) k_scan_one_block(const int* in, int
   }
 }
 
-// out[i] = sum of in[0..i); *total_dev (if given) = sum of all. in and out may alias.
-int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev) {
+// out[i] = sum of in[0..i); *total_dev and *total2 (if given) = sum of all. in and out may alias.
+int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev, int* total2) {
   if (n <= 0) {
     if (total_dev) B2G_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), w->stream));
+    if (total2) B2G_HIP(hipMemsetAsync(total2, 0, sizeof(int), w->stream));
     return B2G_OK;
   }
   if (n <= kOneBlockMax) {
-    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(kOneBlock), 0, w->stream, in, out, n, total_dev);
+    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(kOneBlock), 0, w->stream, in, out, n, total_dev, total2);
     B2G_HIP(hipGetLastError());
     return B2G_OK;
   }
@@ -230,7 +235,7 @@ int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev) {
   }
   ++bp.scan_epoch;
   hipLaunchKernelGGL(k_scan_lookback, dim3(nblocks), dim3(kScanBlock), 0, w->stream, in, out, n, total_dev,
-                     bp.scan_state.p, bp.scan_ctl.p, bp.scan_epoch, nblocks);
+                     bp.scan_state.p, bp.scan_ctl.p, bp.scan_epoch, nblocks, total2);
   B2G_HIP(hipGetLastError());
   return B2G_OK;
 }
@@ -304,6 +309,9 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const unsigned* __
 
 // Sorts n (key, value) pairs by the low `bits` bits of the key, stable. The result ends in
 // keys/vals (the alternate buffers are scratch).
+// Sorted (keys, vals) end in the given arrays; when those are the world's bp.keys / bp.vals and an
+// odd number of passes leaves the result in the alternate buffers, the buffers are swapped
+// instead of copied (callers read bp.keys.p / bp.vals.p again after the call).
 int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits) {
   if (n <= 1) return B2G_OK;
   BroadphaseArrays& bp = w->bp;
@@ -326,8 +334,14 @@ int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits) 
     std::swap(v0, v1);
   }
   if (k0 != keys) {
-    B2G_HIP(hipMemcpyAsync(keys, k0, n * sizeof(unsigned), hipMemcpyDeviceToDevice, w->stream));
-    B2G_HIP(hipMemcpyAsync(vals, v0, n * sizeof(unsigned), hipMemcpyDeviceToDevice, w->stream));
+    if (keys == bp.keys.p && vals == bp.vals.p) {
+      // the result is in the alternate buffers: they become the world's keys and values
+      std::swap(bp.keys, bp.keys_alt);
+      std::swap(bp.vals, bp.vals_alt);
+    } else {
+      B2G_HIP(hipMemcpyAsync(keys, k0, n * sizeof(unsigned), hipMemcpyDeviceToDevice, w->stream));
+      B2G_HIP(hipMemcpyAsync(vals, v0, n * sizeof(unsigned), hipMemcpyDeviceToDevice, w->stream));
+    }
   }
   B2G_HIP(hipGetLastError());
   return B2G_OK;
