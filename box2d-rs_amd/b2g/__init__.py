@@ -46,7 +46,7 @@ JOINT_DEF = np.dtype([("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("c
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
                      ("angular_velocity", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),
-                     ("gravity_scale", "<f4"), ("pad", "<f4")])
+                     ("gravity_scale", "<f4"), ("bullet", "<i4")])
 SHAPE_DEF = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", "<f4"), ("params", "<f4", 36)])
 FIXTURE_DEF = np.dtype([("friction", "<f4"), ("restitution", "<f4"), ("restitution_threshold", "<f4"),
                         ("density", "<f4"), ("is_sensor", "<i4"), ("category", "<u2"), ("mask", "<u2"),
@@ -112,6 +112,7 @@ def _load():
         "b2g_world_get_profile": ([P, P], I),
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
         "b2g_world_set_fixture_frictions": ([P, P, I], I),
+        "b2g_world_set_bullet": ([P, I, I], I),
         "b2g_collide_pairs_noalloc": ([P, I, P, P, P, P], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
         "b2g_world_time_kernel": ([P, I], I),
@@ -443,6 +444,10 @@ class World:
         fr = np.ascontiguousarray(friction, np.float32)
         _check(_lib.b2g_world_set_fixture_frictions(self._h, _ptr(fr), len(fr)))
 
+    def set_bullet(self, body, flag=True):
+        """B2body::set_bullet for one body (continuous collision also against dynamic bodies)."""
+        _check(_lib.b2g_world_set_bullet(self._h, int(body), int(bool(flag))))
+
     def collide_pairs(self, pairs, xf_a, xf_b):
         pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
         n = len(pairs)
@@ -600,7 +605,7 @@ class B2bodyDef:
 
     def __init__(self, body_type=B2_STATIC_BODY, position=(0.0, 0.0), angle=0.0, linear_velocity=(0.0, 0.0),
                  angular_velocity=0.0, linear_damping=0.0, angular_damping=0.0, allow_sleep=True, awake=True,
-                 fixed_rotation=False, gravity_scale=1.0):
+                 fixed_rotation=False, bullet=False, gravity_scale=1.0):
         self.body_type = body_type
         self.position = position
         self.angle = angle
@@ -611,6 +616,7 @@ class B2bodyDef:
         self.allow_sleep = allow_sleep
         self.awake = awake
         self.fixed_rotation = fixed_rotation
+        self.bullet = bullet
         self.gravity_scale = gravity_scale
 
     def _def(self):
@@ -626,6 +632,7 @@ class B2bodyDef:
         d["linear_damping"] = self.linear_damping
         d["angular_damping"] = self.angular_damping
         d["gravity_scale"] = self.gravity_scale
+        d["bullet"] = int(self.bullet)
         return d
 
 
@@ -669,6 +676,13 @@ class B2body:
 
     def is_awake(self):
         return bool(self._state()["awake"] != 0)
+
+    def set_bullet(self, flag):
+        """B2body::set_bullet (src/b2_body.rs:504-506)."""
+        self._world._set_bullet(self.index, bool(flag))
+
+    def is_bullet(self):
+        return self._world._bullets.get(self.index, False)
 
     def get_sleep_time(self):
         return float(self._state()["sleep_time"])
@@ -839,6 +853,7 @@ class B2world:
         self._bodies = []
         self._fixtures = []
         self._n_flushed_bodies = 0
+        self._bullets = {}
         self._cache = None
         # B2world::new defaults (src/private/dynamics/b2_world.rs:37-43): warm starting, sleeping
         # and continuous physics on
@@ -875,7 +890,18 @@ class B2world:
     def create_body(self, body_def):
         self._bodies.append(body_def._def())
         self._cache = None
-        return B2body(self, self._n_flushed_bodies + len(self._bodies) - 1)
+        index = self._n_flushed_bodies + len(self._bodies) - 1
+        if body_def.bullet:
+            self._bullets[index] = True
+        return B2body(self, index)
+
+    def _set_bullet(self, index, flag):
+        k = index - self._n_flushed_bodies
+        if k >= 0:
+            self._bodies[k]["bullet"] = int(flag)
+        else:
+            self._world.set_bullet(index, flag)
+        self._bullets[index] = flag
 
     def _queue_fixture(self, body_index, fixture_def):
         if fixture_def.shape is None:

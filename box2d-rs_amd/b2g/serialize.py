@@ -254,7 +254,7 @@ def defs_to_json(gravity, body_defs, fixture_body, shape_defs, fixture_defs, sta
             # B2body::new sets the awake flag only on non-static bodies (src/private/dynamics/b2_body.rs:34-36)
             "awake": bool(awake) and int(d["type"]) != 0,
             "fixed_rotation": bool(d["fixed_rotation"]),
-            "bullet": False,
+            "bullet": bool(d["bullet"]),
             "enabled": True,
             "user_data": None,
             "gravity_scale": F32(d["gravity_scale"]),
@@ -394,6 +394,7 @@ def read_world(text, joint_bodies="reference"):
         bd[i]["allow_sleep"] = int(bool(d["allow_sleep"]))
         bd[i]["awake"] = int(bool(d["awake"]))
         bd[i]["fixed_rotation"] = int(bool(d["fixed_rotation"]))
+        bd[i]["bullet"] = int(bool(d.get("bullet", False)))
         bd[i]["gravity_scale"] = _parse_f32_any(d["gravity_scale"])
         for fx in body.get("m_fixture_list", []):
             sds.append(_read_shape(fx["m_shape_type"], fx["m_shape"]))

@@ -192,6 +192,7 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_
     if (d.type != B2G_STATIC && !awake) w->created_asleep = true;
     w->h_awake.push_back(awake);
     w->h_auto_sleep.push_back(d.allow_sleep ? 1 : 0);
+    w->h_bullet.push_back(d.bullet ? 1 : 0);
     w->h_sleep_time.push_back(0.0f);
     w->h_mass.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     w->h_props.push_back(make_float4(0.0f, d.gravity_scale, d.linear_damping, d.angular_damping));
@@ -393,6 +394,7 @@ int upload_world(World* w) {
       (st = up(w->f.shapev, w->h_shapev, w->stream)) || (st = up(w->f.world, w->h_fworld, w->stream)) ||
       (st = up(w->b.awake, w->h_awake, w->stream)) || (st = up(w->b.sleep_time, w->h_sleep_time, w->stream)) ||
       (st = up(w->b.auto_sleep, w->h_auto_sleep, w->stream)) || (st = up(w->b.wake, std::vector<int>(w->nb, 0), w->stream)) ||
+      (st = up(w->b.bullet, w->h_bullet, w->stream)) ||
       (st = w->f.moved.reserve(std::max(w->np, 1))))
     return st;
   // Big static proxies (ground, container walls: an AABB side over kLooseExtent) stay out of
@@ -572,7 +574,7 @@ int b2g_world_destroy(b2g_world* h) {
   (void)hipStreamSynchronize(w->stream);
   auto rel = [](auto& a) { a.release(); };
   rel(w->b.pos); rel(w->b.vel); rel(w->b.pos0); rel(w->b.xf); rel(w->b.mass); rel(w->b.props); rel(w->b.force); rel(w->b.awake); rel(w->b.sleep_time); rel(w->b.auto_sleep); rel(w->b.wake);
-  rel(w->b.type);
+  rel(w->b.type); rel(w->b.bullet);
   rel(w->f.body); rel(w->f.shape); rel(w->f.mat); rel(w->f.filter); rel(w->f.aabb); rel(w->f.fat); rel(w->f.moved);
   rel(w->f.shapev); rel(w->f.world); rel(w->f.tree_list); rel(w->f.loose_off); rel(w->f.loose);
   for (auto& c : w->c) {
@@ -581,7 +583,7 @@ int b2g_world_destroy(b2g_world* h) {
   for (auto& t : w->toi_state) {
     rel(t.toi); rel(t.count); rel(t.flags);
   }
-  rel(w->toi_alpha0); rel(w->toi_sel); rel(w->toi_two); rel(w->toi_ctl);
+  rel(w->toi_alpha0); rel(w->toi_ctl);
   rel(w->toi_cand); rel(w->toi_need); rel(w->toi_rank); rel(w->toi_snap);
   rel(w->toi_cadj_off); rel(w->toi_cadj); rel(w->toi_cadj_cur); rel(w->toi_bfix_off); rel(w->toi_bfix);
   rel(w->toi_bfix_cur); rel(w->toi_cl); rel(w->toi_in_list);
@@ -815,6 +817,21 @@ int b2g_world_set_fixture_frictions(b2g_world* h, const float* friction, int32_t
   for (int i = 0; i < n; ++i) w->h_fmat[i].x = friction[i];
   if ((st = up(w->f.mat, w->h_fmat, w->stream))) return st;
   B2G_HIP(hipStreamSynchronize(w->stream));
+  return B2G_OK;
+}
+
+int b2g_world_set_bullet(b2g_world* h, int32_t body, int32_t flag) {
+  if (!h || body < 0 || body >= h->w.nb) {
+    set_error("b2g_world_set_bullet: invalid body");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  w->h_bullet[body] = flag ? 1 : 0;
+  w->body_defs[body].bullet = flag ? 1 : 0;
+  if (w->b.bullet.p && body < w->dev_nb) {
+    B2G_HIP(hipMemcpyAsync(w->b.bullet.p + body, &w->h_bullet[body], sizeof(int), hipMemcpyHostToDevice, w->stream));
+    B2G_HIP(hipStreamSynchronize(w->stream));
+  }
   return B2G_OK;
 }
 

@@ -82,6 +82,7 @@ struct BodyArrays {
   DevArray<float> sleep_time;
   DevArray<int> auto_sleep;  // the body def's allow_sleep
   DevArray<int> wake;        // wake request from the collide phase (touching changed, touching contact gone)
+  DevArray<int> bullet;      // B2body's bullet flag: continuous collision also against dynamic bodies
 };
 
 // Device-side proxy arrays (one proxy per fixture; index = fixture creation order, which is
@@ -290,7 +291,7 @@ struct World {
   std::vector<int> h_tree, h_loose_off, h_loose, h_color_off_dev;
   int n_worlds = 1;
   std::vector<ShapeV> h_shapev;
-  std::vector<int> h_awake, h_auto_sleep;
+  std::vector<int> h_awake, h_auto_sleep, h_bullet;
   std::vector<float> h_sleep_time;
   std::vector<HostShape> h_shapes;
   int nb = 0, np = 0;          // bodies, proxies on the host
@@ -311,7 +312,6 @@ struct World {
   DevArray<int> toi_succ, toi_drop_flag, toi_drop_pos, toi_drop_total;
   ContactArrays toi_dropped;
   DevArray<float> toi_alpha0;         // per body: the sweep's alpha0 during the TOI pass
-  DevArray<int> toi_sel, toi_two;     // bodies re-synchronised after a TOI event; {0, 1}
   DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies, counters
   DevArray<int> toi_cand, toi_need, toi_rank;  // TOI event candidates; contacts to recompute, ranked
   DevArray<float> toi_snap;                    // aligned sweeps of the contacts to recompute

@@ -10,20 +10,17 @@
 // Here:
 // * k_toi_first (grid): at the start of the pass every body's sweep starts at alpha0 = 0, so
 //   each contact's time of impact is independent of the others: one thread per contact.
-// * k_toi_min (grid): the smallest (alpha, contact index) over the contacts with a cached time of
-//   impact, as one 64-bit atomicMin of the float's bits (alpha >= 0) over the index. The
-//   reference takes the first contact of its list among equal times; the GPU's contact set is
-//   ordered by its own (deterministic) pair order instead of the reference's list order.
-// * k_toi_cand (grid) lists the contacts of the event's two bodies; k_toi_event (one workgroup)
-//   runs the event itself, sequential as in the reference: advance, update, island gathering over
-//   those contacts in set order (the two bodies' contact lists), the island's TOI solve, the new
-//   sweeps and velocities. k_toi_post clears the TOI flags of the displaced bodies' contacts.
-// * the host re-synchronises the displaced bodies' proxies and rebuilds the contact set
-//   (find_new_contacts) when one of them was re-fattened (only moved proxies can add pairs).
-// * k_toi_need (grid) lists the contacts whose time of impact is no longer cached, and
-//   k_toi_rescan (one workgroup) ranks them into set order, does the reference's sweep alignment
-//   (the body with the earlier alpha0 is advanced to the other's) on one thread in that order,
-//   and computes their times of impact in parallel.
+// * k_toi_loop (one persistent workgroup) runs the events (see the device event loop below): the
+//   smallest cached time over a candidate list, the event itself sequential as in the reference
+//   (advance, update, island gathering over the event bodies' contacts in set order, the island's
+//   TOI solve, the new sweeps and velocities), the displaced bodies' TOI flags cleared and proxies
+//   re-synchronised, the lost times of impact recomputed. It returns to the host when a proxy was
+//   re-fattened: the host rebuilds the contact set (find_new_contacts), lists the contacts whose
+//   time is not cached (k_toi_need) and recomputes them (k_toi_rescan: ranked into set order, the
+//   reference's sweep alignment on one thread in that order, the times in parallel).
+// * Bullets: a contact gets a time of impact when one of its bodies is static, kinematic or a
+//   bullet, and a bullet's island gathers dynamic bodies too (src/private/dynamics/b2_world.rs:
+//   616-621, 770-777).
 // Time of impact: GJK distance (src/private/collision/b2_distance.rs:404-530) and conservative
 // advancement with separating axes (src/private/collision/b2_time_of_impact.rs:17-463).
 #include "b2g_internal.h"
@@ -36,7 +33,6 @@ namespace {
 
 constexpr int kToiBlock = 256;
 constexpr int kRescanBlock = 1024;
-constexpr int kEventBlock = 256;
 constexpr int kMaxToiBodies = 64;      // bodies of one TOI island (two plus static/kinematic neighbours)
 constexpr int kMaxToiContacts = 64;    // contacts of one TOI island
 constexpr int kMaxBodyContacts = 512;  // contacts of one body looked at when the island is gathered
@@ -417,6 +413,7 @@ struct ToiArgs {
   const float4* fmat;    // friction, restitution, threshold
   const int4* fshape;    // radius bits in .z
   const int* btype;
+  const int* bullet;     // per body: B2body's bullet flag
   float4* pos;           // sweep c, a
   float4* pos0;          // sweep c0, a0
   float4* vel;
@@ -471,7 +468,7 @@ __device__ inline void set_awake(const ToiArgs& a, int b) {
 }
 
 // The reference's conditions for a contact to get a time of impact (src/private/dynamics/
-// b2_world.rs:580-621); bullets are not part of this path, so one body must be static or kinematic.
+// b2_world.rs:580-621): one of its bodies awake, and one static, kinematic or a bullet.
 __device__ inline bool toi_candidate(const ToiArgs& a, int k, int* bA, int* bB) {
   const int f = a.tflags[k];
   if ((f & kToiDisabled) || a.tcount[k] > kMaxSubSteps) return false;
@@ -479,7 +476,7 @@ __device__ inline bool toi_candidate(const ToiArgs& a, int k, int* bA, int* bB) 
   *bA = a.fbody[fab.x];
   *bB = a.fbody[fab.y];
   if (!is_awake(a, *bA) && !is_awake(a, *bB)) return false;
-  return a.btype[*bA] != 2 || a.btype[*bB] != 2;
+  return a.btype[*bA] != 2 || a.btype[*bB] != 2 || a.bullet[*bA] || a.bullet[*bB];
 }
 
 __device__ inline float toi_alpha(const ToiArgs& a, int k, const SweepD& sa, const SweepD& sb, float alpha0) {
@@ -504,17 +501,6 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_first(ToiArgs a) {
   sb.alpha0 = 0.0f;
   a.toi[k] = toi_alpha(a, k, sa, sb, 0.0f);
   a.tflags[k] = kToiValid;
-}
-
-__global__ void __launch_bounds__(kToiBlock) k_toi_min(ToiArgs a) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
-  if (a.tflags[k] != kToiValid || a.tcount[k] > kMaxSubSteps) return;
-  const float alpha = a.toi[k];
-  if (!(alpha < 1.0f)) return;
-  const unsigned long long key =
-      (static_cast<unsigned long long>(__float_as_uint(alpha)) << 32) | static_cast<unsigned>(k);
-  atomicMin(a.key, key);
 }
 
 // Later scans: contacts whose time of impact is not cached. k_toi_need lists them (any order),
@@ -593,26 +579,6 @@ __device__ void toi_rescan_wg(const ToiArgs& a) {
   }
 }
 __global__ void __launch_bounds__(kRescanBlock) k_toi_rescan(ToiArgs a) { toi_rescan_wg(a); }
-
-// The contacts of the coming event's two bodies (dynamic ones only), in any order; the event
-// kernel sorts them into set order.
-__global__ void __launch_bounds__(kToiBlock) k_toi_cand(ToiArgs a) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
-  const int km = static_cast<int>(*a.key & 0xffffffffu);
-  const int2 fm = a.ab[km];
-  const int bA = a.fbody[fm.x], bB = a.fbody[fm.y];
-  const int2 fab = a.ab[k];
-  const int x = a.fbody[fab.x], y = a.fbody[fab.y];
-  if (a.btype[bA] == 2 && (x == bA || y == bA)) {
-    const int slot = atomicAdd(&a.flags4[2], 1);
-    if (slot < kMaxBodyContacts) a.cand[slot] = k;
-  }
-  if (a.btype[bB] == 2 && (x == bB || y == bB)) {
-    const int slot = atomicAdd(&a.flags4[3], 1);
-    if (slot < kMaxBodyContacts) a.cand[kMaxBodyContacts + slot] = k;
-  }
-}
 
 // B2contact::update for one contact (src/private/dynamics/b2_contact.rs:103-217) with the bodies'
 // current transforms; returns whether it is touching.
@@ -803,7 +769,8 @@ __device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alp
       if (in) continue;
       const int2 fab = a.ab[k];
       const int other = a.fbody[fab.x] == body ? a.fbody[fab.y] : a.fbody[fab.x];
-      if (a.btype[other] == 2) continue;
+      // only static, kinematic or bullet bodies join (src/private/dynamics/b2_world.rs:770-777)
+      if (a.btype[other] == 2 && !a.bullet[body] && !a.bullet[other]) continue;
       const int slot = island_slot(s, other);
       const SweepD backup = load_sweep(a, other);
       SweepD so = backup;
@@ -889,32 +856,6 @@ __device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alp
   *a.status = kEventSolved;
 }
 
-__global__ void __launch_bounds__(kEventBlock) k_toi_event(ToiArgs a) {
-  __shared__ EventShared s;
-  __shared__ EventCtl e;
-  toi_event(a, s, e);
-}
-
-// After a solved event: clear the TOI flags of every contact of the event's dynamic bodies (the
-// island's other bodies are static or kinematic; src/private/dynamics/b2_world.rs:871-891) and
-// select those bodies for the proxy synchronisation that follows (sel[b] = 1: a view in which
-// only they are awake). After an event that was not solid nothing is selected.
-__global__ void __launch_bounds__(kToiBlock) k_toi_post(ToiArgs a, int* __restrict__ sel) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool solved = *a.status == kEventSolved;
-  const int2 eb = *a.event_bodies;
-  const int dA = solved && a.btype[eb.x] == 2 ? eb.x : -1, dB = solved && a.btype[eb.y] == 2 ? eb.y : -1;
-  if (k == 0) {
-    a.flags4[0] = 0;
-    a.flags4[1] = 0;
-  }
-  if (k < a.nb) sel[k] = (k == dA || k == dB) ? 1 : 0;
-  if (k >= a.n || !solved) return;
-  const int2 fab = a.ab[k];
-  const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
-  if (bA == dA || bA == dB || bB == dA || bB == dB) a.tflags[k] &= ~kToiValid;
-}
-
 // TOI state of a rebuilt contact set from its predecessors (new pairs start uncached).
 __global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __restrict__ old_index,
                                                          const float* __restrict__ otoi, const int* __restrict__ ocount,
@@ -937,9 +878,8 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __res
 // times of impact of the contacts that lost theirs (toi_rescan_wg) and adds those below 1 to the
 // list. It returns to the host only when no event is left, when an event re-fattened a proxy
 // (the host rebuilds the contact set, as find_new_contacts would add pairs), on a capacity error,
-// or after max_events events. The kernels it replaces (k_toi_min, k_toi_cand, k_toi_post, the
-// proxy synchronisation, k_toi_need) scanned every contact or proxy per event; the lists make
-// each event's work proportional to the event's own contacts.
+// or after max_events events. Per event, the lists keep the work proportional to the event's own
+// contacts instead of scans over every contact or proxy.
 constexpr int kLoopBlock = 512;  // (512 threads leave the event code 256 registers per thread)
 constexpr int kToiQueued = 4;  // TOI flag: listed for recomputation in this event
 enum : int { kLoopDone = 0, kLoopRebuild = 1, kLoopCapacity = 2, kLoopMore = 3 };
@@ -1105,10 +1045,11 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
     }
     __syncthreads();
     if (status == kEventSolved) {
-      // clear the TOI flags of every contact of the event's dynamic bodies
-      // (src/private/dynamics/b2_world.rs:871-891), then re-synchronise their proxies
-      for (int side = 0; side < 2; ++side) {
-        const int b = ev[side];
+      // clear the TOI flags of every contact of the island's dynamic bodies (the event's, and with
+      // a bullet the dynamic bodies it gathered; src/private/dynamics/b2_world.rs:871-891), then
+      // re-synchronise their proxies
+      for (int ib = 0; ib < nbody; ++ib) {
+        const int b = bodies[ib];
         if (a.btype[b] != 2) continue;
         const int e0 = L.cadj_off[b], e1 = L.cadj_off[b + 1];
         for (int i = e0 + t; i < e1; i += kLoopBlock) a.tflags[L.cadj[i]] &= ~kToiValid;
@@ -1165,6 +1106,7 @@ static ToiArgs toi_args(World* w, float dt, int vi) {
   a.fmat = w->f.mat.p;
   a.fshape = w->f.shape.p;
   a.btype = w->b.type.p;
+  a.bullet = w->b.bullet.p;
   a.pos = w->b.pos.p;
   a.pos0 = w->b.pos0.p;
   a.vel = w->b.vel.p;
@@ -1197,15 +1139,9 @@ static int toi_reserve(World* w) {
   if ((st = w->toi_need.reserve(n)) || (st = w->toi_rank.reserve(n)) || (st = w->toi_snap.reserve(19 * n)) ||
       (st = w->toi_cand.reserve(2 * kMaxBodyContacts)))
     return st;
-  if ((st = w->toi_alpha0.reserve(std::max(w->nb, 1))) || (st = w->toi_sel.reserve(std::max(w->nb, 1))) ||
-      (st = w->toi_ctl.reserve(7)) || (st = w->toi_cl.reserve(n)) || (st = w->toi_in_list.reserve(n)))
+  if ((st = w->toi_alpha0.reserve(std::max(w->nb, 1))) || (st = w->toi_ctl.reserve(7)) ||
+      (st = w->toi_cl.reserve(n)) || (st = w->toi_in_list.reserve(n)))
     return st;
-  if (!w->toi_two.p) {
-    if ((st = w->toi_two.reserve(2))) return st;
-    const int two[2] = {0, 1};
-    B2G_HIP(hipMemcpyAsync(w->toi_two.p, two, sizeof(two), hipMemcpyHostToDevice, w->stream));
-    B2G_HIP(hipStreamSynchronize(w->stream));
-  }
   return B2G_OK;
 }
 

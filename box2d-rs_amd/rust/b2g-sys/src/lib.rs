@@ -54,7 +54,7 @@ pub const B2G_KERNEL_COUNT: i32 = 7;
 pub struct b2g_body_def {
     pub type_: i32, pub allow_sleep: i32, pub awake: i32, pub fixed_rotation: i32,
     pub position: [f32; 2], pub angle: f32, pub linear_velocity: [f32; 2], pub angular_velocity: f32,
-    pub linear_damping: f32, pub angular_damping: f32, pub gravity_scale: f32, pub pad: f32,
+    pub linear_damping: f32, pub angular_damping: f32, pub gravity_scale: f32, pub bullet: i32,
 }
 #[repr(C)] #[derive(Clone, Copy)]
 pub struct b2g_shape_def { pub type_: i32, pub count: i32, pub flags: i32, pub radius: f32, pub params: [f32; 36] }
@@ -115,6 +115,7 @@ extern "C" {
     pub fn b2g_world_set_body_states(w: *mut b2g_world, inp: *const b2g_body_state, n: i32) -> c_int;
     pub fn b2g_world_get_mass(w: *mut b2g_world, out: *mut f32, n: i32) -> c_int;
     pub fn b2g_world_set_fixture_frictions(w: *mut b2g_world, friction: *const f32, n: i32) -> c_int;
+    pub fn b2g_world_set_bullet(w: *mut b2g_world, body: i32, flag: i32) -> c_int;
     pub fn b2g_world_get_fat_aabbs(w: *mut b2g_world, out: *mut f32, n: i32) -> c_int;
     pub fn b2g_world_set_fat_aabbs(w: *mut b2g_world, inp: *const f32, n: i32) -> c_int;
     pub fn b2g_world_get_contacts(w: *mut b2g_world, cap: i32, fixtures_ab: *mut i32, flags: *mut i32,

@@ -51,7 +51,7 @@ typedef struct b2g_body_def {
   float linear_velocity[2];
   float angular_velocity;
   float linear_damping, angular_damping, gravity_scale;
-  float pad;
+  int32_t bullet; /* B2bodyDef::bullet: continuous collision also against other dynamic bodies */
 } b2g_body_def;
 
 /* Shape definition: B2circleShape / B2edgeShape::set_two_sided, set_one_sided /
@@ -206,6 +206,9 @@ int b2g_world_get_mass(b2g_world* world, float* out, int32_t n);
  * step here, so existing contacts take the new values at the next step, where the reference keeps
  * a contact's mixed friction until B2contact::reset_friction. Refused while a tape records. */
 int b2g_world_set_fixture_frictions(b2g_world* world, const float* friction, int32_t n);
+/* B2body::set_bullet (src/b2_body.rs:504-506): a bullet gets continuous collision also against other
+ * dynamic bodies (src/private/dynamics/b2_world.rs:616-621, 770-777). */
+int b2g_world_set_bullet(b2g_world* world, int32_t body, int32_t flag);
 
 /* Fat AABB of every proxy (B2broadPhase::get_fat_aabb, src/b2_broad_phase.rs) as
  * lower.x, lower.y, upper.x, upper.y. */

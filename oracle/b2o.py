@@ -21,7 +21,7 @@ def _dtypes(real):
     body_def = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                          ("position", real, 2), ("angle", real), ("linear_velocity", real, 2),
                          ("angular_velocity", real), ("linear_damping", real), ("angular_damping", real),
-                         ("gravity_scale", real), ("pad", real)], align=True)
+                         ("gravity_scale", real), ("bullet", "<i4")], align=True)
     shape_def = np.dtype([("type", "<i4"), ("count", "<i4"), ("flags", "<i4"), ("radius", real),
                           ("params", real, 36)], align=True)
     fixture_def = np.dtype([("friction", real), ("restitution", real), ("restitution_threshold", real),
@@ -82,6 +82,7 @@ class Oracle:
         L.b2o_world_get_bodies.argtypes = [P, P]
         L.b2o_world_set_bodies.argtypes = [P, P]
         L.b2o_world_set_awake.argtypes = [P, I, I]
+        L.b2o_world_set_bullet.argtypes = [P, I, I]
         L.b2o_world_get_mass.argtypes = [P, P]
         L.b2o_world_get_proxies.argtypes = [P, P, P]
         L.b2o_world_get_contacts.argtypes = [P, I, P, P, P, P]
@@ -301,6 +302,10 @@ class OracleWorld:
     def set_awake(self, body, flag=True):
         """B2body::set_awake on one body (src/b2_body.rs:783-801)."""
         self.o.lib.b2o_world_set_awake(self.h, int(body), int(flag))
+
+    def set_bullet(self, body, flag=True):
+        """B2body::set_bullet."""
+        self.o.lib.b2o_world_set_bullet(self.h, int(body), int(flag))
 
     def mass(self):
         out = np.zeros((self.body_count, 6), self.o.real)

@@ -26,7 +26,7 @@ struct B2oBodyDef {
   real linear_velocity[2];
   real angular_velocity;
   real linear_damping, angular_damping, gravity_scale;
-  real pad;
+  int32_t bullet;
 };
 
 // Shape kinds of B2oShapeDef::type
@@ -372,6 +372,7 @@ int b2o_world_add(void* wp, int n, const B2oBodyDef* bodies, int nf, const int32
     w->create_body(d.type, Vec2(d.position[0], d.position[1]), d.angle,
                    Vec2(d.linear_velocity[0], d.linear_velocity[1]), d.angular_velocity, d.linear_damping,
                    d.angular_damping, d.gravity_scale, d.allow_sleep != 0, d.awake != 0, d.fixed_rotation != 0);
+    w->bodies.back()->bullet = d.bullet != 0;
   }
   int status = 0;
   for (int k = 0; k < nf; ++k) {
@@ -524,6 +525,12 @@ void b2o_world_get_mass(void* wp, real* out) {
 void b2o_world_set_awake(void* wp, int body, int flag) {
   World* w = static_cast<World*>(wp);
   if (body >= 0 && body < int(w->bodies.size())) w->bodies[body]->set_awake(flag != 0);
+}
+
+// B2body::set_bullet (src/b2_body.rs:504-506, 775-777)
+void b2o_world_set_bullet(void* wp, int body, int flag) {
+  World* w = static_cast<World*>(wp);
+  if (body >= 0 && body < int(w->bodies.size())) w->bodies[body]->bullet = flag != 0;
 }
 
 void b2o_world_set_bodies(void* wp, const B2oBodyState* in) {

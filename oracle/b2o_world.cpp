@@ -1010,8 +1010,8 @@ static void solve_toi_island(World* w, IslandIO& island, const TimeStep& sub, in
   }
 }
 
-// src/private/dynamics/b2_world.rs:534-901 (bullets are not part of this path: every dynamic body is a
-// non-bullet, so only contacts with a static or kinematic body get a time of impact)
+// src/private/dynamics/b2_world.rs:534-901 (no sub-stepping): a contact gets a time of impact when one of
+// its bodies is static, kinematic or a bullet
 void World::solve_toi(const TimeStep& step) {
   toi_events = 0;
   for (Body* b : bodies) {
@@ -1043,8 +1043,8 @@ void World::solve_toi(const TimeStep& step) {
         bool activeA = ba->awake && ba->type != kStatic;
         bool activeB = bb->awake && bb->type != kStatic;
         if (!activeA && !activeB) continue;
-        bool collideA = ba->type != kDynamic;
-        bool collideB = bb->type != kDynamic;
+        bool collideA = ba->bullet || ba->type != kDynamic;
+        bool collideB = bb->bullet || bb->type != kDynamic;
         if (!collideA && !collideB) continue;
         real alpha0 = ba->sweep.alpha0;
         if (ba->sweep.alpha0 < bb->sweep.alpha0) {
@@ -1109,7 +1109,8 @@ void World::solve_toi(const TimeStep& step) {
         Contact* contact = ce->contact;
         if (contact->island_flag) continue;
         Body* other = ce->other;
-        if (other->type == kDynamic) continue;
+        // only static, kinematic or bullet bodies join (src/private/dynamics/b2_world.rs:770-777)
+        if (other->type == kDynamic && !body->bullet && !other->bullet) continue;
         if (contact->fa->sensor || contact->fb->sensor) continue;
         Sweep backup = other->sweep;
         if (!other->island_flag) other->advance(min_alpha);

@@ -137,6 +137,7 @@ struct Body {
   int index = -1;
   int type = kStatic;
   bool island_flag = false, awake = false, auto_sleep = true, fixed_rotation = false, enabled = true;
+  bool bullet = false;  // B2bodyDef::bullet (src/private/dynamics/b2_body.rs:25-27)
   Transform xf;
   Sweep sweep;
   Vec2 v;

@@ -19,6 +19,10 @@ same in the reference's list order and the GPU's contact-set order:
   * fast boxes dropped into a static corner (a TOI island touching two static fixtures) and fast
     balls landing at distinct times on one moving kinematic platform (events sharing a kinematic
     body);
+  * bullets (B2bodyDef::bullet, B2body::set_bullet): balls and small boxes at 100-142 m/s against
+    free dynamic planks and two bullets meeting head-on, no gravity (each TOI island is one
+    constraint): times of impact against dynamic bodies, states and event counts as the oracle's;
+    without the flag they pass through;
   * determinism: two runs of a 40-box rain with sleeping on are bit-identical; a 1024-body mixed
     world runs 30 steps with continuous physics with finite states and nothing falling out.
 """
@@ -396,3 +400,86 @@ def test_events_sharing_a_kinematic_platform(oracle):
     w, o = _pair(oracle, shared_platform_scene(), True)
     st = _compare(w, o, 40)
     assert (st["c"][1:, 1] > st[0]["c"][1]).all(), (st["c"][1:, 1], st[0]["c"][1])
+
+
+def bullet_scene(n=6):
+    """No gravity: n free dynamic planks (0.1 x 2 m) at different heights and distances, a fast
+    bullet (balls and small boxes in turn, 100-142 m/s) aimed at each, and two bullets flying at
+    each other. Every contact is between two bodies that touch nothing else, so each TOI island
+    is one constraint and the solve order does not matter."""
+    m = 2 * n + 2
+    bd = np.zeros(m, b2g.BODY_DEF)
+    bd["type"] = 2
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    sd = np.zeros(m, b2g.SHAPE_DEF)
+    for i in range(n):
+        bd[i]["position"] = (5.0 + 1.3 * i, 3.0 * i)
+        sd[i]["type"] = b2g.SHAPE_BOX
+        sd[i]["params"][:2] = (0.05, 1.0)
+        b = bd[n + i]
+        b["position"] = (0.0, 3.0 * i + 0.2 * (i % 3))
+        b["linear_velocity"] = (100.0 + 7.0 * i, 0.0)
+        b["angular_velocity"] = 2.0 * (i - n / 2)
+        b["bullet"] = 1
+        if i % 2:
+            sd[n + i]["type"] = b2g.SHAPE_BOX
+            sd[n + i]["params"][:2] = (0.12, 0.08)
+        else:
+            sd[n + i]["type"] = b2g.SHAPE_CIRCLE
+            sd[n + i]["radius"] = 0.1
+    for k, (x, v) in enumerate(((-3.0, 90.0), (3.0, -80.0))):
+        b = bd[2 * n + k]
+        b["position"] = (x, -4.0)
+        b["linear_velocity"] = (v, 0.0)
+        b["bullet"] = 1
+        sd[2 * n + k]["type"] = b2g.SHAPE_CIRCLE
+        sd[2 * n + k]["radius"] = 0.15
+    fd = _fixtures(m, 0.3)
+    return bd, np.arange(m, dtype=np.int32), sd, fd, (0.0, 0.0)
+
+
+def test_bullets_against_dynamic_bodies(oracle):
+    """Bullets get times of impact against dynamic bodies (src/private/dynamics/b2_world.rs:616-621):
+    every step's states and TOI event counts agree with the oracle's (every bullet hits its plank,
+    the two bullets meet), and without the flag they pass through on both."""
+    scene = bullet_scene()
+    n = 6
+    w, o = _pair(oracle, scene, True)
+    events = 0
+    for _ in range(12):
+        _compare(w, o, 1)
+        events += o.toi_events
+    assert events >= n + 1  # every bullet hits its plank, and the two bullets meet
+    bd = scene[0].copy()
+    bd["bullet"] = 0
+    w2, o2 = _pair(oracle, (bd,) + tuple(scene[1:]), True)
+    st2 = _compare(w2, o2, 12)
+    assert (st2["c"][n:2 * n, 0] > st2["c"][:n, 0]).all()  # all passed through
+
+
+def test_reference_style_bullet():
+    """B2bodyDef::bullet and B2body::set_bullet through the reference-style API."""
+    def run(flag, late):
+        world = b2g.B2world((0.0, 0.0))
+        plank = world.create_body(b2g.B2bodyDef(body_type=b2g.B2_DYNAMIC_BODY, position=(5.0, 0.0)))
+        box = b2g.B2polygonShape()
+        box.set_as_box(0.05, 1.0)
+        plank.create_fixture_by_shape(box, 1.0)
+        ball = world.create_body(b2g.B2bodyDef(body_type=b2g.B2_DYNAMIC_BODY, linear_velocity=(120.0, 0.0),
+                                               bullet=flag and not late))
+        circle = b2g.B2circleShape()
+        circle.m_radius = 0.1
+        ball.create_fixture_by_shape(circle, 1.0)
+        world.step(1.0 / 60.0, 8, 3)  # 2 m: still short of the plank
+        if late:
+            ball.set_bullet(flag)
+        assert ball.is_bullet() == flag
+        for _ in range(5):
+            world.step(1.0 / 60.0, 8, 3)
+        return float(ball.get_position().x), float(plank.get_position().x)
+    for late in (False, True):
+        x, px = run(True, late)
+        assert x < px, (x, px)
+        x, px = run(False, late)
+        assert x > px, (x, px)

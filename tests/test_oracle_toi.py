@@ -315,3 +315,40 @@ def test_fast_boxes_rain_on_thin_ground(oracle):
             assert (y > 0.05 + 0.2 - 0.02).all() and (y < 0.05 + 0.2 + 0.3).all(), y
         else:
             assert events == 0 and (y < -0.05).sum() >= n // 2, y
+
+
+def _bullet_scene(bullet):
+    """A ball of radius 0.1 at 120 m/s (2 m per step) towards a free dynamic plank 0.1 m thick and
+    4 m tall, no gravity: only a bullet gets a time of impact against a dynamic body."""
+    bd, fb, sd, fd = _wall_scene(120.0)
+    bd[0]["type"] = 2
+    bd[0]["gravity_scale"] = 0.0
+    bd[1]["bullet"] = int(bullet)
+    fd["friction"] = 0.0
+    return bd, fb, sd, fd
+
+
+@pytest.mark.parametrize("bullet", [False, True])
+def test_bullet_against_a_dynamic_plank(oracle, bullet):
+    """A bullet stops at a free dynamic plank instead of tunnelling through it (TOI events between
+    two dynamic bodies, src/private/dynamics/b2_world.rs:616-621), and the TOI solve's impulses
+    between the two dynamic bodies conserve linear momentum: m_ball v0 = m_ball v + m_plank V in
+    every step. Without the flag the ball passes through (no event between dynamic bodies)."""
+    w = oracle.world((0.0, 0.0))
+    w.set_flags(warm_starting=True, allow_sleep=False, block_solve=True, continuous=True)
+    w.add(*_bullet_scene(bullet))
+    m = w.mass()[:, 0]
+    p0 = m[1] * 120.0
+    events = 0
+    for _ in range(6):
+        w.step(np.float32(1 / 60), 8, 3)
+        events += w.toi_events
+        st = w.bodies()
+        p = m[0] * st["v"][0, 0] + m[1] * st["v"][1, 0]
+        assert abs(p - p0) <= 1e-4 * p0, (p, p0)
+    ball, plank = w.bodies()["c"][1, 0], w.bodies()["c"][0, 0]
+    if bullet:
+        assert events >= 1
+        assert ball < plank - 0.05 - 0.1 + 0.02, (ball, plank)
+    else:
+        assert events == 0 and ball > plank + 0.05, (ball, plank)
