@@ -113,6 +113,8 @@ def _load():
         "b2g_collide_pairs": ([P, I, P, P, P, P], I),
         "b2g_world_set_fixture_frictions": ([P, P, I], I),
         "b2g_world_set_bullet": ([P, I, I], I),
+        "b2g_world_set_sub_stepping": ([P, I], I),
+        "b2g_world_get_sub_stepping": ([P, P, P], I),
         "b2g_collide_pairs_noalloc": ([P, I, P, P, P, P], I),
         "b2g_broadphase_pairs": ([P, P, I64, P, P], I),
         "b2g_world_time_kernel": ([P, I], I),
@@ -447,6 +449,17 @@ class World:
     def set_bullet(self, body, flag=True):
         """B2body::set_bullet for one body (continuous collision also against dynamic bodies)."""
         _check(_lib.b2g_world_set_bullet(self._h, int(body), int(bool(flag))))
+
+    def set_sub_stepping(self, flag=True):
+        """B2world::set_sub_stepping: a TOI pass ends after its first solved event and the next
+        steps resume it without solving until a pass finds no event."""
+        _check(_lib.b2g_world_set_sub_stepping(self._h, int(bool(flag))))
+
+    def sub_stepping(self):
+        """(sub-stepping on, the last TOI pass finished)"""
+        f, c = ctypes.c_int32(), ctypes.c_int32()
+        _check(_lib.b2g_world_get_sub_stepping(self._h, ctypes.byref(f), ctypes.byref(c)))
+        return bool(f.value), bool(c.value)
 
     def collide_pairs(self, pairs, xf_a, xf_b):
         pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
@@ -873,6 +886,15 @@ class B2world:
     def set_continuous_physics(self, flag):
         self._continuous = bool(flag)
         self._apply_flags()
+
+    def get_continuous_physics(self):
+        return self._continuous
+
+    def set_sub_stepping(self, flag):
+        self._world.set_sub_stepping(flag)
+
+    def get_sub_stepping(self):
+        return self._world.sub_stepping()[0]
 
     def set_allow_sleeping(self, flag):
         # B2world::set_allow_sleeping (src/private/dynamics/b2_world.rs:342-353): switching it off

@@ -143,6 +143,10 @@ static void reset_mass(World* w, int b) {
 static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_t* body_world, int nf,
                      const int32_t* fixture_body, const b2g_shape_def* shapes, const b2g_fixture_def* fixtures,
                      int32_t* proxy_ids) {
+  if (!w->step_complete && (nbn > 0 || nf > 0)) {
+    set_error("bodies and fixtures cannot be added while a sub-stepped TOI pass is unfinished");
+    return B2G_ERR_UNSUPPORTED;
+  }
   for (int i = 0; i < nbn; ++i) {
     const b2g_body_def& d = bodies[i];
     if (body_world && (body_world[i] < 0 || body_world[i] >= (1 << 20))) {
@@ -485,6 +489,8 @@ static int run_phase(World* w, int phase, float dt, int vi, int pi) {
       if ((st = phase_broadphase(w))) return st;
       w->manifolds_valid[w->cur] = false;
       w->need_broadphase = false;
+      // an unfinished sub-stepped TOI pass keeps its cached times on the new set
+      if (!w->step_complete && (st = toi_carry_state(w))) return st;
     }
     if ((st = phase_collide(w))) return st;
     w->manifolds_valid[w->cur] = true;
@@ -692,15 +698,26 @@ int b2g_world_step(b2g_world* h, float dt, int32_t vi, int32_t pi) {
     }
     if ((st = tape_step_begin(w))) return st;
   }
+  if (w->sub_stepping && w->continuous && w->n_worlds > 1) {
+    set_error("b2g_world_step: sub-stepping runs one world, not a batch");
+    return B2G_ERR_UNSUPPORTED;
+  }
   (void)hipEventRecord(w->ev[0], w->stream);
   if ((st = run_phase(w, B2G_PHASE_COLLIDE, dt, vi, pi))) return st;
   if (taping && (st = tape_after_collide(w))) return st;
   (void)hipEventRecord(w->ev[1], w->stream);
   const float dt_ratio = w->inv_dt0 * dt;
-  if ((st = run_phase(w, B2G_PHASE_SOLVE, dt, vi, pi))) return st;
-  if (taping && (st = tape_after_solve(w, dt, dt_ratio, vi, pi))) return st;
-  (void)hipEventRecord(w->ev[2], w->stream);
-  if ((st = run_phase(w, B2G_PHASE_FIND, dt, vi, pi))) return st;
+  if (w->step_complete) {
+    if ((st = run_phase(w, B2G_PHASE_SOLVE, dt, vi, pi))) return st;
+    if (taping && (st = tape_after_solve(w, dt, dt_ratio, vi, pi))) return st;
+    (void)hipEventRecord(w->ev[2], w->stream);
+    if ((st = run_phase(w, B2G_PHASE_FIND, dt, vi, pi))) return st;
+  } else {
+    // an unfinished sub-stepped TOI pass: no solve, and so no synchronize or find_new_contacts
+    // (src/private/dynamics/b2_world.rs:935-939); the pass resumes below
+    (void)hipEventRecord(w->ev[2], w->stream);
+    if (dt > 0.0f) w->inv_dt0 = 1.0f / dt;
+  }
   w->toi_events = 0;
   if (w->continuous && dt > 0.0f && (st = phase_toi(w, dt, vi))) return st;
   (void)hipEventRecord(w->ev[3], w->stream);
@@ -832,6 +849,19 @@ int b2g_world_set_bullet(b2g_world* h, int32_t body, int32_t flag) {
     B2G_HIP(hipMemcpyAsync(w->b.bullet.p + body, &w->h_bullet[body], sizeof(int), hipMemcpyHostToDevice, w->stream));
     B2G_HIP(hipStreamSynchronize(w->stream));
   }
+  return B2G_OK;
+}
+
+int b2g_world_set_sub_stepping(b2g_world* h, int32_t flag) {
+  if (!h) { set_error("null world"); return B2G_ERR_INVALID_ARGUMENT; }
+  h->w.sub_stepping = flag != 0;
+  return B2G_OK;
+}
+
+int b2g_world_get_sub_stepping(b2g_world* h, int32_t* flag, int32_t* step_complete) {
+  if (!h || !flag || !step_complete) { set_error("b2g_world_get_sub_stepping: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
+  *flag = h->w.sub_stepping ? 1 : 0;
+  *step_complete = h->w.step_complete ? 1 : 0;
   return B2G_OK;
 }
 
@@ -1366,8 +1396,8 @@ int b2g_tape_begin(b2g_world* h, int32_t checkpoint_every) {
     set_error("b2g_tape_begin: the adjoint does not hold joints");
     return B2G_ERR_UNSUPPORTED;
   }
-  if (w->continuous) {
-    // TOI events move bodies outside the taped solve
+  if (w->continuous || !w->step_complete) {
+    // TOI events move bodies outside the taped solve (and an unfinished sub-stepped pass skips it)
     set_error("b2g_tape_begin: the adjoint needs continuous physics off");
     return B2G_ERR_UNSUPPORTED;
   }

@@ -268,6 +268,11 @@ struct World {
   // B2world::new (src/private/dynamics/b2_world.rs:37-43): warm starting, continuous physics and
   // sleeping on
   bool warm_starting = true, allow_sleep = true, continuous = true, block_solve = true;
+  // single-stepped continuous physics (B2world::set_sub_stepping, src/b2_world.rs:188-193): a TOI
+  // pass ends after its first solved event and step_complete stays false until a pass finds no
+  // event; a step with step_complete false skips the solve and resumes the pass without resetting
+  // it (src/private/dynamics/b2_world.rs:545, 678, 896-897, 935)
+  bool sub_stepping = false, step_complete = true;
   int device = 0;  // the HIP device current at creation: the stream's device
   // sleep_possible: allow_sleep is on or a body was created asleep; otherwise every non-static body
   // is awake at every step and the sleep bookkeeping is skipped
@@ -417,7 +422,8 @@ int phase_broadphase(World* w);  // fat AABBs -> next contact set (pairs + old i
 int phase_collide(World* w);     // narrowphase on the current contact set
 int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters);
 int phase_synchronize(World* w); // swept + fat AABBs of non-static proxies
-int phase_toi(World* w, float dt, int vi);             // continuous collision after the solve
+int phase_toi(World* w, float dt, int vi);  // continuous collision after the solve
+int toi_carry_state(World* w);              // TOI state of the previous contact set onto the current one
 void release_solve_graphs(World* w);
 // primitives
 int exclusive_scan(World* w, const int* in, int* out, int n, int* total_dev, int* total2 = nullptr);

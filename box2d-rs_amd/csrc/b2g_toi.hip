@@ -503,6 +503,15 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_first(ToiArgs a) {
   a.tflags[k] = kToiValid;
 }
 
+// A sub-stepped pass resumed by the next step: alpha0, the cached times of impact and the TOI
+// counts stay (src/private/dynamics/b2_world.rs:545 resets them only when the last pass was
+// complete); the collide in between re-enabled the contacts (src/private/dynamics/b2_contact.rs:
+// update), so an event's disabled contact is a candidate again.
+__global__ void __launch_bounds__(kToiBlock) k_toi_resume(int n, int* __restrict__ tflags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) tflags[k] &= kToiValid;
+}
+
 // Later scans: contacts whose time of impact is not cached. k_toi_need lists them (any order),
 // k_toi_rescan ranks the list into set order, aligns the sweeps on one thread in that order (an
 // alignment advances a body's sweep, which later contacts of the same body see) and then computes
@@ -878,11 +887,13 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __res
 // times of impact of the contacts that lost theirs (toi_rescan_wg) and adds those below 1 to the
 // list. It returns to the host only when no event is left, when an event re-fattened a proxy
 // (the host rebuilds the contact set, as find_new_contacts would add pairs), on a capacity error,
-// or after max_events events. Per event, the lists keep the work proportional to the event's own
-// contacts instead of scans over every contact or proxy.
+// after max_events events, or, with sub-stepping, after the first solved event (before any time
+// of impact is recomputed: the next step's pass recomputes them after its collide). Per event, the
+// lists keep the work proportional to the event's own contacts instead of scans over every
+// contact or proxy.
 constexpr int kLoopBlock = 512;  // (512 threads leave the event code 256 registers per thread)
 constexpr int kToiQueued = 4;  // TOI flag: listed for recomputation in this event
-enum : int { kLoopDone = 0, kLoopRebuild = 1, kLoopCapacity = 2, kLoopMore = 3 };
+enum : int { kLoopDone = 0, kLoopRebuild = 1, kLoopCapacity = 2, kLoopMore = 3, kLoopSubStep = 4 };
 
 struct LoopArgs {
   const int* cadj_off;  // contacts per non-static body of the current set
@@ -897,6 +908,7 @@ struct LoopArgs {
   int* state;      // loop exit state
   int* events;     // [0] events run, [1] events solved
   int max_events;
+  int sub_stepping;  // stop after the first solved event
 };
 
 // contacts per non-static body (the adjacency the loop reads)
@@ -1065,6 +1077,10 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
       if (a.flags4[0]) {
         // a re-fattened proxy can add pairs: the host rebuilds the contact set first
         if (t == 0) *L.state = kLoopRebuild;
+        break;
+      }
+      if (L.sub_stepping) {
+        if (t == 0) *L.state = kLoopSubStep;
         break;
       }
     }
@@ -1241,10 +1257,12 @@ static int reserve_contacts(ContactArrays& c, int n, int keep) {
 
 static int toi_rebuild(World* w) {
   int st;
-  // the predecessor's dropped touching contacts (its extra entries included) before it is overwritten
+  // the predecessor's dropped touching contacts (its extra entries included) before it is
+  // overwritten; when the current set has been through a collide already (a sub-stepped pass
+  // resumed after the step's collide), that collide destroyed them
   const int n_old = w->n_pairs_old, n_cur = w->n_pairs;
   int dropped = 0;
-  if (n_old > 0 && w->manifolds_valid[1 - w->cur]) {
+  if (w->collide_pending && n_old > 0 && w->manifolds_valid[1 - w->cur]) {
     ContactArrays& pred = w->c[1 - w->cur];
     if ((st = w->toi_succ.reserve(n_old)) || (st = w->toi_drop_flag.reserve(n_old)) ||
         (st = w->toi_drop_pos.reserve(n_old)) || (st = w->toi_drop_total.reserve(1)))
@@ -1275,17 +1293,9 @@ static int toi_rebuild(World* w) {
     w->n_pairs_old = base + dropped;
   }
   w->manifolds_valid[w->cur] = false;
+  w->collide_pending = true;
   if ((st = carry_manifolds_for_toi(w))) return st;
-  if ((st = toi_reserve(w))) return st;
-  if (w->n_pairs > 0) {
-    ToiState& o = w->toi_state[1 - w->cur];
-    ToiState& c = w->toi_state[w->cur];
-    hipLaunchKernelGGL(k_toi_carry, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                       w->n_pairs, w->c[w->cur].old_index.p, o.toi.p, o.count.p, o.flags.p, c.toi.p, c.count.p,
-                       c.flags.p);
-    B2G_HIP(hipGetLastError());
-  }
-  return B2G_OK;
+  return toi_carry_state(w);
 }
 
 // fixtures per body (rebuilt when the fixture count changed)
@@ -1327,6 +1337,7 @@ static LoopArgs loop_args(World* w) {
   L.state = ctl + 10;
   L.events = ctl + 11;
   L.max_events = 1 << 16;
+  L.sub_stepping = w->sub_stepping ? 1 : 0;
   return L;
 }
 
@@ -1359,20 +1370,49 @@ static int toi_loop_setup(World* w, float dt, int vi) {
   return B2G_OK;
 }
 
-// The TOI pass after the solve (src/private/dynamics/b2_world.rs:534-901, step_complete, no
-// sub-stepping). The current contact set is the one find_new_contacts left. The events run on
-// the device (k_toi_loop); the host rebuilds the contact set when an event re-fattened a proxy.
+// The TOI state of a rebuilt contact set: each pair's cached time of impact, TOI count and flags
+// from its predecessor in the previous set (a new pair has none: its time is computed when the
+// pass next scans it)
+int toi_carry_state(World* w) {
+  int st;
+  if ((st = toi_reserve(w))) return st;
+  if (w->n_pairs > 0) {
+    ToiState& o = w->toi_state[1 - w->cur];
+    ToiState& c = w->toi_state[w->cur];
+    hipLaunchKernelGGL(k_toi_carry, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                       w->n_pairs, w->c[w->cur].old_index.p, o.toi.p, o.count.p, o.flags.p, c.toi.p, c.count.p,
+                       c.flags.p);
+    B2G_HIP(hipGetLastError());
+  }
+  return B2G_OK;
+}
+
+// The TOI pass after the solve (src/private/dynamics/b2_world.rs:534-901). The current contact
+// set is the one find_new_contacts left. The events run on the device (k_toi_loop); the host
+// rebuilds the contact set when an event re-fattened a proxy. A pass starts from alpha0 = 0 and no
+// cached times when the last pass was complete; with sub-stepping a pass ends after its first
+// solved event and the next step's pass resumes it (step_complete).
 int phase_toi(World* w, float dt, int vi) {
   int st;
   w->toi_events = 0;
+  const bool resume = !w->step_complete;
   if ((st = carry_manifolds_for_toi(w))) return st;
   if ((st = toi_reserve(w))) return st;
   if ((st = toi_body_fixtures(w))) return st;
   static const ToiCtl kReset = {~0ull, 0, 0, {0, 0}, {0, 0, 0, 0}, 0, 0, 0, 0};
   B2G_HIP(hipMemcpyAsync(w->toi_ctl.p, &kReset, sizeof(kReset), hipMemcpyHostToDevice, w->stream));
   if (w->n_pairs > 0) {
-    hipLaunchKernelGGL(k_toi_first, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
-                       toi_args(w, dt, vi));
+    const ToiArgs b = toi_args(w, dt, vi);
+    if (!resume) {
+      hipLaunchKernelGGL(k_toi_first, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b);
+    } else {
+      // the times the last sub-step's event invalidated, and those of pairs found since, in set
+      // order with the sweep alignment (the reference's scan recomputes them in list order)
+      hipLaunchKernelGGL(k_toi_resume, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream,
+                         w->n_pairs, b.tflags);
+      hipLaunchKernelGGL(k_toi_need, dim3(blocks_for(w->n_pairs, kToiBlock)), dim3(kToiBlock), 0, w->stream, b, 0);
+      hipLaunchKernelGGL(k_toi_rescan, dim3(1), dim3(kRescanBlock), 0, w->stream, b);
+    }
     B2G_HIP(hipGetLastError());
   }
   if ((st = toi_loop_setup(w, dt, vi))) return st;
@@ -1382,7 +1422,10 @@ int phase_toi(World* w, float dt, int vi) {
   // is a guard against a logic error, not a limit of the reference
   const long long kMaxEvents = 4ll * (kMaxSubSteps + 2) * (static_cast<long long>(w->n_pairs) + w->nb + 16);
   for (;;) {
-    if (w->n_pairs == 0) return dirty ? toi_rebuild(w) : B2G_OK;
+    if (w->n_pairs == 0) {
+      w->step_complete = true;
+      return dirty ? toi_rebuild(w) : B2G_OK;
+    }
     hipLaunchKernelGGL(k_toi_loop, dim3(1), dim3(kLoopBlock), 0, w->stream, toi_args(w, dt, vi), loop_args(w));
     B2G_HIP(hipGetLastError());
     ToiCtl ctl;
@@ -1402,7 +1445,10 @@ int phase_toi(World* w, float dt, int vi) {
     static const int kZero[3] = {kLoopDone, 0, 0};
     B2G_HIP(hipMemcpyAsync(reinterpret_cast<int*>(w->toi_ctl.p) + 10, kZero, sizeof(kZero), hipMemcpyHostToDevice,
                            w->stream));
-    if (ctl.loop_state == kLoopDone) {
+    if (ctl.loop_state == kLoopDone || ctl.loop_state == kLoopSubStep) {
+      // a pass without another event is complete (src/private/dynamics/b2_world.rs:676-679); a
+      // sub-step ends after its solved event (:896-899)
+      w->step_complete = ctl.loop_state == kLoopDone;
       // manifolds that an event changed after the last rebuild become the next collide's
       // predecessors through one more rebuild
       return dirty ? toi_rebuild(w) : B2G_OK;
@@ -1412,6 +1458,11 @@ int phase_toi(World* w, float dt, int vi) {
       // moved proxies only), so the set is rebuilt only when a proxy was re-fattened
       if ((st = toi_rebuild(w))) return st;
       dirty = false;
+      if (w->sub_stepping) {
+        // the rebuild followed a solved event: the sub-step ends here
+        w->step_complete = false;
+        return B2G_OK;
+      }
       if (w->n_pairs > 0) {
         const ToiArgs b = toi_args(w, dt, vi);
         hipLaunchKernelGGL(k_clear, dim3(1), dim3(kToiBlock), 0, w->stream, 1, b.flags4 + 1);

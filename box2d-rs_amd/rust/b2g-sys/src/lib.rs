@@ -116,6 +116,8 @@ extern "C" {
     pub fn b2g_world_get_mass(w: *mut b2g_world, out: *mut f32, n: i32) -> c_int;
     pub fn b2g_world_set_fixture_frictions(w: *mut b2g_world, friction: *const f32, n: i32) -> c_int;
     pub fn b2g_world_set_bullet(w: *mut b2g_world, body: i32, flag: i32) -> c_int;
+    pub fn b2g_world_set_sub_stepping(w: *mut b2g_world, flag: i32) -> c_int;
+    pub fn b2g_world_get_sub_stepping(w: *mut b2g_world, flag: *mut i32, step_complete: *mut i32) -> c_int;
     pub fn b2g_world_get_fat_aabbs(w: *mut b2g_world, out: *mut f32, n: i32) -> c_int;
     pub fn b2g_world_set_fat_aabbs(w: *mut b2g_world, inp: *const f32, n: i32) -> c_int;
     pub fn b2g_world_get_contacts(w: *mut b2g_world, cap: i32, fixtures_ab: *mut i32, flags: *mut i32,

@@ -145,9 +145,9 @@ int b2g_world_destroy(b2g_world* world);
 
 /* B2world::set_warm_starting / set_allow_sleeping / set_continuous_physics
  * (src/b2_world.rs:164-185) and G_BLOCK_SOLVE (src/b2_contact.rs:25). continuous runs the TOI
- * pass after each step's solve (src/private/dynamics/b2_world.rs:534-901; no sub-stepping, no
- * bullets: times of impact against static and kinematic bodies; among equal times of impact the
- * first contact of the GPU's contact-set order goes first). allow_sleep works as in the reference (island sleep
+ * pass after each step's solve (src/private/dynamics/b2_world.rs:534-901: times of impact against
+ * static and kinematic bodies, and against dynamic bodies for bullets; among equal times of impact
+ * the first contact of the GPU's contact-set order goes first). allow_sleep works as in the reference (island sleep
  * after B2_TIME_TO_SLEEP below the sleep tolerances, waking on touch changes; switching it off
  * wakes every body), with one difference: whether a contact is updated in the collide phase
  * depends on its bodies' awake states at the start of the phase, where the reference's
@@ -209,6 +209,14 @@ int b2g_world_set_fixture_frictions(b2g_world* world, const float* friction, int
 /* B2body::set_bullet (src/b2_body.rs:504-506): a bullet gets continuous collision also against other
  * dynamic bodies (src/private/dynamics/b2_world.rs:616-621, 770-777). */
 int b2g_world_set_bullet(b2g_world* world, int32_t body, int32_t flag);
+/* B2world::set_sub_stepping (src/b2_world.rs:187-190): single-stepped continuous physics. A TOI
+ * pass ends after its first solved event; until a pass finds no event (step_complete), each step
+ * runs its collide and resumes the pass, without a solve (src/private/dynamics/b2_world.rs:545,
+ * 676-679, 896-899, 935-939). As in the reference, switching continuous physics off while a pass
+ * is unfinished leaves the world without solves. One world only (not batched worlds). */
+int b2g_world_set_sub_stepping(b2g_world* world, int32_t flag);
+/* B2world::get_sub_stepping (src/b2_world.rs:191-193), and whether the last TOI pass finished. */
+int b2g_world_get_sub_stepping(b2g_world* world, int32_t* flag, int32_t* step_complete);
 
 /* Fat AABB of every proxy (B2broadPhase::get_fat_aabb, src/b2_broad_phase.rs) as
  * lower.x, lower.y, upper.x, upper.y. */

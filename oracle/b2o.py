@@ -96,6 +96,9 @@ class Oracle:
         L.b2o_world_set_continuous.argtypes = [P, I]
         L.b2o_world_toi_events.argtypes = [P]
         L.b2o_world_toi_events.restype = I
+        L.b2o_world_set_sub_stepping.argtypes = [P, I]
+        L.b2o_world_step_complete.argtypes = [P]
+        L.b2o_world_step_complete.restype = I
         L.b2o_world_manifold.argtypes = [I, P, P, P, P, P, P]
         L.b2o_rot.argtypes = [I, P, P]
         L.b2o_brute_pairs.argtypes = [I, P, ctypes.c_longlong, P]
@@ -243,6 +246,15 @@ class OracleWorld:
     def toi_events(self):
         """TOI events solved in the last step."""
         return self.o.lib.b2o_world_toi_events(self.h)
+
+    def set_sub_stepping(self, flag=True):
+        """One solved TOI event per step; the next step resumes the pass without solving."""
+        self.o.lib.b2o_world_set_sub_stepping(self.h, int(flag))
+
+    @property
+    def step_complete(self):
+        """False while a sub-stepped TOI pass is unfinished."""
+        return bool(self.o.lib.b2o_world_step_complete(self.h))
 
     def add(self, body_defs, fixture_body, shape_defs, fixture_defs):
         bd = self.o.cast(body_defs, "body_def")

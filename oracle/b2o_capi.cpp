@@ -359,6 +359,9 @@ void b2o_world_set_flags(void* wp, int warm_starting, int allow_sleep, int block
 }
 void b2o_world_set_continuous(void* wp, int continuous) { static_cast<World*>(wp)->continuous = continuous != 0; }
 int b2o_world_toi_events(void* wp) { return static_cast<World*>(wp)->toi_events; }
+// B2world::set_sub_stepping (src/b2_world.rs:188-190)
+void b2o_world_set_sub_stepping(void* wp, int flag) { static_cast<World*>(wp)->sub_stepping = flag != 0; }
+int b2o_world_step_complete(void* wp) { return static_cast<World*>(wp)->step_complete ? 1 : 0; }
 
 // Adds n bodies, then nf fixtures (fixture k on body fixture_body[k], in array order).
 // Body indices continue from the existing count. Writes each fixture's proxy id.

@@ -1014,15 +1014,17 @@ static void solve_toi_island(World* w, IslandIO& island, const TimeStep& sub, in
 // its bodies is static, kinematic or a bullet
 void World::solve_toi(const TimeStep& step) {
   toi_events = 0;
-  for (Body* b : bodies) {
-    b->island_flag = false;
-    b->sweep.alpha0 = 0;
-  }
-  for (Contact* c = contact_list; c; c = c->next) {
-    c->toi_flag = false;
-    c->island_flag = false;
-    c->toi_count = 0;
-    c->toi = 1;
+  if (step_complete) {
+    for (Body* b : bodies) {
+      b->island_flag = false;
+      b->sweep.alpha0 = 0;
+    }
+    for (Contact* c = contact_list; c; c = c->next) {
+      c->toi_flag = false;
+      c->island_flag = false;
+      c->toi_count = 0;
+      c->toi = 1;
+    }
   }
   IslandIO island;
   for (;;) {
@@ -1070,7 +1072,10 @@ void World::solve_toi(const TimeStep& step) {
         min_alpha = alpha;
       }
     }
-    if (!min_contact || real(1.0) - real(10.0) * kEpsilon < min_alpha) break;
+    if (!min_contact || real(1.0) - real(10.0) * kEpsilon < min_alpha) {
+      step_complete = true;
+      break;
+    }
 
     Body* ba = min_contact->fa->body;
     Body* bb = min_contact->fb->body;
@@ -1147,6 +1152,10 @@ void World::solve_toi(const TimeStep& step) {
       }
     }
     find_new_contacts();
+    if (sub_stepping) {
+      step_complete = false;
+      break;
+    }
   }
 }
 
@@ -1341,7 +1350,7 @@ void World::step(real dt, int vel_iters, int pos_iters) {
   collide();
   auto t2 = std::chrono::steady_clock::now();
   profile.collide = std::chrono::duration<double, std::milli>(t2 - t1).count();
-  if (st.dt > 0) solve(st);
+  if (step_complete && st.dt > 0) solve(st);
   auto t3 = std::chrono::steady_clock::now();
   profile.solve = std::chrono::duration<double, std::milli>(t3 - t2).count();
   if (continuous && st.dt > 0) solve_toi(st);

@@ -270,6 +270,10 @@ struct World {
   BroadPhase bp;
   bool warm_starting = true, allow_sleep = true, continuous = false, block_solve = true;
   bool new_contacts = false;
+  // single-stepped continuous physics: a TOI pass ends after its first solved event and the next
+  // step resumes it without solving (src/b2_world.rs:188-193, src/private/dynamics/b2_world.rs:
+  // 545, 678, 896-897, 935)
+  bool sub_stepping = false, step_complete = true;
   real inv_dt0 = 0;
   int toi_events = 0;  // TOI events solved in the last step (a count for tests)
   Profile profile;

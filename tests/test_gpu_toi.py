@@ -24,7 +24,11 @@ same in the reference's list order and the GPU's contact-set order:
     constraint): times of impact against dynamic bodies, states and event counts as the oracle's;
     without the flag they pass through;
   * determinism: two runs of a 40-box rain with sleeping on are bit-identical; a 1024-body mixed
-    world runs 30 steps with continuous physics with finite states and nothing falling out.
+    world runs 30 steps with continuous physics with finite states and nothing falling out;
+  * sub-stepping (B2world::set_sub_stepping): one solved event per step, the pass resumed by the
+    next steps without a solve; states, event counts and step_complete as the oracle's at every
+    step (the rain, and the bouncing rain with sleeping), and every completed pass bit-identical
+    to the GPU's pass without sub-stepping.
 """
 import numpy as np
 import pytest
@@ -483,3 +487,75 @@ def test_reference_style_bullet():
         assert x < px, (x, px)
         x, px = run(False, late)
         assert x > px, (x, px)
+
+
+@pytest.mark.parametrize("scene", ["rain", "bounce"])
+def test_sub_stepping_matches_oracle(oracle, scene):
+    """B2world::set_sub_stepping: each step solves at most one TOI event and the next steps resume
+    the pass without a solve until a pass finds no event. Every step's states, event counts and
+    step_complete agree with the oracle's (awake flags and sleep times exactly with sleeping on);
+    and each completed sub-stepped pass leaves the GPU world equal, bit for bit, to a GPU world
+    stepped without sub-stepping, with the same events."""
+    sc, sleep, passes = (rain_scene(), False, 30) if scene == "rain" else (bounce_rain_scene(), True, 120)
+    w, o = _pair(oracle, sc, True, sleep=sleep)
+    full = b2g.World(sc[4])
+    full.set_flags(warm_starting=True, allow_sleep=sleep, continuous=True, block_solve=True)
+    full.add(*sc[:4])
+    w.set_sub_stepping(True)
+    o.set_sub_stepping(True)
+    assert w.sub_stepping() == (True, True)
+    resumed = 0
+    for p in range(passes):
+        full.step(DT, 8, 3)
+        events = 0
+        while True:
+            w.step(DT, 8, 3)
+            o.step(DT, 8, 3)
+            ev = w.profile()["toi_events"]
+            assert ev == o.toi_events and ev <= 1, p
+            assert w.sub_stepping() == (True, o.step_complete), p
+            events += ev
+            g, r = w.body_states(), o.bodies()
+            for f in ("c", "a", "v", "w"):
+                np.testing.assert_allclose(g[f], r[f], rtol=0, atol=1e-4, err_msg="pass %d %s" % (p, f))
+            if sleep:
+                np.testing.assert_array_equal(g["awake"], r["awake"], err_msg="pass %d awake" % p)
+                np.testing.assert_array_equal(g["sleep_time"], r["sleep_time"], err_msg="pass %d sleep" % p)
+            if o.step_complete:
+                break
+            resumed += 1
+        assert events == full.profile()["toi_events"], p
+        a, b = full.body_states(), w.body_states()
+        for f in ("c", "a", "v", "w", "awake", "sleep_time"):
+            assert a[f].tobytes() == b[f].tobytes(), (p, f)
+    assert resumed >= 10
+    full.close()
+
+
+def test_sub_stepping_refusals():
+    """While a sub-stepped pass is unfinished, bodies cannot be added and no tape can start; batched
+    worlds refuse sub-stepping."""
+    bd, fb, sd, fd, g = rain_scene()
+    w = b2g.World(g)
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=True, block_solve=True)
+    w.add(bd, fb, sd, fd)
+    w.set_sub_stepping(True)
+    for _ in range(60):
+        w.step(DT, 8, 3)
+        if not w.sub_stepping()[1]:
+            break
+    assert not w.sub_stepping()[1]
+    with pytest.raises(RuntimeError):
+        w.add(bd[1:2], np.zeros(1, np.int32), sd[1:2], fd[1:2])
+    w.set_flags(warm_starting=True, allow_sleep=False, continuous=False, block_solve=True)
+    with pytest.raises(RuntimeError):
+        w.tape_begin(4)
+    w.close()
+    batch = b2g.World(g)
+    batch.set_flags(warm_starting=True, allow_sleep=False, continuous=True, block_solve=True)
+    batch.add_batched(np.concatenate([bd, bd]), np.repeat(np.arange(2, dtype=np.int32), len(bd)),
+                      np.concatenate([fb, fb + len(bd)]), np.concatenate([sd, sd]), np.concatenate([fd, fd]))
+    batch.set_sub_stepping(True)
+    with pytest.raises(RuntimeError):
+        batch.step(DT, 8, 3)
+    batch.close()

@@ -352,3 +352,63 @@ def test_bullet_against_a_dynamic_plank(oracle, bullet):
         assert ball < plank - 0.05 - 0.1 + 0.02, (ball, plank)
     else:
         assert events == 0 and ball > plank + 0.05, (ball, plank)
+
+
+def _rain(n=12):
+    bd = np.zeros(1 + n, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    bd[1:]["type"] = 2
+    bd[1:]["position"] = np.stack([np.arange(n) * 1.5 - 0.75 * n, 3.0 + 0.37 * np.arange(n)], 1)
+    bd[1:]["linear_velocity"] = (0.0, -60.0)
+    bd[1:]["angle"] = np.linspace(-0.4, 0.4, n)
+    sd = np.zeros(1 + n, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_BOX
+    sd[0]["params"][:2] = (1.5 * n, 0.05)
+    sd[1:]["params"][:, :2] = (0.3, 0.2)
+    fd = np.zeros(1 + n, b2g.FIXTURE_DEF)
+    fd["friction"] = 0.4
+    fd["restitution_threshold"] = 1.0
+    fd["category"] = 1
+    fd["mask"] = 0xFFFF
+    fd["density"] = 1.0
+    return bd, np.arange(1 + n, dtype=np.int32), sd, fd
+
+
+def test_sub_stepping_resumes_the_pass(oracle):
+    """B2world::set_sub_stepping (src/private/dynamics/b2_world.rs:545, 676-679, 896-899, 935): a
+    TOI pass stops after each solved event and the following steps resume it -- collide, no solve,
+    no reset of alpha0, the cached times and the TOI counts -- until a pass finds no event. Nothing
+    between two events of one pass depends on where the steps fall, so once the sub-stepped pass is
+    complete the world equals the one stepped without sub-stepping, bit for bit, after the same
+    number of passes, with the same events; each sub-step solves at most one event, and a resumed
+    step leaves the velocity of a body outside its event unchanged (no solve: no gravity). A reset
+    on resuming, a solve in a resumed step or a pass that ends before its events are done fails."""
+    DT = np.float32(1 / 60)
+    full, sub = oracle.world((0.0, -10.0)), oracle.world((0.0, -10.0))
+    for w in (full, sub):
+        w.set_flags(warm_starting=True, allow_sleep=False, block_solve=True, continuous=True)
+        w.add(*_rain())
+    sub.set_sub_stepping(True)
+    resumed = 0
+    for step in range(30):
+        full.step(DT, 8, 3)
+        sub.step(DT, 8, 3)
+        events = sub.toi_events
+        assert events <= 1
+        while not sub.step_complete:
+            before = sub.bodies()
+            sub.step(DT, 8, 3)
+            assert sub.toi_events <= 1
+            events += sub.toi_events
+            resumed += 1
+            after = sub.bodies()
+            moved = np.any(before["c"] != after["c"], axis=1)
+            assert not moved.all()
+            np.testing.assert_array_equal(after["v"][~moved], before["v"][~moved])
+        assert events == full.toi_events, step
+        a, b = full.bodies(), sub.bodies()
+        for f in ("c", "a", "v", "w"):
+            assert a[f].tobytes() == b[f].tobytes(), (step, f)
+    assert resumed >= 10
