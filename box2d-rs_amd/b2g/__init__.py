@@ -843,6 +843,48 @@ class B2prismaticJointDef:
         return d
 
 
+class B2fixture:
+    """Handle on one fixture of a :class:`B2world` (fixtures are numbered in creation order)."""
+
+    def __init__(self, world, index):
+        self._world = world
+        self.index = index
+
+    def get_body(self):
+        return B2body(self._world, self._world._fixture_info[self.index][0])
+
+    def is_sensor(self):
+        return self._world._fixture_info[self.index][1]
+
+    def __eq__(self, other):
+        return isinstance(other, B2fixture) and other._world is self._world and other.index == self.index
+
+    def __hash__(self):
+        return hash((id(self._world), self.index))
+
+
+class B2contact:
+    """One contact of :meth:`B2world.get_contact_list` (src/b2_contact.rs getters)."""
+
+    def __init__(self, world, fixture_a, fixture_b, touching, manifold):
+        self._world = world
+        self._fa, self._fb = fixture_a, fixture_b
+        self._touching = touching
+        self._manifold = manifold
+
+    def get_fixture_a(self):
+        return B2fixture(self._world, self._fa)
+
+    def get_fixture_b(self):
+        return B2fixture(self._world, self._fb)
+
+    def is_touching(self):
+        return self._touching
+
+    def get_manifold(self):
+        return self._manifold
+
+
 class B2joint:
     """Handle on one joint of a :class:`B2world`."""
 
@@ -866,6 +908,8 @@ class B2world:
         self._bodies = []
         self._fixtures = []
         self._n_flushed_bodies = 0
+        self._n_flushed_fixtures = 0
+        self._fixture_info = []  # per fixture: (body index, is_sensor)
         self._bullets = {}
         self._cache = None
         # B2world::new defaults (src/private/dynamics/b2_world.rs:37-43): warm starting, sleeping
@@ -929,7 +973,9 @@ class B2world:
         if fixture_def.shape is None:
             raise ValueError("fixture definition needs a shape")
         self._fixtures.append((body_index, fixture_def.shape._def(), fixture_def._def()))
+        self._fixture_info.append((body_index, bool(fixture_def.is_sensor)))
         self._cache = None
+        return B2fixture(self, len(self._fixture_info) - 1)
 
     def _flush(self):
         if not self._bodies and not self._fixtures:
@@ -941,6 +987,7 @@ class B2world:
         fd = np.array([f for _, _, f in self._fixtures], FIXTURE_DEF) if self._fixtures else np.zeros(0, FIXTURE_DEF)
         self._world.add(bd, fb, sd, fd)
         self._n_flushed_bodies += len(self._bodies)
+        self._n_flushed_fixtures += len(self._fixtures)
         self._bodies = []
         self._fixtures = []
 
@@ -969,6 +1016,13 @@ class B2world:
     def get_contact_count(self):
         self._flush()
         return self._world.contact_count
+
+    def get_contact_list(self):
+        """B2world::get_contact_list: the contacts of the current set, each with its fixtures,
+        touching flag (for a sensor: the shapes overlap) and manifold."""
+        self._flush()
+        fab, flags, man = self._world.contacts()
+        return [B2contact(self, int(a), int(b), bool(t), m) for (a, b), t, m in zip(fab, flags, man)]
 
     def get_profile(self):
         return self._world.profile()

@@ -167,10 +167,6 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_
       set_error("fixture_body index out of range");
       return B2G_ERR_INVALID_ARGUMENT;
     }
-    if (fixtures[k].is_sensor) {
-      set_error("sensor fixtures are not supported on the GPU path");
-      return B2G_ERR_UNSUPPORTED;
-    }
   }
   // build all shapes first so that a degenerate polygon leaves the world unchanged
   std::vector<HostShape> hs(nf);
@@ -235,7 +231,10 @@ static int add_items(World* w, int nbn, const b2g_body_def* bodies, const int32_
     }
     int radius_bits;
     std::memcpy(&radius_bits, &s.radius, 4);
-    w->h_fshape.push_back(make_int4(s.type | (s.count << 8) | (s.one_sided << 16), 0, radius_bits, 0));
+    // .w: the fixture is a sensor (B2fixtureDef::is_sensor)
+    if (fixtures[k].is_sensor) ++w->n_sensors;
+    w->h_fshape.push_back(
+        make_int4(s.type | (s.count << 8) | (s.one_sided << 16), 0, radius_bits, fixtures[k].is_sensor ? 1 : 0));
     w->h_fmat.push_back(make_float4(fd.friction, fd.restitution, fd.restitution_threshold, 0.0f));
     w->h_ffilter.push_back(make_int2(int(fd.category_bits) | (int(fd.mask_bits) << 16), fd.group_index));
     w->h_fworld.push_back(w->h_bworld[b]);
@@ -924,10 +923,10 @@ int b2g_world_get_contacts(b2g_world* h, int32_t cap, int32_t* fixtures_ab, int3
   B2G_HIP(hipStreamSynchronize(w->stream));
   for (int i = 0; i < m; ++i) {
     if (fixtures_ab) { fixtures_ab[2 * i] = ab[i].x; fixtures_ab[2 * i + 1] = ab[i].y; }
-    if (flags) flags[i] = (hdr[i] >> 8) > 0 ? 1 : 0;
+    if (flags) flags[i] = (hdr[i] >> 8) > 0 || (hdr[i] & kSensorTouching) ? 1 : 0;
     if (manifolds) {
       b2g_manifold& mf = manifolds[i];
-      mf.type = hdr[i] & 0xff;
+      mf.type = hdr[i] & 0x7f;
       mf.point_count = hdr[i] >> 8;
       mf.local_normal[0] = geo[i].x; mf.local_normal[1] = geo[i].y;
       mf.local_point[0] = geo[i].z; mf.local_point[1] = geo[i].w;
@@ -1136,13 +1135,13 @@ int b2g_world_get_destroyed_contacts(b2g_world* h, int32_t cap, int32_t* fixture
     if (oi[k] >= 0 && oi[k] < n_old) kept[oi[k]] = 1;
   int m = 0;
   for (int k = 0; k < n_old; ++k) {
-    if (kept[k] || (hdr[k] >> 8) == 0) continue;
+    if (kept[k] || ((hdr[k] >> 8) == 0 && !(hdr[k] & kSensorTouching))) continue;
     if (m < cap) {
       fixtures_ab[2 * m] = ab[k].x;
       fixtures_ab[2 * m + 1] = ab[k].y;
       b2g_manifold& mf = manifolds[m];
       std::memset(&mf, 0, sizeof(mf));
-      mf.type = hdr[k] & 0xff;
+      mf.type = hdr[k] & 0x7f;
       mf.point_count = hdr[k] >> 8;
       mf.local_normal[0] = geo[k].x; mf.local_normal[1] = geo[k].y;
       mf.local_point[0] = geo[k].z; mf.local_point[1] = geo[k].w;

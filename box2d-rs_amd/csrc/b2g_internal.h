@@ -102,13 +102,19 @@ struct ProxyArrays {
   DevArray<ShapeV> shapev;  // per proxy: its shape as the collide functions read it (by reference)
 };
 
+// A sensor contact has no manifold (point count 0, everything else zero, as the reference's
+// never-evaluated manifold) and keeps its overlap state in this bit of the manifold header
+// (src/private/dynamics/b2_contact.rs:148-162).
+constexpr int kSensorTouching = 0x80;
+constexpr int kSensorPending = 0x40;  // within the collide phase: a sensor contact to be tested
+
 // Contact set: pairs grouped by their emitting proxy, plus manifolds.
 struct ContactArrays {
   DevArray<int2> pair;       // (emitter proxy, other proxy)
   DevArray<int> off;         // CSR offsets by emitter proxy (np + 1)
   DevArray<int> old_index;   // index of the same pair in the previous set, or -1
   DevArray<int2> ab;         // fixtures A, B after registry ordering
-  DevArray<int> hdr;         // manifold type | point_count << 8
+  DevArray<int> hdr;         // manifold type | point_count << 8 (| kSensorTouching)
   DevArray<float4> geo;      // local normal, local point
   DevArray<float4> pt0, pt1; // local point, normal impulse, tangent impulse
   DevArray<uint2> ids;
@@ -273,6 +279,7 @@ struct World {
   // event; a step with step_complete false skips the solve and resumes the pass without resetting
   // it (src/private/dynamics/b2_world.rs:545, 678, 896-897, 935)
   bool sub_stepping = false, step_complete = true;
+  int n_sensors = 0;  // sensor fixtures (B2fixtureDef::is_sensor)
   int device = 0;  // the HIP device current at creation: the stream's device
   // sleep_possible: allow_sleep is on or a body was created asleep; otherwise every non-static body
   // is awake at every step and the sleep bookkeeping is skipped

@@ -11,6 +11,7 @@
 #include <mutex>
 
 #include "b2g_internal.h"
+#include "b2g_distance.h"
 
 namespace b2g {
 
@@ -123,6 +124,16 @@ __global__ void __launch_bounds__(kCollideBlock) k_collide(int n, const int2* __
         st.ids[s] = make_uint2(0u, 0u);
         st.touching[s] = 0;
       }
+    } else if (hq.w | hp.w) {
+      // a sensor: no manifold, no wake (src/private/dynamics/b2_contact.rs:148-162); its overlap
+      // comes from k_sensor_overlap (the GJK distance would give this kernel a scratch frame)
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st.hdr[s] = kSensorPending;
+      st.geo[s] = z;
+      st.pt0[s] = z;
+      st.pt1[s] = z;
+      st.ids[s] = make_uint2(0u, 0u);
+      st.touching[s] = 0;
     } else {
       // shapes are read in place from their records (no private copy of the vertex arrays)
       const ShapeV& sa = shapes[fa];
@@ -188,6 +199,19 @@ __global__ void __launch_bounds__(kBlock) k_wake_destroyed(int n_old, const int2
   if (sv.btype[bB] != 0) sv.wake[bB] = 1;
 }
 
+// The sensor contacts the collide updated: b2_test_overlap of their shapes
+// (src/private/collision/b2_collision.rs:216-245) as the touching flag
+__global__ void __launch_bounds__(kBlock) k_sensor_overlap(int n, const int2* __restrict__ ab,
+                                                           const int* __restrict__ fbody,
+                                                           const ShapeV* __restrict__ shapes,
+                                                           const float4* __restrict__ bxf, int* __restrict__ hdr) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || hdr[k] != kSensorPending) return;
+  const int2 f = ab[k];
+  const bool overlap = test_overlap(shapes[f.x], xf_of(bxf[fbody[f.x]]), shapes[f.y], xf_of(bxf[fbody[f.y]]));
+  hdr[k] = overlap ? kSensorTouching : 0;
+}
+
 // B2body::set_awake(true) for the bodies asked to wake (src/private/dynamics/b2_body.rs:783-801)
 __global__ void __launch_bounds__(kBlock) k_apply_wakes(int nb, int* __restrict__ wake, int* __restrict__ awake,
                                                         float* __restrict__ sleep_time) {
@@ -213,6 +237,9 @@ int phase_collide(World* w) {
     launch_timed(w, B2G_KERNEL_COLLIDE, n, k_collide, dim3(blocks_for(n, kCollideBlock)), dim3(kCollideBlock), n, c.pair.p,
                  c.old_index.p, w->f.body.p, w->f.shape.p, w->f.shapev.p, w->b.xf.p, o.hdr.p, o.pt0.p, o.pt1.p,
                  o.ids.p, sv, c.ab.p, c.hdr.p, c.geo.p, c.pt0.p, c.pt1.p, c.ids.p, w->s.touch_flag.p);
+  if (w->n_sensors > 0 && n > 0)
+    hipLaunchKernelGGL(k_sensor_overlap, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, c.ab.p,
+                       w->f.body.p, w->f.shapev.p, w->b.xf.p, c.hdr.p);
   if (sleepy && w->nb > 0)
     hipLaunchKernelGGL(k_apply_wakes, dim3(blocks_for(w->nb, kBlock)), dim3(kBlock), 0, w->stream, w->nb,
                        w->b.wake.p, w->b.awake.p, w->b.sleep_time.p);

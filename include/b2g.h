@@ -64,7 +64,8 @@ typedef struct b2g_shape_def {
   float params[36];
 } b2g_shape_def;
 
-/* Fixture definition: B2fixtureDef + B2filter (src/b2_fixture.rs:14-58). 32 bytes. */
+/* Fixture definition: B2fixtureDef + B2filter (src/b2_fixture.rs:14-58). 32 bytes. A sensor's contacts
+ * get no manifold; their touching flag is the shapes' overlap (src/private/dynamics/b2_contact.rs:148-162). */
 typedef struct b2g_fixture_def {
   float friction, restitution, restitution_threshold, density;
   int32_t is_sensor;
@@ -226,7 +227,8 @@ int b2g_world_set_fat_aabbs(b2g_world* world, const float* in, int32_t n);
 
 /* The contact set (the world contact list, src/b2_contact_manager.rs:67-74), in the device's
  * order: fixture indices A and B (after the registry ordering of
- * src/private/dynamics/b2_contact_registers.rs:67-103), flags (bit0 touching) and manifolds.
+ * src/private/dynamics/b2_contact_registers.rs:67-103), flags (bit0 touching; for a sensor contact
+ * the shapes' overlap) and manifolds.
  * *count receives the number of contacts; at most cap are written. */
 int b2g_world_get_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, int32_t* flags,
                            b2g_manifold* manifolds, int32_t* count);

@@ -491,6 +491,18 @@ void World::collide() {
   for (Contact* c : to_destroy) destroy_contact(c);
 }
 
+// b2_test_overlap (src/private/collision/b2_collision.rs:216-245): GJK distance with radii from
+// an empty simplex cache, below ten epsilons
+static bool test_overlap(const Shape& a, const Transform& xa, const Shape& b, const Transform& xb) {
+  DistanceProxy pa, pb;
+  pa.set(a);
+  pb.set(b);
+  SimplexCache cache;
+  DistanceOutput out;
+  distance(&out, &cache, pa, xa, pb, xb, true);
+  return out.distance < real(10.0) * kEpsilon;
+}
+
 // B2contact::update (src/private/dynamics/b2_contact.rs:103-217)
 void World::update_contact(Contact* c) {
   {
@@ -504,7 +516,8 @@ void World::update_contact(Contact* c) {
     bool sensor = fa->sensor || fb->sensor;
     bool touching;
     if (sensor) {
-      touching = false;  // sensors are outside this restatement's scope
+      // sensors get no manifold and wake nobody (src/private/dynamics/b2_contact.rs:148-162)
+      touching = test_overlap(fa->shape, ba->xf, fb->shape, bb->xf);
       c->manifold.point_count = 0;
     } else {
       Manifold nm;
