@@ -1118,6 +1118,20 @@ __global__ void __launch_bounds__(kBlock) k_solve_position(int begin, int end, P
   int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   solve_position_one(i, i < end, begin, a);
 }
+
+// The overflow colour: a dynamic body with more contacts than the colour masks hold (kMaxColors)
+// gives its last contacts the highest colour, where two constraints may share a body. That colour
+// is solved one constraint after another on one thread: Gauss-Seidel in its order, as the
+// reference solves every constraint (b2_contact_solver_private.rs:268-730). Only worlds with such
+// a body take this path.
+__global__ void __launch_bounds__(64) k_solve_velocity_serial(int begin, int end, int block_solve, VelArgs a) {
+  if (threadIdx.x != 0) return;
+  for (int i = begin; i < end; ++i) solve_velocity_one(i, block_solve, a);
+}
+__global__ void __launch_bounds__(64) k_solve_position_serial(int begin, int end, PosArgs a) {
+  if (threadIdx.x != 0) return;
+  for (int i = begin; i < end; ++i) solve_position_one(i, true, begin, a);
+}
 __global__ void __launch_bounds__(kTailBlock) k_solve_position_tail(int c0, int c1, const int* __restrict__ off,
                                                                     int ng, PosArgs a) {
   for (int c = c0; c < c1; ++c) {
@@ -1877,20 +1891,24 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // constraints to solve: the coloured ones (touching contacts of awake islands)
     n_touching = w->h_counters[0];
     for (int i = 0; i < kMaxColors; ++i) nc += w->h_counters[4 + i];
-    if (w->h_counters[2]) {
-      set_error("colouring needs more than 127 colours (a dynamic body with too many contacts)");
-      return B2G_ERR_CAPACITY;
-    }
+  }
+  // the highest colour may hold constraints that share a body (a body with more contacts than
+  // the colour masks hold): it is solved serially, so the island bins, the per-world solve and
+  // the solve graph, which solve a colour's constraints in parallel, are not used
+  const bool serial = nc > 0 && w->h_counters[4 + kMaxColors - 1] > 0;
+  if (serial && w->tape.active) {
+    set_error("the adjoint tape needs every colour conflict-free (a body has more than 127 contacts)");
+    return B2G_ERR_UNSUPPORTED;
   }
   w->n_touch = nc;
   const int n_big = want_bins && nc > 0 ? w->h_counters[kBigIslandSlot] : 0;
   const int n_small = want_bins ? nc - n_big : 0;
   // auto: bins only where they take at least half of the constraints (the per-colour launches the
   // large islands need take the few small ones at no extra launch)
-  const bool bins = n_small > 0 && (w->island_solve == 1 || n_small >= n_big);
+  const bool bins = !serial && n_small > 0 && (w->island_solve == 1 || n_small >= n_big);
   const int nbins = bins ? (n_small + bin_cap - 1) / bin_cap : 0;
   w->n_bins = nbins;
-  const bool per_world = nc > 0 && want_world && !want_bins;
+  const bool per_world = nc > 0 && want_world && !want_bins && !serial;
   if (nc > 0) {
     // One launch per non-empty colour, in colour-id order (kept colours can leave gaps).
     // Device copy: launch offsets (kMaxColors + 1, padded with nc), then per colour id its base
@@ -1990,7 +2008,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   // (graph nodes cannot carry the timing events of a timed solve kernel)
   const bool timed_tail = w->kt.kernel == B2G_KERNEL_WARM_START || w->kt.kernel == B2G_KERNEL_SOLVE_VELOCITY ||
                           w->kt.kernel == B2G_KERNEL_SOLVE_POSITION;
-  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0 && !per_world && !bins) {
+  if (nc > 0 && w->solver_graphs && !timed_tail && !sleepy && w->nj == 0 && !per_world && !bins && !serial) {
     // one graph launch replaces ~(1 + vel_iters + pos_iters) x colours launches
     int maxc = 0;
     for (int col = 0; col < ncolors; ++col) maxc = std::max(maxc, co[col + 1] - co[col]);
@@ -2046,14 +2064,16 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   const int ng = bins ? nbins + 1 : 1;
   const int* goff = bins ? s.cw_off.p : s.color_off.p;
   const bool per_colour = !bins || n_big > 0;
-  // colours [tail, ncolors): the small last colours, one launch per iteration (when at least two);
-  // co holds ncolors + 1 offsets only when there are constraints
-  int tail = ncolors;
+  // colours [tail, last): the small last colours, one launch per iteration (when at least two);
+  // [last, ncolors): the serial overflow colour; co holds ncolors + 1 offsets only when there are
+  // constraints
+  const int last = serial ? ncolors - 1 : ncolors;
+  int tail = last;
   if (nc > 0) {
     while (tail > 0 && co[tail] - co[tail - 1] <= kTailMax) --tail;
-    if (ncolors - tail < 2) tail = ncolors;
+    if (last - tail < 2) tail = last;
   }
-  const int tail_items = nc > 0 ? co[ncolors] - co[tail] : 0;
+  const int tail_items = nc > 0 ? co[last] - co[tail] : 0;
   if (bins && vel_iters > 0)
     launch_bins_velocity(w, static_cast<long long>(n_small) * vel_iters, nbins, s.cw_off.p, ncolors, vel_iters,
                          int(w->block_solve), va);
@@ -2068,9 +2088,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
         launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, b1 - b0, k_solve_velocity, dim3(blocks_for(b1 - b0, kBlock)),
                      dim3(kBlock), b0, b1, int(w->block_solve), va);
     }
-    if (tail < ncolors)
+    if (tail < last)
       launch_timed(w, B2G_KERNEL_SOLVE_VELOCITY, tail_items, k_solve_velocity_tail, dim3(1), dim3(kTailBlock), tail,
-                   ncolors, goff, ng, int(w->block_solve), va);
+                   last, goff, ng, int(w->block_solve), va);
+    if (serial)
+      hipLaunchKernelGGL(k_solve_velocity_serial, dim3(1), dim3(64), 0, w->stream, co[last], co[ncolors],
+                         int(w->block_solve), va);
   }
   hipLaunchKernelGGL(k_store_and_integrate, dim3(blocks_for(std::max(nc, nb), kBlock)), dim3(kBlock), 0, w->stream, nc,
                      s.k_contact.p, s.k_info.p, s.lam.p, c.pt0.p, c.pt1.p, nb, w->b.type.p, w->b.pos.p, w->b.vel.p, dt,
@@ -2089,9 +2112,11 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
           launch_timed(w, B2G_KERNEL_SOLVE_POSITION, b1 - b0, k_solve_position, dim3(blocks_for(b1 - b0, kBlock)),
                        dim3(kBlock), b0, b1, pa);
       }
-      if (tail < ncolors)
+      if (tail < last)
         launch_timed(w, B2G_KERNEL_SOLVE_POSITION, tail_items, k_solve_position_tail, dim3(1), dim3(kTailBlock), tail,
-                     ncolors, goff, ng, pa);
+                     last, goff, ng, pa);
+      if (serial)
+        hipLaunchKernelGGL(k_solve_position_serial, dim3(1), dim3(64), 0, w->stream, co[last], co[ncolors], pa);
     }
     if (w->nj > 0) launch_joint_colours(w, 2, ja);
     // (the last iteration's end runs with the copy back below; the binned islands' ends run in
