@@ -33,6 +33,7 @@ KERNELS = {"none": -1, "solve_velocity": 0, "solve_position": 1, "warm_start": 2
 TUNE_COLOR_ROUNDS, TUNE_SOLVER_GRAPH, TUNE_COLOR_REUSE = 2, 3, 4
 TUNE_WORLD_SOLVE, TUNE_WORLD_THREADS_SMALL, TUNE_WORLD_THREADS_MEDIUM, TUNE_WORLD_THREADS_LARGE = 5, 6, 7, 8
 TUNE_ISLAND_SOLVE, TUNE_ISLAND_THREADS = 9, 10
+TUNE_TOI_GLOBAL_ISLAND = 11
 
 # Joint definition (B2G_JOINT_*: B2jointType values, src/b2_joint.rs:46-58)
 JOINT_DISTANCE, JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_WELD = 1, 6, 8, 9

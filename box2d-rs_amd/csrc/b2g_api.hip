@@ -588,7 +588,7 @@ int b2g_world_destroy(b2g_world* h) {
   for (auto& t : w->toi_state) {
     rel(t.toi); rel(t.count); rel(t.flags);
   }
-  rel(w->toi_alpha0); rel(w->toi_ctl);
+  rel(w->toi_alpha0); rel(w->toi_ctl); rel(w->toi_big);
   rel(w->toi_cand); rel(w->toi_need); rel(w->toi_rank); rel(w->toi_snap);
   rel(w->toi_cadj_off); rel(w->toi_cadj); rel(w->toi_cadj_cur); rel(w->toi_bfix_off); rel(w->toi_bfix);
   rel(w->toi_bfix_cur); rel(w->toi_cl); rel(w->toi_in_list);
@@ -1350,6 +1350,10 @@ int b2g_world_tune(b2g_world* h, int32_t param, int32_t value) {
     case B2G_TUNE_ISLAND_THREADS:
       if (value != 64 && value != 128 && value != 256 && value != 512) break;
       w->bin_threads = value;
+      return B2G_OK;
+    case B2G_TUNE_TOI_GLOBAL_ISLAND:
+      if (value != 0 && value != 1) break;
+      w->toi_force_big = value == 1;
       return B2G_OK;
     default: break;
   }

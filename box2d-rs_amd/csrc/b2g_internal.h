@@ -324,6 +324,8 @@ struct World {
   DevArray<int> toi_succ, toi_drop_flag, toi_drop_pos, toi_drop_total;
   ContactArrays toi_dropped;
   DevArray<float> toi_alpha0;         // per body: the sweep's alpha0 during the TOI pass
+  DevArray<unsigned char> toi_big;    // TOI islands too large for the event workgroup's LDS
+  bool toi_force_big = false;         // every TOI island in toi_big (a check of that path)
   DevArray<unsigned long long> toi_ctl;  // smallest TOI key, event status, event bodies, counters
   DevArray<int> toi_cand, toi_need, toi_rank;  // TOI event candidates; contacts to recompute, ranked
   DevArray<float> toi_snap;                    // aligned sweeps of the contacts to recompute

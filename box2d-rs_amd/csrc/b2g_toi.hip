@@ -34,9 +34,15 @@ namespace {
 
 constexpr int kToiBlock = 256;
 constexpr int kRescanBlock = 1024;
-constexpr int kMaxToiBodies = 64;      // bodies of one TOI island (two plus static/kinematic neighbours)
-constexpr int kMaxToiContacts = 64;    // contacts of one TOI island
-constexpr int kMaxBodyContacts = 512;  // contacts of one body looked at when the island is gathered
+// A TOI island lives in the event workgroup's LDS when it fits these sizes (bodies: the two plus
+// their static/kinematic neighbours; contacts; contacts of one event body looked at when the
+// island is gathered), and in a larger store in global memory otherwise.
+constexpr int kMaxToiBodies = 64;
+constexpr int kMaxToiContacts = 64;
+constexpr int kMaxBodyContacts = 512;
+constexpr int kBigToiBodies = 4096;
+constexpr int kBigToiContacts = 4096;
+constexpr int kBigBodyContacts = 16384;
 constexpr float kFltEpsilon = 1.1920928955078125e-7f;
 constexpr float kSlop = 0.005f;
 constexpr float kToiBaumgarte = 0.75f;  // src/b2_common.rs:80
@@ -427,34 +433,41 @@ __device__ bool update_contact(const ToiArgs& a, int k) {
   return touching;
 }
 
-struct EventShared {
-  int bodies[kMaxToiBodies];
-  SweepD sw[kMaxToiBodies];
+template <int NB, int NC, int NX>
+struct EventStore {
+  static constexpr int kBodies = NB, kContacts = NC, kCand = NX;
+  int bodies[NB];
+  SweepD sw[NB];
   int nbody;
-  int contacts[kMaxToiContacts];
+  int contacts[NC];
   int ncontact;
-  int cand[kMaxBodyContacts];
+  int cand[NX];
   int ncand;
   int overflow;
-  TPos<float> P[kMaxToiBodies];
-  TVel<float> V[kMaxToiBodies];
-  TVelC<float> vc[kMaxToiContacts];
-  float lam[kMaxToiContacts][4];
+  TPos<float> P[NB];
+  TVel<float> V[NB];
+  TVelC<float> vc[NC];
+  float lam[NC][4];
   // per island contact, read once: constants for the position solve (TOI masses) and the
   // velocity solve, manifold geometry, friction, island slots
-  ContactConst kp[kMaxToiContacts], kv[kMaxToiContacts];
-  TGeom<float> geom[kMaxToiContacts];
-  float fr[kMaxToiContacts];
-  int2 slot[kMaxToiContacts];
+  ContactConst kp[NC], kv[NC];
+  TGeom<float> geom[NC];
+  float fr[NC];
+  int2 slot[NC];
 };
+using EventShared = EventStore<kMaxToiBodies, kMaxToiContacts, kMaxBodyContacts>;  // LDS
+using EventBig = EventStore<kBigToiBodies, kBigToiContacts, kBigBodyContacts>;      // global memory
+static_assert(sizeof(EventBig) < (8u << 20), "global island store size");
 
-__device__ inline int island_slot(const EventShared& s, int b) {
+template <class S>
+__device__ inline int island_slot(const S& s, int b) {
   for (int i = 0; i < s.nbody; ++i)
     if (s.bodies[i] == b) return i;
   return -1;
 }
 
-__device__ inline ContactConst contact_const(const ToiArgs& a, int k, const EventShared& s, bool toi_masses,
+template <class S>
+__device__ inline ContactConst contact_const(const ToiArgs& a, int k, const S& s, bool toi_masses,
                                              int* ia, int* ib, float* friction, TGeom<float>* g) {
   const int2 fab = a.ab[k];
   const int bA = a.fbody[fab.x], bB = a.fbody[fab.y];
@@ -496,11 +509,14 @@ struct EventCtl {
   int min_k, solid;
 };
 
-__device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alpha);
+template <class S>
+__device__ void toi_event_island(const ToiArgs& a, S& s, float min_alpha);
 
 // One TOI event (src/private/dynamics/b2_world.rs:683-898), called by a whole workgroup (the
-// event itself runs on its first thread, as sequential as the reference's).
-__device__ void toi_event(const ToiArgs& a, EventShared& s, EventCtl& e) {
+// event itself runs on its first thread, as sequential as the reference's). S: the island's
+// store (LDS or global memory).
+template <class S>
+__device__ void toi_event(const ToiArgs& a, S& s, EventCtl& e) {
   const int t = threadIdx.x;
   if (t == 0) {
     float& min_alpha = e.min_alpha;
@@ -553,18 +569,19 @@ __device__ void toi_event(const ToiArgs& a, EventShared& s, EventCtl& e) {
 
 // the solid event's island: its static and kinematic contacts gathered, its TOI solve, the new
 // sweeps and velocities (first thread only)
-__device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alpha) {
+template <class S>
+__device__ void toi_event_island(const ToiArgs& a, S& s, float min_alpha) {
   // gather the static and kinematic contacts of each dynamic TOI body, in set order
   for (int side = 0; side < 2; ++side) {
     const int body = s.bodies[side];
     if (a.btype[body] != 2) continue;
     const int nc = a.flags4[2 + side];
-    if (nc > kMaxBodyContacts) {
+    if (nc > S::kCand) {
       s.overflow = 1;
       break;
     }
     s.ncand = nc;
-    for (int i = 0; i < nc; ++i) s.cand[i] = a.cand[side * kMaxBodyContacts + i];
+    for (int i = 0; i < nc; ++i) s.cand[i] = a.cand[side * S::kCand + i];
     for (int i = 1; i < nc; ++i) {
       const int x = s.cand[i];
       int j = i - 1;
@@ -597,13 +614,13 @@ __device__ void toi_event_island(const ToiArgs& a, EventShared& s, float min_alp
         store_sweep(a, other, backup);
         continue;
       }
-      if (s.ncontact == kMaxToiContacts) {
+      if (s.ncontact == S::kContacts) {
         s.overflow = 1;
         break;
       }
       s.contacts[s.ncontact++] = k;
       if (slot >= 0) continue;
-      if (s.nbody == kMaxToiBodies) {
+      if (s.nbody == S::kBodies) {
         s.overflow = 1;
         break;
       }
@@ -697,7 +714,7 @@ __global__ void __launch_bounds__(kToiBlock) k_toi_carry(int n, const int* __res
 // contact or proxy.
 constexpr int kLoopBlock = 512;  // (512 threads leave the event code 256 registers per thread)
 constexpr int kToiQueued = 4;  // TOI flag: listed for recomputation in this event
-enum : int { kLoopDone = 0, kLoopRebuild = 1, kLoopCapacity = 2, kLoopMore = 3, kLoopSubStep = 4 };
+enum : int { kLoopDone = 0, kLoopRebuild = 1, kLoopCapacity = 2, kLoopMore = 3, kLoopSubStep = 4, kLoopNeedBig = 5 };
 
 struct LoopArgs {
   const int* cadj_off;  // contacts per non-static body of the current set
@@ -713,6 +730,8 @@ struct LoopArgs {
   int* events;     // [0] events run, [1] events solved
   int max_events;
   int sub_stepping;  // stop after the first solved event
+  EventBig* big;     // the global island store (null until an island needed it)
+  int force_big;     // every event in the global store (a check of that path)
 };
 
 // contacts per non-static body (the adjacency the loop reads)
@@ -791,7 +810,7 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
   __shared__ EventShared s;
   __shared__ EventCtl e;
   __shared__ unsigned long long wmin[kLoopBlock / 64];
-  __shared__ int stop, nbody, bodies[kMaxToiBodies];
+  __shared__ int stop;
   const int t = threadIdx.x;
   for (int iter = 0;; ++iter) {
     // the smallest (time of impact, contact) over the candidates
@@ -829,16 +848,29 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
     const int km = static_cast<int>(key & 0xffffffffu);
     const int2 fm = a.ab[km];
     const int ev[2] = {a.fbody[fm.x], a.fbody[fm.y]};
+    // the island has at most 1 + nA + nB contacts and 2 + nA + nB bodies (n: the contacts of a
+    // dynamic event body, the only ones gathered from): beyond the LDS store it runs in the
+    // global one, which the host allocates on first need (nothing has changed yet at this exit)
+    int n_ev[2] = {0, 0};
+    for (int side = 0; side < 2; ++side)
+      if (a.btype[ev[side]] == 2) n_ev[side] = L.cadj_off[ev[side] + 1] - L.cadj_off[ev[side]];
+    const bool big = L.force_big || 2 + n_ev[0] + n_ev[1] > kMaxToiBodies || n_ev[0] > kMaxBodyContacts ||
+                     n_ev[1] > kMaxBodyContacts;
+    if (big && !L.big) {
+      if (t == 0) *L.state = kLoopNeedBig;
+      break;
+    }
+    const int cap = big ? kBigBodyContacts : kMaxBodyContacts;
     for (int side = 0; side < 2; ++side) {
       const int b = ev[side];
       if (a.btype[b] != 2) continue;
       const int e0 = L.cadj_off[b], e1 = L.cadj_off[b + 1];
-      for (int i = t; i < e1 - e0 && i < kMaxBodyContacts; i += kLoopBlock)
-        a.cand[side * kMaxBodyContacts + i] = L.cadj[e0 + i];
+      for (int i = t; i < e1 - e0 && i < cap; i += kLoopBlock) a.cand[side * cap + i] = L.cadj[e0 + i];
       if (t == 0) a.flags4[2 + side] = e1 - e0;
     }
     __syncthreads();
-    toi_event(a, s, e);
+    if (big) toi_event(a, *L.big, e);
+    else toi_event(a, s, e);
     const int status = *a.status;
     if (status == kEventCapacity) {
       if (t == 0) *L.state = kLoopCapacity;
@@ -846,16 +878,13 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
     }
     // the bodies whose contacts may need a new time of impact: the island's non-static bodies
     // after a solved event, the event's two otherwise (the contact update may have woken them)
+    const int* ibody = ev;
+    int nbody = 2;
+    if (status == kEventSolved) {
+      ibody = big ? L.big->bodies : s.bodies;
+      nbody = big ? L.big->nbody : s.nbody;
+    }
     if (t == 0) {
-      int n = 0;
-      if (status == kEventSolved) {
-        for (int i = 0; i < s.nbody; ++i)
-          if (a.btype[s.bodies[i]] != 0) bodies[n++] = s.bodies[i];
-      } else {
-        for (int i = 0; i < 2; ++i)
-          if (a.btype[ev[i]] != 0) bodies[n++] = ev[i];
-      }
-      nbody = n;
       L.events[0] += 1;
       if (status == kEventSolved) L.events[1] += 1;
     }
@@ -865,7 +894,7 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
       // a bullet the dynamic bodies it gathered; src/private/dynamics/b2_world.rs:871-891), then
       // re-synchronise their proxies
       for (int ib = 0; ib < nbody; ++ib) {
-        const int b = bodies[ib];
+        const int b = ibody[ib];
         if (a.btype[b] != 2) continue;
         const int e0 = L.cadj_off[b], e1 = L.cadj_off[b + 1];
         for (int i = e0 + t; i < e1; i += kLoopBlock) a.tflags[L.cadj[i]] &= ~kToiValid;
@@ -888,7 +917,8 @@ __global__ void __launch_bounds__(kLoopBlock) k_toi_loop(ToiArgs a, LoopArgs L) 
         break;
       }
     }
-    for (int i = 0; i < nbody; ++i) need_of_body(a, L, bodies[i]);
+    for (int i = 0; i < nbody; ++i)
+      if (a.btype[ibody[i]] != 0) need_of_body(a, L, ibody[i]);
     __syncthreads();
     const int m = a.flags4[1];
     toi_rescan_wg(a);
@@ -957,7 +987,7 @@ static int toi_reserve(World* w) {
   ToiState& ts = w->toi_state[w->cur];
   if ((st = ts.toi.reserve(n)) || (st = ts.count.reserve(n)) || (st = ts.flags.reserve(n))) return st;
   if ((st = w->toi_need.reserve(n)) || (st = w->toi_rank.reserve(n)) || (st = w->toi_snap.reserve(19 * n)) ||
-      (st = w->toi_cand.reserve(2 * kMaxBodyContacts)))
+      (st = w->toi_cand.reserve(2 * kBigBodyContacts)))
     return st;
   if ((st = w->toi_alpha0.reserve(std::max(w->nb, 1))) || (st = w->toi_ctl.reserve(7)) ||
       (st = w->toi_cl.reserve(n)) || (st = w->toi_in_list.reserve(n)))
@@ -1142,6 +1172,8 @@ static LoopArgs loop_args(World* w) {
   L.events = ctl + 11;
   L.max_events = 1 << 16;
   L.sub_stepping = w->sub_stepping ? 1 : 0;
+  L.big = w->toi_big.p ? reinterpret_cast<EventBig*>(w->toi_big.p) : nullptr;
+  L.force_big = w->toi_force_big ? 1 : 0;
   return L;
 }
 
@@ -1238,7 +1270,8 @@ int phase_toi(World* w, float dt, int vi) {
     total += ctl.events;
     if (ctl.events > 0) dirty = true;
     if (ctl.loop_state == kLoopCapacity) {
-      set_error("continuous collision: a TOI island has more than 64 bodies, 64 contacts or 512 contacts of one body");
+      set_error("continuous collision: a TOI island has more than 4096 bodies, 4096 contacts or 16384 contacts of "
+                "one body");
       return B2G_ERR_CAPACITY;
     }
     if (total > kMaxEvents) {
@@ -1249,6 +1282,15 @@ int phase_toi(World* w, float dt, int vi) {
     static const int kZero[3] = {kLoopDone, 0, 0};
     B2G_HIP(hipMemcpyAsync(reinterpret_cast<int*>(w->toi_ctl.p) + 10, kZero, sizeof(kZero), hipMemcpyHostToDevice,
                            w->stream));
+    if (ctl.loop_state == kLoopNeedBig) {
+      // the global island store, once per world; the loop runs the same event again with it
+      if (w->toi_big.p) {
+        set_error("continuous collision: the island store was not taken");
+        return B2G_ERR_DEVICE;
+      }
+      if ((st = w->toi_big.reserve(sizeof(EventBig)))) return st;
+      continue;
+    }
     if (ctl.loop_state == kLoopDone || ctl.loop_state == kLoopSubStep) {
       // a pass without another event is complete (src/private/dynamics/b2_world.rs:676-679); a
       // sub-step ends after its solved event (:896-899)

@@ -38,6 +38,7 @@ pub const B2G_TUNE_WORLD_THREADS_MEDIUM: i32 = 7;
 pub const B2G_TUNE_WORLD_THREADS_LARGE: i32 = 8;
 pub const B2G_TUNE_ISLAND_SOLVE: i32 = 9;
 pub const B2G_TUNE_ISLAND_THREADS: i32 = 10;
+pub const B2G_TUNE_TOI_GLOBAL_ISLAND: i32 = 11;
 pub const B2G_KERNEL_NONE: i32 = -1;
 pub const B2G_KERNEL_SOLVE_VELOCITY: i32 = 0;
 pub const B2G_KERNEL_SOLVE_POSITION: i32 = 1;

@@ -387,7 +387,10 @@ enum {
  *                           in steps whose small islands hold at least half of the constraints, unless
  *                           the per-world solve is on. Islands share no dynamic body, so the results
  *                           are the same bit for bit.
- *   B2G_TUNE_ISLAND_THREADS threads per bin of that solve: 64, 128, 256 (default) or 512. */
+ *   B2G_TUNE_ISLAND_THREADS threads per bin of that solve: 64, 128, 256 (default) or 512.
+ *   B2G_TUNE_TOI_GLOBAL_ISLAND 1: every TOI island in the global-memory store that islands too large
+ *                           for the event workgroup's LDS use (a check of that path; same results);
+ *                           0 (default): only those islands. */
 #define B2G_TUNE_COLOR_ROUNDS 2
 #define B2G_TUNE_SOLVER_GRAPH 3
 #define B2G_TUNE_COLOR_REUSE 4
@@ -397,6 +400,7 @@ enum {
 #define B2G_TUNE_WORLD_THREADS_LARGE 8
 #define B2G_TUNE_ISLAND_SOLVE 9
 #define B2G_TUNE_ISLAND_THREADS 10
+#define B2G_TUNE_TOI_GLOBAL_ISLAND 11
 int b2g_world_tune(b2g_world* world, int32_t param, int32_t value);
 
 /* Measurement hook with no counterpart in the reference (its B2Profile times whole phases on

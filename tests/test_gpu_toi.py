@@ -28,7 +28,11 @@ same in the reference's list order and the GPU's contact-set order:
   * sub-stepping (B2world::set_sub_stepping): one solved event per step, the pass resumed by the
     next steps without a solve; states, event counts and step_complete as the oracle's at every
     step (the rain, and the bouncing rain with sleeping), and every completed pass bit-identical
-    to the GPU's pass without sub-stepping.
+    to the GPU's pass without sub-stepping;
+  * TOI islands beyond the event workgroup's LDS store: every island forced into the global store
+    gives the same runs bit for bit, and a plank landing on 600 static fixtures (hundreds of
+    island contacts) stops on them with the oracle's event counts and positions within the TOI
+    position solve's 1.5-slop band.
 """
 import numpy as np
 import pytest
@@ -559,3 +563,76 @@ def test_sub_stepping_refusals():
     with pytest.raises(RuntimeError):
         batch.step(DT, 8, 3)
     batch.close()
+
+
+def _run_tuned(scene, steps, sleep, **tune):
+    bd, fb, sd, fd, g = scene
+    w = b2g.World(g)
+    w.set_flags(warm_starting=True, allow_sleep=sleep, continuous=True, block_solve=True)
+    for k, v in tune.items():
+        w.tune(getattr(b2g, k), v)
+    w.add(bd, fb, sd, fd)
+    events = []
+    for _ in range(steps):
+        w.step(DT, 8, 3)
+        events.append(w.profile()["toi_events"])
+    out = w.body_states().tobytes(), events
+    w.close()
+    return out
+
+
+@pytest.mark.parametrize("scene", ["bounce", "bullets"])
+def test_global_island_store_matches_lds(scene):
+    """TOI islands too large for the event workgroup's LDS run in a global-memory store (the same
+    event code); forcing every island there gives the same run bit for bit."""
+    sc, sleep, steps = (bounce_rain_scene(), True, 150) if scene == "bounce" else (bullet_scene(), False, 12)
+    lds = _run_tuned(sc, steps, sleep)
+    glob = _run_tuned(sc, steps, sleep, TUNE_TOI_GLOBAL_ISLAND=1)
+    assert sum(lds[1]) >= 6
+    assert glob == lds
+
+
+def plank_on_many_fixtures(n=600):
+    """A 40 m plank falling at 60 m/s onto one static body made of n thin boxes side by side (its
+    contact list holds every one of them: beyond the LDS store's 64 contacts and 512 contacts of
+    one body)."""
+    bd = np.zeros(2, b2g.BODY_DEF)
+    bd["gravity_scale"] = 1.0
+    bd["awake"] = 1
+    bd["allow_sleep"] = 1
+    bd[1]["type"] = 2
+    bd[1]["position"] = (0.0, 2.0)
+    bd[1]["linear_velocity"] = (0.0, -60.0)
+    sd = np.zeros(n + 1, b2g.SHAPE_DEF)
+    xs = (np.arange(n) - 0.5 * (n - 1)) * 0.05
+    sd[:n]["type"] = b2g.SHAPE_POLYGON
+    sd[:n]["count"] = 4
+    for i, x in enumerate(xs):
+        sd[i]["params"][:8] = (x - 0.025, -0.25, x + 0.025, -0.25, x + 0.025, 0.25, x - 0.025, 0.25)
+    sd[n]["type"] = b2g.SHAPE_BOX
+    sd[n]["params"][:2] = (20.0, 0.1)
+    fb = np.zeros(n + 1, np.int32)
+    fb[n] = 1
+    return bd, fb, sd, _fixtures(n + 1), (0.0, -10.0)
+
+
+def test_toi_island_beyond_the_lds_store(oracle):
+    """The plank's event gathers hundreds of contacts with the static body: the island runs in the
+    global store and the plank stops on the boxes instead of passing into them, with the oracle's
+    event counts at every step. The island's contacts are solved in the GPU's set order, the
+    reference's in its contact-list order (which follows its dynamic tree's history); the TOI
+    position solve stops once every contact is within 1.5 linear slops of its target, so the two
+    orders may leave the plank anywhere in that band: positions are compared at 1.5 slops."""
+    w, o = _pair(oracle, plank_on_many_fixtures(), True)
+    events = 0
+    for step in range(20):
+        w.step(DT, 8, 3)
+        o.step(DT, 8, 3)
+        assert w.profile()["toi_events"] == o.toi_events, step
+        events += o.toi_events
+        g, r = w.body_states(), o.bodies()
+        np.testing.assert_allclose(g["c"], r["c"], rtol=0, atol=1.5 * 0.005, err_msg="step %d" % step)
+    assert events >= 1
+    for st in (w.body_states()[1], o.bodies()[1]):
+        assert 0.25 + 0.1 < float(st["c"][1]) < 0.25 + 0.1 + 0.05, st["c"]
+        assert abs(float(st["v"][1])) < 0.05
