@@ -37,12 +37,17 @@ TUNE_TOI_GLOBAL_ISLAND = 11
 
 # Joint definition (B2G_JOINT_*: B2jointType values, src/b2_joint.rs:46-58)
 JOINT_DISTANCE, JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_WELD = 1, 6, 8, 9
+JOINT_FRICTION, JOINT_GEAR, JOINT_MOTOR, JOINT_MOUSE, JOINT_PULLEY, JOINT_WHEEL = 2, 3, 4, 5, 7, 10
 JOINT_DEF = np.dtype([("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
                       ("local_anchor_a", "<f4", 2), ("local_anchor_b", "<f4", 2), ("length", "<f4"),
                       ("min_length", "<f4"), ("max_length", "<f4"), ("stiffness", "<f4"), ("damping", "<f4"),
                       ("reference_angle", "<f4"), ("lower_angle", "<f4"), ("upper_angle", "<f4"),
                       ("max_motor_torque", "<f4"), ("motor_speed", "<f4"), ("enable_limit", "<i4"),
-                      ("enable_motor", "<i4"), ("local_axis_a", "<f4", 2)])
+                      ("enable_motor", "<i4"), ("local_axis_a", "<f4", 2), ("max_force", "<f4"),
+                      ("max_torque", "<f4"), ("linear_offset", "<f4", 2), ("angular_offset", "<f4"),
+                      ("correction_factor", "<f4"), ("target", "<f4", 2), ("ground_anchor_a", "<f4", 2),
+                      ("ground_anchor_b", "<f4", 2), ("length_a", "<f4"), ("length_b", "<f4"), ("ratio", "<f4"),
+                      ("joint1", "<i4"), ("joint2", "<i4")])
 
 BODY_DEF = np.dtype([("type", "<i4"), ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"),
                      ("position", "<f4", 2), ("angle", "<f4"), ("linear_velocity", "<f4", 2),
@@ -134,6 +139,7 @@ def _load():
         "b2g_world_get_destroyed_contacts": ([P, I, P, P, P], I),
         "b2g_world_add_joints": ([P, I, P], I),
         "b2g_world_get_joint_impulses": ([P, P, I], I),
+        "b2g_world_set_joint_target": ([P, I, ctypes.c_float, ctypes.c_float], I),
         "b2g_world_set_joint_impulses": ([P, P, I], I),
         "b2g_world_get_solver_state": ([P, P], I),
         "b2g_world_set_solver_state": ([P, P], I),
@@ -215,7 +221,7 @@ class World:
         return proxies
 
     def add_joints(self, joint_defs):
-        """Distance and revolute joints (JOINT_DEF; bodies are creation indices)."""
+        """Joints of any kind (JOINT_DEF; bodies and a gear's joints are creation indices)."""
         jd = _as(joint_defs, JOINT_DEF)
         _check(_lib.b2g_world_add_joints(self._h, len(jd), _ptr(jd)))
         self._n_joints = getattr(self, "_n_joints", 0) + len(jd)
@@ -234,9 +240,15 @@ class World:
     def joint_count(self):
         return getattr(self, "_n_joints", 0)
 
+    def set_joint_target(self, joint, target):
+        """B2mouseJoint::set_target: a new target wakes body B."""
+        _check(_lib.b2g_world_set_joint_target(self._h, int(joint), float(target[0]), float(target[1])))
+
     def joint_impulses(self):
-        """Warm-start impulses per joint: (n, 5) (distance: impulse, lower, upper; revolute:
-        impulse.x, impulse.y, motor, lower, upper)."""
+        """Warm-start impulses per joint: (n, 5) (distance: impulse, lower, upper; revolute and
+        prismatic: impulse.x, impulse.y, motor, lower, upper; weld: impulse x, y, z; wheel: impulse,
+        spring, motor, lower, upper; friction and motor: linear x, y, angular; mouse: x, y; pulley and
+        gear: impulse)."""
         out = np.zeros((max(self.joint_count, 1), 5), np.float32)
         _check(_lib.b2g_world_get_joint_impulses(self._h, _ptr(out), self.joint_count))
         return out[:self.joint_count]
@@ -886,6 +898,175 @@ class B2contact:
         return self._manifold
 
 
+def _local_vector(body, v):
+    """B2body::get_local_vector: mul_t(q, v) in f32."""
+    st = body._state()
+    f = np.float32
+    s, c = f(st["q"][0]), f(st["q"][1])
+    return (float(c * f(v[0]) + s * f(v[1])), float(-s * f(v[0]) + c * f(v[1])))
+
+
+class _JointDefBase:
+    kind = 0
+
+    def _base(self):
+        d = np.zeros((), JOINT_DEF)
+        d["type"] = self.kind
+        d["body_a"], d["body_b"] = self.body_a.index, self.body_b.index
+        d["collide_connected"] = int(self.collide_connected)
+        return d
+
+
+class B2wheelJointDef(_JointDefBase):
+    """``B2wheelJointDef`` (src/joints/b2_wheel_joint.rs:39-92): anchors, the axis in body A's
+    frame, translation limits, motor torque and speed, a spring (stiffness, damping)."""
+    kind = JOINT_WHEEL
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.local_anchor_a = (0.0, 0.0)
+        self.local_anchor_b = (0.0, 0.0)
+        self.local_axis_a = (1.0, 0.0)
+        self.enable_limit, self.lower_translation, self.upper_translation = False, 0.0, 0.0
+        self.enable_motor, self.max_motor_torque, self.motor_speed = False, 0.0, 0.0
+        self.stiffness = self.damping = 0.0
+
+    def initialize(self, body_a, body_b, anchor, axis):
+        self.body_a, self.body_b = body_a, body_b
+        self.local_anchor_a = _local_point(body_a, anchor)
+        self.local_anchor_b = _local_point(body_b, anchor)
+        self.local_axis_a = _local_vector(body_a, axis)
+
+    def _def(self):
+        d = self._base()
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["local_axis_a"] = self.local_axis_a
+        d["enable_limit"], d["lower_angle"], d["upper_angle"] = (int(self.enable_limit), self.lower_translation,
+                                                                 self.upper_translation)
+        d["enable_motor"], d["max_motor_torque"], d["motor_speed"] = (int(self.enable_motor), self.max_motor_torque,
+                                                                       self.motor_speed)
+        d["stiffness"], d["damping"] = self.stiffness, self.damping
+        return d
+
+
+class B2frictionJointDef(_JointDefBase):
+    """``B2frictionJointDef`` (src/joints/b2_friction_joint.rs): anchors, max force and torque."""
+    kind = JOINT_FRICTION
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.local_anchor_a = (0.0, 0.0)
+        self.local_anchor_b = (0.0, 0.0)
+        self.max_force = self.max_torque = 0.0
+
+    def initialize(self, body_a, body_b, anchor):
+        self.body_a, self.body_b = body_a, body_b
+        self.local_anchor_a = _local_point(body_a, anchor)
+        self.local_anchor_b = _local_point(body_b, anchor)
+
+    def _def(self):
+        d = self._base()
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["max_force"], d["max_torque"] = self.max_force, self.max_torque
+        return d
+
+
+class B2motorJointDef(_JointDefBase):
+    """``B2motorJointDef`` (src/joints/b2_motor_joint.rs): linear and angular offset of body B from
+    body A, max force and torque, correction factor."""
+    kind = JOINT_MOTOR
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.linear_offset = (0.0, 0.0)
+        self.angular_offset = 0.0
+        self.max_force, self.max_torque, self.correction_factor = 1.0, 1.0, 0.3
+
+    def initialize(self, body_a, body_b):
+        """B2motorJointDef::initialize: the current offset of body B from body A."""
+        self.body_a, self.body_b = body_a, body_b
+        self.linear_offset = _local_point(body_a, tuple(body_b._state()["p"]))
+        self.angular_offset = float(np.float32(body_b.get_angle()) - np.float32(body_a.get_angle()))
+
+    def _def(self):
+        d = self._base()
+        d["linear_offset"], d["angular_offset"] = self.linear_offset, self.angular_offset
+        d["max_force"], d["max_torque"], d["correction_factor"] = self.max_force, self.max_torque, self.correction_factor
+        return d
+
+
+class B2mouseJointDef(_JointDefBase):
+    """``B2mouseJointDef`` (src/joints/b2_mouse_joint.rs:35-52): body B's point pulled to the target
+    (body A is a reference body, usually the ground)."""
+    kind = JOINT_MOUSE
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.target = (0.0, 0.0)
+        self.max_force = self.stiffness = self.damping = 0.0
+
+    def _def(self):
+        d = self._base()
+        d["target"] = self.target
+        d["max_force"], d["stiffness"], d["damping"] = self.max_force, self.stiffness, self.damping
+        return d
+
+
+class B2pulleyJointDef(_JointDefBase):
+    """``B2pulleyJointDef`` (src/joints/b2_pulley_joint.rs): ground anchors, anchors, lengths and
+    ratio; collide_connected defaults to true."""
+    kind = JOINT_PULLEY
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = True
+        self.ground_anchor_a, self.ground_anchor_b = (-1.0, 1.0), (1.0, 1.0)
+        self.local_anchor_a, self.local_anchor_b = (-1.0, 0.0), (1.0, 0.0)
+        self.length_a = self.length_b = 0.0
+        self.ratio = 1.0
+
+    def initialize(self, body_a, body_b, ground_anchor_a, ground_anchor_b, anchor_a, anchor_b, ratio):
+        """B2pulleyJointDef::initialize (src/joints/b2_pulley_joint.rs:59-82)."""
+        self.body_a, self.body_b = body_a, body_b
+        self.ground_anchor_a, self.ground_anchor_b = tuple(ground_anchor_a), tuple(ground_anchor_b)
+        self.local_anchor_a = _local_point(body_a, anchor_a)
+        self.local_anchor_b = _local_point(body_b, anchor_b)
+        f = np.float32
+        da = (f(anchor_a[0]) - f(ground_anchor_a[0]), f(anchor_a[1]) - f(ground_anchor_a[1]))
+        db = (f(anchor_b[0]) - f(ground_anchor_b[0]), f(anchor_b[1]) - f(ground_anchor_b[1]))
+        self.length_a = float(np.sqrt(da[0] * da[0] + da[1] * da[1]))
+        self.length_b = float(np.sqrt(db[0] * db[0] + db[1] * db[1]))
+        self.ratio = ratio
+
+    def _def(self):
+        d = self._base()
+        d["ground_anchor_a"], d["ground_anchor_b"] = self.ground_anchor_a, self.ground_anchor_b
+        d["local_anchor_a"], d["local_anchor_b"] = self.local_anchor_a, self.local_anchor_b
+        d["length_a"], d["length_b"], d["ratio"] = self.length_a, self.length_b, self.ratio
+        return d
+
+
+class B2gearJointDef(_JointDefBase):
+    """``B2gearJointDef`` (src/joints/b2_gear_joint.rs): two revolute or prismatic joints and the
+    ratio; body_a and body_b are the joints' bodies B."""
+    kind = JOINT_GEAR
+
+    def __init__(self):
+        self.body_a = self.body_b = None
+        self.collide_connected = False
+        self.joint1 = self.joint2 = None
+        self.ratio = 1.0
+
+    def _def(self):
+        d = self._base()
+        d["joint1"], d["joint2"], d["ratio"] = self.joint1.index, self.joint2.index, self.ratio
+        return d
+
+
 class B2joint:
     """Handle on one joint of a :class:`B2world`."""
 
@@ -895,6 +1076,11 @@ class B2joint:
 
     def get_impulses(self):
         return self._world._world.joint_impulses()[self.index]
+
+    def set_target(self, target):
+        """B2mouseJoint::set_target."""
+        self._world._world.set_joint_target(self.index, target)
+        self._world._cache = None
 
 
 class B2world:

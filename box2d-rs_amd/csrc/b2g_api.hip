@@ -310,7 +310,7 @@ int upload_joints(World* w) {
   int st;
   std::vector<int2> jb(nj);
   std::vector<int4> jt(nj);
-  std::vector<float4> d0(nj), d1(nj), d2(nj);
+  std::vector<float4> d0(nj), d1(nj), d2(nj), d3(nj, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
   std::vector<std::vector<int>> nc(nb);
   for (int i = 0; i < nj; ++i) {
     const b2g_joint_def& d = w->h_joints[i];
@@ -329,6 +329,44 @@ int upload_joints(World* w) {
       normalize(&axis);
       d1[i] = make_float4(d.reference_angle, d.lower_angle, d.upper_angle, d.max_motor_torque);
       d2[i] = make_float4(d.motor_speed, axis.x, axis.y, 0.0f);
+    } else if (d.type == B2G_JOINT_FRICTION) {
+      d1[i] = make_float4(d.max_force, d.max_torque, 0.0f, 0.0f);
+      d2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else if (d.type == B2G_JOINT_MOTOR) {
+      d0[i] = make_float4(d.linear_offset[0], d.linear_offset[1], 0.0f, 0.0f);
+      d1[i] = make_float4(d.max_force, d.max_torque, d.angular_offset, d.correction_factor);
+      d2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else if (d.type == B2G_JOINT_MOUSE) {
+      d1[i] = make_float4(d.target[0], d.target[1], d.max_force, d.stiffness);
+      d2[i] = make_float4(d.damping, 0.0f, 0.0f, 0.0f);
+    } else if (d.type == B2G_JOINT_WHEEL) {
+      // the axis as given (B2wheelJoint::new does not normalise it, src/joints/b2_wheel_joint.rs:334)
+      d1[i] = make_float4(d.lower_angle, d.upper_angle, d.max_motor_torque, d.motor_speed);
+      d2[i] = make_float4(d.stiffness, d.damping, d.local_axis_a[0], d.local_axis_a[1]);
+    } else if (d.type == B2G_JOINT_PULLEY) {
+      d1[i] = make_float4(d.ground_anchor_a[0], d.ground_anchor_a[1], d.ground_anchor_b[0], d.ground_anchor_b[1]);
+      d2[i] = make_float4(d.ratio, w->h_joint_constant[i], 0.0f, 0.0f);
+    } else if (d.type == B2G_JOINT_GEAR) {
+      // bodies A and B as the coupled joints' bodies B, C and D as their bodies A, with their
+      // anchors, axes (prismatic) and reference angles (src/private/dynamics/joints/b2_gear_joint.rs:8-85)
+      const b2g_joint_def& j1 = w->h_joints[d.joint1];
+      const b2g_joint_def& j2 = w->h_joints[d.joint2];
+      V2 axis_c = v2(0.0f, 0.0f), axis_d = v2(0.0f, 0.0f);
+      if (j1.type == B2G_JOINT_PRISMATIC) {
+        axis_c = v2(j1.local_axis_a[0], j1.local_axis_a[1]);
+        normalize(&axis_c);
+      }
+      if (j2.type == B2G_JOINT_PRISMATIC) {
+        axis_d = v2(j2.local_axis_a[0], j2.local_axis_a[1]);
+        normalize(&axis_d);
+      }
+      jt[i].y |= (j1.type == B2G_JOINT_PRISMATIC ? 8 : 0) | (j2.type == B2G_JOINT_PRISMATIC ? 16 : 0);
+      jt[i].z = j1.body_a;
+      jt[i].w = j2.body_a;
+      d0[i] = make_float4(j1.local_anchor_b[0], j1.local_anchor_b[1], j2.local_anchor_b[0], j2.local_anchor_b[1]);
+      d1[i] = make_float4(j1.local_anchor_a[0], j1.local_anchor_a[1], j2.local_anchor_a[0], j2.local_anchor_a[1]);
+      d2[i] = make_float4(axis_c.x, axis_c.y, axis_d.x, axis_d.y);
+      d3[i] = make_float4(j1.reference_angle, j2.reference_angle, d.ratio, w->h_joint_constant[i]);
     } else {
       d1[i] = make_float4(d.reference_angle, d.lower_angle, d.upper_angle, d.max_motor_torque);
       d2[i] = make_float4(d.motor_speed, 0.0f, 0.0f, 0.0f);
@@ -348,14 +386,16 @@ int upload_joints(World* w) {
   // impulses: grow keeping the uploaded prefix
   const int had = w->dev_nj;
   if ((st = J.imp.reserve(std::max(nj, 1), true, had)) || (st = J.imp2.reserve(std::max(nj, 1), true, had)) ||
-      (st = J.r.reserve(std::max(nj, 1))) || (st = J.k.reserve(std::max(nj, 1))) || (st = J.k2.reserve(std::max(nj, 1))))
+      (st = J.r.reserve(std::max(nj, 1))) || (st = J.k.reserve(std::max(nj, 1))) || (st = J.k2.reserve(std::max(nj, 1))) ||
+      (st = J.k3.reserve(std::max(nj, 1))))
     return st;
   if (nj > had) {
     B2G_HIP(hipMemsetAsync(J.imp.p + had, 0, sizeof(float4) * (nj - had), w->stream));
     B2G_HIP(hipMemsetAsync(J.imp2.p + had, 0, sizeof(float) * (nj - had), w->stream));
   }
   if ((st = up(J.bodies, jb, w->stream)) || (st = up(J.type, jt, w->stream)) || (st = up(J.d0, d0, w->stream)) ||
-      (st = up(J.d1, d1, w->stream)) || (st = up(J.d2, d2, w->stream)) || (st = up(J.order, w->h_joint_order, w->stream)) ||
+      (st = up(J.d1, d1, w->stream)) || (st = up(J.d2, d2, w->stream)) || (st = up(J.d3, d3, w->stream)) ||
+      (st = up(J.order, w->h_joint_order, w->stream)) ||
       (st = up(J.nc_off, nc_off, w->stream)) || (st = up(J.nc_list, nc_list, w->stream)) ||
       (st = up(J.nc_flag, nc_flag, w->stream)))
     return st;
@@ -945,45 +985,105 @@ int b2g_world_add_joints(b2g_world* h, int32_t n, const b2g_joint_def* defs) {
   if (!h || n < 0 || (n > 0 && !defs)) { set_error("b2g_world_add_joints: invalid argument"); return B2G_ERR_INVALID_ARGUMENT; }
   World* w = &h->w;
   if (w->tape.active) { set_error("b2g_world_add_joints: a tape is being recorded"); return B2G_ERR_UNSUPPORTED; }
+  const int nj0 = w->nj;
+  bool poses = false;
+  // the kind of joint k, existing or earlier in this call
+  auto kind_of = [&](int k) { return k < nj0 ? w->h_joints[k].type : defs[k - nj0].type; };
   for (int i = 0; i < n; ++i) {
     const b2g_joint_def& d = defs[i];
-    if (d.type != B2G_JOINT_DISTANCE && d.type != B2G_JOINT_REVOLUTE && d.type != B2G_JOINT_WELD &&
-        d.type != B2G_JOINT_PRISMATIC) {
-      set_error("b2g_world_add_joints: joint " + std::to_string(i) +
-                ": only distance, revolute, weld and prismatic joints");
+    const std::string who = "b2g_world_add_joints: joint " + std::to_string(i);
+    if (d.type < B2G_JOINT_DISTANCE || d.type > B2G_JOINT_WHEEL) {
+      set_error(who + ": unknown kind");
       return B2G_ERR_UNSUPPORTED;
     }
     if (d.body_a < 0 || d.body_a >= w->nb || d.body_b < 0 || d.body_b >= w->nb || d.body_a == d.body_b ||
         w->h_bworld[d.body_a] != w->h_bworld[d.body_b]) {
-      set_error("b2g_world_add_joints: joint " + std::to_string(i) + ": bad bodies");
+      set_error(who + ": bad bodies");
       return B2G_ERR_INVALID_ARGUMENT;
     }
+    if (d.type == B2G_JOINT_PULLEY && d.ratio == 0.0f) {
+      set_error(who + ": a pulley's ratio must not be 0");
+      return B2G_ERR_INVALID_ARGUMENT;
+    }
+    if (d.type == B2G_JOINT_GEAR) {
+      // b2_gear_joint::new (src/private/dynamics/joints/b2_gear_joint.rs:8-20): two revolute or
+      // prismatic joints created before it
+      const int avail = nj0 + i;
+      if (d.joint1 < 0 || d.joint1 >= avail || d.joint2 < 0 || d.joint2 >= avail ||
+          (kind_of(d.joint1) != B2G_JOINT_REVOLUTE && kind_of(d.joint1) != B2G_JOINT_PRISMATIC) ||
+          (kind_of(d.joint2) != B2G_JOINT_REVOLUTE && kind_of(d.joint2) != B2G_JOINT_PRISMATIC)) {
+        set_error(who + ": a gear couples two revolute or prismatic joints created before it");
+        return B2G_ERR_INVALID_ARGUMENT;
+      }
+    }
+    if (d.type == B2G_JOINT_MOUSE || d.type == B2G_JOINT_GEAR) poses = true;
   }
+  int st;
+  if (poses && w->dev_nb > 0 && (st = download_state(w))) return st;
   for (int i = 0; i < n; ++i) {
     b2g_joint_def d = defs[i];
+    float constant = 0.0f;
     if (d.type == B2G_JOINT_DISTANCE) {
       // b2_distance_joint_new (src/private/dynamics/joints/b2_distance_joint.rs:44-78)
       d.length = std::max(d.length, kLinearSlop);
       d.min_length = std::max(d.min_length, kLinearSlop);
       d.max_length = std::max(d.max_length, d.min_length);
+    } else if (d.type == B2G_JOINT_MOUSE) {
+      // B2mouseJoint::new (src/joints/b2_mouse_joint.rs:141-150): body B's anchor at the target,
+      // b2_mul_t_transform_by_vec2 of body B's transform
+      const float4 xf = w->h_xf[d.body_b];
+      const float px = d.target[0] - xf.x, py = d.target[1] - xf.y;
+      d.local_anchor_b[0] = xf.w * px + xf.z * py;
+      d.local_anchor_b[1] = -xf.z * px + xf.w * py;
+    } else if (d.type == B2G_JOINT_PULLEY) {
+      // b2_pulley_joint::new (src/private/dynamics/joints/b2_pulley_joint.rs:17)
+      constant = d.length_a + d.ratio * d.length_b;
+    } else if (d.type == B2G_JOINT_GEAR) {
+      // the coupled joints' coordinates at the bodies' current poses (b2_gear_joint.rs:21-85)
+      float coordinate[2];
+      for (int side = 0; side < 2; ++side) {
+        const int k = side == 0 ? d.joint1 : d.joint2;
+        const b2g_joint_def& c = w->h_joints[k];  // (joints of this call before it are pushed already)
+        const float4 pa = w->h_pos[c.body_b], pc = w->h_pos[c.body_a];
+        if (c.type == B2G_JOINT_REVOLUTE) {
+          coordinate[side] = (pa.z - pc.z) - c.reference_angle;
+        } else {
+          const float4 xa = w->h_xf[c.body_b], xc = w->h_xf[c.body_a];
+          V2 axis = v2(c.local_axis_a[0], c.local_axis_a[1]);
+          normalize(&axis);
+          const Rot qa{xa.z, xa.w}, qc{xc.z, xc.w};
+          const V2 pC = v2(c.local_anchor_a[0], c.local_anchor_a[1]);
+          const V2 pA = mulT(qc, mul(qa, v2(c.local_anchor_b[0], c.local_anchor_b[1])) +
+                                     (v2(xa.x, xa.y) - v2(xc.x, xc.y)));
+          coordinate[side] = dot(pA - pC, axis);
+        }
+      }
+      constant = coordinate[0] + d.ratio * coordinate[1];
     }
     w->h_joints.push_back(d);
+    w->h_joint_constant.push_back(constant);
   }
   w->nj = int(w->h_joints.size());
-  // colours: first fit over the dynamic bodies, joints in creation order
+  // colours: first fit over the dynamic bodies (a gear's four), joints in creation order
   std::vector<unsigned long long> used(w->nb, 0ull);
   std::vector<int> colour(w->nj);
   int ncol = 0;
   for (int i = 0; i < w->nj; ++i) {
     const b2g_joint_def& d = w->h_joints[i];
-    const bool da = w->host_type[d.body_a] == B2G_DYNAMIC, db = w->host_type[d.body_b] == B2G_DYNAMIC;
-    const unsigned long long busy = (da ? used[d.body_a] : 0ull) | (db ? used[d.body_b] : 0ull);
+    int jb[4] = {d.body_a, d.body_b, -1, -1};
+    if (d.type == B2G_JOINT_GEAR) {
+      jb[2] = w->h_joints[d.joint1].body_a;
+      jb[3] = w->h_joints[d.joint2].body_a;
+    }
+    unsigned long long busy = 0ull;
+    for (int b : jb)
+      if (b >= 0 && w->host_type[b] == B2G_DYNAMIC) busy |= used[b];
     int c = 0;
     while (c < 64 && (busy >> c) & 1ull) ++c;
     if (c == 64) { set_error("b2g_world_add_joints: a body has more than 63 joints"); return B2G_ERR_CAPACITY; }
     colour[i] = c;
-    if (da) used[d.body_a] |= 1ull << c;
-    if (db) used[d.body_b] |= 1ull << c;
+    for (int b : jb)
+      if (b >= 0 && w->host_type[b] == B2G_DYNAMIC) used[b] |= 1ull << c;
     ncol = std::max(ncol, c + 1);
   }
   w->h_joint_colour_off.assign(ncol + 1, 0);
@@ -996,6 +1096,32 @@ int b2g_world_add_joints(b2g_world* h, int32_t n, const b2g_joint_def* defs) {
   // the next collide works on a contact set found with the joints' collision filter
   // (create_joint flags the bodies' contacts for filtering, src/private/dynamics/b2_world.rs:257-271)
   w->need_broadphase = true;
+  return B2G_OK;
+}
+
+int b2g_world_set_joint_target(b2g_world* h, int32_t joint, float x, float y) {
+  if (!h || joint < 0 || joint >= h->w.nj || h->w.h_joints[joint].type != B2G_JOINT_MOUSE) {
+    set_error("b2g_world_set_joint_target: not a mouse joint");
+    return B2G_ERR_INVALID_ARGUMENT;
+  }
+  World* w = &h->w;
+  b2g_joint_def& d = w->h_joints[joint];
+  if (d.target[0] == x && d.target[1] == y) return B2G_OK;
+  d.target[0] = x;
+  d.target[1] = y;
+  int st;
+  if ((st = ensure_uploaded(w))) return st;
+  // the joint row (target in d1.xy) and B2body::set_awake(true) of body B (src/b2_body.rs:783-801)
+  const float4 d1 = make_float4(x, y, d.max_force, d.stiffness);
+  B2G_HIP(hipMemcpyAsync(w->j.d1.p + joint, &d1, sizeof(float4), hipMemcpyHostToDevice, w->stream));
+  const int b = d.body_b;
+  if (w->host_type[b] != B2G_STATIC) {
+    const int one = 1;
+    const float zero = 0.0f;
+    B2G_HIP(hipMemcpyAsync(w->b.awake.p + b, &one, sizeof(int), hipMemcpyHostToDevice, w->stream));
+    B2G_HIP(hipMemcpyAsync(w->b.sleep_time.p + b, &zero, sizeof(float), hipMemcpyHostToDevice, w->stream));
+  }
+  B2G_HIP(hipStreamSynchronize(w->stream));
   return B2G_OK;
 }
 
@@ -1018,7 +1144,9 @@ int b2g_world_get_joint_impulses(b2g_world* h, float* out, int32_t n) {
     o[2] = imp[i].z;
     o[3] = imp[i].w;
     o[4] = imp2[i];
-    if (w->h_joints[i].type != B2G_JOINT_REVOLUTE && w->h_joints[i].type != B2G_JOINT_PRISMATIC) o[3] = o[4] = 0.0f;
+    // only revolute, prismatic and wheel joints keep limit impulses in the last two slots
+    const int t = w->h_joints[i].type;
+    if (t != B2G_JOINT_REVOLUTE && t != B2G_JOINT_PRISMATIC && t != B2G_JOINT_WHEEL) o[3] = o[4] = 0.0f;
   }
   return B2G_OK;
 }

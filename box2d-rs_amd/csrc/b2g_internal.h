@@ -253,15 +253,18 @@ struct Tape {
 // indexed by joint creation order.
 struct JointArrays {
   DevArray<int2> bodies;   // body A, body B
-  DevArray<int4> type;     // B2G_JOINT_*, flags (bit0 collide_connected, bit1 limit, bit2 motor)
+  DevArray<int4> type;     // B2G_JOINT_*, flags (bit0 collide_connected, bit1 limit, bit2 motor; gear:
+                           // bit3 / bit4 joint 1 / 2 prismatic), gear: bodies C, D
   DevArray<float4> d0;     // local anchor A, local anchor B
   DevArray<float4> d1;     // distance: length, min, max, stiffness; revolute: reference, lower, upper, max torque
   DevArray<float4> d2;     // distance: damping; revolute: motor speed
+  DevArray<float4> d3;     // the other kinds' fourth parameter record (see upload_joints)
   DevArray<float4> imp;    // distance: impulse, lower, upper; revolute: impulse.x, .y, motor, lower
   DevArray<float> imp2;    // revolute: upper impulse
   DevArray<float4> r;      // rA, rB of the step
   DevArray<float4> k;      // distance: u, mass, soft mass; revolute: K.ex.x, K.ex.y (= K.ey.x), K.ey.y, axial mass
   DevArray<float4> k2;     // distance: gamma, bias, current length; revolute: angle
+  DevArray<float4> k3;     // wheel and gear: the rest of the step's solver data
   DevArray<int> order;     // joints in colour order
   DevArray<int> nc_off;    // per body: offsets into nc_list (nb + 1)
   DevArray<int> nc_list;   // per body: bodies it is joined to by a joint that forbids their collision
@@ -384,6 +387,7 @@ struct World {
   int nj = 0, dev_nj = 0;
   int n_nocollide = 0;  // entries of the per-body no-collide lists
   std::vector<b2g_joint_def> h_joints;
+  std::vector<float> h_joint_constant;  // per joint: a pulley's or a gear's constant
   std::vector<int> h_joint_order, h_joint_colour_off;
   bool joints_dirty = false;
   JointArrays j;
@@ -472,10 +476,10 @@ struct JointArgs {
   const int* order;
   const int2* jb;
   const int4* jt;
-  const float4 *jd0, *jd1, *jd2;
+  const float4 *jd0, *jd1, *jd2, *jd3;
   float4* jimp;
   float* jimp2;
-  float4 *jr, *jk, *jk2;
+  float4 *jr, *jk, *jk2, *jk3;
   const int* btype;
   float4* pos;
   float4* vel;

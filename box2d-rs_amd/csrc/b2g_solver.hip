@@ -2043,9 +2043,10 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
                  w->b.type.p, s.adj_off.p, s.adj.p, s.inv_order.p, va, iv);
   // joints (b2g_joints.hip): initialised after the contacts' warm start, solved before the
   // contacts in each velocity iteration and after them in each position iteration
-  const JointArgs ja{w->j.order.p, w->j.bodies.p, w->j.type.p, w->j.d0.p, w->j.d1.p, w->j.d2.p, w->j.imp.p,
-                     w->j.imp2.p, w->j.r.p, w->j.k.p, w->j.k2.p, w->b.type.p, w->b.pos.p, w->b.vel.p, w->b.mass.p,
-                     iv, s.isl_violated.p, s.isl_done.p, dt, 1.0f / dt, dt_ratio, int(w->warm_starting)};
+  const JointArgs ja{w->j.order.p, w->j.bodies.p, w->j.type.p, w->j.d0.p,   w->j.d1.p,   w->j.d2.p,
+                     w->j.d3.p,    w->j.imp.p,    w->j.imp2.p, w->j.r.p,    w->j.k.p,    w->j.k2.p,
+                     w->j.k3.p,    w->b.type.p,   w->b.pos.p,  w->b.vel.p,  w->b.mass.p, iv,
+                     s.isl_violated.p, s.isl_done.p, dt, 1.0f / dt, dt_ratio, int(w->warm_starting)};
   if (w->nj > 0) launch_joint_colours(w, 0, ja);
   if (per_world) {
     launch_worlds_velocity(w, static_cast<long long>(nc) * vel_iters, s.cw_off.p, ncolors, vel_iters,

@@ -25,6 +25,12 @@ pub const B2G_JOINT_DISTANCE: i32 = 1;
 pub const B2G_JOINT_PRISMATIC: i32 = 6;
 pub const B2G_JOINT_REVOLUTE: i32 = 8;
 pub const B2G_JOINT_WELD: i32 = 9;
+pub const B2G_JOINT_FRICTION: i32 = 2;
+pub const B2G_JOINT_GEAR: i32 = 3;
+pub const B2G_JOINT_MOTOR: i32 = 4;
+pub const B2G_JOINT_MOUSE: i32 = 5;
+pub const B2G_JOINT_PULLEY: i32 = 7;
+pub const B2G_JOINT_WHEEL: i32 = 10;
 pub const B2G_PHASE_COLLIDE: i32 = 0;
 pub const B2G_PHASE_SOLVE: i32 = 1;
 pub const B2G_PHASE_FIND: i32 = 2;
@@ -77,6 +83,9 @@ pub struct b2g_joint_def {
     pub length: f32, pub min_length: f32, pub max_length: f32, pub stiffness: f32, pub damping: f32,
     pub reference_angle: f32, pub lower_angle: f32, pub upper_angle: f32, pub max_motor_torque: f32,
     pub motor_speed: f32, pub enable_limit: i32, pub enable_motor: i32, pub local_axis_a: [f32; 2],
+    pub max_force: f32, pub max_torque: f32, pub linear_offset: [f32; 2], pub angular_offset: f32,
+    pub correction_factor: f32, pub target: [f32; 2], pub ground_anchor_a: [f32; 2], pub ground_anchor_b: [f32; 2],
+    pub length_a: f32, pub length_b: f32, pub ratio: f32, pub joint1: i32, pub joint2: i32,
 }
 #[repr(C)] #[derive(Clone, Copy, Default)]
 pub struct b2g_manifold_point { pub local_point: [f32; 2], pub normal_impulse: f32, pub tangent_impulse: f32, pub id: u32, pub pad: i32 }
@@ -124,6 +133,7 @@ extern "C" {
     pub fn b2g_world_get_contacts(w: *mut b2g_world, cap: i32, fixtures_ab: *mut i32, flags: *mut i32,
                                   manifolds: *mut b2g_manifold, count: *mut i32) -> c_int;
     pub fn b2g_world_add_joints(w: *mut b2g_world, n: i32, defs: *const b2g_joint_def) -> c_int;
+    pub fn b2g_world_set_joint_target(w: *mut b2g_world, joint: i32, x: f32, y: f32) -> c_int;
     pub fn b2g_world_get_joint_impulses(w: *mut b2g_world, out: *mut f32, n: i32) -> c_int;
     pub fn b2g_world_set_joint_impulses(w: *mut b2g_world, inp: *const f32, n: i32) -> c_int;
     pub fn b2g_world_get_shapes(w: *mut b2g_world, out: *mut b2g_shape_def, n: i32) -> c_int;

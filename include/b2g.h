@@ -77,17 +77,37 @@ typedef struct b2g_fixture_def {
 
 /* Joint kinds of b2g_joint_def::type: B2jointType values (src/b2_joint.rs:46-58). */
 #define B2G_JOINT_DISTANCE 1
+#define B2G_JOINT_FRICTION 2
+#define B2G_JOINT_GEAR 3
+#define B2G_JOINT_MOTOR 4
+#define B2G_JOINT_MOUSE 5
 #define B2G_JOINT_PRISMATIC 6
+#define B2G_JOINT_PULLEY 7
 #define B2G_JOINT_REVOLUTE 8
 #define B2G_JOINT_WELD 9
+#define B2G_JOINT_WHEEL 10
 
-/* Joint definition: B2jointDef (src/b2_joint.rs:149-164) with B2distanceJointDef
- * (src/joints/b2_distance_joint.rs), B2revoluteJointDef (src/joints/b2_revolute_joint.rs:41-73),
- * B2weldJointDef (src/joints/b2_weld_joint.rs: local anchors, reference_angle, stiffness, damping) or
- * B2prismaticJointDef (src/joints/b2_prismatic_joint.rs: local anchors, local_axis_a,
- * reference_angle, lower/upper translation in lower_angle/upper_angle, max motor force in
- * max_motor_torque, motor_speed, enable_limit, enable_motor); the bodies are creation indices.
- * 88 bytes. */
+/* Joint definition: B2jointDef (src/b2_joint.rs:149-164) with the fields of its kind; the bodies
+ * are creation indices. 156 bytes.
+ *   distance  B2distanceJointDef (src/joints/b2_distance_joint.rs): anchors, length, min/max
+ *             length, stiffness, damping
+ *   revolute  B2revoluteJointDef (src/joints/b2_revolute_joint.rs:41-73): anchors, reference_angle,
+ *             limit (lower/upper_angle), motor (motor_speed, max_motor_torque)
+ *   weld      B2weldJointDef (src/joints/b2_weld_joint.rs): anchors, reference_angle, stiffness, damping
+ *   prismatic B2prismaticJointDef (src/joints/b2_prismatic_joint.rs): anchors, local_axis_a,
+ *             reference_angle, lower/upper translation in lower_angle/upper_angle, max motor force
+ *             in max_motor_torque, motor_speed
+ *   wheel     B2wheelJointDef (src/joints/b2_wheel_joint.rs:39-75): anchors, local_axis_a, lower/upper
+ *             translation in lower_angle/upper_angle, max_motor_torque, motor_speed, stiffness, damping
+ *   friction  B2frictionJointDef (src/joints/b2_friction_joint.rs): anchors, max_force, max_torque
+ *   motor     B2motorJointDef (src/joints/b2_motor_joint.rs): linear_offset, angular_offset,
+ *             max_force, max_torque, correction_factor
+ *   mouse     B2mouseJointDef (src/joints/b2_mouse_joint.rs:35-52): target, max_force, stiffness,
+ *             damping (body B's anchor is the target in body B's frame when the joint is created)
+ *   pulley    B2pulleyJointDef (src/joints/b2_pulley_joint.rs): ground anchors, anchors, length_a,
+ *             length_b, ratio
+ *   gear      B2gearJointDef (src/joints/b2_gear_joint.rs): joint1, joint2 (creation indices of
+ *             revolute or prismatic joints; bodies A and B as their bodies B), ratio */
 typedef struct b2g_joint_def {
   int32_t type;
   int32_t body_a, body_b;
@@ -95,8 +115,13 @@ typedef struct b2g_joint_def {
   float local_anchor_a[2], local_anchor_b[2];
   float length, min_length, max_length, stiffness, damping;                       /* distance (weld: stiffness, damping) */
   float reference_angle, lower_angle, upper_angle, max_motor_torque, motor_speed; /* revolute (weld: reference_angle) */
-  int32_t enable_limit, enable_motor;                                             /* revolute, prismatic */
-  float local_axis_a[2];                                                          /* prismatic */
+  int32_t enable_limit, enable_motor;                                             /* revolute, prismatic, wheel */
+  float local_axis_a[2];                                                          /* prismatic, wheel */
+  float max_force, max_torque;                                                    /* friction, motor, mouse */
+  float linear_offset[2], angular_offset, correction_factor;                      /* motor */
+  float target[2];                                                                /* mouse */
+  float ground_anchor_a[2], ground_anchor_b[2], length_a, length_b, ratio;       /* pulley (gear: ratio) */
+  int32_t joint1, joint2;                                                         /* gear */
 } b2g_joint_def;
 
 /* Body state as B2body holds it: transform m_xf, sweep, velocities. 64 bytes. */
@@ -237,11 +262,17 @@ int b2g_world_get_contacts(b2g_world* world, int32_t cap, int32_t* fixtures_ab, 
  * constructors' clamping (distance: length, min_length >= linear slop, max_length >= min_length,
  * src/private/dynamics/joints/b2_distance_joint.rs:44-78). Joints join islands, are solved before
  * the contacts in each velocity iteration and after them in each position iteration, and a joint
- * with collide_connected = 0 removes its bodies' contacts from the next step on. Kinds other than
- * distance, revolute, weld and prismatic: B2G_ERR_UNSUPPORTED. Not with a tape (b2g_tape_begin refuses joints). */
+ * with collide_connected = 0 removes its bodies' contacts from the next step on. A gear joint
+ * couples two existing revolute or prismatic joints; its constant, and a mouse joint's anchor,
+ * come from the bodies' current poses (src/private/dynamics/joints/b2_gear_joint.rs:8-110,
+ * src/joints/b2_mouse_joint.rs:141-150). Not with a tape (b2g_tape_begin refuses joints). */
 int b2g_world_add_joints(b2g_world* world, int32_t n, const b2g_joint_def* defs);
+/* B2mouseJoint::set_target (src/joints/b2_mouse_joint.rs:114-119): a new target wakes body B. */
+int b2g_world_set_joint_target(b2g_world* world, int32_t joint, float x, float y);
 /* Warm-start impulses of every joint, 5 floats each (distance: impulse, lower, upper, 0, 0;
- * revolute and prismatic: impulse.x, impulse.y, motor, lower, upper; weld: impulse.x, .y, .z, 0, 0). */
+ * revolute and prismatic: impulse.x, impulse.y, motor, lower, upper; weld: impulse.x, .y, .z, 0, 0;
+ * wheel: impulse, spring, motor, lower, upper; friction and motor: linear.x, linear.y, angular, 0,
+ * 0; mouse: impulse.x, impulse.y, 0, 0, 0; pulley and gear: impulse, 0, 0, 0, 0). */
 int b2g_world_get_joint_impulses(b2g_world* world, float* out, int32_t n);
 /* Overwrites them (same layout): the checkpoint extension of the serde world format restores a
  * world's joints with their impulses. */

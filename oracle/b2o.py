@@ -42,7 +42,11 @@ def _dtypes(real):
                           ("min_length", real), ("max_length", real), ("stiffness", real), ("damping", real),
                           ("reference_angle", real), ("lower_angle", real), ("upper_angle", real),
                           ("max_motor_torque", real), ("motor_speed", real), ("enable_limit", "<i4"),
-                          ("enable_motor", "<i4"), ("local_axis_a", real, 2)], align=True)
+                          ("enable_motor", "<i4"), ("local_axis_a", real, 2), ("max_force", real),
+                          ("max_torque", real), ("linear_offset", real, 2), ("angular_offset", real),
+                          ("correction_factor", real), ("target", real, 2), ("ground_anchor_a", real, 2),
+                          ("ground_anchor_b", real, 2), ("length_a", real), ("length_b", real), ("ratio", real),
+                          ("joint1", "<i4"), ("joint2", "<i4")], align=True)
     joint_state = np.dtype([("impulse", real, 5), ("pad", real, 3)], align=True)
     return dict(body_def=body_def, shape_def=shape_def, fixture_def=fixture_def, body_state=body_state,
                 manifold=manifold, shape=shape, joint_def=joint_def, joint_state=joint_state)
@@ -97,6 +101,7 @@ class Oracle:
         L.b2o_world_toi_events.argtypes = [P]
         L.b2o_world_toi_events.restype = I
         L.b2o_world_set_sub_stepping.argtypes = [P, I]
+        L.b2o_world_set_joint_target.argtypes = [P, I, R, R]
         L.b2o_world_step_complete.argtypes = [P]
         L.b2o_world_step_complete.restype = I
         L.b2o_world_manifold.argtypes = [I, P, P, P, P, P, P]
@@ -273,6 +278,10 @@ class OracleWorld:
         if made != len(jd):
             raise ValueError("joint %d: bad bodies or type" % made)
         self.n_joints = getattr(self, "n_joints", 0) + len(jd)
+
+    def set_joint_target(self, joint, target):
+        """B2mouseJoint::set_target."""
+        self.o.lib.b2o_world_set_joint_target(self.h, int(joint), float(target[0]), float(target[1]))
 
     def joints(self):
         out = np.zeros(getattr(self, "n_joints", 0), self.o.dt["joint_state"])

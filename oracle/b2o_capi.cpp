@@ -43,14 +43,20 @@ struct B2oShapeDef {
 // Joint definition: B2distanceJointDef / B2revoluteJointDef (src/joints/b2_distance_joint.rs,
 // src/joints/b2_revolute_joint.rs) with the bodies as creation indices
 struct B2oJointDef {
-  int32_t type;  // 1 distance, 6 prismatic, 8 revolute, 9 weld (B2jointType)
+  int32_t type;  // B2jointType: 1 distance, 2 friction, 3 gear, 4 motor, 5 mouse, 6 prismatic, 7 pulley,
+                 // 8 revolute, 9 weld, 10 wheel
   int32_t body_a, body_b;
   int32_t collide_connected;
   real local_anchor_a[2], local_anchor_b[2];
   real length, min_length, max_length, stiffness, damping;
   real reference_angle, lower_angle, upper_angle, max_motor_torque, motor_speed;
   int32_t enable_limit, enable_motor;
-  real local_axis_a[2];  // prismatic
+  real local_axis_a[2];  // prismatic, wheel
+  real max_force, max_torque;                                   // friction, motor, mouse
+  real linear_offset[2], angular_offset, correction_factor;     // motor
+  real target[2];                                               // mouse
+  real ground_anchor_a[2], ground_anchor_b[2], length_a, length_b, ratio;  // pulley (gear: ratio)
+  int32_t joint1, joint2;                                       // gear
 };
 // Joint state: the warm-start impulses (distance: impulse, lower, upper; revolute: impulse.x,
 // impulse.y, motor, lower, upper)
@@ -404,8 +410,7 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
   for (int i = 0; i < n; ++i) {
     const B2oJointDef& d = defs[i];
     if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) return i;
-    if (d.type != kDistanceJoint && d.type != kRevoluteJoint && d.type != kWeldJoint && d.type != kPrismaticJoint)
-      return i;
+    if (d.type < kDistanceJoint || d.type > kWheelJoint) return i;
     Joint j;
     j.type = d.type;
     j.a = w->bodies[d.body_a];
@@ -437,6 +442,92 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
       j.reference_angle = d.reference_angle;
       j.stiffness = d.stiffness;
       j.damping = d.damping;
+    } else if (d.type == kFrictionJoint || d.type == kMotorJoint) {
+      // B2frictionJoint::new / B2motorJoint::new (src/joints/b2_friction_joint.rs,
+      // src/joints/b2_motor_joint.rs)
+      j.max_force = d.max_force;
+      j.max_torque = d.max_torque;
+      j.linear_offset = Vec2(d.linear_offset[0], d.linear_offset[1]);
+      j.angular_offset = d.angular_offset;
+      j.correction_factor = d.correction_factor;
+    } else if (d.type == kMouseJoint) {
+      // B2mouseJoint::new (src/joints/b2_mouse_joint.rs:141-150): body B's anchor at the target
+      j.target = Vec2(d.target[0], d.target[1]);
+      j.local_anchor_b = mulT(j.b->xf, j.target);
+      j.max_force = d.max_force;
+      j.stiffness = d.stiffness;
+      j.damping = d.damping;
+    } else if (d.type == kWheelJoint) {
+      // B2wheelJoint::new (src/joints/b2_wheel_joint.rs:329-360): the axis as given, the
+      // perpendicular cross(1, axis); limits as translations, motor as a torque, a spring
+      j.local_xaxis_a = Vec2(d.local_axis_a[0], d.local_axis_a[1]);
+      j.local_yaxis_a = cross(real(1.0), j.local_xaxis_a);
+      j.lower_translation = d.lower_angle;
+      j.upper_translation = d.upper_angle;
+      j.max_motor_torque = d.max_motor_torque;
+      j.motor_speed = d.motor_speed;
+      j.stiffness = d.stiffness;
+      j.damping = d.damping;
+      j.enable_limit = d.enable_limit != 0;
+      j.enable_motor = d.enable_motor != 0;
+    } else if (d.type == kPulleyJoint) {
+      // b2_pulley_joint::new (src/private/dynamics/joints/b2_pulley_joint.rs:8-47)
+      if (d.ratio == 0) return i;
+      j.ground_anchor_a = Vec2(d.ground_anchor_a[0], d.ground_anchor_a[1]);
+      j.ground_anchor_b = Vec2(d.ground_anchor_b[0], d.ground_anchor_b[1]);
+      j.length_a = d.length_a;
+      j.length_b = d.length_b;
+      j.ratio = d.ratio;
+      j.constant = d.length_a + j.ratio * d.length_b;
+    } else if (d.type == kGearJoint) {
+      // b2_gear_joint::new (src/private/dynamics/joints/b2_gear_joint.rs:8-110): bodies C and D are
+      // the coupled joints' bodies A; the constant from the bodies' current poses
+      const int nj = int(w->joints.size());
+      if (d.joint1 < 0 || d.joint1 >= nj || d.joint2 < 0 || d.joint2 >= nj) return i;
+      const Joint* j1 = w->joints[d.joint1];
+      const Joint* j2 = w->joints[d.joint2];
+      if ((j1->type != kRevoluteJoint && j1->type != kPrismaticJoint) ||
+          (j2->type != kRevoluteJoint && j2->type != kPrismaticJoint))
+        return i;
+      j.type_a = j1->type;
+      j.type_b = j2->type;
+      j.c = j1->a;
+      j.d = j2->a;
+      j.ratio = d.ratio;
+      real coordinate_a, coordinate_b;
+      {
+        const Body* bA = j1->b;
+        const Body* bC = j1->a;
+        j.local_anchor_c = j1->local_anchor_a;
+        j.local_anchor_a = j1->local_anchor_b;
+        j.reference_angle_a = j1->reference_angle;
+        if (j1->type == kRevoluteJoint) {
+          j.local_axis_c.set_zero();
+          coordinate_a = bA->sweep.a - bC->sweep.a - j.reference_angle_a;
+        } else {
+          j.local_axis_c = j1->local_xaxis_a;
+          const Vec2 pC = j.local_anchor_c;
+          const Vec2 pA = mulT(bC->xf.q, mul(bA->xf.q, j.local_anchor_a) + (bA->xf.p - bC->xf.p));
+          coordinate_a = dot(pA - pC, j.local_axis_c);
+        }
+      }
+      {
+        const Body* bB = j2->b;
+        const Body* bD = j2->a;
+        j.local_anchor_d = j2->local_anchor_a;
+        j.local_anchor_b = j2->local_anchor_b;
+        j.reference_angle_b = j2->reference_angle;
+        if (j2->type == kRevoluteJoint) {
+          j.local_axis_d.set_zero();
+          coordinate_b = bB->sweep.a - bD->sweep.a - j.reference_angle_b;
+        } else {
+          j.local_axis_d = j2->local_xaxis_a;
+          const Vec2 pD = j.local_anchor_d;
+          const Vec2 pB = mulT(bD->xf.q, mul(bB->xf.q, j.local_anchor_b) + (bB->xf.p - bD->xf.p));
+          coordinate_b = dot(pB - pD, j.local_axis_d);
+        }
+      }
+      j.constant = coordinate_a + j.ratio * coordinate_b;
     } else {
       j.reference_angle = d.reference_angle;
       j.lower_angle = d.lower_angle;
@@ -449,6 +540,17 @@ int b2o_world_add_joints(void* wp, int n, const B2oJointDef* defs) {
     w->create_joint(j);
   }
   return n;
+}
+
+// B2mouseJoint::set_target (src/joints/b2_mouse_joint.rs:114-119)
+void b2o_world_set_joint_target(void* wp, int joint, real x, real y) {
+  World* w = static_cast<World*>(wp);
+  Joint* j = w->joints[joint];
+  const Vec2 t(x, y);
+  if (!(t == j->target)) {
+    j->b->set_awake(true);
+    j->target = t;
+  }
 }
 
 void b2o_world_get_joints(void* wp, B2oJointState* out) {
@@ -471,6 +573,21 @@ void b2o_world_get_joints(void* wp, B2oJointState* out) {
       o.impulse[0] = j->impulse3[0];
       o.impulse[1] = j->impulse3[1];
       o.impulse[2] = j->impulse3[2];
+    } else if (j->type == kFrictionJoint || j->type == kMotorJoint) {
+      o.impulse[0] = j->linear_impulse.x;
+      o.impulse[1] = j->linear_impulse.y;
+      o.impulse[2] = j->angular_impulse;
+    } else if (j->type == kMouseJoint) {
+      o.impulse[0] = j->impulse2.x;
+      o.impulse[1] = j->impulse2.y;
+    } else if (j->type == kWheelJoint) {
+      o.impulse[0] = j->impulse;
+      o.impulse[1] = j->spring_impulse;
+      o.impulse[2] = j->motor_impulse;
+      o.impulse[3] = j->lower_impulse;
+      o.impulse[4] = j->upper_impulse;
+    } else if (j->type == kPulleyJoint || j->type == kGearJoint) {
+      o.impulse[0] = j->impulse;
     } else {
       o.impulse[0] = j->impulse2.x;
       o.impulse[1] = j->impulse2.y;

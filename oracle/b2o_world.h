@@ -112,8 +112,19 @@ struct ContactEdge {
 
 struct Filter { uint16_t category = 1, mask = 0xFFFF; int16_t group = 0; };
 
-// Joints: B2jointType values (src/b2_joint.rs:46-58) of the two restated kinds
-enum JointType { kDistanceJoint = 1, kPrismaticJoint = 6, kRevoluteJoint = 8, kWeldJoint = 9 };
+// Joints: B2jointType values (src/b2_joint.rs:46-58)
+enum JointType {
+  kDistanceJoint = 1,
+  kFrictionJoint = 2,
+  kGearJoint = 3,
+  kMotorJoint = 4,
+  kMouseJoint = 5,
+  kPrismaticJoint = 6,
+  kPulleyJoint = 7,
+  kRevoluteJoint = 8,
+  kWeldJoint = 9,
+  kWheelJoint = 10
+};
 struct Joint;
 // src/b2_joint.rs B2jointEdge; a body's list is push_front / newest-first
 struct JointEdge {
@@ -199,6 +210,29 @@ struct Joint {
   // weld (src/joints/b2_weld_joint.rs): impulse (x, y, angular), the 3x3 mass (symmetric)
   real impulse3[3] = {0, 0, 0};
   real mass33[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // [column][row]
+  // friction and motor (src/joints/b2_friction_joint.rs, b2_motor_joint.rs): linear and angular
+  // impulses and masses, the motor's offsets, errors and correction factor
+  real max_force = 0, max_torque = 0, angular_impulse = 0, angular_mass = 0;
+  Vec2 linear_impulse, linear_offset, linear_error;
+  real angular_offset = 0, angular_error = 0, correction_factor = 0;
+  Mat22 linear_mass;
+  // mouse (src/joints/b2_mouse_joint.rs): target, soft point constraint (impulse in impulse2)
+  Vec2 target, mouse_c;
+  real beta = 0;
+  Mat22 mouse_mass;
+  // wheel (src/joints/b2_wheel_joint.rs): axes in A's frame, spring, motor torque
+  Vec2 ax, ay;
+  real s_ax = 0, s_bx = 0, s_ay = 0, s_by = 0, spring_mass = 0, spring_impulse = 0, motor_mass = 0;
+  // pulley (src/joints/b2_pulley_joint.rs)
+  Vec2 ground_anchor_a, ground_anchor_b, u_a, u_b;
+  real length_a = 0, length_b = 0, ratio = 1, constant = 0;
+  // gear (src/joints/b2_gear_joint.rs): bodies C and D of the coupled joints, their frames
+  Body *c = nullptr, *d = nullptr;
+  int type_a = 0, type_b = 0, index_c = 0, index_d = 0;
+  Vec2 local_anchor_c, local_anchor_d, local_axis_c, local_axis_d, lc_a, lc_b, lc_c, lc_d, jv_ac, jv_bd;
+  real reference_angle_a = 0, reference_angle_b = 0;
+  real m_a = 0, m_b = 0, m_c = 0, m_d = 0, i_a = 0, i_b = 0, i_c = 0, i_d = 0;
+  real jw_a = 0, jw_b = 0, jw_c = 0, jw_d = 0;
   // solver temporaries
   int index_a = 0, index_b = 0;
   Vec2 local_center_a, local_center_b, rA, rB;

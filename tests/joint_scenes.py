@@ -124,3 +124,152 @@ def sliders(angles, half=(0.5, 0.25), spacing=4.0):
         j["local_anchor_b"] = (0.0, 0.0)
         j["local_axis_a"] = (np.cos(ang), np.sin(ang))
     return bd, fb, sd, _fixture(n), jd
+
+
+def _box_shapes(n, hx, hy):
+    sd = np.zeros(n, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_BOX
+    sd["params"][:, 0] = hx
+    sd["params"][:, 1] = hy
+    return sd
+
+
+def friction_boxes(speeds, spins, forces, torques):
+    """No gravity: one static ground and, per entry, a 1 x 0.5 m box sliding and spinning, held by
+    a friction joint to the ground at its centre (each box its own island)."""
+    n = len(speeds)
+    bd = _bodies(1 + n)
+    bd[1:]["type"] = 2
+    bd[1:]["position"] = np.stack([np.arange(n) * 4.0, np.zeros(n)], 1)
+    bd[1:]["linear_velocity"] = np.asarray(speeds, np.float32)
+    bd[1:]["angular_velocity"] = spins
+    jd = np.zeros(n, b2g.JOINT_DEF)
+    jd["type"] = b2g.JOINT_FRICTION
+    jd["body_a"] = 0
+    jd["body_b"] = np.arange(1, n + 1)
+    jd["local_anchor_a"] = bd[1:]["position"]
+    jd["max_force"] = forces
+    jd["max_torque"] = torques
+    return bd, np.arange(1, n + 1, dtype=np.int32), _box_shapes(n, 0.5, 0.25), _fixture(n), jd
+
+
+def motor_boxes(starts, angles, offsets, angle_offsets, max_force=1e6, max_torque=1e6, cf=0.3):
+    """No gravity: boxes at rest driven by motor joints from the static ground (at the origin) to
+    the given offsets (each box its own island)."""
+    n = len(starts)
+    bd = _bodies(1 + n)
+    bd[1:]["type"] = 2
+    bd[1:]["position"] = starts
+    bd[1:]["angle"] = angles
+    jd = np.zeros(n, b2g.JOINT_DEF)
+    jd["type"] = b2g.JOINT_MOTOR
+    jd["body_a"] = 0
+    jd["body_b"] = np.arange(1, n + 1)
+    jd["linear_offset"] = offsets
+    jd["angular_offset"] = angle_offsets
+    jd["max_force"] = max_force
+    jd["max_torque"] = max_torque
+    jd["correction_factor"] = cf
+    return bd, np.arange(1, n + 1, dtype=np.int32), _box_shapes(n, 0.5, 0.25), _fixture(n), jd
+
+
+def mouse_boxes(starts, targets, hz, zetas, max_force=1e6):
+    """No gravity: boxes pulled by mouse joints (body A the static ground) toward targets with
+    spring frequency hz and damping ratios zetas (stiffness m w^2, damping 2 m zeta w)."""
+    n = len(starts)
+    bd = _bodies(1 + n)
+    bd[1:]["type"] = 2
+    bd[1:]["position"] = starts
+    fd = _fixture(n)
+    m = float(fd["density"][0]) * 1.0 * 0.5
+    w = 2.0 * np.pi * np.asarray(hz)
+    jd = np.zeros(n, b2g.JOINT_DEF)
+    jd["type"] = b2g.JOINT_MOUSE
+    jd["body_a"] = 0
+    jd["body_b"] = np.arange(1, n + 1)
+    jd["target"] = starts  # at creation: the target is the grabbed point (B's anchor)
+    jd["stiffness"] = m * w * w
+    jd["damping"] = 2.0 * m * np.asarray(zetas) * w
+    jd["max_force"] = max_force
+    return bd, np.arange(1, n + 1, dtype=np.int32), _box_shapes(n, 0.5, 0.25), fd, jd, np.asarray(targets), m
+
+
+def wheels_on_springs(n=3, stiffness=(40.0, 80.0, 160.0), motor_speed=0.0, torque=0.0, limit=None):
+    """Circle wheels of radius 0.4 on wheel joints to the static ground, anchored at their centres
+    with the vertical as the axis and springs at rest at the start, gravity on."""
+    bd = _bodies(1 + n)
+    bd[1:]["type"] = 2
+    xs = np.arange(n) * 3.0
+    bd[1:]["position"] = np.stack([xs, np.full(n, 9.0)], 1)
+    sd = np.zeros(n, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_CIRCLE
+    sd["radius"] = 0.4
+    jd = np.zeros(n, b2g.JOINT_DEF)
+    jd["type"] = b2g.JOINT_WHEEL
+    jd["body_a"] = 0
+    jd["body_b"] = np.arange(1, n + 1)
+    jd["local_anchor_a"] = np.stack([xs, np.full(n, 9.0)], 1)
+    jd["local_anchor_b"] = (0.0, 0.0)
+    jd["local_axis_a"] = (0.0, 1.0)
+    jd["stiffness"] = stiffness
+    if torque:
+        jd["enable_motor"] = 1
+        jd["motor_speed"] = motor_speed
+        jd["max_motor_torque"] = torque
+    if limit is not None:
+        jd["enable_limit"] = 1
+        jd["lower_angle"], jd["upper_angle"] = limit
+    return bd, np.arange(1, n + 1, dtype=np.int32), sd, _fixture(n), jd
+
+
+def atwood(mass_a, mass_b, ratio):
+    """Two boxes (1 x 1 m, densities giving the masses) hanging from ground anchors 4 m above their
+    tops on one pulley joint, gravity on, rotation fixed."""
+    bd = _bodies(3)
+    bd[1:]["type"] = 2
+    bd[1:]["fixed_rotation"] = 1
+    bd[1]["position"] = (-2.0, 5.0)
+    bd[2]["position"] = (2.0, 5.0)
+    fd = _fixture(2)
+    fd["density"] = (mass_a, mass_b)
+    jd = np.zeros(1, b2g.JOINT_DEF)
+    jd["type"] = b2g.JOINT_PULLEY
+    jd["body_a"], jd["body_b"] = 1, 2
+    jd["collide_connected"] = 1
+    jd["ground_anchor_a"] = (-2.0, 9.5)
+    jd["ground_anchor_b"] = (2.0, 9.5)
+    jd["local_anchor_a"] = (0.0, 0.5)
+    jd["local_anchor_b"] = (0.0, 0.5)
+    jd["length_a"] = 4.0
+    jd["length_b"] = 4.0
+    jd["ratio"] = ratio
+    return bd, np.arange(1, 3, dtype=np.int32), _box_shapes(2, 0.5, 0.5), fd, jd
+
+
+def geared_wheels(ratio, motor_speed, torque=1e4, prismatic_b=False):
+    """Two circles pinned to the static ground by revolute joints (or the second on a horizontal
+    prismatic joint: a rack), coupled by a gear joint; a motor drives the first. No gravity."""
+    bd = _bodies(3)
+    bd[1:]["type"] = 2
+    bd[1]["position"] = (0.0, 5.0)
+    bd[2]["position"] = (3.0, 5.0)
+    sd = np.zeros(2, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_CIRCLE
+    sd["radius"] = (1.0, 0.5)
+    jd = np.zeros(3, b2g.JOINT_DEF)
+    jd[0]["type"] = b2g.JOINT_REVOLUTE
+    jd[0]["body_a"], jd[0]["body_b"] = 0, 1
+    jd[0]["local_anchor_a"] = (0.0, 5.0)
+    jd[0]["enable_motor"], jd[0]["motor_speed"], jd[0]["max_motor_torque"] = 1, motor_speed, torque
+    jd[1]["body_a"], jd[1]["body_b"] = 0, 2
+    jd[1]["local_anchor_a"] = (3.0, 5.0)
+    if prismatic_b:
+        jd[1]["type"] = b2g.JOINT_PRISMATIC
+        jd[1]["local_axis_a"] = (1.0, 0.0)
+    else:
+        jd[1]["type"] = b2g.JOINT_REVOLUTE
+    jd[2]["type"] = b2g.JOINT_GEAR
+    jd[2]["body_a"], jd[2]["body_b"] = 1, 2
+    jd[2]["joint1"], jd[2]["joint2"] = 0, 1
+    jd[2]["ratio"] = ratio
+    return bd, np.arange(1, 3, dtype=np.int32), sd, _fixture(2), jd
