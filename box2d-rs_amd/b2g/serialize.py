@@ -36,8 +36,7 @@ reader rejects unknown members (its field enums have no catch-all variant,
 src/serialize/serialize_b2_world.rs:189-198), so only a file written without the extension is
 for the reference.
 
-Joints (distance, revolute, weld, prismatic) are ``{"jtype", "joint_def"}`` records in the world's
-joint list, newest first (src/serialize/serialize_b2_world.rs:58-129), each definition with its
+Joints are ``{"jtype", "joint_def"}`` records in the world's joint list, newest first (src/serialize/serialize_b2_world.rs:58-129), each definition with its
 ``base`` (``jtype``, ``user_data``, ``body_a``, ``body_b``, ``collide_connected``,
 src/serialize/serialize_b2_joint.rs:33-53) and the kind's fields in their serializer's order
 (src/joints/serialize/serialize_b2_*_joint.rs). A joint names its bodies by index: the writer
@@ -46,22 +45,28 @@ reference's reader looks that index up in the list of the world it has just buil
 the opposite order (serialize_b2_joint.rs:88-150 over a newest-first list), so a round trip
 through the reference attaches a joint to the mirrored body. ``read_world`` follows the
 reference's reader by default (``joint_bodies="reference"``); ``joint_bodies="file"`` takes the
-index as the writer meant it. Other joint kinds, chain shapes and disabled bodies are not part
-of the GPU path: reading them raises.
+index as the writer meant it. Gear joints go in their own list after the others and name their
+two joints by position in the written joint list (src/serialize/serialize_b2_world.rs:144-174),
+which the reference's reader also looks up mirrored; a mouse joint is not written (the
+reference's writer skips it). A checkpoint does not hold gear or mouse joints. Chain shapes and
+disabled bodies are not part of the GPU path: reading them raises.
 """
 import json
 from fractions import Fraction
 
 import numpy as np
 
-from . import (BODY_DEF, FIXTURE_DEF, JOINT_DEF, JOINT_DISTANCE, JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_WELD, MANIFOLD,
-               SHAPE_DEF, SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON_EXACT, B2gError, ERR_UNSUPPORTED, World)
+from . import (BODY_DEF, FIXTURE_DEF, JOINT_DEF, JOINT_DISTANCE, JOINT_FRICTION, JOINT_GEAR, JOINT_MOTOR, JOINT_MOUSE,
+               JOINT_PRISMATIC, JOINT_PULLEY, JOINT_REVOLUTE, JOINT_WELD, JOINT_WHEEL, MANIFOLD, SHAPE_DEF, SHAPE_CIRCLE,
+               SHAPE_EDGE, SHAPE_POLYGON_EXACT, B2gError, ERR_UNSUPPORTED, World)
 
 BODY_TYPES = ("B2StaticBody", "B2KinematicBody", "B2DynamicBody")          # src/b2_body.rs:25-31
 SHAPE_TYPES = ("ECircle", "EEdge", "EPolygon", "EChain")                  # src/b2_shape.rs:23-31
 MANIFOLD_TYPES = ("ECircles", "EFaceA", "EFaceB")                         # src/b2_collision.rs:63-67
 JOINT_TYPES = {JOINT_DISTANCE: "EDistanceJoint", JOINT_PRISMATIC: "EPrismaticJoint",  # src/b2_joint.rs:46-58
-               JOINT_REVOLUTE: "ERevoluteJoint", JOINT_WELD: "EWeldJoint"}
+               JOINT_REVOLUTE: "ERevoluteJoint", JOINT_WELD: "EWeldJoint", JOINT_WHEEL: "EWheelJoint",
+               JOINT_FRICTION: "EFrictionJoint", JOINT_MOTOR: "EMotorJoint", JOINT_PULLEY: "EPulleyJoint",
+               JOINT_GEAR: "EGearJoint", JOINT_MOUSE: "EMouseJoint"}
 LINEAR_SLOP = np.float32(0.005)
 
 
@@ -188,12 +193,15 @@ def _manifold_json(m):
             "m_type": MANIFOLD_TYPES[int(m["type"])], "point_count": int(m["point_count"])}
 
 
-def _joint_json(d, nb):
-    """A joint record; body indices are positions in the written (newest-first) body list."""
+def _joint_json(d, nb, joint_pos=None):
+    """A joint record; body indices are positions in the written (newest-first) body list, a
+    gear's joints positions in the written joint list (joint_pos: creation index -> position)."""
     t = int(d["type"])
     base = {"jtype": JOINT_TYPES[t], "user_data": None, "body_a": nb - 1 - int(d["body_a"]),
             "body_b": nb - 1 - int(d["body_b"]), "collide_connected": bool(d["collide_connected"])}
-    out = {"base": base, "local_anchor_a": _vec(*d["local_anchor_a"]), "local_anchor_b": _vec(*d["local_anchor_b"])}
+    out = {"base": base}
+    if t not in (JOINT_MOTOR, JOINT_GEAR, JOINT_PULLEY):
+        out.update({"local_anchor_a": _vec(*d["local_anchor_a"]), "local_anchor_b": _vec(*d["local_anchor_b"])})
     if t == JOINT_DISTANCE:
         # the joint's lengths as its constructor clamps them (b2_distance_joint.rs:44-78)
         length = max(np.float32(d["length"]), LINEAR_SLOP)
@@ -220,9 +228,44 @@ def _joint_json(d, nb):
                     "enable_limit": bool(d["enable_limit"]), "lower_translation": F32(d["lower_angle"]),
                     "upper_translation": F32(d["upper_angle"]), "enable_motor": bool(d["enable_motor"]),
                     "motor_speed": F32(d["motor_speed"]), "max_motor_force": F32(d["max_motor_torque"])})
+    elif t == JOINT_WHEEL:  # src/joints/serialize/serialize_b2_wheel_joint.rs:44-61 (the axis as given)
+        out.update({"local_axis_a": _vec(*d["local_axis_a"]), "enable_limit": bool(d["enable_limit"]),
+                    "lower_translation": F32(d["lower_angle"]), "upper_translation": F32(d["upper_angle"]),
+                    "enable_motor": bool(d["enable_motor"]), "motor_speed": F32(d["motor_speed"]),
+                    "max_motor_torque": F32(d["max_motor_torque"]), "stiffness": F32(d["stiffness"]),
+                    "damping": F32(d["damping"])})
+    elif t == JOINT_FRICTION:  # serialize_b2_friction_joint.rs:37-47
+        out.update({"max_force": F32(d["max_force"]), "max_torque": F32(d["max_torque"])})
+    elif t == JOINT_MOTOR:  # serialize_b2_motor_joint.rs:38-49
+        out.update({"linear_offset": _vec(*d["linear_offset"]), "angular_offset": F32(d["angular_offset"]),
+                    "max_force": F32(d["max_force"]), "max_torque": F32(d["max_torque"]),
+                    "correction_factor": F32(d["correction_factor"])})
+    elif t == JOINT_PULLEY:  # serialize_b2_pulley_joint.rs:40-53
+        out.update({"ground_anchor_a": _vec(*d["ground_anchor_a"]), "ground_anchor_b": _vec(*d["ground_anchor_b"]),
+                    "local_anchor_a": _vec(*d["local_anchor_a"]), "local_anchor_b": _vec(*d["local_anchor_b"]),
+                    "length_a": F32(d["length_a"]), "length_b": F32(d["length_b"]), "ratio": F32(d["ratio"])})
+    elif t == JOINT_GEAR:  # serialize_b2_gear_joint.rs:36-55: the coupled joints by list position
+        out.update({"joint1": joint_pos[int(d["joint1"])], "joint2": joint_pos[int(d["joint2"])],
+                    "ratio": F32(d["ratio"])})
     else:
         raise B2gError(ERR_UNSUPPORTED, "serde world: joint kind %d" % t)
     return {"jtype": JOINT_TYPES[t], "joint_def": out}
+
+
+def _joint_lists(joints, nb):
+    """The world's joint list and gear joint list as the reference writes them
+    (src/serialize/serialize_b2_world.rs:144-174): both newest first; a joint's position in the
+    non-gear list is its index (a mouse joint counts but writes nothing, serialize_b2_world.rs:64-66)."""
+    if joints is None:
+        return [], [], []
+    newest_first = list(reversed(range(len(joints))))
+    non_gear = [i for i in newest_first if int(joints[i]["type"]) != JOINT_GEAR]
+    gears = [i for i in newest_first if int(joints[i]["type"]) == JOINT_GEAR]
+    pos = {i: k for k, i in enumerate(non_gear)}
+    written = [i for i in non_gear if int(joints[i]["type"]) != JOINT_MOUSE]
+    jl = [_joint_json(joints[i], nb) for i in written]
+    gl = [_joint_json(joints[i], nb, pos) for i in gears]
+    return jl, gl, written + gears
 
 
 def defs_to_json(gravity, body_defs, fixture_body, shape_defs, fixture_defs, states=None, extension=None, joints=None):
@@ -271,8 +314,8 @@ def defs_to_json(gravity, body_defs, fixture_body, shape_defs, fixture_defs, sta
                              "group_index": int(x["group"])},
                 "m_shape_type": kind, "m_shape": shape})
         bodies.append({"m_definition": definition, "m_fixture_list": fixtures})
-    jl = [] if joints is None else [_joint_json(joints[i], len(bd)) for i in reversed(range(len(joints)))]
-    out = {"m_gravity": _vec(*gravity), "m_bodies_list": bodies, "m_joints_list": jl, "m_gear_joints_list": []}
+    jl, gl, _ = _joint_lists(joints, len(bd))
+    out = {"m_gravity": _vec(*gravity), "m_bodies_list": bodies, "m_joints_list": jl, "m_gear_joints_list": gl}
     if extension is not None:
         out["b2g_checkpoint"] = extension(per_body)
     return _dump(out) + "\n"
@@ -290,6 +333,12 @@ def world_to_json(world, checkpoint=False):
 
 
 def _checkpoint(world, fbody, per_body):
+    if world.joint_count:
+        kinds = world.joint_definitions()["type"]
+        if (kinds == JOINT_GEAR).any() or (kinds == JOINT_MOUSE).any():
+            # a gear's constant comes from the poses when it was created, and the reference's format
+            # writes no mouse joint: neither could be restored exactly
+            raise B2gError(ERR_UNSUPPORTED, "serde checkpoint: a world with gear or mouse joints")
     st = world.body_states()
     fat = world.fat_aabbs()
     dfab, dman = world.destroyed_contacts()
@@ -312,7 +361,7 @@ def _checkpoint(world, fbody, per_body):
                       "manifold": _manifold_json(man[k])} for k in range(len(fab))],
         "destroyed": [{"fixture_a": int(dfab[k, 0]), "fixture_b": int(dfab[k, 1]), "manifold": _manifold_json(dman[k])}
                       for k in range(len(dfab))],
-        "joint_index": list(reversed(range(world.joint_count))),
+        "joint_index": _joint_lists(world.joint_definitions(), len(per_body))[2] if world.joint_count else [],
         # prismatic axes as given (the record holds the normalised axis; normalising it again could
         # move a last bit)
         "joint_axes": [_vec(*a) for a in world.joint_definitions()["local_axis_a"]] if world.joint_count else [],
@@ -330,7 +379,7 @@ def _loads(text):
 
 def _read_joint(rec, nb, joint_bodies):
     kind = rec["jtype"]
-    names = {v: k for k, v in JOINT_TYPES.items()}
+    names = {v: k for k, v in JOINT_TYPES.items() if k != JOINT_MOUSE}
     if kind not in names:
         raise B2gError(ERR_UNSUPPORTED, "serde world: %s is not on the GPU path" % kind)
     d = rec["joint_def"]
@@ -344,8 +393,9 @@ def _read_joint(rec, nb, joint_bodies):
         ia, ib = nb - 1 - ia, nb - 1 - ib
     j["body_a"], j["body_b"] = ia, ib
     j["collide_connected"] = int(bool(base["collide_connected"]))
-    j["local_anchor_a"] = _getvec(d["local_anchor_a"])
-    j["local_anchor_b"] = _getvec(d["local_anchor_b"])
+    if "local_anchor_a" in d:
+        j["local_anchor_a"] = _getvec(d["local_anchor_a"])
+        j["local_anchor_b"] = _getvec(d["local_anchor_b"])
     f = _parse_f32_any
     if kind == "EDistanceJoint":
         for key in ("length", "min_length", "max_length", "stiffness", "damping"):
@@ -357,6 +407,25 @@ def _read_joint(rec, nb, joint_bodies):
     elif kind == "EWeldJoint":
         for key in ("reference_angle", "stiffness", "damping"):
             j[key] = f(d[key])
+    elif kind == "EWheelJoint":
+        j["local_axis_a"] = _getvec(d["local_axis_a"])
+        j["lower_angle"], j["upper_angle"] = f(d["lower_translation"]), f(d["upper_translation"])
+        for key in ("motor_speed", "max_motor_torque", "stiffness", "damping"):
+            j[key] = f(d[key])
+        j["enable_limit"], j["enable_motor"] = int(bool(d["enable_limit"])), int(bool(d["enable_motor"]))
+    elif kind == "EFrictionJoint":
+        j["max_force"], j["max_torque"] = f(d["max_force"]), f(d["max_torque"])
+    elif kind == "EMotorJoint":
+        j["linear_offset"] = _getvec(d["linear_offset"])
+        for key in ("angular_offset", "max_force", "max_torque", "correction_factor"):
+            j[key] = f(d[key])
+    elif kind == "EPulleyJoint":
+        j["ground_anchor_a"], j["ground_anchor_b"] = _getvec(d["ground_anchor_a"]), _getvec(d["ground_anchor_b"])
+        for key in ("length_a", "length_b", "ratio"):
+            j[key] = f(d[key])
+    elif kind == "EGearJoint":
+        # list positions; read_world turns them into creation indices
+        j["joint1"], j["joint2"], j["ratio"] = int(d["joint1"]), int(d["joint2"]), f(d["ratio"])
     else:
         j["local_axis_a"] = _getvec(d["local_axis_a"])
         j["reference_angle"] = f(d["reference_angle"])
@@ -375,8 +444,6 @@ def read_world(text, joint_bodies="reference"):
     if joint_bodies not in ("reference", "file"):
         raise ValueError("joint_bodies: 'reference' or 'file'")
     doc = _loads(text)
-    if doc.get("m_gear_joints_list"):
-        raise B2gError(ERR_UNSUPPORTED, "serde world: gear joints are not on the GPU path")
     bodies = doc.get("m_bodies_list", [])
     bd = np.zeros(len(bodies), BODY_DEF)
     fixture_body, sds, fds = [], [], []
@@ -413,7 +480,17 @@ def read_world(text, joint_bodies="reference"):
     sd = np.array(sds, SHAPE_DEF) if sds else np.zeros(0, SHAPE_DEF)
     fd = np.array(fds, FIXTURE_DEF) if fds else np.zeros(0, FIXTURE_DEF)
     jl = doc.get("m_joints_list", [])
-    jd = np.array([_read_joint(r, len(bodies), joint_bodies) for r in jl], JOINT_DEF) if jl else np.zeros(0, JOINT_DEF)
+    gl = doc.get("m_gear_joints_list", [])
+    recs = [_read_joint(r, len(bodies), joint_bodies) for r in jl + gl]
+    jd = np.array(recs, JOINT_DEF) if recs else np.zeros(0, JOINT_DEF)
+    for g in range(len(jl), len(recs)):
+        # a gear's joints: the reference's reader looks the positions up in the newest-first joint
+        # list of the world it has built from m_joints_list in file order (mirrored, as bodies are)
+        for key in ("joint1", "joint2"):
+            k = int(jd[g][key])
+            if not 0 <= k < len(jl):
+                raise B2gError(ERR_UNSUPPORTED, "serde world: gear joint %d names joint %d" % (g - len(jl), k))
+            jd[g][key] = len(jl) - 1 - k if joint_bodies == "reference" else k
     return {"gravity": _getvec(doc["m_gravity"]), "body_defs": bd, "fixture_body": np.array(fixture_body, np.int32),
             "shape_defs": sd, "fixture_defs": fd, "joint_defs": jd, "checkpoint": doc.get("b2g_checkpoint")}
 

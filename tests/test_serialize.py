@@ -157,7 +157,7 @@ def test_f32_text_round_trip():
 def test_unsupported_members_are_refused(what):
     doc = json.loads(open(GOLDEN).read())
     if what == "joint":
-        doc["m_joints_list"] = [{"jtype": "EWheelJoint", "joint_def": {}}]
+        doc["m_joints_list"] = [{"jtype": "EMouseJoint", "joint_def": {}}]  # never written by the reference
     elif what == "chain":
         doc["m_bodies_list"][1]["m_fixture_list"][0]["m_shape_type"] = "EChain"
     else:
@@ -230,3 +230,86 @@ def test_f32_text_is_serde_json_layout():
             for cand in ("%.*e" % (n - 2, float(x)), "%.*e" % (n - 2, np.nextafter(float(x), 0.0)),
                          "%.*e" % (n - 2, np.nextafter(float(x), np.inf))):
                 assert np.float32(float(cand)).view(np.uint32) != x.view(np.uint32), (x, t, cand)
+
+
+def _every_joint_kind():
+    """Bodies (two static, eight dynamic) and one joint of each kind, two gears (revolute-revolute,
+    revolute-prismatic) after the joints they couple, and a mouse joint."""
+    import joint_scenes as J  # noqa: F401  (the scenes module sits next to this file)
+    bd = np.zeros(10, b2g.BODY_DEF)
+    bd["type"] = 2
+    bd[:2]["type"] = 0
+    bd["position"] = np.stack([np.arange(10) * 2.0, np.full(10, 3.0)], 1)
+    sd = np.zeros(10, b2g.SHAPE_DEF)
+    sd["type"] = b2g.SHAPE_CIRCLE
+    sd["radius"] = 0.4
+    fd = np.zeros(10, b2g.FIXTURE_DEF)
+    fd["density"], fd["friction"], fd["mask"], fd["category"] = 1.0, 0.3, 0xFFFF, 1
+    jd = np.zeros(11, b2g.JOINT_DEF)
+    kinds = [b2g.JOINT_DISTANCE, b2g.JOINT_REVOLUTE, b2g.JOINT_WELD, b2g.JOINT_PRISMATIC, b2g.JOINT_WHEEL,
+             b2g.JOINT_FRICTION, b2g.JOINT_MOTOR, b2g.JOINT_PULLEY, b2g.JOINT_MOUSE, b2g.JOINT_GEAR, b2g.JOINT_GEAR]
+    jd["type"] = kinds
+    jd["body_a"] = [0, 0, 2, 1, 3, 0, 1, 4, 0, 3, 3]
+    jd["body_b"] = [2, 3, 4, 5, 6, 7, 8, 9, 9, 3, 5]
+    jd["local_anchor_a"] = np.float32(np.arange(22).reshape(11, 2) * 0.25)
+    jd["local_anchor_b"] = np.float32(-np.arange(22).reshape(11, 2) * 0.125)
+    jd["length"], jd["min_length"], jd["max_length"] = 2.5, 1.0, 3.0
+    jd["stiffness"], jd["damping"], jd["reference_angle"] = 4.0, 0.5, 0.125
+    jd["lower_angle"], jd["upper_angle"], jd["max_motor_torque"], jd["motor_speed"] = -0.5, 0.75, 20.0, 1.5
+    jd["enable_limit"], jd["enable_motor"] = 1, 1
+    jd["local_axis_a"] = (0.6, 0.8)
+    jd["max_force"], jd["max_torque"], jd["angular_offset"], jd["correction_factor"] = 30.0, 5.0, 0.25, 0.3
+    jd["linear_offset"] = (1.0, -2.0)
+    jd["ground_anchor_a"], jd["ground_anchor_b"] = (0.0, 10.0), (8.0, 10.0)
+    jd["length_a"], jd["length_b"], jd["ratio"] = 6.0, 7.0, 2.0
+    jd["target"] = (18.0, 3.0)
+    jd[9]["body_a"], jd[9]["body_b"], jd[9]["joint1"], jd[9]["joint2"] = 3, 5, 1, 3  # revolute and prismatic
+    jd[10]["body_a"], jd[10]["body_b"], jd[10]["joint1"], jd[10]["joint2"] = 3, 9, 1, 1
+    jd["collide_connected"] = [0, 1, 0, 0, 0, 1, 0, 1, 0, 0, 0]
+    return bd, np.arange(10, dtype=np.int32), sd, fd, jd
+
+
+def test_every_joint_kind_round_trip():
+    """Every kind the reference's writer writes (all but the mouse joint) reads back with its
+    fields; gears go in their own list and name their joints by position in the joint list; the
+    reference's reader resolves those positions mirrored, as it does bodies. The writer numbers
+    the joint list before it leaves the mouse joints out (src/serialize/serialize_b2_world.rs:
+    149-166 and 64-66), so with a mouse joint a gear's positions count it, as the reference's."""
+    bd, fb, sd, fd, jd = _every_joint_kind()
+    text = ser.defs_to_json((0.0, -10.0), bd, fb, sd, fd, joints=jd)
+    doc = json.loads(text)
+    assert [r["jtype"] for r in doc["m_joints_list"]] == [
+        "EPulleyJoint", "EMotorJoint", "EFrictionJoint", "EWheelJoint", "EPrismaticJoint", "EWeldJoint",
+        "ERevoluteJoint", "EDistanceJoint"]  # newest first, the mouse joint not written
+    assert [r["jtype"] for r in doc["m_gear_joints_list"]] == ["EGearJoint", "EGearJoint"]
+    # positions in the numbered list (newest first, the mouse joint at 0): revolute 7, prismatic 5
+    assert [(r["joint_def"]["joint1"], r["joint_def"]["joint2"]) for r in doc["m_gear_joints_list"]] == [(7, 7), (7, 5)]
+    assert list(doc["m_joints_list"][1]["joint_def"]) == ["base", "linear_offset", "angular_offset", "max_force",
+                                                          "max_torque", "correction_factor"]
+    assert list(doc["m_joints_list"][0]["joint_def"])[:3] == ["base", "ground_anchor_a", "ground_anchor_b"]
+    # without the mouse joint the positions read back as the joints they name
+    keep = [i for i in range(len(jd)) if jd[i]["type"] != b2g.JOINT_MOUSE]
+    jd = jd[keep].copy()  # gears now 8 and 9, coupling creation indices 1 and 3 as before
+    text = ser.defs_to_json((0.0, -10.0), bd, fb, sd, fd, joints=jd)
+    got = ser.read_world(text, joint_bodies="file")["joint_defs"]
+    nb = len(bd)
+    written = [7, 6, 5, 4, 3, 2, 1, 0, 9, 8]  # creation index of each read joint
+    assert list(got["type"]) == list(jd["type"][written])
+    np.testing.assert_array_equal(got["body_a"], nb - 1 - jd["body_a"][written])
+    np.testing.assert_array_equal(got["body_b"], nb - 1 - jd["body_b"][written])
+    for k, i in enumerate(written[:8]):
+        t = int(jd[i]["type"])
+        fields = {b2g.JOINT_WHEEL: ["local_anchor_a", "local_anchor_b", "local_axis_a", "lower_angle", "upper_angle",
+                                    "motor_speed", "max_motor_torque", "stiffness", "damping"],
+                  b2g.JOINT_FRICTION: ["local_anchor_a", "local_anchor_b", "max_force", "max_torque"],
+                  b2g.JOINT_MOTOR: ["linear_offset", "angular_offset", "max_force", "max_torque", "correction_factor"],
+                  b2g.JOINT_PULLEY: ["ground_anchor_a", "ground_anchor_b", "local_anchor_a", "local_anchor_b",
+                                     "length_a", "length_b", "ratio"]}.get(t, [])
+        for f in fields:
+            np.testing.assert_array_equal(got[k][f], jd[i][f], err_msg="%d %s" % (t, f))
+    # the gears' joints as creation indices of the read world: the revolute (creation 1) is read
+    # 7th (index 6), the prismatic (creation 3) 5th (index 4)
+    assert (got[8]["joint1"], got[8]["joint2"], got[9]["joint1"], got[9]["joint2"]) == (6, 6, 6, 4)
+    assert got[9]["ratio"] == np.float32(2.0)
+    ref = ser.read_world(text)["joint_defs"]
+    assert (ref[8]["joint1"], ref[9]["joint2"]) == (8 - 1 - 6, 8 - 1 - 4)
