@@ -64,7 +64,7 @@ static int carry_manifolds(World* w) {
   if (w->manifolds_valid[w->cur]) return B2G_OK;
   int n = w->n_pairs;
   int st;
-  if ((st = w->s.touch_flag.reserve(std::max(n, 1)))) return st;
+  if ((st = w->s.touch_flag.reserve(pair_capacity(w, n)))) return st;
   ContactArrays& c = w->c[w->cur];
   ContactArrays& o = w->c[1 - w->cur];
   if (n > 0) {

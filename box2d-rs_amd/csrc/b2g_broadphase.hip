@@ -42,15 +42,18 @@ __global__ void __launch_bounds__(kBlock) k_synchronize(int np, const int* __res
 }
 
 // ---- scene bounds and Morton codes ---------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_bounds(int n, const int* __restrict__ list, const float4* __restrict__ fat,
-                                                   int* __restrict__ out) {
+// The scene's extent for the Morton codes, in two steps without atomics: each block of k_bounds
+// writes the box of its proxies' fat-AABB centres to partial[block], and every block of k_morton
+// reduces those partial boxes itself before coding its proxies (min and max are exact, so the
+// result does not depend on the order). Block 0 also clears the pair search's flags.
+static constexpr int kBoundsBlocks = 256;  // most partial boxes (one per k_bounds block)
+__device__ inline void pair_flags_init(int* flags, const int* scan_ctl) {
+  flags[0] = 0;                           // traversal stack overflow
+  flags[1] = 0;                           // proxies whose pairs overflow their slots
+  flags[2] = scan_ctl ? scan_ctl[1] : 0;  // a single-pass scan gave up waiting (read back with the flags)
+}
+__device__ inline void block_box(float& lx, float& ly, float& hx, float& hy) {
   __shared__ float red[4][kBlock / 64];
-  float lx = kMaxFloat, ly = kMaxFloat, hx = -kMaxFloat, hy = -kMaxFloat;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 a = fat[list[i]];
-    float cx = 0.5f * (a.x + a.z), cy = 0.5f * (a.y + a.w);
-    lx = fminf(lx, cx); ly = fminf(ly, cy); hx = fmaxf(hx, cx); hy = fmaxf(hy, cy);
-  }
   for (int d = 32; d >= 1; d >>= 1) {
     lx = fminf(lx, __shfl_xor(lx, d, 64));
     ly = fminf(ly, __shfl_xor(ly, d, 64));
@@ -60,26 +63,27 @@ __global__ void __launch_bounds__(kBlock) k_bounds(int n, const int* __restrict_
   const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) { red[0][wid] = lx; red[1][wid] = ly; red[2][wid] = hx; red[3][wid] = hy; }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < kBlock / 64; ++w) {
-      lx = fminf(lx, red[0][w]); ly = fminf(ly, red[1][w]); hx = fmaxf(hx, red[2][w]); hy = fmaxf(hy, red[3][w]);
-    }
-    atomicMin(&out[0], float_flip(lx));
-    atomicMin(&out[1], float_flip(ly));
-    atomicMax(&out[2], float_flip(hx));
-    atomicMax(&out[3], float_flip(hy));
+  for (int w = 0; w < kBlock / 64; ++w) {
+    lx = fminf(lx, red[0][w]); ly = fminf(ly, red[1][w]); hx = fmaxf(hx, red[2][w]); hy = fmaxf(hy, red[3][w]);
   }
 }
 
-__global__ void k_bounds_init(int* out, const int* __restrict__ scan_ctl) {
-  out[0] = float_flip(kMaxFloat);
-  out[1] = float_flip(kMaxFloat);
-  out[2] = float_flip(-kMaxFloat);
-  out[3] = float_flip(-kMaxFloat);
-  out[4] = 0;  // traversal stack overflow
-  out[5] = 0;  // proxies whose pairs overflow their slots
-  out[6] = scan_ctl ? scan_ctl[1] : 0;  // a single-pass scan gave up waiting (read back with the flags)
+__global__ void __launch_bounds__(kBlock) k_bounds(int n, const int* __restrict__ list, const float4* __restrict__ fat,
+                                                   float4* __restrict__ partial, int* __restrict__ flags,
+                                                   const int* __restrict__ scan_ctl) {
+  float lx = kMaxFloat, ly = kMaxFloat, hx = -kMaxFloat, hy = -kMaxFloat;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 a = fat[list[i]];
+    float cx = 0.5f * (a.x + a.z), cy = 0.5f * (a.y + a.w);
+    lx = fminf(lx, cx); ly = fminf(ly, cy); hx = fmaxf(hx, cx); hy = fmaxf(hy, cy);
+  }
+  block_box(lx, ly, hx, hy);
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_float4(lx, ly, hx, hy);
+  if (blockIdx.x == 0 && threadIdx.x == 0) pair_flags_init(flags, scan_ctl);
 }
+
+// the pair search's flags for a search without tree proxies (no k_bounds launch)
+__global__ void k_pair_flags_init(int* flags, const int* __restrict__ scan_ctl) { pair_flags_init(flags, scan_ctl); }
 
 __device__ inline unsigned spread15(unsigned x) {
   x &= 0x7fffu;
@@ -93,14 +97,19 @@ __device__ inline unsigned spread15(unsigned x) {
 // key = world id in the high bits, then a Morton code of `mbits` bits per axis of the fat-AABB
 // centre; sorting by it groups each world's proxies and orders them along a Z curve
 __global__ void __launch_bounds__(kBlock) k_morton(int n, const int* __restrict__ list, const float4* __restrict__ fat,
-                                                   const int* __restrict__ bounds, const int* __restrict__ world,
-                                                   int mbits, unsigned* __restrict__ keys, unsigned* __restrict__ vals) {
+                                                   const float4* __restrict__ partial, int nparts,
+                                                   const int* __restrict__ world, int mbits, unsigned* __restrict__ keys,
+                                                   unsigned* __restrict__ vals) {
+  float lx = kMaxFloat, ly = kMaxFloat, hx = -kMaxFloat, hy = -kMaxFloat;
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+    const float4 b = partial[p];
+    lx = fminf(lx, b.x); ly = fminf(ly, b.y); hx = fmaxf(hx, b.z); hy = fmaxf(hy, b.w);
+  }
+  block_box(lx, ly, hx, hy);
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int i = list[k];
   const float cells = float((1u << mbits) - 1u);
-  float lx = float_unflip(bounds[0]), ly = float_unflip(bounds[1]);
-  float hx = float_unflip(bounds[2]), hy = float_unflip(bounds[3]);
   float sx = hx > lx ? cells / (hx - lx) : 0.0f;
   float sy = hy > ly ? cells / (hy - ly) : 0.0f;
   float4 a = fat[i];
@@ -516,11 +525,14 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
       (st = bp.nodes4.reserve(std::max(nt, 1))) ||
       (st = bp.leafrec.reserve(std::max(nt, 1))) ||
       (st = bp.count.reserve(np + 1)) || (st = bp.slots.reserve(size_t(np) * kSlots)) ||
-      (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(2)))
+      (st = dst.off.reserve(np + 1)) || (st = bp.bounds.reserve(1 + kBoundsBlocks)))
     return st;
-  int* bnd = reinterpret_cast<int*>(bp.bounds.p);
-  int* flags = bnd + 4;  // [0] stack overflow, [1] proxies with more pairs than slots
-  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, w->stream, bnd, static_cast<const int*>(bp.scan_ctl.p));
+  // bounds[0]: the pair search's flags ([0] stack overflow, [1] proxies with more pairs than
+  // slots, [2] scan error, [3] pair total); bounds[1..]: k_bounds' partial boxes
+  int* flags = reinterpret_cast<int*>(bp.bounds.p);
+  float4* partial = bp.bounds.p + 1;
+  const int* scan_ctl = static_cast<const int*>(bp.scan_ctl.p);
+  if (nt == 0) hipLaunchKernelGGL(k_pair_flags_init, dim3(1), dim3(1), 0, w->stream, flags, scan_ctl);
   // world bits above Morton bits: 12 bits per axis (4096 cells across the scene's extent, finer
   // than any body on the scenes here; equal keys keep proxy order), fewer for many worlds, so one
   // world sorts in three 8-bit passes
@@ -531,10 +543,11 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   const bool nc = w->n_nocollide > 0;
   const LeafInputs li{w->f.body.p, w->b.type.p, w->f.filter.p, w->f.shape.p, nc ? w->j.nc_flag.p : nullptr};
   if (nt > 0) {
-    int gb = std::min(blocks_for(nt, kBlock), 1024);
-    hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat, bnd);
+    const int gb = std::min(blocks_for(nt, kBlock), kBoundsBlocks);
+    hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat, partial, flags,
+                       scan_ctl);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, w->stream, nt, w->f.tree_list.p, fat,
-                       bnd, w->f.world.p, mbits, bp.keys.p, bp.vals.p);
+                       partial, gb, w->f.world.p, mbits, bp.keys.p, bp.vals.p);
     if ((st = radix_sort_pairs(w, bp.keys.p, bp.vals.p, nt, key_bits))) return st;
     for (int L0 = 0; L0 == 0 || L0 < lv.count - 1; L0 += kReduceLevels)
       hipLaunchKernelGGL(k_seg_reduce, dim3(blocks_for(lv.size[L0], kBlock)), dim3(kBlock), 0, w->stream, lv, L0,
@@ -575,7 +588,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   }
   int total = h[0];
   kt_aux(w, B2G_KERNEL_TRAVERSE, total);
-  if ((st = dst.pair.reserve(std::max(total, 1)))) return st;
+  if ((st = dst.pair.reserve(pair_capacity(w, total)))) return st;
   if (nt > 0)
     hipLaunchKernelGGL(k_pairs_from_slots, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, w->stream, nt,
                        bp.leafrec.p, bp.count.p, bp.slots.p, dst.off.p, dst.pair.p);
@@ -587,9 +600,9 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   return B2G_OK;
 }
 
-static int reserve_contacts(ContactArrays& c, int n) {
+static int reserve_contacts(World* w, ContactArrays& c, int pairs) {
   int st;
-  n = std::max(n, 1);
+  const size_t n = pair_capacity(w, pairs);
   if ((st = c.pair.reserve(n)) || (st = c.old_index.reserve(n)) || (st = c.ab.reserve(n)) || (st = c.hdr.reserve(n)) ||
       (st = c.geo.reserve(n)) || (st = c.pt0.reserve(n)) || (st = c.pt1.reserve(n)) || (st = c.ids.reserve(n)) ||
       (st = c.colour.reserve(n)))
@@ -626,7 +639,7 @@ int phase_broadphase(World* w) {
     return B2G_OK;
   }
   if ((st = find_pairs(w, w->f.fat.p, w->np, dst, &n))) return st;
-  if ((st = reserve_contacts(dst, n))) return st;
+  if ((st = reserve_contacts(w, dst, n))) return st;
   if (n > 0) {
     int old_np = w->set_np[w->cur];  // proxies when the old set was built (0: no old set)
     hipLaunchKernelGGL(k_merge, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, w->stream, n, dst.pair.p, old_np,

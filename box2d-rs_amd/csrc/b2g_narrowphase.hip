@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_wakes(int nb, int* __restrict_
 int phase_collide(World* w) {
   int n = w->n_pairs;
   int st;
-  if ((st = w->s.touch_flag.reserve(std::max(n, 1)))) return st;
+  if ((st = w->s.touch_flag.reserve(pair_capacity(w, n)))) return st;
   ContactArrays& c = w->c[w->cur];
   ContactArrays& o = w->c[1 - w->cur];
   const bool sleepy = w->sleep_possible();

@@ -263,7 +263,12 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const unsigned* __res
 }
 
 // stable scatter: the tile is processed as kSortItems sub-tiles of kSortBlock keys in index
-// order; within a sub-tile each wave ranks its keys by digit with ballots.
+// order; within a sub-tile each wave ranks its keys by digit with ballots. kFromHist: offs holds
+// the per-tile histograms themselves (few tiles) and the block computes its digits' offsets
+// (the keys of smaller digits plus the digit's keys in earlier tiles) instead of reading a scan.
+static constexpr int kRadixFuseMax = 128;  // tiles up to which the scatter sums the histograms itself
+static_assert(kSortBlock == kRadix && kSortBlock == kScanBlock, "one thread per digit, block scan of 256");
+template <bool kFromHist>
 __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const unsigned* __restrict__ keys,
                                                               const unsigned* __restrict__ vals, int n, int shift,
                                                               const unsigned* __restrict__ offs, int nblocks,
@@ -271,7 +276,20 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const unsigned* __
   __shared__ unsigned running[kRadix];
   __shared__ unsigned wcount[kSortBlock / 64][kRadix];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  running[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  if constexpr (kFromHist) {
+    const unsigned* row = offs + threadIdx.x * nblocks;
+    unsigned before = 0, total = 0;
+#pragma unroll 8
+    for (int t = 0; t < nblocks; ++t) {
+      const unsigned h = row[t];
+      total += h;
+      before += t < static_cast<int>(blockIdx.x) ? h : 0u;
+    }
+    int all;
+    running[threadIdx.x] = static_cast<unsigned>(block_exclusive(static_cast<int>(total), &all)) + before;
+  } else {
+    running[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  }
   const int base = blockIdx.x * kSortTile;
   for (int sub = 0; sub < kSortItems; ++sub) {
     for (int w = 0; w < kSortBlock / 64; ++w) wcount[w][threadIdx.x] = 0;
@@ -325,11 +343,16 @@ int radix_sort_pairs(World* w, unsigned* keys, unsigned* vals, int n, int bits) 
   for (int p = 0; p < passes; ++p) {
     int shift = 8 * p;
     hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortBlock), 0, w->stream, k0, n, shift, bp.hist.p, nblocks);
-    if ((st = exclusive_scan(w, reinterpret_cast<int*>(bp.hist.p), reinterpret_cast<int*>(bp.hist.p),
-                             kRadix * nblocks, nullptr)))
-      return st;
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortBlock), 0, w->stream, k0, v0, n, shift, bp.hist.p,
-                       nblocks, k1, v1);
+    if (nblocks <= kRadixFuseMax) {
+      hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nblocks), dim3(kSortBlock), 0, w->stream, k0, v0, n, shift,
+                         bp.hist.p, nblocks, k1, v1);
+    } else {
+      if ((st = exclusive_scan(w, reinterpret_cast<int*>(bp.hist.p), reinterpret_cast<int*>(bp.hist.p),
+                               kRadix * nblocks, nullptr)))
+        return st;
+      hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nblocks), dim3(kSortBlock), 0, w->stream, k0, v0, n, shift,
+                         bp.hist.p, nblocks, k1, v1);
+    }
     std::swap(k0, k1);
     std::swap(v0, v1);
   }

@@ -1767,9 +1767,9 @@ static int solve_tail_graph(World* w, const SolveGraphKey& k, const VelArgs& va,
 // ---- driver ---------------------------------------------------------------------------------------
 static int reserve_solver(World* w, int nc) {
   SolverArrays& s = w->s;
-  int n = std::max(nc, 1);
+  const size_t n = pair_capacity(w, nc);  // nc: the pairs, which bound the constraints
   int st;
-  if ((st = s.touch_scan.reserve(std::max(w->n_pairs, 1) + 1)) || (st = s.touch_list.reserve(n)) ||
+  if ((st = s.touch_scan.reserve(pair_capacity(w, w->n_pairs) + 1)) || (st = s.touch_list.reserve(n)) ||
       (st = s.k_contact.reserve(n)) || (st = s.k_bodies.reserve(n)) || (st = s.k_info.reserve(n)) ||
       (st = s.vnf.reserve(n)) || (st = s.vm.reserve(n)) || (st = s.vr0.reserve(n)) || (st = s.vr1.reserve(n)) ||
       (st = s.vms.reserve(n)) || (st = s.vbias.reserve(n)) || (st = s.K.reserve(n)) || (st = s.NM.reserve(n)) ||
@@ -1796,7 +1796,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       (st = s.parent.reserve(nb)) || (st = s.isl_violated.reserve(nb)) || (st = s.isl_done.reserve(nb)) ||
       (sleepy && ((st = s.isl_awake.reserve(nb)) || (st = s.isl_restless.reserve(nb)))) ||
       (st = s.color_off.reserve(3 * kMaxColors + 1)) ||
-      (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(std::max(n_pairs, 1))))
+      (st = s.color_cursor.reserve(kMaxColors)) || (st = s.prev_colour.reserve(pair_capacity(w, n_pairs))))
     return st;
   const IntegrateArgs ia{w->b.type.p, w->b.pos.p, w->b.vel.p, w->b.pos0.p, w->b.props.p, w->b.mass.p, w->b.force.p,
                          w->gravity.x, w->gravity.y, dt};
@@ -1807,7 +1807,7 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
   const bool want_bins = w->nj == 0 && (w->island_solve == 1 || (w->island_solve == -1 && !want_world));
   const int bin_cap = w->bin_threads / 2;  // islands of at most bin_cap constraints are binned
   if (want_bins && ((st = s.isl_count.reserve(nb)) || (st = s.isl_small.reserve(nb)) ||
-                    (st = s.isl_start.reserve(nb)) || (st = s.croot.reserve(std::max(n_pairs, 1)))))
+                    (st = s.isl_start.reserve(nb)) || (st = s.croot.reserve(pair_capacity(w, n_pairs)))))
     return st;
   // per body: island roots, degrees, adjacency cursors, island counts and (no sleeping possible)
   // the velocity integration

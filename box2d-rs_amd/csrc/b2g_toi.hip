@@ -983,7 +983,7 @@ static ToiArgs toi_args(World* w, float dt, int vi) {
 
 static int toi_reserve(World* w) {
   int st;
-  const size_t n = size_t(std::max(w->n_pairs, 1));
+  const size_t n = pair_capacity(w, w->n_pairs);
   ToiState& ts = w->toi_state[w->cur];
   if ((st = ts.toi.reserve(n)) || (st = ts.count.reserve(n)) || (st = ts.flags.reserve(n))) return st;
   if ((st = w->toi_need.reserve(n)) || (st = w->toi_rank.reserve(n)) || (st = w->toi_snap.reserve(19 * n)) ||
