@@ -602,13 +602,14 @@ int b2g_world_create(float gx, float gy, b2g_world** out) {
   w->device = dev;
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
       w->counters.reserve(4 + 128 + 2) != B2G_OK ||
-      hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&w->h_counters), kPinnedInts * sizeof(int), hipHostMallocDefault) !=
+          hipSuccess) {
     set_error("world allocation failed");
     delete h;
     return B2G_ERR_DEVICE;
   }
   for (int i = 0; i < 8; ++i) (void)hipEventCreate(&w->ev[i]);
-  std::memset(w->h_counters, 0, 256 * sizeof(int));
+  std::memset(w->h_counters, 0, kPinnedInts * sizeof(int));
   *out = h;
   return B2G_OK;
 }

@@ -573,7 +573,7 @@ static int find_pairs(World* w, const float4* fat, int np, ContactArrays& dst, i
   // the pair total also lands beside the flags (flags[3]), so one read-back brings both
   if ((st = exclusive_scan(w, bp.count.p, dst.off.p, np, dst.off.p + np, flags + 3))) return st;
   // flags: [0] stack overflow, [1] slot overflow, [2] scan error, [3] pair total
-  int f[4];
+  int* f = w->h_counters + kPinnedPairFlags;  // pinned
   B2G_HIP(hipMemcpyAsync(f, flags, 4 * sizeof(int), hipMemcpyDeviceToHost, w->stream));
   B2G_HIP(hipStreamSynchronize(w->stream));
   const int h[4] = {f[3], f[0], f[1], f[2]};

@@ -304,7 +304,7 @@ struct World {
   std::vector<int> h_bworld, h_fworld;  // world id per body / per proxy
   int n_tree = 0;                       // proxies in the BVH
   // host sources of asynchronous uploads must outlive the copy: kept here, not on the stack
-  std::vector<int> h_tree, h_loose_off, h_loose, h_color_off_dev;
+  std::vector<int> h_tree, h_loose_off, h_loose;
   int n_worlds = 1;
   std::vector<ShapeV> h_shapev;
   std::vector<int> h_awake, h_auto_sleep, h_bullet;
@@ -347,7 +347,11 @@ struct World {
   SolverArrays s;
   BroadphaseArrays bp;
   DevArray<int> counters;   // small device counters: 4 solver counters, then the colour histogram
-  int* h_counters = nullptr;  // pinned host mirror (4 counters, 128 colour counts, last round)
+  // pinned host memory (kPinnedInts ints): [0, 134) the solver's counters read back after the
+  // colouring (4 counters, 128 colour counts, last round, large-island constraints),
+  // [kPinnedPairFlags, +4) the pair search's flags and total, [kPinnedColourOffsets, +385) the
+  // colour offsets uploaded for the solve
+  int* h_counters = nullptr;
   int n_touch = 0, n_colors = 0, n_islands = 0;
   int color_rounds = 16;       // colouring rounds launched before the first readback
   bool color_reuse = true;     // persisting contacts keep their previous colour
@@ -393,6 +397,8 @@ struct World {
   bool joints_dirty = false;
   JointArrays j;
 };
+
+constexpr int kPinnedInts = 1024, kPinnedPairFlags = 192, kPinnedColourOffsets = 256;
 
 // Capacity floor of the arrays sized by the contact set (the pairs, and the solver's arrays,
 // which the pairs bound): reserving for a multiple of the proxy count from the first step keeps a

@@ -1914,9 +1914,12 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
     // Device copy: launch offsets (kMaxColors + 1, padded with nc), then per colour id its base
     // offset and its launch index.
     const int* hc = w->h_counters + 4;
-    std::vector<int>& offs = w->h_color_off_dev;  // outlives the asynchronous copy
-    offs.assign(3 * kMaxColors + 1, 0);
-    int* base = offs.data() + kMaxColors + 1;
+    // pinned (the copy reads it asynchronously; the next step writes it only after its colouring's
+    // read-back, when this copy has long finished)
+    static_assert(kPinnedColourOffsets + 3 * kMaxColors + 1 <= kPinnedInts, "pinned colour offsets");
+    int* offs = w->h_counters + kPinnedColourOffsets;
+    std::fill(offs, offs + 3 * kMaxColors + 1, 0);
+    int* base = offs + kMaxColors + 1;
     int* launch = base + kMaxColors;
     w->h_color_off.assign(1, 0);
     for (int i = 0; i < kMaxColors; ++i) {
@@ -1927,7 +1930,8 @@ int phase_solve(World* w, float dt, float dt_ratio, int vel_iters, int pos_iters
       w->h_color_off.push_back(w->h_color_off.back() + hc[i]);
     }
     for (int i = 0; i <= kMaxColors; ++i) offs[i] = i <= ncolors ? w->h_color_off[i] : nc;
-    B2G_HIP(hipMemcpyAsync(s.color_off.p, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, w->stream));
+    B2G_HIP(hipMemcpyAsync(s.color_off.p, offs, sizeof(int) * (3 * kMaxColors + 1), hipMemcpyHostToDevice,
+                           w->stream));
     if (bins) {
       // constraints grouped by bin inside each colour, the large islands' last (see k_color_bin_hist)
       const int ng = nbins + 1, ncw = ncolors * ng;
