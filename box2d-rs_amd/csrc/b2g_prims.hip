@@ -278,12 +278,25 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const unsigned* __
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if constexpr (kFromHist) {
     const unsigned* row = offs + threadIdx.x * nblocks;
+    const int me = static_cast<int>(blockIdx.x);
     unsigned before = 0, total = 0;
+    if ((nblocks & 3) == 0) {
+      // rows start 16-byte aligned: read them four tiles at a time, all loads in flight together
+      const uint4* row4 = reinterpret_cast<const uint4*>(row);
 #pragma unroll 8
-    for (int t = 0; t < nblocks; ++t) {
-      const unsigned h = row[t];
-      total += h;
-      before += t < static_cast<int>(blockIdx.x) ? h : 0u;
+      for (int q = 0; q < nblocks / 4; ++q) {
+        const uint4 h = row4[q];
+        const int t = 4 * q;
+        total += h.x + h.y + h.z + h.w;
+        before += (t < me ? h.x : 0u) + (t + 1 < me ? h.y : 0u) + (t + 2 < me ? h.z : 0u) + (t + 3 < me ? h.w : 0u);
+      }
+    } else {
+#pragma unroll 8
+      for (int t = 0; t < nblocks; ++t) {
+        const unsigned h = row[t];
+        total += h;
+        before += t < me ? h : 0u;
+      }
     }
     int all;
     running[threadIdx.x] = static_cast<unsigned>(block_exclusive(static_cast<int>(total), &all)) + before;
