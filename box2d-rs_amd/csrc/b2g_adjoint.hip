@@ -526,37 +526,89 @@ __global__ void __launch_bounds__(kBlock) k_ab_store(AdjCtx a) {
   a.lam0bar[i] = eb;
 }
 
+// The reverse velocity solve and warm start run kVelGroups threads per constraint, one per group
+// of kVelLanes dual lanes of the vector-Jacobian product (a colour holds a few thousand
+// constraints: one thread each would leave most of the GPU idle, and one thread's four passes
+// over 8-lane duals run at one wave per SIMD). The groups of a constraint are adjacent lanes of
+// one wave: each reads the body and impulse cotangents before any of them writes, and each
+// writes only the entries of its own inputs.
+constexpr int kVelLanes = 2;
+constexpr int kVelGroups = (kVelIn + kVelLanes - 1) / kVelLanes;
+static_assert(64 % kVelGroups == 0, "a constraint's groups share one wave");
+
+// cotangents of the velocity inputs of group [lo, lo + kVelLanes): body velocities replaced,
+// impulses replaced (kAssignLam) or accumulated, the rest accumulated
+template <bool kAssignLam>
+__device__ inline void vel_group_out(const AdjCtx& a, int i, int4 sl, bool dA, bool dB, const float* xb, int lo) {
+  const int hi = lo + kVelLanes;
+  auto put = [&](float* d, int j, bool assign) {
+    if (j >= lo && j < hi) {
+      if (assign) *d = xb[j];
+      else *d += xb[j];
+    }
+  };
+  float* va = &a.vbar[sl.x].x;
+  float* vb = &a.vbar[sl.y].x;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (dA) put(va + c, c, true);
+    if (dB) put(vb + c, 3 + c, true);
+  }
+  if (lo == 0) {  // the unused fourth component, as a whole-record write would leave it
+    if (dA) va[3] = 0.0f;
+    if (dB) vb[3] = 0.0f;
+  }
+  float* lam = &a.lambar[i].x;
+  float* q0 = &a.qnbar[2 * i].x;
+  float* q1 = &a.qnbar[2 * i + 1].x;
+  float* g0 = &a.gbar[2 * i].x;
+  float* g1 = &a.gbar[2 * i + 1].x;
+  float* s0 = &a.vsbar[2 * i].x;
+  float* s1 = &a.vsbar[2 * i + 1].x;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) put(lam + c, 6 + c, kAssignLam);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    put(q0 + c, 10 + c, false);
+    put(q1 + c, 13 + c, false);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    put(g0 + c, 16 + c, false);
+    put(g1 + c, 20 + c, false);
+  }
+  put(&a.fricbar[i], 24, false);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    put(s0 + c, 25 + c, false);
+    put(s1 + c, 28 + c, false);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_ab_vel(AdjCtx a, int b0, int b1, int it) {
-  int i = b0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = b0 + t / kVelGroups, lo = (t % kVelGroups) * kVelLanes;
   if (i >= b1) return;
   const long long s = a.off + i;
   const int4 sl = a.slot[s];
   const ContactConst k = contact_const(a, sl, a.meta[s].x);
   const bool dA = a.btype[sl.x] == 2, dB = a.btype[sl.y] == 2;
-  const float4* t = a.t_vel + 3 * (size_t(it) * a.n + i);
+  const float4* tv = a.t_vel + 3 * (size_t(it) * a.n + i);
   float x[kVelIn], yb[10], xb[kVelIn];
-  vel_inputs(a, i, sl, t[0], t[1], t[2], x);
+  vel_inputs(a, i, sl, tv[0], tv[1], tv[2], x);
   const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   put3(yb, dA ? a.vbar[sl.x] : z);
   put3(yb + 3, dB ? a.vbar[sl.y] : z);
   const float4 lb = a.lambar[i];
   yb[6] = lb.x; yb[7] = lb.y; yb[8] = lb.z; yb[9] = lb.w;
   for (int j = 0; j < kVelIn; ++j) xb[j] = 0.0f;
-  vjp<kVelIn, 10, 8>(x, yb, xb, VelFn<float>{k, a.block != 0});
-  if (dA) a.vbar[sl.x] = f4(xb);
-  if (dB) a.vbar[sl.y] = f4(xb + 3);
-  a.lambar[i] = make_float4(xb[6], xb[7], xb[8], xb[9]);
-  add3(a.qnbar[2 * i], xb + 10);
-  add3(a.qnbar[2 * i + 1], xb + 13);
-  add4(a.gbar[2 * i], xb + 16);
-  add4(a.gbar[2 * i + 1], xb + 20);
-  a.fricbar[i] += xb[24];
-  add3(a.vsbar[2 * i], xb + 25);
-  add3(a.vsbar[2 * i + 1], xb + 28);
+  vjp_group<kVelIn, 10, kVelLanes>(x, yb, xb, VelFn<float>{k, a.block != 0}, lo);
+  vel_group_out<true>(a, i, sl, dA, dB, xb, lo);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ab_warm(AdjCtx a, int b0, int b1) {
-  int i = b0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = b0 + t / kVelGroups, lo = (t % kVelGroups) * kVelLanes;
   if (i >= b1) return;
   const long long s = a.off + i;
   const int4 sl = a.slot[s];
@@ -568,17 +620,8 @@ __global__ void __launch_bounds__(kBlock) k_ab_warm(AdjCtx a, int b0, int b1) {
   put3(yb, dA ? a.vbar[sl.x] : z);
   put3(yb + 3, dB ? a.vbar[sl.y] : z);
   for (int j = 0; j < kVelIn; ++j) xb[j] = 0.0f;
-  vjp<kVelIn, 6, 8>(x, yb, xb, WarmFn<float>{k, a.block != 0});
-  if (dA) a.vbar[sl.x] = f4(xb);
-  if (dB) a.vbar[sl.y] = f4(xb + 3);
-  add4(a.lambar[i], xb + 6);
-  add3(a.qnbar[2 * i], xb + 10);
-  add3(a.qnbar[2 * i + 1], xb + 13);
-  add4(a.gbar[2 * i], xb + 16);
-  add4(a.gbar[2 * i + 1], xb + 20);
-  a.fricbar[i] += xb[24];
-  add3(a.vsbar[2 * i], xb + 25);
-  add3(a.vsbar[2 * i + 1], xb + 28);
+  vjp_group<kVelIn, 6, kVelLanes>(x, yb, xb, WarmFn<float>{k, a.block != 0}, lo);
+  vel_group_out<false>(a, i, sl, dA, dB, xb, lo);
 }
 
 __device__ inline void atomic_add3(float4* p, float4 v) {
@@ -783,6 +826,7 @@ static int reverse_step(World* w, const AdjCtx& a, const TapeStep& ts) {
   const std::vector<int>& co = ts.color_off;
   const int ncol = int(co.size()) - 1;
   auto range = [&](int c) { return dim3(blocks_for(co[c + 1] - co[c], kBlock)); };
+  auto range_groups = [&](int c) { return dim3(blocks_for((co[c + 1] - co[c]) * kVelGroups, kBlock)); };
   if (a.n > 0) {
     hipLaunchKernelGGL(k_ab_clear, dim3(gs), dim3(kBlock), 0, st, a);
     for (int it = ts.pos_iters - 1; it >= 0; --it)
@@ -794,10 +838,12 @@ static int reverse_step(World* w, const AdjCtx& a, const TapeStep& ts) {
     hipLaunchKernelGGL(k_ab_store, dim3(gs), dim3(kBlock), 0, st, a);
     for (int it = ts.vel_iters - 1; it >= 0; --it)
       for (int c = ncol - 1; c >= 0; --c)
-        if (co[c + 1] > co[c]) hipLaunchKernelGGL(k_ab_vel, range(c), dim3(kBlock), 0, st, a, co[c], co[c + 1], it);
+        if (co[c + 1] > co[c])
+          hipLaunchKernelGGL(k_ab_vel, range_groups(c), dim3(kBlock), 0, st, a, co[c], co[c + 1], it);
     if (ts.warm)
       for (int c = ncol - 1; c >= 0; --c)
-        if (co[c + 1] > co[c]) hipLaunchKernelGGL(k_ab_warm, range(c), dim3(kBlock), 0, st, a, co[c], co[c + 1]);
+        if (co[c + 1] > co[c])
+          hipLaunchKernelGGL(k_ab_warm, range_groups(c), dim3(kBlock), 0, st, a, co[c], co[c + 1]);
     hipLaunchKernelGGL(k_ab_setup, dim3(gs), dim3(kBlock), 0, st, a);
   }
   hipLaunchKernelGGL(k_ab_intvel, dim3(gb), dim3(kBlock), 0, st, a);

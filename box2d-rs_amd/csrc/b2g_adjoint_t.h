@@ -15,18 +15,24 @@
 
 namespace b2g {
 
+// one group of L input lanes, [base, base + L): xbar's entries of the group receive their
+// contractions, the others are left alone (groups are independent: a kernel may run them on
+// separate threads)
+template <int NIN, int NOUT, int L, class S, class F>
+B2G_HD void vjp_group(const S* x, const S* ybar, S* xbar, const F& f, int base) {
+  Dual<L, S> xd[NIN], yd[NOUT];
+  for (int i = 0; i < NIN; ++i) xd[i] = dual_var<L, S>(x[i], i - base);
+  f(xd, yd);
+  for (int j = 0; j < L && base + j < NIN; ++j) {
+    S acc = S(0);
+    for (int o = 0; o < NOUT; ++o) acc += ybar[o] * yd[o].d[j];
+    xbar[base + j] += acc;
+  }
+}
+
 template <int NIN, int NOUT, int L, class S, class F>
 B2G_HD void vjp(const S* x, const S* ybar, S* xbar, const F& f) {
-  for (int base = 0; base < NIN; base += L) {
-    Dual<L, S> xd[NIN], yd[NOUT];
-    for (int i = 0; i < NIN; ++i) xd[i] = dual_var<L, S>(x[i], i - base);
-    f(xd, yd);
-    for (int j = 0; j < L && base + j < NIN; ++j) {
-      S acc = S(0);
-      for (int o = 0; o < NOUT; ++o) acc += ybar[o] * yd[o].d[j];
-      xbar[base + j] += acc;
-    }
-  }
+  for (int base = 0; base < NIN; base += L) vjp_group<NIN, NOUT, L>(x, ybar, xbar, f, base);
 }
 
 // ---- flat layouts ----------------------------------------------------------------------------
