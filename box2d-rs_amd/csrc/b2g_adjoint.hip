@@ -465,31 +465,59 @@ __global__ void __launch_bounds__(kBlock) k_ab_clear(AdjCtx a) {
   a.fricbar[i] = 0.0f;
 }
 
+// The reverse position solve runs kPosGroups threads per constraint, one per group of kPosLanes
+// dual lanes over its 14 inputs (adjacent lanes of one wave, as for the velocity solve below:
+// all read the position cotangents before any writes, each writes only its own inputs' entries)
+constexpr int kPosIn = 14, kPosLanes = 2, kPosGroups = 8;
+static_assert(kPosGroups * kPosLanes >= kPosIn && 64 % kPosGroups == 0, "position groups");
+
 __global__ void __launch_bounds__(kBlock) k_ab_pos(AdjCtx a, int b0, int b1, int it) {
-  int i = b0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b1) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = b0 + t / kPosGroups, lo = (t % kPosGroups) * kPosLanes;
+  if (i >= b1 || lo >= kPosIn) return;
   const long long s = a.off + i;
   const int4 mt = a.meta[s];
   if (mt.w <= it) return;
   const int4 sl = a.slot[s];
   const ContactConst k = contact_const(a, sl, mt.x);
   const bool dA = a.btype[sl.x] == 2, dB = a.btype[sl.y] == 2;
-  const float4* t = a.t_pos + 2 * (size_t(it) * a.n + i);
-  float x[14], yb[6], xb[14];
-  put3(x, t[0]);
-  put3(x + 3, t[1]);
+  const float4* tp = a.t_pos + 2 * (size_t(it) * a.n + i);
+  float x[kPosIn], yb[6], xb[kPosIn];
+  put3(x, tp[0]);
+  put3(x + 3, tp[1]);
   const float4 g0 = a.geom[2 * i], g1 = a.geom[2 * i + 1];
   x[6] = g0.x; x[7] = g0.y; x[8] = g0.z; x[9] = g0.w;
   x[10] = g1.x; x[11] = g1.y; x[12] = g1.z; x[13] = g1.w;
   const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   put3(yb, dA ? a.pbar[sl.x] : z);
   put3(yb + 3, dB ? a.pbar[sl.y] : z);
-  for (int j = 0; j < 14; ++j) xb[j] = 0.0f;
-  vjp<14, 6, 7>(x, yb, xb, PosFn<float>{k});
-  if (dA) a.pbar[sl.x] = f4(xb);
-  if (dB) a.pbar[sl.y] = f4(xb + 3);
-  add4(a.gbar[2 * i], xb + 6);
-  add4(a.gbar[2 * i + 1], xb + 10);
+  for (int j = 0; j < kPosIn; ++j) xb[j] = 0.0f;
+  vjp_group<kPosIn, 6, kPosLanes>(x, yb, xb, PosFn<float>{k}, lo);
+  const int hi = lo + kPosLanes;
+  auto put = [&](float* d, int j, bool assign) {
+    if (j >= lo && j < hi) {
+      if (assign) *d = xb[j];
+      else *d += xb[j];
+    }
+  };
+  float* pa = &a.pbar[sl.x].x;
+  float* pb = &a.pbar[sl.y].x;
+  float* ga = &a.gbar[2 * i].x;
+  float* gb = &a.gbar[2 * i + 1].x;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (dA) put(pa + c, c, true);
+    if (dB) put(pb + c, 3 + c, true);
+  }
+  if (lo == 0) {  // the unused fourth component, as a whole-record write would leave it
+    if (dA) pa[3] = 0.0f;
+    if (dB) pb[3] = 0.0f;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    put(ga + c, 6 + c, false);
+    put(gb + c, 10 + c, false);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_ab_intpos(AdjCtx a) {
@@ -825,13 +853,14 @@ static int reverse_step(World* w, const AdjCtx& a, const TapeStep& ts) {
   const int gs = blocks_for(std::max(a.n, 1), kBlock);
   const std::vector<int>& co = ts.color_off;
   const int ncol = int(co.size()) - 1;
-  auto range = [&](int c) { return dim3(blocks_for(co[c + 1] - co[c], kBlock)); };
   auto range_groups = [&](int c) { return dim3(blocks_for((co[c + 1] - co[c]) * kVelGroups, kBlock)); };
+  auto range_pos = [&](int c) { return dim3(blocks_for((co[c + 1] - co[c]) * kPosGroups, kBlock)); };
   if (a.n > 0) {
     hipLaunchKernelGGL(k_ab_clear, dim3(gs), dim3(kBlock), 0, st, a);
     for (int it = ts.pos_iters - 1; it >= 0; --it)
       for (int c = ncol - 1; c >= 0; --c)
-        if (co[c + 1] > co[c]) hipLaunchKernelGGL(k_ab_pos, range(c), dim3(kBlock), 0, st, a, co[c], co[c + 1], it);
+        if (co[c + 1] > co[c])
+          hipLaunchKernelGGL(k_ab_pos, range_pos(c), dim3(kBlock), 0, st, a, co[c], co[c + 1], it);
   }
   hipLaunchKernelGGL(k_ab_intpos, dim3(gb), dim3(kBlock), 0, st, a);
   if (a.n > 0) {
